@@ -1,0 +1,151 @@
+/*
+ * walkray.h -- C ABI of libwalkray, the MI355X (gfx950) implementation of the
+ * WalkRay path-tracing hot path.
+ *
+ * Conventions (kept from the reference, scene.cpp:102-113, kdtree.cpp:300-305):
+ *   - a miss is prim = WR_NONE and t = -1;
+ *   - object ids are the order of wr_scene_add_mesh calls (Scene::objects);
+ *   - triangle ids ("prim") are global in that order: object 0's triangles
+ *     first, each object's in its own order (Scene::buildKDTree, scene.cpp:20-44).
+ * Every function returns 0 on success or a negative WR_E* code; the message is
+ * in wr_last_error(ctx) (or wr_last_error(NULL) for errors before a context
+ * exists).  A context is one HIP device plus one stream and is not thread-safe;
+ * use one per host thread.  Pointers named d_* are device pointers (caller
+ * owned, e.g. from wr_device_alloc); all other pointers are host memory.
+ */
+#ifndef WALKRAY_H
+#define WALKRAY_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WR_NONE 0xFFFFFFFFu
+#define WR_OK 0
+#define WR_EINVAL (-1)
+#define WR_EHIP (-2)
+#define WR_ESTATE (-3)
+#define WR_ENOMEM (-4)
+#define WR_EBUSY (-5)
+
+typedef struct wr_ctx wr_ctx;
+typedef struct wr_scene wr_scene;
+
+/* KDTree::Ray (kdtree.h:12-15): 24 B. */
+typedef struct {
+    float o[3];
+    float d[3];
+} wr_ray;
+
+/* The LFJ::Ray fields the NoSelf condition reads (ray.h:13-26, noself.cpp:7-11):
+ * 32 B.  inside_obj / contact_obj are object ids or WR_NONE (NULL). */
+typedef struct {
+    float o[3];
+    float d[3];
+    uint32_t inside_obj;
+    uint32_t contact_obj;
+} wr_path_ray;
+
+/* Closest hit: 16 B.  prim = WR_NONE and t = -1 on a miss; b1, b2 are the
+ * barycentrics of Mesh::getWorldNormal (mesh.h:67-72) at o + d t.
+ * prim = 0xFFFFFFFE flags a ray whose traversal stack overflowed (never
+ * silently wrong; trees deeper than 64 levels are rejected at build). */
+typedef struct {
+    uint32_t prim;
+    float t;
+    float b1;
+    float b2;
+} wr_hit;
+
+/* Materials of the four hot object classes (diffuseObj.h, glossyObj.h,
+ * refractiveObj.h, emissiveObj.h). */
+enum { WR_DIFFUSE = 0, WR_GLOSSY = 1, WR_REFRACTIVE = 2, WR_EMISSIVE = 3 };
+typedef struct {
+    int32_t type;
+    float color[3]; /* albedo (diffuse, glossy) or emitted radiance (emissive) */
+    float power;    /* Phong exponent (glossy) */
+    float ior;      /* index of refraction (refractive) */
+} wr_material;
+
+/* KDTree::Strategy (kdtree.h:20). Scene always builds with BEST. */
+enum { WR_LOOP = 0, WR_BEST = 1 };
+
+typedef struct {
+    uint32_t n_objects;
+    uint32_t n_triangles;
+    uint32_t n_nodes;       /* internal nodes of the global tree */
+    uint32_t n_leaves;
+    uint32_t depth;         /* internal levels of the global tree */
+    uint32_t max_obj_depth; /* deepest per-object tree */
+    uint64_t device_bytes;  /* device memory held by the scene */
+    double build_seconds;   /* host build (all trees) */
+} wr_scene_info;
+
+/* ---- context ----------------------------------------------------------- */
+/* Replaces nothing in the reference (single process, CPU). */
+int wr_ctx_create(int device, wr_ctx** out);
+void wr_ctx_destroy(wr_ctx* ctx);
+const char* wr_last_error(const wr_ctx* ctx);
+int wr_synchronize(wr_ctx* ctx);
+int wr_device_alloc(wr_ctx* ctx, size_t bytes, void** d_out);
+int wr_device_free(wr_ctx* ctx, void* d_ptr);
+int wr_memcpy_htod(wr_ctx* ctx, void* d_dst, const void* src, size_t bytes);
+int wr_memcpy_dtoh(wr_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+/* HIP events on the context's stream, bracketing kernel work. */
+int wr_timer_start(wr_ctx* ctx);
+int wr_timer_stop(wr_ctx* ctx, float* ms);
+
+/* ---- scene ------------------------------------------------------------- */
+/* Scene (scene.h:10-45): objects + the global tree + per-object trees. */
+int wr_scene_create(wr_ctx* ctx, wr_scene** out);
+void wr_scene_destroy(wr_scene* scene);
+/* Adds one object (AbstractObject + Mesh, abstractObj.h:10-49, mesh.h:9-19) in
+ * world space.  normals: per-vertex shading normals (nv * 3) or NULL (flat,
+ * mesh.h:59-65).  *obj_id receives the object id. */
+int wr_scene_add_mesh(wr_scene* scene, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                      const float* normals, const wr_material* mat, uint32_t* obj_id);
+/* Scene::buildKDTree + buildObjKDTrees (scene.cpp:20-65) with
+ * KDTree::build(BEST) (kdtree.cpp:242-258), flattened and uploaded. */
+int wr_scene_build(wr_scene* scene, int strategy);
+int wr_scene_get_info(const wr_scene* scene, wr_scene_info* info);
+/* prim -> (object id, triangle id in the object): the ObjSourceInfo of
+ * scene.cpp:34-36,110. */
+int wr_scene_prim_source(const wr_scene* scene, uint32_t prim, uint32_t* obj, uint32_t* tri);
+/* Canonical preorder export of the global tree (obj = WR_NONE) or of object
+ * obj's tree, for comparison with another builder.  Pass NULL arrays to get
+ * the sizes only. */
+int wr_scene_export_tree(const wr_scene* scene, uint32_t obj, uint32_t* n_nodes, uint32_t* n_leaf_tris,
+                         float* box, uint32_t* right, uint32_t* leaf_first, uint32_t* leaf_count,
+                         uint32_t* tri_list);
+
+/* Host-only (no device needed): runs the builder of wr_scene_build on one
+ * triangle soup (KDTree::build(BEST), kdtree.cpp:242-258) and keeps the
+ * result for wr_tree_export, in the same canonical preorder as
+ * wr_scene_export_tree.  Used to check the builder on machines without a GPU. */
+typedef struct wr_tree wr_tree;
+int wr_tree_build(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, int nthreads, wr_tree** out);
+int wr_tree_export(const wr_tree* tree, uint32_t* n_nodes, uint32_t* n_leaf_tris, uint32_t* depth, float* box,
+                   uint32_t* right, uint32_t* leaf_first, uint32_t* leaf_count, uint32_t* tri_list);
+void wr_tree_free(wr_tree* tree);
+
+/* ---- ray queries ------------------------------------------------------- */
+/* KDTree::intersect(ray, triID) without Condition (kdtree.cpp:349-353) for n
+ * rays, as Scene::intersect uses it for camera rays (camera.cpp:52).
+ * Asynchronous on the context stream.  d_counters (nullable, 3 x u64,
+ * accumulated): box records fetched, triangle records tested, fallback rays. */
+int wr_closest_hit(wr_ctx* ctx, const wr_scene* scene, const wr_ray* d_rays, uint64_t n, wr_hit* d_hits,
+                   uint64_t* d_counters);
+/* Scene::intersect with NoSelfCondition(scene, ray) (mcRenderer.cpp:42-44,
+ * noself.cpp:13-20) for n path rays. */
+int wr_closest_hit_noself(wr_ctx* ctx, const wr_scene* scene, const wr_path_ray* d_rays, uint64_t n,
+                          wr_hit* d_hits);
+/* Scene::findInsideObject(ray, cur) (scene.cpp:115-144); cur is taken from
+ * d_rays[i].contact_obj (WR_NONE = NULL). */
+int wr_inside_object(wr_ctx* ctx, const wr_scene* scene, const wr_path_ray* d_rays, uint64_t n, uint32_t* d_obj);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

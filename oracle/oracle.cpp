@@ -1,0 +1,876 @@
+/*
+ * oracle.cpp -- CPU oracle for the WalkRay path-tracing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file restates the reference algorithms in
+ * plain C++ (libstdc++ + OpenMP, nothing else) so that the tests can check the
+ * HIP product against it.  It is compiled with -ffp-contract=off on x86-64
+ * (SSE2 scalar float, IEEE single precision, correctly rounded + - * / sqrt),
+ * which is the arithmetic the reference's own operation order implies.
+ *
+ * What is faithful to the reference and what is not:
+ *   - KDTree::build(BEST) / splitBest  (kdtree.cpp:141-258): faithful.  The
+ *     reference collects unique vertex indices in an unordered_set; only the
+ *     multiset of their coordinates matters to nth_element's result, so a
+ *     sorted-unique vector is used instead.
+ *   - Node::BoundingBox::intersect (kdtree.cpp:46-64), triangle test
+ *     (kdtree.cpp:279-306), recursive unpruned traversal with its tie rules
+ *     (kdtree.cpp:308-353): faithful, same operation order.
+ *   - NoSelfCondition (noself.cpp:13-20), Mesh::getWorldNormal (mesh.h:47-77),
+ *     Scene::intersect / findInsideObject (scene.cpp:102-144): faithful, with
+ *     identity object transforms (all scenes here are generated in world space).
+ *   - Camera::generateRay / fixVignetting (camera.cpp:6-60, 90-123), materials
+ *     (diffuseObj.h, glossyObj.h, isotropicPhong.h, refractiveObj.h,
+ *     emissiveObj.h, frame.h, cos/powerCos samplers), samplePath
+ *     (mcRenderer.cpp:16-57) and the PathTracer pixel loop
+ *     (pathTracer.cpp:20-70): faithful estimator.  The reference's single
+ *     shared std::mt19937_64 (rng.h:6-59, racy across OpenMP threads) is
+ *     replaced by a counter RNG keyed by (seed, pixel, pass, vertex): this
+ *     changes which random numbers are drawn, not the estimator.
+ *   - The running-mean film (pathTracer.cpp:24,69) is replaced by FP64 sums of
+ *     the per-pass sample values and of their squares, so tests can form the
+ *     mean and its standard error.
+ *   - The per-pass tree rebuild (pathTracer.cpp:18 -> scene.cpp:67-80) is not
+ *     repeated: the builder is deterministic, so every pass would rebuild the
+ *     identical tree.
+ */
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include <omp.h>
+
+namespace orc {
+
+static const double PI_D = 3.1415926535897932384626433832795;      /* macros.h:5 */
+static const double INV_PI_D = 0.31830988618379067153776752674503; /* macros.h:6 */
+static const double EPS_D = 1e-5;                                  /* macros.h:7 */
+
+/* ---- vec3 with the reference's float semantics (nvVector.h:363-525) ---- */
+struct V {
+    float x, y, z;
+    V() : x(0), y(0), z(0) {}
+    V(float a, float b, float c) : x(a), y(b), z(c) {}
+    float& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+static inline V add(V a, V b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V sub(V a, V b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V neg(V a) { return V(-a.x, -a.y, -a.z); }
+static inline V mul(V a, float s) { return V(a.x * s, a.y * s, a.z * s); }
+static inline V mulv(V a, V b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+/* operator/=(vec3, T): a zero divisor leaves the vector unchanged (nvVector.h:375-379). */
+static inline V divs(V a, float s) {
+    if (s == 0) return a;
+    return V(a.x / s, a.y / s, a.z / s);
+}
+static inline V cross(V a, V b) { /* nvVector.h:490-499 */
+    return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline float dot(V a, V b) { /* nvVector.h:501-507: r = 0; r += ... */
+    float r = 0.f;
+    r += a.x * b.x;
+    r += a.y * b.y;
+    r += a.z * b.z;
+    return r;
+}
+static inline float length(V a) { return std::sqrt(dot(a, a)); }
+static inline V normalize(V a) { /* nvVector.h:520-525 */
+    float r = length(a);
+    if (r > 0) return divs(a, r);
+    return a;
+}
+static inline float maxc(V a) { /* std::max_element over x,y,z */
+    float m = a.x;
+    if (a.y > m) m = a.y;
+    if (a.z > m) m = a.z;
+    return m;
+}
+static inline bool legal(V c) { /* util.h:8-18 isLegalColor */
+    for (int i = 0; i < 3; i++) {
+        if (std::isnan(c[i])) return false;
+        if (!std::isfinite(c[i])) return false;
+        if (c[i] < 0) return false;
+    }
+    return true;
+}
+
+/* ---- KDTree ---------------------------------------------------------- */
+struct Ray { V o, d; };
+
+struct Node {
+    Node* left = nullptr;
+    Node* right = nullptr;
+    std::vector<unsigned> tris;
+    float box[6];  /* minCoord xyz, maxCoord xyz: (float*)this order of kdtree.cpp:52 */
+};
+
+struct Tree {
+    std::vector<V> verts;
+    std::vector<unsigned> idx;  /* 3 per triangle */
+    Node* root = nullptr;
+    unsigned depth = 0;
+
+    ~Tree() { destroy(root); }
+    void destroy(Node* n) {
+        if (!n) return;
+        destroy(n->left);
+        destroy(n->right);
+        delete n;
+    }
+
+    /* kdtree.cpp:141-240 (splitBest) */
+    void splitBest(Node* node, unsigned d) {
+        if (d > depth) depth = d;
+        float bestRatio = 10;
+        unsigned bestDim = 0;
+        float bestSplit = 0;
+        const size_t n = node->tris.size();
+        node->left = node->right = nullptr;
+        if (n == 0) return;
+        /* Tight box over every vertex of every triangle, in list order
+         * (boundvec<1>/<0>, kdtree.cpp:5-21,151-165). */
+        V mn = verts[idx[3 * node->tris[0]]], mx = mn;
+        std::vector<unsigned> uniq;
+        uniq.reserve(3 * n);
+        for (size_t i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++) {
+                unsigned vi = idx[3 * node->tris[i] + k];
+                V p = verts[vi];
+                mn.x = p.x < mn.x ? p.x : mn.x;
+                mn.y = p.y < mn.y ? p.y : mn.y;
+                mn.z = p.z < mn.z ? p.z : mn.z;
+                mx.x = p.x > mx.x ? p.x : mx.x;
+                mx.y = p.y > mx.y ? p.y : mx.y;
+                mx.z = p.z > mx.z ? p.z : mx.z;
+                uniq.push_back(vi);
+            }
+        node->box[0] = mn.x; node->box[1] = mn.y; node->box[2] = mn.z;
+        node->box[3] = mx.x; node->box[4] = mx.y; node->box[5] = mx.z;
+        if (n <= 1) return;  /* maxLeafTriangleNum = 1 (kdtree.cpp:25,170) */
+        std::sort(uniq.begin(), uniq.end());
+        uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+        std::vector<float> coord(uniq.size());
+        for (unsigned dim = 0; dim < 3; dim++) {
+            for (size_t i = 0; i < uniq.size(); i++) coord[i] = verts[uniq[i]][dim];
+            std::nth_element(coord.begin(), coord.begin() + (coord.size() >> 1), coord.end());
+            float split = coord[coord.size() >> 1];  /* kdtree.cpp:179-183 */
+            unsigned leftCount = 0;
+            for (size_t i = 0; i < n; i++) {
+                unsigned mask = 0;
+                for (int k = 0; k < 3; k++)
+                    mask += (verts[idx[3 * node->tris[i] + k]][dim] <= split) << k;
+                if (mask != 0) leftCount++;
+            }
+            float ratio = leftCount / float(n);
+            /* abs(ratio-0.5) < abs(bestRatio-0.5) in double (kdtree.cpp:202-208) */
+            if (std::fabs((double)ratio - 0.5) < std::fabs((double)bestRatio - 0.5)) {
+                bestRatio = ratio;
+                bestDim = dim;
+                bestSplit = split;
+            }
+        }
+        if (bestRatio == 0 || bestRatio == 1) return;  /* kdtree.cpp:211-215 */
+        node->left = new Node;
+        node->right = new Node;
+        for (size_t i = 0; i < n; i++) {  /* order-preserving partition, kdtree.cpp:220-236 */
+            unsigned mask = 0;
+            for (int k = 0; k < 3; k++)
+                mask += (verts[idx[3 * node->tris[i] + k]][bestDim] <= bestSplit) << k;
+            if (mask == 0) node->right->tris.push_back(node->tris[i]);
+            else node->left->tris.push_back(node->tris[i]);
+        }
+        splitBest(node->left, d + 1);
+        splitBest(node->right, d + 1);
+    }
+
+    void build() { /* kdtree.cpp:242-258 */
+        destroy(root);
+        root = new Node;
+        const unsigned nt = (unsigned)(idx.size() / 3);
+        for (unsigned i = 0; i < nt; i++) root->tris.push_back(i);
+        depth = 0;
+        splitBest(root, 0);
+    }
+
+    /* kdtree.cpp:279-306; cond is evaluated by the caller-supplied functor. */
+    template <class Cond>
+    float triIsect(const Ray& r, unsigned ti, const Cond& cond) const {
+        V vs[3], lines[3], r2v, n;
+        for (int i = 0; i < 3; i++) vs[i] = verts[idx[3 * ti + i]];
+        unsigned mask_pos = 0, mask_neg = 0;
+        for (int i = 0; i < 3; i++) {
+            lines[i] = sub(vs[(i + 1) % 3], vs[i]);
+            r2v = sub(vs[i], r.o);
+            n = cross(lines[i], r2v);
+            mask_pos += (dot(r.d, n) >= 0) << i;
+            mask_neg += (dot(r.d, n) <= 0) << i;
+        }
+        n = cross(lines[0], lines[1]);
+        n = normalize(n);
+        float orthoDist = dot(n, r2v);
+        float d = dot(n, r.d);
+        if (d == 0 || orthoDist / d < 0 || (mask_pos != 7 && mask_neg != 7)) return -1;
+        float dist = orthoDist / d;
+        if (!cond(r, ti, dist)) return -1;
+        return dist;
+    }
+
+    template <class Cond>
+    float isect(const Ray& r, const Node* node, unsigned& tid, const Cond& cond) const {
+        if (boxIsect(node->box, r) < 0) return -1;  /* kdtree.cpp:310-312 */
+        if (!node->left && !node->right) {
+            float minDist = -1;
+            for (size_t k = 0; k < node->tris.size(); k++) {
+                float dist = triIsect(r, node->tris[k], cond);
+                if (dist >= 0 && (dist < minDist || minDist < 0)) {  /* earlier wins ties */
+                    minDist = dist;
+                    tid = node->tris[k];
+                }
+            }
+            return minDist;
+        }
+        unsigned ltid = 0, rtid = 0;
+        float ld = isect(r, node->left, ltid, cond);
+        float rd = isect(r, node->right, rtid, cond);
+        if (ld >= 0 && rd >= 0) {  /* equal t goes to the right subtree (kdtree.cpp:331-334) */
+            tid = ld < rd ? ltid : rtid;
+            return ld < rd ? ld : rd;
+        } else if (rd < 0) {
+            tid = ltid;
+            return ld;
+        } else if (ld < 0) {
+            tid = rtid;
+            return rd;
+        }
+        return -1;
+    }
+
+    template <class Cond>
+    float intersect(const Ray& r, unsigned& tid, const Cond& cond) const {
+        if (!root || root->tris.empty()) return -1;  /* empty tree: no box was ever set */
+        return isect(r, root, tid, cond);
+    }
+
+    static float boxIsect(const float* b, const Ray& r) { /* kdtree.cpp:46-64 */
+        float minDist = -1;
+        for (unsigned i = 0; i < 6; i++) {
+            const int a = i % 3;
+            if (r.d[a] == 0) continue;
+            float dist = (b[i] - r.o[a]) / r.d[a];
+            V ip = add(r.o, mul(r.d, dist));
+            bool inter = dist >= 0;
+            const int a1 = (i + 1) % 3, a2 = (i + 2) % 3;
+            inter &= b[a1] <= ip[a1];
+            inter &= b[3 + a1] >= ip[a1];
+            inter &= b[a2] <= ip[a2];
+            inter &= b[3 + a2] >= ip[a2];
+            minDist = (inter && (dist < minDist || minDist < 0)) ? dist : minDist;
+        }
+        return minDist;
+    }
+};
+
+struct NoCond {
+    bool operator()(const Ray&, unsigned, float) const { return true; }
+};
+
+/* ---- Scene --------------------------------------------------------------- */
+struct Object {
+    std::vector<V> verts;
+    std::vector<unsigned> idx;
+    std::vector<V> normals;  /* empty: flat (mesh.h:59-65) */
+    int mat;
+    V color;      /* albedo or emitted radiance */
+    float power;  /* Phong exponent */
+    float ior;
+};
+
+struct Scene {
+    std::vector<Object> objs;
+    Tree tree;                         /* global tree, scene.cpp:20-44 */
+    std::vector<Tree*> objTrees;       /* per-object trees, scene.cpp:46-65 */
+    std::vector<unsigned> srcObj, srcTri;  /* ObjSourceInfo per global triangle */
+    ~Scene() { for (auto* t : objTrees) delete t; }
+
+    void build() {
+        tree.verts.clear(); tree.idx.clear(); srcObj.clear(); srcTri.clear();
+        unsigned off = 0;
+        for (unsigned oi = 0; oi < objs.size(); oi++) {
+            const Object& o = objs[oi];
+            for (const V& v : o.verts) tree.verts.push_back(v);
+            const unsigned nt = (unsigned)(o.idx.size() / 3);
+            for (unsigned i = 0; i < nt; i++) {
+                for (int k = 0; k < 3; k++) tree.idx.push_back(o.idx[3 * i + k] + off);
+                srcObj.push_back(oi);
+                srcTri.push_back(i);
+            }
+            off += (unsigned)o.verts.size();
+        }
+        tree.build();
+        for (auto* t : objTrees) delete t;
+        objTrees.clear();
+        for (unsigned oi = 0; oi < objs.size(); oi++) {
+            Tree* t = new Tree;
+            t->verts = objs[oi].verts;
+            t->idx = objs[oi].idx;
+            t->build();
+            objTrees.push_back(t);
+        }
+    }
+
+    /* Mesh::getWorldNormal (mesh.h:47-77) with an identity transform. */
+    V worldNormal(unsigned oi, unsigned fi, V pos, float* u1o = nullptr, float* u2o = nullptr) const {
+        const Object& o = objs[oi];
+        V vps[3];
+        for (int i = 0; i < 3; i++) vps[i] = o.verts[o.idx[3 * fi + i]];
+        V b1 = sub(vps[1], vps[0]);
+        V b2 = sub(vps[2], vps[0]);
+        V face = normalize(cross(b1, b2));
+        V v = sub(pos, vps[0]);
+        float d12 = dot(b1, b2);
+        float l1 = length(b1);
+        float l2 = length(b2);
+        float u2 = (dot(v, b1) * d12 - dot(v, b2) * l1 * l1) / (d12 * d12 - l1 * l1 * l2 * l2);
+        float u1 = (dot(v, b1) - u2 * d12) / (l1 * l1);
+        if (u1o) *u1o = u1;
+        if (u2o) *u2o = u2;
+        if (o.normals.empty()) return face;
+        V n0 = o.normals[o.idx[3 * fi + 0]], n1 = o.normals[o.idx[3 * fi + 1]],
+          n2 = o.normals[o.idx[3 * fi + 2]];
+        float w0 = 1 - u1 - u2;
+        V nn = add(add(mul(n0, w0), mul(n1, u1)), mul(n2, u2));
+        return normalize(nn);
+    }
+
+    /* NoSelfCondition::legal (noself.cpp:13-20) */
+    struct NoSelf {
+        const Scene* s;
+        unsigned inside, contact;
+        bool operator()(const Ray& r, unsigned gtri, float dist) const {
+            unsigned hitObj = s->srcObj[gtri];
+            unsigned fi = s->srcTri[gtri];
+            float d = dot(r.d, s->worldNormal(hitObj, fi, add(r.o, mul(r.d, dist))));
+            bool in = d <= 0, out = d >= 0;
+            if (inside == contact) return !(in && contact == hitObj);
+            return !(out && inside != hitObj);
+        }
+    };
+
+    /* Scene::intersect (scene.cpp:102-113) */
+    float intersect(const Ray& r, unsigned& gtri, bool noself, unsigned inside,
+                    unsigned contact) const {
+        unsigned tid = ORC_NONE;
+        float dist;
+        if (noself) dist = tree.intersect(r, tid, NoSelf{this, inside, contact});
+        else dist = tree.intersect(r, tid, NoCond{});
+        if (dist >= 0) gtri = tid;
+        return dist;
+    }
+
+    /* Scene::findInsideObject (scene.cpp:115-144) */
+    unsigned findInside(const Ray& r, unsigned cur) const {
+        Ray front = r, back;
+        back.o = r.o;
+        back.d = neg(r.d);
+        for (int i = (int)objTrees.size() - 1; i >= 0; i--) {
+            if (cur == (unsigned)i) continue;
+            unsigned tid = 0;
+            float dist = objTrees[i]->intersect(back, tid, NoCond{});
+            if (dist < 0) continue;
+            V n = worldNormal(i, tid, add(back.o, mul(back.d, dist)));
+            if (dot(n, back.d) < 0) continue;
+            dist = objTrees[i]->intersect(front, tid, NoCond{});
+            if (dist < 0) continue;
+            n = worldNormal(i, tid, add(front.o, mul(front.d, dist)));
+            if (dot(n, front.d) < 0) continue;
+            return (unsigned)i;
+        }
+        return ORC_NONE;
+    }
+};
+
+/* ---- Counter RNG: Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) ---- */
+static inline void philox(const uint32_t c_in[4], const uint32_t k_in[2], uint32_t out[4]) {
+    uint32_t c0 = c_in[0], c1 = c_in[1], c2 = c_in[2], c3 = c_in[3];
+    uint32_t k0 = k_in[0], k1 = k_in[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+static inline void rng4(uint64_t seed, uint32_t pixel, uint32_t pass, uint32_t vertex, float f[4]) {
+    uint32_t c[4] = {pixel, pass, vertex, 0x57414C4Bu};
+    uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    philox(c, k, o);
+    for (int i = 0; i < 4; i++) f[i] = (float)(o[i] >> 8) * (1.0f / 16777216.0f);
+}
+
+/* ---- Camera (camera.cpp) -------------------------------------------------- */
+struct Cam {
+    V pos, focus, up, front, right, top;
+    unsigned W, H;
+    float dist;
+    explicit Cam(const orc_camera* c) {
+        pos = V(c->pos[0], c->pos[1], c->pos[2]);
+        focus = V(c->focus[0], c->focus[1], c->focus[2]);
+        up = V(c->up[0], c->up[1], c->up[2]);
+        W = c->width; H = c->height; dist = c->sight_dist;
+        front = normalize(sub(focus, pos));   /* camera.cpp:7-11 */
+        right = normalize(cross(front, up));
+        top = cross(right, front);
+    }
+    V dir(unsigned p, float u, float v) const { /* camera.cpp:13-23 */
+        unsigned x = p % W, y = p / W;
+        V d = add(add(mul(front, dist), mul(right, ((float)x + u) - (W / 2.f))),
+                  mul(top, ((H / 2.f) - (float)y) - v));
+        return normalize(d);
+    }
+    V fixVignetting(V color, unsigned p) const { /* camera.cpp:107-122 */
+        unsigned x = p % W, y = p / W;
+        V d = add(add(mul(front, dist), mul(right, (float)(x + 0.5 - (W / 2.0)))),
+                  mul(top, (float)((H / 2.0) - 0.5 - y)));
+        d = normalize(d);
+        if (dot(d, front) == 0) return V(0, 0, 0);
+        V res = divs(color, std::fabs(dot(d, front)));
+        return legal(res) ? res : V(0, 0, 0);
+    }
+};
+
+/* ---- One path vertex produced by AbstractObject::scatter ----------------- */
+struct Out {
+    V o, d, rad, n;  /* n: contact normal, == contactObj->getWorldNormal(contactTri, o) */
+    float op = 1.f, dp = 1.f;
+    unsigned inside = ORC_NONE, contact = ORC_NONE, ctri = 0;
+};
+
+/* Ray::cosineTerm (ray.cpp:52-58) for a scatter vertex (contactObj set, hasCosine true). */
+static inline float cosineTerm(const Out& r) {
+    float c = 1;
+    if (length(r.d) > 0.5) c = std::fabs(dot(r.n, r.d));
+    return c > EPS_D ? c : (float)EPS_D;
+}
+
+/* Frame::setFromN + toWorldFrame (frame.h:23-49) */
+static inline V toWorld(V w, V local) {
+    V mX, mY = w, mZ;
+    if (std::fabs(w.x) > EPS_D) {
+        mX = normalize(cross(V(0, 1, 0), w));
+        mZ = cross(mX, mY);
+    } else {
+        mZ = normalize(cross(V(1, 0, 0), w));
+        mX = cross(mY, mZ);
+    }
+    return add(add(mul(mX, local.x), mul(mY, local.y)), mul(mZ, local.z));
+}
+
+struct Tracer {
+    const Scene* s;
+    Cam cam;
+    unsigned maxDepth;
+    uint64_t seed;
+
+    struct Hit {
+        bool valid = false;
+        float dist = -1;
+        unsigned obj = ORC_NONE, tri = 0;
+    };
+
+    /* dispatch of AbstractObject::scatter for the four hot materials */
+    Out scatter(unsigned self, const Ray& in, const Hit& h, unsigned inInside, const float* rn) const {
+        const Object& ob = s->objs[self];
+        Out out;
+        V position = add(in.o, mul(in.d, h.dist));
+        V N = s->worldNormal(h.obj, h.tri, position);
+        if (ob.mat == ORC_REFRACTIVE && h.obj != self && inInside == self)  /* refractiveObj.h:32-33 */
+            return scatter(h.obj, in, h, inInside, rn);
+        out.o = position;
+        out.n = N;
+        out.contact = self;
+        out.ctri = h.tri;
+        switch (ob.mat) {
+        case ORC_DIFFUSE: { /* diffuseObj.h:34-57, diffuseBsdf.h:17-26, cosHemiSphereSampler.h:6-18 */
+            out.inside = inInside;
+            float p = maxc(ob.color);
+            if (rn[0] < p) {
+                const float term1 = (float)(2.0 * PI_D * (double)rn[1]);
+                const float term2 = std::sqrt(1.f - rn[2]);
+                V local(std::sin(term1) * term2, std::sqrt(rn[2]), std::cos(term1) * term2);
+                out.d = toWorld(N, local);
+                float c = dot(N, out.d);
+                out.dp = c > 0 ? (float)((double)c * INV_PI_D) : 0.f;
+                out.rad = mul(ob.color, (float)INV_PI_D);
+                out.dp *= p;
+            } else {
+                out.d = V(0, 0, 0);
+                out.rad = V(0, 0, 0);
+                out.dp = 1 - p;
+            }
+            break;
+        }
+        case ORC_GLOSSY: { /* glossyObj.h:39-62, isotropicPhong.h:20-38, powerCosHemiSphereSampler.h:8-21 */
+            out.inside = inInside;
+            float p = maxc(ob.color);
+            if (rn[0] < p) {
+                const float power = ob.power;
+                const float term1 = (float)(2.0 * PI_D * (double)rn[1]);
+                const float term2 = (float)std::pow((double)rn[2], 1.0 / (power + 1.0));
+                const float term3 = (float)std::sqrt(1.0 - (double)(term2 * term2));
+                V local(std::sin(term1) * term3, term2, std::cos(term1) * term3);
+                V w = toWorld(N, local);
+                float c = dot(N, w);
+                out.dp = c > 0 ? (float)((power + 1.0) * (double)std::pow(c, power) * (INV_PI_D * 0.5)) : 0.f;
+                V refl = add(mul(mul(N, -dot(N, in.d)), 2.f), in.d);
+                refl = normalize(refl);
+                V rho = mul(mul(mul(ob.color, (float)(power + 2.0)), 0.5f), (float)INV_PI_D);
+                float dRW = dot(refl, w);
+                if (dot(w, N) < 0) {
+                    out.d = V(0, 0, 0);
+                    out.rad = V((float)EPS_D, (float)EPS_D, (float)EPS_D);
+                } else {
+                    out.d = w;
+                    out.rad = mul(rho, std::pow(dRW > 0 ? dRW : 0.f, power));
+                }
+                out.dp *= p;
+            } else {
+                out.d = V(0, 0, 0);
+                out.rad = V(0, 0, 0);
+                out.dp = 1 - p;
+            }
+            break;
+        }
+        case ORC_REFRACTIVE: { /* refractiveObj.h:27-88 */
+            out.d = in.d;
+            out.rad = V(1, 1, 1);
+            V refl = add(mul(mul(N, -dot(N, in.d)), 2.f), in.d);
+            refl = normalize(refl);
+            float theta = std::acos(dot(in.d, N));
+            unsigned curIn = inInside, curOut = self;
+            if (inInside == self) {
+                Ray q;
+                q.o = out.o;
+                q.d = out.d;
+                curOut = s->findInside(q, self);
+            }
+            float cn = curIn != ORC_NONE ? (s->objs[curIn].mat == ORC_REFRACTIVE ? s->objs[curIn].ior : 1.f) : 1.f;
+            float nn = curOut != ORC_NONE ? (s->objs[curOut].mat == ORC_REFRACTIVE ? s->objs[curOut].ior : 1.f) : 1.f;
+            float sin_phi = cn / nn * std::sin(theta);
+            if (sin_phi > 1) {  /* total internal reflection: radiance 1, prob 1 */
+                out.d = refl;
+                out.inside = inInside;
+            } else {
+                float phi = std::asin(sin_phi);
+                if (theta > PI_D / 2) phi = (float)(PI_D - phi);
+                V axis = normalize(cross(N, in.d));
+                /* RotateMatrix(axis, phi) * vec4f(N, 0)  (util.h:40-54, nvMatrix.h:98-105,424-433) */
+                float c = std::cos(phi), sn = std::sin(phi);
+                float _c = 1 - c;
+                float x = axis.x, y = axis.y, z = axis.z;
+                float m00 = c + _c * x * x, m01 = _c * x * y - sn * z, m02 = _c * x * z + sn * y;
+                float m10 = _c * x * y + sn * z, m11 = c + _c * y * y, m12 = _c * y * z - sn * x;
+                float m20 = _c * x * z - sn * y, m21 = _c * y * z + sn * x, m22 = c + _c * z * z;
+                V rd;
+                rd.x = N.x * m00 + N.y * m01 + N.z * m02 + 0.f * 0.f;
+                rd.y = N.x * m10 + N.y * m11 + N.z * m12 + 0.f * 0.f;
+                rd.z = N.x * m20 + N.y * m21 + N.z * m22 + 0.f * 0.f;
+                out.d = normalize(rd);
+                float cos_theta = std::fabs(std::cos(theta));
+                float cos_phi = std::fabs(std::cos(phi));
+                float a = std::fabs(cn * cos_theta - nn * cos_phi) / (cn * cos_theta + nn * cos_phi);
+                float b = std::fabs(nn * cos_theta - cn * cos_phi) / (nn * cos_theta + cn * cos_phi);
+                float esr = a * a, epr = b * b;
+                float er = (esr + epr) / 2;
+                float p = er;
+                if (rn[0] < p) {
+                    out.d = refl;
+                    out.rad = mul(out.rad, er / cosineTerm(out));
+                    out.dp = p;
+                    out.inside = inInside;
+                } else {
+                    out.rad = mul(out.rad, (1 - er) / cosineTerm(out));
+                    out.dp = 1 - p;
+                    out.inside = curOut;
+                }
+            }
+            out.d = normalize(out.d);
+            break;
+        }
+        default: { /* ORC_EMISSIVE, emissiveObj.h:15-28 */
+            out.d = V(0, 0, 0);
+            out.inside = ORC_NONE;
+            out.rad = dot(in.d, N) <= 0 ? ob.color : V(0, 0, 0);
+            break;
+        }
+        }
+        return out;
+    }
+
+    Hit trace(const Ray& r, bool noself, unsigned inside, unsigned contact) const {
+        Hit h;
+        unsigned g = 0;
+        float d = s->intersect(r, g, noself, inside, contact);
+        if (d >= 0) {
+            h.valid = true;
+            h.dist = d;
+            h.obj = s->srcObj[g];
+            h.tri = s->srcTri[g];
+        }
+        return h;
+    }
+
+    /* One eye path (mcRenderer.cpp:16-57) and its weight (pathTracer.cpp:22-69). */
+    V sample(unsigned p, unsigned pass, unsigned* nv) const {
+        float rn[4];
+        rng4(seed, p, pass, 0, rn);
+        Ray ray;
+        ray.o = cam.pos;
+        ray.d = cam.dir(p, rn[0], rn[1]);
+        Hit hit = trace(ray, false, 0, 0);
+        unsigned inside = s->findInside(ray, ORC_NONE);
+        /* camera vertex: cosineTerm = |front . d| (camera.cpp:182-186, ray.cpp:52-58) */
+        float c0 = std::fabs(dot(normalize(sub(cam.focus, cam.pos)), ray.d));
+        c0 = c0 > EPS_D ? c0 : (float)EPS_D;
+        V T = mul(V(1, 1, 1), c0);
+        T = mulv(T, divs(divs(V(1, 1, 1), 1.f), 1.f));
+        V color(0, 0, 0);
+        unsigned depth = 0, verts = 1;
+        for (;;) {
+            unsigned self;
+            if (inside != ORC_NONE) self = inside;
+            else if (hit.valid) self = hit.obj;
+            else { verts++; break; }  /* terminateRay: contributes 0 */
+            rng4(seed, p, pass, depth + 1, rn);
+            Out out = scatter(self, ray, hit, inside, rn);
+            verts++;
+            if (length(out.d) < 0.5) {  /* emission / absorption ends the path */
+                color = mulv(T, divs(divs(out.rad, out.op), out.dp));
+                break;
+            }
+            if (depth + 1 > maxDepth) { verts++; break; }
+            Ray nr;
+            nr.o = out.o;
+            nr.d = out.d;
+            Hit nh = trace(nr, true, out.inside, out.contact);
+            if (!nh.valid) { verts++; break; }
+            /* out becomes an inner vertex: cosine term, decay (1 for these materials), weight */
+            T = mul(T, cosineTerm(out));
+            T = mulv(T, divs(divs(out.rad, out.op), out.dp));
+            ray = nr;
+            hit = nh;
+            inside = out.inside;
+            depth++;
+        }
+        if (nv) *nv = verts;
+        if (!legal(color)) color = V(0, 0, 0);
+        return cam.fixVignetting(color, p);
+    }
+};
+
+}  // namespace orc
+
+using namespace orc;
+
+/* ---- C ABI -------------------------------------------------------------- */
+extern "C" {
+
+void* orc_tree_build(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt) {
+    Tree* t = new Tree;
+    t->verts.resize(nv);
+    for (uint32_t i = 0; i < nv; i++) t->verts[i] = V(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+    t->idx.assign(tris, tris + 3 * (size_t)nt);
+    t->build();
+    return t;
+}
+
+void orc_tree_free(void* tree) { delete (Tree*)tree; }
+
+static void count_nodes(const Node* n, uint32_t& nodes, uint32_t& ltris, uint32_t& leaves) {
+    if (!n) return;
+    nodes++;
+    if (!n->left && !n->right) { ltris += (uint32_t)n->tris.size(); leaves++; return; }
+    count_nodes(n->left, nodes, ltris, leaves);
+    count_nodes(n->right, nodes, ltris, leaves);
+}
+
+void orc_tree_stats(void* tree, uint32_t* n_nodes, uint32_t* n_leaf_tris, uint32_t* depth, uint32_t* n_leaves) {
+    Tree* t = (Tree*)tree;
+    uint32_t a = 0, b = 0, c = 0;
+    if (t->root && !t->root->tris.empty()) count_nodes(t->root, a, b, c);
+    *n_nodes = a; *n_leaf_tris = b; *depth = t->depth; *n_leaves = c;
+}
+
+static void export_rec(const Node* n, uint32_t& ni, uint32_t& li, float* box, uint32_t* right,
+                       uint32_t* lf, uint32_t* lc, uint32_t* tl) {
+    uint32_t me = ni++;
+    memcpy(box + 6 * (size_t)me, n->box, 6 * sizeof(float));
+    if (!n->left && !n->right) {
+        right[me] = ORC_NONE;
+        lf[me] = li;
+        lc[me] = (uint32_t)n->tris.size();
+        for (unsigned t : n->tris) tl[li++] = t;
+        return;
+    }
+    lf[me] = 0;
+    lc[me] = 0;
+    export_rec(n->left, ni, li, box, right, lf, lc, tl);
+    right[me] = ni;
+    export_rec(n->right, ni, li, box, right, lf, lc, tl);
+}
+
+void orc_tree_export(void* tree, float* box, uint32_t* right, uint32_t* leaf_first,
+                     uint32_t* leaf_count, uint32_t* tri_list) {
+    Tree* t = (Tree*)tree;
+    if (!t->root || t->root->tris.empty()) return;
+    uint32_t ni = 0, li = 0;
+    export_rec(t->root, ni, li, box, right, leaf_first, leaf_count, tri_list);
+}
+
+void orc_tree_intersect(void* tree, const float* rays, uint64_t n, uint32_t* prim, float* t, int nthreads) {
+    Tree* tr = (Tree*)tree;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+        Ray r;
+        r.o = V(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]);
+        r.d = V(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
+        unsigned tid = ORC_NONE;
+        float d = tr->intersect(r, tid, NoCond{});
+        prim[i] = d >= 0 ? tid : ORC_NONE;
+        t[i] = d >= 0 ? d : -1.f;
+    }
+}
+
+float orc_box_intersect(const float* box6, const float* ray6) {
+    Ray r;
+    r.o = V(ray6[0], ray6[1], ray6[2]);
+    r.d = V(ray6[3], ray6[4], ray6[5]);
+    return Tree::boxIsect(box6, r);
+}
+
+float orc_tri_intersect(const float* v0, const float* v1, const float* v2, const float* ray6) {
+    Tree t;
+    t.verts = {V(v0[0], v0[1], v0[2]), V(v1[0], v1[1], v1[2]), V(v2[0], v2[1], v2[2])};
+    t.idx = {0, 1, 2};
+    Ray r;
+    r.o = V(ray6[0], ray6[1], ray6[2]);
+    r.d = V(ray6[3], ray6[4], ray6[5]);
+    return t.triIsect(r, 0, NoCond{});
+}
+
+void* orc_scene_create(void) { return new Scene; }
+void orc_scene_free(void* s) { delete (Scene*)s; }
+
+uint32_t orc_scene_add_object(void* sp, const float* verts, uint32_t nv, const uint32_t* tris,
+                              uint32_t nt, const float* normals, int mat, const float* params) {
+    Scene* s = (Scene*)sp;
+    Object o;
+    o.verts.resize(nv);
+    for (uint32_t i = 0; i < nv; i++) o.verts[i] = V(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+    o.idx.assign(tris, tris + 3 * (size_t)nt);
+    if (normals) {
+        o.normals.resize(nv);
+        for (uint32_t i = 0; i < nv; i++) o.normals[i] = V(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]);
+    }
+    o.mat = mat;
+    o.power = 0;
+    o.ior = 1;
+    if (mat == ORC_REFRACTIVE) o.ior = params[0];
+    else o.color = V(params[0], params[1], params[2]);
+    if (mat == ORC_GLOSSY) o.power = params[3];
+    s->objs.push_back(o);
+    return (uint32_t)(s->objs.size() - 1);
+}
+
+void orc_scene_build(void* s) { ((Scene*)s)->build(); }
+
+void orc_scene_intersect(void* sp, const float* rays, const uint32_t* inside, const uint32_t* contact,
+                         uint64_t n, int noself, uint32_t* prim, uint32_t* obj, uint32_t* tri,
+                         float* t, float* b1, float* b2, int nthreads) {
+    Scene* s = (Scene*)sp;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+        Ray r;
+        r.o = V(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]);
+        r.d = V(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
+        unsigned g = ORC_NONE;
+        unsigned in = inside ? inside[i] : ORC_NONE, ct = contact ? contact[i] : ORC_NONE;
+        float d = s->intersect(r, g, noself != 0, in, ct);
+        if (d >= 0) {
+            prim[i] = g;
+            obj[i] = s->srcObj[g];
+            tri[i] = s->srcTri[g];
+            t[i] = d;
+            float u1 = 0, u2 = 0;
+            s->worldNormal(s->srcObj[g], s->srcTri[g], add(r.o, mul(r.d, d)), &u1, &u2);
+            b1[i] = u1;
+            b2[i] = u2;
+        } else {
+            prim[i] = ORC_NONE; obj[i] = ORC_NONE; tri[i] = ORC_NONE;
+            t[i] = -1.f; b1[i] = 0.f; b2[i] = 0.f;
+        }
+    }
+}
+
+void orc_scene_find_inside(void* sp, const float* rays, const uint32_t* cur, uint64_t n,
+                           uint32_t* out, int nthreads) {
+    Scene* s = (Scene*)sp;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+        Ray r;
+        r.o = V(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]);
+        r.d = V(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
+        out[i] = s->findInside(r, cur ? cur[i] : ORC_NONE);
+    }
+}
+
+void orc_world_normal(void* sp, uint32_t obj, uint32_t tri, const float* p, float* n_out,
+                      float* u1_out, float* u2_out) {
+    Scene* s = (Scene*)sp;
+    V n = s->worldNormal(obj, tri, V(p[0], p[1], p[2]), u1_out, u2_out);
+    n_out[0] = n.x; n_out[1] = n.y; n_out[2] = n.z;
+}
+
+void orc_render(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t seed, uint32_t pass0,
+                uint32_t npass, double* sum, double* sumsq, int nthreads) {
+    Scene* s = (Scene*)sp;
+    Tracer tr{s, Cam(c), max_depth, seed};
+    const int64_t np = (int64_t)c->width * c->height;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    for (uint32_t pass = pass0; pass < pass0 + npass; pass++) {
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+        for (int64_t p = 0; p < np; p++) {
+            V v = tr.sample((unsigned)p, pass, nullptr);
+            for (int k = 0; k < 3; k++) {
+                sum[3 * p + k] += v[k];
+                sumsq[3 * p + k] += (double)v[k] * v[k];
+            }
+        }
+    }
+}
+
+void orc_render_sample(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t seed,
+                       uint32_t pixel, uint32_t pass, float* rgb, uint32_t* n_vertices) {
+    Scene* s = (Scene*)sp;
+    Tracer tr{s, Cam(c), max_depth, seed};
+    V v = tr.sample(pixel, pass, n_vertices);
+    rgb[0] = v.x; rgb[1] = v.y; rgb[2] = v.z;
+}
+
+void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox(ctr, key, out); }
+
+void orc_rng4(uint64_t seed, uint32_t pixel, uint32_t pass, uint32_t vertex, float out[4]) {
+    rng4(seed, pixel, pass, vertex, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+/*
+ * oracle.h -- C ABI of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+ *
+ * The oracle is a plain C++ restatement of the WalkRay reference's path-tracing
+ * hot path, written to follow the reference's algorithms and floating-point
+ * operation order.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker / CPU baseline: the
+ * product library (walkray_amd/, include/walkray.h) never calls it.
+ *
+ * Every function below names the reference file:line it restates.
+ */
+#ifndef WALKRAY_ORACLE_H
+#define WALKRAY_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_NONE 0xFFFFFFFFu
+
+/* ---- KDTree (kdtree.h:9-48, kdtree.cpp) ------------------------------- */
+/* Builds KDTree::build(BEST) (kdtree.cpp:242-258 -> splitBest 141-240). */
+void*    orc_tree_build(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt);
+void     orc_tree_free(void* tree);
+/* Canonical preorder export: node count, leaf-triangle count, depth. */
+void     orc_tree_stats(void* tree, uint32_t* n_nodes, uint32_t* n_leaf_tris, uint32_t* depth, uint32_t* n_leaves);
+/* box[6*i] = min xyz, max xyz; right[i] = preorder index of right child or ORC_NONE for a leaf;
+ * leaf_first/leaf_count index into tri_list (valid for leaves). */
+void     orc_tree_export(void* tree, float* box, uint32_t* right, uint32_t* leaf_first,
+                         uint32_t* leaf_count, uint32_t* tri_list);
+/* KDTree::intersect (kdtree.cpp:349-353, 308-347, 279-306, 46-64), no Condition.
+ * rays: 6 floats per ray (origin xyz, direction xyz).  prim = ORC_NONE, t = -1 on a miss. */
+void     orc_tree_intersect(void* tree, const float* rays, uint64_t n, uint32_t* prim, float* t,
+                            int nthreads);
+/* Reference box test Node::BoundingBox::intersect (kdtree.cpp:46-64) on a single box. */
+float    orc_box_intersect(const float* box6, const float* ray6);
+/* Reference triangle test (kdtree.cpp:279-306) without Condition. */
+float    orc_tri_intersect(const float* v0, const float* v1, const float* v2, const float* ray6);
+
+/* ---- Scene (scene.cpp) with materials (diffuse/glossy/refractive/emissive) ---- */
+enum { ORC_DIFFUSE = 0, ORC_GLOSSY = 1, ORC_REFRACTIVE = 2, ORC_EMISSIVE = 3 };
+void*    orc_scene_create(void);
+void     orc_scene_free(void* s);
+/* params: diffuse {r,g,b}; glossy {r,g,b,power}; refractive {ior}; emissive {r,g,b}.
+ * normals: per-vertex normals (nv*3) or NULL (flat shading, mesh.h:59-65).  Returns object id. */
+uint32_t orc_scene_add_object(void* s, const float* verts, uint32_t nv, const uint32_t* tris,
+                              uint32_t nt, const float* normals, int mat, const float* params);
+/* Scene::buildKDTree + buildObjKDTrees (scene.cpp:20-65). */
+void     orc_scene_build(void* s);
+/* Scene::intersect (scene.cpp:102-113), optionally with NoSelfCondition (noself.cpp:13-20).
+ * rays: 6 floats per ray; inside/contact: object ids (ORC_NONE = NULL).
+ * out: prim (global id), obj, tri, t, b1, b2 (barycentrics of mesh.h:67-72 at o+d*t). */
+void     orc_scene_intersect(void* s, const float* rays, const uint32_t* inside,
+                             const uint32_t* contact, uint64_t n, int noself, uint32_t* prim,
+                             uint32_t* obj, uint32_t* tri, float* t, float* b1, float* b2,
+                             int nthreads);
+/* Scene::findInsideObject (scene.cpp:115-144); cur = ORC_NONE for NULL. */
+void     orc_scene_find_inside(void* s, const float* rays, const uint32_t* cur, uint64_t n,
+                               uint32_t* out, int nthreads);
+/* Mesh::getWorldNormal (mesh.h:47-77) for object obj, local triangle tri, at position p. */
+void     orc_world_normal(void* s, uint32_t obj, uint32_t tri, const float* p, float* n_out,
+                          float* u1_out, float* u2_out);
+
+/* ---- Path tracer (camera.cpp, mcRenderer.cpp:16-57, pathTracer.cpp:7-76) ---- */
+typedef struct {
+    float pos[3], focus[3], up[3];
+    uint32_t width, height;
+    float sight_dist;
+} orc_camera;
+/* Renders passes [pass0, pass0+npass) of the reference PathTracer estimator with
+ * the counter RNG.  For every pixel (row-major, top row first) accumulates, in
+ * FP64, sum[3*p+c] += fixVignetting(color) and sumsq[3*p+c] += value^2.
+ * The buffers are accumulated into (caller zeroes them). */
+void     orc_render(void* s, const orc_camera* cam, uint32_t max_depth, uint64_t seed,
+                    uint32_t pass0, uint32_t npass, double* sum, double* sumsq, int nthreads);
+/* Single pixel, single pass: returns the per-sample value and the path length;
+ * used by the tests that compare individual paths. */
+void     orc_render_sample(void* s, const orc_camera* cam, uint32_t max_depth, uint64_t seed,
+                           uint32_t pixel, uint32_t pass, float* rgb, uint32_t* n_vertices);
+
+/* ---- Counter RNG (Philox-4x32-10, Salmon et al., SC'11) ---------------- */
+void     orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* 4 uniform floats in [0,1) for (seed, pixel, pass, vertex). */
+void     orc_rng4(uint64_t seed, uint32_t pixel, uint32_t pass, uint32_t vertex, float out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,203 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker and the CPU baseline.  The product
+(walkray_amd) never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+NONE = 0xFFFFFFFF
+
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        vp, u32, u64, f32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float
+        P = C.POINTER
+        L.orc_tree_build.restype = vp
+        L.orc_tree_build.argtypes = [vp, u32, vp, u32]
+        L.orc_tree_free.argtypes = [vp]
+        L.orc_tree_stats.argtypes = [vp, P(u32), P(u32), P(u32), P(u32)]
+        L.orc_tree_export.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.orc_tree_intersect.argtypes = [vp, vp, u64, vp, vp, C.c_int]
+        L.orc_box_intersect.restype = f32
+        L.orc_box_intersect.argtypes = [vp, vp]
+        L.orc_tri_intersect.restype = f32
+        L.orc_tri_intersect.argtypes = [vp, vp, vp, vp]
+        L.orc_scene_create.restype = vp
+        L.orc_scene_free.argtypes = [vp]
+        L.orc_scene_add_object.restype = u32
+        L.orc_scene_add_object.argtypes = [vp, vp, u32, vp, u32, vp, C.c_int, vp]
+        L.orc_scene_build.argtypes = [vp]
+        L.orc_scene_intersect.argtypes = [vp, vp, vp, vp, u64, C.c_int, vp, vp, vp, vp, vp, vp, C.c_int]
+        L.orc_scene_find_inside.argtypes = [vp, vp, vp, u64, vp, C.c_int]
+        L.orc_world_normal.argtypes = [vp, u32, u32, vp, vp, vp, vp]
+        L.orc_render.argtypes = [vp, vp, u32, u64, u32, u32, vp, vp, C.c_int]
+        L.orc_render_sample.argtypes = [vp, vp, u32, u64, u32, u32, vp, vp]
+        L.orc_philox4x32.argtypes = [vp, vp, vp]
+        L.orc_rng4.argtypes = [u64, u32, u32, u32, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("focus", C.c_float * 3), ("up", C.c_float * 3),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("sight_dist", C.c_float)]
+
+    @classmethod
+    def from_dict(cls, cam):
+        c = cls()
+        c.pos[:] = cam["pos"]
+        c.focus[:] = cam["focus"]
+        c.up[:] = cam["up"]
+        c.width = cam["width"]
+        c.height = cam["height"]
+        c.sight_dist = cam["sight_dist"]
+        return c
+
+
+class Tree:
+    """KDTree restatement (kdtree.h:9-48): build(BEST) + intersect without Condition."""
+
+    def __init__(self, verts, tris):
+        self.verts = np.ascontiguousarray(verts, dtype=np.float32)
+        self.tris = np.ascontiguousarray(tris, dtype=np.uint32)
+        self.h = lib().orc_tree_build(_p(self.verts), len(self.verts), _p(self.tris), len(self.tris))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_tree_free(self.h)
+            self.h = None
+
+    def stats(self):
+        a, b, c, d = (C.c_uint32() for _ in range(4))
+        lib().orc_tree_stats(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
+        return {"nodes": a.value, "leaf_tris": b.value, "depth": c.value, "leaves": d.value}
+
+    def export(self):
+        st = self.stats()
+        n, m = st["nodes"], st["leaf_tris"]
+        box = np.zeros((n, 6), np.float32)
+        right = np.zeros(n, np.uint32)
+        lf = np.zeros(n, np.uint32)
+        lc = np.zeros(n, np.uint32)
+        tl = np.zeros(max(m, 1), np.uint32)
+        lib().orc_tree_export(self.h, _p(box), _p(right), _p(lf), _p(lc), _p(tl))
+        return {"box": box, "right": right, "leaf_first": lf, "leaf_count": lc, "tri_list": tl[:m]}
+
+    def intersect(self, rays, nthreads=0):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = len(rays)
+        prim = np.empty(n, np.uint32)
+        t = np.empty(n, np.float32)
+        lib().orc_tree_intersect(self.h, _p(rays), n, _p(prim), _p(t), nthreads)
+        return prim, t
+
+
+def box_intersect(box6, ray6):
+    b = np.ascontiguousarray(box6, dtype=np.float32)
+    r = np.ascontiguousarray(ray6, dtype=np.float32)
+    return float(lib().orc_box_intersect(_p(b), _p(r)))
+
+
+def tri_intersect(v0, v1, v2, ray6):
+    a = [np.ascontiguousarray(x, dtype=np.float32) for x in (v0, v1, v2, ray6)]
+    return float(lib().orc_tri_intersect(*[_p(x) for x in a]))
+
+
+class Scene:
+    """Scene restatement (scene.cpp) with the four hot materials."""
+
+    def __init__(self, objs):
+        L = lib()
+        self.h = L.orc_scene_create()
+        self._keep = []
+        for o in objs:
+            params = np.array(list(o["params"]) + [0.0] * 4, dtype=np.float32)
+            normals = o["normals"]
+            self._keep += [o["verts"], o["tris"], normals, params]
+            L.orc_scene_add_object(self.h, _p(o["verts"]), len(o["verts"]), _p(o["tris"]), len(o["tris"]),
+                                   _p(normals), o["mat"], _p(params))
+        L.orc_scene_build(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_scene_free(self.h)
+            self.h = None
+
+    def intersect(self, rays, inside=None, contact=None, noself=False, nthreads=0):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = len(rays)
+        out = {k: np.empty(n, np.uint32) for k in ("prim", "obj", "tri")}
+        out.update({k: np.empty(n, np.float32) for k in ("t", "b1", "b2")})
+        ins = None if inside is None else np.ascontiguousarray(inside, dtype=np.uint32)
+        con = None if contact is None else np.ascontiguousarray(contact, dtype=np.uint32)
+        lib().orc_scene_intersect(self.h, _p(rays), _p(ins), _p(con), n, int(noself), _p(out["prim"]),
+                                  _p(out["obj"]), _p(out["tri"]), _p(out["t"]), _p(out["b1"]),
+                                  _p(out["b2"]), nthreads)
+        return out
+
+    def find_inside(self, rays, cur=None, nthreads=0):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = len(rays)
+        out = np.empty(n, np.uint32)
+        cu = None if cur is None else np.ascontiguousarray(cur, dtype=np.uint32)
+        lib().orc_scene_find_inside(self.h, _p(rays), _p(cu), n, _p(out), nthreads)
+        return out
+
+    def world_normal(self, obj, tri, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        n = np.zeros(3, np.float32)
+        u1, u2 = C.c_float(), C.c_float()
+        lib().orc_world_normal(self.h, obj, tri, _p(p), _p(n), C.byref(u1), C.byref(u2))
+        return n, u1.value, u2.value
+
+    def render(self, cam, spp, max_depth, seed, pass0=0, nthreads=0, sums=None):
+        c = Camera.from_dict(cam)
+        npx = cam["width"] * cam["height"]
+        if sums is None:
+            sums = (np.zeros(3 * npx, np.float64), np.zeros(3 * npx, np.float64))
+        lib().orc_render(self.h, C.byref(c), max_depth, seed, pass0, spp, _p(sums[0]), _p(sums[1]), nthreads)
+        return sums
+
+    def render_sample(self, cam, max_depth, seed, pixel, pss):
+        c = Camera.from_dict(cam)
+        rgb = np.zeros(3, np.float32)
+        nv = C.c_uint32()
+        lib().orc_render_sample(self.h, C.byref(c), max_depth, seed, pixel, pss, _p(rgb), C.byref(nv))
+        return rgb, nv.value
+
+
+def philox4x32(ctr, key):
+    c = np.array(ctr, dtype=np.uint32)
+    k = np.array(key, dtype=np.uint32)
+    o = np.zeros(4, np.uint32)
+    lib().orc_philox4x32(_p(c), _p(k), _p(o))
+    return o
+
+
+def rng4(seed, pixel, pss, vertex):
+    o = np.zeros(4, np.float32)
+    lib().orc_rng4(seed, pixel, pss, vertex, _p(o))
+    return o

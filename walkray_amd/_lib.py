@@ -1,0 +1,323 @@
+"""ctypes binding of libwalkray's C ABI (include/walkray.h).
+
+Loading fails loudly when the HIP library has not been built: there is no
+CPU fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwalkray.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "walkray.h")
+NONE = 0xFFFFFFFF
+OVERFLOW = 0xFFFFFFFE
+
+WR_DIFFUSE, WR_GLOSSY, WR_REFRACTIVE, WR_EMISSIVE = 0, 1, 2, 3
+WR_BEST = 1
+
+_lib = None
+
+
+class WalkRayError(RuntimeError):
+    pass
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_int32), ("color", C.c_float * 3), ("power", C.c_float), ("ior", C.c_float)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("n_objects", C.c_uint32), ("n_triangles", C.c_uint32), ("n_nodes", C.c_uint32),
+                ("n_leaves", C.c_uint32), ("depth", C.c_uint32), ("max_obj_depth", C.c_uint32),
+                ("device_bytes", C.c_uint64), ("build_seconds", C.c_double)]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("focus", C.c_float * 3), ("up", C.c_float * 3),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("sight_dist", C.c_float)]
+
+    @classmethod
+    def from_dict(cls, cam):
+        c = cls()
+        c.pos[:] = cam["pos"]
+        c.focus[:] = cam["focus"]
+        c.up[:] = cam["up"]
+        c.width = cam["width"]
+        c.height = cam["height"]
+        c.sight_dist = cam["sight_dist"]
+        return c
+
+
+def header_symbols():
+    """Function names declared in include/walkray.h."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(wr_[a-z0-9_]+)\s*\(", txt)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise WalkRayError(f"libwalkray.so is not built ({LIB_PATH}); run __graft_entry__.build() "
+                           "or make -C walkray_amd")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
+    P = C.POINTER
+    sig = {
+        "wr_ctx_create": (i32, [i32, P(vp)]),
+        "wr_ctx_destroy": (None, [vp]),
+        "wr_last_error": (C.c_char_p, [vp]),
+        "wr_synchronize": (i32, [vp]),
+        "wr_device_alloc": (i32, [vp, C.c_size_t, P(vp)]),
+        "wr_device_free": (i32, [vp, vp]),
+        "wr_memcpy_htod": (i32, [vp, vp, vp, C.c_size_t]),
+        "wr_memcpy_dtoh": (i32, [vp, vp, vp, C.c_size_t]),
+        "wr_timer_start": (i32, [vp]),
+        "wr_timer_stop": (i32, [vp, P(C.c_float)]),
+        "wr_scene_create": (i32, [vp, P(vp)]),
+        "wr_scene_destroy": (None, [vp]),
+        "wr_scene_add_mesh": (i32, [vp, vp, u32, vp, u32, vp, P(Material), P(u32)]),
+        "wr_scene_build": (i32, [vp, i32]),
+        "wr_scene_get_info": (i32, [vp, P(SceneInfo)]),
+        "wr_scene_prim_source": (i32, [vp, u32, P(u32), P(u32)]),
+        "wr_scene_export_tree": (i32, [vp, u32, P(u32), P(u32), vp, vp, vp, vp, vp]),
+        "wr_tree_build": (i32, [vp, u32, vp, u32, i32, P(vp)]),
+        "wr_tree_export": (i32, [vp, P(u32), P(u32), P(u32), vp, vp, vp, vp, vp]),
+        "wr_tree_free": (None, [vp]),
+        "wr_closest_hit": (i32, [vp, vp, vp, u64, vp, vp]),
+        "wr_closest_hit_noself": (i32, [vp, vp, vp, u64, vp]),
+        "wr_inside_object": (i32, [vp, vp, vp, u64, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc, ctx=None, what=""):
+    if rc != 0:
+        msg = lib().wr_last_error(ctx)
+        raise WalkRayError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def build_tree_export(verts, tris, nthreads=0):
+    """Host-only builder run + canonical export (same format as the oracle's Tree.export)."""
+    L = lib()
+    v = np.ascontiguousarray(verts, dtype=np.float32)
+    t = np.ascontiguousarray(tris, dtype=np.uint32)
+    h = C.c_void_p()
+    _check(L.wr_tree_build(_p(v), len(v), _p(t), len(t), nthreads, C.byref(h)), None, "wr_tree_build")
+    try:
+        n, m, d = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(L.wr_tree_export(h, C.byref(n), C.byref(m), C.byref(d), None, None, None, None, None), None,
+               "wr_tree_export")
+        box = np.zeros((n.value, 6), np.float32)
+        right, lf, lc = (np.zeros(n.value, np.uint32) for _ in range(3))
+        tl = np.zeros(max(m.value, 1), np.uint32)
+        _check(L.wr_tree_export(h, C.byref(n), C.byref(m), C.byref(d), _p(box), _p(right), _p(lf), _p(lc),
+                                _p(tl)), None, "wr_tree_export")
+        return {"box": box, "right": right, "leaf_first": lf, "leaf_count": lc, "tri_list": tl[:m.value],
+                "depth": d.value}
+    finally:
+        L.wr_tree_free(h)
+
+
+class DeviceBuffer:
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        _check(lib().wr_device_alloc(ctx.h, max(self.nbytes, 1), C.byref(self.ptr)), ctx.h, "wr_device_alloc")
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _check(lib().wr_memcpy_htod(self.ctx.h, self.ptr, _p(arr), arr.nbytes), self.ctx.h, "wr_memcpy_htod")
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib().wr_memcpy_dtoh(self.ctx.h, _p(out), self.ptr, out.nbytes), self.ctx.h, "wr_memcpy_dtoh")
+        return out
+
+    def free(self):
+        if self.ptr and self.ptr.value:
+            lib().wr_device_free(self.ctx.h, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One HIP device + stream (wr_ctx)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        _check(lib().wr_ctx_create(device, C.byref(self.h)), None, "wr_ctx_create")
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        b = DeviceBuffer(self, arr.nbytes)
+        b.upload(arr)
+        return b
+
+    def synchronize(self):
+        _check(lib().wr_synchronize(self.h), self.h, "wr_synchronize")
+
+    def timer_start(self):
+        _check(lib().wr_timer_start(self.h), self.h, "wr_timer_start")
+
+    def timer_stop(self):
+        ms = C.c_float()
+        _check(lib().wr_timer_stop(self.h, C.byref(ms)), self.h, "wr_timer_stop")
+        return ms.value
+
+    def close(self):
+        if self.h:
+            lib().wr_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+HIT_DTYPE = np.dtype([("prim", np.uint32), ("t", np.float32), ("b1", np.float32), ("b2", np.float32)])
+
+
+class Scene:
+    """wr_scene: objects in world space, built with the reference builder and uploaded."""
+
+    def __init__(self, ctx, objs=None, build=True):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        _check(lib().wr_scene_create(ctx.h, C.byref(self.h)), ctx.h, "wr_scene_create")
+        for o in objs or []:
+            self.add(o)
+        if objs is not None and build:
+            self.build()
+
+    def add(self, o):
+        m = Material()
+        m.type = o["mat"]
+        p = list(o["params"])
+        if o["mat"] == WR_REFRACTIVE:
+            m.ior = p[0]
+        else:
+            m.color[:] = p[:3]
+            if o["mat"] == WR_GLOSSY:
+                m.power = p[3]
+        v = np.ascontiguousarray(o["verts"], dtype=np.float32)
+        t = np.ascontiguousarray(o["tris"], dtype=np.uint32)
+        n = None if o["normals"] is None else np.ascontiguousarray(o["normals"], dtype=np.float32)
+        oid = C.c_uint32()
+        _check(lib().wr_scene_add_mesh(self.h, _p(v), len(v), _p(t), len(t), _p(n), C.byref(m), C.byref(oid)),
+               self.ctx.h, "wr_scene_add_mesh")
+        return oid.value
+
+    def build(self):
+        _check(lib().wr_scene_build(self.h, WR_BEST), self.ctx.h, "wr_scene_build")
+
+    def info(self):
+        i = SceneInfo()
+        _check(lib().wr_scene_get_info(self.h, C.byref(i)), self.ctx.h, "wr_scene_get_info")
+        return {k: getattr(i, k) for k, _ in SceneInfo._fields_}
+
+    def prim_source(self, prim):
+        o, t = C.c_uint32(), C.c_uint32()
+        _check(lib().wr_scene_prim_source(self.h, prim, C.byref(o), C.byref(t)), self.ctx.h, "wr_scene_prim_source")
+        return o.value, t.value
+
+    def export_tree(self, obj=NONE):
+        L = lib()
+        n, m = C.c_uint32(), C.c_uint32()
+        _check(L.wr_scene_export_tree(self.h, obj, C.byref(n), C.byref(m), None, None, None, None, None),
+               self.ctx.h, "wr_scene_export_tree")
+        box = np.zeros((n.value, 6), np.float32)
+        right, lf, lc = (np.zeros(n.value, np.uint32) for _ in range(3))
+        tl = np.zeros(max(m.value, 1), np.uint32)
+        _check(L.wr_scene_export_tree(self.h, obj, C.byref(n), C.byref(m), _p(box), _p(right), _p(lf), _p(lc),
+                                      _p(tl)), self.ctx.h, "wr_scene_export_tree")
+        return {"box": box, "right": right, "leaf_first": lf, "leaf_count": lc, "tri_list": tl[:m.value]}
+
+    # -- device-pointer entry points (inputs already resident) -------------
+    def closest_hit_device(self, d_rays, n, d_hits, d_counters=None):
+        _check(lib().wr_closest_hit(self.ctx.h, self.h, d_rays.ptr, n, d_hits.ptr,
+                                    d_counters.ptr if d_counters is not None else None),
+               self.ctx.h, "wr_closest_hit")
+
+    # -- host convenience wrappers ------------------------------------------
+    def closest_hit(self, rays, counters=False):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        n = len(rays)
+        dr = self.ctx.to_device(rays)
+        dh = self.ctx.alloc(16 * max(n, 1))
+        dc = None
+        if counters:
+            dc = self.ctx.to_device(np.zeros(3, np.uint64))
+        self.closest_hit_device(dr, n, dh, dc)
+        self.ctx.synchronize()
+        hits = dh.download(np.uint32, 4 * n).view(HIT_DTYPE) if n else np.zeros(0, HIT_DTYPE)
+        if counters:
+            return hits, dc.download(np.uint64, 3)
+        return hits
+
+    def closest_hit_noself(self, rays, inside, contact):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        n = len(rays)
+        pr = np.empty((n, 8), np.uint32)
+        pr[:, :6] = rays.view(np.uint32)
+        pr[:, 6] = inside
+        pr[:, 7] = contact
+        dr = self.ctx.to_device(pr)
+        dh = self.ctx.alloc(16 * max(n, 1))
+        _check(lib().wr_closest_hit_noself(self.ctx.h, self.h, dr.ptr, n, dh.ptr), self.ctx.h,
+               "wr_closest_hit_noself")
+        self.ctx.synchronize()
+        return dh.download(np.uint32, 4 * n).view(HIT_DTYPE) if n else np.zeros(0, HIT_DTYPE)
+
+    def inside_object(self, rays, cur=None):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        n = len(rays)
+        pr = np.empty((n, 8), np.uint32)
+        pr[:, :6] = rays.view(np.uint32)
+        pr[:, 6] = NONE
+        pr[:, 7] = NONE if cur is None else cur
+        dr = self.ctx.to_device(pr)
+        do = self.ctx.alloc(4 * max(n, 1))
+        _check(lib().wr_inside_object(self.ctx.h, self.h, dr.ptr, n, do.ptr), self.ctx.h, "wr_inside_object")
+        self.ctx.synchronize()
+        return do.download(np.uint32, n) if n else np.zeros(0, np.uint32)
+
+    def close(self):
+        if self.h:
+            lib().wr_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,348 @@
+// builder.cpp -- the reference KD-tree builder, re-implemented for large
+// scenes, and its flattening to the GPU layout of wr_layout.h.
+//
+// Algorithm kept from KDTree::splitBest (kdtree.cpp:141-240); it must be
+// identical because tie-breaks between equal-distance hits and the reference's
+// box-test rounding both depend on the exact tree:
+//   * node box = componentwise min/max over every vertex of every triangle,
+//     scanned in list order with `p < m ? p : m` (kdtree.cpp:5-21,151-165);
+//   * leaf if the node holds <= 1 triangle (maxLeafTriangleNum = 1,
+//     kdtree.cpp:25,170);
+//   * for dim 0,1,2: split = element of rank floor(U/2) among the coordinates
+//     of the node's U distinct vertex indices (kdtree.cpp:154-183); leftCount =
+//     triangles with any vertex coordinate <= split; keep the dim whose
+//     |leftCount/n - 0.5| (evaluated in double) is strictly smallest
+//     (kdtree.cpp:187-208);
+//   * leaf if the best ratio is 0 or 1 (kdtree.cpp:211-215);
+//   * order-preserving partition, "any vertex <= split" goes left
+//     (kdtree.cpp:220-236).
+// What differs is only how it runs: one shared triangle-index array is
+// partitioned in place (stable), distinct vertices are found with a per-thread
+// stamp array instead of an unordered_set, large subtrees are built on their
+// own threads, and the recursion leaves the triangles of each leaf contiguous
+// and in left-to-right leaf order, which is exactly the record order the GPU
+// layout wants.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <thread>
+
+#include "wr_host.h"
+
+namespace wr {
+namespace {
+
+struct BNode {
+    uint32_t b = 0, e = 0;
+    float box[6];
+    std::unique_ptr<BNode> left, right;
+};
+
+struct Scratch {
+    std::vector<uint32_t> stamp;
+    uint32_t stamp_id = 0;
+    std::vector<uint32_t> uniq;
+    std::vector<float> coords;
+};
+
+struct Builder {
+    const float* V;
+    const uint32_t* I;
+    uint32_t nv, nt;
+    std::vector<uint32_t> order, tmp;
+    std::atomic<int> spare{0};
+    uint32_t max_depth = 0;
+    std::atomic<uint32_t> depth_seen{0};
+    static constexpr uint32_t kParallelMin = 1u << 14;
+
+    float c(uint32_t vi, int dim) const { return V[3 * (size_t)vi + dim]; }
+
+    void note_depth(uint32_t d) {
+        uint32_t cur = depth_seen.load(std::memory_order_relaxed);
+        while (d > cur && !depth_seen.compare_exchange_weak(cur, d)) {
+        }
+    }
+
+    void split(BNode* node, Scratch& s, uint32_t depth) {
+        const uint32_t b = node->b, e = node->e, n = e - b;
+        if (n == 0) return;
+        note_depth(depth);
+        float mn[3], mx[3];
+        {
+            const uint32_t v0 = I[3 * (size_t)order[b]];
+            for (int k = 0; k < 3; k++) mn[k] = mx[k] = c(v0, k);
+        }
+        if (s.stamp.size() != nv) s.stamp.assign(nv, 0u);
+        const uint32_t sid = ++s.stamp_id;
+        s.uniq.clear();
+        for (uint32_t i = b; i < e; i++) {
+            const uint32_t* t = I + 3 * (size_t)order[i];
+            for (int k = 0; k < 3; k++) {
+                const uint32_t vi = t[k];
+                for (int a = 0; a < 3; a++) {
+                    const float p = c(vi, a);
+                    mn[a] = p < mn[a] ? p : mn[a];
+                    mx[a] = p > mx[a] ? p : mx[a];
+                }
+                if (s.stamp[vi] != sid) {
+                    s.stamp[vi] = sid;
+                    s.uniq.push_back(vi);
+                }
+            }
+        }
+        for (int a = 0; a < 3; a++) {
+            node->box[a] = mn[a];
+            node->box[3 + a] = mx[a];
+        }
+        if (n <= 1) return;
+
+        float bestRatio = 10.f, bestSplit = 0.f;
+        int bestDim = 0;
+        uint32_t bestLeft = 0;
+        const size_t U = s.uniq.size();
+        s.coords.resize(U);
+        for (int dim = 0; dim < 3; dim++) {
+            for (size_t i = 0; i < U; i++) s.coords[i] = c(s.uniq[i], dim);
+            std::nth_element(s.coords.begin(), s.coords.begin() + (U >> 1), s.coords.end());
+            const float split = s.coords[U >> 1];
+            uint32_t left = 0;
+            for (uint32_t i = b; i < e; i++) {
+                const uint32_t* t = I + 3 * (size_t)order[i];
+                left += (uint32_t)((int)(c(t[0], dim) <= split) | (int)(c(t[1], dim) <= split) | (int)(c(t[2], dim) <= split));
+            }
+            const float ratio = left / float(n);
+            if (std::fabs((double)ratio - 0.5) < std::fabs((double)bestRatio - 0.5)) {
+                bestRatio = ratio;
+                bestDim = dim;
+                bestSplit = split;
+                bestLeft = left;
+            }
+        }
+        if (bestRatio == 0 || bestRatio == 1) return;
+
+        // stable partition of order[b, e): left part first
+        uint32_t wl = b, wr_ = b;
+        for (uint32_t i = b; i < e; i++) {
+            const uint32_t ti = order[i];
+            const uint32_t* t = I + 3 * (size_t)ti;
+            const bool goes_left = ((int)(c(t[0], bestDim) <= bestSplit) | (int)(c(t[1], bestDim) <= bestSplit) |
+                                    (int)(c(t[2], bestDim) <= bestSplit)) != 0;
+            if (goes_left) order[wl++] = ti;
+            else tmp[wr_++] = ti;
+        }
+        std::memcpy(order.data() + wl, tmp.data() + b, (size_t)(wr_ - b) * sizeof(uint32_t));
+        (void)bestLeft;
+
+        node->left.reset(new BNode);
+        node->right.reset(new BNode);
+        node->left->b = b;
+        node->left->e = wl;
+        node->right->b = wl;
+        node->right->e = e;
+
+        BNode* L = node->left.get();
+        BNode* R = node->right.get();
+        if (n >= kParallelMin && spare.fetch_sub(1) > 0) {
+            std::thread th([this, L, depth]() {
+                Scratch s2;
+                split(L, s2, depth + 1);
+            });
+            split(R, s, depth + 1);
+            th.join();
+            spare.fetch_add(1);
+        } else {
+            if (n >= kParallelMin) spare.fetch_add(1);
+            split(L, s, depth + 1);
+            split(R, s, depth + 1);
+        }
+    }
+};
+
+// normalize((v1 - v0) x (v2 - v1)) with the reference's float semantics
+// (kdtree.cpp:288-296; nvVector.h:490-525: sequential dot from 0, sqrt, r > 0
+// guard, true division).
+void ref_normal(const float* v0, const float* v1, const float* v2, float* out) {
+    float l0[3], l1[3];
+    for (int k = 0; k < 3; k++) {
+        l0[k] = v1[k] - v0[k];
+        l1[k] = v2[k] - v1[k];
+    }
+    float n[3] = {l0[1] * l1[2] - l0[2] * l1[1], l0[2] * l1[0] - l0[0] * l1[2], l0[0] * l1[1] - l0[1] * l1[0]};
+    float r2 = 0.f;
+    r2 += n[0] * n[0];
+    r2 += n[1] * n[1];
+    r2 += n[2] * n[2];
+    const float r = std::sqrt(r2);
+    if (r > 0) {
+        for (int k = 0; k < 3; k++) n[k] = n[k] / r;
+    }
+    for (int k = 0; k < 3; k++) out[k] = n[k];
+}
+
+struct Flattener {
+    FlatTree& F;
+    uint32_t leaves = 0;
+    explicit Flattener(FlatTree& f) : F(f) {}
+
+    uint32_t leaf_ref(const BNode* n) {
+        const uint32_t cnt = n->e - n->b;
+        leaves++;
+        uint32_t field = cnt <= WR_CNT_MASK ? cnt : 0u;
+        if (field == 0) F.leaf_cnt[n->b] = cnt;
+        return WR_LEAF_BIT | (field << WR_CNT_SHIFT) | n->b;
+    }
+
+    // Returns the reference to `n`; `up` is (parent << 1 | side) or WR_NONE.
+    uint32_t visit(const BNode* n, uint32_t up) {
+        if (!n->left) {
+            uint32_t r = leaf_ref(n);
+            F.leaf_up[n->b] = up;
+            return r;
+        }
+        const uint32_t k = (uint32_t)F.nodes.size();
+        F.nodes.push_back(wr_node{});
+        F.node_up.push_back(up);
+        const uint32_t lr = visit(n->left.get(), k << 1);
+        const uint32_t rr = visit(n->right.get(), (k << 1) | 1u);
+        wr_node& w = F.nodes[k];
+        for (int a = 0; a < 3; a++) {
+            w.lmin[a] = n->left->box[a];
+            w.lmax[a] = n->left->box[3 + a];
+            w.rmin[a] = n->right->box[a];
+            w.rmax[a] = n->right->box[3 + a];
+        }
+        w.lref = lr;
+        w.rref = rr;
+        return k;
+    }
+};
+
+uint32_t internal_depth(const BNode* n) {
+    if (!n->left) return 0;
+    return 1 + std::max(internal_depth(n->left.get()), internal_depth(n->right.get()));
+}
+
+}  // namespace
+
+FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, int nthreads) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (nt > WR_MAX_RECORDS) throw std::runtime_error("too many triangles for one tree (max 2^27)");
+    for (size_t i = 0; i < 3 * (size_t)nt; i++)
+        if (idx[i] >= nv) throw std::runtime_error("triangle vertex index out of range");
+    FlatTree F;
+    F.hdr.root_ref = WR_NONE;
+    F.hdr.n_recs = nt;
+    float scale = 0.f;
+    for (size_t i = 0; i < 3 * (size_t)nv; i++) scale = std::max(scale, std::fabs(verts[i]));
+    F.hdr.scale = scale;
+    if (nt == 0) {
+        for (int a = 0; a < 3; a++) {
+            F.hdr.root_min[a] = 0.f;
+            F.hdr.root_max[a] = 0.f;
+        }
+        return F;
+    }
+    Builder B;
+    B.V = verts;
+    B.I = idx;
+    B.nv = nv;
+    B.nt = nt;
+    B.order.resize(nt);
+    B.tmp.resize(nt);
+    for (uint32_t i = 0; i < nt; i++) B.order[i] = i;
+    if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+    B.spare = nthreads - 1;
+    BNode root;
+    root.b = 0;
+    root.e = nt;
+    {
+        Scratch s;
+        B.split(&root, s, 0);
+    }
+
+    F.tris.resize(nt);
+    F.rec_prim.resize(nt);
+    F.leaf_up.assign(nt, WR_NONE);
+    F.leaf_cnt.assign(nt, 0u);
+    for (uint32_t r = 0; r < nt; r++) {
+        const uint32_t ti = B.order[r];
+        wr_tri& t = F.tris[r];
+        for (int a = 0; a < 3; a++) {
+            t.v0[a] = verts[3 * (size_t)idx[3 * (size_t)ti + 0] + a];
+            t.v1[a] = verts[3 * (size_t)idx[3 * (size_t)ti + 1] + a];
+            t.v2[a] = verts[3 * (size_t)idx[3 * (size_t)ti + 2] + a];
+        }
+        ref_normal(t.v0, t.v1, t.v2, t.n);
+        F.rec_prim[r] = ti;
+    }
+    Flattener fl(F);
+    F.nodes.reserve(nt);
+    const uint32_t rref = fl.visit(&root, WR_NONE);
+    F.hdr.root_ref = rref;
+    for (int a = 0; a < 3; a++) {
+        F.hdr.root_min[a] = root.box[a];
+        F.hdr.root_max[a] = root.box[3 + a];
+    }
+    F.hdr.n_nodes = (uint32_t)F.nodes.size();
+    F.hdr.depth = internal_depth(&root);
+    F.n_leaves = fl.leaves;
+    F.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return F;
+}
+
+TreeExport export_tree(const FlatTree& t) {
+    TreeExport X;
+    if (t.hdr.root_ref == WR_NONE) return X;
+    struct Item {
+        uint32_t ref;
+        float box[6];
+    };
+    // explicit-stack preorder: node, left subtree, right subtree
+    std::vector<std::pair<Item, uint32_t>> st;  // item, index of the parent slot to patch (WR_NONE if none)
+    Item root;
+    root.ref = t.hdr.root_ref;
+    for (int a = 0; a < 3; a++) {
+        root.box[a] = t.hdr.root_min[a];
+        root.box[3 + a] = t.hdr.root_max[a];
+    }
+    st.push_back({root, WR_NONE});
+    while (!st.empty()) {
+        auto [it, patch] = st.back();
+        st.pop_back();
+        const uint32_t me = (uint32_t)X.right.size();
+        if (patch != WR_NONE) X.right[patch] = me;
+        X.box.insert(X.box.end(), it.box, it.box + 6);
+        if (it.ref & WR_LEAF_BIT) {
+            const uint32_t first = it.ref & WR_FIRST_MASK;
+            uint32_t cnt = (it.ref >> WR_CNT_SHIFT) & WR_CNT_MASK;
+            if (cnt == 0) cnt = t.leaf_cnt[first];
+            X.right.push_back(WR_NONE);
+            X.leaf_first.push_back((uint32_t)X.tri_list.size());
+            X.leaf_count.push_back(cnt);
+            for (uint32_t k = 0; k < cnt; k++) X.tri_list.push_back(t.rec_prim[first + k]);
+        } else {
+            const wr_node& n = t.nodes[it.ref];
+            X.right.push_back(WR_NONE);
+            X.leaf_first.push_back(0);
+            X.leaf_count.push_back(0);
+            Item L, R;
+            L.ref = n.lref;
+            R.ref = n.rref;
+            for (int a = 0; a < 3; a++) {
+                L.box[a] = n.lmin[a];
+                L.box[3 + a] = n.lmax[a];
+                R.box[a] = n.rmin[a];
+                R.box[3 + a] = n.rmax[a];
+            }
+            st.push_back({R, me});        // patched into right[me] when popped
+            st.push_back({L, WR_NONE});   // left child is the preorder successor
+        }
+    }
+    return X;
+}
+
+}  // namespace wr

@@ -1,0 +1,446 @@
+// capi.cpp -- extern "C" entry points of libwalkray: contexts, device memory,
+// scenes (build + upload) and ray queries.  See include/walkray.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/walkray.h"
+#include "wr_capi_internal.h"
+#include "wr_host.h"
+#include "wr_kernels.h"
+
+static thread_local std::string g_err;
+
+namespace wr {
+
+int fail(wr_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(wr_ctx* ctx, hipError_t e, const char* what) {
+    return fail(ctx, WR_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+static int upload(wr_ctx* ctx, const std::vector<T>& v, T** d, uint64_t& bytes) {
+    *d = nullptr;
+    const size_t b = std::max<size_t>(v.size(), 1) * sizeof(T);
+    hipError_t e = hipMalloc((void**)d, b);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc(scene)");
+    bytes += b;
+    if (!v.empty()) {
+        e = hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpy(scene)");
+    }
+    return WR_OK;
+}
+
+void scene_free_device(wr_scene* s) {
+    for (void* p : s->dbufs)
+        if (p) (void)hipFree(p);
+    s->dbufs.clear();
+    s->dev = SceneDev{};
+    s->device_bytes = 0;
+}
+
+}  // namespace wr
+
+using namespace wr;
+
+extern "C" {
+
+const char* wr_last_error(const wr_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int wr_ctx_create(int device, wr_ctx** out) {
+    if (!out) return fail(nullptr, WR_EINVAL, "wr_ctx_create: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return hip_fail(nullptr, e, "hipGetDeviceCount");
+    if (device < 0 || device >= n) return fail(nullptr, WR_EINVAL, "wr_ctx_create: no such device");
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(nullptr, e, "hipSetDevice");
+    std::unique_ptr<wr_ctx> c(new wr_ctx);
+    c->device = device;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        return hip_fail(nullptr, e, "hipStreamCreate");
+    if ((e = hipEventCreate(&c->ev0)) != hipSuccess) return hip_fail(nullptr, e, "hipEventCreate");
+    if ((e = hipEventCreate(&c->ev1)) != hipSuccess) return hip_fail(nullptr, e, "hipEventCreate");
+    *out = c.release();
+    return WR_OK;
+}
+
+void wr_ctx_destroy(wr_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int wr_synchronize(wr_ctx* ctx) {
+    if (!ctx) return fail(nullptr, WR_EINVAL, "wr_synchronize: ctx is NULL");
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    return WR_OK;
+}
+
+int wr_device_alloc(wr_ctx* ctx, size_t bytes, void** d_out) {
+    if (!ctx || !d_out) return fail(ctx, WR_EINVAL, "wr_device_alloc: bad argument");
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipMalloc(d_out, std::max<size_t>(bytes, 1));
+    if (e != hipSuccess) return fail(ctx, WR_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return WR_OK;
+}
+
+int wr_device_free(wr_ctx* ctx, void* d_ptr) {
+    if (!ctx) return fail(nullptr, WR_EINVAL, "wr_device_free: ctx is NULL");
+    if (!d_ptr) return WR_OK;
+    hipError_t e = hipFree(d_ptr);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipFree");
+    return WR_OK;
+}
+
+int wr_memcpy_htod(wr_ctx* ctx, void* d_dst, const void* src, size_t bytes) {
+    if (!ctx || (!d_dst && bytes) || (!src && bytes)) return fail(ctx, WR_EINVAL, "wr_memcpy_htod: bad argument");
+    hipError_t e = hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpy(htod)");
+    return WR_OK;
+}
+
+int wr_memcpy_dtoh(wr_ctx* ctx, void* dst, const void* d_src, size_t bytes) {
+    if (!ctx || (!d_src && bytes) || (!dst && bytes)) return fail(ctx, WR_EINVAL, "wr_memcpy_dtoh: bad argument");
+    hipError_t e = hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpy(dtoh)");
+    return WR_OK;
+}
+
+int wr_timer_start(wr_ctx* ctx) {
+    if (!ctx) return fail(nullptr, WR_EINVAL, "wr_timer_start: ctx is NULL");
+    hipError_t e = hipEventRecord(ctx->ev0, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipEventRecord");
+    return WR_OK;
+}
+
+int wr_timer_stop(wr_ctx* ctx, float* ms) {
+    if (!ctx || !ms) return fail(ctx, WR_EINVAL, "wr_timer_stop: bad argument");
+    hipError_t e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, ctx->ev0, ctx->ev1);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipEventElapsedTime");
+    return WR_OK;
+}
+
+int wr_scene_create(wr_ctx* ctx, wr_scene** out) {
+    if (!ctx || !out) return fail(ctx, WR_EINVAL, "wr_scene_create: bad argument");
+    *out = new wr_scene;
+    (*out)->ctx = ctx;
+    return WR_OK;
+}
+
+void wr_scene_destroy(wr_scene* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    scene_free_device(s);
+    delete s;
+}
+
+int wr_scene_add_mesh(wr_scene* s, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                      const float* normals, const wr_material* mat, uint32_t* obj_id) {
+    if (!s) return fail(nullptr, WR_EINVAL, "wr_scene_add_mesh: scene is NULL");
+    wr_ctx* ctx = s->ctx;
+    if ((!verts && nv) || (!idx && nt) || !mat) return fail(ctx, WR_EINVAL, "wr_scene_add_mesh: NULL array");
+    if (mat->type < WR_DIFFUSE || mat->type > WR_EMISSIVE) return fail(ctx, WR_EINVAL, "wr_scene_add_mesh: unknown material");
+    for (size_t i = 0; i < 3 * (size_t)nt; i++)
+        if (idx[i] >= nv) return fail(ctx, WR_EINVAL, "wr_scene_add_mesh: vertex index out of range");
+    HostObject o;
+    o.verts.assign(verts, verts + 3 * (size_t)nv);
+    o.idx.assign(idx, idx + 3 * (size_t)nt);
+    if (normals) o.normals.assign(normals, normals + 3 * (size_t)nv);
+    o.mat = *mat;
+    s->objs.push_back(std::move(o));
+    s->built = false;
+    if (obj_id) *obj_id = (uint32_t)(s->objs.size() - 1);
+    return WR_OK;
+}
+
+int wr_scene_build(wr_scene* s, int strategy) {
+    if (!s) return fail(nullptr, WR_EINVAL, "wr_scene_build: scene is NULL");
+    wr_ctx* ctx = s->ctx;
+    if (strategy != WR_BEST)
+        return fail(ctx, WR_EINVAL, "wr_scene_build: only BEST is supported (Scene always builds BEST, scene.cpp:41)");
+    try {
+        (void)hipSetDevice(ctx->device);
+        scene_free_device(s);
+        const int nth = (int)std::max(1u, std::thread::hardware_concurrency());
+        // global tree over the concatenated objects (scene.cpp:20-44)
+        std::vector<float> V;
+        std::vector<uint32_t> I;
+        s->prim_base.assign(s->objs.size() + 1, 0);
+        uint32_t voff = 0;
+        for (size_t oi = 0; oi < s->objs.size(); oi++) {
+            const HostObject& o = s->objs[oi];
+            V.insert(V.end(), o.verts.begin(), o.verts.end());
+            for (uint32_t x : o.idx) I.push_back(x + voff);
+            voff += (uint32_t)(o.verts.size() / 3);
+            s->prim_base[oi + 1] = s->prim_base[oi] + (uint32_t)(o.idx.size() / 3);
+        }
+        const uint32_t nt = (uint32_t)(I.size() / 3);
+        s->trees.clear();
+        s->trees.push_back(build_tree(V.data(), (uint32_t)(V.size() / 3), I.data(), nt, nth));
+        // per-object trees (scene.cpp:46-65)
+        for (const HostObject& o : s->objs)
+            s->trees.push_back(
+                build_tree(o.verts.data(), (uint32_t)(o.verts.size() / 3), o.idx.data(), (uint32_t)(o.idx.size() / 3), nth));
+        if (s->trees[0].hdr.depth > 64) return fail(ctx, WR_EINVAL, "wr_scene_build: tree deeper than 64 levels");
+        uint32_t max_obj_depth = 0;
+        for (size_t i = 1; i < s->trees.size(); i++) {
+            if (s->trees[i].hdr.depth > 64) return fail(ctx, WR_EINVAL, "wr_scene_build: object tree deeper than 64 levels");
+            max_obj_depth = std::max(max_obj_depth, s->trees[i].hdr.depth);
+        }
+        // concatenate into device arrays
+        std::vector<wr_node> nodes;
+        std::vector<wr_tri> tris;
+        std::vector<uint32_t> leaf_up, leaf_cnt, node_up, rec_prim, rec_obj;
+        std::vector<float> rec_nrm;
+        std::vector<wr_tree_hdr> hdrs;
+        std::vector<wr_object_gpu> objs;
+        std::vector<uint32_t> has_n;
+        bool any_normals = false;
+        for (const HostObject& o : s->objs) any_normals |= !o.normals.empty();
+        for (size_t ti = 0; ti < s->trees.size(); ti++) {
+            FlatTree& F = s->trees[ti];
+            wr_tree_hdr h = F.hdr;
+            h.node_base = (uint32_t)nodes.size();
+            h.rec_base = (uint32_t)tris.size();
+            if ((uint64_t)h.rec_base + F.tris.size() > 0xFFFFFFFFull)
+                return fail(ctx, WR_EINVAL, "wr_scene_build: scene too large");
+            hdrs.push_back(h);
+            nodes.insert(nodes.end(), F.nodes.begin(), F.nodes.end());
+            node_up.insert(node_up.end(), F.node_up.begin(), F.node_up.end());
+            tris.insert(tris.end(), F.tris.begin(), F.tris.end());
+            leaf_up.insert(leaf_up.end(), F.leaf_up.begin(), F.leaf_up.end());
+            leaf_cnt.insert(leaf_cnt.end(), F.leaf_cnt.begin(), F.leaf_cnt.end());
+            rec_prim.insert(rec_prim.end(), F.rec_prim.begin(), F.rec_prim.end());
+            for (size_t r = 0; r < F.rec_prim.size(); r++) {
+                uint32_t oi, local;
+                if (ti == 0) {
+                    const uint32_t p = F.rec_prim[r];
+                    oi = (uint32_t)(std::upper_bound(s->prim_base.begin(), s->prim_base.end(), p) -
+                                    s->prim_base.begin() - 1);
+                    local = p - s->prim_base[oi];
+                } else {
+                    oi = (uint32_t)(ti - 1);
+                    local = F.rec_prim[r];
+                }
+                rec_obj.push_back(oi);
+                if (any_normals) {
+                    const HostObject& o = s->objs[oi];
+                    for (int k = 0; k < 3; k++)
+                        for (int a = 0; a < 3; a++)
+                            rec_nrm.push_back(o.normals.empty() ? 0.f : o.normals[3 * (size_t)o.idx[3 * (size_t)local + k] + a]);
+                }
+            }
+        }
+        for (size_t oi = 0; oi < s->objs.size(); oi++) {
+            const HostObject& o = s->objs[oi];
+            wr_object_gpu g{};
+            const wr_tree_hdr& h = s->trees[oi + 1].hdr;
+            for (int a = 0; a < 3; a++) {
+                g.aabb_min[a] = h.root_min[a];
+                g.aabb_max[a] = h.root_max[a];
+                g.color[a] = o.mat.color[a];
+            }
+            if (h.root_ref == WR_NONE) {  // no triangles: an empty box nothing is inside
+                for (int a = 0; a < 3; a++) {
+                    g.aabb_min[a] = 1.f;
+                    g.aabb_max[a] = -1.f;
+                }
+            }
+            g.power = o.mat.power;
+            g.ior = o.mat.type == WR_REFRACTIVE ? o.mat.ior : 1.f;
+            float pm = o.mat.color[0];
+            if (o.mat.color[1] > pm) pm = o.mat.color[1];
+            if (o.mat.color[2] > pm) pm = o.mat.color[2];
+            g.pmax = pm;
+            g.mat = o.mat.type;
+            g.has_normals = o.normals.empty() ? 0u : 1u;
+            g.tree = (uint32_t)(oi + 1);
+            objs.push_back(g);
+            has_n.push_back(g.has_normals);
+        }
+        uint64_t bytes = 0;
+        wr_node* d_nodes;
+        wr_tri* d_tris;
+        uint32_t *d_lu, *d_lc, *d_nu, *d_rp, *d_ro, *d_hn;
+        float* d_rn;
+        wr_tree_hdr* d_h;
+        wr_object_gpu* d_o;
+        int rc;
+        if ((rc = upload(ctx, nodes, &d_nodes, bytes))) return rc;
+        s->dbufs.push_back(d_nodes);
+        if ((rc = upload(ctx, tris, &d_tris, bytes))) return rc;
+        s->dbufs.push_back(d_tris);
+        if ((rc = upload(ctx, leaf_up, &d_lu, bytes))) return rc;
+        s->dbufs.push_back(d_lu);
+        if ((rc = upload(ctx, leaf_cnt, &d_lc, bytes))) return rc;
+        s->dbufs.push_back(d_lc);
+        if ((rc = upload(ctx, node_up, &d_nu, bytes))) return rc;
+        s->dbufs.push_back(d_nu);
+        if ((rc = upload(ctx, rec_prim, &d_rp, bytes))) return rc;
+        s->dbufs.push_back(d_rp);
+        if ((rc = upload(ctx, rec_obj, &d_ro, bytes))) return rc;
+        s->dbufs.push_back(d_ro);
+        if ((rc = upload(ctx, rec_nrm, &d_rn, bytes))) return rc;
+        s->dbufs.push_back(d_rn);
+        if ((rc = upload(ctx, hdrs, &d_h, bytes))) return rc;
+        s->dbufs.push_back(d_h);
+        if ((rc = upload(ctx, objs, &d_o, bytes))) return rc;
+        s->dbufs.push_back(d_o);
+        if ((rc = upload(ctx, has_n, &d_hn, bytes))) return rc;
+        s->dbufs.push_back(d_hn);
+        SceneDev& D = s->dev;
+        D.nodes = d_nodes;
+        D.tris = d_tris;
+        D.leaf_up = d_lu;
+        D.leaf_cnt = d_lc;
+        D.node_up = d_nu;
+        D.rec_prim = d_rp;
+        D.rec_obj = d_ro;
+        D.rec_nrm = d_rn;
+        D.hdrs = d_h;
+        D.objs = d_o;
+        D.obj_has_normals = d_hn;
+        D.n_objs = (uint32_t)s->objs.size();
+        D.max_obj_depth = max_obj_depth;
+        s->hdrs = hdrs;
+        s->host_objs = objs;
+        s->device_bytes = bytes;
+        s->max_obj_depth = max_obj_depth;
+        s->build_seconds = 0;
+        for (auto& F : s->trees) s->build_seconds += F.build_seconds;
+        s->built = true;
+        return WR_OK;
+    } catch (const std::exception& ex) {
+        scene_free_device(s);
+        return fail(ctx, WR_EINVAL, std::string("wr_scene_build: ") + ex.what());
+    }
+}
+
+int wr_scene_get_info(const wr_scene* s, wr_scene_info* info) {
+    if (!s || !info) return fail(nullptr, WR_EINVAL, "wr_scene_get_info: bad argument");
+    if (!s->built) return fail(s->ctx, WR_ESTATE, "wr_scene_get_info: scene not built");
+    info->n_objects = (uint32_t)s->objs.size();
+    info->n_triangles = s->prim_base.back();
+    info->n_nodes = s->trees[0].hdr.n_nodes;
+    info->n_leaves = s->trees[0].n_leaves;
+    info->depth = s->trees[0].hdr.depth;
+    info->max_obj_depth = s->max_obj_depth;
+    info->device_bytes = s->device_bytes;
+    info->build_seconds = s->build_seconds;
+    return WR_OK;
+}
+
+int wr_scene_prim_source(const wr_scene* s, uint32_t prim, uint32_t* obj, uint32_t* tri) {
+    if (!s || !obj || !tri) return fail(nullptr, WR_EINVAL, "wr_scene_prim_source: bad argument");
+    if (s->prim_base.empty() || prim >= s->prim_base.back())
+        return fail(s->ctx, WR_EINVAL, "wr_scene_prim_source: prim out of range");
+    const uint32_t oi =
+        (uint32_t)(std::upper_bound(s->prim_base.begin(), s->prim_base.end(), prim) - s->prim_base.begin() - 1);
+    *obj = oi;
+    *tri = prim - s->prim_base[oi];
+    return WR_OK;
+}
+
+int wr_scene_export_tree(const wr_scene* s, uint32_t obj, uint32_t* n_nodes, uint32_t* n_leaf_tris, float* box,
+                         uint32_t* right, uint32_t* leaf_first, uint32_t* leaf_count, uint32_t* tri_list) {
+    if (!s || !n_nodes || !n_leaf_tris) return fail(nullptr, WR_EINVAL, "wr_scene_export_tree: bad argument");
+    if (!s->built) return fail(s->ctx, WR_ESTATE, "wr_scene_export_tree: scene not built");
+    const size_t ti = obj == WR_NONE ? 0 : (size_t)obj + 1;
+    if (ti >= s->trees.size()) return fail(s->ctx, WR_EINVAL, "wr_scene_export_tree: no such object");
+    TreeExport X = export_tree(s->trees[ti]);
+    *n_nodes = (uint32_t)X.right.size();
+    *n_leaf_tris = (uint32_t)X.tri_list.size();
+    if (box) std::memcpy(box, X.box.data(), X.box.size() * sizeof(float));
+    if (right) std::memcpy(right, X.right.data(), X.right.size() * sizeof(uint32_t));
+    if (leaf_first) std::memcpy(leaf_first, X.leaf_first.data(), X.leaf_first.size() * sizeof(uint32_t));
+    if (leaf_count) std::memcpy(leaf_count, X.leaf_count.data(), X.leaf_count.size() * sizeof(uint32_t));
+    if (tri_list) std::memcpy(tri_list, X.tri_list.data(), X.tri_list.size() * sizeof(uint32_t));
+    return WR_OK;
+}
+
+struct wr_tree {
+    wr::FlatTree t;
+};
+
+int wr_tree_build(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, int nthreads, wr_tree** out) {
+    if (!out || (!verts && nv) || (!idx && nt)) return fail(nullptr, WR_EINVAL, "wr_tree_build: bad argument");
+    *out = nullptr;
+    try {
+        std::unique_ptr<wr_tree> t(new wr_tree);
+        t->t = build_tree(verts, nv, idx, nt, nthreads);
+        *out = t.release();
+        return WR_OK;
+    } catch (const std::exception& ex) {
+        return fail(nullptr, WR_EINVAL, std::string("wr_tree_build: ") + ex.what());
+    }
+}
+
+int wr_tree_export(const wr_tree* tree, uint32_t* n_nodes, uint32_t* n_leaf_tris, uint32_t* depth, float* box,
+                   uint32_t* right, uint32_t* leaf_first, uint32_t* leaf_count, uint32_t* tri_list) {
+    if (!tree || !n_nodes || !n_leaf_tris) return fail(nullptr, WR_EINVAL, "wr_tree_export: bad argument");
+    TreeExport X = export_tree(tree->t);
+    *n_nodes = (uint32_t)X.right.size();
+    *n_leaf_tris = (uint32_t)X.tri_list.size();
+    if (depth) *depth = tree->t.hdr.depth;
+    if (box) std::memcpy(box, X.box.data(), X.box.size() * sizeof(float));
+    if (right) std::memcpy(right, X.right.data(), X.right.size() * sizeof(uint32_t));
+    if (leaf_first) std::memcpy(leaf_first, X.leaf_first.data(), X.leaf_first.size() * sizeof(uint32_t));
+    if (leaf_count) std::memcpy(leaf_count, X.leaf_count.data(), X.leaf_count.size() * sizeof(uint32_t));
+    if (tri_list) std::memcpy(tri_list, X.tri_list.data(), X.tri_list.size() * sizeof(uint32_t));
+    return WR_OK;
+}
+
+void wr_tree_free(wr_tree* tree) { delete tree; }
+
+int wr_closest_hit(wr_ctx* ctx, const wr_scene* s, const wr_ray* d_rays, uint64_t n, wr_hit* d_hits,
+                   uint64_t* d_counters) {
+    if (!ctx || !s || (n && (!d_rays || !d_hits))) return fail(ctx, WR_EINVAL, "wr_closest_hit: bad argument");
+    if (!s->built) return fail(ctx, WR_ESTATE, "wr_closest_hit: scene not built");
+    hipError_t e = launch_closest_hit(s->dev, s->hdrs[0], (const float*)d_rays, n, d_hits,
+                                      (unsigned long long*)d_counters, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "k_closest_hit launch");
+    return WR_OK;
+}
+
+int wr_closest_hit_noself(wr_ctx* ctx, const wr_scene* s, const wr_path_ray* d_rays, uint64_t n, wr_hit* d_hits) {
+    if (!ctx || !s || (n && (!d_rays || !d_hits))) return fail(ctx, WR_EINVAL, "wr_closest_hit_noself: bad argument");
+    if (!s->built) return fail(ctx, WR_ESTATE, "wr_closest_hit_noself: scene not built");
+    hipError_t e = launch_closest_hit_path(s->dev, s->hdrs[0], (const uint32_t*)d_rays, n, d_hits, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "k_closest_hit_path launch");
+    return WR_OK;
+}
+
+int wr_inside_object(wr_ctx* ctx, const wr_scene* s, const wr_path_ray* d_rays, uint64_t n, uint32_t* d_obj) {
+    if (!ctx || !s || (n && (!d_rays || !d_obj))) return fail(ctx, WR_EINVAL, "wr_inside_object: bad argument");
+    if (!s->built) return fail(ctx, WR_ESTATE, "wr_inside_object: scene not built");
+    hipError_t e = launch_inside_object(s->dev, s->max_obj_depth, (const uint32_t*)d_rays, n, d_obj, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "k_inside_object launch");
+    return WR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,212 @@
+// trace.hip -- batch closest-hit and inside-object kernels (gfx950).
+//
+//  k_closest_hit        KDTree::intersect without Condition (kdtree.cpp:349-353),
+//                       as used by Scene::intersect for camera rays (camera.cpp:52)
+//  k_closest_hit_path   the same query with NoSelfCondition (mcRenderer.cpp:42-44,
+//                       noself.cpp:13-20)
+//  k_inside_object      Scene::findInsideObject (scene.cpp:115-144)
+//
+// One lane per ray, wave64, 256-lane workgroups; per-lane node stack in LDS;
+// workgroup ids are remapped so each XCD walks a contiguous range of rays
+// (neighbouring rays share tree nodes, and each XCD has its own L2).
+#include <hip/hip_runtime.h>
+
+#include "trace_dev.h"
+#include "wr_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace wrd {
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t G) {
+    const uint32_t xcd = b & 7u, local = b >> 3;
+    const uint32_t per = G >> 3, rem = G & 7u;
+    return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+// Gram-system barycentrics of Mesh::getWorldNormal (mesh.h:67-72) at o + d t.
+__device__ __forceinline__ void bary(const wr_tri& T, const f3& o, const f3& d, float t, float& u1, float& u2) {
+    const f3 v0 = mk(T.v0[0], T.v0[1], T.v0[2]);
+    const f3 b1 = sub(mk(T.v1[0], T.v1[1], T.v1[2]), v0);
+    const f3 b2 = sub(mk(T.v2[0], T.v2[1], T.v2[2]), v0);
+    const f3 v = sub(add(o, muls(d, t)), v0);
+    const float d12 = dot(b1, b2);
+    const float l1 = len(b1);
+    const float l2 = len(b2);
+    const float vb1 = dot(v, b1), vb2 = dot(v, b2);
+    u2 = (vb1 * d12 - vb2 * l1 * l1) / (d12 * d12 - l1 * l1 * l2 * l2);
+    u1 = (vb1 - u2 * d12) / (l1 * l1);
+}
+
+__device__ __forceinline__ void add_counters(unsigned long long* out, const Counters& c) {
+    unsigned long long a = c.nodes, b = c.tris, f = c.fallback;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+        f += __shfl_xor(f, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(out + 0, a);
+        atomicAdd(out + 1, b);
+        atomicAdd(out + 2, f);
+    }
+}
+
+template <int STACK, bool COUNT>
+__global__ __launch_bounds__(256) void k_closest_hit(SceneDev S, wr_tree_hdr H, const float* __restrict__ rays,
+                                                     uint64_t n, uint4* __restrict__ hits,
+                                                     unsigned long long* counters) {
+    __shared__ uint32_t stk[STACK * 256];
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const uint64_t i = (uint64_t)b * 256 + threadIdx.x;
+    Counters cnt{0, 0, 0};
+    if (i < n) {
+        const float* rp = rays + 6 * i;
+        const f3 o = mk(rp[0], rp[1], rp[2]);
+        const f3 d = mk(rp[3], rp[4], rp[5]);
+        const Tree tr = make_tree(H, S.nodes, S.tris, S.leaf_up, S.leaf_cnt, S.node_up);
+        const RayPre r = make_ray(o, d, tr.scale);
+        NoSelf ns{};
+        bool failed;
+        const HitState h =
+            closest_hit<false, COUNT>(tr, r, ns, stk + threadIdx.x, 256, STACK, cnt, failed);
+        uint4 out;
+        if (h.rec != WR_NONE) {
+            float u1, u2;
+            bary(tr.tris[h.rec], o, d, h.t, u1, u2);
+            out = make_uint4(S.rec_prim[H.rec_base + h.rec], __float_as_uint(h.t), __float_as_uint(u1),
+                             __float_as_uint(u2));
+        } else {
+            out = make_uint4(WR_NONE, __float_as_uint(-1.f), 0u, 0u);
+        }
+        if (failed) out.x = 0xFFFFFFFEu;  // stack overflow: reported, never silently wrong
+        hits[i] = out;
+    }
+    if (COUNT) add_counters(counters, cnt);
+}
+
+template <int STACK>
+__global__ __launch_bounds__(256) void k_closest_hit_path(SceneDev S, wr_tree_hdr H, const uint32_t* __restrict__ rays,
+                                                          uint64_t n, uint4* __restrict__ hits) {
+    __shared__ uint32_t stk[STACK * 256];
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const uint64_t i = (uint64_t)b * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* rp = rays + 8 * i;
+    const f3 o = mk(__uint_as_float(rp[0]), __uint_as_float(rp[1]), __uint_as_float(rp[2]));
+    const f3 d = mk(__uint_as_float(rp[3]), __uint_as_float(rp[4]), __uint_as_float(rp[5]));
+    const Tree tr = make_tree(H, S.nodes, S.tris, S.leaf_up, S.leaf_cnt, S.node_up);
+    const RayPre r = make_ray(o, d, tr.scale);
+    NoSelf ns{rp[6], rp[7], S.rec_obj + H.rec_base, S.rec_nrm + 9 * (size_t)H.rec_base, S.obj_has_normals};
+    Counters cnt{0, 0, 0};
+    bool failed;
+    const HitState h = closest_hit<true, false>(tr, r, ns, stk + threadIdx.x, 256, STACK, cnt, failed);
+    uint4 out;
+    if (h.rec != WR_NONE) {
+        float u1, u2;
+        bary(tr.tris[h.rec], o, d, h.t, u1, u2);
+        out = make_uint4(S.rec_prim[H.rec_base + h.rec], __float_as_uint(h.t), __float_as_uint(u1),
+                         __float_as_uint(u2));
+    } else {
+        out = make_uint4(WR_NONE, __float_as_uint(-1.f), 0u, 0u);
+    }
+    if (failed) out.x = 0xFFFFFFFEu;
+    hits[i] = out;
+}
+
+// Scene::findInsideObject for one ray; objects whose root box does not contain
+// the origin are skipped: one of the two opposite half-rays then provably fails
+// the reference box test of that object's root (its face points move away from
+// the box along the separating axis), so the reference skips them too.
+template <int STACK>
+__device__ __forceinline__ uint32_t find_inside(const SceneDev& S, const f3& o, const f3& d, uint32_t cur,
+                                                uint32_t* stk, uint32_t stride) {
+    NoSelf ns{};
+    Counters cnt{0, 0, 0};
+    const f3 db = mk(-d.x, -d.y, -d.z);
+    for (int i = (int)S.n_objs - 1; i >= 0; i--) {
+        if ((uint32_t)i == cur) continue;
+        const wr_object_gpu& ob = S.objs[i];
+        if (!(ob.aabb_min[0] <= o.x && o.x <= ob.aabb_max[0] && ob.aabb_min[1] <= o.y && o.y <= ob.aabb_max[1] &&
+              ob.aabb_min[2] <= o.z && o.z <= ob.aabb_max[2]))
+            continue;
+        const wr_tree_hdr H = S.hdrs[ob.tree];
+        const Tree tr = make_tree(H, S.nodes, S.tris, S.leaf_up, S.leaf_cnt, S.node_up);
+        bool failed;
+        const float* nrm_base = ob.has_normals ? S.rec_nrm + 9 * (size_t)H.rec_base : nullptr;
+        {
+            const RayPre r = make_ray(o, db, tr.scale);
+            const HitState h = closest_hit<false, false>(tr, r, ns, stk, stride, STACK, cnt, failed);
+            if (h.rec == WR_NONE) continue;
+            float u1, u2;
+            const f3 N = world_normal(tr.tris[h.rec], nrm_base ? nrm_base + 9 * (size_t)h.rec : nullptr,
+                                      add(o, muls(db, h.t)), u1, u2);
+            if (dot(N, db) < 0.f) continue;
+        }
+        {
+            const RayPre r = make_ray(o, d, tr.scale);
+            const HitState h = closest_hit<false, false>(tr, r, ns, stk, stride, STACK, cnt, failed);
+            if (h.rec == WR_NONE) continue;
+            float u1, u2;
+            const f3 N = world_normal(tr.tris[h.rec], nrm_base ? nrm_base + 9 * (size_t)h.rec : nullptr,
+                                      add(o, muls(d, h.t)), u1, u2);
+            if (dot(N, d) < 0.f) continue;
+        }
+        return (uint32_t)i;
+    }
+    return WR_NONE;
+}
+
+template <int STACK>
+__global__ __launch_bounds__(256) void k_inside_object(SceneDev S, const uint32_t* __restrict__ rays, uint64_t n,
+                                                       uint32_t* __restrict__ out) {
+    __shared__ uint32_t stk[STACK * 256];
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* rp = rays + 8 * i;
+    const f3 o = mk(__uint_as_float(rp[0]), __uint_as_float(rp[1]), __uint_as_float(rp[2]));
+    const f3 d = mk(__uint_as_float(rp[3]), __uint_as_float(rp[4]), __uint_as_float(rp[5]));
+    out[i] = find_inside<STACK>(S, o, d, rp[7], stk + threadIdx.x, 256);
+}
+
+}  // namespace wrd
+
+namespace wr {
+
+static int stack_for(uint32_t depth) { return depth <= 32 ? 32 : 64; }
+
+hipError_t launch_closest_hit(const SceneDev& S, const wr_tree_hdr& H, const float* rays, uint64_t n, void* hits,
+                              unsigned long long* counters, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint32_t grid = (uint32_t)((n + 255) / 256);
+    const int sk = stack_for(H.depth);
+    if (counters) {
+        if (sk == 32) wrd::k_closest_hit<32, true><<<grid, 256, 0, st>>>(S, H, rays, n, (uint4*)hits, counters);
+        else wrd::k_closest_hit<64, true><<<grid, 256, 0, st>>>(S, H, rays, n, (uint4*)hits, counters);
+    } else {
+        if (sk == 32) wrd::k_closest_hit<32, false><<<grid, 256, 0, st>>>(S, H, rays, n, (uint4*)hits, nullptr);
+        else wrd::k_closest_hit<64, false><<<grid, 256, 0, st>>>(S, H, rays, n, (uint4*)hits, nullptr);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_closest_hit_path(const SceneDev& S, const wr_tree_hdr& H, const uint32_t* rays, uint64_t n,
+                                   void* hits, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint32_t grid = (uint32_t)((n + 255) / 256);
+    if (stack_for(H.depth) == 32) wrd::k_closest_hit_path<32><<<grid, 256, 0, st>>>(S, H, rays, n, (uint4*)hits);
+    else wrd::k_closest_hit_path<64><<<grid, 256, 0, st>>>(S, H, rays, n, (uint4*)hits);
+    return hipGetLastError();
+}
+
+hipError_t launch_inside_object(const SceneDev& S, uint32_t max_obj_depth, const uint32_t* rays, uint64_t n,
+                                uint32_t* out, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint32_t grid = (uint32_t)((n + 255) / 256);
+    if (stack_for(max_obj_depth) == 32) wrd::k_inside_object<32><<<grid, 256, 0, st>>>(S, rays, n, out);
+    else wrd::k_inside_object<64><<<grid, 256, 0, st>>>(S, rays, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace wr

@@ -1,0 +1,323 @@
+// trace_dev.h -- device-side closest-hit traversal for gfx950 (wave64).
+//
+// Semantics are those of KDTree::intersect (kdtree.cpp:279-353): the result is
+// the minimum-t triangle among those whose every ancestor box passes the
+// reference box test (kdtree.cpp:46-64) and whose reference triangle test
+// (kdtree.cpp:279-306) accepts, ties going to the right subtree
+// (kdtree.cpp:331-346) and, inside a leaf, to the earlier triangle
+// (kdtree.cpp:316-324).  t, the triangle test and the box test use the
+// reference operation order with correctly rounded division / sqrt and no
+// contraction, so they are bit-identical to the reference's FP32 arithmetic.
+//
+// Execution differs (and is proven equal, see below):
+//   1. fast pass: ordered stack traversal (nearer child first, stack of node
+//      references per lane in LDS) with a conservative slab test - each box
+//      is enlarged by e = 2^-19 * max(scene scale, |origin|), a bound well
+//      above the rounding error of the reference's face-point computation,
+//      so every box the reference accepts is accepted here - and t-pruning
+//      with a relative margin of 2^-10;
+//   2. the winner's root-to-leaf chain is re-checked with the exact reference
+//      box test; if every box passes, the winner is in the reference's
+//      candidate set and is the minimum over a superset of it, hence the
+//      reference's answer (ties use the same total order: t, then larger
+//      first-record, then earlier slot);
+//   3. otherwise the ray is re-traced with the exact reference box test at
+//      every node (the fallback; rare: only rays grazing box edges).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "wr_layout.h"
+
+#pragma clang fp contract(off)
+
+namespace wrd {
+
+struct Tree {
+    const wr_node* nodes;      // already offset to this tree
+    const wr_tri* tris;        // already offset to this tree
+    const uint32_t* leaf_up;   // offset like tris
+    const uint32_t* leaf_cnt;  // offset like tris
+    const uint32_t* node_up;   // offset like nodes
+    float rmin[3], rmax[3];
+    uint32_t root_ref;
+    float scale;
+};
+
+__device__ __forceinline__ Tree make_tree(const wr_tree_hdr& h, const wr_node* nodes, const wr_tri* tris,
+                                          const uint32_t* leaf_up, const uint32_t* leaf_cnt,
+                                          const uint32_t* node_up) {
+    Tree t;
+    t.nodes = nodes + h.node_base;
+    t.tris = tris + h.rec_base;
+    t.leaf_up = leaf_up + h.rec_base;
+    t.leaf_cnt = leaf_cnt + h.rec_base;
+    t.node_up = node_up + h.node_base;
+    for (int a = 0; a < 3; a++) {
+        t.rmin[a] = h.root_min[a];
+        t.rmax[a] = h.root_max[a];
+    }
+    t.root_ref = h.root_ref;
+    t.scale = h.scale;
+    return t;
+}
+
+// ---- reference float helpers (nvVector.h semantics) ------------------------
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 sub(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 add(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 muls(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float dot(f3 a, f3 b) {
+    float r = 0.f;  // nvVector.h:501-507 starts from 0 (matters for signed zeros)
+    r += a.x * b.x;
+    r += a.y * b.y;
+    r += a.z * b.z;
+    return r;
+}
+__device__ __forceinline__ float len(f3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ f3 divs(f3 a, float s) {  // operator/=: zero divisor is a no-op
+    if (s == 0.f) return a;
+    return mk(a.x / s, a.y / s, a.z / s);
+}
+__device__ __forceinline__ f3 normalize(f3 a) {
+    float r = len(a);
+    if (r > 0.f) return divs(a, r);
+    return a;
+}
+
+// ---- ray pre-computation --------------------------------------------------
+struct RayPre {
+    f3 o, d;
+    float ix, iy, iz;     // 1/d (finite stand-in for d == 0)
+    float olx, oly, olz;  // (o + e) / d
+    float ohx, ohy, ohz;  // (o - e) / d
+    float e;              // box enlargement in world units
+};
+
+__device__ __forceinline__ float safe_inv(float d) {
+    return fabsf(d) > 1e-30f ? 1.0f / d : copysignf(1e30f, d);
+}
+
+__device__ __forceinline__ RayPre make_ray(f3 o, f3 d, float scale) {
+    RayPre r;
+    r.o = o;
+    r.d = d;
+    float m = fmaxf(fmaxf(scale, fabsf(o.x)), fmaxf(fabsf(o.y), fabsf(o.z)));
+    r.e = m * 1.9073486328125e-06f;  // 2^-19
+    r.ix = safe_inv(d.x);
+    r.iy = safe_inv(d.y);
+    r.iz = safe_inv(d.z);
+    r.olx = (o.x + r.e) * r.ix;
+    r.oly = (o.y + r.e) * r.iy;
+    r.olz = (o.z + r.e) * r.iz;
+    r.ohx = (o.x - r.e) * r.ix;
+    r.ohy = (o.y - r.e) * r.iy;
+    r.ohz = (o.z - r.e) * r.iz;
+    return r;
+}
+
+// Conservative slab test on the enlarged box, clipped to [0, tcut].
+__device__ __forceinline__ bool slab(const RayPre& r, float lx, float ly, float lz, float hx, float hy,
+                                     float hz, float tcut, float& tn_out) {
+    const float t0x = __builtin_fmaf(lx, r.ix, -r.olx), t1x = __builtin_fmaf(hx, r.ix, -r.ohx);
+    const float t0y = __builtin_fmaf(ly, r.iy, -r.oly), t1y = __builtin_fmaf(hy, r.iy, -r.ohy);
+    const float t0z = __builtin_fmaf(lz, r.iz, -r.olz), t1z = __builtin_fmaf(hz, r.iz, -r.ohz);
+    const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
+    const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tcut));
+    tn_out = tn;
+    return tn <= tf;
+}
+
+// Node::BoundingBox::intersect (kdtree.cpp:46-64) reduced to "returns >= 0".
+__device__ __noinline__ bool ref_box(const f3& o, const f3& d, const float* mn, const float* mx) {
+    const float oo[3] = {o.x, o.y, o.z};
+    const float dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int a = i % 3, a1 = (i + 1) % 3, a2 = (i + 2) % 3;
+        if (dd[a] == 0.f) continue;
+        const float plane = i < 3 ? mn[a] : mx[a];
+        const float dist = (plane - oo[a]) / dd[a];
+        const float p1 = oo[a1] + dist * dd[a1];
+        const float p2 = oo[a2] + dist * dd[a2];
+        if (dist >= 0.f && mn[a1] <= p1 && mx[a1] >= p1 && mn[a2] <= p2 && mx[a2] >= p2) return true;
+    }
+    return false;
+}
+
+// Reference triangle test (kdtree.cpp:279-306) on a 48 B record; -1 on a miss.
+__device__ __forceinline__ float ref_tri(const wr_tri& T, const f3& o, const f3& d) {
+    const f3 v0 = mk(T.v0[0], T.v0[1], T.v0[2]);
+    const f3 v1 = mk(T.v1[0], T.v1[1], T.v1[2]);
+    const f3 v2 = mk(T.v2[0], T.v2[1], T.v2[2]);
+    const f3 l0 = sub(v1, v0), l1 = sub(v2, v1), l2 = sub(v0, v2);
+    const f3 r0 = sub(v0, o), r1 = sub(v1, o), r2 = sub(v2, o);
+    const float s0 = dot(d, cross(l0, r0));
+    const float s1 = dot(d, cross(l1, r1));
+    const float s2 = dot(d, cross(l2, r2));
+    const bool pos = (s0 >= 0.f) & (s1 >= 0.f) & (s2 >= 0.f);
+    const bool neg = (s0 <= 0.f) & (s1 <= 0.f) & (s2 <= 0.f);
+    const f3 n = mk(T.n[0], T.n[1], T.n[2]);
+    const float od = dot(n, r2);
+    const float dn = dot(n, d);
+    if (dn == 0.f) return -1.f;
+    const float t = od / dn;
+    if (t < 0.f || !(pos | neg)) return -1.f;
+    return t;
+}
+
+// Mesh::getWorldNormal (mesh.h:47-77) for a record, identity transform.
+// nrm: 9 floats (vertex normals n0, n1, n2) or nullptr for flat shading.
+__device__ __forceinline__ f3 world_normal(const wr_tri& T, const float* nrm, f3 pos, float& u1, float& u2) {
+    const f3 v0 = mk(T.v0[0], T.v0[1], T.v0[2]);
+    const f3 b1 = sub(mk(T.v1[0], T.v1[1], T.v1[2]), v0);
+    const f3 b2 = sub(mk(T.v2[0], T.v2[1], T.v2[2]), v0);
+    const f3 v = sub(pos, v0);
+    const float d12 = dot(b1, b2);
+    const float l1 = len(b1);
+    const float l2 = len(b2);
+    const float vb1 = dot(v, b1), vb2 = dot(v, b2);
+    u2 = (vb1 * d12 - vb2 * l1 * l1) / (d12 * d12 - l1 * l1 * l2 * l2);
+    u1 = (vb1 - u2 * d12) / (l1 * l1);
+    if (!nrm) return normalize(cross(b1, b2));
+    const float w0 = 1.f - u1 - u2;
+    const f3 n0 = mk(nrm[0], nrm[1], nrm[2]), n1 = mk(nrm[3], nrm[4], nrm[5]), n2 = mk(nrm[6], nrm[7], nrm[8]);
+    return normalize(add(add(muls(n0, w0), muls(n1, u1)), muls(n2, u2)));
+}
+
+// NoSelfCondition::legal (noself.cpp:13-20) on object ids (WR_NONE = NULL).
+struct NoSelf {
+    uint32_t inside, contact;
+    const uint32_t* rec_obj;    // record -> object (global tree), already offset
+    const float* rec_nrm;       // record -> 9 normals, already offset
+    const uint32_t* obj_flags;  // object -> has_normals
+};
+
+__device__ __forceinline__ bool noself_legal(const NoSelf& ns, const wr_tri& T, uint32_t rec, const f3& o,
+                                             const f3& d, float t) {
+    const uint32_t obj = ns.rec_obj[rec];
+    const float* nrm = ns.obj_flags[obj] ? ns.rec_nrm + 9 * (size_t)rec : nullptr;
+    float u1, u2;
+    const f3 pos = add(o, muls(d, t));
+    const f3 N = world_normal(T, nrm, pos, u1, u2);
+    const float s = dot(d, N);
+    const bool in = s <= 0.f, out = s >= 0.f;
+    if (ns.inside == ns.contact) return !(in && ns.contact == obj);
+    return !(out && ns.inside != obj);
+}
+
+struct Counters {
+    uint32_t nodes, tris, fallback;
+};
+
+struct HitState {
+    float t;
+    uint32_t rec;    // record index within the tree, WR_NONE on a miss
+    uint32_t first;  // first record of the winner's leaf (tie order)
+};
+
+// Stack of node references in LDS, lane-interleaved: stk[level * stride].
+template <bool NOSELF, bool EXACT, bool COUNT>
+__device__ __forceinline__ void traverse(const Tree& tr, const RayPre& r, const NoSelf& ns, uint32_t* stk,
+                                         uint32_t stride, int stack_max, HitState& h, bool& overflow,
+                                         Counters& cnt) {
+    h.t = __builtin_inff();
+    h.rec = WR_NONE;
+    h.first = 0;
+    overflow = false;
+    if (tr.root_ref == WR_NONE) return;
+    {
+        float tn;
+        if (!slab(r, tr.rmin[0], tr.rmin[1], tr.rmin[2], tr.rmax[0], tr.rmax[1], tr.rmax[2], __builtin_inff(), tn))
+            return;
+        if (EXACT && !ref_box(r.o, r.d, tr.rmin, tr.rmax)) return;
+    }
+    uint32_t ref = tr.root_ref;
+    int sp = 0;
+    for (;;) {
+        if (ref & WR_LEAF_BIT) {
+            const uint32_t first = ref & WR_FIRST_MASK;
+            uint32_t c = (ref >> WR_CNT_SHIFT) & WR_CNT_MASK;
+            if (c == 0) c = tr.leaf_cnt[first];
+            for (uint32_t k = 0; k < c; k++) {
+                const wr_tri T = tr.tris[first + k];
+                if (COUNT) cnt.tris++;
+                const float t = ref_tri(T, r.o, r.d);
+                if (t >= 0.f && (t < h.t || (t == h.t && first > h.first))) {
+                    if (!NOSELF || noself_legal(ns, T, first + k, r.o, r.d, t)) {
+                        h.t = t;
+                        h.rec = first + k;
+                        h.first = first;
+                    }
+                }
+            }
+            if (sp == 0) break;
+            ref = stk[(--sp) * stride];
+            continue;
+        }
+        const wr_node N = tr.nodes[ref];
+        if (COUNT) cnt.nodes += 2;
+        const float tcut = h.t * 1.0009765625f + 4.f * r.e;
+        float tl, tr_;
+        bool hl = slab(r, N.lmin[0], N.lmin[1], N.lmin[2], N.lmax[0], N.lmax[1], N.lmax[2], tcut, tl);
+        bool hr = slab(r, N.rmin[0], N.rmin[1], N.rmin[2], N.rmax[0], N.rmax[1], N.rmax[2], tcut, tr_);
+        if (EXACT) {
+            hl = hl && ref_box(r.o, r.d, N.lmin, N.lmax);
+            hr = hr && ref_box(r.o, r.d, N.rmin, N.rmax);
+        }
+        if (hl && hr) {
+            const bool left_first = tl <= tr_;
+            const uint32_t nearr = left_first ? N.lref : N.rref;
+            const uint32_t farr = left_first ? N.rref : N.lref;
+            if (sp >= stack_max) {
+                overflow = true;
+                return;
+            }
+            stk[(sp++) * stride] = farr;
+            ref = nearr;
+        } else if (hl) {
+            ref = N.lref;
+        } else if (hr) {
+            ref = N.rref;
+        } else {
+            if (sp == 0) break;
+            ref = stk[(--sp) * stride];
+        }
+    }
+}
+
+// Re-check the winner's root-to-leaf chain with the exact reference box test.
+__device__ __forceinline__ bool verify_chain(const Tree& tr, const RayPre& r, uint32_t first) {
+    uint32_t up = tr.leaf_up[first];
+    while (up != WR_NONE) {
+        const uint32_t node = up >> 1;
+        const wr_node& N = tr.nodes[node];
+        const bool ok = (up & 1u) ? ref_box(r.o, r.d, N.rmin, N.rmax) : ref_box(r.o, r.d, N.lmin, N.lmax);
+        if (!ok) return false;
+        up = tr.node_up[node];
+    }
+    return ref_box(r.o, r.d, tr.rmin, tr.rmax);
+}
+
+// Full closest-hit query: fast pass, chain verification, exact fallback.
+template <bool NOSELF, bool COUNT>
+__device__ __forceinline__ HitState closest_hit(const Tree& tr, const RayPre& r, const NoSelf& ns, uint32_t* stk,
+                                                uint32_t stride, int stack_max, Counters& cnt, bool& failed) {
+    HitState h;
+    bool overflow;
+    failed = false;
+    traverse<NOSELF, false, COUNT>(tr, r, ns, stk, stride, stack_max, h, overflow, cnt);
+    if (!overflow && (h.rec == WR_NONE || verify_chain(tr, r, h.first))) return h;
+    if (COUNT) cnt.fallback++;
+    traverse<NOSELF, true, COUNT>(tr, r, ns, stk, stride, stack_max, h, overflow, cnt);
+    failed = overflow;
+    return h;
+}
+
+}  // namespace wrd

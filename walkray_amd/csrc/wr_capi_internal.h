@@ -1,0 +1,42 @@
+// wr_capi_internal.h -- definitions of the opaque C ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/walkray.h"
+#include "wr_host.h"
+#include "wr_kernels.h"
+
+struct wr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+};
+
+namespace wr {
+struct HostObject {
+    std::vector<float> verts, normals;
+    std::vector<uint32_t> idx;
+    wr_material mat;
+};
+int fail(wr_ctx* ctx, int code, const std::string& msg);
+int hip_fail(wr_ctx* ctx, hipError_t e, const char* what);
+}  // namespace wr
+
+struct wr_scene {
+    wr_ctx* ctx = nullptr;
+    std::vector<wr::HostObject> objs;
+    bool built = false;
+    std::vector<wr::FlatTree> trees;       // [0] global, [1 + i] object i
+    std::vector<uint32_t> prim_base;       // first global prim of each object (+ total)
+    std::vector<wr_tree_hdr> hdrs;         // host copy of the device headers
+    std::vector<wr_object_gpu> host_objs;  // host copy of the device object table
+    std::vector<void*> dbufs;
+    SceneDev dev{};
+    uint64_t device_bytes = 0;
+    uint32_t max_obj_depth = 0;
+    double build_seconds = 0;
+};

@@ -1,0 +1,38 @@
+// wr_host.h -- host-side internals of libwalkray (not part of the C ABI).
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "wr_layout.h"
+
+namespace wr {
+
+// Output of the reference builder flattened to the GPU layout (wr_layout.h).
+struct FlatTree {
+    std::vector<wr_node> nodes;
+    std::vector<wr_tri> tris;          // records, in left-to-right leaf order
+    std::vector<uint32_t> rec_prim;    // record -> input triangle index
+    std::vector<uint32_t> leaf_up;     // first record of a leaf -> (parent << 1 | side); WR_NONE for a root leaf
+    std::vector<uint32_t> leaf_cnt;    // first record of a long leaf -> triangle count
+    std::vector<uint32_t> node_up;     // internal node -> (parent << 1 | side); WR_NONE for the root
+    wr_tree_hdr hdr{};
+    uint32_t n_leaves = 0;
+    double build_seconds = 0.0;
+};
+
+// KDTree::build(BEST) (kdtree.cpp:242-258, splitBest 141-240) over the given
+// triangle soup, then flattened.  Deterministic; threads only change speed.
+FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, int nthreads);
+
+// Canonical preorder export (same format as the oracle's): used by tests to
+// compare this builder with the oracle's restatement node by node.
+struct TreeExport {
+    std::vector<float> box;        // 6 per node
+    std::vector<uint32_t> right;   // preorder index of the right child, WR_NONE for leaves
+    std::vector<uint32_t> leaf_first, leaf_count, tri_list;
+};
+TreeExport export_tree(const FlatTree& t);
+
+}  // namespace wr

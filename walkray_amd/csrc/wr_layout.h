@@ -1,0 +1,74 @@
+// wr_layout.h -- GPU-resident scene layout shared by the host builder and the
+// HIP kernels.  Plain PODs, no HIP or STL types, so both compilers see the
+// same bytes.
+//
+// The tree is the reference KDTree (kdtree.cpp:31-42,141-258) flattened for
+// traversal on CDNA4:
+//   * internal nodes become 64 B "child-pair" records holding BOTH children's
+//     boxes, so one 64 B fetch (one half of a 128 B L2 line) feeds two box
+//     tests; left child = preorder next, right child index stored;
+//   * a child reference with bit 31 set is a leaf: bits 27..30 hold the
+//     triangle count (1..15; 0 = long leaf, count in leaf_cnt[first]) and bits
+//     0..26 the first triangle record;
+//   * triangle records (48 B) are stored in left-to-right leaf order, so the
+//     first-record index of a leaf orders leaves the way kdtree.cpp:331-346
+//     breaks ties (equal t -> right subtree -> larger first record).
+#pragma once
+#include <stdint.h>
+
+#define WR_LEAF_BIT 0x80000000u
+#define WR_CNT_SHIFT 27u
+#define WR_CNT_MASK 0xFu
+#define WR_FIRST_MASK 0x07FFFFFFu
+#define WR_NONE 0xFFFFFFFFu
+#define WR_MAX_RECORDS (WR_FIRST_MASK + 1u)
+
+// 64 B internal node: boxes of both children + their references.
+struct wr_node {
+    float lmin[3];
+    float lmax[3];
+    float rmin[3];
+    float rmax[3];
+    uint32_t lref, rref;
+    uint32_t pad0, pad1;
+};
+static_assert(sizeof(wr_node) == 64, "node must be 64 bytes");
+
+// 48 B triangle record: the three vertices in reference order and the
+// precomputed reference normal normalize((v1-v0) x (v2-v1)) (kdtree.cpp:294-295).
+struct wr_tri {
+    float v0[3];
+    float v1[3];
+    float v2[3];
+    float n[3];
+};
+static_assert(sizeof(wr_tri) == 48, "triangle record must be 48 bytes");
+
+// Per-tree header.  root_ref may itself be a leaf.
+struct wr_tree_hdr {
+    float root_min[3];
+    float root_max[3];
+    uint32_t root_ref;
+    uint32_t node_base;   // offset of this tree's nodes in the node array
+    uint32_t rec_base;    // offset of this tree's records in the record arrays
+    uint32_t n_nodes;
+    uint32_t n_recs;
+    uint32_t depth;       // max number of internal nodes on a root-to-leaf path
+    float scale;          // max |coordinate| over the tree's vertices
+    uint32_t pad;
+};
+
+// Object (material) record; mat follows the reference object classes.
+enum { WR_MAT_DIFFUSE = 0, WR_MAT_GLOSSY = 1, WR_MAT_REFRACTIVE = 2, WR_MAT_EMISSIVE = 3 };
+struct wr_object_gpu {
+    float aabb_min[3];
+    float aabb_max[3];
+    float color[3];       // albedo (diffuse/glossy) or emitted radiance
+    float power;          // Phong exponent
+    float ior;
+    float pmax;           // max(albedo): the Russian-roulette survival probability
+    int32_t mat;
+    uint32_t has_normals;
+    uint32_t tree;        // index of the object's own tree header (findInsideObject)
+    uint32_t pad[3];
+};

@@ -1,0 +1,266 @@
+"""Procedural, seeded scene and ray generators.
+
+The reference loads its scenes from XML/OBJ files (config.cpp:21-429,
+mesh.cpp:5-130); none ship with it, so every workload here is generated
+deterministically in world space (identity object transforms, mesh.h:106).
+
+An object is a dict with
+    verts   (nv, 3) float32   world-space vertex positions
+    tris    (nt, 3) uint32    vertex indices, winding gives the face normal
+                              (v1 - v0) x (v2 - v0)  (mesh.h:52-55)
+    normals (nv, 3) float32 or None   per-vertex shading normals
+    mat     one of DIFFUSE, GLOSSY, REFRACTIVE, EMISSIVE
+    params  material parameters: diffuse (r, g, b); glossy (r, g, b, power);
+            refractive (ior,); emissive (r, g, b)
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+DIFFUSE, GLOSSY, REFRACTIVE, EMISSIVE = 0, 1, 2, 3
+NONE = 0xFFFFFFFF
+
+
+def _obj(verts, tris, normals, mat, params):
+    return {
+        "verts": np.ascontiguousarray(verts, dtype=np.float32),
+        "tris": np.ascontiguousarray(tris, dtype=np.uint32),
+        "normals": None if normals is None else np.ascontiguousarray(normals, dtype=np.float32),
+        "mat": int(mat),
+        "params": tuple(float(p) for p in params),
+    }
+
+
+# ---------------------------------------------------------------- hashing ---
+def _hash_u32(*keys):
+    """Integer hash (murmur3 finaliser over a combined key), vectorised, wraps mod 2^32."""
+    h = np.uint32(0x9E3779B9)
+    with np.errstate(over="ignore"):
+        for k in keys:
+            k = np.asarray(k).astype(np.uint32)
+            h = (h ^ k) * np.uint32(0x85EBCA6B)
+            h = h ^ (h >> np.uint32(13))
+            h = h * np.uint32(0xC2B2AE35)
+            h = h ^ (h >> np.uint32(16))
+    return h
+
+
+def _value_noise(x, y, seed, octave):
+    """Lattice value noise in [-1, 1) with smoothstep interpolation."""
+    xi = np.floor(x)
+    yi = np.floor(y)
+    fx = x - xi
+    fy = y - yi
+    xi = xi.astype(np.int64) & 0xFFFFFFFF
+    yi = yi.astype(np.int64) & 0xFFFFFFFF
+
+    def lat(ix, iy):
+        return _hash_u32(ix, iy, seed, octave).astype(np.float64) / 2147483648.0 - 1.0
+
+    sx = fx * fx * (3.0 - 2.0 * fx)
+    sy = fy * fy * (3.0 - 2.0 * fy)
+    v00 = lat(xi, yi)
+    v10 = lat(xi + 1, yi)
+    v01 = lat(xi, yi + 1)
+    v11 = lat(xi + 1, yi + 1)
+    a = v00 + sx * (v10 - v00)
+    b = v01 + sx * (v11 - v01)
+    return a + sy * (b - a)
+
+
+def fbm(x, y, seed, octaves=6, lacunarity=2.0, gain=0.5, base_freq=4.0):
+    total = np.zeros_like(x, dtype=np.float64)
+    amp = 1.0
+    freq = base_freq
+    for o in range(octaves):
+        total += amp * _value_noise(x * freq, y * freq, seed, o)
+        amp *= gain
+        freq *= lacunarity
+    return total
+
+
+# ------------------------------------------------------------ primitives ---
+def heightfield(n, seed=1, amplitude=0.2, octaves=6, mat=DIFFUSE, params=(0.7, 0.7, 0.7),
+                origin=(0.0, 0.0, 0.0), size=1.0):
+    """n x n vertex grid on [0,size]^2 (x, y), z = amplitude * fBm; 2 (n-1)^2 triangles, +z facing."""
+    g = np.arange(n, dtype=np.float64) / (n - 1)
+    xx, yy = np.meshgrid(g, g)  # yy varies along rows (j), xx along columns (i)
+    z = amplitude * fbm(xx, yy, seed, octaves)
+    verts = np.stack([origin[0] + size * xx, origin[1] + size * yy, origin[2] + z], axis=-1).reshape(-1, 3)
+    i = np.arange(n - 1, dtype=np.uint32)
+    ii, jj = np.meshgrid(i, i)
+    v00 = (jj * n + ii).ravel()
+    v10 = v00 + 1
+    v01 = v00 + n
+    v11 = v01 + 1
+    t1 = np.stack([v00, v10, v11], axis=-1)
+    t2 = np.stack([v00, v11, v01], axis=-1)
+    tris = np.stack([t1, t2], axis=1).reshape(-1, 3)
+    return _obj(verts, tris, None, mat, params)
+
+
+def icosphere(k, center=(0.0, 0.0, 0.0), radius=1.0, mat=DIFFUSE, params=(0.8, 0.8, 0.8),
+              smooth=True, inward=False):
+    """Icosahedron subdivided k times: 20*4^k triangles, 10*4^k+2 vertices, outward winding."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
+         (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(p, dtype=np.float64) / np.linalg.norm(p) for p in v]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(k):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    unit = np.array(verts)
+    tris = np.array(faces, dtype=np.uint32)
+    # make the winding outward: (v1-v0)x(v2-v0) . centroid > 0
+    p0, p1, p2 = unit[tris[:, 0]], unit[tris[:, 1]], unit[tris[:, 2]]
+    out = np.einsum("ij,ij->i", np.cross(p1 - p0, p2 - p0), p0 + p1 + p2) > 0
+    flip = out == inward
+    tris[flip] = tris[flip][:, [0, 2, 1]]
+    pos = np.asarray(center, dtype=np.float64) + radius * unit
+    normals = (-unit if inward else unit) if smooth else None
+    return _obj(pos, tris, normals, mat, params)
+
+
+def box(lo, hi, mat=DIFFUSE, params=(0.7, 0.7, 0.7), inward=False):
+    """Axis-aligned box, 12 triangles, flat shading, outward (or inward) winding."""
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    c = [np.array([hi[0] if i & 1 else lo[0], hi[1] if i & 2 else lo[1], hi[2] if i & 4 else lo[2]])
+         for i in range(8)]
+    quads = [(0, 2, 6, 4), (1, 5, 7, 3), (0, 4, 5, 1), (2, 3, 7, 6), (0, 1, 3, 2), (4, 6, 7, 5)]
+    tris = []
+    center = (lo + hi) / 2
+    for a, b, cc, d in quads:
+        for tri in ((a, b, cc), (a, cc, d)):
+            p0, p1, p2 = c[tri[0]], c[tri[1]], c[tri[2]]
+            n = np.cross(p1 - p0, p2 - p0)
+            outward = np.dot(n, (p0 + p1 + p2) / 3 - center) > 0
+            tris.append(tri if outward != inward else (tri[0], tri[2], tri[1]))
+    return _obj(np.array(c), np.array(tris, dtype=np.uint32), None, mat, params)
+
+
+def quad(center, e1, e2, normal, mat, params):
+    """Rectangle center +- e1/2 +- e2/2, two triangles whose face normal points along `normal`."""
+    c = np.asarray(center, dtype=np.float64)
+    e1 = np.asarray(e1, dtype=np.float64) / 2
+    e2 = np.asarray(e2, dtype=np.float64) / 2
+    p = [c - e1 - e2, c + e1 - e2, c + e1 + e2, c - e1 + e2]
+    tris = [(0, 1, 2), (0, 2, 3)]
+    if np.dot(np.cross(p[1] - p[0], p[2] - p[0]), normal) < 0:
+        tris = [(0, 2, 1), (0, 3, 2)]
+    return _obj(np.array(p), np.array(tris, dtype=np.uint32), None, mat, params)
+
+
+def merge(objs, mat, params):
+    """Concatenate flat-shaded meshes into one object."""
+    vs, ts, off = [], [], 0
+    for o in objs:
+        vs.append(o["verts"])
+        ts.append(o["tris"].astype(np.int64) + off)
+        off += len(o["verts"])
+    return _obj(np.concatenate(vs), np.concatenate(ts), None, mat, params)
+
+
+# ---------------------------------------------------------------- cameras ---
+def camera(pos, focus, up, width, height, sight_dist):
+    return {"pos": tuple(map(float, pos)), "focus": tuple(map(float, focus)),
+            "up": tuple(map(float, up)), "width": int(width), "height": int(height),
+            "sight_dist": float(sight_dist)}
+
+
+# ------------------------------------------------------------------ scenes ---
+def cornell(k=5, width=256, height=256):
+    """Cornell box [0,1]^3, 12 wall triangles (inward), two k-icospheres, a 0.25^2 light of 17."""
+    white, red, green = (0.75, 0.75, 0.75), (0.63, 0.06, 0.04), (0.15, 0.45, 0.09)
+    floor = quad((0.5, 0.0, 0.5), (1, 0, 0), (0, 0, 1), (0, 1, 0), DIFFUSE, white)
+    ceil = quad((0.5, 1.0, 0.5), (1, 0, 0), (0, 0, 1), (0, -1, 0), DIFFUSE, white)
+    back = quad((0.5, 0.5, 0.0), (1, 0, 0), (0, 1, 0), (0, 0, 1), DIFFUSE, white)
+    front = quad((0.5, 0.5, 1.0), (1, 0, 0), (0, 1, 0), (0, 0, -1), DIFFUSE, white)
+    objs = [
+        merge([floor, ceil, back, front], DIFFUSE, white),
+        quad((0.0, 0.5, 0.5), (0, 1, 0), (0, 0, 1), (1, 0, 0), DIFFUSE, red),
+        quad((1.0, 0.5, 0.5), (0, 1, 0), (0, 0, 1), (-1, 0, 0), DIFFUSE, green),
+        quad((0.5, 1.0 - 1e-3, 0.5), (0.25, 0, 0), (0, 0, 0.25), (0, -1, 0), EMISSIVE, (17.0, 17.0, 17.0)),
+        icosphere(k, (0.28, 0.201, 0.35), 0.2, DIFFUSE, (0.8, 0.8, 0.8)),
+        icosphere(k, (0.72, 0.201, 0.62), 0.2, REFRACTIVE, (1.5,)),
+    ]
+    cam = camera((0.5, 0.5, 0.97), (0.5, 0.5, 0.0), (0.0, 1.0, 0.0), width, height,
+                 sight_dist=250.0 * width / 256.0)
+    return objs, cam
+
+
+def heightfield_scene(n=1024, seed=7):
+    """Ray-query workload: one fBm heightfield of 2 (n-1)^2 triangles."""
+    return [heightfield(n, seed=seed)]
+
+
+def scene_bounds(objs):
+    v = np.concatenate([o["verts"] for o in objs])
+    return v.min(axis=0), v.max(axis=0)
+
+
+# -------------------------------------------------------------------- rays ---
+def pinhole_rays(cam, width=None, height=None):
+    """Camera rays through pixel centres (camera.cpp:6-23 with u = v = 0.5), float32 (N, 6)."""
+    W = width or cam["width"]
+    H = height or cam["height"]
+    pos = np.array(cam["pos"], dtype=np.float64)
+    front = np.array(cam["focus"], dtype=np.float64) - pos
+    front /= np.linalg.norm(front)
+    right = np.cross(front, np.array(cam["up"], dtype=np.float64))
+    right /= np.linalg.norm(right)
+    top = np.cross(right, front)
+    x = np.arange(W, dtype=np.float64) + 0.5 - W / 2.0
+    y = H / 2.0 - np.arange(H, dtype=np.float64) - 0.5
+    sd = cam["sight_dist"] * W / cam["width"]
+    d = (front[None, None, :] * sd + right[None, None, :] * x[None, :, None]
+         + top[None, None, :] * y[:, None, None])
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    rays = np.empty((H * W, 6), dtype=np.float32)
+    rays[:, :3] = pos.astype(np.float32)
+    rays[:, 3:] = d.reshape(-1, 3).astype(np.float32)
+    return rays
+
+
+def heightfield_camera(width=4096, height=4096):
+    return camera((0.5, -0.35, 0.9), (0.5, 0.45, 0.0), (0.0, 0.0, 1.0), width, height,
+                  sight_dist=1.15 * width)
+
+
+def incoherent_rays(n, lo, hi, seed=11):
+    """Origins uniform in the box [lo, hi], directions uniform on the sphere (z = 1-2u, phi = 2 pi v)."""
+    idx = np.arange(n, dtype=np.uint64)
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+
+    def u(k):
+        return (_hash_u32(idx & np.uint64(0xFFFFFFFF), idx >> np.uint64(32), seed, k) >> np.uint32(8)).astype(
+            np.float64) / 16777216.0
+
+    o = lo + (hi - lo) * np.stack([u(0), u(1), u(2)], axis=-1)
+    z = 1.0 - 2.0 * u(3)
+    phi = 2.0 * math.pi * u(4)
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    d = np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=-1)
+    rays = np.empty((n, 6), dtype=np.float32)
+    rays[:, :3] = o
+    rays[:, 3:] = d
+    return rays
