@@ -53,14 +53,15 @@ __device__ __forceinline__ void add_counters(unsigned long long* out, const Coun
     }
 }
 
-template <int STACK, bool COUNT>
+template <bool COUNT>
 __global__ __launch_bounds__(256) void k_closest_hit(SceneDev S, wr_tree_hdr H, const float* __restrict__ rays,
                                                      uint64_t n, uint4* __restrict__ hits,
                                                      unsigned long long* counters) {
-    __shared__ uint32_t stk[STACK * 256];
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     const uint64_t i = (uint64_t)b * 256 + threadIdx.x;
     Counters cnt{0, 0, 0};
+    LaneStack st;
+    st.lane = threadIdx.x;
     if (i < n) {
         const float* rp = rays + 6 * i;
         const f3 o = mk(rp[0], rp[1], rp[2]);
@@ -69,8 +70,7 @@ __global__ __launch_bounds__(256) void k_closest_hit(SceneDev S, wr_tree_hdr H, 
         const RayPre r = make_ray(o, d, tr.scale);
         NoSelf ns{};
         bool failed;
-        const HitState h =
-            closest_hit<false, COUNT>(tr, r, ns, stk + threadIdx.x, 256, STACK, cnt, failed);
+        const HitState h = closest_hit<false, COUNT>(tr, r, ns, st, cnt, failed);
         uint4 out;
         if (h.rec != WR_NONE) {
             float u1, u2;
@@ -86,13 +86,13 @@ __global__ __launch_bounds__(256) void k_closest_hit(SceneDev S, wr_tree_hdr H, 
     if (COUNT) add_counters(counters, cnt);
 }
 
-template <int STACK>
 __global__ __launch_bounds__(256) void k_closest_hit_path(SceneDev S, wr_tree_hdr H, const uint32_t* __restrict__ rays,
                                                           uint64_t n, uint4* __restrict__ hits) {
-    __shared__ uint32_t stk[STACK * 256];
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     const uint64_t i = (uint64_t)b * 256 + threadIdx.x;
     if (i >= n) return;
+    LaneStack st;
+    st.lane = threadIdx.x;
     const uint32_t* rp = rays + 8 * i;
     const f3 o = mk(__uint_as_float(rp[0]), __uint_as_float(rp[1]), __uint_as_float(rp[2]));
     const f3 d = mk(__uint_as_float(rp[3]), __uint_as_float(rp[4]), __uint_as_float(rp[5]));
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void k_closest_hit_path(SceneDev S, wr_tree_hd
     NoSelf ns{rp[6], rp[7], S.rec_obj + H.rec_base, S.rec_nrm + 9 * (size_t)H.rec_base, S.obj_has_normals};
     Counters cnt{0, 0, 0};
     bool failed;
-    const HitState h = closest_hit<true, false>(tr, r, ns, stk + threadIdx.x, 256, STACK, cnt, failed);
+    const HitState h = closest_hit<true, false>(tr, r, ns, st, cnt, failed);
     uint4 out;
     if (h.rec != WR_NONE) {
         float u1, u2;
@@ -119,9 +119,8 @@ __global__ __launch_bounds__(256) void k_closest_hit_path(SceneDev S, wr_tree_hd
 // the origin are skipped: one of the two opposite half-rays then provably fails
 // the reference box test of that object's root (its face points move away from
 // the box along the separating axis), so the reference skips them too.
-template <int STACK>
 __device__ __forceinline__ uint32_t find_inside(const SceneDev& S, const f3& o, const f3& d, uint32_t cur,
-                                                uint32_t* stk, uint32_t stride) {
+                                                LaneStack& st) {
     NoSelf ns{};
     Counters cnt{0, 0, 0};
     const f3 db = mk(-d.x, -d.y, -d.z);
@@ -137,7 +136,7 @@ __device__ __forceinline__ uint32_t find_inside(const SceneDev& S, const f3& o, 
         const float* nrm_base = ob.has_normals ? S.rec_nrm + 9 * (size_t)H.rec_base : nullptr;
         {
             const RayPre r = make_ray(o, db, tr.scale);
-            const HitState h = closest_hit<false, false>(tr, r, ns, stk, stride, STACK, cnt, failed);
+            const HitState h = closest_hit<false, false>(tr, r, ns, st, cnt, failed);
             if (h.rec == WR_NONE) continue;
             float u1, u2;
             const f3 N = world_normal(tr.tris[h.rec], nrm_base ? nrm_base + 9 * (size_t)h.rec : nullptr,
@@ -146,7 +145,7 @@ __device__ __forceinline__ uint32_t find_inside(const SceneDev& S, const f3& o, 
         }
         {
             const RayPre r = make_ray(o, d, tr.scale);
-            const HitState h = closest_hit<false, false>(tr, r, ns, stk, stride, STACK, cnt, failed);
+            const HitState h = closest_hit<false, false>(tr, r, ns, st, cnt, failed);
             if (h.rec == WR_NONE) continue;
             float u1, u2;
             const f3 N = world_normal(tr.tris[h.rec], nrm_base ? nrm_base + 9 * (size_t)h.rec : nullptr,
@@ -158,36 +157,28 @@ __device__ __forceinline__ uint32_t find_inside(const SceneDev& S, const f3& o, 
     return WR_NONE;
 }
 
-template <int STACK>
 __global__ __launch_bounds__(256) void k_inside_object(SceneDev S, const uint32_t* __restrict__ rays, uint64_t n,
                                                        uint32_t* __restrict__ out) {
-    __shared__ uint32_t stk[STACK * 256];
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    LaneStack st;
+    st.lane = threadIdx.x;
     const uint32_t* rp = rays + 8 * i;
     const f3 o = mk(__uint_as_float(rp[0]), __uint_as_float(rp[1]), __uint_as_float(rp[2]));
     const f3 d = mk(__uint_as_float(rp[3]), __uint_as_float(rp[4]), __uint_as_float(rp[5]));
-    out[i] = find_inside<STACK>(S, o, d, rp[7], stk + threadIdx.x, 256);
+    out[i] = find_inside(S, o, d, rp[7], st);
 }
 
 }  // namespace wrd
 
 namespace wr {
 
-static int stack_for(uint32_t depth) { return depth <= 32 ? 32 : 64; }
-
 hipError_t launch_closest_hit(const SceneDev& S, const wr_tree_hdr& H, const float* rays, uint64_t n, void* hits,
                               unsigned long long* counters, hipStream_t st) {
     if (n == 0) return hipSuccess;
     const uint32_t grid = (uint32_t)((n + 255) / 256);
-    const int sk = stack_for(H.depth);
-    if (counters) {
-        if (sk == 32) wrd::k_closest_hit<32, true><<<grid, 256, 0, st>>>(S, H, rays, n, (uint4*)hits, counters);
-        else wrd::k_closest_hit<64, true><<<grid, 256, 0, st>>>(S, H, rays, n, (uint4*)hits, counters);
-    } else {
-        if (sk == 32) wrd::k_closest_hit<32, false><<<grid, 256, 0, st>>>(S, H, rays, n, (uint4*)hits, nullptr);
-        else wrd::k_closest_hit<64, false><<<grid, 256, 0, st>>>(S, H, rays, n, (uint4*)hits, nullptr);
-    }
+    if (counters) wrd::k_closest_hit<true><<<grid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, counters);
+    else wrd::k_closest_hit<false><<<grid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, nullptr);
     return hipGetLastError();
 }
 
@@ -195,17 +186,16 @@ hipError_t launch_closest_hit_path(const SceneDev& S, const wr_tree_hdr& H, cons
                                    void* hits, hipStream_t st) {
     if (n == 0) return hipSuccess;
     const uint32_t grid = (uint32_t)((n + 255) / 256);
-    if (stack_for(H.depth) == 32) wrd::k_closest_hit_path<32><<<grid, 256, 0, st>>>(S, H, rays, n, (uint4*)hits);
-    else wrd::k_closest_hit_path<64><<<grid, 256, 0, st>>>(S, H, rays, n, (uint4*)hits);
+    wrd::k_closest_hit_path<<<grid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits);
     return hipGetLastError();
 }
 
 hipError_t launch_inside_object(const SceneDev& S, uint32_t max_obj_depth, const uint32_t* rays, uint64_t n,
                                 uint32_t* out, hipStream_t st) {
+    (void)max_obj_depth;
     if (n == 0) return hipSuccess;
     const uint32_t grid = (uint32_t)((n + 255) / 256);
-    if (stack_for(max_obj_depth) == 32) wrd::k_inside_object<32><<<grid, 256, 0, st>>>(S, rays, n, out);
-    else wrd::k_inside_object<64><<<grid, 256, 0, st>>>(S, rays, n, out);
+    wrd::k_inside_object<<<grid, 256, wrd::kStackLdsBytes, st>>>(S, rays, n, out);
     return hipGetLastError();
 }
 

@@ -98,6 +98,7 @@ struct RayPre {
     float olx, oly, olz;  // (o + e) / d
     float ohx, ohy, ohz;  // (o - e) / d
     float e;              // box enlargement in world units
+    float tscale;         // 2^-20 * max(1, |d|_inf): t-proportional part of the face-point error bound
 };
 
 __device__ __forceinline__ float safe_inv(float d) {
@@ -119,6 +120,7 @@ __device__ __forceinline__ RayPre make_ray(f3 o, f3 d, float scale) {
     r.ohx = (o.x - r.e) * r.ix;
     r.ohy = (o.y - r.e) * r.iy;
     r.ohz = (o.z - r.e) * r.iz;
+    r.tscale = fmaxf(1.f, fmaxf(fmaxf(fabsf(d.x), fabsf(d.y)), fabsf(d.z))) * 9.5367431640625e-07f;  // 2^-20
     return r;
 }
 
@@ -135,9 +137,13 @@ __device__ __forceinline__ bool slab(const RayPre& r, float lx, float ly, float 
 }
 
 // Node::BoundingBox::intersect (kdtree.cpp:46-64) reduced to "returns >= 0".
-__device__ __noinline__ bool ref_box(const f3& o, const f3& d, const float* mn, const float* mx) {
-    const float oo[3] = {o.x, o.y, o.z};
-    const float dd[3] = {d.x, d.y, d.z};
+// Arguments by value so the (rare) call keeps everything in VGPRs.
+__device__ __noinline__ bool ref_box_exact(float ox, float oy, float oz, float dx, float dy, float dz, float lx,
+                                           float ly, float lz, float hx, float hy, float hz) {
+    const float oo[3] = {ox, oy, oz};
+    const float dd[3] = {dx, dy, dz};
+    const float mn[3] = {lx, ly, lz};
+    const float mx[3] = {hx, hy, hz};
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         const int a = i % 3, a1 = (i + 1) % 3, a2 = (i + 2) % 3;
@@ -151,24 +157,62 @@ __device__ __noinline__ bool ref_box(const f3& o, const f3& d, const float* mn, 
     return false;
 }
 
+// One far-plane face of the reference box test decided without division:
+// the sign of the reference's dist = (plane - o)/d is exact from the signs of
+// (plane - o) and d; the face point on the two other axes is approximated with
+// a reciprocal and an fma, whose distance to the reference's rounded point is
+// below delta = e + |t| * tscale (e >= 2^-19 max(scale, |o|)).  A point at
+// least delta inside the face rectangle is therefore inside for the reference
+// too, and the reference box test returns >= 0.
+__device__ __forceinline__ bool far_face_accepts(float far_a, float o_a, float d_a, float inv_a, float o_b, float d_b,
+                                                 float lo_b, float hi_b, float o_c, float d_c, float lo_c,
+                                                 float hi_c, float e, float tscale) {
+    const float num = far_a - o_a;
+    const bool sign_ok = (d_a != 0.f) & ((num == 0.f) | ((num > 0.f) == (d_a > 0.f)));
+    const float t = num * inv_a;
+    const float delta = __builtin_fmaf(fabsf(t), tscale, e);
+    const float pb = __builtin_fmaf(t, d_b, o_b);
+    const float pc = __builtin_fmaf(t, d_c, o_c);
+    return sign_ok & (pb >= lo_b + delta) & (pb <= hi_b - delta) & (pc >= lo_c + delta) & (pc <= hi_c - delta);
+}
+
+// The reference box test (exact result): a division-free proof on the three
+// far-plane faces first, the exact six-face test only when none is conclusive.
+__device__ __forceinline__ bool ref_box(const RayPre& r, const float* mn, const float* mx) {
+    const float fx = r.d.x > 0.f ? mx[0] : mn[0];
+    const float fy = r.d.y > 0.f ? mx[1] : mn[1];
+    const float fz = r.d.z > 0.f ? mx[2] : mn[2];
+    if (far_face_accepts(fx, r.o.x, r.d.x, r.ix, r.o.y, r.d.y, mn[1], mx[1], r.o.z, r.d.z, mn[2], mx[2], r.e, r.tscale) ||
+        far_face_accepts(fy, r.o.y, r.d.y, r.iy, r.o.z, r.d.z, mn[2], mx[2], r.o.x, r.d.x, mn[0], mx[0], r.e, r.tscale) ||
+        far_face_accepts(fz, r.o.z, r.d.z, r.iz, r.o.x, r.d.x, mn[0], mx[0], r.o.y, r.d.y, mn[1], mx[1], r.e, r.tscale))
+        return true;
+    return ref_box_exact(r.o.x, r.o.y, r.o.z, r.d.x, r.d.y, r.d.z, mn[0], mn[1], mn[2], mx[0], mx[1], mx[2]);
+}
+
 // Reference triangle test (kdtree.cpp:279-306) on a 48 B record; -1 on a miss.
-__device__ __forceinline__ float ref_tri(const wr_tri& T, const f3& o, const f3& d) {
+// The test is a pure function, so its three conditions may be evaluated in
+// any order: t (exactly the reference's orthoDist / d) comes first, and the
+// three edge-sign tests are skipped when t cannot beat the current best
+// (t > best_t, or equal with a tie the current best wins) - an exact cut.
+__device__ __forceinline__ float ref_tri(const wr_tri& T, const f3& o, const f3& d, float best_t, bool tie_loses) {
     const f3 v0 = mk(T.v0[0], T.v0[1], T.v0[2]);
     const f3 v1 = mk(T.v1[0], T.v1[1], T.v1[2]);
     const f3 v2 = mk(T.v2[0], T.v2[1], T.v2[2]);
-    const f3 l0 = sub(v1, v0), l1 = sub(v2, v1), l2 = sub(v0, v2);
-    const f3 r0 = sub(v0, o), r1 = sub(v1, o), r2 = sub(v2, o);
-    const float s0 = dot(d, cross(l0, r0));
-    const float s1 = dot(d, cross(l1, r1));
-    const float s2 = dot(d, cross(l2, r2));
-    const bool pos = (s0 >= 0.f) & (s1 >= 0.f) & (s2 >= 0.f);
-    const bool neg = (s0 <= 0.f) & (s1 <= 0.f) & (s2 <= 0.f);
+    const f3 r2 = sub(v2, o);
     const f3 n = mk(T.n[0], T.n[1], T.n[2]);
     const float od = dot(n, r2);
     const float dn = dot(n, d);
     if (dn == 0.f) return -1.f;
     const float t = od / dn;
-    if (t < 0.f || !(pos | neg)) return -1.f;
+    if (!(t >= 0.f) || t > best_t || (t == best_t && tie_loses)) return -1.f;
+    const f3 l0 = sub(v1, v0), l1 = sub(v2, v1), l2 = sub(v0, v2);
+    const f3 r0 = sub(v0, o), r1 = sub(v1, o);
+    const float s0 = dot(d, cross(l0, r0));
+    const float s1 = dot(d, cross(l1, r1));
+    const float s2 = dot(d, cross(l2, r2));
+    const bool pos = (s0 >= 0.f) & (s1 >= 0.f) & (s2 >= 0.f);
+    const bool neg = (s0 <= 0.f) & (s1 <= 0.f) & (s2 <= 0.f);
+    if (!(pos | neg)) return -1.f;
     return t;
 }
 
@@ -222,34 +266,66 @@ struct HitState {
     uint32_t first;  // first record of the winner's leaf (tie order)
 };
 
-// Stack of node references in LDS, lane-interleaved: stk[level * stride].
+// Per-lane node stack: the first LDS_DEPTH levels live in LDS (lane-interleaved,
+// stk[level * stride]); deeper levels spill to a private array (scratch), which
+// only very deep traversals touch.  Trees are limited to 64 internal levels, and
+// an ordered traversal never holds more entries than that.
+constexpr int kLdsDepth = 16;
+constexpr int kMaxDepth = 64;
+constexpr int kBlock = 256;  // every traversal kernel runs 256-lane workgroups
+
+// Dynamic LDS of the launching kernel: kLdsDepth * kBlock words.
+extern __shared__ uint32_t wr_lds_stack[];
+
+struct LaneStack {
+    uint32_t lane;  // threadIdx.x
+    int sp;
+    uint32_t spill[kMaxDepth - kLdsDepth];
+    // The LDS access is unconditional (a wrapped slot when spilling) so it stays
+    // a ds_read/ds_write instead of being merged with the scratch access into a
+    // generic (flat) one.
+    __device__ __forceinline__ void push(uint32_t v) {
+        if (sp >= kLdsDepth) spill[sp - kLdsDepth] = wr_lds_stack[(sp & (kLdsDepth - 1)) * kBlock + lane];
+        wr_lds_stack[(sp & (kLdsDepth - 1)) * kBlock + lane] = v;
+        sp++;
+    }
+    __device__ __forceinline__ uint32_t pop() {
+        --sp;
+        const uint32_t v = wr_lds_stack[(sp & (kLdsDepth - 1)) * kBlock + lane];
+        if (sp >= kLdsDepth) wr_lds_stack[(sp & (kLdsDepth - 1)) * kBlock + lane] = spill[sp - kLdsDepth];
+        return v;
+    }
+};
+constexpr size_t kStackLdsBytes = sizeof(uint32_t) * kLdsDepth * kBlock;
+
 template <bool NOSELF, bool EXACT, bool COUNT>
-__device__ __forceinline__ void traverse(const Tree& tr, const RayPre& r, const NoSelf& ns, uint32_t* stk,
-                                         uint32_t stride, int stack_max, HitState& h, bool& overflow,
-                                         Counters& cnt) {
+__device__ __forceinline__ void traverse(const Tree& tr, const RayPre& r, const NoSelf& ns, LaneStack& st,
+                                         HitState& h, bool& overflow, Counters& cnt) {
     h.t = __builtin_inff();
     h.rec = WR_NONE;
     h.first = 0;
     overflow = false;
+    st.sp = 0;
     if (tr.root_ref == WR_NONE) return;
     {
         float tn;
         if (!slab(r, tr.rmin[0], tr.rmin[1], tr.rmin[2], tr.rmax[0], tr.rmax[1], tr.rmax[2], __builtin_inff(), tn))
             return;
-        if (EXACT && !ref_box(r.o, r.d, tr.rmin, tr.rmax)) return;
+        if (EXACT && !ref_box(r, tr.rmin, tr.rmax)) return;
     }
     uint32_t ref = tr.root_ref;
-    int sp = 0;
     for (;;) {
         if (ref & WR_LEAF_BIT) {
             const uint32_t first = ref & WR_FIRST_MASK;
             uint32_t c = (ref >> WR_CNT_SHIFT) & WR_CNT_MASK;
             if (c == 0) c = tr.leaf_cnt[first];
+            const bool tie_loses = h.rec != WR_NONE && first <= h.first;
             for (uint32_t k = 0; k < c; k++) {
                 const wr_tri T = tr.tris[first + k];
                 if (COUNT) cnt.tris++;
-                const float t = ref_tri(T, r.o, r.d);
-                if (t >= 0.f && (t < h.t || (t == h.t && first > h.first))) {
+                // tie_loses stays valid inside the leaf: an accepted slot makes h.first == first
+                const float t = ref_tri(T, r.o, r.d, h.t, tie_loses || (h.rec != WR_NONE && h.first == first));
+                if (t >= 0.f) {
                     if (!NOSELF || noself_legal(ns, T, first + k, r.o, r.d, t)) {
                         h.t = t;
                         h.rec = first + k;
@@ -257,8 +333,8 @@ __device__ __forceinline__ void traverse(const Tree& tr, const RayPre& r, const 
                     }
                 }
             }
-            if (sp == 0) break;
-            ref = stk[(--sp) * stride];
+            if (st.sp == 0) break;
+            ref = st.pop();
             continue;
         }
         const wr_node N = tr.nodes[ref];
@@ -268,54 +344,54 @@ __device__ __forceinline__ void traverse(const Tree& tr, const RayPre& r, const 
         bool hl = slab(r, N.lmin[0], N.lmin[1], N.lmin[2], N.lmax[0], N.lmax[1], N.lmax[2], tcut, tl);
         bool hr = slab(r, N.rmin[0], N.rmin[1], N.rmin[2], N.rmax[0], N.rmax[1], N.rmax[2], tcut, tr_);
         if (EXACT) {
-            hl = hl && ref_box(r.o, r.d, N.lmin, N.lmax);
-            hr = hr && ref_box(r.o, r.d, N.rmin, N.rmax);
+            hl = hl && ref_box(r, N.lmin, N.lmax);
+            hr = hr && ref_box(r, N.rmin, N.rmax);
         }
         if (hl && hr) {
             const bool left_first = tl <= tr_;
             const uint32_t nearr = left_first ? N.lref : N.rref;
             const uint32_t farr = left_first ? N.rref : N.lref;
-            if (sp >= stack_max) {
+            if (st.sp >= kMaxDepth) {
                 overflow = true;
                 return;
             }
-            stk[(sp++) * stride] = farr;
+            st.push(farr);
             ref = nearr;
         } else if (hl) {
             ref = N.lref;
         } else if (hr) {
             ref = N.rref;
         } else {
-            if (sp == 0) break;
-            ref = stk[(--sp) * stride];
+            if (st.sp == 0) break;
+            ref = st.pop();
         }
     }
 }
 
-// Re-check the winner's root-to-leaf chain with the exact reference box test.
+// Re-check the winner's root-to-leaf chain with the reference box test.
 __device__ __forceinline__ bool verify_chain(const Tree& tr, const RayPre& r, uint32_t first) {
     uint32_t up = tr.leaf_up[first];
     while (up != WR_NONE) {
         const uint32_t node = up >> 1;
-        const wr_node& N = tr.nodes[node];
-        const bool ok = (up & 1u) ? ref_box(r.o, r.d, N.rmin, N.rmax) : ref_box(r.o, r.d, N.lmin, N.lmax);
+        const wr_node N = tr.nodes[node];
+        const bool ok = (up & 1u) ? ref_box(r, N.rmin, N.rmax) : ref_box(r, N.lmin, N.lmax);
         if (!ok) return false;
         up = tr.node_up[node];
     }
-    return ref_box(r.o, r.d, tr.rmin, tr.rmax);
+    return ref_box(r, tr.rmin, tr.rmax);
 }
 
 // Full closest-hit query: fast pass, chain verification, exact fallback.
 template <bool NOSELF, bool COUNT>
-__device__ __forceinline__ HitState closest_hit(const Tree& tr, const RayPre& r, const NoSelf& ns, uint32_t* stk,
-                                                uint32_t stride, int stack_max, Counters& cnt, bool& failed) {
+__device__ __forceinline__ HitState closest_hit(const Tree& tr, const RayPre& r, const NoSelf& ns, LaneStack& st,
+                                                Counters& cnt, bool& failed) {
     HitState h;
     bool overflow;
     failed = false;
-    traverse<NOSELF, false, COUNT>(tr, r, ns, stk, stride, stack_max, h, overflow, cnt);
+    traverse<NOSELF, false, COUNT>(tr, r, ns, st, h, overflow, cnt);
     if (!overflow && (h.rec == WR_NONE || verify_chain(tr, r, h.first))) return h;
     if (COUNT) cnt.fallback++;
-    traverse<NOSELF, true, COUNT>(tr, r, ns, stk, stride, stack_max, h, overflow, cnt);
+    traverse<NOSELF, true, COUNT>(tr, r, ns, st, h, overflow, cnt);
     failed = overflow;
     return h;
 }
