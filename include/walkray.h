@@ -145,6 +145,53 @@ int wr_closest_hit_noself(wr_ctx* ctx, const wr_scene* scene, const wr_path_ray*
  * d_rays[i].contact_obj (WR_NONE = NULL). */
 int wr_inside_object(wr_ctx* ctx, const wr_scene* scene, const wr_path_ray* d_rays, uint64_t n, uint32_t* d_obj);
 
+/* ---- path tracing ------------------------------------------------------ */
+/* Camera::setup arguments (camera.h:28-48, config.cpp:504-511): pinhole, no
+ * depth of field. */
+typedef struct {
+    float pos[3];
+    float focus[3];
+    float up[3];
+    uint32_t width;
+    uint32_t height;
+    float sight_dist; /* mImagePlaneDist, in pixels */
+} wr_camera;
+
+enum { WR_MEGAKERNEL = 0, WR_WAVEFRONT = 1 };
+typedef struct {
+    uint32_t spp;        /* passes to add, one sample per pixel each (PathTracer spp) */
+    uint32_t max_depth;  /* MCRenderer maxPathLen (mcRenderer.h:16) */
+    uint64_t seed;       /* counter-RNG key */
+    uint32_t mode;       /* WR_MEGAKERNEL (one lane per pixel) or WR_WAVEFRONT */
+    uint32_t wave_slots; /* wavefront path-slot pool size, 0 = automatic */
+    uint32_t wave_grid;  /* wavefront workgroups per stage launch, 0 = automatic */
+    uint32_t reserved;
+} wr_render_params;
+
+typedef struct {
+    uint64_t paths;       /* paths traced by this call */
+    uint64_t iterations;  /* wavefront iterations (0 for the megakernel) */
+    float kernel_ms;      /* device time of this call (HIP events on the context stream) */
+} wr_render_stats;
+
+/* Film: per pixel and channel a Kahan-compensated FP32 sum of the per-pass
+ * samples, its compensation term and a plain sum of squares, plus the number
+ * of passes.  Replaces the running-mean vector<vec3f> of pathTracer.cpp:10,24,69;
+ * mean = sum / passes.  Pixels are row-major, top row first (camera.cpp:14-15). */
+typedef struct wr_film wr_film;
+int wr_film_create(wr_ctx* ctx, uint32_t width, uint32_t height, wr_film** out);
+void wr_film_destroy(wr_film* film);
+int wr_film_clear(wr_film* film);
+/* Any output pointer may be NULL.  mean_rgb = sum / passes (3 floats per pixel). */
+int wr_film_read(wr_film* film, float* mean_rgb, float* sum, float* comp, float* sumsq, uint32_t* passes);
+/* Restores a film read with wr_film_read (checkpoint / resume). */
+int wr_film_write(wr_film* film, const float* sum, const float* comp, const float* sumsq, uint32_t passes);
+
+/* PathTracer::renderPixels (pathTracer.cpp:7-76) for passes
+ * [passes, passes + spp) of the film, accumulated into it.  stats nullable. */
+int wr_render(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, const wr_render_params* params,
+              wr_film* film, wr_render_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -648,6 +648,9 @@ struct Tracer {
             if (inside != ORC_NONE) self = inside;
             else if (hit.valid) self = hit.obj;
             else { verts++; break; }  /* terminateRay: contributes 0 */
+            /* inside an object but no hit (an open mesh): the reference would
+             * dereference a NULL intersectObj; treated as terminateRay */
+            if (!hit.valid) { verts++; break; }
             rng4(seed, p, pass, depth + 1, rn);
             Out out = scatter(self, ray, hit, inside, rn);
             verts++;

@@ -1,0 +1,113 @@
+"""GPU path tracer through the C ABI: parity with the oracle and closed forms.
+
+Bars:
+  * megakernel and wavefront films are bit-identical (same per-sample values,
+    folded into the Kahan film in pass order);
+  * GPU vs oracle image: per pixel and channel |mean_gpu - mean_oracle| <= 3 sigma
+    of the combined Monte Carlo standard error, allowing the Gaussian false-alarm
+    rate (at most 1% of pixel-channels beyond 3 sigma, none beyond 6 sigma);
+  * closed-form furnaces: expected values within 4 standard errors.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from walkray_amd import _lib as W
+from walkray_amd import render as R
+from walkray_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def test_megakernel_equals_wavefront(ctx):
+    objs, cam = S.cornell(k=3, width=40, height=32)
+    sc = W.Scene(ctx, objs)
+    a = R.render_scene(ctx, objs, cam, 12, 8, 99, W.WR_MEGAKERNEL, scene=sc)
+    b = R.render_scene(ctx, objs, cam, 12, 8, 99, W.WR_WAVEFRONT, scene=sc)
+    for k in ("sum", "comp", "sumsq"):
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), f"film {k} differs"
+    # a small slot pool (paths of many passes in flight, pass window wraps) must not change anything
+    c = R.render_scene(ctx, objs, cam, 12, 8, 99, W.WR_WAVEFRONT, scene=sc, wave_slots=1000)
+    assert np.array_equal(a["sum"].view(np.uint32), c["sum"].view(np.uint32))
+    assert a["mean"].mean() > 0.05
+
+
+def _zcheck(gpu, orc_sum, orc_sq, n):
+    mg, seg = R.film_stats(gpu)
+    mo = orc_sum / n
+    vo = np.maximum(orc_sq / n - mo * mo, 0)
+    se = np.sqrt(seg ** 2 + vo / n)
+    diff = np.abs(mg - mo)
+    nz = se > 0
+    z = np.zeros_like(diff)
+    z[nz] = diff[nz] / se[nz]
+    zero_ok = np.all(diff[~nz] <= 1e-5 * np.maximum(1.0, np.abs(mo[~nz])))
+    return z, zero_ok
+
+
+@pytest.mark.parametrize("mode", [W.WR_MEGAKERNEL, W.WR_WAVEFRONT])
+def test_cornell_matches_oracle(ctx, mode):
+    objs, cam = S.cornell(k=3, width=32, height=32)
+    spp, depth, seed = 64, 8, 2024
+    g = R.render_scene(ctx, objs, cam, spp, depth, seed, mode)
+    osc = O.Scene(objs)
+    s, q = osc.render(cam, spp, depth, seed)
+    z, zero_ok = _zcheck(g, s, q, spp)
+    assert zero_ok
+    assert (z > 3).mean() <= 0.01, f"{(z > 3).mean():.4f} of pixel-channels beyond 3 sigma"
+    assert z.max() <= 6, f"max z {z.max():.2f}"
+    # same RNG stream: most samples are the very same paths
+    same = np.isclose(g["sum"], s.astype(np.float32), rtol=1e-4, atol=1e-6).mean()
+    assert same > 0.5, f"only {same:.3f} of pixel sums agree closely"
+
+
+def _furnace(floor_mat, floor_params, le=2.0, width=16):
+    floor = S.quad((0, 0, 0), (2000, 0, 0), (0, 0, 2000), (0, 1, 0), floor_mat, floor_params)
+    ceil = S.quad((0, 1, 0), (2000, 0, 0), (0, 0, 2000), (0, -1, 0), S.EMISSIVE, (le, le, le))
+    cam = S.camera((0.0, 0.5, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), width, width, sight_dist=100.0 * width)
+    return [floor, ceil], cam
+
+
+@pytest.mark.parametrize("mode", [W.WR_MEGAKERNEL, W.WR_WAVEFRONT])
+def test_furnace_diffuse(ctx, mode):
+    """Diffuse floor (albedo a) under an emitting ceiling Le: pixel value a*Le for depth >= 1, 0 for depth 0."""
+    a, le, spp = 0.5, 2.0, 64
+    objs, cam = _furnace(S.DIFFUSE, (a, a, a), le)
+    g = R.render_scene(ctx, objs, cam, spp, 1, 5, mode)
+    n = cam["width"] * cam["height"] * spp
+    mean = g["mean"].reshape(-1, 3).mean(axis=0)
+    se = le * np.sqrt(a * (1 - a) / n)
+    assert np.all(np.abs(mean - a * le) < 4 * se + 1e-3), (mean, a * le)
+    g0 = R.render_scene(ctx, objs, cam, 4, 0, 5, mode)
+    assert np.all(g0["sum"] == 0)
+
+
+@pytest.mark.parametrize("mode", [W.WR_MEGAKERNEL, W.WR_WAVEFRONT])
+def test_furnace_phong_normal_incidence(ctx, mode):
+    """Phong lobe sampled about the normal, evaluated about the mirror direction:
+    at normal incidence E[weight] = albedo exactly ((n+2)/(n+1) * E[cos] = 1)."""
+    a, le, spp, power = 0.6, 1.5, 64, 20.0
+    objs, cam = _furnace(S.GLOSSY, (a, a, a, power), le)
+    g = R.render_scene(ctx, objs, cam, spp, 3, 11, mode)
+    n = cam["width"] * cam["height"] * spp
+    v = g["sumsq"].reshape(-1, 3).sum(axis=0) / n - (g["sum"].reshape(-1, 3).sum(axis=0) / n) ** 2
+    se = np.sqrt(np.maximum(v, 0) / n)
+    mean = g["mean"].reshape(-1, 3).mean(axis=0)
+    assert np.all(np.abs(mean - a * le) < 4 * se + 2e-3), (mean, a * le, se)
+
+
+def test_checkpoint_resume(ctx):
+    objs, cam = S.cornell(k=2, width=24, height=20)
+    sc = W.Scene(ctx, objs)
+    one = W.Film(ctx, 24, 20)
+    W.render(sc, cam, one, 10, 8, 3, W.WR_WAVEFRONT)
+    part = W.Film(ctx, 24, 20)
+    W.render(sc, cam, part, 4, 8, 3, W.WR_MEGAKERNEL)
+    state = part.read()
+    resumed = W.Film(ctx, 24, 20)
+    resumed.write(state)
+    W.render(sc, cam, resumed, 6, 8, 3, W.WR_WAVEFRONT)
+    a, b = one.read(), resumed.read()
+    assert a["passes"] == b["passes"] == 10
+    for k in ("sum", "comp", "sumsq"):
+        assert np.array_equal(a[k], b[k]), k

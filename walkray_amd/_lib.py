@@ -94,6 +94,12 @@ def lib():
         "wr_closest_hit": (i32, [vp, vp, vp, u64, vp, vp]),
         "wr_closest_hit_noself": (i32, [vp, vp, vp, u64, vp]),
         "wr_inside_object": (i32, [vp, vp, vp, u64, vp]),
+        "wr_film_create": (i32, [vp, u32, u32, P(vp)]),
+        "wr_film_destroy": (None, [vp]),
+        "wr_film_clear": (i32, [vp]),
+        "wr_film_read": (i32, [vp, vp, vp, vp, vp, P(u32)]),
+        "wr_film_write": (i32, [vp, vp, vp, vp, u32]),
+        "wr_render": (i32, [vp, vp, P(CameraDesc), P(RenderParams), vp, P(RenderStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -202,6 +208,66 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("seed", C.c_uint64), ("mode", C.c_uint32),
+                ("wave_slots", C.c_uint32), ("wave_grid", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("iterations", C.c_uint64), ("kernel_ms", C.c_float)]
+
+
+WR_MEGAKERNEL, WR_WAVEFRONT = 0, 1
+
+
+class Film:
+    """wr_film: Kahan-compensated per-pixel sums of the per-pass samples."""
+
+    def __init__(self, ctx, width, height):
+        self.ctx = ctx
+        self.width, self.height = int(width), int(height)
+        self.h = C.c_void_p()
+        _check(lib().wr_film_create(ctx.h, self.width, self.height, C.byref(self.h)), ctx.h, "wr_film_create")
+
+    def clear(self):
+        _check(lib().wr_film_clear(self.h), self.ctx.h, "wr_film_clear")
+
+    def read(self):
+        n = 3 * self.width * self.height
+        mean, s, c, q = (np.zeros(n, np.float32) for _ in range(4))
+        passes = C.c_uint32()
+        _check(lib().wr_film_read(self.h, _p(mean), _p(s), _p(c), _p(q), C.byref(passes)), self.ctx.h,
+               "wr_film_read")
+        return {"mean": mean, "sum": s, "comp": c, "sumsq": q, "passes": passes.value}
+
+    def write(self, state):
+        _check(lib().wr_film_write(self.h, _p(np.ascontiguousarray(state["sum"], np.float32)),
+                                   _p(np.ascontiguousarray(state["comp"], np.float32)),
+                                   _p(np.ascontiguousarray(state["sumsq"], np.float32)), int(state["passes"])),
+               self.ctx.h, "wr_film_write")
+
+    def close(self):
+        if self.h:
+            lib().wr_film_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def render(scene, cam, film, spp, max_depth, seed, mode=WR_WAVEFRONT, wave_slots=0, wave_grid=0):
+    """PathTracer::renderPixels for `spp` more passes into `film`; returns stats."""
+    cd = CameraDesc.from_dict(cam)
+    prm = RenderParams(spp, max_depth, seed, mode, wave_slots, wave_grid, 0)
+    st = RenderStats()
+    _check(lib().wr_render(scene.ctx.h, scene.h, C.byref(cd), C.byref(prm), film.h, C.byref(st)), scene.ctx.h,
+           "wr_render")
+    return {"paths": st.paths, "iterations": st.iterations, "kernel_ms": st.kernel_ms}
 
 
 HIT_DTYPE = np.dtype([("prim", np.uint32), ("t", np.float32), ("b1", np.float32), ("b2", np.float32)])

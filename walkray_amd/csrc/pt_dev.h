@@ -1,0 +1,336 @@
+// pt_dev.h -- device-side path tracing for gfx950: counter RNG, camera,
+// the four hot materials' scatter, the per-path loop and the Kahan film.
+//
+// Estimator (kept): MCRenderer::samplePath (mcRenderer.cpp:16-57) and the
+// PathTracer pixel loop (pathTracer.cpp:20-70): path weight = product over
+// inner vertices of cosineTerm * radiance / originProb / directionProb
+// (vec3 / 0 is a no-op, nvVector.h:375-379), times the terminal vertex's
+// radiance; NaN/Inf/negative samples are zeroed (util.h:8-18); the camera
+// vignetting fix (camera.cpp:107-122) divides by |cos| of the pixel-centre
+// direction.  Randomness comes from Philox-4x32-10 keyed by (seed, pixel, pass,
+// vertex) instead of the reference's one shared mt19937_64 (rng.h:6-59); the
+// draws of one vertex are: camera (u, v); diffuse/glossy (roulette, s.x, s.y);
+// refractive (Fresnel choice).
+#pragma once
+#include "trace_dev.h"
+#include "wr_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace wrd {
+
+constexpr double kPi = 3.1415926535897932384626433832795;      // macros.h:5
+constexpr double kInvPi = 0.31830988618379067153776752674503;  // macros.h:6
+constexpr double kEps = 1e-5;                                  // macros.h:7
+
+// ---- Philox-4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers:
+// as easy as 1, 2, 3", SC'11) -------------------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
+        const uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// Four uniforms in [0, 1) (24-bit) for one path vertex.
+__device__ __forceinline__ void rng4(uint64_t seed, uint32_t pixel, uint32_t pass, uint32_t vertex, float f[4]) {
+    uint32_t c[4] = {pixel, pass, vertex, 0x57414C4Bu};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+#pragma unroll
+    for (int i = 0; i < 4; i++) f[i] = (float)(c[i] >> 8) * (1.0f / 16777216.0f);
+}
+
+__device__ __forceinline__ bool legal(f3 c) {  // util.h:8-18
+    return isfinite(c.x) && isfinite(c.y) && isfinite(c.z) && c.x >= 0.f && c.y >= 0.f && c.z >= 0.f;
+}
+__device__ __forceinline__ f3 mulv(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+
+// ---- camera (camera.cpp:6-60, 90-123) -------------------------------------
+struct CamDev {
+    f3 pos, front, right, top;
+    float dist;
+    uint32_t W, H;
+};
+
+__device__ __forceinline__ f3 cam_dir(const CamDev& c, uint32_t p, float u, float v) {
+    const uint32_t x = p % c.W, y = p / c.W;
+    const f3 d = add(add(muls(c.front, c.dist), muls(c.right, ((float)x + u) - (c.W / 2.f))),
+                     muls(c.top, ((c.H / 2.f) - (float)y) - v));
+    return normalize(d);
+}
+
+__device__ __forceinline__ f3 fix_vignetting(const CamDev& c, f3 color, uint32_t p) {
+    const uint32_t x = p % c.W, y = p / c.W;
+    // x + 0.5 - W/2.0 and H/2.0 - 0.5 - y are exact in float for sizes < 2^23
+    f3 d = add(add(muls(c.front, c.dist), muls(c.right, (float)x + 0.5f - c.W * 0.5f)),
+               muls(c.top, c.H * 0.5f - 0.5f - (float)y));
+    d = normalize(d);
+    const float fc = dot(d, c.front);
+    if (fc == 0.f) return mk(0.f, 0.f, 0.f);
+    const f3 r = divs(color, fabsf(fc));
+    return legal(r) ? r : mk(0.f, 0.f, 0.f);
+}
+
+// ---- scene access -----------------------------------------------------------
+struct SceneCtx {
+    SceneDev S;
+    Tree tr;               // global tree
+    const uint32_t* gobj;  // global record -> object
+    const float* gnrm;     // global record -> 9 vertex-normal floats
+};
+
+__device__ __forceinline__ SceneCtx make_scene_ctx(const SceneDev& S) {
+    SceneCtx c;
+    c.S = S;
+    const wr_tree_hdr H = S.hdrs[0];
+    c.tr = make_tree(H, S.nodes, S.tris, S.leaf_up, S.leaf_cnt, S.node_up);
+    c.gobj = S.rec_obj + H.rec_base;
+    c.gnrm = S.rec_nrm + 9 * (size_t)H.rec_base;
+    return c;
+}
+
+struct PHit {
+    float t;
+    uint32_t rec;  // global-tree record, WR_NONE on a miss
+    uint32_t obj;
+};
+
+// Scene::intersect (scene.cpp:102-113), with or without NoSelfCondition.
+template <bool NOSELF>
+__device__ __forceinline__ PHit trace_scene(const SceneCtx& sc, f3 o, f3 d, uint32_t inside, uint32_t contact,
+                                            LaneStack& st) {
+    const RayPre r = make_ray(o, d, sc.tr.scale);
+    NoSelf ns{inside, contact, sc.gobj, sc.gnrm, sc.S.obj_has_normals};
+    Counters cnt{0, 0, 0};
+    bool failed;
+    const HitState h = closest_hit<NOSELF, false>(sc.tr, r, ns, st, cnt, failed);
+    PHit ph;
+    ph.t = h.t;
+    ph.rec = h.rec;
+    ph.obj = h.rec == WR_NONE ? WR_NONE : sc.gobj[h.rec];
+    return ph;
+}
+
+// Mesh::getWorldNormal of a global record at p.
+__device__ __forceinline__ f3 rec_normal(const SceneCtx& sc, uint32_t rec, uint32_t obj, f3 p) {
+    const float* nrm = sc.S.obj_has_normals[obj] ? sc.gnrm + 9 * (size_t)rec : nullptr;
+    float u1, u2;
+    return world_normal(sc.tr.tris[rec], nrm, p, u1, u2);
+}
+
+// ---- scatter (AbstractObject::scatter of the hot materials) -----------------
+struct Out {
+    f3 o, d, rad, n;  // n: contactObj->getWorldNormal(contactTriangleID, o)
+    float op, dp;
+    uint32_t inside, contact;
+};
+
+// Ray::cosineTerm (ray.cpp:52-58) of a scatter vertex.
+__device__ __forceinline__ float cosine_term(const Out& r) {
+    float c = 1.f;
+    if ((double)len(r.d) > 0.5) c = fabsf(dot(r.n, r.d));
+    return (double)c > kEps ? c : (float)kEps;
+}
+
+// Frame::setFromN + toWorldFrame (frame.h:23-49).
+__device__ __forceinline__ f3 to_world(f3 w, f3 l) {
+    f3 mX, mZ;
+    if ((double)fabsf(w.x) > kEps) {
+        mX = normalize(cross(mk(0.f, 1.f, 0.f), w));
+        mZ = cross(mX, w);
+    } else {
+        mZ = normalize(cross(mk(1.f, 0.f, 0.f), w));
+        mX = cross(w, mZ);
+    }
+    return add(add(muls(mX, l.x), muls(w, l.y)), muls(mZ, l.z));
+}
+
+__device__ __forceinline__ f3 mirror(f3 N, f3 d) {  // -N.dot(d)*N*2 + d, normalized
+    return normalize(add(muls(muls(N, -dot(N, d)), 2.f), d));
+}
+
+__device__ Out scatter(const SceneCtx& sc, uint32_t self, f3 in_o, f3 in_d, const PHit& h, uint32_t in_inside,
+                       const float* rn, LaneStack& st) {
+    const f3 pos = add(in_o, muls(in_d, h.t));
+    const f3 N = rec_normal(sc, h.rec, h.obj, pos);
+    // refractiveObj.h:32-33: inside this refractive object and hitting another -> that object scatters
+    if (sc.S.objs[self].mat == WR_MAT_REFRACTIVE && h.obj != self && in_inside == self) self = h.obj;
+    const wr_object_gpu& ob = sc.S.objs[self];
+    Out out;
+    out.o = pos;
+    out.n = N;
+    out.contact = self;
+    out.op = 1.f;
+    out.dp = 1.f;
+    out.rad = mk(1.f, 1.f, 1.f);
+    out.inside = in_inside;
+    const f3 color = mk(ob.color[0], ob.color[1], ob.color[2]);
+    if (ob.mat == WR_MAT_DIFFUSE) {  // diffuseObj.h:34-57, diffuseBsdf.h:17-26, cosHemiSphereSampler.h:6-18
+        const float p = ob.pmax;
+        if (rn[0] < p) {
+            const float term1 = (float)(2.0 * kPi * (double)rn[1]);
+            const float term2 = sqrtf(1.f - rn[2]);
+            const f3 local = mk(sinf(term1) * term2, sqrtf(rn[2]), cosf(term1) * term2);
+            out.d = to_world(N, local);
+            const float c = dot(N, out.d);
+            out.dp = c > 0.f ? (float)((double)c * kInvPi) : 0.f;
+            out.rad = muls(color, (float)kInvPi);
+            out.dp *= p;
+        } else {
+            out.d = mk(0.f, 0.f, 0.f);
+            out.rad = mk(0.f, 0.f, 0.f);
+            out.dp = 1.f - p;
+        }
+    } else if (ob.mat == WR_MAT_GLOSSY) {  // glossyObj.h:39-62, isotropicPhong.h:20-38, powerCos...h:8-21
+        const float p = ob.pmax;
+        if (rn[0] < p) {
+            const float power = ob.power;
+            const float term1 = (float)(2.0 * kPi * (double)rn[1]);
+            const float term2 = (float)pow((double)rn[2], 1.0 / ((double)power + 1.0));
+            const float term3 = (float)sqrt(1.0 - (double)(term2 * term2));
+            const f3 local = mk(sinf(term1) * term3, term2, cosf(term1) * term3);
+            const f3 w = to_world(N, local);
+            const float c = dot(N, w);
+            out.dp = c > 0.f ? (float)(((double)power + 1.0) * (double)powf(c, power) * (kInvPi * 0.5)) : 0.f;
+            const f3 refl = mirror(N, in_d);
+            const f3 rho = muls(muls(muls(color, (float)((double)power + 2.0)), 0.5f), (float)kInvPi);
+            const float dRW = dot(refl, w);
+            if (dot(w, N) < 0.f) {
+                out.d = mk(0.f, 0.f, 0.f);
+                out.rad = mk((float)kEps, (float)kEps, (float)kEps);
+            } else {
+                out.d = w;
+                out.rad = muls(rho, powf(dRW > 0.f ? dRW : 0.f, power));
+            }
+            out.dp *= p;
+        } else {
+            out.d = mk(0.f, 0.f, 0.f);
+            out.rad = mk(0.f, 0.f, 0.f);
+            out.dp = 1.f - p;
+        }
+    } else if (ob.mat == WR_MAT_REFRACTIVE) {  // refractiveObj.h:27-88
+        out.d = in_d;
+        const f3 refl = mirror(N, in_d);
+        const float theta = acosf(dot(in_d, N));
+        uint32_t cur_out = self;
+        if (in_inside == self) cur_out = find_inside(sc.S, out.o, in_d, self, st);
+        const float cn = in_inside != WR_NONE ? sc.S.objs[in_inside].ior : 1.f;
+        const float nn = cur_out != WR_NONE ? sc.S.objs[cur_out].ior : 1.f;
+        const float sin_phi = cn / nn * sinf(theta);
+        if (sin_phi > 1.f) {  // total internal reflection: radiance 1, probabilities 1
+            out.d = refl;
+            out.inside = in_inside;
+        } else {
+            float phi = asinf(sin_phi);
+            if ((double)theta > kPi / 2) phi = (float)(kPi - (double)phi);
+            const f3 ax = normalize(cross(N, in_d));
+            // RotateMatrix(axis, phi) * vec4f(N, 0)  (util.h:40-54, nvMatrix.h:98-105,424-433)
+            const float c = cosf(phi), s = sinf(phi);
+            const float _c = 1.f - c;
+            const float x = ax.x, y = ax.y, z = ax.z;
+            const float m00 = c + _c * x * x, m01 = _c * x * y - s * z, m02 = _c * x * z + s * y;
+            const float m10 = _c * x * y + s * z, m11 = c + _c * y * y, m12 = _c * y * z - s * x;
+            const float m20 = _c * x * z - s * y, m21 = _c * y * z + s * x, m22 = c + _c * z * z;
+            f3 rd;
+            rd.x = N.x * m00 + N.y * m01 + N.z * m02 + 0.f * 0.f;
+            rd.y = N.x * m10 + N.y * m11 + N.z * m12 + 0.f * 0.f;
+            rd.z = N.x * m20 + N.y * m21 + N.z * m22 + 0.f * 0.f;
+            out.d = normalize(rd);
+            const float cos_theta = fabsf(cosf(theta));
+            const float cos_phi = fabsf(cosf(phi));
+            const float a = fabsf(cn * cos_theta - nn * cos_phi) / (cn * cos_theta + nn * cos_phi);
+            const float b = fabsf(nn * cos_theta - cn * cos_phi) / (nn * cos_theta + cn * cos_phi);
+            const float er = (a * a + b * b) / 2.f;
+            if (rn[0] < er) {
+                out.d = refl;
+                out.rad = muls(out.rad, er / cosine_term(out));
+                out.dp = er;
+                out.inside = in_inside;
+            } else {
+                out.rad = muls(out.rad, (1.f - er) / cosine_term(out));
+                out.dp = 1.f - er;
+                out.inside = cur_out;
+            }
+        }
+        out.d = normalize(out.d);
+    } else {  // emissive, emissiveObj.h:15-28
+        out.d = mk(0.f, 0.f, 0.f);
+        out.inside = WR_NONE;
+        out.rad = dot(in_d, N) <= 0.f ? color : mk(0.f, 0.f, 0.f);
+    }
+    return out;
+}
+
+// ---- Kahan film ---------------------------------------------------------------
+struct FilmDev {
+    float* sum;    // 3 per pixel
+    float* comp;   // 3 per pixel (Kahan compensation)
+    float* sumsq;  // 3 per pixel, plain sum of squares (for standard errors)
+};
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+__device__ __forceinline__ void film_add(const FilmDev& f, uint32_t p, f3 v) {
+    const float vv[3] = {v.x, v.y, v.z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float s = f.sum[3 * p + k], c = f.comp[3 * p + k];
+        kahan_add(s, c, vv[k]);
+        f.sum[3 * p + k] = s;
+        f.comp[3 * p + k] = c;
+        f.sumsq[3 * p + k] += vv[k] * vv[k];
+    }
+}
+
+// ---- one eye path, megakernel style (mcRenderer.cpp:16-57, pathTracer.cpp:22-69) ----
+__device__ f3 sample_path(const SceneCtx& sc, const CamDev& cam, uint32_t max_depth, uint64_t seed, uint32_t p,
+                          uint32_t pass, LaneStack& st) {
+    float rn[4];
+    rng4(seed, p, pass, 0, rn);
+    f3 o = cam.pos;
+    f3 d = cam_dir(cam, p, rn[0], rn[1]);
+    PHit hit = trace_scene<false>(sc, o, d, WR_NONE, WR_NONE, st);
+    uint32_t inside = find_inside(sc.S, o, d, WR_NONE, st);
+    float c0 = fabsf(dot(cam.front, d));
+    c0 = (double)c0 > kEps ? c0 : (float)kEps;
+    f3 T = mk(c0, c0, c0);
+    f3 color = mk(0.f, 0.f, 0.f);
+    uint32_t depth = 0;
+    for (;;) {
+        uint32_t self = inside != WR_NONE ? inside : hit.obj;
+        if (self == WR_NONE || hit.rec == WR_NONE) break;  // terminateRay: contributes 0
+        rng4(seed, p, pass, depth + 1, rn);
+        const Out out = scatter(sc, self, o, d, hit, inside, rn, st);
+        if ((double)len(out.d) < 0.5) {  // emission / absorption ends the path
+            color = mulv(T, divs(divs(out.rad, out.op), out.dp));
+            break;
+        }
+        if (depth + 1 > max_depth) break;
+        const PHit nh = trace_scene<true>(sc, out.o, out.d, out.inside, out.contact, st);
+        if (nh.rec == WR_NONE) break;
+        T = muls(T, cosine_term(out));
+        T = mulv(T, divs(divs(out.rad, out.op), out.dp));
+        o = out.o;
+        d = out.d;
+        hit = nh;
+        inside = out.inside;
+        depth++;
+    }
+    if (!legal(color)) color = mk(0.f, 0.f, 0.f);
+    return fix_vignetting(cam, color, p);
+}
+
+}  // namespace wrd

@@ -1,0 +1,179 @@
+// render.cpp -- extern "C" film and render entry points (include/walkray.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/walkray.h"
+#include "wr_capi_internal.h"
+#include "wr_pt.h"
+
+struct wr_film {
+    wr_ctx* ctx = nullptr;
+    uint32_t W = 0, H = 0;
+    float* sum = nullptr;
+    float* comp = nullptr;
+    float* sumsq = nullptr;
+    uint32_t passes = 0;
+};
+
+namespace {
+
+// nvVector semantics (nvVector.h:490-525): sequential dot from 0, sqrt, r > 0
+// guard, true division.
+void vnormalize(float v[3]) {
+    float r2 = 0.f;
+    r2 += v[0] * v[0];
+    r2 += v[1] * v[1];
+    r2 += v[2] * v[2];
+    const float r = std::sqrt(r2);
+    if (r > 0)
+        for (int k = 0; k < 3; k++) v[k] = v[k] / r;
+}
+void vcross(const float a[3], const float b[3], float o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Camera basis of camera.cpp:7-11.
+wr::CamHost make_cam(const wr_camera& c) {
+    wr::CamHost h;
+    for (int k = 0; k < 3; k++) {
+        h.pos[k] = c.pos[k];
+        h.front[k] = c.focus[k] - c.pos[k];
+    }
+    vnormalize(h.front);
+    vcross(h.front, c.up, h.right);
+    vnormalize(h.right);
+    vcross(h.right, h.front, h.top);
+    h.dist = c.sight_dist;
+    h.W = c.width;
+    h.H = c.height;
+    return h;
+}
+
+}  // namespace
+
+using namespace wr;
+
+extern "C" {
+
+int wr_film_create(wr_ctx* ctx, uint32_t width, uint32_t height, wr_film** out) {
+    if (!ctx || !out || width == 0 || height == 0) return fail(ctx, WR_EINVAL, "wr_film_create: bad argument");
+    *out = nullptr;
+    (void)hipSetDevice(ctx->device);
+    wr_film* f = new wr_film;
+    f->ctx = ctx;
+    f->W = width;
+    f->H = height;
+    const size_t b = sizeof(float) * 3 * (size_t)width * height;
+    if (hipMalloc((void**)&f->sum, b) != hipSuccess || hipMalloc((void**)&f->comp, b) != hipSuccess ||
+        hipMalloc((void**)&f->sumsq, b) != hipSuccess) {
+        wr_film_destroy(f);
+        return fail(ctx, WR_ENOMEM, "wr_film_create: out of device memory");
+    }
+    int rc = wr_film_clear(f);
+    if (rc) {
+        wr_film_destroy(f);
+        return rc;
+    }
+    *out = f;
+    return WR_OK;
+}
+
+void wr_film_destroy(wr_film* f) {
+    if (!f) return;
+    (void)hipStreamSynchronize(f->ctx->stream);
+    if (f->sum) (void)hipFree(f->sum);
+    if (f->comp) (void)hipFree(f->comp);
+    if (f->sumsq) (void)hipFree(f->sumsq);
+    delete f;
+}
+
+int wr_film_clear(wr_film* f) {
+    if (!f) return fail(nullptr, WR_EINVAL, "wr_film_clear: film is NULL");
+    const size_t b = sizeof(float) * 3 * (size_t)f->W * f->H;
+    hipError_t e = hipMemsetAsync(f->sum, 0, b, f->ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(f->comp, 0, b, f->ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(f->sumsq, 0, b, f->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(f->ctx->stream);
+    if (e != hipSuccess) return hip_fail(f->ctx, e, "wr_film_clear");
+    f->passes = 0;
+    return WR_OK;
+}
+
+int wr_film_read(wr_film* f, float* mean_rgb, float* sum, float* comp, float* sumsq, uint32_t* passes) {
+    if (!f) return fail(nullptr, WR_EINVAL, "wr_film_read: film is NULL");
+    const size_t n = 3 * (size_t)f->W * f->H;
+    std::vector<float> tmp;
+    float* s = sum;
+    if (!s && mean_rgb) {
+        tmp.resize(n);
+        s = tmp.data();
+    }
+    hipError_t e = hipSuccess;
+    if (s) e = hipMemcpyAsync(s, f->sum, n * sizeof(float), hipMemcpyDeviceToHost, f->ctx->stream);
+    if (e == hipSuccess && comp) e = hipMemcpyAsync(comp, f->comp, n * sizeof(float), hipMemcpyDeviceToHost, f->ctx->stream);
+    if (e == hipSuccess && sumsq) e = hipMemcpyAsync(sumsq, f->sumsq, n * sizeof(float), hipMemcpyDeviceToHost, f->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(f->ctx->stream);
+    if (e != hipSuccess) return hip_fail(f->ctx, e, "wr_film_read");
+    if (mean_rgb) {
+        const float inv = f->passes ? 1.f / (float)f->passes : 0.f;
+        for (size_t i = 0; i < n; i++) mean_rgb[i] = f->passes ? s[i] * inv : 0.f;
+    }
+    if (passes) *passes = f->passes;
+    return WR_OK;
+}
+
+int wr_film_write(wr_film* f, const float* sum, const float* comp, const float* sumsq, uint32_t passes) {
+    if (!f || !sum || !comp || !sumsq) return fail(f ? f->ctx : nullptr, WR_EINVAL, "wr_film_write: bad argument");
+    const size_t b = sizeof(float) * 3 * (size_t)f->W * f->H;
+    hipError_t e = hipMemcpyAsync(f->sum, sum, b, hipMemcpyHostToDevice, f->ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(f->comp, comp, b, hipMemcpyHostToDevice, f->ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(f->sumsq, sumsq, b, hipMemcpyHostToDevice, f->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(f->ctx->stream);
+    if (e != hipSuccess) return hip_fail(f->ctx, e, "wr_film_write");
+    f->passes = passes;
+    return WR_OK;
+}
+
+int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_render_params* prm, wr_film* film,
+              wr_render_stats* stats) {
+    if (!ctx || !s || !cam || !prm || !film) return fail(ctx, WR_EINVAL, "wr_render: bad argument");
+    if (!s->built) return fail(ctx, WR_ESTATE, "wr_render: scene not built");
+    if (film->W != cam->width || film->H != cam->height)
+        return fail(ctx, WR_EINVAL, "wr_render: film size differs from the camera resolution");
+    if (prm->mode != WR_MEGAKERNEL && prm->mode != WR_WAVEFRONT) return fail(ctx, WR_EINVAL, "wr_render: unknown mode");
+    if ((uint64_t)cam->width * cam->height >= (1ull << 31)) return fail(ctx, WR_EINVAL, "wr_render: image too large");
+    (void)hipSetDevice(ctx->device);
+    const CamHost ch = make_cam(*cam);
+    FilmBufs fb{film->sum, film->comp, film->sumsq};
+    hipError_t e = hipEventRecord(ctx->ev0, ctx->stream);
+    WaveStats ws;
+    if (e == hipSuccess) {
+        if (prm->mode == WR_MEGAKERNEL) {
+            for (uint32_t k = 0; k < prm->spp && e == hipSuccess; k++)
+                e = launch_pt_mega(s->dev, ch, prm->max_depth, prm->seed, film->passes + k, fb, ctx->stream);
+        } else {
+            e = run_wavefront(s->dev, ch, prm->max_depth, prm->seed, film->passes, prm->spp, fb, ctx->pool,
+                              prm->wave_slots, prm->wave_grid, &ws, ctx->stream);
+        }
+    }
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    if (e != hipSuccess) return hip_fail(ctx, e, "wr_render");
+    film->passes += prm->spp;
+    if (stats) {
+        stats->paths = (uint64_t)prm->spp * cam->width * cam->height;
+        stats->iterations = ws.iterations;
+        stats->kernel_ms = ms;
+    }
+    return WR_OK;
+}
+
+}  // extern "C"

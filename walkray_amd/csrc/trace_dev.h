@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "wr_kernels.h"
 #include "wr_layout.h"
 
 #pragma clang fp contract(off)
@@ -394,6 +395,48 @@ __device__ __forceinline__ HitState closest_hit(const Tree& tr, const RayPre& r,
     traverse<NOSELF, true, COUNT>(tr, r, ns, st, h, overflow, cnt);
     failed = overflow;
     return h;
+}
+
+// Scene::findInsideObject for one ray; objects whose root box does not contain
+// the origin are skipped: one of the two opposite half-rays then provably fails
+// the reference box test of that object's root (its face points move away from
+// the box along the separating axis), so the reference skips them too.
+__device__ __forceinline__ uint32_t find_inside(const SceneDev& S, const f3& o, const f3& d, uint32_t cur,
+                                                LaneStack& st) {
+    NoSelf ns{};
+    Counters cnt{0, 0, 0};
+    const f3 db = mk(-d.x, -d.y, -d.z);
+    for (int i = (int)S.n_objs - 1; i >= 0; i--) {
+        if ((uint32_t)i == cur) continue;
+        const wr_object_gpu& ob = S.objs[i];
+        if (!(ob.aabb_min[0] <= o.x && o.x <= ob.aabb_max[0] && ob.aabb_min[1] <= o.y && o.y <= ob.aabb_max[1] &&
+              ob.aabb_min[2] <= o.z && o.z <= ob.aabb_max[2]))
+            continue;
+        const wr_tree_hdr H = S.hdrs[ob.tree];
+        const Tree tr = make_tree(H, S.nodes, S.tris, S.leaf_up, S.leaf_cnt, S.node_up);
+        bool failed;
+        const float* nrm_base = ob.has_normals ? S.rec_nrm + 9 * (size_t)H.rec_base : nullptr;
+        {
+            const RayPre r = make_ray(o, db, tr.scale);
+            const HitState h = closest_hit<false, false>(tr, r, ns, st, cnt, failed);
+            if (h.rec == WR_NONE) continue;
+            float u1, u2;
+            const f3 N = world_normal(tr.tris[h.rec], nrm_base ? nrm_base + 9 * (size_t)h.rec : nullptr,
+                                      add(o, muls(db, h.t)), u1, u2);
+            if (dot(N, db) < 0.f) continue;
+        }
+        {
+            const RayPre r = make_ray(o, d, tr.scale);
+            const HitState h = closest_hit<false, false>(tr, r, ns, st, cnt, failed);
+            if (h.rec == WR_NONE) continue;
+            float u1, u2;
+            const f3 N = world_normal(tr.tris[h.rec], nrm_base ? nrm_base + 9 * (size_t)h.rec : nullptr,
+                                      add(o, muls(d, h.t)), u1, u2);
+            if (dot(N, d) < 0.f) continue;
+        }
+        return (uint32_t)i;
+    }
+    return WR_NONE;
 }
 
 }  // namespace wrd

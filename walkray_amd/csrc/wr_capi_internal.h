@@ -8,12 +8,14 @@
 #include "../../include/walkray.h"
 #include "wr_host.h"
 #include "wr_kernels.h"
+#include "wr_pt.h"
 
 struct wr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
+    wr::WavefrontPool pool;  // path-slot pool of the wavefront renderer, reused across calls
 };
 
 namespace wr {

@@ -1,0 +1,67 @@
+"""Path-tracing helpers over the C ABI: image statistics, benchmark and smoke check."""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _lib as W
+from . import scenes as S
+
+
+def film_stats(state):
+    """Per pixel/channel mean and standard error of the mean from a film state."""
+    n = max(int(state["passes"]), 1)
+    s = state["sum"].astype(np.float64)
+    q = state["sumsq"].astype(np.float64)
+    mean = s / n
+    var = np.maximum(q / n - mean * mean, 0.0)
+    return mean, np.sqrt(var / n)
+
+
+def rel_rmse(img, ref):
+    """||I - R||_2 / ||R||_2 over all pixels and channels."""
+    img = np.asarray(img, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.linalg.norm(img - ref) / max(np.linalg.norm(ref), 1e-300))
+
+
+def render_scene(ctx, objs, cam, spp, max_depth, seed, mode, scene=None, wave_slots=0):
+    sc = scene or W.Scene(ctx, objs)
+    film = W.Film(ctx, cam["width"], cam["height"])
+    st = W.render(sc, cam, film, spp, max_depth, seed, mode, wave_slots=wave_slots)
+    out = film.read()
+    out["stats"] = st
+    return out
+
+
+def bench_path_tracing(W_, S_, ctx, log, spp=16, max_depth=8, repeats=2):
+    """Cornell box (40,974 triangles), 256^2, 16 spp, depth 8: megakernel vs wavefront samples/s."""
+    objs, cam = S.cornell(k=5, width=256, height=256)
+    sc = W.Scene(ctx, objs)
+    res = {"workload": "Cornell box, 2 icospheres k=5 (40,974 triangles), 256x256, 16 spp, max depth 8, "
+                       "Russian roulette at every diffuse bounce (reference rule)"}
+    films = {}
+    for name, mode in (("megakernel", W.WR_MEGAKERNEL), ("wavefront", W.WR_WAVEFRONT)):
+        best = None
+        for _ in range(repeats):
+            film = W.Film(ctx, cam["width"], cam["height"])
+            st = W.render(sc, cam, film, spp, max_depth, seed=1234, mode=mode)
+            best = st["kernel_ms"] if best is None else min(best, st["kernel_ms"])
+            films[name] = film.read()
+        samples = cam["width"] * cam["height"] * spp
+        res[name] = {"ms": round(best, 3), "samples_per_s": round(samples / (best * 1e-3), 1)}
+        log(f"[bench] path tracing {name}: {best:.2f} ms, {samples / (best * 1e-3) / 1e6:.1f} Msamples/s")
+    res["speedup_wavefront_vs_megakernel"] = round(res["wavefront"]["samples_per_s"] /
+                                                   res["megakernel"]["samples_per_s"], 3)
+    res["films_identical"] = bool(np.array_equal(films["megakernel"]["sum"], films["wavefront"]["sum"]))
+    return res
+
+
+def smoke_check(ctx):
+    """A 24x24 Cornell render: megakernel and wavefront films must be identical."""
+    objs, cam = S.cornell(k=2, width=24, height=24)
+    a = render_scene(ctx, objs, cam, 4, 8, 7, W.WR_MEGAKERNEL)
+    b = render_scene(ctx, objs, cam, 4, 8, 7, W.WR_WAVEFRONT)
+    assert np.array_equal(a["sum"], b["sum"]), "megakernel and wavefront films differ"
+    print(f"smoke: render 24x24x4spp megakernel == wavefront, mean radiance {a['mean'].mean():.4f}")
