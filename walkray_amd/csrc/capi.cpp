@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -283,7 +284,51 @@ int wr_scene_build(wr_scene* s, int strategy) {
             objs.push_back(g);
             has_n.push_back(g.has_normals);
         }
+        // object-candidate grid for findInsideObject (see SceneDev)
+        float gmin[3] = {INFINITY, INFINITY, INFINITY}, gmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (size_t oi = 0; oi < objs.size(); oi++) {
+            if (s->trees[oi + 1].hdr.root_ref == WR_NONE) continue;
+            for (int a = 0; a < 3; a++) {
+                gmin[a] = std::min(gmin[a], objs[oi].aabb_min[a]);
+                gmax[a] = std::max(gmax[a], objs[oi].aabb_max[a]);
+            }
+        }
+        uint32_t gdim[3] = {1, 1, 1};
+        float ginv[3] = {0.f, 0.f, 0.f};
+        const bool any_obj = gmin[0] <= gmax[0];
+        if (any_obj) {
+            const uint32_t want = (uint32_t)std::min(64.0, std::max(1.0, std::round(2.0 * std::cbrt((double)objs.size()))));
+            for (int a = 0; a < 3; a++) {
+                const float ext = gmax[a] - gmin[a];
+                if (ext > 0.f) {
+                    gdim[a] = want;
+                    ginv[a] = (float)want / ext;
+                }
+            }
+        }
+        const uint32_t ncells = gdim[0] * gdim[1] * gdim[2];
+        std::vector<std::vector<uint32_t>> cell_lists(ncells);
+        for (int oi = (int)objs.size() - 1; oi >= 0 && any_obj; oi--) {
+            if (s->trees[oi + 1].hdr.root_ref == WR_NONE) continue;
+            uint32_t lo[3], hi[3];
+            for (int a = 0; a < 3; a++) {
+                lo[a] = wr_grid_cell(objs[oi].aabb_min[a], gmin[a], ginv[a], gdim[a]);
+                hi[a] = wr_grid_cell(objs[oi].aabb_max[a], gmin[a], ginv[a], gdim[a]);
+            }
+            for (uint32_t z = lo[2]; z <= hi[2]; z++)
+                for (uint32_t y = lo[1]; y <= hi[1]; y++)
+                    for (uint32_t x = lo[0]; x <= hi[0]; x++)
+                        cell_lists[(z * gdim[1] + y) * gdim[0] + x].push_back((uint32_t)oi);
+        }
+        std::vector<uint32_t> cell_start(ncells + 1, 0), cell_objs;
+        for (uint32_t c = 0; c < ncells; c++) {
+            cell_start[c] = (uint32_t)cell_objs.size();
+            cell_objs.insert(cell_objs.end(), cell_lists[c].begin(), cell_lists[c].end());
+        }
+        cell_start[ncells] = (uint32_t)cell_objs.size();
+
         uint64_t bytes = 0;
+        uint32_t *d_cs, *d_co;
         wr_node* d_nodes;
         wr_tri* d_tris;
         uint32_t *d_lu, *d_lc, *d_nu, *d_rp, *d_ro, *d_hn;
@@ -313,6 +358,10 @@ int wr_scene_build(wr_scene* s, int strategy) {
         s->dbufs.push_back(d_o);
         if ((rc = upload(ctx, has_n, &d_hn, bytes))) return rc;
         s->dbufs.push_back(d_hn);
+        if ((rc = upload(ctx, cell_start, &d_cs, bytes))) return rc;
+        s->dbufs.push_back(d_cs);
+        if ((rc = upload(ctx, cell_objs, &d_co, bytes))) return rc;
+        s->dbufs.push_back(d_co);
         SceneDev& D = s->dev;
         D.nodes = d_nodes;
         D.tris = d_tris;
@@ -327,6 +376,14 @@ int wr_scene_build(wr_scene* s, int strategy) {
         D.obj_has_normals = d_hn;
         D.n_objs = (uint32_t)s->objs.size();
         D.max_obj_depth = max_obj_depth;
+        D.cell_start = d_cs;
+        D.cell_objs = d_co;
+        for (int a = 0; a < 3; a++) {
+            D.grid_min[a] = gmin[a];
+            D.grid_max[a] = gmax[a];
+            D.grid_inv[a] = ginv[a];
+            D.grid_dim[a] = gdim[a];
+        }
         s->hdrs = hdrs;
         s->host_objs = objs;
         s->device_bytes = bytes;

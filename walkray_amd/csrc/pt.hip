@@ -35,183 +35,214 @@ __global__ __launch_bounds__(256) void k_pt_mega(SceneDev S, CamDev cam, uint32_
 }
 
 // ---- wavefront --------------------------------------------------------------
-struct WaveDev {
-    // per-slot path state (SoA); pass is relative to the render call's first pass
+// Live paths are kept dense: every stage reads path i of the current buffer
+// (coalesced SoA loads) and the shade stage appends survivors, then the
+// generate stage appends new camera paths, to the other buffer with one atomic
+// per wave.  Primary (camera) rays are traced by the extend stage like any
+// other ray.  Finished paths write their sample to a per-(pass, pixel) slot;
+// the fold stage adds complete passes to the Kahan film in pass order.
+constexpr uint32_t kPrimary = 1u;
+constexpr uint32_t kDead = 2u;
+
+struct PathBuf {
     float *ox, *oy, *oz, *dx, *dy, *dz;
-    float *tx, *ty, *tz;
-    uint32_t *pixel, *pass, *depth, *inside, *contact;
-    float* ht;
-    uint32_t *hrec, *hobj;
-    // queues
-    uint32_t* q_shade[2];
-    uint32_t* q_extend;
-    uint32_t* q_free;  // ring of capacity nslots
-    wr::WaveCounters* cnt;
-    // per-(pass window slot, pixel) samples
-    float* contrib;   // R * npix * 3
-    uint32_t nslots, npix, R;
+    float *tx, *ty, *tz, *ht;
+    uint32_t *pixel, *pass, *depth, *inside, *contact, *hrec, *hobj, *flags;
 };
 
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter) {
-    // one atomic per wave: lanes that append get consecutive indices
+struct WaveDev {
+    PathBuf buf[2];
+    wr::WaveCounters* cnt;
+    float* contrib;  // R * npix * 3: samples of the passes in flight
+    uint32_t cap, npix, R, npass;
+    uint64_t total;
+};
+
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, uint32_t amount = 1) {
+    // one atomic per wave; lanes get consecutive slots in lane order
     const uint64_t mask = __ballot(1);
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
     const uint32_t rank = __popcll(mask & ((1ull << lane) - 1));
     uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask) * amount);
     base = __shfl(base, leader);
-    return base + rank;
+    return base + rank * amount;
 }
 
-__device__ __forceinline__ void finish_path(const WaveDev& W, const CamDev& cam, uint32_t slot, f3 color) {
+// pass_done[key] += 1 for every active lane, with one atomic per distinct key in
+// the wave (the lanes of a wave almost always finish paths of one pass).
+__device__ __forceinline__ void count_done(uint32_t* pass_done, uint32_t key) {
+    for (;;) {
+        const uint64_t act = __ballot(1);
+        const uint32_t leader = __ffsll((unsigned long long)act) - 1;
+        const uint32_t k = __shfl(key, leader);
+        const uint64_t same = __ballot(key == k);
+        if (key == k) {
+            if ((threadIdx.x & 63) == leader) atomicAdd(&pass_done[k], (uint32_t)__popcll(same));
+            return;
+        }
+    }
+}
+
+__device__ __forceinline__ void finish(const WaveDev& W, const CamDev& cam, uint32_t p, uint32_t ps, f3 color) {
     if (!legal(color)) color = mk(0.f, 0.f, 0.f);
-    const uint32_t p = W.pixel[slot], ps = W.pass[slot];
     const f3 v = fix_vignetting(cam, color, p);
-    float* c = W.contrib + 3 * ((size_t)(ps % W.R) * W.npix + p);
+    const uint32_t key = ps % W.R;
+    float* c = W.contrib + 3 * ((size_t)key * W.npix + p);
     c[0] = v.x;
     c[1] = v.y;
     c[2] = v.z;
-    atomicAdd(&W.cnt->pass_done[ps % W.R], 1u);
-    atomicAdd(&W.cnt->finished, 1u);
-    const uint32_t at = wave_append(&W.cnt->free_tail);
-    W.q_free[at % W.nslots] = slot;
+    count_done(W.cnt->pass_done, key);
 }
 
-// Iteration prologue (one lane): reset the queues the iteration appends to and
-// decide how many new paths the generate stage starts.
-__global__ void k_wf_plan(WaveDev W, uint32_t parity, uint64_t work_limit) {
-    wr::WaveCounters& c = *W.cnt;
-    c.n_extend = 0;
-    c.n_shade[parity ^ 1] = 0;
-    const uint32_t free_n = c.free_tail - c.free_head;
-    const uint64_t remaining = work_limit > c.next_work ? work_limit - c.next_work : 0;
-    const uint32_t take = (uint32_t)(remaining < free_n ? remaining : free_n);
-    c.take = take;
-    c.take_head = c.free_head;
-    c.take_work = c.next_work;
-    c.free_head += take;
-    c.next_work += take;
-}
-
-__global__ __launch_bounds__(256) void k_wf_generate(SceneDev S, WaveDev W, CamDev cam, uint64_t seed,
-                                                     uint32_t pass0, uint32_t parity) {
-    const uint32_t take = W.cnt->take, head = W.cnt->take_head;
-    const uint64_t work0 = W.cnt->take_work;
+__global__ __launch_bounds__(256) void k_wf_extend(SceneDev S, WaveDev W, CamDev cam, uint32_t par) {
+    const PathBuf& A = W.buf[par];
+    const uint32_t n = W.cnt->n[par];
     LaneStack st;
     st.lane = threadIdx.x;
     const SceneCtx sc = make_scene_ctx(S);
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < take; i += gridDim.x * 256) {
-        const uint32_t slot = W.q_free[(head + i) % W.nslots];
-        const uint64_t w = work0 + i;
-        const uint32_t ps = (uint32_t)(w / W.npix), p = (uint32_t)(w % W.npix);
-        W.pixel[slot] = p;
-        W.pass[slot] = ps;
-        float rn[4];
-        rng4(seed, p, pass0 + ps, 0, rn);
-        const f3 o = cam.pos;
-        const f3 d = cam_dir(cam, p, rn[0], rn[1]);
-        const PHit hit = trace_scene<false>(sc, o, d, WR_NONE, WR_NONE, st);
-        const uint32_t inside = find_inside(S, o, d, WR_NONE, st);
-        float c0 = fabsf(dot(cam.front, d));
-        c0 = (double)c0 > kEps ? c0 : (float)kEps;
-        const uint32_t self = inside != WR_NONE ? inside : hit.obj;
-        if (self == WR_NONE || hit.rec == WR_NONE) {
-            finish_path(W, cam, slot, mk(0.f, 0.f, 0.f));
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const f3 o = mk(A.ox[i], A.oy[i], A.oz[i]);
+        const f3 d = mk(A.dx[i], A.dy[i], A.dz[i]);
+        const uint32_t fl = A.flags[i];
+        PHit h;
+        if (fl & kPrimary) {  // camera ray: Scene::intersect without Condition + findInsideObject (camera.cpp:52-56)
+            h = trace_scene<false>(sc, o, d, WR_NONE, WR_NONE, st);
+            A.inside[i] = find_inside(S, o, d, WR_NONE, st);
+        } else {
+            h = trace_scene<true>(sc, o, d, A.inside[i], A.contact[i], st);
+        }
+        if (h.rec == WR_NONE) {  // escaped (no environment light) or no object to scatter: sample 0
+            finish(W, cam, A.pixel[i], A.pass[i], mk(0.f, 0.f, 0.f));
+            A.flags[i] = kDead;
             continue;
         }
-        W.ox[slot] = o.x; W.oy[slot] = o.y; W.oz[slot] = o.z;
-        W.dx[slot] = d.x; W.dy[slot] = d.y; W.dz[slot] = d.z;
-        W.tx[slot] = c0; W.ty[slot] = c0; W.tz[slot] = c0;
-        W.depth[slot] = 0;
-        W.inside[slot] = inside;
-        W.ht[slot] = hit.t;
-        W.hrec[slot] = hit.rec;
-        W.hobj[slot] = hit.obj;
-        const uint32_t at = wave_append(&W.cnt->n_shade[parity]);
-        W.q_shade[parity][at] = slot;
+        A.ht[i] = h.t;
+        A.hrec[i] = h.rec;
+        A.hobj[i] = h.obj;
+        A.flags[i] = 0;
     }
 }
 
 __global__ __launch_bounds__(256) void k_wf_shade(SceneDev S, WaveDev W, CamDev cam, uint32_t max_depth,
-                                                  uint64_t seed, uint32_t pass0, uint32_t parity) {
-    const uint32_t n = W.cnt->n_shade[parity];
+                                                  uint64_t seed, uint32_t pass0, uint32_t par) {
+    const PathBuf& A = W.buf[par];
+    const PathBuf& B = W.buf[par ^ 1];
+    const uint32_t n = W.cnt->n[par];
     LaneStack st;
     st.lane = threadIdx.x;
     const SceneCtx sc = make_scene_ctx(S);
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const uint32_t slot = W.q_shade[parity][i];
-        const f3 o = mk(W.ox[slot], W.oy[slot], W.oz[slot]);
-        const f3 d = mk(W.dx[slot], W.dy[slot], W.dz[slot]);
+        if (A.flags[i] & kDead) continue;
+        const f3 o = mk(A.ox[i], A.oy[i], A.oz[i]);
+        const f3 d = mk(A.dx[i], A.dy[i], A.dz[i]);
         PHit h;
-        h.t = W.ht[slot];
-        h.rec = W.hrec[slot];
-        h.obj = W.hobj[slot];
-        const uint32_t inside = W.inside[slot];
-        const uint32_t depth = W.depth[slot];
+        h.t = A.ht[i];
+        h.rec = A.hrec[i];
+        h.obj = A.hobj[i];
+        const uint32_t inside = A.inside[i], depth = A.depth[i], p = A.pixel[i], ps = A.pass[i];
         const uint32_t self = inside != WR_NONE ? inside : h.obj;
         float rn[4];
-        rng4(seed, W.pixel[slot], pass0 + W.pass[slot], depth + 1, rn);
+        rng4(seed, p, pass0 + ps, depth + 1, rn);
         const Out out = scatter(sc, self, o, d, h, inside, rn, st);
-        f3 T = mk(W.tx[slot], W.ty[slot], W.tz[slot]);
-        if ((double)len(out.d) < 0.5) {
-            finish_path(W, cam, slot, mulv(T, divs(divs(out.rad, out.op), out.dp)));
+        f3 T = mk(A.tx[i], A.ty[i], A.tz[i]);
+        if ((double)len(out.d) < 0.5) {  // emission / absorption ends the path
+            finish(W, cam, p, ps, mulv(T, divs(divs(out.rad, out.op), out.dp)));
             continue;
         }
         if (depth + 1 > max_depth) {
-            finish_path(W, cam, slot, mk(0.f, 0.f, 0.f));
+            finish(W, cam, p, ps, mk(0.f, 0.f, 0.f));
             continue;
         }
         // the weight of this vertex only matters if the next ray hits; applying it
-        // now keeps the same multiplication order as the per-pixel loop
+        // now keeps the per-pixel loop's multiplication order
         T = muls(T, cosine_term(out));
         T = mulv(T, divs(divs(out.rad, out.op), out.dp));
-        W.tx[slot] = T.x; W.ty[slot] = T.y; W.tz[slot] = T.z;
-        W.ox[slot] = out.o.x; W.oy[slot] = out.o.y; W.oz[slot] = out.o.z;
-        W.dx[slot] = out.d.x; W.dy[slot] = out.d.y; W.dz[slot] = out.d.z;
-        W.inside[slot] = out.inside;
-        W.contact[slot] = out.contact;
-        const uint32_t at = wave_append(&W.cnt->n_extend);
-        W.q_extend[at] = slot;
+        const uint32_t j = wave_append(&W.cnt->n[par ^ 1]);
+        B.ox[j] = out.o.x; B.oy[j] = out.o.y; B.oz[j] = out.o.z;
+        B.dx[j] = out.d.x; B.dy[j] = out.d.y; B.dz[j] = out.d.z;
+        B.tx[j] = T.x; B.ty[j] = T.y; B.tz[j] = T.z;
+        B.pixel[j] = p;
+        B.pass[j] = ps;
+        B.depth[j] = depth + 1;
+        B.inside[j] = out.inside;
+        B.contact[j] = out.contact;
+        B.flags[j] = 0;
     }
 }
 
-__global__ __launch_bounds__(256) void k_wf_extend(SceneDev S, WaveDev W, CamDev cam, uint32_t parity) {
-    const uint32_t n = W.cnt->n_extend;
-    LaneStack st;
-    st.lane = threadIdx.x;
-    const SceneCtx sc = make_scene_ctx(S);
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const uint32_t slot = W.q_extend[i];
-        const f3 o = mk(W.ox[slot], W.oy[slot], W.oz[slot]);
-        const f3 d = mk(W.dx[slot], W.dy[slot], W.dz[slot]);
-        const PHit h = trace_scene<true>(sc, o, d, W.inside[slot], W.contact[slot], st);
-        if (h.rec == WR_NONE) {
-            finish_path(W, cam, slot, mk(0.f, 0.f, 0.f));
-            continue;
-        }
-        W.ht[slot] = h.t;
-        W.hrec[slot] = h.rec;
-        W.hobj[slot] = h.obj;
-        W.depth[slot] = W.depth[slot] + 1;
-        const uint32_t at = wave_append(&W.cnt->n_shade[parity ^ 1]);
-        W.q_shade[parity ^ 1][at] = slot;
+// One lane: how many new camera paths fit behind the survivors, inside the
+// window of passes whose sample slots are free.
+__global__ void k_wf_plan(WaveDev W, uint32_t par) {
+    wr::WaveCounters& c = *W.cnt;
+    const uint32_t nb = c.n[par ^ 1];
+    const uint64_t lim_w = (uint64_t)(c.cursor + W.R) * W.npix;
+    const uint64_t limit = lim_w < W.total ? lim_w : W.total;
+    const uint64_t remaining = limit > c.next_work ? limit - c.next_work : 0;
+    const uint32_t room = W.cap - nb;
+    const uint32_t take = (uint32_t)(remaining < room ? remaining : room);
+    c.take = take;
+    c.take_base = nb;
+    c.take_work = c.next_work;
+    c.n[par ^ 1] = nb + take;
+    c.next_work += take;
+}
+
+__global__ __launch_bounds__(256) void k_wf_generate(WaveDev W, CamDev cam, uint64_t seed, uint32_t pass0,
+                                                     uint32_t par) {
+    const PathBuf& B = W.buf[par ^ 1];
+    const uint32_t take = W.cnt->take, base = W.cnt->take_base;
+    const uint64_t work0 = W.cnt->take_work;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < take; i += gridDim.x * 256) {
+        const uint64_t w = work0 + i;
+        const uint32_t ps = (uint32_t)(w / W.npix), p = (uint32_t)(w % W.npix);
+        float rn[4];
+        rng4(seed, p, pass0 + ps, 0, rn);
+        const f3 d = cam_dir(cam, p, rn[0], rn[1]);
+        float c0 = fabsf(dot(cam.front, d));  // camera cosineTerm (ray.cpp:52-58, camera.cpp:182-186)
+        c0 = (double)c0 > kEps ? c0 : (float)kEps;
+        const uint32_t j = base + i;
+        B.ox[j] = cam.pos.x; B.oy[j] = cam.pos.y; B.oz[j] = cam.pos.z;
+        B.dx[j] = d.x; B.dy[j] = d.y; B.dz[j] = d.z;
+        B.tx[j] = c0; B.ty[j] = c0; B.tz[j] = c0;
+        B.pixel[j] = p;
+        B.pass[j] = ps;
+        B.depth[j] = 0;
+        B.inside[j] = WR_NONE;
+        B.contact[j] = WR_NONE;
+        B.flags[j] = kPrimary;
     }
 }
 
-// Adds one complete pass to the film (pixel-parallel, Kahan), then re-arms its window slot.
-__global__ __launch_bounds__(256) void k_wf_fold(WaveDev W, FilmDev film, uint32_t pass) {
+// Number of consecutive complete passes starting at the fold cursor.
+__device__ __forceinline__ uint32_t foldable(const WaveDev& W) {
+    const wr::WaveCounters& c = *W.cnt;
+    uint32_t k = 0;
+    while (k < W.R && c.cursor + k < W.npass && c.pass_done[(c.cursor + k) % W.R] == W.npix) k++;
+    return k;
+}
+
+// Adds every complete pass (in pass order) to the film; pixel-parallel.
+__global__ __launch_bounds__(256) void k_wf_fold(WaveDev W, FilmDev film) {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t k = foldable(W);
     if (p >= W.npix) return;
-    const float* c = W.contrib + 3 * ((size_t)(pass % W.R) * W.npix + p);
-    film_add(film, p, mk(c[0], c[1], c[2]));
+    const uint32_t cur = W.cnt->cursor;
+    for (uint32_t j = 0; j < k; j++) {
+        const float* c = W.contrib + 3 * ((size_t)((cur + j) % W.R) * W.npix + p);
+        film_add(film, p, mk(c[0], c[1], c[2]));
+    }
 }
 
-__global__ void k_wf_rearm(WaveDev W, uint32_t pass) { W.cnt->pass_done[pass % W.R] = 0; }
-
-__global__ void k_wf_init_free(WaveDev W) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i < W.nslots) W.q_free[i] = i;
+// One lane: advance the fold cursor, re-arm folded pass slots, empty the consumed buffer.
+__global__ void k_wf_advance(WaveDev W, uint32_t par) {
+    const uint32_t k = foldable(W);
+    wr::WaveCounters& c = *W.cnt;
+    for (uint32_t j = 0; j < k; j++) c.pass_done[(c.cursor + j) % W.R] = 0;
+    c.cursor += k;
+    c.n[par] = 0;
+    c.iterations++;
 }
 
 }  // namespace wrd
@@ -242,10 +273,10 @@ hipError_t launch_pt_mega(const SceneDev& S, const CamHost& cam, uint32_t max_de
     return hipGetLastError();
 }
 
-#define WR_CHECK(x)                          \
-    do {                                     \
-        hipError_t e_ = (x);                 \
-        if (e_ != hipSuccess) return e_;     \
+#define WR_CHECK(x)                      \
+    do {                                 \
+        hipError_t e_ = (x);             \
+        if (e_ != hipSuccess) return e_; \
     } while (0)
 
 WavefrontPool::~WavefrontPool() { release(); }
@@ -254,96 +285,109 @@ void WavefrontPool::release() {
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     bufs.clear();
-    nslots = npix = R = 0;
+    cap = contrib_floats = 0;
 }
 
-hipError_t WavefrontPool::ensure(uint32_t want_slots, uint32_t want_npix, uint32_t want_R) {
-    if (nslots >= want_slots && npix == want_npix && R == want_R && !bufs.empty()) return hipSuccess;
+hipError_t WavefrontPool::ensure(uint32_t want_cap, size_t want_contrib_floats) {
+    if (cap >= want_cap && contrib_floats >= want_contrib_floats && !bufs.empty()) return hipSuccess;
     release();
-    nslots = want_slots;
-    npix = want_npix;
-    R = want_R;
+    cap = want_cap;
+    contrib_floats = want_contrib_floats;
     auto alloc = [&](size_t bytes) -> void* {
         void* p = nullptr;
         if (hipMalloc(&p, std::max<size_t>(bytes, 4)) != hipSuccess) return nullptr;
         bufs.push_back(p);
         return p;
     };
-    const size_t f4 = sizeof(float) * (size_t)nslots, u4 = sizeof(uint32_t) * (size_t)nslots;
-    float** fp[] = {&ox, &oy, &oz, &dx, &dy, &dz, &tx, &ty, &tz, &ht};
-    for (float** p : fp)
-        if (!(*p = (float*)alloc(f4))) return hipErrorOutOfMemory;
-    uint32_t** up[] = {&pixel, &pass, &depth, &inside, &contact, &hrec, &hobj, &q_shade0, &q_shade1, &q_extend, &q_free};
-    for (uint32_t** p : up)
-        if (!(*p = (uint32_t*)alloc(u4))) return hipErrorOutOfMemory;
+    const size_t w4 = 4 * (size_t)cap;
+    for (int b = 0; b < 2; b++) {
+        for (int k = 0; k < kPathWords; k++)
+            if (!(words[b][k] = alloc(w4))) return hipErrorOutOfMemory;
+    }
     if (!(cnt = (WaveCounters*)alloc(sizeof(WaveCounters)))) return hipErrorOutOfMemory;
-    if (!(contrib = (float*)alloc(sizeof(float) * 3 * (size_t)R * npix))) return hipErrorOutOfMemory;
+    if (!(contrib = (float*)alloc(sizeof(float) * contrib_floats))) return hipErrorOutOfMemory;
     return hipSuccess;
 }
 
-static WaveDev to_wave(const WavefrontPool& P) {
-    WaveDev W;
-    W.ox = P.ox; W.oy = P.oy; W.oz = P.oz;
-    W.dx = P.dx; W.dy = P.dy; W.dz = P.dz;
-    W.tx = P.tx; W.ty = P.ty; W.tz = P.tz;
-    W.pixel = P.pixel; W.pass = P.pass; W.depth = P.depth; W.inside = P.inside; W.contact = P.contact;
-    W.ht = P.ht; W.hrec = P.hrec; W.hobj = P.hobj;
-    W.q_shade[0] = P.q_shade0; W.q_shade[1] = P.q_shade1;
-    W.q_extend = P.q_extend; W.q_free = P.q_free;
-    W.cnt = P.cnt; W.contrib = P.contrib;
-    W.nslots = P.nslots; W.npix = P.npix; W.R = P.R;
-    return W;
+static PathBuf to_pathbuf(void* const* w) {
+    PathBuf b;
+    float** f[] = {&b.ox, &b.oy, &b.oz, &b.dx, &b.dy, &b.dz, &b.tx, &b.ty, &b.tz, &b.ht};
+    uint32_t** u[] = {&b.pixel, &b.pass, &b.depth, &b.inside, &b.contact, &b.hrec, &b.hobj, &b.flags};
+    int k = 0;
+    for (float** p : f) *p = (float*)w[k++];
+    for (uint32_t** p : u) *p = (uint32_t*)w[k++];
+    return b;
 }
 
-// Renders passes [pass0, pass0 + npass) with the wavefront stages.
+// Renders passes [pass0, pass0 + npass) of the film with the wavefront stages.
+// Iterations run as replays of a captured graph of `kGraphIters` iterations;
+// the host reads one counter per replay to detect completion.
 hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
-                         uint32_t npass, const FilmBufs& film, WavefrontPool& P, uint32_t slots, uint32_t grid,
+                         uint32_t npass, const FilmBufs& film, WavefrontPool& P, uint32_t cap, uint32_t grid,
                          WaveStats* stats, hipStream_t st) {
     const uint32_t npix = cam.W * cam.H;
-    const uint32_t R = 4;
     if (npass == 0 || npix == 0) return hipSuccess;
     const uint64_t total = (uint64_t)npass * npix;
-    if (slots == 0) slots = (uint32_t)std::min<uint64_t>(total, 1u << 21);
-    WR_CHECK(P.ensure(slots, npix, R));
-    WaveDev W = to_wave(P);
-    W.nslots = slots;
-    CamDev C = to_cam(cam);
-    WaveCounters init{};
-    init.free_head = 0;
-    init.free_tail = slots;
-    init.next_work = 0;
-    WR_CHECK(hipMemcpyAsync(P.cnt, &init, sizeof(init), hipMemcpyHostToDevice, st));
-    k_wf_init_free<<<(slots + 255) / 256, 256, 0, st>>>(W);
-    WR_CHECK(hipGetLastError());
-    // work items are numbered pass-major from 0; passes are relative to pass0
-    uint32_t folded = 0;  // passes (relative) already folded into the film
-    WaveCounters host{};
-    uint32_t parity = 0;
-    uint64_t iters = 0;
+    if (cap == 0) cap = (uint32_t)std::min<uint64_t>(total, 1u << 22);
+    cap = std::max<uint32_t>(cap, 256);
+    uint32_t R = (uint32_t)std::min<uint64_t>(npass, cap / npix + 2);
+    R = std::min<uint32_t>(R, kMaxPassWindow);
+    WR_CHECK(P.ensure(cap, (size_t)3 * R * npix));
+    WaveDev W;
+    W.buf[0] = to_pathbuf(P.words[0]);
+    W.buf[1] = to_pathbuf(P.words[1]);
+    W.cnt = P.cnt;
+    W.contrib = P.contrib;
+    W.cap = cap;
+    W.npix = npix;
+    W.R = R;
+    W.npass = npass;
+    W.total = total;
+    const CamDev C = to_cam(cam);
+    const FilmDev F = to_film(film);
     if (grid == 0) grid = 2048;
-    while (folded < npass) {
-        const uint64_t limit = std::min<uint64_t>(total, (uint64_t)(folded + R) * npix);
-        k_wf_plan<<<1, 1, 0, st>>>(W, parity, limit);
-        k_wf_generate<<<grid, 256, kStackLdsBytes, st>>>(S, W, C, seed, pass0, parity);
-        k_wf_shade<<<grid, 256, kStackLdsBytes, st>>>(S, W, C, max_depth, seed, pass0, parity);
-        k_wf_extend<<<grid, 256, kStackLdsBytes, st>>>(S, W, C, parity);
-        WR_CHECK(hipGetLastError());
-        WR_CHECK(hipMemcpyAsync(&host, P.cnt, sizeof(host), hipMemcpyDeviceToHost, st));
-        WR_CHECK(hipStreamSynchronize(st));
-        iters++;
-        parity ^= 1;
-        // fold every pass whose samples are all written, in pass order
-        while (folded < npass && host.pass_done[folded % R] == npix) {
-            k_wf_fold<<<(npix + 255) / 256, 256, 0, st>>>(W, FilmDev{film.sum, film.comp, film.sumsq}, folded);
-            k_wf_rearm<<<1, 1, 0, st>>>(W, folded);
-            WR_CHECK(hipGetLastError());
-            host.pass_done[folded % R] = 0;
-            folded++;
+    WaveCounters init{};
+    WR_CHECK(hipMemcpyAsync(P.cnt, &init, sizeof(init), hipMemcpyHostToDevice, st));
+    // capture kGraphIters iterations (an even number: buffer parity alternates)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    WR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int it = 0; it < kGraphIters; it++) {
+        const uint32_t par = (uint32_t)(it & 1);
+        k_wf_extend<<<grid, 256, kStackLdsBytes, st>>>(S, W, C, par);
+        k_wf_shade<<<grid, 256, kStackLdsBytes, st>>>(S, W, C, max_depth, seed, pass0, par);
+        k_wf_plan<<<1, 1, 0, st>>>(W, par);
+        k_wf_generate<<<grid, 256, 0, st>>>(W, C, seed, pass0, par);
+        k_wf_fold<<<(npix + 255) / 256, 256, 0, st>>>(W, F);
+        k_wf_advance<<<1, 1, 0, st>>>(W, par);
+    }
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (e != hipSuccess) return e;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return e;
+    }
+    WaveCounters host{};
+    uint64_t replays = 0;
+    for (;;) {
+        e = hipGraphLaunch(exec, st);
+        if (e != hipSuccess) break;
+        replays++;
+        e = hipMemcpyAsync(&host, P.cnt, sizeof(host), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess || host.cursor >= npass) break;
+        if (replays > 4 * (uint64_t)(max_depth + 3) * (total / cap + 2)) {  // cannot happen: every iteration advances every path
+            e = hipErrorUnknown;
+            break;
         }
     }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return e;
     if (stats) {
-        stats->iterations = iters;
-        stats->paths = host.finished;
+        stats->iterations = host.iterations;
+        stats->paths = total;
     }
     return hipSuccess;
 }

@@ -406,8 +406,17 @@ __device__ __forceinline__ uint32_t find_inside(const SceneDev& S, const f3& o, 
     NoSelf ns{};
     Counters cnt{0, 0, 0};
     const f3 db = mk(-d.x, -d.y, -d.z);
-    for (int i = (int)S.n_objs - 1; i >= 0; i--) {
-        if ((uint32_t)i == cur) continue;
+    if (!(S.grid_min[0] <= o.x && o.x <= S.grid_max[0] && S.grid_min[1] <= o.y && o.y <= S.grid_max[1] &&
+          S.grid_min[2] <= o.z && o.z <= S.grid_max[2]))
+        return WR_NONE;  // outside every object's box
+    const uint32_t cx = wr_grid_cell(o.x, S.grid_min[0], S.grid_inv[0], S.grid_dim[0]);
+    const uint32_t cy = wr_grid_cell(o.y, S.grid_min[1], S.grid_inv[1], S.grid_dim[1]);
+    const uint32_t cz = wr_grid_cell(o.z, S.grid_min[2], S.grid_inv[2], S.grid_dim[2]);
+    const uint32_t cell = (cz * S.grid_dim[1] + cy) * S.grid_dim[0] + cx;
+    const uint32_t c_end = S.cell_start[cell + 1];
+    for (uint32_t k = S.cell_start[cell]; k < c_end; k++) {  // descending object ids (scene.cpp:121)
+        const uint32_t i = S.cell_objs[k];
+        if (i == cur) continue;
         const wr_object_gpu& ob = S.objs[i];
         if (!(ob.aabb_min[0] <= o.x && o.x <= ob.aabb_max[0] && ob.aabb_min[1] <= o.y && o.y <= ob.aabb_max[1] &&
               ob.aabb_min[2] <= o.z && o.z <= ob.aabb_max[2]))

@@ -21,7 +21,26 @@ struct SceneDev {
     const uint32_t* obj_has_normals;
     uint32_t n_objs;
     uint32_t max_obj_depth;
+    // object-candidate grid for findInsideObject: cell c lists, in descending
+    // object order, every object whose root box overlaps the cell
+    const uint32_t* cell_start;  // ncells + 1
+    const uint32_t* cell_objs;
+    float grid_min[3], grid_max[3], grid_inv[3];
+    uint32_t grid_dim[3];
 };
+
+// Cell coordinate of x on one grid axis: monotone in x, so a point inside a
+// closed box maps into the box's cell range (same arithmetic on host and GPU).
+#if defined(__HIPCC__)
+__host__ __device__
+#endif
+inline uint32_t wr_grid_cell(float x, float gmin, float inv, uint32_t dim) {
+    const float f = (x - gmin) * inv;
+    int c = (int)floorf(f);
+    if (c < 0) c = 0;
+    if (c > (int)dim - 1) c = (int)dim - 1;
+    return (uint32_t)c;
+}
 
 namespace wr {
 hipError_t launch_closest_hit(const SceneDev& S, const wr_tree_hdr& H, const float* rays, uint64_t n, void* hits,

@@ -24,15 +24,18 @@ struct FilmBufs {
     float* sumsq;
 };
 
-// Device counters of one wavefront run (all 32-bit but next_work / take_work).
+constexpr int kPathWords = 18;       // 32-bit words of path state per slot (SoA arrays)
+constexpr uint32_t kMaxPassWindow = 64;  // passes whose samples may be in flight at once
+constexpr int kGraphIters = 8;       // wavefront iterations per captured graph (even)
+
+// Device counters of one wavefront run.
 struct WaveCounters {
-    uint32_t n_shade[2];
-    uint32_t n_extend;
-    uint32_t free_head, free_tail;
-    uint32_t take, take_head;
-    uint32_t finished;
+    uint32_t n[2];       // live paths in path buffer 0 / 1
+    uint32_t take, take_base;
     uint64_t next_work, take_work;
-    uint32_t pass_done[4];
+    uint32_t cursor;     // passes folded into the film
+    uint32_t iterations;
+    uint32_t pass_done[kMaxPassWindow];
 };
 
 struct WaveStats {
@@ -40,20 +43,18 @@ struct WaveStats {
     uint64_t paths = 0;
 };
 
-// Path-slot pool, reused across render calls of one context.
+// Two path buffers (ping-pong), counters and the per-(pass, pixel) sample
+// window; reused across render calls of one context.
 struct WavefrontPool {
-    float *ox = nullptr, *oy = nullptr, *oz = nullptr, *dx = nullptr, *dy = nullptr, *dz = nullptr;
-    float *tx = nullptr, *ty = nullptr, *tz = nullptr, *ht = nullptr;
-    uint32_t *pixel = nullptr, *pass = nullptr, *depth = nullptr, *inside = nullptr, *contact = nullptr;
-    uint32_t *hrec = nullptr, *hobj = nullptr, *q_shade0 = nullptr, *q_shade1 = nullptr, *q_extend = nullptr,
-             *q_free = nullptr;
+    void* words[2][kPathWords] = {};
     WaveCounters* cnt = nullptr;
     float* contrib = nullptr;
-    uint32_t nslots = 0, npix = 0, R = 0;
+    uint32_t cap = 0;
+    size_t contrib_floats = 0;
     std::vector<void*> bufs;
     ~WavefrontPool();
     void release();
-    hipError_t ensure(uint32_t slots, uint32_t npix, uint32_t R);
+    hipError_t ensure(uint32_t cap, size_t contrib_floats);
 };
 
 hipError_t launch_pt_mega(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass,

@@ -207,6 +207,44 @@ def cornell(k=5, width=256, height=256):
     return objs, cam
 
 
+def random_field(n_objects=512, seed=3, width=1920, height=1080, sphere_k=3, sphere_frac=0.75):
+    """Wavefront workload: seeded icospheres (k=3, 1,280 triangles) and boxes on a ground plane.
+
+    Objects sit on a 32 x 16 jittered grid of unit cells (no overlaps); materials are drawn
+    uniformly from diffuse (albedo U[0.1, 0.9] per channel), isotropic Phong (albedo
+    U[0.1, 0.9], exponent log-uniform in [10, 1000]) and refractive (IOR U[1.33, 1.7]);
+    two 6 x 6 area lights of radiance 10 face down from y = 6.
+    """
+    rng = np.random.default_rng(seed)
+    objs = [quad((0.0, 0.0, 0.0), (40.0, 0, 0), (0, 0, 40.0), (0, 1, 0), DIFFUSE, (0.5, 0.5, 0.5))]
+    cols, rows = 32, 16
+    cells = rng.permutation(cols * rows)[:n_objects]
+    for c in cells:
+        cx = -cols / 2 + (c % cols) + 0.5
+        cz = -rows / 2 + (c // cols) + 0.5
+        kind = rng.integers(3)
+        if kind == 0:
+            mat, params = DIFFUSE, tuple(rng.uniform(0.1, 0.9, 3))
+        elif kind == 1:
+            mat, params = GLOSSY, tuple(rng.uniform(0.1, 0.9, 3)) + (float(np.exp(rng.uniform(np.log(10), np.log(1000)))),)
+        else:
+            mat, params = REFRACTIVE, (float(rng.uniform(1.33, 1.7)),)
+        jx, jz = rng.uniform(-0.05, 0.05, 2)
+        if rng.random() < sphere_frac:
+            r = float(rng.uniform(0.2, 0.42))
+            objs.append(icosphere(sphere_k, (cx + jx, r + 1e-3, cz + jz), r, mat, params))
+        else:
+            h = rng.uniform(0.15, 0.4, 3)
+            lo = (cx + jx - h[0], 1e-3, cz + jz - h[2])
+            hi = (cx + jx + h[0], 1e-3 + 2 * h[1], cz + jz + h[2])
+            objs.append(box(lo, hi, mat, params))
+    for x in (-6.0, 6.0):
+        objs.append(quad((x, 6.0, 0.0), (6.0, 0, 0), (0, 0, 6.0), (0, -1, 0), EMISSIVE, (10.0, 10.0, 10.0)))
+    cam = camera((0.0, 5.0, 18.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), width, height,
+                 sight_dist=960.0 / math.tan(math.radians(30.0)) * width / 1920.0)
+    return objs, cam
+
+
 def heightfield_scene(n=1024, seed=7):
     """Ray-query workload: one fBm heightfield of 2 (n-1)^2 triangles."""
     return [heightfield(n, seed=seed)]
