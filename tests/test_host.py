@@ -1,0 +1,106 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports every
+declared symbol, the product builder equals the oracle's builder node for node,
+and the procedural scene generators produce the sizes the workloads state."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from walkray_amd import _lib as W
+from walkray_amd import scenes as S
+
+
+def test_library_exports_every_header_symbol():
+    L = ctypes.CDLL(W.LIB_PATH)
+    names = W.header_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    W.lib()
+    import inspect
+    src = inspect.getsource(W.lib)
+    for n in W.header_symbols():
+        assert f'"{n}"' in src, f"{n} has no ctypes signature"
+
+
+def test_no_gpu_errors_are_reported_not_crashed():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    with pytest.raises(W.WalkRayError):
+        W.Context(0)
+
+
+def _soup(objs):
+    V = np.concatenate([o["verts"] for o in objs])
+    T, off = [], 0
+    for o in objs:
+        T.append(o["tris"].astype(np.int64) + off)
+        off += len(o["verts"])
+    return V, np.concatenate(T).astype(np.uint32)
+
+
+def _same(a, b):
+    for k in ("right", "leaf_first", "leaf_count", "tri_list"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(a["box"], b["box"])
+
+
+@pytest.mark.parametrize("case", ["heightfield", "cornell", "field", "soup", "tiny"])
+def test_product_builder_equals_oracle(case):
+    if case == "heightfield":
+        objs = [S.heightfield(129, seed=4)]
+    elif case == "cornell":
+        objs = S.cornell(k=3)[0]
+    elif case == "field":
+        objs = S.random_field(n_objects=24, seed=1)[0]
+    elif case == "soup":
+        rng = np.random.default_rng(2)
+        v = np.round(rng.normal(size=(300, 3)), 1).astype(np.float32)  # coincident coordinates
+        t = rng.integers(0, 300, (500, 3)).astype(np.uint32)  # degenerate and repeated triangles
+        objs = [{"verts": v, "tris": t, "normals": None, "mat": 0, "params": (0.5,) * 3}]
+    else:
+        objs = [{"verts": np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32),
+                 "tris": np.array([[0, 1, 2]], np.uint32), "normals": None, "mat": 0, "params": (0.5,) * 3}]
+    V, T = _soup(objs)
+    _same(O.Tree(V, T).export(), W.build_tree_export(V, T))
+    # threads change speed, not the tree
+    _same(W.build_tree_export(V, T, nthreads=1), W.build_tree_export(V, T, nthreads=8))
+
+
+def test_builder_large_heightfield_equals_oracle():
+    hf = S.heightfield(400, seed=7)
+    _same(O.Tree(hf["verts"], hf["tris"]).export(), W.build_tree_export(hf["verts"], hf["tris"]))
+
+
+def test_builder_rejects_bad_indices():
+    v = np.zeros((3, 3), np.float32)
+    with pytest.raises(W.WalkRayError):
+        W.build_tree_export(v, np.array([[0, 1, 3]], np.uint32))
+
+
+def test_workload_sizes():
+    assert len(S.heightfield(1024, seed=7)["tris"]) == 2093058
+    objs, cam = S.cornell(k=5)
+    assert sum(len(o["tris"]) for o in objs) == 40974
+    assert (cam["width"], cam["height"]) == (256, 256)
+    ico = S.icosphere(5)
+    assert len(ico["tris"]) == 20480 and len(ico["verts"]) == 10242
+    objs, cam = S.random_field()
+    n = sum(len(o["tris"]) for o in objs)
+    assert 450_000 < n < 550_000 and len(objs) == 515
+    mats = [o["mat"] for o in objs[1:-2]]
+    assert min(mats.count(m) for m in (S.DIFFUSE, S.GLOSSY, S.REFRACTIVE)) > 120
+
+
+def test_icosphere_and_box_winding_outward():
+    for o in (S.icosphere(2, (1, 2, 3), 0.5), S.box((0, 0, 0), (1, 2, 3))):
+        v = o["verts"].astype(np.float64)
+        c = v.mean(axis=0)
+        a, b, cc = v[o["tris"][:, 0]], v[o["tris"][:, 1]], v[o["tris"][:, 2]]
+        n = np.cross(b - a, cc - a)
+        assert np.all(np.einsum("ij,ij->i", n, (a + b + cc) / 3 - c) > 0)

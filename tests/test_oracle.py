@@ -1,0 +1,414 @@
+"""Pins for the CPU oracle (runs without a GPU).
+
+The reference ships no tests, fixtures or golden vectors and cannot be built
+here (every translation unit includes stdafx.h -> <SDKDDKVer.h>/<tchar.h>,
+Windows-SDK headers that are absent), so each oracle function is pinned by a
+known-answer vector, a closed form, an invariant or brute force, chosen so a
+plausible transcription mistake (dropped term, wrong sign, wrong index,
+swapped operand, wrong tie rule) fails at least one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from walkray_amd import scenes as S
+
+
+# ---------------------------------------------------------------- RNG ------
+def test_philox_known_answers():
+    """Random123 known-answer vectors for Philox-4x32-10 (kat_vectors of the SC'11 paper's code)."""
+    assert list(O.philox4x32([0, 0, 0, 0], [0, 0])) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert list(O.philox4x32([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2)) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6,
+                                                                        0x6D5451FD]
+    assert list(O.philox4x32([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344],
+                             [0xA4093822, 0x299F31D0])) == [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_rng_uniforms_range_and_moments():
+    vals = np.array([O.rng4(77, p, 3, 1) for p in range(20000)]).ravel()
+    assert vals.min() >= 0.0 and vals.max() < 1.0
+    assert abs(vals.mean() - 0.5) < 0.01 and abs(vals.var() - 1 / 12) < 0.005
+    # distinct vertices / passes give different streams
+    assert not np.array_equal(O.rng4(77, 5, 3, 1), O.rng4(77, 5, 3, 2))
+    assert not np.array_equal(O.rng4(77, 5, 3, 1), O.rng4(77, 5, 4, 1))
+
+
+# ------------------------------------------------------------ box test ------
+BOX = [0.0, 0.0, 0.0, 1.0, 2.0, 3.0]
+
+
+def test_box_entry_exit_and_miss():
+    # from outside along +x: min of the accepted face distances = entry distance (kdtree.cpp:60)
+    assert O.box_intersect(BOX, [-2.0, 1.0, 1.0, 1.0, 0.0, 0.0]) == 2.0
+    # from inside: only the exit face has dist >= 0
+    assert O.box_intersect(BOX, [0.25, 1.0, 1.0, 1.0, 0.0, 0.0]) == 0.75
+    assert O.box_intersect(BOX, [0.5, 1.0, 1.0, 0.0, -1.0, 0.0]) == 1.0
+    # pointing away and beside the box: miss
+    assert O.box_intersect(BOX, [-2.0, 1.0, 1.0, -1.0, 0.0, 0.0]) == -1.0
+    assert O.box_intersect(BOX, [-2.0, 5.0, 1.0, 1.0, 0.0, 0.0]) == -1.0
+    # zero direction: every face is skipped
+    assert O.box_intersect(BOX, [0.5, 0.5, 0.5, 0.0, 0.0, 0.0]) == -1.0
+
+
+def test_box_closed_faces_and_flat_boxes():
+    # grazing along a face is inclusive (minCoord <= p <= maxCoord)
+    assert O.box_intersect(BOX, [-1.0, 0.0, 1.0, 1.0, 0.0, 0.0]) == 1.0
+    flat = [0.0, 0.5, 0.0, 1.0, 0.5, 1.0]
+    assert O.box_intersect(flat, [0.5, 1.5, 0.5, 0.0, -1.0, 0.0]) == 1.0
+    assert O.box_intersect(flat, [0.5, 1.5, 2.5, 0.0, -1.0, 0.0]) == -1.0
+
+
+def test_box_matches_float64_on_random_rays():
+    rng = np.random.default_rng(1)
+    for _ in range(2000):
+        lo = rng.uniform(-1, 1, 3)
+        box = np.concatenate([lo, lo + rng.uniform(0.1, 1, 3)]).astype(np.float32)
+        o = rng.uniform(-3, 3, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        ray = np.concatenate([o, d]).astype(np.float32)
+        got = O.box_intersect(box, ray)
+        # exact slab answer in float64: entry (or exit if inside), -1 on a miss
+        b = box.astype(np.float64)
+        r = ray.astype(np.float64)
+        t0 = (b[:3] - r[:3]) / r[3:]
+        t1 = (b[3:] - r[:3]) / r[3:]
+        tn, tf = np.max(np.minimum(t0, t1)), np.min(np.maximum(t0, t1))
+        if tf < 0 or tn > tf:
+            want = -1.0
+        else:
+            want = tn if tn >= 0 else tf
+        if want < 0:
+            if got >= 0:  # only rounding at an edge may disagree
+                assert abs(tf - tn) < 1e-5 or tf > -1e-5
+        else:
+            assert got >= 0 and abs(got - want) <= 1e-5 * max(1, abs(want)) or abs(tf - tn) < 1e-5
+
+
+# ------------------------------------------------------- triangle test ------
+def test_triangle_axis_aligned_closed_form():
+    # normal exactly +-z: orthoDist = v2.z - o.z and d = dir.z, so t is one IEEE division
+    rng = np.random.default_rng(2)
+    for _ in range(500):
+        z = np.float32(rng.uniform(-2, 2))
+        v0, v1, v2 = np.array([[0, 0, z], [1, 0, z], [0, 1, z]], np.float32)
+        o = np.array([rng.uniform(0.05, 0.4), rng.uniform(0.05, 0.4), rng.uniform(3, 5)], np.float32)
+        d = np.array([0, 0, -1], np.float32)
+        t = O.tri_intersect(v0, v1, v2, np.concatenate([o, d]))
+        # n = (0,0,1) exactly, orthoDist = fl(z - o.z), d = -1: t = fl(o.z - z), bit for bit
+        assert np.float32(t) == o[2] - z
+
+
+def test_triangle_edges_backface_parallel_degenerate():
+    v0, v1, v2 = [0, 0, 0], [1, 0, 0], [0, 1, 0]
+    down = [0, 0, -1]
+    assert O.tri_intersect(v0, v1, v2, [0.5, 0.0, 1.0] + down) == 1.0    # on an edge: both masks count
+    assert O.tri_intersect(v0, v1, v2, [0.0, 0.0, 1.0] + down) == 1.0    # on a vertex
+    assert O.tri_intersect(v0, v1, v2, [0.25, 0.25, -1.0, 0, 0, 1]) == 1.0  # back face hit (two-sided)
+    assert O.tri_intersect(v0, v1, v2, [0.25, 0.25, 0.0] + down) == 0.0  # t = 0 accepted
+    assert O.tri_intersect(v0, v1, v2, [0.6, 0.6, 1.0] + down) == -1.0   # outside the hypotenuse
+    assert O.tri_intersect(v0, v1, v2, [0.25, 0.25, 1.0, 0, 0, 1]) == -1.0  # behind the origin
+    assert O.tri_intersect(v0, v1, v2, [0.25, 0.25, 0.0, 1, 0, 0]) == -1.0  # parallel (n.d == 0)
+    assert O.tri_intersect(v0, v1, [2, 0, 0], [0.5, 0.0, 1.0] + down) == -1.0  # degenerate
+
+
+def test_triangle_matches_float64_moller_trumbore():
+    rng = np.random.default_rng(3)
+    agree = 0
+    for _ in range(3000):
+        v = rng.uniform(-1, 1, (3, 3)).astype(np.float32)
+        o = rng.uniform(-2, 2, 3)
+        if rng.random() < 0.6:  # aim at a point of the triangle (or just outside it)
+            a, b = rng.uniform(-0.1, 1.0, 2)
+            d = v[0] + a * (v[1] - v[0]) + b * (v[2] - v[0]) - o
+        else:
+            d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        ray = np.concatenate([o, d]).astype(np.float32)
+        got = O.tri_intersect(v[0], v[1], v[2], ray)
+        V = v.astype(np.float64)
+        r = ray.astype(np.float64)
+        e1, e2 = V[1] - V[0], V[2] - V[0]
+        p = np.cross(r[3:], e2)
+        det = e1 @ p
+        u = (r[:3] - V[0]) @ p / det
+        q = np.cross(r[:3] - V[0], e1)
+        w = r[3:] @ q / det
+        t = e2 @ q / det
+        hit = u >= 0 and w >= 0 and u + w <= 1 and t >= 0
+        if hit:
+            assert got >= 0 and abs(got - t) <= 1e-4 * max(1, t), (got, t)
+            agree += 1
+        elif got >= 0:  # only near an edge or at grazing incidence can FP32 disagree
+            assert min(u, w, 1 - u - w) > -1e-4 or abs(det) < 1e-4
+    assert agree > 200
+
+
+# ----------------------------------------------------------- builder --------
+def _soup(objs):
+    V = np.concatenate([o["verts"] for o in objs])
+    T, off = [], 0
+    for o in objs:
+        T.append(o["tris"].astype(np.int64) + off)
+        off += len(o["verts"])
+    return V, np.concatenate(T).astype(np.uint32)
+
+
+def _check_tree_invariants(verts, tris, ex):
+    """Each triangle in exactly one leaf, leaf lists ascending (order-preserving partition),
+    tight boxes, and the split rule of kdtree.cpp:141-240 restated in numpy."""
+    n = len(ex["right"])
+    box, right = ex["box"], ex["right"]
+    tl = ex["tri_list"]
+    assert sorted(tl.tolist()) == list(range(len(tris)))
+
+    def subtree_tris(i):
+        out = []
+        stack = [i]
+        while stack:
+            j = stack.pop()
+            if right[j] == O.NONE:
+                out.extend(tl[ex["leaf_first"][j]:ex["leaf_first"][j] + ex["leaf_count"][j]].tolist())
+            else:
+                stack += [right[j], j + 1]
+        return out
+
+    for i in range(n):
+        ts = sorted(subtree_tris(i))
+        p = verts[tris[ts]].reshape(-1, 3)
+        assert np.array_equal(box[i], np.concatenate([p.min(0), p.max(0)]).astype(np.float32)), i
+        if right[i] == O.NONE:
+            lt = tl[ex["leaf_first"][i]:ex["leaf_first"][i] + ex["leaf_count"][i]]
+            assert list(lt) == sorted(lt)
+            if len(lt) > 1:  # a leaf with several triangles: every dim's split ratio is 0 or 1
+                uv = np.unique(tris[lt].ravel())
+                for dim in range(3):
+                    c = np.sort(verts[uv, dim])
+                    split = c[len(c) // 2]
+                    left = np.any(verts[tris[lt], dim] <= split, axis=1).sum()
+                    assert left in (0, len(lt))
+            continue
+        uv = np.unique(tris[ts].ravel())
+        best = None
+        for dim in range(3):
+            c = np.sort(verts[uv, dim])
+            split = c[len(c) // 2]
+            goes_left = np.any(verts[tris[ts], dim] <= split, axis=1)
+            ratio = np.float32(goes_left.sum()) / np.float32(len(ts))
+            if best is None or abs(float(ratio) - 0.5) < abs(best[0] - 0.5):
+                best = (float(ratio), goes_left)
+        want_left = [t for t, g in zip(ts, best[1]) if g]
+        assert sorted(subtree_tris(i + 1)) == want_left, i
+
+
+@pytest.mark.parametrize("case", ["tiny", "heightfield", "cornell", "duplicates"])
+def test_builder_invariants(case):
+    if case == "tiny":
+        objs = [S.icosphere(0)]
+    elif case == "heightfield":
+        objs = [S.heightfield(9, seed=2)]
+    elif case == "cornell":
+        objs = S.cornell(k=1)[0]
+    else:
+        rng = np.random.default_rng(5)
+        v = rng.integers(0, 3, (40, 3)).astype(np.float32)  # many coincident coordinates
+        t = rng.integers(0, 40, (60, 3)).astype(np.uint32)
+        objs = [{"verts": v, "tris": t, "normals": None, "mat": 0, "params": (0.5,) * 3}]
+    V, T = _soup(objs)
+    ex = O.Tree(V, T).export()
+    _check_tree_invariants(V, T, ex)
+
+
+def test_builder_small_sizes():
+    for nt in (1, 2, 3):
+        v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0.5], [2, 0, 1]], np.float32)
+        t = np.array([[0, 1, 2], [1, 3, 2], [1, 4, 3]][:nt], np.uint32)
+        ex = O.Tree(v, t).export()
+        _check_tree_invariants(v, t, ex)
+        if nt == 1:
+            assert len(ex["right"]) == 1
+
+
+# --------------------------------------------------------- traversal --------
+def _brute(verts, tris, ray):
+    ts = [O.tri_intersect(verts[a], verts[b], verts[c], ray) for a, b, c in tris]
+    ts = np.array(ts)
+    ok = ts >= 0
+    if not ok.any():
+        return O.NONE, -1.0, []
+    tmin = ts[ok].min()
+    return int(np.argmin(np.where(ok, ts, np.inf))), float(tmin), np.nonzero(ok & (ts == tmin))[0].tolist()
+
+
+def test_traversal_vs_brute_force():
+    """The tree returns the brute-force minimum t (bitwise) and, on ties, one of the tied triangles."""
+    objs = [S.icosphere(1, (0.2, 0.1, 0.0), 0.5), S.heightfield(5, seed=1), S.box((-1, -1, -1), (1.2, 0.9, 1.1),
+                                                                                 inward=True)]
+    V, T = _soup(objs)
+    tree = O.Tree(V, T)
+    rays = S.incoherent_rays(600, (-0.9, -0.9, -0.9), (1.1, 0.8, 1.0), seed=4)
+    prim, t = tree.intersect(rays)
+    mism = 0
+    for i, r in enumerate(rays):
+        bp, bt, tied = _brute(V, T, r)
+        if bp == O.NONE:
+            assert prim[i] == O.NONE
+            continue
+        if np.float32(t[i]) != np.float32(bt):
+            mism += 1  # the reference box test may reject a box at an edge: rare
+            continue
+        assert prim[i] in tied
+    assert mism <= 2, mism
+
+
+def test_tie_goes_to_right_subtree_and_earlier_slot():
+    # two coplanar triangles sharing an edge, ray exactly through the shared edge
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], np.float32)
+    t = np.array([[0, 1, 2], [1, 3, 2]], np.uint32)
+    tree = O.Tree(v, t)
+    ex = tree.export()
+    ray = np.array([[0.5, 0.5, 1.0, 0.0, 0.0, -1.0]], np.float32)
+    prim, tt = tree.intersect(ray)
+    assert tt[0] == 1.0
+    if len(ex["right"]) == 1:  # one leaf with both: the earlier triangle wins (strict <)
+        assert prim[0] == ex["tri_list"][0]
+    else:  # split: the right subtree wins the tie
+        assert prim[0] == ex["tri_list"][ex["leaf_first"][ex["right"][0]]]
+
+
+def test_heightfield_vertical_rays_closed_form():
+    n = 33
+    hf = S.heightfield(n, seed=9)
+    tree = O.Tree(hf["verts"], hf["tris"])
+    rng = np.random.default_rng(6)
+    xy = rng.uniform(0.001, 0.999, (2000, 2))
+    rays = np.zeros((2000, 6), np.float32)
+    rays[:, :2] = xy
+    rays[:, 2] = 5.0
+    rays[:, 5] = -1.0
+    prim, t = tree.intersect(rays)
+    xy = rays[:, :2].astype(np.float64)
+    g = xy * (n - 1)
+    i = np.minimum(np.floor(g).astype(int), n - 2)
+    fx, fy = g[:, 0] - i[:, 0], g[:, 1] - i[:, 1]
+    cell = i[:, 1] * (n - 1) + i[:, 0]
+    want = 2 * cell + (fy > fx)  # (v00, v10, v11) below the diagonal, (v00, v11, v01) above
+    near_diag = np.abs(fy - fx) < 1e-5
+    assert np.all((prim == want) | near_diag)
+    v = hf["verts"].astype(np.float64)
+    tri = hf["tris"][prim]
+    a, b, c = v[tri[:, 0]], v[tri[:, 1]], v[tri[:, 2]]
+    nrm = np.cross(b - a, c - a)
+    z = a[:, 2] - (nrm[:, 0] * (xy[:, 0] - a[:, 0]) + nrm[:, 1] * (xy[:, 1] - a[:, 1])) / nrm[:, 2]
+    assert np.max(np.abs((5.0 - z) - t)) < 2e-6
+
+
+# ---------------------------------------------- normals / NoSelf / inside ----
+def test_world_normal_barycentrics_and_interpolation():
+    sph = S.icosphere(2, (0.3, -0.2, 0.1), 0.7)
+    sc = O.Scene([sph])
+    rng = np.random.default_rng(7)
+    v = sph["verts"].astype(np.float64)
+    for _ in range(300):
+        f = int(rng.integers(len(sph["tris"])))
+        a, b = rng.random(2)
+        if a + b > 1:
+            a, b = 1 - a, 1 - b
+        i0, i1, i2 = sph["tris"][f]
+        p = v[i0] + a * (v[i1] - v[i0]) + b * (v[i2] - v[i0])
+        n, u1, u2 = sc.world_normal(0, f, p.astype(np.float32))
+        assert abs(u1 - a) < 1e-4 and abs(u2 - b) < 1e-4
+        nn = (1 - a - b) * sph["normals"][i0] + a * sph["normals"][i1] + b * sph["normals"][i2]
+        nn /= np.linalg.norm(nn)
+        assert np.allclose(n, nn, atol=1e-4)
+        assert abs(np.linalg.norm(n) - 1) < 1e-5
+
+
+def test_noself_and_find_inside():
+    objs, cam = S.cornell(k=2)
+    sc = O.Scene(objs)
+    glass, diffuse = 5, 4
+    sph = objs[glass]
+    c = np.array([0.72, 0.201, 0.62])
+    rng = np.random.default_rng(8)
+    # rays leaving the glass surface outward: never re-hit the glass at t ~ 0
+    f = rng.integers(len(sph["tris"]), size=200)
+    p = sph["verts"][sph["tris"][f]].mean(axis=1)
+    d = (p - c) / np.linalg.norm(p - c, axis=1, keepdims=True)
+    rays = np.concatenate([p, d], axis=1).astype(np.float32)
+    h = sc.intersect(rays, np.full(200, O.NONE, np.uint32), np.full(200, glass, np.uint32), noself=True)
+    assert np.all(h["obj"] != glass)
+    # refracted into the glass (inside == contact == glass): the next hit is the glass's far side
+    h = sc.intersect(np.concatenate([p, -d], axis=1).astype(np.float32), np.full(200, glass, np.uint32),
+                     np.full(200, glass, np.uint32), noself=True)
+    assert np.all(h["obj"] == glass) and np.all(h["t"] > 0.2)
+    # findInsideObject: points inside the spheres, the camera, a point in the box's air
+    u = rng.normal(size=(100, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    dd = rng.normal(size=(100, 3))
+    dd /= np.linalg.norm(dd, axis=1, keepdims=True)
+    inner = np.concatenate([c + 0.1 * u, dd], axis=1).astype(np.float32)
+    assert np.all(sc.find_inside(inner) == glass)
+    assert np.all(sc.find_inside(inner, np.full(100, glass, np.uint32)) == O.NONE)
+    inner_d = np.concatenate([np.array([0.28, 0.201, 0.35]) + 0.1 * u, dd], axis=1).astype(np.float32)
+    assert np.all(sc.find_inside(inner_d) == diffuse)
+    air = np.concatenate([np.tile([0.5, 0.8, 0.5], (100, 1)), dd], axis=1).astype(np.float32)
+    assert np.all(sc.find_inside(air) == O.NONE)
+
+
+# ------------------------------------------------------------- estimator ----
+def _furnace(floor_mat, params, le, width=12):
+    floor = S.quad((0, 0, 0), (2000, 0, 0), (0, 0, 2000), (0, 1, 0), floor_mat, params)
+    ceil = S.quad((0, 1, 0), (2000, 0, 0), (0, 0, 2000), (0, -1, 0), S.EMISSIVE, (le, le, le))
+    cam = S.camera((0.0, 0.5, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), width, width, sight_dist=100.0 * width)
+    return [floor, ceil], cam
+
+
+@pytest.mark.parametrize("mat,params,depth,expect", [
+    (S.DIFFUSE, (0.5, 0.5, 0.5), 0, 0.0),       # maxPathLen 0: the scatter's continuation is cut
+    (S.DIFFUSE, (0.5, 0.5, 0.5), 1, 0.5),       # one diffuse bounce: albedo * Le (RR-unbiased)
+    (S.DIFFUSE, (0.3, 0.7, 0.5), 3, None),      # per-channel albedo with roulette on max(albedo)
+    (S.GLOSSY, (0.6, 0.6, 0.6, 20.0), 2, 0.6),  # Phong at normal incidence: E[weight] = albedo
+])
+def test_furnace_closed_forms(mat, params, depth, expect):
+    le = 2.0
+    objs, cam = _furnace(mat, params, le)
+    sc = O.Scene(objs)
+    spp = 96
+    s, q = sc.render(cam, spp, depth, 21)
+    n = cam["width"] * cam["height"] * spp
+    mean = s.reshape(-1, 3).sum(0) / n
+    var = q.reshape(-1, 3).sum(0) / n - mean ** 2
+    se = np.sqrt(np.maximum(var, 0) / n)
+    want = np.array(params[:3]) * le if expect is None else np.full(3, expect * le)
+    assert np.all(np.abs(mean - want) <= 4 * se + 2e-3), (mean, want, se)
+
+
+def test_fresnel_normal_incidence_slab():
+    """Camera -> glass slab (IOR 1.5) at normal incidence -> emitter.  Each interface reflects
+    with probability R = ((n-1)/(n+1))^2 = 0.04 and the weight of a delta scatter is 1, so
+    the pixel value is Le * sum_k (1-R)^2 R^(2k) = Le (1-R)/(1+R) (cut at the depth limit)."""
+    le, depth = 1.0, 30
+    slab = S.box((-50, -50, -0.5), (50, 50, 0.5), S.REFRACTIVE, (1.5,))
+    light = S.quad((0, 0, -3.0), (200, 0, 0), (0, 200, 0), (0, 0, 1), S.EMISSIVE, (le, le, le))
+    cam = S.camera((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 8, 8, sight_dist=8000.0)
+    sc = O.Scene([slab, light])
+    spp = 400
+    s, q = sc.render(cam, spp, depth, 5)
+    n = 64 * spp
+    mean = s.reshape(-1, 3).sum(0)[0] / n
+    R = ((1.5 - 1) / (1.5 + 1)) ** 2
+    want = le * (1 - R) / (1 + R)
+    se = math.sqrt(max(q.reshape(-1, 3).sum(0)[0] / n - mean ** 2, 0) / n)
+    assert abs(mean - want) <= 4 * se + 1e-3, (mean, want, se)
+
+
+def test_render_deterministic_and_pass_additive():
+    objs, cam = S.cornell(k=1, width=8, height=8)
+    sc = O.Scene(objs)
+    a = sc.render(cam, 6, 8, 3)
+    b = sc.render(cam, 3, 8, 3)
+    b = sc.render(cam, 3, 8, 3, pass0=3, sums=b)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
