@@ -192,6 +192,56 @@ int wr_film_write(wr_film* film, const float* sum, const float* comp, const floa
 int wr_render(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, const wr_render_params* params,
               wr_film* film, wr_render_stats* stats);
 
+/* ---- serving (no counterpart in the reference: RenderEngine runs one job at
+ * a time from a key loop, renderEngine.cpp:72-91) ------------------------- */
+/* A server keeps scenes resident, accepts render jobs at any time, admits them
+ * against a device-memory budget and a job-slot limit, and renders every
+ * admitted job's paths interleaved in one shared wavefront pool.  A job's film
+ * is bit-identical to rendering it alone (randomness is keyed per job, pixel,
+ * pass and vertex, and passes are folded in order). */
+typedef struct wr_server wr_server;
+typedef struct {
+    uint64_t job_memory_budget; /* device bytes for job state (films + sample windows); 0 = 8 GiB */
+    uint32_t path_slots;        /* shared path pool; 0 = 4,194,304 */
+    uint32_t max_jobs;          /* concurrently admitted jobs, <= 64; 0 = 64 */
+} wr_server_config;
+
+typedef struct {
+    uint32_t scene; /* id returned by wr_server_add_scene */
+    wr_camera cam;
+    uint32_t spp;
+    uint32_t max_depth;
+    uint64_t seed;
+} wr_job_desc;
+
+enum { WR_JOB_QUEUED = 0, WR_JOB_RUNNING = 1, WR_JOB_DONE = 2, WR_JOB_FAILED = 3 };
+typedef struct {
+    int32_t state;
+    uint32_t reserved;
+    double submit_s, admit_s, done_s; /* wr_server_clock() times; 0 until reached */
+    uint64_t samples;                 /* width * height * spp */
+    uint64_t device_bytes;            /* job state held while admitted */
+} wr_job_status;
+
+int wr_server_create(wr_ctx* ctx, const wr_server_config* cfg, wr_server** out);
+void wr_server_destroy(wr_server* srv);
+/* Makes a built scene resident for jobs; the scene must outlive the server. */
+int wr_server_add_scene(wr_server* srv, const wr_scene* scene, uint32_t* scene_id);
+/* Queues a job (never blocks); a job whose state alone exceeds the budget fails. */
+int wr_job_submit(wr_server* srv, const wr_job_desc* job, uint64_t* job_id);
+/* Admits queued jobs that fit, runs one batch of wavefront iterations on every
+ * admitted job, and retires finished ones.  *n_active (nullable) = admitted +
+ * queued jobs left. */
+int wr_server_step(wr_server* srv, uint32_t* n_active);
+int wr_job_get_status(wr_server* srv, uint64_t job_id, wr_job_status* status);
+/* Film of a finished job (any pointer may be NULL), as wr_film_read. */
+int wr_job_read(wr_server* srv, uint64_t job_id, float* mean_rgb, float* sum, float* comp, float* sumsq,
+                uint32_t* passes);
+/* Frees a finished or failed job's film. */
+int wr_job_release(wr_server* srv, uint64_t job_id);
+/* Seconds since the server was created (steady clock). */
+double wr_server_clock(const wr_server* srv);
+
 #ifdef __cplusplus
 }
 #endif

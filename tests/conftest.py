@@ -15,6 +15,6 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def ctx():
     from walkray_amd import _lib as W
-    c = W.Context(0)
-    yield c
-    c.close()
+    # not closed explicitly: scenes, films and servers keep a reference to the
+    # context and are destroyed before it
+    yield W.Context(0)

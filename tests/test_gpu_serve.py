@@ -1,0 +1,86 @@
+"""Serving: continuous batching of render jobs over one wavefront pool.
+
+Bar: every job's film is bit-identical whether the job runs alone, in a mix of
+concurrent jobs on several resident scenes, or through wr_render (megakernel);
+admission honours the memory budget and the job-slot limit.
+"""
+import numpy as np
+import pytest
+
+from walkray_amd import _lib as W
+from walkray_amd import render as R
+from walkray_amd import scenes as S
+from walkray_amd import serve as V
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenes(ctx):
+    scs = V.serving_scenes(small=True)
+    return scs, [W.Scene(ctx, objs) for objs, _ in scs]
+
+
+def _drain(srv, jids, limit=100000):
+    for _ in range(limit):
+        srv.step()
+        if all(srv.status(j)["state"] in (W.JOB_DONE, W.JOB_FAILED) for j in jids):
+            return
+    raise AssertionError("jobs did not finish")
+
+
+JOBS = [(0, 48, 6, 9, 101), (4, 40, 10, 12, 102), (2, 32, 3, 8, 103), (5, 56, 5, 16, 104), (7, 24, 7, 10, 105)]
+
+
+def test_alone_equals_mix_equals_render(ctx):
+    scs, built = _scenes(ctx)
+    srv = W.Server(ctx)
+    ids = [srv.add_scene(b) for b in built]
+    cams = [V.job_camera(scs[s][1], res) for s, res, *_ in JOBS]
+    mix = [srv.submit(ids[s], cams[k], spp, depth, seed) for k, (s, res, spp, depth, seed) in enumerate(JOBS)]
+    _drain(srv, mix)
+    mixed = [srv.read(j, cams[k]["width"], cams[k]["height"]) for k, j in enumerate(mix)]
+    for k, (s, res, spp, depth, seed) in enumerate(JOBS):
+        alone_srv = W.Server(ctx)
+        aid = [alone_srv.add_scene(b) for b in built]
+        j = alone_srv.submit(aid[s], cams[k], spp, depth, seed)
+        _drain(alone_srv, [j])
+        alone = alone_srv.read(j, res, res)
+        alone_srv.close()
+        assert np.array_equal(alone["sum"].view(np.uint32), mixed[k]["sum"].view(np.uint32)), f"job {k}: mix != alone"
+        film = W.Film(ctx, res, res)
+        W.render(built[s], cams[k], film, spp, depth, seed, W.WR_MEGAKERNEL)
+        ref = film.read()
+        for key in ("sum", "comp", "sumsq"):
+            assert np.array_equal(ref[key].view(np.uint32), mixed[k][key].view(np.uint32)), (k, key)
+    st = [srv.status(j) for j in mix]
+    assert all(s["state"] == W.JOB_DONE and s["done_s"] >= s["admit_s"] >= s["submit_s"] for s in st)
+
+
+def test_budget_and_slots(ctx):
+    scs, built = _scenes(ctx)
+    cam = V.job_camera(scs[0][1], 64)
+    one_job = 3 * 4 * 3 * 64 * 64 + 3 * 4 * 64 * 64 * 8 + 4 * 8
+    srv = W.Server(ctx, budget_bytes=int(one_job * 1.5), max_jobs=4)
+    sid = srv.add_scene(built[0])
+    big = srv.submit(sid, V.job_camera(scs[0][1], 512), 8, 8, 1)  # alone exceeds the budget
+    assert srv.status(big)["state"] == W.JOB_FAILED
+    a = srv.submit(sid, cam, 8, 8, 2)
+    b = srv.submit(sid, cam, 8, 8, 3)
+    srv.step()
+    assert srv.status(a)["state"] in (W.JOB_RUNNING, W.JOB_DONE)
+    assert srv.status(b)["state"] == W.JOB_QUEUED  # waits for memory
+    _drain(srv, [a, b])
+    sa, sb = srv.status(a), srv.status(b)
+    assert sb["admit_s"] >= sa["done_s"]
+    with pytest.raises(W.WalkRayError):
+        srv.read(big, 512, 512)
+
+
+def test_poisson_run_small(ctx):
+    scs, built = _scenes(ctx)
+    srv = W.Server(ctx)
+    ids = [srv.add_scene(b) for b in built]
+    jobs = V.make_jobs(24, len(built), rate=200.0, max_res=128, max_spp=16)
+    recs = V.run_poisson(srv, ids, [c for _, c in scs], jobs)
+    summ = V.summarize(recs)
+    assert summ["completed"] == 24 and summ["p99_ms"] >= summ["p50_ms"] > 0

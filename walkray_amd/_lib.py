@@ -53,6 +53,32 @@ class CameraDesc(C.Structure):
         return c
 
 
+class RenderParams(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("seed", C.c_uint64), ("mode", C.c_uint32),
+                ("wave_slots", C.c_uint32), ("wave_grid", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("iterations", C.c_uint64), ("kernel_ms", C.c_float)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("job_memory_budget", C.c_uint64), ("path_slots", C.c_uint32), ("max_jobs", C.c_uint32)]
+
+
+class JobDescC(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("cam", CameraDesc), ("spp", C.c_uint32), ("max_depth", C.c_uint32),
+                ("seed", C.c_uint64)]
+
+
+class JobStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("reserved", C.c_uint32), ("submit_s", C.c_double), ("admit_s", C.c_double),
+                ("done_s", C.c_double), ("samples", C.c_uint64), ("device_bytes", C.c_uint64)]
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
+
+
 def header_symbols():
     """Function names declared in include/walkray.h."""
     txt = open(HEADER).read()
@@ -100,6 +126,15 @@ def lib():
         "wr_film_read": (i32, [vp, vp, vp, vp, vp, P(u32)]),
         "wr_film_write": (i32, [vp, vp, vp, vp, u32]),
         "wr_render": (i32, [vp, vp, P(CameraDesc), P(RenderParams), vp, P(RenderStats)]),
+        "wr_server_create": (i32, [vp, P(ServerConfig), P(vp)]),
+        "wr_server_destroy": (None, [vp]),
+        "wr_server_add_scene": (i32, [vp, vp, P(u32)]),
+        "wr_job_submit": (i32, [vp, P(JobDescC), P(u64)]),
+        "wr_server_step": (i32, [vp, P(u32)]),
+        "wr_job_get_status": (i32, [vp, u64, P(JobStatus)]),
+        "wr_job_read": (i32, [vp, u64, vp, vp, vp, vp, P(u32)]),
+        "wr_job_release": (i32, [vp, u64]),
+        "wr_server_clock": (C.c_double, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -210,15 +245,6 @@ class Context:
             pass
 
 
-class RenderParams(C.Structure):
-    _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("seed", C.c_uint64), ("mode", C.c_uint32),
-                ("wave_slots", C.c_uint32), ("wave_grid", C.c_uint32), ("reserved", C.c_uint32)]
-
-
-class RenderStats(C.Structure):
-    _fields_ = [("paths", C.c_uint64), ("iterations", C.c_uint64), ("kernel_ms", C.c_float)]
-
-
 WR_MEGAKERNEL, WR_WAVEFRONT = 0, 1
 
 
@@ -249,9 +275,9 @@ class Film:
                self.ctx.h, "wr_film_write")
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:  # a destroyed context already took the device memory with it
             lib().wr_film_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -378,9 +404,69 @@ class Scene:
         return do.download(np.uint32, n) if n else np.zeros(0, np.uint32)
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().wr_scene_destroy(self.h)
-            self.h = None
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """wr_server: resident scenes, queued jobs, continuous batching in one wavefront pool."""
+
+    def __init__(self, ctx, budget_bytes=0, path_slots=0, max_jobs=0):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        cfg = ServerConfig(budget_bytes, path_slots, max_jobs)
+        _check(lib().wr_server_create(ctx.h, C.byref(cfg), C.byref(self.h)), ctx.h, "wr_server_create")
+        self._scenes = []
+
+    def add_scene(self, scene):
+        sid = C.c_uint32()
+        _check(lib().wr_server_add_scene(self.h, scene.h, C.byref(sid)), self.ctx.h, "wr_server_add_scene")
+        self._scenes.append(scene)
+        return sid.value
+
+    def submit(self, scene_id, cam, spp, max_depth, seed):
+        d = JobDescC(scene_id, CameraDesc.from_dict(cam), spp, max_depth, seed)
+        jid = C.c_uint64()
+        rc = lib().wr_job_submit(self.h, C.byref(d), C.byref(jid))
+        if rc != 0 and jid.value == 0:
+            _check(rc, self.ctx.h, "wr_job_submit")
+        return jid.value
+
+    def step(self):
+        n = C.c_uint32()
+        _check(lib().wr_server_step(self.h, C.byref(n)), self.ctx.h, "wr_server_step")
+        return n.value
+
+    def status(self, jid):
+        s = JobStatus()
+        _check(lib().wr_job_get_status(self.h, jid, C.byref(s)), self.ctx.h, "wr_job_get_status")
+        return {k: getattr(s, k) for k, _ in JobStatus._fields_}
+
+    def read(self, jid, width, height):
+        n = 3 * width * height
+        mean, s, c, q = (np.zeros(n, np.float32) for _ in range(4))
+        passes = C.c_uint32()
+        _check(lib().wr_job_read(self.h, jid, _p(mean), _p(s), _p(c), _p(q), C.byref(passes)), self.ctx.h,
+               "wr_job_read")
+        return {"mean": mean, "sum": s, "comp": c, "sumsq": q, "passes": passes.value}
+
+    def release(self, jid):
+        _check(lib().wr_job_release(self.h, jid), self.ctx.h, "wr_job_release")
+
+    def clock(self):
+        return lib().wr_server_clock(self.h)
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib().wr_server_destroy(self.h)
+        self.h = None
 
     def __del__(self):
         try:

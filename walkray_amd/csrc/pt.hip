@@ -4,13 +4,13 @@
 //                   pass, each lane runs its pixel's whole path
 //                   (pathTracer.cpp:20-70 with mcRenderer.cpp:16-57) and adds
 //                   the sample to the Kahan film it owns.
-//  wavefront        the same estimator split into stages over a pool of path
-//                   slots: generate (camera ray + primary hit + inside object),
-//                   shade (material scatter + Russian roulette), extend
-//                   (NoSelf closest hit); finished paths write their sample to
-//                   a per-(pass, pixel) slot; complete passes are folded into
-//                   the Kahan film in pass order, so the film is bit-identical
-//                   to the megakernel's.
+//  wavefront        the same estimator split into stages (extend, shade, plan,
+//                   generate, fold, advance) over a shared pool of path slots
+//                   that any number of jobs (render calls, served requests)
+//                   fill; finished paths write their sample to a per-(job,
+//                   pass, pixel) slot and complete passes are folded into the
+//                   job's Kahan film in pass order, so every film is
+//                   bit-identical to the megakernel's.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,87 +34,107 @@ __global__ __launch_bounds__(256) void k_pt_mega(SceneDev S, CamDev cam, uint32_
     film_add(film, p, v);
 }
 
-// ---- wavefront --------------------------------------------------------------
-// Live paths are kept dense: every stage reads path i of the current buffer
-// (coalesced SoA loads) and the shade stage appends survivors, then the
-// generate stage appends new camera paths, to the other buffer with one atomic
-// per wave.  Primary (camera) rays are traced by the extend stage like any
-// other ray.  Finished paths write their sample to a per-(pass, pixel) slot;
-// the fold stage adds complete passes to the Kahan film in pass order.
+// ---- wavefront engine ---------------------------------------------------------
+// Live paths of every admitted job share one pool of dense path buffers
+// (ping-pong, SoA, coalesced): extend traces every live ray (camera rays
+// without NoSelf plus findInsideObject, later rays with NoSelf), shade scatters
+// and appends survivors to the other buffer with one atomic per wave, plan
+// hands the free slots to the running jobs round-robin (each job keeps at most
+// R passes in flight), generate appends their new camera paths, fold adds
+// every complete pass of every job to that job's Kahan film in pass order, and
+// advance retires finished jobs.  Per-path randomness is keyed by the job's
+// seed and the (pixel, pass, vertex) of the path, so a job's film does not
+// depend on which other jobs share the pool.
 constexpr uint32_t kPrimary = 1u;
 constexpr uint32_t kDead = 2u;
 
 struct PathBuf {
     float *ox, *oy, *oz, *dx, *dy, *dz;
     float *tx, *ty, *tz, *ht;
-    uint32_t *pixel, *pass, *depth, *inside, *contact, *hrec, *hobj, *flags;
+    uint32_t *pixel, *pass, *depth, *inside, *contact, *hrec, *hobj, *flags, *job;
 };
 
-struct WaveDev {
+struct EngineDev {
     PathBuf buf[2];
-    wr::WaveCounters* cnt;
-    float* contrib;  // R * npix * 3: samples of the passes in flight
-    uint32_t cap, npix, R, npass;
-    uint64_t total;
+    wr::EngineCounters* cnt;
+    const wr::JobDesc* jobs;
+    wr::JobCtl* ctl;
+    const SceneDev* scenes;
+    uint32_t cap;
 };
 
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, uint32_t amount = 1) {
+__device__ __forceinline__ CamDev job_cam(const wr::JobDesc& J) {
+    CamDev c;
+    c.pos = mk(J.pos[0], J.pos[1], J.pos[2]);
+    c.front = mk(J.front[0], J.front[1], J.front[2]);
+    c.right = mk(J.right[0], J.right[1], J.right[2]);
+    c.top = mk(J.top[0], J.top[1], J.top[2]);
+    c.dist = J.dist;
+    c.W = J.W;
+    c.H = J.H;
+    return c;
+}
+
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter) {
     // one atomic per wave; lanes get consecutive slots in lane order
     const uint64_t mask = __ballot(1);
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
     const uint32_t rank = __popcll(mask & ((1ull << lane) - 1));
     uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask) * amount);
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
     base = __shfl(base, leader);
-    return base + rank * amount;
+    return base + rank;
 }
 
-// pass_done[key] += 1 for every active lane, with one atomic per distinct key in
-// the wave (the lanes of a wave almost always finish paths of one pass).
-__device__ __forceinline__ void count_done(uint32_t* pass_done, uint32_t key) {
+// *addr += 1 for every active lane, one atomic per distinct address in the wave
+// (lanes of a wave almost always finish paths of the same job and pass).
+__device__ __forceinline__ void count_done(uint32_t* addr) {
+    const uint64_t a = (uint64_t)addr;
     for (;;) {
         const uint64_t act = __ballot(1);
         const uint32_t leader = __ffsll((unsigned long long)act) - 1;
-        const uint32_t k = __shfl(key, leader);
-        const uint64_t same = __ballot(key == k);
-        if (key == k) {
-            if ((threadIdx.x & 63) == leader) atomicAdd(&pass_done[k], (uint32_t)__popcll(same));
+        const uint32_t lo = __shfl((uint32_t)a, leader), hi = __shfl((uint32_t)(a >> 32), leader);
+        const bool mine = (uint32_t)a == lo && (uint32_t)(a >> 32) == hi;
+        const uint64_t same = __ballot(mine);
+        if (mine) {
+            if ((threadIdx.x & 63) == leader) atomicAdd(addr, (uint32_t)__popcll(same));
             return;
         }
     }
 }
 
-__device__ __forceinline__ void finish(const WaveDev& W, const CamDev& cam, uint32_t p, uint32_t ps, f3 color) {
+__device__ __forceinline__ void finish(const wr::JobDesc& J, const CamDev& cam, uint32_t p, uint32_t ps, f3 color) {
     if (!legal(color)) color = mk(0.f, 0.f, 0.f);
     const f3 v = fix_vignetting(cam, color, p);
-    const uint32_t key = ps % W.R;
-    float* c = W.contrib + 3 * ((size_t)key * W.npix + p);
+    const uint32_t key = ps % J.R;
+    float* c = J.contrib + 3 * ((size_t)key * J.npix + p);
     c[0] = v.x;
     c[1] = v.y;
     c[2] = v.z;
-    count_done(W.cnt->pass_done, key);
+    count_done(J.pass_done + key);
 }
 
-__global__ __launch_bounds__(256) void k_wf_extend(SceneDev S, WaveDev W, CamDev cam, uint32_t par) {
-    const PathBuf& A = W.buf[par];
-    const uint32_t n = W.cnt->n[par];
+__global__ __launch_bounds__(256) void k_wf_extend(EngineDev E, uint32_t par) {
+    const PathBuf& A = E.buf[par];
+    const uint32_t n = E.cnt->n[par];
     LaneStack st;
     st.lane = threadIdx.x;
-    const SceneCtx sc = make_scene_ctx(S);
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const wr::JobDesc& J = E.jobs[A.job[i]];
+        const SceneDev& S = E.scenes[J.scene];
+        const SceneCtx sc = make_scene_ctx(S);
         const f3 o = mk(A.ox[i], A.oy[i], A.oz[i]);
         const f3 d = mk(A.dx[i], A.dy[i], A.dz[i]);
-        const uint32_t fl = A.flags[i];
         PHit h;
-        if (fl & kPrimary) {  // camera ray: Scene::intersect without Condition + findInsideObject (camera.cpp:52-56)
+        if (A.flags[i] & kPrimary) {  // camera ray: Scene::intersect without Condition + findInsideObject (camera.cpp:52-56)
             h = trace_scene<false>(sc, o, d, WR_NONE, WR_NONE, st);
             A.inside[i] = find_inside(S, o, d, WR_NONE, st);
         } else {
             h = trace_scene<true>(sc, o, d, A.inside[i], A.contact[i], st);
         }
-        if (h.rec == WR_NONE) {  // escaped (no environment light) or no object to scatter: sample 0
-            finish(W, cam, A.pixel[i], A.pass[i], mk(0.f, 0.f, 0.f));
+        if (h.rec == WR_NONE) {  // escaped (no environment light) or nothing to scatter: sample 0
+            finish(J, job_cam(J), A.pixel[i], A.pass[i], mk(0.f, 0.f, 0.f));
             A.flags[i] = kDead;
             continue;
         }
@@ -125,16 +145,17 @@ __global__ __launch_bounds__(256) void k_wf_extend(SceneDev S, WaveDev W, CamDev
     }
 }
 
-__global__ __launch_bounds__(256) void k_wf_shade(SceneDev S, WaveDev W, CamDev cam, uint32_t max_depth,
-                                                  uint64_t seed, uint32_t pass0, uint32_t par) {
-    const PathBuf& A = W.buf[par];
-    const PathBuf& B = W.buf[par ^ 1];
-    const uint32_t n = W.cnt->n[par];
+__global__ __launch_bounds__(256) void k_wf_shade(EngineDev E, uint32_t par) {
+    const PathBuf& A = E.buf[par];
+    const PathBuf& B = E.buf[par ^ 1];
+    const uint32_t n = E.cnt->n[par];
     LaneStack st;
     st.lane = threadIdx.x;
-    const SceneCtx sc = make_scene_ctx(S);
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         if (A.flags[i] & kDead) continue;
+        const uint32_t jb = A.job[i];
+        const wr::JobDesc& J = E.jobs[jb];
+        const SceneCtx sc = make_scene_ctx(E.scenes[J.scene]);
         const f3 o = mk(A.ox[i], A.oy[i], A.oz[i]);
         const f3 d = mk(A.dx[i], A.dy[i], A.dz[i]);
         PHit h;
@@ -144,22 +165,22 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneDev S, WaveDev W, CamDev 
         const uint32_t inside = A.inside[i], depth = A.depth[i], p = A.pixel[i], ps = A.pass[i];
         const uint32_t self = inside != WR_NONE ? inside : h.obj;
         float rn[4];
-        rng4(seed, p, pass0 + ps, depth + 1, rn);
+        rng4(J.seed, p, J.pass0 + ps, depth + 1, rn);
         const Out out = scatter(sc, self, o, d, h, inside, rn, st);
         f3 T = mk(A.tx[i], A.ty[i], A.tz[i]);
         if ((double)len(out.d) < 0.5) {  // emission / absorption ends the path
-            finish(W, cam, p, ps, mulv(T, divs(divs(out.rad, out.op), out.dp)));
+            finish(J, job_cam(J), p, ps, mulv(T, divs(divs(out.rad, out.op), out.dp)));
             continue;
         }
-        if (depth + 1 > max_depth) {
-            finish(W, cam, p, ps, mk(0.f, 0.f, 0.f));
+        if (depth + 1 > J.max_depth) {
+            finish(J, job_cam(J), p, ps, mk(0.f, 0.f, 0.f));
             continue;
         }
         // the weight of this vertex only matters if the next ray hits; applying it
         // now keeps the per-pixel loop's multiplication order
         T = muls(T, cosine_term(out));
         T = mulv(T, divs(divs(out.rad, out.op), out.dp));
-        const uint32_t j = wave_append(&W.cnt->n[par ^ 1]);
+        const uint32_t j = wave_append(&E.cnt->n[par ^ 1]);
         B.ox[j] = out.o.x; B.oy[j] = out.o.y; B.oz[j] = out.o.z;
         B.dx[j] = out.d.x; B.dy[j] = out.d.y; B.dz[j] = out.d.z;
         B.tx[j] = T.x; B.ty[j] = T.y; B.tz[j] = T.z;
@@ -169,40 +190,61 @@ __global__ __launch_bounds__(256) void k_wf_shade(SceneDev S, WaveDev W, CamDev 
         B.inside[j] = out.inside;
         B.contact[j] = out.contact;
         B.flags[j] = 0;
+        B.job[j] = jb;
     }
 }
 
-// One lane: how many new camera paths fit behind the survivors, inside the
-// window of passes whose sample slots are free.
-__global__ void k_wf_plan(WaveDev W, uint32_t par) {
-    wr::WaveCounters& c = *W.cnt;
-    const uint32_t nb = c.n[par ^ 1];
-    const uint64_t lim_w = (uint64_t)(c.cursor + W.R) * W.npix;
-    const uint64_t limit = lim_w < W.total ? lim_w : W.total;
-    const uint64_t remaining = limit > c.next_work ? limit - c.next_work : 0;
-    const uint32_t room = W.cap - nb;
-    const uint32_t take = (uint32_t)(remaining < room ? remaining : room);
-    c.take = take;
-    c.take_base = nb;
-    c.take_work = c.next_work;
-    c.n[par ^ 1] = nb + take;
-    c.next_work += take;
+// One lane: hand the free slots behind the survivors to the running jobs,
+// round-robin from a rotating start, each within its window of free pass slots.
+__global__ void k_wf_plan(EngineDev E, uint32_t par) {
+    wr::EngineCounters& c = *E.cnt;
+    uint32_t base = c.n[par ^ 1];
+    uint32_t room = E.cap - base;
+    uint32_t nseg = 0;
+    for (uint32_t k = 0; k < wr::kMaxJobs && room > 0; k++) {
+        const uint32_t j = (c.rr + k) % wr::kMaxJobs;
+        wr::JobCtl& C = E.ctl[j];
+        if (C.state != wr::kJobRunning) continue;
+        const wr::JobDesc& J = E.jobs[j];
+        const uint64_t total = (uint64_t)J.npass * J.npix;
+        const uint64_t lim_w = (uint64_t)(C.cursor + J.R) * J.npix;
+        const uint64_t limit = lim_w < total ? lim_w : total;
+        const uint64_t avail = limit > C.next_work ? limit - C.next_work : 0;
+        const uint32_t take = (uint32_t)(avail < room ? avail : room);
+        if (take == 0) continue;
+        c.seg_job[nseg] = j;
+        c.seg_base[nseg] = base;
+        c.seg_take[nseg] = take;
+        c.seg_work[nseg] = C.next_work;
+        nseg++;
+        C.next_work += take;
+        base += take;
+        room -= take;
+    }
+    c.nseg = nseg;
+    c.seg_end = base;
+    c.seg_start = c.n[par ^ 1];
+    c.n[par ^ 1] = base;
+    c.rr = (c.rr + 1) % wr::kMaxJobs;
 }
 
-__global__ __launch_bounds__(256) void k_wf_generate(WaveDev W, CamDev cam, uint64_t seed, uint32_t pass0,
-                                                     uint32_t par) {
-    const PathBuf& B = W.buf[par ^ 1];
-    const uint32_t take = W.cnt->take, base = W.cnt->take_base;
-    const uint64_t work0 = W.cnt->take_work;
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < take; i += gridDim.x * 256) {
-        const uint64_t w = work0 + i;
-        const uint32_t ps = (uint32_t)(w / W.npix), p = (uint32_t)(w % W.npix);
+__global__ __launch_bounds__(256) void k_wf_generate(EngineDev E, uint32_t par) {
+    const PathBuf& B = E.buf[par ^ 1];
+    const wr::EngineCounters& c = *E.cnt;
+    const uint32_t start = c.seg_start, end = c.seg_end, nseg = c.nseg;
+    for (uint32_t j = start + blockIdx.x * 256 + threadIdx.x; j < end; j += gridDim.x * 256) {
+        uint32_t s = 0;
+        while (s + 1 < nseg && c.seg_base[s + 1] <= j) s++;
+        const uint32_t jb = c.seg_job[s];
+        const wr::JobDesc& J = E.jobs[jb];
+        const CamDev cam = job_cam(J);
+        const uint64_t w = c.seg_work[s] + (j - c.seg_base[s]);
+        const uint32_t ps = (uint32_t)(w / J.npix), p = (uint32_t)(w % J.npix);
         float rn[4];
-        rng4(seed, p, pass0 + ps, 0, rn);
+        rng4(J.seed, p, J.pass0 + ps, 0, rn);
         const f3 d = cam_dir(cam, p, rn[0], rn[1]);
         float c0 = fabsf(dot(cam.front, d));  // camera cosineTerm (ray.cpp:52-58, camera.cpp:182-186)
         c0 = (double)c0 > kEps ? c0 : (float)kEps;
-        const uint32_t j = base + i;
         B.ox[j] = cam.pos.x; B.oy[j] = cam.pos.y; B.oz[j] = cam.pos.z;
         B.dx[j] = d.x; B.dy[j] = d.y; B.dz[j] = d.z;
         B.tx[j] = c0; B.ty[j] = c0; B.tz[j] = c0;
@@ -212,37 +254,53 @@ __global__ __launch_bounds__(256) void k_wf_generate(WaveDev W, CamDev cam, uint
         B.inside[j] = WR_NONE;
         B.contact[j] = WR_NONE;
         B.flags[j] = kPrimary;
+        B.job[j] = jb;
     }
 }
 
-// Number of consecutive complete passes starting at the fold cursor.
-__device__ __forceinline__ uint32_t foldable(const WaveDev& W) {
-    const wr::WaveCounters& c = *W.cnt;
+// Number of consecutive complete passes of a job starting at its fold cursor.
+__device__ __forceinline__ uint32_t foldable(const wr::JobDesc& J, const wr::JobCtl& C) {
     uint32_t k = 0;
-    while (k < W.R && c.cursor + k < W.npass && c.pass_done[(c.cursor + k) % W.R] == W.npix) k++;
+    while (k < J.R && C.cursor + k < J.npass && J.pass_done[(C.cursor + k) % J.R] == J.npix) k++;
     return k;
 }
 
-// Adds every complete pass (in pass order) to the film; pixel-parallel.
-__global__ __launch_bounds__(256) void k_wf_fold(WaveDev W, FilmDev film) {
-    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t k = foldable(W);
-    if (p >= W.npix) return;
-    const uint32_t cur = W.cnt->cursor;
-    for (uint32_t j = 0; j < k; j++) {
-        const float* c = W.contrib + 3 * ((size_t)((cur + j) % W.R) * W.npix + p);
-        film_add(film, p, mk(c[0], c[1], c[2]));
+// Adds every complete pass of every running job to its film, in pass order.
+__global__ __launch_bounds__(256) void k_wf_fold(EngineDev E) {
+    for (uint32_t jb = 0; jb < wr::kMaxJobs; jb++) {
+        const wr::JobCtl& C = E.ctl[jb];
+        if (C.state != wr::kJobRunning) continue;
+        const wr::JobDesc& J = E.jobs[jb];
+        const uint32_t k = foldable(J, C);
+        if (k == 0) continue;
+        const FilmDev film{J.fsum, J.fcomp, J.fsumsq};
+        for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < J.npix; p += gridDim.x * 256) {
+            for (uint32_t q = 0; q < k; q++) {
+                const float* c = J.contrib + 3 * ((size_t)((C.cursor + q) % J.R) * J.npix + p);
+                film_add(film, p, mk(c[0], c[1], c[2]));
+            }
+        }
     }
 }
 
-// One lane: advance the fold cursor, re-arm folded pass slots, empty the consumed buffer.
-__global__ void k_wf_advance(WaveDev W, uint32_t par) {
-    const uint32_t k = foldable(W);
-    wr::WaveCounters& c = *W.cnt;
-    for (uint32_t j = 0; j < k; j++) c.pass_done[(c.cursor + j) % W.R] = 0;
-    c.cursor += k;
+// One lane: advance fold cursors, re-arm folded pass slots, retire finished
+// jobs, empty the consumed path buffer.
+__global__ void k_wf_advance(EngineDev E, uint32_t par) {
+    wr::EngineCounters& c = *E.cnt;
+    for (uint32_t jb = 0; jb < wr::kMaxJobs; jb++) {
+        wr::JobCtl& C = E.ctl[jb];
+        if (C.state != wr::kJobRunning) continue;
+        const wr::JobDesc& J = E.jobs[jb];
+        const uint32_t k = foldable(J, C);
+        for (uint32_t q = 0; q < k; q++) J.pass_done[(C.cursor + q) % J.R] = 0;
+        C.cursor += k;
+        if (C.cursor >= J.npass) {
+            C.state = wr::kJobDone;
+            C.done_iter = c.iteration;
+        }
+    }
     c.n[par] = 0;
-    c.iterations++;
+    c.iteration++;
 }
 
 }  // namespace wrd
@@ -263,13 +321,11 @@ static CamDev to_cam(const CamHost& c) {
     return d;
 }
 
-static FilmDev to_film(const FilmBufs& f) { return FilmDev{f.sum, f.comp, f.sumsq}; }
-
 hipError_t launch_pt_mega(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass,
                           const FilmBufs& film, hipStream_t st) {
     const uint32_t npix = cam.W * cam.H;
     k_pt_mega<<<(npix + 255) / 256, 256, kStackLdsBytes, st>>>(S, to_cam(cam), max_depth, seed, pass,
-                                                                to_film(film), npix);
+                                                                FilmDev{film.sum, film.comp, film.sumsq}, npix);
     return hipGetLastError();
 }
 
@@ -279,114 +335,170 @@ hipError_t launch_pt_mega(const SceneDev& S, const CamHost& cam, uint32_t max_de
         if (e_ != hipSuccess) return e_; \
     } while (0)
 
-WavefrontPool::~WavefrontPool() { release(); }
+Engine::~Engine() { release(); }
 
-void WavefrontPool::release() {
+void Engine::release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     bufs.clear();
-    cap = contrib_floats = 0;
-}
-
-hipError_t WavefrontPool::ensure(uint32_t want_cap, size_t want_contrib_floats) {
-    if (cap >= want_cap && contrib_floats >= want_contrib_floats && !bufs.empty()) return hipSuccess;
-    release();
-    cap = want_cap;
-    contrib_floats = want_contrib_floats;
-    auto alloc = [&](size_t bytes) -> void* {
-        void* p = nullptr;
-        if (hipMalloc(&p, std::max<size_t>(bytes, 4)) != hipSuccess) return nullptr;
-        bufs.push_back(p);
-        return p;
-    };
-    const size_t w4 = 4 * (size_t)cap;
-    for (int b = 0; b < 2; b++) {
-        for (int k = 0; k < kPathWords; k++)
-            if (!(words[b][k] = alloc(w4))) return hipErrorOutOfMemory;
-    }
-    if (!(cnt = (WaveCounters*)alloc(sizeof(WaveCounters)))) return hipErrorOutOfMemory;
-    if (!(contrib = (float*)alloc(sizeof(float) * contrib_floats))) return hipErrorOutOfMemory;
-    return hipSuccess;
+    cap = 0;
 }
 
 static PathBuf to_pathbuf(void* const* w) {
     PathBuf b;
     float** f[] = {&b.ox, &b.oy, &b.oz, &b.dx, &b.dy, &b.dz, &b.tx, &b.ty, &b.tz, &b.ht};
-    uint32_t** u[] = {&b.pixel, &b.pass, &b.depth, &b.inside, &b.contact, &b.hrec, &b.hobj, &b.flags};
+    uint32_t** u[] = {&b.pixel, &b.pass, &b.depth, &b.inside, &b.contact, &b.hrec, &b.hobj, &b.flags, &b.job};
     int k = 0;
     for (float** p : f) *p = (float*)w[k++];
     for (uint32_t** p : u) *p = (uint32_t*)w[k++];
     return b;
 }
 
-// Renders passes [pass0, pass0 + npass) of the film with the wavefront stages.
-// Iterations run as replays of a captured graph of `kGraphIters` iterations;
-// the host reads one counter per replay to detect completion.
+hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st) {
+    if (want_grid == 0) want_grid = 2048;
+    want_cap = std::max<uint32_t>(256, (want_cap + 255) / 256 * 256);
+    if (cap == want_cap && grid == want_grid && exec) return hipSuccess;
+    release();
+    cap = want_cap;
+    grid = want_grid;
+    auto alloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 4)) != hipSuccess) return nullptr;
+        bufs.push_back(p);
+        return p;
+    };
+    void* words[2][kPathWords];
+    for (int b = 0; b < 2; b++)
+        for (int k = 0; k < kPathWords; k++)
+            if (!(words[b][k] = alloc(4 * (size_t)cap))) return hipErrorOutOfMemory;
+    if (!(cnt = (EngineCounters*)alloc(sizeof(EngineCounters)))) return hipErrorOutOfMemory;
+    if (!(jobs = (JobDesc*)alloc(sizeof(JobDesc) * kMaxJobs))) return hipErrorOutOfMemory;
+    if (!(ctl = (JobCtl*)alloc(sizeof(JobCtl) * kMaxJobs))) return hipErrorOutOfMemory;
+    if (!(scenes = (SceneDev*)alloc(sizeof(SceneDev) * kMaxScenes))) return hipErrorOutOfMemory;
+    WR_CHECK(hipMemsetAsync(cnt, 0, sizeof(EngineCounters), st));
+    WR_CHECK(hipMemsetAsync(jobs, 0, sizeof(JobDesc) * kMaxJobs, st));
+    WR_CHECK(hipMemsetAsync(ctl, 0, sizeof(JobCtl) * kMaxJobs, st));
+    WR_CHECK(hipMemsetAsync(scenes, 0, sizeof(SceneDev) * kMaxScenes, st));
+    WR_CHECK(hipStreamSynchronize(st));
+    EngineDev E;
+    E.buf[0] = to_pathbuf(words[0]);
+    E.buf[1] = to_pathbuf(words[1]);
+    E.cnt = cnt;
+    E.jobs = jobs;
+    E.ctl = ctl;
+    E.scenes = scenes;
+    E.cap = cap;
+    // the iteration graph: kGraphIters iterations (even, the buffer parity alternates)
+    WR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int it = 0; it < kGraphIters; it++) {
+        const uint32_t par = (uint32_t)(it & 1);
+        k_wf_extend<<<grid, 256, kStackLdsBytes, st>>>(E, par);
+        k_wf_shade<<<grid, 256, kStackLdsBytes, st>>>(E, par);
+        k_wf_plan<<<1, 1, 0, st>>>(E, par);
+        k_wf_generate<<<grid, 256, 0, st>>>(E, par);
+        k_wf_fold<<<grid, 256, 0, st>>>(E);
+        k_wf_advance<<<1, 1, 0, st>>>(E, par);
+    }
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (e != hipSuccess) return e;
+    return hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+}
+
+hipError_t Engine::set_scene(uint32_t slot, const SceneDev& S, hipStream_t st) {
+    WR_CHECK(hipMemcpyAsync(scenes + slot, &S, sizeof(SceneDev), hipMemcpyHostToDevice, st));
+    return hipStreamSynchronize(st);
+}
+
+// Admits a job into a free slot (between replays, on the engine's stream).
+hipError_t Engine::admit(uint32_t slot, const JobDesc& J, hipStream_t st) {
+    WR_CHECK(hipMemcpyAsync(jobs + slot, &J, sizeof(JobDesc), hipMemcpyHostToDevice, st));
+    JobCtl c{};
+    c.state = kJobRunning;
+    WR_CHECK(hipMemcpyAsync(ctl + slot, &c, sizeof(JobCtl), hipMemcpyHostToDevice, st));
+    return hipStreamSynchronize(st);  // the sources are host stack copies
+}
+
+hipError_t Engine::release_slot(uint32_t slot, hipStream_t st) {
+    JobCtl c{};
+    WR_CHECK(hipMemcpyAsync(ctl + slot, &c, sizeof(JobCtl), hipMemcpyHostToDevice, st));
+    return hipStreamSynchronize(st);
+}
+
+// One replay of the iteration graph, then a read-back of the job states.
+hipError_t Engine::step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt) {
+    WR_CHECK(hipGraphLaunch(exec, st));
+    if (host_ctl) WR_CHECK(hipMemcpyAsync(host_ctl, ctl, sizeof(JobCtl) * kMaxJobs, hipMemcpyDeviceToHost, st));
+    if (host_cnt) WR_CHECK(hipMemcpyAsync(host_cnt, cnt, sizeof(EngineCounters), hipMemcpyDeviceToHost, st));
+    return hipStreamSynchronize(st);
+}
+
+// Fills the camera part of a job record.
+void fill_job_camera(JobDesc& J, const CamHost& c) {
+    for (int k = 0; k < 3; k++) {
+        J.pos[k] = c.pos[k];
+        J.front[k] = c.front[k];
+        J.right[k] = c.right[k];
+        J.top[k] = c.top[k];
+    }
+    J.dist = c.dist;
+    J.W = c.W;
+    J.H = c.H;
+}
+
+// Renders passes [pass0, pass0 + npass) of one film on the engine, alone.
 hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
-                         uint32_t npass, const FilmBufs& film, WavefrontPool& P, uint32_t cap, uint32_t grid,
+                         uint32_t npass, const FilmBufs& film, Engine& E, uint32_t cap, uint32_t grid,
                          WaveStats* stats, hipStream_t st) {
     const uint32_t npix = cam.W * cam.H;
     if (npass == 0 || npix == 0) return hipSuccess;
     const uint64_t total = (uint64_t)npass * npix;
-    if (cap == 0) cap = (uint32_t)std::min<uint64_t>(total, 1u << 22);
-    cap = std::max<uint32_t>(cap, 256);
-    uint32_t R = (uint32_t)std::min<uint64_t>(npass, cap / npix + 2);
-    R = std::min<uint32_t>(R, kMaxPassWindow);
-    WR_CHECK(P.ensure(cap, (size_t)3 * R * npix));
-    WaveDev W;
-    W.buf[0] = to_pathbuf(P.words[0]);
-    W.buf[1] = to_pathbuf(P.words[1]);
-    W.cnt = P.cnt;
-    W.contrib = P.contrib;
-    W.cap = cap;
-    W.npix = npix;
-    W.R = R;
-    W.npass = npass;
-    W.total = total;
-    const CamDev C = to_cam(cam);
-    const FilmDev F = to_film(film);
-    if (grid == 0) grid = 2048;
-    WaveCounters init{};
-    WR_CHECK(hipMemcpyAsync(P.cnt, &init, sizeof(init), hipMemcpyHostToDevice, st));
-    // capture kGraphIters iterations (an even number: buffer parity alternates)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    WR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int it = 0; it < kGraphIters; it++) {
-        const uint32_t par = (uint32_t)(it & 1);
-        k_wf_extend<<<grid, 256, kStackLdsBytes, st>>>(S, W, C, par);
-        k_wf_shade<<<grid, 256, kStackLdsBytes, st>>>(S, W, C, max_depth, seed, pass0, par);
-        k_wf_plan<<<1, 1, 0, st>>>(W, par);
-        k_wf_generate<<<grid, 256, 0, st>>>(W, C, seed, pass0, par);
-        k_wf_fold<<<(npix + 255) / 256, 256, 0, st>>>(W, F);
-        k_wf_advance<<<1, 1, 0, st>>>(W, par);
-    }
-    hipError_t e = hipStreamEndCapture(st, &graph);
-    if (e != hipSuccess) return e;
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-        (void)hipGraphDestroy(graph);
-        return e;
-    }
-    WaveCounters host{};
+    if (cap == 0) cap = kDefaultPathSlots;
+    WR_CHECK(E.init(cap, grid, st));
+    JobDesc J{};
+    fill_job_camera(J, cam);
+    J.scene = 0;
+    J.max_depth = max_depth;
+    J.seed = seed;
+    J.pass0 = pass0;
+    J.npass = npass;
+    J.npix = npix;
+    J.R = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(npass, E.cap / npix + 2), kMaxPassWindow);
+    float* contrib = nullptr;
+    uint32_t* pass_done = nullptr;
+    WR_CHECK(hipMalloc((void**)&contrib, sizeof(float) * 3 * (size_t)J.R * npix));
+    hipError_t e = hipMalloc((void**)&pass_done, sizeof(uint32_t) * J.R);
+    if (e == hipSuccess) e = hipMemsetAsync(pass_done, 0, sizeof(uint32_t) * J.R, st);
+    J.contrib = contrib;
+    J.pass_done = pass_done;
+    J.fsum = film.sum;
+    J.fcomp = film.comp;
+    J.fsumsq = film.sumsq;
+    if (e == hipSuccess) e = E.set_scene(0, S, st);
+    if (e == hipSuccess) e = E.admit(0, J, st);
+    std::vector<JobCtl> hc(kMaxJobs);
+    EngineCounters hcnt{};
+    const uint32_t it0 = 0;
     uint64_t replays = 0;
-    for (;;) {
-        e = hipGraphLaunch(exec, st);
-        if (e != hipSuccess) break;
+    while (e == hipSuccess) {
+        e = E.step(st, hc.data(), &hcnt);
         replays++;
-        e = hipMemcpyAsync(&host, P.cnt, sizeof(host), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess || host.cursor >= npass) break;
-        if (replays > 4 * (uint64_t)(max_depth + 3) * (total / cap + 2)) {  // cannot happen: every iteration advances every path
+        if (e != hipSuccess || hc[0].state == kJobDone) break;
+        if (replays > 4 * (uint64_t)(max_depth + 3) * (total / E.cap + 2) + 16) {  // cannot happen
             e = hipErrorUnknown;
             break;
         }
     }
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
+    if (e == hipSuccess) e = E.release_slot(0, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(contrib);
+    (void)hipFree(pass_done);
     if (e != hipSuccess) return e;
     if (stats) {
-        stats->iterations = host.iterations;
+        stats->iterations = hc[0].done_iter + 1 - it0;
         stats->paths = total;
     }
     return hipSuccess;

@@ -158,7 +158,7 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
             for (uint32_t k = 0; k < prm->spp && e == hipSuccess; k++)
                 e = launch_pt_mega(s->dev, ch, prm->max_depth, prm->seed, film->passes + k, fb, ctx->stream);
         } else {
-            e = run_wavefront(s->dev, ch, prm->max_depth, prm->seed, film->passes, prm->spp, fb, ctx->pool,
+            e = run_wavefront(s->dev, ch, prm->max_depth, prm->seed, film->passes, prm->spp, fb, ctx->engine,
                               prm->wave_slots, prm->wave_grid, &ws, ctx->stream);
         }
     }

@@ -15,7 +15,7 @@ struct wr_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
-    wr::WavefrontPool pool;  // path-slot pool of the wavefront renderer, reused across calls
+    wr::Engine engine;  // wavefront engine of wr_render, reused across calls
 };
 
 namespace wr {

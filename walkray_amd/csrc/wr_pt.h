@@ -24,18 +24,43 @@ struct FilmBufs {
     float* sumsq;
 };
 
-constexpr int kPathWords = 18;       // 32-bit words of path state per slot (SoA arrays)
-constexpr uint32_t kMaxPassWindow = 64;  // passes whose samples may be in flight at once
-constexpr int kGraphIters = 8;       // wavefront iterations per captured graph (even)
+constexpr int kPathWords = 19;           // 32-bit words of path state per slot (SoA arrays)
+constexpr uint32_t kMaxPassWindow = 64;  // passes of one job whose samples may be in flight
+constexpr uint32_t kMaxJobs = 64;        // concurrently admitted jobs
+constexpr uint32_t kMaxScenes = 64;      // resident scenes of one engine
+constexpr int kGraphIters = 8;           // wavefront iterations per graph replay (even)
+constexpr uint32_t kDefaultPathSlots = 1u << 22;  // 4M paths in flight: 2 x 304 MB of path state
 
-// Device counters of one wavefront run.
-struct WaveCounters {
-    uint32_t n[2];       // live paths in path buffer 0 / 1
-    uint32_t take, take_base;
-    uint64_t next_work, take_work;
+enum : uint32_t { kJobFree = 0, kJobRunning = 1, kJobDone = 2 };
+
+// Device record of one admitted job.
+struct JobDesc {
+    float pos[3], front[3], right[3], top[3];
+    float dist;
+    uint32_t W, H;
+    uint32_t scene, max_depth;
+    uint64_t seed;
+    uint32_t pass0, npass, npix, R;  // R: passes in flight (sample window)
+    float* contrib;                  // R * npix * 3 samples of the passes in flight
+    uint32_t* pass_done;             // R counters: finished paths per window slot
+    float *fsum, *fcomp, *fsumsq;    // the job's film
+};
+
+struct JobCtl {
+    uint64_t next_work;  // camera paths issued (pass-major)
     uint32_t cursor;     // passes folded into the film
-    uint32_t iterations;
-    uint32_t pass_done[kMaxPassWindow];
+    uint32_t state;      // kJobFree / kJobRunning / kJobDone
+    uint32_t done_iter;  // engine iteration that folded the last pass
+    uint32_t pad;
+};
+
+struct EngineCounters {
+    uint32_t n[2];  // live paths in path buffer 0 / 1
+    uint32_t iteration;
+    uint32_t rr;    // round-robin start of the planner
+    uint32_t nseg, seg_start, seg_end, pad;
+    uint32_t seg_job[kMaxJobs], seg_base[kMaxJobs], seg_take[kMaxJobs];
+    uint64_t seg_work[kMaxJobs];
 };
 
 struct WaveStats {
@@ -43,24 +68,32 @@ struct WaveStats {
     uint64_t paths = 0;
 };
 
-// Two path buffers (ping-pong), counters and the per-(pass, pixel) sample
-// window; reused across render calls of one context.
-struct WavefrontPool {
-    void* words[2][kPathWords] = {};
-    WaveCounters* cnt = nullptr;
-    float* contrib = nullptr;
-    uint32_t cap = 0;
-    size_t contrib_floats = 0;
+// The shared wavefront: path pool, job and scene tables, and the captured
+// iteration graph.  One per context (wr_render) or per server.
+struct Engine {
+    uint32_t cap = 0, grid = 0;
+    EngineCounters* cnt = nullptr;
+    JobDesc* jobs = nullptr;
+    JobCtl* ctl = nullptr;
+    SceneDev* scenes = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
     std::vector<void*> bufs;
-    ~WavefrontPool();
+    ~Engine();
     void release();
-    hipError_t ensure(uint32_t cap, size_t contrib_floats);
+    hipError_t init(uint32_t cap, uint32_t grid, hipStream_t st);
+    hipError_t set_scene(uint32_t slot, const SceneDev& S, hipStream_t st);
+    hipError_t admit(uint32_t slot, const JobDesc& J, hipStream_t st);
+    hipError_t release_slot(uint32_t slot, hipStream_t st);
+    hipError_t step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt);
 };
+
+void fill_job_camera(JobDesc& J, const CamHost& c);
 
 hipError_t launch_pt_mega(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass,
                           const FilmBufs& film, hipStream_t st);
 hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
-                         uint32_t npass, const FilmBufs& film, WavefrontPool& pool, uint32_t slots, uint32_t grid,
+                         uint32_t npass, const FilmBufs& film, Engine& engine, uint32_t cap, uint32_t grid,
                          WaveStats* stats, hipStream_t st);
 
 }  // namespace wr
