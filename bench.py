@@ -8,6 +8,11 @@ box, directions uniform on the sphere).  One step = closest hit of both ray
 sets, inputs already resident in HBM.  value = Mrays/s over the timed steps,
 measured with HIP events on the stream the kernels run on.
 
+The same JSON line also reports, outside `value`, the path-tracing workloads
+(Cornell box 256^2 x 16 spp and the 512-object field at 1920x1080 x 64 spp,
+megakernel vs wavefront samples/s) and the served-jobs workload (p50/p99
+latency and aggregate samples/s under Poisson arrivals).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pt]
 """
 from __future__ import annotations
@@ -152,6 +157,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pt", action="store_true")
     ap.add_argument("--side", type=int, default=4096)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="first run one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -161,15 +168,19 @@ def main():
     from walkray_amd import scenes as S
 
     ctx = W.Context(0)
+    if args.copy_bytes:
+        a, b = ctx.alloc(args.copy_bytes), ctx.alloc(args.copy_bytes)
+        ctx.copy(b, a, args.copy_bytes)
+        ctx.synchronize()
+        del a, b
     r = closest_hit_bench(W, S, ctx, args.steps, args.warmup, n_side=args.side)
     log(f"[bench] closest-hit: {r['mrays']:.1f} Mrays/s ({r['ms_per_step']:.3f} ms/step)")
     extra = {}
     if not args.no_pt:
-        try:
-            from walkray_amd import render as R
-            extra["path_tracing"] = R.bench_path_tracing(W, S, ctx, log)
-        except ImportError:
-            pass
+        from walkray_amd import render as R
+        from walkray_amd import serve as V
+        extra["path_tracing"] = R.bench_path_tracing(W, S, ctx, log)
+        extra["serving"] = V.bench_serving(ctx, log)
     out = {
         "metric": "Mrays/s closest-hit + path samples/s at fixed RMSE; p50/p99 job latency",
         "value": round(r["mrays"], 2),

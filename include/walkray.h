@@ -93,6 +93,8 @@ int wr_device_alloc(wr_ctx* ctx, size_t bytes, void** d_out);
 int wr_device_free(wr_ctx* ctx, void* d_ptr);
 int wr_memcpy_htod(wr_ctx* ctx, void* d_dst, const void* src, size_t bytes);
 int wr_memcpy_dtoh(wr_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+/* Asynchronous device-to-device copy on the context stream. */
+int wr_memcpy_dtod(wr_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
 /* HIP events on the context's stream, bracketing kernel work. */
 int wr_timer_start(wr_ctx* ctx);
 int wr_timer_stop(wr_ctx* ctx, float* ms);

@@ -105,6 +105,7 @@ def lib():
         "wr_device_free": (i32, [vp, vp]),
         "wr_memcpy_htod": (i32, [vp, vp, vp, C.c_size_t]),
         "wr_memcpy_dtoh": (i32, [vp, vp, vp, C.c_size_t]),
+        "wr_memcpy_dtod": (i32, [vp, vp, vp, C.c_size_t]),
         "wr_timer_start": (i32, [vp]),
         "wr_timer_stop": (i32, [vp, P(C.c_float)]),
         "wr_scene_create": (i32, [vp, P(vp)]),
@@ -224,6 +225,10 @@ class Context:
 
     def synchronize(self):
         _check(lib().wr_synchronize(self.h), self.h, "wr_synchronize")
+
+    def copy(self, dst, src, nbytes):
+        """Device-to-device copy (asynchronous on the context stream)."""
+        _check(lib().wr_memcpy_dtod(self.h, dst.ptr, src.ptr, nbytes), self.h, "wr_memcpy_dtod")
 
     def timer_start(self):
         _check(lib().wr_timer_start(self.h), self.h, "wr_timer_start")

@@ -129,6 +129,13 @@ int wr_memcpy_dtoh(wr_ctx* ctx, void* dst, const void* d_src, size_t bytes) {
     return WR_OK;
 }
 
+int wr_memcpy_dtod(wr_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
+    if (!ctx || (!d_src && bytes) || (!d_dst && bytes)) return fail(ctx, WR_EINVAL, "wr_memcpy_dtod: bad argument");
+    hipError_t e = hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpy(dtod)");
+    return WR_OK;
+}
+
 int wr_timer_start(wr_ctx* ctx) {
     if (!ctx) return fail(nullptr, WR_EINVAL, "wr_timer_start: ctx is NULL");
     hipError_t e = hipEventRecord(ctx->ev0, ctx->stream);

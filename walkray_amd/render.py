@@ -1,12 +1,15 @@
-"""Path-tracing helpers over the C ABI: image statistics, benchmark and smoke check."""
+"""Path-tracing helpers over the C ABI: image statistics, benchmarks and smoke check."""
 from __future__ import annotations
 
-import time
+import os
 
 import numpy as np
 
 from . import _lib as W
 from . import scenes as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+C1_REFERENCE = os.path.join(ROOT, "tests", "golden", "cornell256_ref.npz")
 
 
 def film_stats(state):
@@ -26,6 +29,12 @@ def rel_rmse(img, ref):
     return float(np.linalg.norm(img - ref) / max(np.linalg.norm(ref), 1e-300))
 
 
+def expected_rel_rmse(state):
+    """Expected ||I - E[I]|| / ||E[I]|| of a film from its own per-pixel sample variance."""
+    mean, se = film_stats(state)
+    return float(np.sqrt((se ** 2).sum()) / max(np.linalg.norm(mean), 1e-300))
+
+
 def render_scene(ctx, objs, cam, spp, max_depth, seed, mode, scene=None, wave_slots=0):
     sc = scene or W.Scene(ctx, objs)
     film = W.Film(ctx, cam["width"], cam["height"])
@@ -35,27 +44,47 @@ def render_scene(ctx, objs, cam, spp, max_depth, seed, mode, scene=None, wave_sl
     return out
 
 
-def bench_path_tracing(W_, S_, ctx, log, spp=16, max_depth=8, repeats=2):
-    """Cornell box (40,974 triangles), 256^2, 16 spp, depth 8: megakernel vs wavefront samples/s."""
-    objs, cam = S.cornell(k=5, width=256, height=256)
-    sc = W.Scene(ctx, objs)
-    res = {"workload": "Cornell box, 2 icospheres k=5 (40,974 triangles), 256x256, 16 spp, max depth 8, "
-                       "Russian roulette at every diffuse bounce (reference rule)"}
+def _compare(ctx, sc, cam, spp, depth, seed, log, label, repeats=2):
+    res = {}
     films = {}
     for name, mode in (("megakernel", W.WR_MEGAKERNEL), ("wavefront", W.WR_WAVEFRONT)):
         best = None
         for _ in range(repeats):
             film = W.Film(ctx, cam["width"], cam["height"])
-            st = W.render(sc, cam, film, spp, max_depth, seed=1234, mode=mode)
+            st = W.render(sc, cam, film, spp, depth, seed, mode)
             best = st["kernel_ms"] if best is None else min(best, st["kernel_ms"])
             films[name] = film.read()
         samples = cam["width"] * cam["height"] * spp
         res[name] = {"ms": round(best, 3), "samples_per_s": round(samples / (best * 1e-3), 1)}
-        log(f"[bench] path tracing {name}: {best:.2f} ms, {samples / (best * 1e-3) / 1e6:.1f} Msamples/s")
+        log(f"[bench] {label} {name}: {best:.2f} ms, {samples / (best * 1e-3) / 1e6:.1f} Msamples/s")
     res["speedup_wavefront_vs_megakernel"] = round(res["wavefront"]["samples_per_s"] /
                                                    res["megakernel"]["samples_per_s"], 3)
-    res["films_identical"] = bool(np.array_equal(films["megakernel"]["sum"], films["wavefront"]["sum"]))
-    return res
+    # same estimator and the same samples: the two films (hence their RMSE) are identical
+    res["films_bit_identical"] = bool(np.array_equal(films["megakernel"]["sum"].view(np.uint32),
+                                                     films["wavefront"]["sum"].view(np.uint32)))
+    res["expected_rel_rmse"] = round(expected_rel_rmse(films["wavefront"]), 5)
+    return res, films["wavefront"]
+
+
+def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
+    out = {}
+    objs, cam = S.cornell(k=5, width=256, height=256)
+    sc = W.Scene(ctx, objs)
+    r, film = _compare(ctx, sc, cam, 16, 8, 1234, log, "C1 cornell")
+    r["workload"] = ("Cornell box + 2 icospheres k=5 (40,974 triangles), 256x256, 16 spp, max depth 8, "
+                     "Russian roulette at every diffuse/Phong bounce (reference rule)")
+    if os.path.exists(C1_REFERENCE):
+        ref = np.load(C1_REFERENCE)
+        r["rel_rmse_vs_reference"] = round(rel_rmse(film["mean"], ref["mean"]), 5)
+        r["reference"] = f"{int(ref['spp'])}-spp render, FP64 accumulation of per-batch Kahan films"
+    out["c1"] = r
+    objs, cam = S.random_field()
+    sc = W.Scene(ctx, objs)
+    r, film = _compare(ctx, sc, cam, c3_spp, 16, 77, log, "C3 field", repeats=1)
+    r["workload"] = (f"512 random icospheres/boxes on a ground plane ({sc.info()['n_triangles']} triangles), "
+                     f"diffuse/Phong/refractive, 2 area lights, 1920x1080, {c3_spp} spp, max depth 16")
+    out["c3"] = r
+    return out
 
 
 def smoke_check(ctx):
