@@ -29,7 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 32   # one child box record (a 64 B node holds two)
+NODE_BYTES = 32   # node bytes are counted in 32 B units (a 64 B binary or wide node counts 2)
 TRI_BYTES = 48
 RAY_IN_BYTES = 24
 HIT_OUT_BYTES = 16

@@ -221,6 +221,96 @@ struct Flattener {
     }
 };
 
+// Outward quantization of one bound on the grid origin + q * scale, checked
+// against the GPU's decode (one fused multiply-add, fmaf).
+uint8_t quant_lo(float v, float origin, float scale) {
+    const double q = std::floor(((double)v - (double)origin) / (double)scale);
+    int qi = (int)std::max(0.0, std::min(255.0, q));
+    while (qi > 0 && std::fmaf((float)qi, scale, origin) > v) qi--;
+    return (uint8_t)qi;
+}
+uint8_t quant_hi(float v, float origin, float scale) {
+    const double q = std::ceil(((double)v - (double)origin) / (double)scale);
+    int qi = (int)std::max(0.0, std::min(255.0, q));
+    while (qi < 255 && std::fmaf((float)qi, scale, origin) < v) qi++;
+    return (uint8_t)qi;
+}
+
+// Collapses the flattened reference tree two levels at a time into wide
+// nodes (preorder); leaves keep their references.
+struct Collapser {
+    const FlatTree& F;
+    std::vector<wr_wnode>& out;
+
+    uint32_t collapse(uint32_t ref) {
+        if (ref & WR_LEAF_BIT) return ref;
+        uint32_t cref[4];
+        float cbox[4][6];
+        int n = 0;
+        auto put = [&](uint32_t r, const float* mn, const float* mx) {
+            cref[n] = r;
+            for (int a = 0; a < 3; a++) {
+                cbox[n][a] = mn[a];
+                cbox[n][3 + a] = mx[a];
+            }
+            n++;
+        };
+        auto add_side = [&](uint32_t r, const float* mn, const float* mx) {
+            if (r & WR_LEAF_BIT) {
+                put(r, mn, mx);
+            } else {
+                const wr_node& C = F.nodes[r];
+                put(C.lref, C.lmin, C.lmax);
+                put(C.rref, C.rmin, C.rmax);
+            }
+        };
+        const wr_node& N = F.nodes[ref];
+        add_side(N.lref, N.lmin, N.lmax);
+        add_side(N.rref, N.rmin, N.rmax);
+        const uint32_t idx = (uint32_t)out.size();
+        out.push_back(wr_wnode{});
+        float origin[3], scale[3];
+        for (int a = 0; a < 3; a++) {
+            float lo = cbox[0][a], hi = cbox[0][3 + a];
+            for (int c = 1; c < n; c++) {
+                lo = std::min(lo, cbox[c][a]);
+                hi = std::max(hi, cbox[c][3 + a]);
+            }
+            origin[a] = lo;
+            const double ext = (double)hi - (double)lo;
+            int e = 0;
+            if (ext > 0) {
+                std::frexp(ext / 255.0, &e);  // ext / 255 < 2^e
+            } else {
+                e = -100;
+            }
+            float s = std::ldexp(1.0f, std::max(e, -126));
+            while (std::fmaf(255.0f, s, origin[a]) < hi) s *= 2.0f;
+            scale[a] = s;
+        }
+        uint32_t wref[4];
+        for (int c = 0; c < n; c++) wref[c] = collapse(cref[c]);
+        wr_wnode& W = out[idx];
+        for (int a = 0; a < 3; a++) {
+            W.origin[a] = origin[a];
+            W.scale[a] = scale[a];
+        }
+        for (int c = 0; c < 4; c++) {
+            if (c < n) {
+                for (int a = 0; a < 3; a++) {
+                    W.q[a][c] = quant_lo(cbox[c][a], origin[a], scale[a]);
+                    W.q[3 + a][c] = quant_hi(cbox[c][3 + a], origin[a], scale[a]);
+                }
+                W.ref[c] = wref[c];
+            } else {
+                for (int k = 0; k < 6; k++) W.q[k][c] = 0;
+                W.ref[c] = WR_NONE;
+            }
+        }
+        return idx;
+    }
+};
+
 uint32_t internal_depth(const BNode* n) {
     if (!n->left) return 0;
     return 1 + std::max(internal_depth(n->left.get()), internal_depth(n->right.get()));
@@ -235,6 +325,7 @@ FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32
         if (idx[i] >= nv) throw std::runtime_error("triangle vertex index out of range");
     FlatTree F;
     F.hdr.root_ref = WR_NONE;
+    F.hdr.wide_root = WR_NONE;
     F.hdr.n_recs = nt;
     float scale = 0.f;
     for (size_t i = 0; i < 3 * (size_t)nv; i++) scale = std::max(scale, std::fabs(verts[i]));
@@ -289,6 +380,10 @@ FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32
     }
     F.hdr.n_nodes = (uint32_t)F.nodes.size();
     F.hdr.depth = internal_depth(&root);
+    Collapser col{F, F.wnodes};
+    F.wnodes.reserve(F.nodes.size() / 2 + 1);
+    F.hdr.wide_root = col.collapse(rref);
+    F.hdr.n_wide = (uint32_t)F.wnodes.size();
     F.n_leaves = fl.leaves;
     F.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return F;

@@ -76,6 +76,8 @@ int wr_ctx_create(int device, wr_ctx** out) {
         return hip_fail(nullptr, e, "hipStreamCreate");
     if ((e = hipEventCreate(&c->ev0)) != hipSuccess) return hip_fail(nullptr, e, "hipEventCreate");
     if ((e = hipEventCreate(&c->ev1)) != hipSuccess) return hip_fail(nullptr, e, "hipEventCreate");
+    if ((e = hipMalloc(&c->d_work, 4 * sizeof(unsigned long long))) != hipSuccess) return hip_fail(nullptr, e, "hipMalloc");
+    if ((e = hipMemset(c->d_work, 0, 4 * sizeof(unsigned long long))) != hipSuccess) return hip_fail(nullptr, e, "hipMemset");
     *out = c.release();
     return WR_OK;
 }
@@ -86,6 +88,7 @@ void wr_ctx_destroy(wr_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
+    (void)hipFree(ctx->d_work);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -222,6 +225,7 @@ int wr_scene_build(wr_scene* s, int strategy) {
         }
         // concatenate into device arrays
         std::vector<wr_node> nodes;
+        std::vector<wr_wnode> wnodes;
         std::vector<wr_tri> tris;
         std::vector<uint32_t> leaf_up, leaf_cnt, node_up, rec_prim, rec_obj;
         std::vector<float> rec_nrm;
@@ -234,6 +238,8 @@ int wr_scene_build(wr_scene* s, int strategy) {
             FlatTree& F = s->trees[ti];
             wr_tree_hdr h = F.hdr;
             h.node_base = (uint32_t)nodes.size();
+            h.wide_base = (uint32_t)wnodes.size();
+            wnodes.insert(wnodes.end(), F.wnodes.begin(), F.wnodes.end());
             h.rec_base = (uint32_t)tris.size();
             if ((uint64_t)h.rec_base + F.tris.size() > 0xFFFFFFFFull)
                 return fail(ctx, WR_EINVAL, "wr_scene_build: scene too large");
@@ -337,6 +343,7 @@ int wr_scene_build(wr_scene* s, int strategy) {
         uint64_t bytes = 0;
         uint32_t *d_cs, *d_co;
         wr_node* d_nodes;
+        wr_wnode* d_wnodes;
         wr_tri* d_tris;
         uint32_t *d_lu, *d_lc, *d_nu, *d_rp, *d_ro, *d_hn;
         float* d_rn;
@@ -345,6 +352,8 @@ int wr_scene_build(wr_scene* s, int strategy) {
         int rc;
         if ((rc = upload(ctx, nodes, &d_nodes, bytes))) return rc;
         s->dbufs.push_back(d_nodes);
+        if ((rc = upload(ctx, wnodes, &d_wnodes, bytes))) return rc;
+        s->dbufs.push_back(d_wnodes);
         if ((rc = upload(ctx, tris, &d_tris, bytes))) return rc;
         s->dbufs.push_back(d_tris);
         if ((rc = upload(ctx, leaf_up, &d_lu, bytes))) return rc;
@@ -371,6 +380,7 @@ int wr_scene_build(wr_scene* s, int strategy) {
         s->dbufs.push_back(d_co);
         SceneDev& D = s->dev;
         D.nodes = d_nodes;
+        D.wnodes = d_wnodes;
         D.tris = d_tris;
         D.leaf_up = d_lu;
         D.leaf_cnt = d_lc;
@@ -486,7 +496,7 @@ int wr_closest_hit(wr_ctx* ctx, const wr_scene* s, const wr_ray* d_rays, uint64_
     if (!ctx || !s || (n && (!d_rays || !d_hits))) return fail(ctx, WR_EINVAL, "wr_closest_hit: bad argument");
     if (!s->built) return fail(ctx, WR_ESTATE, "wr_closest_hit: scene not built");
     hipError_t e = launch_closest_hit(s->dev, s->hdrs[0], (const float*)d_rays, n, d_hits,
-                                      (unsigned long long*)d_counters, ctx->stream);
+                                      (unsigned long long*)d_counters, ctx->d_work, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "k_closest_hit launch");
     return WR_OK;
 }

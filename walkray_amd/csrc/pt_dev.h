@@ -91,7 +91,7 @@ __device__ __forceinline__ SceneCtx make_scene_ctx(const SceneDev& S) {
     SceneCtx c;
     c.S = S;
     const wr_tree_hdr H = S.hdrs[0];
-    c.tr = make_tree(H, S.nodes, S.tris, S.leaf_up, S.leaf_cnt, S.node_up);
+    c.tr = make_tree(H, S);
     c.gobj = S.rec_obj + H.rec_base;
     c.gnrm = S.rec_nrm + 9 * (size_t)H.rec_base;
     return c;

@@ -53,34 +53,114 @@ __device__ __forceinline__ void add_counters(unsigned long long* out, const Coun
     }
 }
 
+// Idle lanes of a wave are refilled once at least this many are idle.
+constexpr uint32_t kRefillMin = 8;
+
+// Fast pass of the batch closest-hit query as a persistent kernel: each wave
+// takes rays in batches of 64 from a global counter and refills a lane with
+// the next ray as soon as its traversal ends, so a wave never waits for its
+// slowest ray.  The result of the fast pass (record, t, first record of the
+// leaf, overflow flag) goes to hits[ray]; k_closest_hit_finish completes it.
+// work[0] hands out batches, work[1] counts finished waves; the last wave
+// out zeroes both for the next launch.
 template <bool COUNT>
 __global__ __launch_bounds__(256) void k_closest_hit(SceneDev S, wr_tree_hdr H, const float* __restrict__ rays,
                                                      uint64_t n, uint4* __restrict__ hits,
-                                                     unsigned long long* counters) {
+                                                     unsigned long long* counters, unsigned long long* work) {
+    const Tree tr = make_tree(H, S);
+    LaneStack st;
+    st.lane = threadIdx.x;
+    Counters cnt{0, 0, 0};
+    const NoSelf ns{};
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint64_t bnext = 0, bend = 0;  // the wave's current batch [bnext, bend)
+    bool exhausted = false;
+    bool active = false, overflow = false;
+    uint64_t rid = 0;
+    uint32_t ref = 0;
+    RayPre r{};
+    HitState h{};
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        if (!exhausted && (uint32_t)__popcll(idle) >= kRefillMin) {
+            while (idle != 0) {
+                if (bnext == bend) {
+                    unsigned long long base = 0;
+                    if (lane == 0) base = atomicAdd(work, 64ull);
+                    base = __shfl(base, 0);
+                    if (base >= n) {
+                        exhausted = true;
+                        break;
+                    }
+                    bnext = base;
+                    bend = base + 64 < n ? base + 64 : n;
+                }
+                const uint64_t avail = bend - bnext;
+                const uint64_t rank = (uint64_t)__popcll(idle & below);
+                if (!active && rank < avail) {
+                    rid = bnext + rank;
+                    const float* rp = rays + 6 * rid;
+                    r = make_ray(mk(rp[0], rp[1], rp[2]), mk(rp[3], rp[4], rp[5]), tr.scale);
+                    overflow = false;
+                    if (wide_begin(tr, r, st, h, ref)) active = true;
+                    else hits[rid] = make_uint4(WR_NONE, 0u, 0u, 0u);  // misses the root box
+                }
+                const uint64_t took = (uint64_t)__popcll(idle);
+                bnext += took < avail ? took : avail;
+                idle = __ballot(!active);
+            }
+        }
+        if (__ballot(active) == 0) break;  // only once the counter is exhausted
+        if (active && !wide_step<false, COUNT>(tr, r, ns, st, h, ref, overflow, cnt)) {
+            hits[rid] = make_uint4(h.rec, __float_as_uint(h.t), h.first, overflow ? 1u : 0u);
+            active = false;
+        }
+    }
+    if (COUNT) add_counters(counters, cnt);
+    if (lane == 0) {
+        const unsigned long long waves = (unsigned long long)gridDim.x * (blockDim.x / 64);
+        if (atomicAdd(work + 1, 1ull) == waves - 1) {
+            atomicExch(work, 0ull);
+            atomicExch(work + 1, 0ull);
+        }
+    }
+}
+
+// Completion of k_closest_hit's records: binary fast pass after a wide-stack
+// overflow, chain verification, exact fallback, barycentrics, output.  One
+// lane per ray, so the verification walks of a wave run together.
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_closest_hit_finish(SceneDev S, wr_tree_hdr H,
+                                                            const float* __restrict__ rays, uint64_t n,
+                                                            uint4* __restrict__ hits, unsigned long long* counters) {
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     const uint64_t i = (uint64_t)b * 256 + threadIdx.x;
     Counters cnt{0, 0, 0};
     LaneStack st;
     st.lane = threadIdx.x;
     if (i < n) {
-        const float* rp = rays + 6 * i;
-        const f3 o = mk(rp[0], rp[1], rp[2]);
-        const f3 d = mk(rp[3], rp[4], rp[5]);
-        const Tree tr = make_tree(H, S.nodes, S.tris, S.leaf_up, S.leaf_cnt, S.node_up);
-        const RayPre r = make_ray(o, d, tr.scale);
-        NoSelf ns{};
-        bool failed;
-        const HitState h = closest_hit<false, COUNT>(tr, r, ns, st, cnt, failed);
-        uint4 out;
-        if (h.rec != WR_NONE) {
-            float u1, u2;
-            bary(tr.tris[h.rec], o, d, h.t, u1, u2);
-            out = make_uint4(S.rec_prim[H.rec_base + h.rec], __float_as_uint(h.t), __float_as_uint(u1),
-                             __float_as_uint(u2));
-        } else {
-            out = make_uint4(WR_NONE, __float_as_uint(-1.f), 0u, 0u);
+        const uint4 f = hits[i];
+        HitState h{__uint_as_float(f.y), f.x, f.z};
+        const bool overflow = f.w != 0;
+        uint4 out = make_uint4(WR_NONE, __float_as_uint(-1.f), 0u, 0u);
+        if (h.rec != WR_NONE || overflow) {
+            const float* rp = rays + 6 * i;
+            const f3 o = mk(rp[0], rp[1], rp[2]);
+            const f3 d = mk(rp[3], rp[4], rp[5]);
+            const Tree tr = make_tree(H, S);
+            const RayPre r = make_ray(o, d, tr.scale);
+            const NoSelf ns{};
+            bool failed;
+            finish_hit<false, COUNT>(tr, r, ns, st, cnt, h, overflow, failed);
+            if (h.rec != WR_NONE) {
+                float u1, u2;
+                bary(tr.tris[h.rec], o, d, h.t, u1, u2);
+                out = make_uint4(S.rec_prim[H.rec_base + h.rec], __float_as_uint(h.t), __float_as_uint(u1),
+                                 __float_as_uint(u2));
+            }
+            if (failed) out.x = 0xFFFFFFFEu;  // stack overflow: reported, never silently wrong
         }
-        if (failed) out.x = 0xFFFFFFFEu;  // stack overflow: reported, never silently wrong
         hits[i] = out;
     }
     if (COUNT) add_counters(counters, cnt);
@@ -96,7 +176,7 @@ __global__ __launch_bounds__(256) void k_closest_hit_path(SceneDev S, wr_tree_hd
     const uint32_t* rp = rays + 8 * i;
     const f3 o = mk(__uint_as_float(rp[0]), __uint_as_float(rp[1]), __uint_as_float(rp[2]));
     const f3 d = mk(__uint_as_float(rp[3]), __uint_as_float(rp[4]), __uint_as_float(rp[5]));
-    const Tree tr = make_tree(H, S.nodes, S.tris, S.leaf_up, S.leaf_cnt, S.node_up);
+    const Tree tr = make_tree(H, S);
     const RayPre r = make_ray(o, d, tr.scale);
     NoSelf ns{rp[6], rp[7], S.rec_obj + H.rec_base, S.rec_nrm + 9 * (size_t)H.rec_base, S.obj_has_normals};
     Counters cnt{0, 0, 0};
@@ -131,12 +211,37 @@ __global__ __launch_bounds__(256) void k_inside_object(SceneDev S, const uint32_
 
 namespace wr {
 
+// Workgroups of the persistent kernel: as many as can be resident at once.
+static uint32_t persistent_blocks(uint64_t n) {
+    static uint32_t resident = 0;
+    if (!resident) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wrd::k_closest_hit<false>, 256,
+                                                         wrd::kStackLdsBytes) != hipSuccess ||
+            cus <= 0 || per_cu <= 0) {
+            cus = 256;
+            per_cu = 1;
+        }
+        resident = (uint32_t)(cus * per_cu);
+    }
+    const uint64_t need = (n + 255) / 256;
+    return (uint32_t)(need < resident ? need : resident);
+}
+
 hipError_t launch_closest_hit(const SceneDev& S, const wr_tree_hdr& H, const float* rays, uint64_t n, void* hits,
-                              unsigned long long* counters, hipStream_t st) {
+                              unsigned long long* counters, unsigned long long* work, hipStream_t st) {
     if (n == 0) return hipSuccess;
+    const uint32_t pgrid = persistent_blocks(n);
     const uint32_t grid = (uint32_t)((n + 255) / 256);
-    if (counters) wrd::k_closest_hit<true><<<grid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, counters);
-    else wrd::k_closest_hit<false><<<grid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, nullptr);
+    if (counters) {
+        wrd::k_closest_hit<true><<<pgrid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, counters, work);
+        wrd::k_closest_hit_finish<true><<<grid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, counters);
+    } else {
+        wrd::k_closest_hit<false><<<pgrid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, nullptr, work);
+        wrd::k_closest_hit_finish<false><<<grid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, nullptr);
+    }
     return hipGetLastError();
 }
 

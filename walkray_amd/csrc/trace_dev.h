@@ -36,29 +36,31 @@ namespace wrd {
 
 struct Tree {
     const wr_node* nodes;      // already offset to this tree
+    const wr_wnode* wnodes;    // already offset to this tree
     const wr_tri* tris;        // already offset to this tree
     const uint32_t* leaf_up;   // offset like tris
     const uint32_t* leaf_cnt;  // offset like tris
     const uint32_t* node_up;   // offset like nodes
     float rmin[3], rmax[3];
     uint32_t root_ref;
+    uint32_t wide_root;
     float scale;
 };
 
-__device__ __forceinline__ Tree make_tree(const wr_tree_hdr& h, const wr_node* nodes, const wr_tri* tris,
-                                          const uint32_t* leaf_up, const uint32_t* leaf_cnt,
-                                          const uint32_t* node_up) {
+__device__ __forceinline__ Tree make_tree(const wr_tree_hdr& h, const SceneDev& S) {
     Tree t;
-    t.nodes = nodes + h.node_base;
-    t.tris = tris + h.rec_base;
-    t.leaf_up = leaf_up + h.rec_base;
-    t.leaf_cnt = leaf_cnt + h.rec_base;
-    t.node_up = node_up + h.node_base;
+    t.nodes = S.nodes + h.node_base;
+    t.wnodes = S.wnodes + h.wide_base;
+    t.tris = S.tris + h.rec_base;
+    t.leaf_up = S.leaf_up + h.rec_base;
+    t.leaf_cnt = S.leaf_cnt + h.rec_base;
+    t.node_up = S.node_up + h.node_base;
     for (int a = 0; a < 3; a++) {
         t.rmin[a] = h.root_min[a];
         t.rmax[a] = h.root_max[a];
     }
     t.root_ref = h.root_ref;
+    t.wide_root = h.wide_root;
     t.scale = h.scale;
     return t;
 }
@@ -267,38 +269,96 @@ struct HitState {
     uint32_t first;  // first record of the winner's leaf (tie order)
 };
 
-// Per-lane node stack: the first LDS_DEPTH levels live in LDS (lane-interleaved,
-// stk[level * stride]); deeper levels spill to a private array (scratch), which
-// only very deep traversals touch.  Trees are limited to 64 internal levels, and
-// an ordered traversal never holds more entries than that.
+// Per-lane stack of (reference, entry distance) pairs: the first kLdsDepth
+// levels live in LDS (lane-interleaved, stk[level * kBlock + lane]); deeper
+// levels spill to a private array (scratch), which only very deep traversals
+// touch.  An entry whose entry distance exceeds the current cut when it is
+// popped is dropped unvisited: that is the slab test it would fail anyway.
 constexpr int kLdsDepth = 16;
 constexpr int kMaxDepth = 64;
 constexpr int kBlock = 256;  // every traversal kernel runs 256-lane workgroups
 
-// Dynamic LDS of the launching kernel: kLdsDepth * kBlock words.
+// Dynamic LDS of the launching kernel: 2 * kLdsDepth * kBlock words
+// (references, then entry distances).
 extern __shared__ uint32_t wr_lds_stack[];
 
 struct LaneStack {
     uint32_t lane;  // threadIdx.x
     int sp;
-    uint32_t spill[kMaxDepth - kLdsDepth];
-    // The LDS access is unconditional (a wrapped slot when spilling) so it stays
-    // a ds_read/ds_write instead of being merged with the scratch access into a
-    // generic (flat) one.
-    __device__ __forceinline__ void push(uint32_t v) {
-        if (sp >= kLdsDepth) spill[sp - kLdsDepth] = wr_lds_stack[(sp & (kLdsDepth - 1)) * kBlock + lane];
-        wr_lds_stack[(sp & (kLdsDepth - 1)) * kBlock + lane] = v;
+    uint32_t spill_ref[kMaxDepth - kLdsDepth];
+    float spill_t[kMaxDepth - kLdsDepth];
+    // The LDS accesses are unconditional (a wrapped slot when spilling) so they
+    // stay ds_read/ds_write instead of being merged with the scratch access
+    // into a generic (flat) one.
+    __device__ __forceinline__ void push(uint32_t v, float t) {
+        const uint32_t slot = (sp & (kLdsDepth - 1)) * kBlock + lane;
+        if (sp >= kLdsDepth) {
+            spill_ref[sp - kLdsDepth] = wr_lds_stack[slot];
+            spill_t[sp - kLdsDepth] = __uint_as_float(wr_lds_stack[kLdsDepth * kBlock + slot]);
+        }
+        wr_lds_stack[slot] = v;
+        wr_lds_stack[kLdsDepth * kBlock + slot] = __float_as_uint(t);
         sp++;
     }
-    __device__ __forceinline__ uint32_t pop() {
+    __device__ __forceinline__ uint32_t pop(float& t) {
         --sp;
-        const uint32_t v = wr_lds_stack[(sp & (kLdsDepth - 1)) * kBlock + lane];
-        if (sp >= kLdsDepth) wr_lds_stack[(sp & (kLdsDepth - 1)) * kBlock + lane] = spill[sp - kLdsDepth];
+        const uint32_t slot = (sp & (kLdsDepth - 1)) * kBlock + lane;
+        const uint32_t v = wr_lds_stack[slot];
+        t = __uint_as_float(wr_lds_stack[kLdsDepth * kBlock + slot]);
+        if (sp >= kLdsDepth) {
+            wr_lds_stack[slot] = spill_ref[sp - kLdsDepth];
+            wr_lds_stack[kLdsDepth * kBlock + slot] = __float_as_uint(spill_t[sp - kLdsDepth]);
+        }
         return v;
     }
+    // Pop the next entry that can still hold a hit at t <= tcut; false when
+    // the stack runs empty.
+    __device__ __forceinline__ bool next(float tcut, uint32_t& ref) {
+        while (sp > 0) {
+            float t;
+            const uint32_t v = pop(t);
+            if (t <= tcut) {
+                ref = v;
+                return true;
+            }
+        }
+        return false;
+    }
 };
-constexpr size_t kStackLdsBytes = sizeof(uint32_t) * kLdsDepth * kBlock;
+constexpr size_t kStackLdsBytes = 2 * sizeof(uint32_t) * kLdsDepth * kBlock;
 
+// Distance cut of the t-pruning: boxes entered beyond it cannot hold a
+// triangle whose reference t is <= best t (relative margin 2^-10 plus 4 e).
+__device__ __forceinline__ float prune_cut(const RayPre& r, float best_t) {
+    return best_t * 1.0009765625f + 4.f * r.e;
+}
+
+// Reference triangle tests of one leaf (kdtree.cpp:308-327), folded into h
+// with the total order (t, then larger first record, then earlier slot).
+template <bool NOSELF, bool COUNT>
+__device__ __forceinline__ void visit_leaf(const Tree& tr, const RayPre& r, const NoSelf& ns, uint32_t ref,
+                                           HitState& h, Counters& cnt) {
+    const uint32_t first = ref & WR_FIRST_MASK;
+    uint32_t c = (ref >> WR_CNT_SHIFT) & WR_CNT_MASK;
+    if (c == 0) c = tr.leaf_cnt[first];
+    const bool tie_loses = h.rec != WR_NONE && first <= h.first;
+    for (uint32_t k = 0; k < c; k++) {
+        const wr_tri T = tr.tris[first + k];
+        if (COUNT) cnt.tris++;
+        // tie_loses stays valid inside the leaf: an accepted slot makes h.first == first
+        const float t = ref_tri(T, r.o, r.d, h.t, tie_loses || (h.rec != WR_NONE && h.first == first));
+        if (t >= 0.f) {
+            if (!NOSELF || noself_legal(ns, T, first + k, r.o, r.d, t)) {
+                h.t = t;
+                h.rec = first + k;
+                h.first = first;
+            }
+        }
+    }
+}
+
+// Ordered traversal of the reference binary tree.  EXACT = false: the fast
+// conservative pass; EXACT = true: every box also passes the reference test.
 template <bool NOSELF, bool EXACT, bool COUNT>
 __device__ __forceinline__ void traverse(const Tree& tr, const RayPre& r, const NoSelf& ns, LaneStack& st,
                                          HitState& h, bool& overflow, Counters& cnt) {
@@ -317,30 +377,13 @@ __device__ __forceinline__ void traverse(const Tree& tr, const RayPre& r, const 
     uint32_t ref = tr.root_ref;
     for (;;) {
         if (ref & WR_LEAF_BIT) {
-            const uint32_t first = ref & WR_FIRST_MASK;
-            uint32_t c = (ref >> WR_CNT_SHIFT) & WR_CNT_MASK;
-            if (c == 0) c = tr.leaf_cnt[first];
-            const bool tie_loses = h.rec != WR_NONE && first <= h.first;
-            for (uint32_t k = 0; k < c; k++) {
-                const wr_tri T = tr.tris[first + k];
-                if (COUNT) cnt.tris++;
-                // tie_loses stays valid inside the leaf: an accepted slot makes h.first == first
-                const float t = ref_tri(T, r.o, r.d, h.t, tie_loses || (h.rec != WR_NONE && h.first == first));
-                if (t >= 0.f) {
-                    if (!NOSELF || noself_legal(ns, T, first + k, r.o, r.d, t)) {
-                        h.t = t;
-                        h.rec = first + k;
-                        h.first = first;
-                    }
-                }
-            }
-            if (st.sp == 0) break;
-            ref = st.pop();
+            visit_leaf<NOSELF, COUNT>(tr, r, ns, ref, h, cnt);
+            if (!st.next(prune_cut(r, h.t), ref)) break;
             continue;
         }
         const wr_node N = tr.nodes[ref];
         if (COUNT) cnt.nodes += 2;
-        const float tcut = h.t * 1.0009765625f + 4.f * r.e;
+        const float tcut = prune_cut(r, h.t);
         float tl, tr_;
         bool hl = slab(r, N.lmin[0], N.lmin[1], N.lmin[2], N.lmax[0], N.lmax[1], N.lmax[2], tcut, tl);
         bool hr = slab(r, N.rmin[0], N.rmin[1], N.rmin[2], N.rmax[0], N.rmax[1], N.rmax[2], tcut, tr_);
@@ -350,22 +393,109 @@ __device__ __forceinline__ void traverse(const Tree& tr, const RayPre& r, const 
         }
         if (hl && hr) {
             const bool left_first = tl <= tr_;
-            const uint32_t nearr = left_first ? N.lref : N.rref;
-            const uint32_t farr = left_first ? N.rref : N.lref;
             if (st.sp >= kMaxDepth) {
                 overflow = true;
                 return;
             }
-            st.push(farr);
-            ref = nearr;
+            st.push(left_first ? N.rref : N.lref, left_first ? tr_ : tl);
+            ref = left_first ? N.lref : N.rref;
         } else if (hl) {
             ref = N.lref;
         } else if (hr) {
             ref = N.rref;
-        } else {
-            if (st.sp == 0) break;
-            ref = st.pop();
+        } else if (!st.next(tcut, ref)) {
+            break;
         }
+    }
+}
+
+// Conditional swap of two (distance, reference) pairs into ascending order.
+__device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
+    const bool s = tb < ta;
+    const float t = s ? tb : ta;
+    const uint32_t r = s ? rb : ra;
+    tb = s ? ta : tb;
+    rb = s ? ra : rb;
+    ta = t;
+    ra = r;
+}
+
+// Fast conservative pass over the wide hierarchy.  Every decoded child box
+// contains the reference box of the same subtree (outward rounding on the
+// host, and the decode below is the host's fmaf), the slab test is monotone
+// in the box, and skipping the intermediate level's boxes only adds
+// candidates, so the set of triangles tested is a superset of the binary fast
+// pass's, with the same leaves and first-record tie keys.
+//
+// The pass is split into wide_begin / wide_step so a persistent kernel can
+// interleave the steps of many rays on one lane.
+__device__ __forceinline__ bool wide_begin(const Tree& tr, const RayPre& r, LaneStack& st, HitState& h,
+                                           uint32_t& ref) {
+    h.t = __builtin_inff();
+    h.rec = WR_NONE;
+    h.first = 0;
+    st.sp = 0;
+    ref = tr.wide_root;
+    if (ref == WR_NONE) return false;
+    float tn;
+    return slab(r, tr.rmin[0], tr.rmin[1], tr.rmin[2], tr.rmax[0], tr.rmax[1], tr.rmax[2], __builtin_inff(), tn);
+}
+
+// Visit `ref` (a wide node or a leaf) and pick the next reference; false when
+// the pass has ended (stack exhausted) or the stack would overflow (overflow
+// set: the caller re-runs the ray on the binary tree).
+template <bool NOSELF, bool COUNT>
+__device__ __forceinline__ bool wide_step(const Tree& tr, const RayPre& r, const NoSelf& ns, LaneStack& st,
+                                          HitState& h, uint32_t& ref, bool& overflow, Counters& cnt) {
+    if (ref & WR_LEAF_BIT) {
+        visit_leaf<NOSELF, COUNT>(tr, r, ns, ref, h, cnt);
+        return st.next(prune_cut(r, h.t), ref);
+    }
+    const wr_wnode W = tr.wnodes[ref];
+    if (COUNT) cnt.nodes += 2;  // node bytes in 32 B units, as for a binary node
+    const float tcut = prune_cut(r, h.t);
+    float t[4];
+    uint32_t c[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float lx = __builtin_fmaf((float)W.q[0][k], W.scale[0], W.origin[0]);
+        const float ly = __builtin_fmaf((float)W.q[1][k], W.scale[1], W.origin[1]);
+        const float lz = __builtin_fmaf((float)W.q[2][k], W.scale[2], W.origin[2]);
+        const float hx = __builtin_fmaf((float)W.q[3][k], W.scale[0], W.origin[0]);
+        const float hy = __builtin_fmaf((float)W.q[4][k], W.scale[1], W.origin[1]);
+        const float hz = __builtin_fmaf((float)W.q[5][k], W.scale[2], W.origin[2]);
+        float tn;
+        const bool hit = slab(r, lx, ly, lz, hx, hy, hz, tcut, tn) && W.ref[k] != WR_NONE;
+        // a hit's key is finite, a miss's +inf, so misses sort last
+        t[k] = hit ? fminf(tn, 3.0e38f) : __builtin_inff();
+        c[k] = W.ref[k];
+    }
+    cswap(t[0], c[0], t[1], c[1]);
+    cswap(t[2], c[2], t[3], c[3]);
+    cswap(t[0], c[0], t[2], c[2]);
+    cswap(t[1], c[1], t[3], c[3]);
+    cswap(t[1], c[1], t[2], c[2]);
+    if (st.sp + 3 > kMaxDepth) {
+        overflow = true;
+        return false;
+    }
+    if (t[3] != __builtin_inff()) st.push(c[3], t[3]);
+    if (t[2] != __builtin_inff()) st.push(c[2], t[2]);
+    if (t[1] != __builtin_inff()) st.push(c[1], t[1]);
+    if (t[0] != __builtin_inff()) {
+        ref = c[0];
+        return true;
+    }
+    return st.next(tcut, ref);
+}
+
+template <bool NOSELF, bool COUNT>
+__device__ __forceinline__ void traverse_wide(const Tree& tr, const RayPre& r, const NoSelf& ns, LaneStack& st,
+                                              HitState& h, bool& overflow, Counters& cnt) {
+    overflow = false;
+    uint32_t ref;
+    if (!wide_begin(tr, r, st, h, ref)) return;
+    while (wide_step<NOSELF, COUNT>(tr, r, ns, st, h, ref, overflow, cnt)) {
     }
 }
 
@@ -382,18 +512,28 @@ __device__ __forceinline__ bool verify_chain(const Tree& tr, const RayPre& r, ui
     return ref_box(r, tr.rmin, tr.rmax);
 }
 
-// Full closest-hit query: fast pass, chain verification, exact fallback.
+// Completion of a fast-pass result: the binary fast pass when the wide stack
+// overflowed, chain verification, and the exact fallback when it fails.
+template <bool NOSELF, bool COUNT>
+__device__ __forceinline__ void finish_hit(const Tree& tr, const RayPre& r, const NoSelf& ns, LaneStack& st,
+                                           Counters& cnt, HitState& h, bool overflow, bool& failed) {
+    failed = false;
+    if (overflow) traverse<NOSELF, false, COUNT>(tr, r, ns, st, h, overflow, cnt);
+    if (!overflow && (h.rec == WR_NONE || verify_chain(tr, r, h.first))) return;
+    if (COUNT) cnt.fallback++;
+    traverse<NOSELF, true, COUNT>(tr, r, ns, st, h, overflow, cnt);
+    failed = overflow;
+}
+
+// Full closest-hit query: fast pass (wide hierarchy; the binary tree if the
+// wide stack overflows), chain verification, exact fallback.
 template <bool NOSELF, bool COUNT>
 __device__ __forceinline__ HitState closest_hit(const Tree& tr, const RayPre& r, const NoSelf& ns, LaneStack& st,
                                                 Counters& cnt, bool& failed) {
     HitState h;
     bool overflow;
-    failed = false;
-    traverse<NOSELF, false, COUNT>(tr, r, ns, st, h, overflow, cnt);
-    if (!overflow && (h.rec == WR_NONE || verify_chain(tr, r, h.first))) return h;
-    if (COUNT) cnt.fallback++;
-    traverse<NOSELF, true, COUNT>(tr, r, ns, st, h, overflow, cnt);
-    failed = overflow;
+    traverse_wide<NOSELF, COUNT>(tr, r, ns, st, h, overflow, cnt);
+    finish_hit<NOSELF, COUNT>(tr, r, ns, st, cnt, h, overflow, failed);
     return h;
 }
 
@@ -422,7 +562,7 @@ __device__ __forceinline__ uint32_t find_inside(const SceneDev& S, const f3& o, 
               ob.aabb_min[2] <= o.z && o.z <= ob.aabb_max[2]))
             continue;
         const wr_tree_hdr H = S.hdrs[ob.tree];
-        const Tree tr = make_tree(H, S.nodes, S.tris, S.leaf_up, S.leaf_cnt, S.node_up);
+        const Tree tr = make_tree(H, S);
         bool failed;
         const float* nrm_base = ob.has_normals ? S.rec_nrm + 9 * (size_t)H.rec_base : nullptr;
         {

@@ -15,6 +15,7 @@ struct wr_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
+    unsigned long long* d_work = nullptr;  // batch counter of the persistent closest-hit kernel (self-resetting)
     wr::Engine engine;  // wavefront engine of wr_render, reused across calls
 };
 

@@ -17,6 +17,7 @@ struct FlatTree {
     std::vector<uint32_t> leaf_up;     // first record of a leaf -> (parent << 1 | side); WR_NONE for a root leaf
     std::vector<uint32_t> leaf_cnt;    // first record of a long leaf -> triangle count
     std::vector<uint32_t> node_up;     // internal node -> (parent << 1 | side); WR_NONE for the root
+    std::vector<wr_wnode> wnodes;      // fast-pass hierarchy (the same leaves, four children per node)
     wr_tree_hdr hdr{};
     uint32_t n_leaves = 0;
     double build_seconds = 0.0;

@@ -9,6 +9,7 @@
 // followed by every object's own tree (offsets in the tree headers).
 struct SceneDev {
     const wr_node* nodes;
+    const wr_wnode* wnodes;          // fast-pass hierarchy of every tree (offsets in the headers)
     const wr_tri* tris;
     const uint32_t* leaf_up;
     const uint32_t* leaf_cnt;
@@ -43,8 +44,10 @@ inline uint32_t wr_grid_cell(float x, float gmin, float inv, uint32_t dim) {
 }
 
 namespace wr {
+// work: two zeroed 64-bit words owned by the caller's stream; the kernel
+// leaves them zeroed again when it ends.
 hipError_t launch_closest_hit(const SceneDev& S, const wr_tree_hdr& H, const float* rays, uint64_t n, void* hits,
-                              unsigned long long* counters, hipStream_t st);
+                              unsigned long long* counters, unsigned long long* work, hipStream_t st);
 hipError_t launch_closest_hit_path(const SceneDev& S, const wr_tree_hdr& H, const uint32_t* rays, uint64_t n,
                                    void* hits, hipStream_t st);
 hipError_t launch_inside_object(const SceneDev& S, uint32_t max_obj_depth, const uint32_t* rays, uint64_t n,

@@ -44,6 +44,21 @@ struct wr_tri {
 };
 static_assert(sizeof(wr_tri) == 48, "triangle record must be 48 bytes");
 
+// 64 B wide node of the fast-pass hierarchy: the reference tree collapsed two
+// levels at a time (up to four children: grandchildren, or a child that is a
+// leaf).  Child boxes are 8-bit coordinates on the grid origin + q * scale
+// (scale a power of two), rounded outward on the host so the decoded box
+// contains the reference box.  q[k][c]: k = 0..2 min x,y,z, 3..5 max x,y,z of
+// child c.  ref[c]: wide-node index, a leaf reference (as in wr_node), or
+// WR_NONE.
+struct wr_wnode {
+    float origin[3];
+    float scale[3];
+    uint8_t q[6][4];
+    uint32_t ref[4];
+};
+static_assert(sizeof(wr_wnode) == 64, "wide node must be 64 bytes");
+
 // Per-tree header.  root_ref may itself be a leaf.
 struct wr_tree_hdr {
     float root_min[3];
@@ -55,7 +70,9 @@ struct wr_tree_hdr {
     uint32_t n_recs;
     uint32_t depth;       // max number of internal nodes on a root-to-leaf path
     float scale;          // max |coordinate| over the tree's vertices
-    uint32_t pad;
+    uint32_t wide_root;   // root of the wide hierarchy (wide-node index or leaf reference)
+    uint32_t wide_base;   // offset of this tree's wide nodes
+    uint32_t n_wide;
 };
 
 // Object (material) record; mat follows the reference object classes.
