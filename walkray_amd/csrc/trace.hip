@@ -54,7 +54,10 @@ __device__ __forceinline__ void add_counters(unsigned long long* out, const Coun
 }
 
 // Idle lanes of a wave are refilled once at least this many are idle.
-constexpr uint32_t kRefillMin = 8;
+constexpr uint32_t kRefillMin = 16;
+// A wave runs node-only steps while at least this many lanes are at a wide
+// node: a mixed step pays for both the node code and the triangle code.
+constexpr uint32_t kNodePhaseMin = 32;
 
 // Fast pass of the batch closest-hit query as a persistent kernel: each wave
 // takes rays in batches of 64 from a global counter and refills a lane with
@@ -62,9 +65,10 @@ constexpr uint32_t kRefillMin = 8;
 // slowest ray.  The result of the fast pass (record, t, first record of the
 // leaf, overflow flag) goes to hits[ray]; k_closest_hit_finish completes it.
 // work[0] hands out batches, work[1] counts finished waves; the last wave
-// out zeroes both for the next launch.
+// out zeroes both for the next launch.  Compiled for 8 waves per SIMD (64
+// VGPRs): more waves in flight hide more of the node-fetch latency.
 template <bool COUNT>
-__global__ __launch_bounds__(256) void k_closest_hit(SceneDev S, wr_tree_hdr H, const float* __restrict__ rays,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_closest_hit(SceneDev S, wr_tree_hdr H, const float* __restrict__ rays,
                                                      uint64_t n, uint4* __restrict__ hits,
                                                      unsigned long long* counters, unsigned long long* work) {
     const Tree tr = make_tree(H, S);
@@ -112,6 +116,16 @@ __global__ __launch_bounds__(256) void k_closest_hit(SceneDev S, wr_tree_hdr H, 
             }
         }
         if (__ballot(active) == 0) break;  // only once the counter is exhausted
+        // node phase: while at least kNodePhaseMin lanes sit at wide nodes, only
+        // those lanes step (no leaf code runs); then one step for every lane
+        for (;;) {
+            const bool at_node = active && !(ref & WR_LEAF_BIT);
+            if ((uint32_t)__popcll(__ballot(at_node)) < kNodePhaseMin) break;
+            if (at_node && !wide_step<false, COUNT>(tr, r, ns, st, h, ref, overflow, cnt)) {
+                hits[rid] = make_uint4(h.rec, __float_as_uint(h.t), h.first, overflow ? 1u : 0u);
+                active = false;
+            }
+        }
         if (active && !wide_step<false, COUNT>(tr, r, ns, st, h, ref, overflow, cnt)) {
             hits[rid] = make_uint4(h.rec, __float_as_uint(h.t), h.first, overflow ? 1u : 0u);
             active = false;

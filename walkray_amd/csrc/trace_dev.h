@@ -274,7 +274,10 @@ struct HitState {
 // levels spill to a private array (scratch), which only very deep traversals
 // touch.  An entry whose entry distance exceeds the current cut when it is
 // popped is dropped unvisited: that is the slab test it would fail anyway.
-constexpr int kLdsDepth = 16;
+// Eight LDS levels keep a 256-lane workgroup at 16 KB of LDS, so LDS never
+// holds occupancy below 8 waves per SIMD (traversal is latency-bound: waves
+// in flight matter more than the occasional spilled entry).
+constexpr int kLdsDepth = 8;
 constexpr int kMaxDepth = 64;
 constexpr int kBlock = 256;  // every traversal kernel runs 256-lane workgroups
 
