@@ -13,7 +13,10 @@ The same JSON line also reports, outside `value`, the path-tracing workloads
 megakernel vs wavefront samples/s) and the served-jobs workload (p50/p99
 latency and aggregate samples/s under Poisson arrivals).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pt]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pt] [--legs ch,pt,serve]
+
+--legs picks the workloads (closest hit, path tracing, serving); a run
+without the closest-hit leg reports value null (used to profile one leg).
 """
 from __future__ import annotations
 
@@ -104,13 +107,13 @@ def closest_hit_bench(W, S, ctx, steps, warmup, n_side=4096, hf_n=1024):
             "tri_records_per_ray": counters[name]["tri_records"] / n,
             "fallback_rays": counters[name]["fallback_rays"],
         }
-    # the dominant kernel is k_closest_hit, launched twice per step (one launch per ray set):
-    # per-launch algorithmic bytes and duration are the averages over the two launches
+    # one wr_closest_hit call (persistent traversal kernel + finish kernel) per ray set, two per
+    # step: per-call algorithmic bytes and duration are the averages over the two calls
     total_b = alg_bytes(counters["coherent"]) + alg_bytes(counters["incoherent"])
     avg_launch_ms = kern_ms / steps / 2
     roofline = {
         "bound": "hbm",
-        "kernel": "k_closest_hit",
+        "kernel": "k_closest_hit + k_closest_hit_finish (the two kernels of one wr_closest_hit call)",
         "achieved": round(total_b / 2 / (avg_launch_ms * 1e-3) / 1e9, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -155,7 +158,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-pt", action="store_true")
+    ap.add_argument("--no-pt", action="store_true", help="same as --legs ch")
+    ap.add_argument("--legs", default="ch,pt,serve")
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (counter calibration)")
@@ -173,22 +177,26 @@ def main():
         ctx.copy(b, a, args.copy_bytes)
         ctx.synchronize()
         del a, b
-    r = closest_hit_bench(W, S, ctx, args.steps, args.warmup, n_side=args.side)
-    log(f"[bench] closest-hit: {r['mrays']:.1f} Mrays/s ({r['ms_per_step']:.3f} ms/step)")
+    legs = {"ch"} if args.no_pt else set(args.legs.split(","))
+    r = None
+    if "ch" in legs:
+        r = closest_hit_bench(W, S, ctx, args.steps, args.warmup, n_side=args.side)
+        log(f"[bench] closest-hit: {r['mrays']:.1f} Mrays/s ({r['ms_per_step']:.3f} ms/step)")
     extra = {}
-    if not args.no_pt:
+    if "pt" in legs:
         from walkray_amd import render as R
-        from walkray_amd import serve as V
         extra["path_tracing"] = R.bench_path_tracing(W, S, ctx, log)
+    if "serve" in legs:
+        from walkray_amd import serve as V
         extra["serving"] = V.bench_serving(ctx, log)
     out = {
         "metric": "Mrays/s closest-hit + path samples/s at fixed RMSE; p50/p99 job latency",
-        "value": round(r["mrays"], 2),
+        "value": round(r["mrays"], 2) if r else None,
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(r["ms_per_step"], 4),
+        "ms_per_step": round(r["ms_per_step"], 4) if r else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -198,15 +206,16 @@ def main():
             "workload": "closest-hit ray queries: 1024x1024 fBm heightfield (2,093,058 triangles, reference "
                         "builder); 16,777,216 coherent 4096^2 pinhole rays + 16,777,216 incoherent rays; "
                         "outputs prim id, t, barycentrics (FP32)",
-            "triangles": r["info"]["n_triangles"],
-            "rays_per_step": 2 * r["n"],
+            "triangles": r["info"]["n_triangles"] if r else None,
+            "rays_per_step": 2 * r["n"] if r else None,
         },
-        "roofline": r["roofline"],
-        "closest_hit": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
-                        for k, v in r["per_set"].items()},
     }
+    if r:
+        out["roofline"] = r["roofline"]
+        out["closest_hit"] = {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                              for k, v in r["per_set"].items()}
     out.update(extra)
-    if not args.no_cpu:
+    if r and not args.no_cpu:
         log("[bench] cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(S, r["objs"], r["coh"], r["inc"])
         log(f"[bench] cpu baseline: {out['cpu_baseline']['value']} Mrays/s on {out['cpu_baseline']['cores']} cores")

@@ -245,7 +245,13 @@ int wr_scene_build(wr_scene* s, int strategy) {
                 return fail(ctx, WR_EINVAL, "wr_scene_build: scene too large");
             hdrs.push_back(h);
             nodes.insert(nodes.end(), F.nodes.begin(), F.nodes.end());
-            node_up.insert(node_up.end(), F.node_up.begin(), F.node_up.end());
+            for (size_t nd = 0; nd < F.node_up.size(); nd++) {  // next four ancestor links of every node
+                uint32_t up = F.node_up[nd];
+                for (int k = 0; k < 4; k++) {
+                    node_up.push_back(up);
+                    if (up != WR_NONE) up = F.node_up[up >> 1];
+                }
+            }
             tris.insert(tris.end(), F.tris.begin(), F.tris.end());
             leaf_up.insert(leaf_up.end(), F.leaf_up.begin(), F.leaf_up.end());
             leaf_cnt.insert(leaf_cnt.end(), F.leaf_cnt.begin(), F.leaf_cnt.end());
@@ -384,7 +390,7 @@ int wr_scene_build(wr_scene* s, int strategy) {
         D.tris = d_tris;
         D.leaf_up = d_lu;
         D.leaf_cnt = d_lc;
-        D.node_up = d_nu;
+        D.node_anc = (const uint4*)d_nu;
         D.rec_prim = d_rp;
         D.rec_obj = d_ro;
         D.rec_nrm = d_rn;

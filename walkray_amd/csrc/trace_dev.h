@@ -40,7 +40,7 @@ struct Tree {
     const wr_tri* tris;        // already offset to this tree
     const uint32_t* leaf_up;   // offset like tris
     const uint32_t* leaf_cnt;  // offset like tris
-    const uint32_t* node_up;   // offset like nodes
+    const uint4* node_anc;     // offset like nodes
     float rmin[3], rmax[3];
     uint32_t root_ref;
     uint32_t wide_root;
@@ -54,7 +54,7 @@ __device__ __forceinline__ Tree make_tree(const wr_tree_hdr& h, const SceneDev& 
     t.tris = S.tris + h.rec_base;
     t.leaf_up = S.leaf_up + h.rec_base;
     t.leaf_cnt = S.leaf_cnt + h.rec_base;
-    t.node_up = S.node_up + h.node_base;
+    t.node_anc = S.node_anc + h.node_base;
     for (int a = 0; a < 3; a++) {
         t.rmin[a] = h.root_min[a];
         t.rmax[a] = h.root_max[a];
@@ -502,15 +502,35 @@ __device__ __forceinline__ void traverse_wide(const Tree& tr, const RayPre& r, c
     }
 }
 
-// Re-check the winner's root-to-leaf chain with the reference box test.
+// Re-check the winner's root-to-leaf chain with the reference box test.  The
+// links come four at a time (node_anc), so the node fetches of four levels
+// are issued together instead of one dependent fetch per level.
+__device__ __forceinline__ bool chain_box(const Tree& tr, const RayPre& r, uint32_t up) {
+    const wr_node& N = tr.nodes[up >> 1];
+    return (up & 1u) ? ref_box(r, N.rmin, N.rmax) : ref_box(r, N.lmin, N.lmax);
+}
+
 __device__ __forceinline__ bool verify_chain(const Tree& tr, const RayPre& r, uint32_t first) {
-    uint32_t up = tr.leaf_up[first];
-    while (up != WR_NONE) {
-        const uint32_t node = up >> 1;
-        const wr_node N = tr.nodes[node];
-        const bool ok = (up & 1u) ? ref_box(r, N.rmin, N.rmax) : ref_box(r, N.lmin, N.lmax);
-        if (!ok) return false;
-        up = tr.node_up[node];
+    const uint32_t up = tr.leaf_up[first];
+    if (up != WR_NONE) {
+        if (!chain_box(tr, r, up)) return false;
+        uint32_t node = up >> 1;
+        for (;;) {
+            const uint4 a = tr.node_anc[node];
+            if (a.x == WR_NONE) break;
+            const wr_node N0 = tr.nodes[a.x >> 1];
+            const wr_node N1 = tr.nodes[a.y == WR_NONE ? 0 : a.y >> 1];
+            const wr_node N2 = tr.nodes[a.z == WR_NONE ? 0 : a.z >> 1];
+            const wr_node N3 = tr.nodes[a.w == WR_NONE ? 0 : a.w >> 1];
+            if (!((a.x & 1u) ? ref_box(r, N0.rmin, N0.rmax) : ref_box(r, N0.lmin, N0.lmax))) return false;
+            if (a.y == WR_NONE) break;
+            if (!((a.y & 1u) ? ref_box(r, N1.rmin, N1.rmax) : ref_box(r, N1.lmin, N1.lmax))) return false;
+            if (a.z == WR_NONE) break;
+            if (!((a.z & 1u) ? ref_box(r, N2.rmin, N2.rmax) : ref_box(r, N2.lmin, N2.lmax))) return false;
+            if (a.w == WR_NONE) break;
+            if (!((a.w & 1u) ? ref_box(r, N3.rmin, N3.rmax) : ref_box(r, N3.lmin, N3.lmax))) return false;
+            node = a.w >> 1;
+        }
     }
     return ref_box(r, tr.rmin, tr.rmax);
 }
