@@ -131,6 +131,14 @@ int wr_tree_build(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t
 int wr_tree_export(const wr_tree* tree, uint32_t* n_nodes, uint32_t* n_leaf_tris, uint32_t* depth, float* box,
                    uint32_t* right, uint32_t* leaf_first, uint32_t* leaf_count, uint32_t* tri_list);
 void wr_tree_free(wr_tree* tree);
+/* Host-only: the fast-pass hierarchy the GPU traverses for this tree (no
+ * counterpart in the reference; exposed so its invariants can be checked
+ * without a GPU).  wide_nodes receives n_wide 64-byte nodes (wr_wnode of
+ * walkray_amd/csrc/wr_layout.h); record_first[r] is the first record of the
+ * reference leaf holding record r, record_tri[r] the input triangle of record
+ * r.  Call with NULL arrays to get the sizes first. */
+int wr_tree_wide(const wr_tree* tree, uint32_t* n_wide, uint32_t* n_records, uint32_t* wide_root, void* wide_nodes,
+                 uint32_t* record_first, uint32_t* record_tri);
 
 /* ---- ray queries ------------------------------------------------------- */
 /* KDTree::intersect(ray, triID) without Condition (kdtree.cpp:349-353) for n

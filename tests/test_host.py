@@ -104,3 +104,68 @@ def test_icosphere_and_box_winding_outward():
         a, b, cc = v[o["tris"][:, 0]], v[o["tris"][:, 1]], v[o["tris"][:, 2]]
         n = np.cross(b - a, cc - a)
         assert np.all(np.einsum("ij,ij->i", n, (a + b + cc) / 3 - c) > 0)
+
+
+def _wide_walk(wide, V, T):
+    """Walks the fast-pass hierarchy; returns {record: number of visits} and checks, for every
+    child box, that it contains the boxes of all triangles below it (decode: fma(q, scale, origin))."""
+    nodes = wide["nodes"]
+    tri_lo = V[T].min(axis=1)
+    tri_hi = V[T].max(axis=1)
+    rec_tri = wide["record_tri"]
+    seen = {}
+
+    def below(ref):  # records under a reference
+        if ref & 0x80000000:
+            assert (ref >> 27) & 0xF == 1, "wide leaves hold one triangle"
+            rec = ref & 0x07FFFFFF
+            seen[rec] = seen.get(rec, 0) + 1
+            return [rec]
+        nd = nodes[ref]
+        out = []
+        for c in range(4):
+            r = int(nd["ref"][c])
+            if r == 0xFFFFFFFF:
+                continue
+            recs = below(r)
+            q = nd["q"][:, c].astype(np.float64)
+            box = (q * nd["scale"][[0, 1, 2, 0, 1, 2]].astype(np.float64)
+                   + nd["origin"][[0, 1, 2, 0, 1, 2]].astype(np.float64)).astype(np.float32)
+            tris = rec_tri[recs]
+            assert np.all(box[:3] <= tri_lo[tris].min(axis=0)), "child box must contain its triangles"
+            assert np.all(box[3:] >= tri_hi[tris].max(axis=0)), "child box must contain its triangles"
+            out += recs
+        return out
+
+    below(int(wide["root"]))
+    return seen
+
+
+@pytest.mark.parametrize("case", ["heightfield", "cornell", "field", "soup", "tiny"])
+def test_wide_hierarchy_covers_every_record_once(case):
+    if case == "heightfield":
+        objs = [S.heightfield(65, seed=4)]
+    elif case == "cornell":
+        objs = S.cornell(k=2)[0]
+    elif case == "field":
+        objs = S.random_field(n_objects=12, seed=1)[0]
+    elif case == "soup":
+        rng = np.random.default_rng(2)
+        v = np.round(rng.normal(size=(300, 3)), 1).astype(np.float32)
+        t = rng.integers(0, 300, (500, 3)).astype(np.uint32)
+        objs = [{"verts": v, "tris": t, "normals": None, "mat": 0, "params": (0.5,) * 3}]
+    else:
+        objs = [{"verts": np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32),
+                 "tris": np.array([[0, 1, 2]], np.uint32), "normals": None, "mat": 0, "params": (0.5,) * 3}]
+    V, T = _soup(objs)
+    wide = W.build_tree_wide(V, T)
+    ex = W.build_tree_export(V, T)
+    # records are the reference leaves' triangles in left-to-right leaf order
+    assert np.array_equal(wide["record_tri"], ex["tri_list"])
+    seen = _wide_walk(wide, V, T)
+    assert sorted(seen) == list(range(len(ex["tri_list"])))
+    assert set(seen.values()) == {1}
+    # record -> first record of its reference leaf (the tie key)
+    leaves = ex["right"] == 0xFFFFFFFF
+    for f, c in zip(ex["leaf_first"][leaves], ex["leaf_count"][leaves]):
+        assert np.all(wide["record_first"][f:f + c] == f)

@@ -118,6 +118,7 @@ def lib():
         "wr_tree_build": (i32, [vp, u32, vp, u32, i32, P(vp)]),
         "wr_tree_export": (i32, [vp, P(u32), P(u32), P(u32), vp, vp, vp, vp, vp]),
         "wr_tree_free": (None, [vp]),
+        "wr_tree_wide": (i32, [vp, P(u32), P(u32), P(u32), vp, vp, vp]),
         "wr_closest_hit": (i32, [vp, vp, vp, u64, vp, vp]),
         "wr_closest_hit_noself": (i32, [vp, vp, vp, u64, vp]),
         "wr_inside_object": (i32, [vp, vp, vp, u64, vp]),
@@ -173,6 +174,31 @@ def build_tree_export(verts, tris, nthreads=0):
                                 _p(tl)), None, "wr_tree_export")
         return {"box": box, "right": right, "leaf_first": lf, "leaf_count": lc, "tri_list": tl[:m.value],
                 "depth": d.value}
+    finally:
+        L.wr_tree_free(h)
+
+
+WNODE_DTYPE = np.dtype([("origin", np.float32, 3), ("scale", np.float32, 3), ("q", np.uint8, (6, 4)),
+                        ("ref", np.uint32, 4)])
+
+
+def build_tree_wide(verts, tris, nthreads=0):
+    """Host-only builder run; returns the fast-pass hierarchy (wide nodes, root, record tables)."""
+    L = lib()
+    v = np.ascontiguousarray(verts, dtype=np.float32)
+    t = np.ascontiguousarray(tris, dtype=np.uint32)
+    h = C.c_void_p()
+    _check(L.wr_tree_build(_p(v), len(v), _p(t), len(t), nthreads, C.byref(h)), None, "wr_tree_build")
+    try:
+        nw, nr, root = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(L.wr_tree_wide(h, C.byref(nw), C.byref(nr), C.byref(root), None, None, None), None, "wr_tree_wide")
+        nodes = np.zeros(max(nw.value, 1), WNODE_DTYPE)
+        rf = np.zeros(max(nr.value, 1), np.uint32)
+        rt = np.zeros(max(nr.value, 1), np.uint32)
+        _check(L.wr_tree_wide(h, C.byref(nw), C.byref(nr), C.byref(root), _p(nodes), _p(rf), _p(rt)), None,
+               "wr_tree_wide")
+        return {"nodes": nodes[:nw.value], "root": root.value, "record_first": rf[:nr.value],
+                "record_tri": rt[:nr.value]}
     finally:
         L.wr_tree_free(h)
 

@@ -23,6 +23,7 @@
 // and in left-to-right leaf order, which is exactly the record order the GPU
 // layout wants.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -237,13 +238,126 @@ uint8_t quant_hi(float v, float origin, float scale) {
 }
 
 // Collapses the flattened reference tree two levels at a time into wide
-// nodes (preorder); leaves keep their references.
+// nodes (preorder).  A reference leaf with two or more triangles becomes a
+// small wide subtree whose children are single triangles, each under its own
+// box, so the fast pass box-tests triangles before running the triangle test;
+// rec_first keeps every record's reference leaf (the tie order's key).
 struct Collapser {
-    const FlatTree& F;
+    FlatTree& F;
     std::vector<wr_wnode>& out;
 
+    static uint32_t tri_ref(uint32_t rec) { return WR_LEAF_BIT | (1u << WR_CNT_SHIFT) | rec; }
+
+    void tri_box(uint32_t rec, float* box) const {
+        const wr_tri& T = F.tris[rec];
+        for (int a = 0; a < 3; a++) {
+            box[a] = std::min(std::min(T.v0[a], T.v1[a]), T.v2[a]);
+            box[3 + a] = std::max(std::max(T.v0[a], T.v1[a]), T.v2[a]);
+        }
+    }
+
+    // One wide node over n <= 4 children; child_ref(c) is called after the
+    // node's slot is taken (preorder) and returns the child's final reference.
+    template <class ChildRef>
+    uint32_t make_node(int n, const float (*cbox)[6], ChildRef child_ref) {
+        const uint32_t idx = (uint32_t)out.size();
+        out.push_back(wr_wnode{});
+        float origin[3], scale[3];
+        for (int a = 0; a < 3; a++) {
+            float lo = cbox[0][a], hi = cbox[0][3 + a];
+            for (int c = 1; c < n; c++) {
+                lo = std::min(lo, cbox[c][a]);
+                hi = std::max(hi, cbox[c][3 + a]);
+            }
+            origin[a] = lo;
+            const double ext = (double)hi - (double)lo;
+            int e = 0;
+            if (ext > 0) {
+                std::frexp(ext / 255.0, &e);  // ext / 255 < 2^e
+            } else {
+                e = -100;
+            }
+            float s = std::ldexp(1.0f, std::max(e, -126));
+            while (std::fmaf(255.0f, s, origin[a]) < hi) s *= 2.0f;
+            scale[a] = s;
+        }
+        uint32_t wref[4];
+        for (int c = 0; c < n; c++) wref[c] = child_ref(c);
+        wr_wnode& W = out[idx];
+        for (int a = 0; a < 3; a++) {
+            W.origin[a] = origin[a];
+            W.scale[a] = scale[a];
+        }
+        for (int c = 0; c < 4; c++) {
+            if (c < n) {
+                for (int a = 0; a < 3; a++) {
+                    W.q[a][c] = quant_lo(cbox[c][a], origin[a], scale[a]);
+                    W.q[3 + a][c] = quant_hi(cbox[c][3 + a], origin[a], scale[a]);
+                }
+                W.ref[c] = wref[c];
+            } else {
+                for (int k = 0; k < 6; k++) W.q[k][c] = 0;
+                W.ref[c] = WR_NONE;
+            }
+        }
+        return idx;
+    }
+
+    // Wide subtree over the triangles recs (at least two): up to four per
+    // node, larger sets split into four runs along the widest centroid axis.
+    uint32_t tri_tree(std::vector<uint32_t> recs) {
+        const int n = (int)recs.size();
+        float cbox[4][6];
+        if (n <= 4) {
+            for (int c = 0; c < n; c++) tri_box(recs[c], cbox[c]);
+            return make_node(n, cbox, [&](int c) { return tri_ref(recs[c]); });
+        }
+        std::vector<std::array<float, 3>> cen(n);
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int i = 0; i < n; i++) {
+            float b[6];
+            tri_box(recs[i], b);
+            for (int a = 0; a < 3; a++) {
+                cen[i][a] = 0.5f * (b[a] + b[3 + a]);
+                lo[a] = std::min(lo[a], cen[i][a]);
+                hi[a] = std::max(hi[a], cen[i][a]);
+            }
+        }
+        int ax = 0;
+        for (int a = 1; a < 3; a++)
+            if (hi[a] - lo[a] > hi[ax] - lo[ax]) ax = a;
+        std::vector<int> ord(n);
+        for (int i = 0; i < n; i++) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cen[x][ax] < cen[y][ax]; });
+        std::vector<uint32_t> part[4];
+        for (int k = 0; k < n; k++) part[(size_t)k * 4 / n].push_back(recs[ord[k]]);
+        for (int c = 0; c < 4; c++) {
+            tri_box(part[c][0], cbox[c]);
+            for (size_t i = 1; i < part[c].size(); i++) {
+                float b[6];
+                tri_box(part[c][i], b);
+                for (int a = 0; a < 3; a++) {
+                    cbox[c][a] = std::min(cbox[c][a], b[a]);
+                    cbox[c][3 + a] = std::max(cbox[c][3 + a], b[3 + a]);
+                }
+            }
+        }
+        return make_node(4, cbox, [&](int c) {
+            return part[c].size() == 1 ? tri_ref(part[c][0]) : tri_tree(part[c]);
+        });
+    }
+
     uint32_t collapse(uint32_t ref) {
-        if (ref & WR_LEAF_BIT) return ref;
+        if (ref & WR_LEAF_BIT) {
+            const uint32_t first = ref & WR_FIRST_MASK;
+            uint32_t cnt = (ref >> WR_CNT_SHIFT) & WR_CNT_MASK;
+            if (cnt == 0) cnt = F.leaf_cnt[first];
+            for (uint32_t k = 0; k < cnt; k++) F.rec_first[first + k] = first;
+            if (cnt <= 1) return ref;
+            std::vector<uint32_t> recs(cnt);
+            for (uint32_t k = 0; k < cnt; k++) recs[k] = first + k;
+            return tri_tree(std::move(recs));
+        }
         uint32_t cref[4];
         float cbox[4][6];
         int n = 0;
@@ -267,47 +381,7 @@ struct Collapser {
         const wr_node& N = F.nodes[ref];
         add_side(N.lref, N.lmin, N.lmax);
         add_side(N.rref, N.rmin, N.rmax);
-        const uint32_t idx = (uint32_t)out.size();
-        out.push_back(wr_wnode{});
-        float origin[3], scale[3];
-        for (int a = 0; a < 3; a++) {
-            float lo = cbox[0][a], hi = cbox[0][3 + a];
-            for (int c = 1; c < n; c++) {
-                lo = std::min(lo, cbox[c][a]);
-                hi = std::max(hi, cbox[c][3 + a]);
-            }
-            origin[a] = lo;
-            const double ext = (double)hi - (double)lo;
-            int e = 0;
-            if (ext > 0) {
-                std::frexp(ext / 255.0, &e);  // ext / 255 < 2^e
-            } else {
-                e = -100;
-            }
-            float s = std::ldexp(1.0f, std::max(e, -126));
-            while (std::fmaf(255.0f, s, origin[a]) < hi) s *= 2.0f;
-            scale[a] = s;
-        }
-        uint32_t wref[4];
-        for (int c = 0; c < n; c++) wref[c] = collapse(cref[c]);
-        wr_wnode& W = out[idx];
-        for (int a = 0; a < 3; a++) {
-            W.origin[a] = origin[a];
-            W.scale[a] = scale[a];
-        }
-        for (int c = 0; c < 4; c++) {
-            if (c < n) {
-                for (int a = 0; a < 3; a++) {
-                    W.q[a][c] = quant_lo(cbox[c][a], origin[a], scale[a]);
-                    W.q[3 + a][c] = quant_hi(cbox[c][3 + a], origin[a], scale[a]);
-                }
-                W.ref[c] = wref[c];
-            } else {
-                for (int k = 0; k < 6; k++) W.q[k][c] = 0;
-                W.ref[c] = WR_NONE;
-            }
-        }
-        return idx;
+        return make_node(n, cbox, [&](int c) { return collapse(cref[c]); });
     }
 };
 
@@ -380,8 +454,9 @@ FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32
     }
     F.hdr.n_nodes = (uint32_t)F.nodes.size();
     F.hdr.depth = internal_depth(&root);
+    F.rec_first.resize(F.tris.size());
     Collapser col{F, F.wnodes};
-    F.wnodes.reserve(F.nodes.size() / 2 + 1);
+    F.wnodes.reserve(F.nodes.size() / 2 + F.tris.size() / 3 + 1);
     F.hdr.wide_root = col.collapse(rref);
     F.hdr.n_wide = (uint32_t)F.wnodes.size();
     F.n_leaves = fl.leaves;

@@ -227,7 +227,7 @@ int wr_scene_build(wr_scene* s, int strategy) {
         std::vector<wr_node> nodes;
         std::vector<wr_wnode> wnodes;
         std::vector<wr_tri> tris;
-        std::vector<uint32_t> leaf_up, leaf_cnt, node_up, rec_prim, rec_obj;
+        std::vector<uint32_t> leaf_up, leaf_cnt, node_up, rec_prim, rec_obj, rec_first;
         std::vector<float> rec_nrm;
         std::vector<wr_tree_hdr> hdrs;
         std::vector<wr_object_gpu> objs;
@@ -255,6 +255,7 @@ int wr_scene_build(wr_scene* s, int strategy) {
             tris.insert(tris.end(), F.tris.begin(), F.tris.end());
             leaf_up.insert(leaf_up.end(), F.leaf_up.begin(), F.leaf_up.end());
             leaf_cnt.insert(leaf_cnt.end(), F.leaf_cnt.begin(), F.leaf_cnt.end());
+            rec_first.insert(rec_first.end(), F.rec_first.begin(), F.rec_first.end());
             rec_prim.insert(rec_prim.end(), F.rec_prim.begin(), F.rec_prim.end());
             for (size_t r = 0; r < F.rec_prim.size(); r++) {
                 uint32_t oi, local;
@@ -351,7 +352,7 @@ int wr_scene_build(wr_scene* s, int strategy) {
         wr_node* d_nodes;
         wr_wnode* d_wnodes;
         wr_tri* d_tris;
-        uint32_t *d_lu, *d_lc, *d_nu, *d_rp, *d_ro, *d_hn;
+        uint32_t *d_lu, *d_lc, *d_rf, *d_nu, *d_rp, *d_ro, *d_hn;
         float* d_rn;
         wr_tree_hdr* d_h;
         wr_object_gpu* d_o;
@@ -366,6 +367,8 @@ int wr_scene_build(wr_scene* s, int strategy) {
         s->dbufs.push_back(d_lu);
         if ((rc = upload(ctx, leaf_cnt, &d_lc, bytes))) return rc;
         s->dbufs.push_back(d_lc);
+        if ((rc = upload(ctx, rec_first, &d_rf, bytes))) return rc;
+        s->dbufs.push_back(d_rf);
         if ((rc = upload(ctx, node_up, &d_nu, bytes))) return rc;
         s->dbufs.push_back(d_nu);
         if ((rc = upload(ctx, rec_prim, &d_rp, bytes))) return rc;
@@ -390,6 +393,7 @@ int wr_scene_build(wr_scene* s, int strategy) {
         D.tris = d_tris;
         D.leaf_up = d_lu;
         D.leaf_cnt = d_lc;
+        D.rec_first = d_rf;
         D.node_anc = (const uint4*)d_nu;
         D.rec_prim = d_rp;
         D.rec_obj = d_ro;
@@ -496,6 +500,19 @@ int wr_tree_export(const wr_tree* tree, uint32_t* n_nodes, uint32_t* n_leaf_tris
 }
 
 void wr_tree_free(wr_tree* tree) { delete tree; }
+
+int wr_tree_wide(const wr_tree* tree, uint32_t* n_wide, uint32_t* n_records, uint32_t* wide_root, void* wide_nodes,
+                 uint32_t* record_first, uint32_t* record_tri) {
+    if (!tree || !n_wide || !n_records || !wide_root) return fail(nullptr, WR_EINVAL, "wr_tree_wide: bad argument");
+    const FlatTree& F = tree->t;
+    *n_wide = (uint32_t)F.wnodes.size();
+    *n_records = (uint32_t)F.tris.size();
+    *wide_root = F.hdr.wide_root;
+    if (wide_nodes) std::memcpy(wide_nodes, F.wnodes.data(), F.wnodes.size() * sizeof(wr_wnode));
+    if (record_first) std::memcpy(record_first, F.rec_first.data(), F.rec_first.size() * sizeof(uint32_t));
+    if (record_tri) std::memcpy(record_tri, F.rec_prim.data(), F.rec_prim.size() * sizeof(uint32_t));
+    return WR_OK;
+}
 
 int wr_closest_hit(wr_ctx* ctx, const wr_scene* s, const wr_ray* d_rays, uint64_t n, wr_hit* d_hits,
                    uint64_t* d_counters) {

@@ -40,6 +40,7 @@ struct Tree {
     const wr_tri* tris;        // already offset to this tree
     const uint32_t* leaf_up;   // offset like tris
     const uint32_t* leaf_cnt;  // offset like tris
+    const uint32_t* rec_first; // offset like tris
     const uint4* node_anc;     // offset like nodes
     float rmin[3], rmax[3];
     uint32_t root_ref;
@@ -54,6 +55,7 @@ __device__ __forceinline__ Tree make_tree(const wr_tree_hdr& h, const SceneDev& 
     t.tris = S.tris + h.rec_base;
     t.leaf_up = S.leaf_up + h.rec_base;
     t.leaf_cnt = S.leaf_cnt + h.rec_base;
+    t.rec_first = S.rec_first + h.rec_base;
     t.node_anc = S.node_anc + h.node_base;
     for (int a = 0; a < 3; a++) {
         t.rmin[a] = h.root_min[a];
@@ -360,6 +362,28 @@ __device__ __forceinline__ void visit_leaf(const Tree& tr, const RayPre& r, cons
     }
 }
 
+// Reference triangle test of one record reached through the wide hierarchy
+// (its own box), folded into h with the same total order as visit_leaf: the
+// leaf key is the record's reference leaf (rec_first), looked up only for a
+// tie or an accepted hit.
+template <bool NOSELF, bool COUNT>
+__device__ __forceinline__ void visit_tri(const Tree& tr, const RayPre& r, const NoSelf& ns, uint32_t ref,
+                                          HitState& h, Counters& cnt) {
+    const uint32_t rec = ref & WR_FIRST_MASK;
+    const wr_tri T = tr.tris[rec];
+    if (COUNT) cnt.tris++;
+    const float t = ref_tri(T, r.o, r.d, h.t, false);
+    if (!(t >= 0.f)) return;
+    const uint32_t f = tr.rec_first[rec];
+    // t <= h.t here; on equality the larger leaf key wins, then the earlier slot
+    const bool better = h.rec == WR_NONE || t < h.t || f > h.first || (f == h.first && rec < h.rec);
+    if (better && (!NOSELF || noself_legal(ns, T, rec, r.o, r.d, t))) {
+        h.t = t;
+        h.rec = rec;
+        h.first = f;
+    }
+}
+
 // Ordered traversal of the reference binary tree.  EXACT = false: the fast
 // conservative pass; EXACT = true: every box also passes the reference test.
 template <bool NOSELF, bool EXACT, bool COUNT>
@@ -444,19 +468,22 @@ __device__ __forceinline__ bool wide_begin(const Tree& tr, const RayPre& r, Lane
     return slab(r, tr.rmin[0], tr.rmin[1], tr.rmin[2], tr.rmax[0], tr.rmax[1], tr.rmax[2], __builtin_inff(), tn);
 }
 
-// Visit `ref` (a wide node or a leaf) and pick the next reference; false when
-// the pass has ended (stack exhausted) or the stack would overflow (overflow
-// set: the caller re-runs the ray on the binary tree).
-template <bool NOSELF, bool COUNT>
-__device__ __forceinline__ bool wide_step(const Tree& tr, const RayPre& r, const NoSelf& ns, LaneStack& st,
-                                          HitState& h, uint32_t& ref, bool& overflow, Counters& cnt) {
-    if (ref & WR_LEAF_BIT) {
-        visit_leaf<NOSELF, COUNT>(tr, r, ns, ref, h, cnt);
-        return st.next(prune_cut(r, h.t), ref);
-    }
+// Next stack entry that can still hold a hit, WR_NONE when there is none.
+__device__ __forceinline__ uint32_t next_or_end(LaneStack& st, float tcut) {
+    uint32_t ref;
+    return st.next(tcut, ref) ? ref : WR_NONE;
+}
+
+// Visit wide node `ref`: push the far children the ray enters, farthest
+// first, and return the nearest one, else the next stack entry, else WR_NONE
+// (pass ended).  Sets overflow (and returns WR_NONE) when the stack is full:
+// the caller then re-runs the ray on the binary tree.
+template <bool COUNT>
+__device__ __forceinline__ uint32_t wide_node(const Tree& tr, const RayPre& r, LaneStack& st, float best_t,
+                                              uint32_t ref, bool& overflow, Counters& cnt) {
     const wr_wnode W = tr.wnodes[ref];
     if (COUNT) cnt.nodes += 2;  // node bytes in 32 B units, as for a binary node
-    const float tcut = prune_cut(r, h.t);
+    const float tcut = prune_cut(r, best_t);
     float t[4];
     uint32_t c[4];
 #pragma unroll
@@ -480,16 +507,27 @@ __device__ __forceinline__ bool wide_step(const Tree& tr, const RayPre& r, const
     cswap(t[1], c[1], t[2], c[2]);
     if (st.sp + 3 > kMaxDepth) {
         overflow = true;
-        return false;
+        return WR_NONE;
     }
     if (t[3] != __builtin_inff()) st.push(c[3], t[3]);
     if (t[2] != __builtin_inff()) st.push(c[2], t[2]);
     if (t[1] != __builtin_inff()) st.push(c[1], t[1]);
-    if (t[0] != __builtin_inff()) {
-        ref = c[0];
-        return true;
+    if (t[0] != __builtin_inff()) return c[0];
+    return next_or_end(st, tcut);
+}
+
+// Visit `ref` (a wide node or a leaf) and pick the next reference; false when
+// the pass has ended (stack exhausted) or the stack would overflow (overflow
+// set: the caller re-runs the ray on the binary tree).
+template <bool NOSELF, bool COUNT>
+__device__ __forceinline__ bool wide_step(const Tree& tr, const RayPre& r, const NoSelf& ns, LaneStack& st,
+                                          HitState& h, uint32_t& ref, bool& overflow, Counters& cnt) {
+    if (ref & WR_LEAF_BIT) {  // always a single triangle in the wide hierarchy
+        visit_tri<NOSELF, COUNT>(tr, r, ns, ref, h, cnt);
+        return st.next(prune_cut(r, h.t), ref);
     }
-    return st.next(tcut, ref);
+    ref = wide_node<COUNT>(tr, r, st, h.t, ref, overflow, cnt);
+    return ref != WR_NONE;
 }
 
 template <bool NOSELF, bool COUNT>

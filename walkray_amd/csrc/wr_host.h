@@ -17,7 +17,8 @@ struct FlatTree {
     std::vector<uint32_t> leaf_up;     // first record of a leaf -> (parent << 1 | side); WR_NONE for a root leaf
     std::vector<uint32_t> leaf_cnt;    // first record of a long leaf -> triangle count
     std::vector<uint32_t> node_up;     // internal node -> (parent << 1 | side); WR_NONE for the root
-    std::vector<wr_wnode> wnodes;      // fast-pass hierarchy (the same leaves, four children per node)
+    std::vector<wr_wnode> wnodes;      // fast-pass hierarchy: four children per node, single triangles at the bottom
+    std::vector<uint32_t> rec_first;   // record -> first record of its reference leaf (tie order key)
     wr_tree_hdr hdr{};
     uint32_t n_leaves = 0;
     double build_seconds = 0.0;

@@ -13,6 +13,7 @@ struct SceneDev {
     const wr_tri* tris;
     const uint32_t* leaf_up;
     const uint32_t* leaf_cnt;
+    const uint32_t* rec_first;       // record -> first record of its reference leaf
     const uint4* node_anc;           // internal node -> its next four ancestor links (parent << 1 | side), WR_NONE past the root
     const uint32_t* rec_prim;        // record -> triangle id (global id for the global tree, local for object trees)
     const uint32_t* rec_obj;         // record -> object id
