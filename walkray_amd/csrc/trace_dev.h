@@ -541,32 +541,43 @@ __device__ __forceinline__ void traverse_wide(const Tree& tr, const RayPre& r, c
 }
 
 // Re-check the winner's root-to-leaf chain with the reference box test.  The
-// links come four at a time (node_anc), so the node fetches of four levels
-// are issued together instead of one dependent fetch per level.
-__device__ __forceinline__ bool chain_box(const Tree& tr, const RayPre& r, uint32_t up) {
-    const wr_node& N = tr.nodes[up >> 1];
-    return (up & 1u) ? ref_box(r, N.rmin, N.rmax) : ref_box(r, N.lmin, N.lmax);
+// links come four at a time (node_anc), so the box fetches of four levels are
+// issued together instead of one dependent fetch per level; each fetch reads
+// only the 24 B box of the chain's side of the node.
+struct Box6 {
+    float mn[3], mx[3];
+};
+
+__device__ __forceinline__ Box6 side_box(const Tree& tr, uint32_t up) {
+    const float* p = reinterpret_cast<const float*>(tr.nodes + (up >> 1)) + 6 * (up & 1u);
+    Box6 b;
+    for (int k = 0; k < 3; k++) {
+        b.mn[k] = p[k];
+        b.mx[k] = p[3 + k];
+    }
+    return b;
 }
 
 __device__ __forceinline__ bool verify_chain(const Tree& tr, const RayPre& r, uint32_t first) {
     const uint32_t up = tr.leaf_up[first];
     if (up != WR_NONE) {
-        if (!chain_box(tr, r, up)) return false;
+        const Box6 b = side_box(tr, up);
+        if (!ref_box(r, b.mn, b.mx)) return false;
         uint32_t node = up >> 1;
         for (;;) {
             const uint4 a = tr.node_anc[node];
             if (a.x == WR_NONE) break;
-            const wr_node N0 = tr.nodes[a.x >> 1];
-            const wr_node N1 = tr.nodes[a.y == WR_NONE ? 0 : a.y >> 1];
-            const wr_node N2 = tr.nodes[a.z == WR_NONE ? 0 : a.z >> 1];
-            const wr_node N3 = tr.nodes[a.w == WR_NONE ? 0 : a.w >> 1];
-            if (!((a.x & 1u) ? ref_box(r, N0.rmin, N0.rmax) : ref_box(r, N0.lmin, N0.lmax))) return false;
+            const Box6 b0 = side_box(tr, a.x);
+            const Box6 b1 = side_box(tr, a.y == WR_NONE ? a.x : a.y);
+            const Box6 b2 = side_box(tr, a.z == WR_NONE ? a.x : a.z);
+            const Box6 b3 = side_box(tr, a.w == WR_NONE ? a.x : a.w);
+            if (!ref_box(r, b0.mn, b0.mx)) return false;
             if (a.y == WR_NONE) break;
-            if (!((a.y & 1u) ? ref_box(r, N1.rmin, N1.rmax) : ref_box(r, N1.lmin, N1.lmax))) return false;
+            if (!ref_box(r, b1.mn, b1.mx)) return false;
             if (a.z == WR_NONE) break;
-            if (!((a.z & 1u) ? ref_box(r, N2.rmin, N2.rmax) : ref_box(r, N2.lmin, N2.lmax))) return false;
+            if (!ref_box(r, b2.mn, b2.mx)) return false;
             if (a.w == WR_NONE) break;
-            if (!((a.w & 1u) ? ref_box(r, N3.rmin, N3.rmax) : ref_box(r, N3.lmin, N3.lmax))) return false;
+            if (!ref_box(r, b3.mn, b3.mx)) return false;
             node = a.w >> 1;
         }
     }
