@@ -38,6 +38,21 @@ RAY_IN_BYTES = 24
 HIT_OUT_BYTES = 16
 
 
+TRAFFIC_FILE = "profiles/closest_hit_heightfield_2M.md"
+
+
+def measured_traffic():
+    """HBM bytes per closest-hit call from the committed PMC summary (None when absent)."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
+            for line in f:
+                if "traffic_bytes_per_call:" in line:
+                    return int(line.split("traffic_bytes_per_call:")[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -118,7 +133,8 @@ def closest_hit_bench(W, S, ctx, steps, warmup, n_side=4096, hf_n=1024):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(total_b / 2 / (avg_launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(),
+        "traffic_source": TRAFFIC_FILE + " (rocprofv3 FETCH_SIZE and WRITE_SIZE per call, calibrated on a 1 GiB copy)",
         "avg_launch_ms": round(avg_launch_ms, 4),
         "alg_bytes_per_launch": int(total_b / 2),
     }
