@@ -227,7 +227,8 @@ int wr_scene_build(wr_scene* s, int strategy) {
         std::vector<wr_node> nodes;
         std::vector<wr_wnode> wnodes;
         std::vector<wr_tri> tris;
-        std::vector<uint32_t> leaf_up, leaf_cnt, node_up, rec_prim, rec_obj, rec_first;
+        std::vector<uint32_t> leaf_up, leaf_cnt, node_up, rec_prim, rec_obj, rec_first, leaf_chain;
+        std::vector<wr_chain> chains;
         std::vector<float> rec_nrm;
         std::vector<wr_tree_hdr> hdrs;
         std::vector<wr_object_gpu> objs;
@@ -256,6 +257,27 @@ int wr_scene_build(wr_scene* s, int strategy) {
             leaf_up.insert(leaf_up.end(), F.leaf_up.begin(), F.leaf_up.end());
             leaf_cnt.insert(leaf_cnt.end(), F.leaf_cnt.begin(), F.leaf_cnt.end());
             rec_first.insert(rec_first.end(), F.rec_first.begin(), F.rec_first.end());
+            for (size_t rr = 0; rr < F.rec_first.size(); rr++) {  // chain record of every non-root leaf
+                uint32_t up = F.leaf_up[rr];
+                if (F.rec_first[rr] != rr || up == WR_NONE) {
+                    leaf_chain.push_back(WR_NONE);
+                    continue;
+                }
+                leaf_chain.push_back((uint32_t)chains.size());
+                wr_chain c{};
+                while (up != WR_NONE && c.count < 5) {
+                    const wr_node& N = F.nodes[up >> 1];
+                    const float* b = (up & 1u) ? N.rmin : N.lmin;  // min then max, contiguous
+                    for (int k = 0; k < 3; k++) {
+                        c.box[c.count][k] = b[k];
+                        c.box[c.count][3 + k] = b[3 + k];
+                    }
+                    c.count++;
+                    up = F.node_up[up >> 1];
+                }
+                c.next = up;
+                chains.push_back(c);
+            }
             rec_prim.insert(rec_prim.end(), F.rec_prim.begin(), F.rec_prim.end());
             for (size_t r = 0; r < F.rec_prim.size(); r++) {
                 uint32_t oi, local;
@@ -352,7 +374,8 @@ int wr_scene_build(wr_scene* s, int strategy) {
         wr_node* d_nodes;
         wr_wnode* d_wnodes;
         wr_tri* d_tris;
-        uint32_t *d_lu, *d_lc, *d_rf, *d_nu, *d_rp, *d_ro, *d_hn;
+        uint32_t *d_lu, *d_lc, *d_rf, *d_lch, *d_nu, *d_rp, *d_ro, *d_hn;
+        wr_chain* d_ch;
         float* d_rn;
         wr_tree_hdr* d_h;
         wr_object_gpu* d_o;
@@ -369,6 +392,10 @@ int wr_scene_build(wr_scene* s, int strategy) {
         s->dbufs.push_back(d_lc);
         if ((rc = upload(ctx, rec_first, &d_rf, bytes))) return rc;
         s->dbufs.push_back(d_rf);
+        if ((rc = upload(ctx, leaf_chain, &d_lch, bytes))) return rc;
+        s->dbufs.push_back(d_lch);
+        if ((rc = upload(ctx, chains, &d_ch, bytes))) return rc;
+        s->dbufs.push_back(d_ch);
         if ((rc = upload(ctx, node_up, &d_nu, bytes))) return rc;
         s->dbufs.push_back(d_nu);
         if ((rc = upload(ctx, rec_prim, &d_rp, bytes))) return rc;
@@ -394,6 +421,8 @@ int wr_scene_build(wr_scene* s, int strategy) {
         D.leaf_up = d_lu;
         D.leaf_cnt = d_lc;
         D.rec_first = d_rf;
+        D.leaf_chain = d_lch;
+        D.chains = d_ch;
         D.node_anc = (const uint4*)d_nu;
         D.rec_prim = d_rp;
         D.rec_obj = d_ro;

@@ -4,8 +4,9 @@
 //                   pass, each lane runs its pixel's whole path
 //                   (pathTracer.cpp:20-70 with mcRenderer.cpp:16-57) and adds
 //                   the sample to the Kahan film it owns.
-//  wavefront        the same estimator split into stages (extend, shade, plan,
-//                   generate, fold, advance) over a shared pool of path slots
+//  wavefront        the same estimator split into stages (extend = trace +
+//                   finish, shade, plan, generate, fold, advance) over a
+//                   shared pool of path slots
 //                   that any number of jobs (render calls, served requests)
 //                   fill; finished paths write their sample to a per-(job,
 //                   pass, pixel) slot and complete passes are folded into the
@@ -115,7 +116,108 @@ __device__ __forceinline__ void finish(const wr::JobDesc& J, const CamDev& cam, 
     count_done(J.pass_done + key);
 }
 
-__global__ __launch_bounds__(256) void k_wf_extend(EngineDev E, uint32_t par) {
+// extend, part 1: the fast pass of every live ray (Scene::intersect; camera
+// rays without a condition, later rays with NoSelf) as a persistent kernel
+// that refills a lane with the next path as soon as its traversal ends, and
+// runs node-only steps while most lanes sit at wide nodes (as
+// k_closest_hit).  The fast-pass result goes to ht / hrec / hobj (first record
+// of the leaf) and the overflow flag; k_wf_finish completes it.
+constexpr uint32_t kOverflow = 4u;
+constexpr uint32_t kTraceRefillMin = 16;
+constexpr uint32_t kTraceNodePhaseMin = 32;
+
+__global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
+    const PathBuf& A = E.buf[par];
+    const uint32_t n = E.cnt->n[par];
+    unsigned long long* work = E.cnt->trace_work;
+    // workgroups beyond what n paths need leave at once (and are not counted below)
+    const uint32_t used = (uint32_t)(((uint64_t)n + 255) / 256 < gridDim.x ? ((uint64_t)n + 255) / 256 : gridDim.x);
+    if (blockIdx.x >= used) return;
+    LaneStack st;
+    st.lane = threadIdx.x;
+    Counters cnt{0, 0, 0};
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t bnext = 0, bend = 0;  // the wave's current batch of paths [bnext, bend)
+    bool exhausted = false;
+    bool active = false, overflow = false;
+    uint32_t pid = 0, ref = 0;
+    RayPre r{};
+    HitState h{};
+    Tree tr{};
+    NoSelf ns{};
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        if (!exhausted && (uint32_t)__popcll(idle) >= kTraceRefillMin) {
+            while (idle != 0) {
+                if (bnext == bend) {
+                    unsigned long long base = 0;
+                    if (lane == 0) base = atomicAdd(work, 64ull);
+                    base = __shfl(base, 0);
+                    if (base >= n) {
+                        exhausted = true;
+                        break;
+                    }
+                    bnext = (uint32_t)base;
+                    bend = base + 64 < n ? (uint32_t)base + 64 : n;
+                }
+                const uint32_t avail = bend - bnext;
+                const uint32_t rank = (uint32_t)__popcll(idle & below);
+                if (!active && rank < avail) {
+                    pid = bnext + rank;
+                    const SceneDev& S = E.scenes[E.jobs[A.job[pid]].scene];
+                    const wr_tree_hdr H = S.hdrs[0];
+                    tr = make_tree(H, S);
+                    r = make_ray(mk(A.ox[pid], A.oy[pid], A.oz[pid]), mk(A.dx[pid], A.dy[pid], A.dz[pid]), tr.scale);
+                    const bool primary = A.flags[pid] & kPrimary;
+                    ns = NoSelf{primary ? WR_NONE : A.inside[pid], primary ? WR_NONE : A.contact[pid],
+                                S.rec_obj + H.rec_base, S.rec_nrm + 9 * (size_t)H.rec_base, S.obj_has_normals};
+                    overflow = false;
+                    if (wide_begin(tr, r, st, h, ref)) {
+                        active = true;
+                    } else {
+                        A.hrec[pid] = WR_NONE;  // misses the root box
+                    }
+                }
+                const uint32_t took = (uint32_t)__popcll(idle);
+                bnext += took < avail ? took : avail;
+                idle = __ballot(!active);
+            }
+        }
+        if (__ballot(active) == 0) break;  // only once the counter is exhausted
+        for (;;) {
+            const bool at_node = active && !(ref & WR_LEAF_BIT);
+            if ((uint32_t)__popcll(__ballot(at_node)) < kTraceNodePhaseMin) break;
+            if (at_node && !wide_step<true, false>(tr, r, ns, st, h, ref, overflow, cnt)) {
+                active = false;
+                A.ht[pid] = h.t;
+                A.hrec[pid] = h.rec;
+                A.hobj[pid] = h.first;
+                if (overflow) A.flags[pid] |= kOverflow;
+            }
+        }
+        if (active && !wide_step<true, false>(tr, r, ns, st, h, ref, overflow, cnt)) {
+            active = false;
+            A.ht[pid] = h.t;
+            A.hrec[pid] = h.rec;
+            A.hobj[pid] = h.first;
+            if (overflow) A.flags[pid] |= kOverflow;
+        }
+    }
+    if (lane == 0) {
+        const unsigned long long waves = (unsigned long long)used * (blockDim.x / 64);
+        if (atomicAdd(work + 1, 1ull) == waves - 1) {
+            atomicExch(work, 0ull);
+            atomicExch(work + 1, 0ull);
+        }
+    }
+}
+
+// extend, part 2 (one lane per path): binary fast pass after an overflow,
+// chain verification and exact fallback (trace_dev.h: finish_hit),
+// findInsideObject for camera rays (camera.cpp:52-56), and the end of paths
+// that escape.
+__global__ __launch_bounds__(256) void k_wf_finish(EngineDev E, uint32_t par) {
     const PathBuf& A = E.buf[par];
     const uint32_t n = E.cnt->n[par];
     LaneStack st;
@@ -126,13 +228,18 @@ __global__ __launch_bounds__(256) void k_wf_extend(EngineDev E, uint32_t par) {
         const SceneCtx sc = make_scene_ctx(S);
         const f3 o = mk(A.ox[i], A.oy[i], A.oz[i]);
         const f3 d = mk(A.dx[i], A.dy[i], A.dz[i]);
-        PHit h;
-        if (A.flags[i] & kPrimary) {  // camera ray: Scene::intersect without Condition + findInsideObject (camera.cpp:52-56)
-            h = trace_scene<false>(sc, o, d, WR_NONE, WR_NONE, st);
-            A.inside[i] = find_inside(S, o, d, WR_NONE, st);
-        } else {
-            h = trace_scene<true>(sc, o, d, A.inside[i], A.contact[i], st);
+        const uint32_t fl = A.flags[i];
+        const bool primary = fl & kPrimary;
+        HitState h{A.ht[i], A.hrec[i], A.hobj[i]};
+        if (h.rec != WR_NONE || (fl & kOverflow)) {
+            const RayPre r = make_ray(o, d, sc.tr.scale);
+            const NoSelf ns{primary ? WR_NONE : A.inside[i], primary ? WR_NONE : A.contact[i], sc.gobj, sc.gnrm,
+                            S.obj_has_normals};
+            Counters cnt{0, 0, 0};
+            bool failed;
+            finish_hit<true, false>(sc.tr, r, ns, st, cnt, h, (fl & kOverflow) != 0, failed);
         }
+        if (primary) A.inside[i] = find_inside(S, o, d, WR_NONE, st);
         if (h.rec == WR_NONE) {  // escaped (no environment light) or nothing to scatter: sample 0
             finish(J, job_cam(J), A.pixel[i], A.pass[i], mk(0.f, 0.f, 0.f));
             A.flags[i] = kDead;
@@ -140,7 +247,7 @@ __global__ __launch_bounds__(256) void k_wf_extend(EngineDev E, uint32_t par) {
         }
         A.ht[i] = h.t;
         A.hrec[i] = h.rec;
-        A.hobj[i] = h.obj;
+        A.hobj[i] = sc.gobj[h.rec];
         A.flags[i] = 0;
     }
 }
@@ -392,11 +499,18 @@ hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st) {
     E.ctl = ctl;
     E.scenes = scenes;
     E.cap = cap;
+    // the persistent trace kernel runs as many workgroups as fit at once
+    int dev = 0, cus = 0, per_cu = 0;
+    WR_CHECK(hipGetDevice(&dev));
+    WR_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    WR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_wf_trace, 256, kStackLdsBytes));
+    const uint32_t trace_grid = (uint32_t)std::max(1, std::min<int>(cus * std::max(per_cu, 1), (int)(cap / 256)));
     // the iteration graph: kGraphIters iterations (even, the buffer parity alternates)
     WR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int it = 0; it < kGraphIters; it++) {
         const uint32_t par = (uint32_t)(it & 1);
-        k_wf_extend<<<grid, 256, kStackLdsBytes, st>>>(E, par);
+        k_wf_trace<<<trace_grid, 256, kStackLdsBytes, st>>>(E, par);
+        k_wf_finish<<<grid, 256, kStackLdsBytes, st>>>(E, par);
         k_wf_shade<<<grid, 256, kStackLdsBytes, st>>>(E, par);
         k_wf_plan<<<1, 1, 0, st>>>(E, par);
         k_wf_generate<<<grid, 256, 0, st>>>(E, par);

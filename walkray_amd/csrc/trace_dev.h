@@ -41,6 +41,8 @@ struct Tree {
     const uint32_t* leaf_up;   // offset like tris
     const uint32_t* leaf_cnt;  // offset like tris
     const uint32_t* rec_first; // offset like tris
+    const uint32_t* leaf_chain;  // offset like tris (values index chains directly)
+    const wr_chain* chains;
     const uint4* node_anc;     // offset like nodes
     float rmin[3], rmax[3];
     uint32_t root_ref;
@@ -56,6 +58,8 @@ __device__ __forceinline__ Tree make_tree(const wr_tree_hdr& h, const SceneDev& 
     t.leaf_up = S.leaf_up + h.rec_base;
     t.leaf_cnt = S.leaf_cnt + h.rec_base;
     t.rec_first = S.rec_first + h.rec_base;
+    t.leaf_chain = S.leaf_chain + h.rec_base;
+    t.chains = S.chains;
     t.node_anc = S.node_anc + h.node_base;
     for (int a = 0; a < 3; a++) {
         t.rmin[a] = h.root_min[a];
@@ -250,6 +254,8 @@ struct NoSelf {
 
 __device__ __forceinline__ bool noself_legal(const NoSelf& ns, const wr_tri& T, uint32_t rec, const f3& o,
                                              const f3& d, float t) {
+    // both NULL: legal() returns !(in && NULL == obj), true for every object
+    if (ns.inside == WR_NONE && ns.contact == WR_NONE) return true;
     const uint32_t obj = ns.rec_obj[rec];
     const float* nrm = ns.obj_flags[obj] ? ns.rec_nrm + 9 * (size_t)rec : nullptr;
     float u1, u2;
@@ -541,9 +547,10 @@ __device__ __forceinline__ void traverse_wide(const Tree& tr, const RayPre& r, c
 }
 
 // Re-check the winner's root-to-leaf chain with the reference box test.  The
-// links come four at a time (node_anc), so the box fetches of four levels are
-// issued together instead of one dependent fetch per level; each fetch reads
-// only the 24 B box of the chain's side of the node.
+// five levels nearest the leaf come from the leaf's chain record (one line);
+// above them the links come four at a time (node_anc), so the box fetches of
+// four levels are issued together instead of one dependent fetch per level,
+// each reading only the 24 B box of the chain's side of the node.
 struct Box6 {
     float mn[3], mx[3];
 };
@@ -559,26 +566,37 @@ __device__ __forceinline__ Box6 side_box(const Tree& tr, uint32_t up) {
 }
 
 __device__ __forceinline__ bool verify_chain(const Tree& tr, const RayPre& r, uint32_t first) {
-    const uint32_t up = tr.leaf_up[first];
-    if (up != WR_NONE) {
-        const Box6 b = side_box(tr, up);
-        if (!ref_box(r, b.mn, b.mx)) return false;
-        uint32_t node = up >> 1;
-        for (;;) {
-            const uint4 a = tr.node_anc[node];
-            if (a.x == WR_NONE) break;
-            const Box6 b0 = side_box(tr, a.x);
-            const Box6 b1 = side_box(tr, a.y == WR_NONE ? a.x : a.y);
-            const Box6 b2 = side_box(tr, a.z == WR_NONE ? a.x : a.z);
-            const Box6 b3 = side_box(tr, a.w == WR_NONE ? a.x : a.w);
-            if (!ref_box(r, b0.mn, b0.mx)) return false;
-            if (a.y == WR_NONE) break;
-            if (!ref_box(r, b1.mn, b1.mx)) return false;
-            if (a.z == WR_NONE) break;
-            if (!ref_box(r, b2.mn, b2.mx)) return false;
-            if (a.w == WR_NONE) break;
-            if (!ref_box(r, b3.mn, b3.mx)) return false;
-            node = a.w >> 1;
+    const uint32_t ci = tr.leaf_chain[first];
+    if (ci != WR_NONE) {
+        // the leaf's first five levels come in one 128 B line
+        const wr_chain C = tr.chains[ci];
+        const uint32_t count = C.count;
+        uint32_t up = C.next;
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+            if ((uint32_t)k < count && !ref_box(r, C.box[k], C.box[k] + 3)) return false;
+        if (up != WR_NONE) {
+            {
+                const Box6 b = side_box(tr, up);
+                if (!ref_box(r, b.mn, b.mx)) return false;
+            }
+            uint32_t node = up >> 1;
+            for (;;) {
+                const uint4 a = tr.node_anc[node];
+                if (a.x == WR_NONE) break;
+                const Box6 b0 = side_box(tr, a.x);
+                const Box6 b1 = side_box(tr, a.y == WR_NONE ? a.x : a.y);
+                const Box6 b2 = side_box(tr, a.z == WR_NONE ? a.x : a.z);
+                const Box6 b3 = side_box(tr, a.w == WR_NONE ? a.x : a.w);
+                if (!ref_box(r, b0.mn, b0.mx)) return false;
+                if (a.y == WR_NONE) break;
+                if (!ref_box(r, b1.mn, b1.mx)) return false;
+                if (a.z == WR_NONE) break;
+                if (!ref_box(r, b2.mn, b2.mx)) return false;
+                if (a.w == WR_NONE) break;
+                if (!ref_box(r, b3.mn, b3.mx)) return false;
+                node = a.w >> 1;
+            }
         }
     }
     return ref_box(r, tr.rmin, tr.rmax);

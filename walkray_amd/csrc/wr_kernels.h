@@ -14,6 +14,8 @@ struct SceneDev {
     const uint32_t* leaf_up;
     const uint32_t* leaf_cnt;
     const uint32_t* rec_first;       // record -> first record of its reference leaf
+    const uint32_t* leaf_chain;      // first record of a non-root leaf -> its chain record (global index)
+    const wr_chain* chains;
     const uint4* node_anc;           // internal node -> its next four ancestor links (parent << 1 | side), WR_NONE past the root
     const uint32_t* rec_prim;        // record -> triangle id (global id for the global tree, local for object trees)
     const uint32_t* rec_obj;         // record -> object id

@@ -59,6 +59,18 @@ struct wr_wnode {
 };
 static_assert(sizeof(wr_wnode) == 64, "wide node must be 64 bytes");
 
+// 128 B chain record of one reference leaf (one L2 line): the boxes that the
+// reference tests on the way to the leaf at its first up-to-five ancestors,
+// leaf-side first (box[k] is the box of the ancestor-k child on the path,
+// stored in ancestor k + 1), and the link (parent << 1 | side) of the next
+// level up, WR_NONE when the chain ends at the root.
+struct wr_chain {
+    float box[5][6];
+    uint32_t next;
+    uint32_t count;
+};
+static_assert(sizeof(wr_chain) == 128, "chain record must be 128 bytes");
+
 // Per-tree header.  root_ref may itself be a leaf.
 struct wr_tree_hdr {
     float root_min[3];

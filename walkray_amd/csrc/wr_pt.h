@@ -61,6 +61,7 @@ struct EngineCounters {
     uint32_t nseg, seg_start, seg_end, pad;
     uint32_t seg_job[kMaxJobs], seg_base[kMaxJobs], seg_take[kMaxJobs];
     uint64_t seg_work[kMaxJobs];
+    unsigned long long trace_work[2];  // k_wf_trace: path batches handed out, waves finished (self-resetting)
 };
 
 struct WaveStats {
