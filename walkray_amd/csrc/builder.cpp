@@ -237,30 +237,176 @@ uint8_t quant_hi(float v, float origin, float scale) {
     return (uint8_t)qi;
 }
 
-// Collapses the flattened reference tree two levels at a time into wide
-// nodes (preorder).  A reference leaf with two or more triangles becomes a
-// small wide subtree whose children are single triangles, each under its own
-// box, so the fast pass box-tests triangles before running the triangle test;
-// rec_first keeps every record's reference leaf (the tie order's key).
-struct Collapser {
-    FlatTree& F;
+// The fast-pass hierarchy: a 4-wide BVH over the triangle records, built
+// top-down with a binned surface-area heuristic (two binary splits per wide
+// node), one triangle per leaf reference.  It is independent of the reference
+// tree: the fast pass only needs every triangle under a box that contains it
+// (superset of the reference's candidates) and every record's tie key
+// (rec_first); exactness comes from verifying the winner on the reference
+// tree.
+struct WideBuilder {
+    const FlatTree& F;
     std::vector<wr_wnode>& out;
+    std::vector<std::array<float, 6>> box;  // per record
+    std::vector<std::array<float, 3>> cen;  // per record (box centre)
+    std::vector<uint32_t> idx;              // record order, partitioned in place
 
     static uint32_t tri_ref(uint32_t rec) { return WR_LEAF_BIT | (1u << WR_CNT_SHIFT) | rec; }
 
-    void tri_box(uint32_t rec, float* box) const {
-        const wr_tri& T = F.tris[rec];
-        for (int a = 0; a < 3; a++) {
-            box[a] = std::min(std::min(T.v0[a], T.v1[a]), T.v2[a]);
-            box[3 + a] = std::max(std::max(T.v0[a], T.v1[a]), T.v2[a]);
+    WideBuilder(const FlatTree& f, std::vector<wr_wnode>& o) : F(f), out(o) {
+        const size_t n = F.tris.size();
+        box.resize(n);
+        cen.resize(n);
+        idx.resize(n);
+        for (size_t r = 0; r < n; r++) {
+            const wr_tri& T = F.tris[r];
+            for (int a = 0; a < 3; a++) {
+                box[r][a] = std::min(std::min(T.v0[a], T.v1[a]), T.v2[a]);
+                box[r][3 + a] = std::max(std::max(T.v0[a], T.v1[a]), T.v2[a]);
+                cen[r][a] = 0.5f * box[r][a] + 0.5f * box[r][3 + a];
+            }
+            idx[r] = (uint32_t)r;
         }
+    }
+
+    void range_box(uint32_t b, uint32_t e, float* bx) const {
+        for (int a = 0; a < 3; a++) {
+            bx[a] = INFINITY;
+            bx[3 + a] = -INFINITY;
+        }
+        for (uint32_t i = b; i < e; i++) {
+            const auto& t = box[idx[i]];
+            for (int a = 0; a < 3; a++) {
+                bx[a] = std::min(bx[a], t[a]);
+                bx[3 + a] = std::max(bx[3 + a], t[3 + a]);
+            }
+        }
+    }
+
+    static double half_area(const float* lo, const float* hi) {
+        const double x = std::max(0.0, (double)hi[0] - lo[0]), y = std::max(0.0, (double)hi[1] - lo[1]),
+                     z = std::max(0.0, (double)hi[2] - lo[2]);
+        return x * y + y * z + z * x;
+    }
+
+    // Splits [b, e) (at least two records) in two by a binned SAH over the
+    // box centres; returns the split point.  Deterministic.
+    uint32_t split(uint32_t b, uint32_t e) {
+        constexpr int kBins = 16;
+        float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t i = b; i < e; i++)
+            for (int a = 0; a < 3; a++) {
+                clo[a] = std::min(clo[a], cen[idx[i]][a]);
+                chi[a] = std::max(chi[a], cen[idx[i]][a]);
+            }
+        double best = INFINITY;
+        int best_axis = -1, best_bin = 0;
+        for (int a = 0; a < 3; a++) {
+            const double ext = (double)chi[a] - clo[a];
+            if (!(ext > 0)) continue;
+            uint32_t cnt[kBins] = {};
+            float blo[kBins][3], bhi[kBins][3];
+            for (int k = 0; k < kBins; k++)
+                for (int c = 0; c < 3; c++) {
+                    blo[k][c] = INFINITY;
+                    bhi[k][c] = -INFINITY;
+                }
+            const double sc = kBins / ext;
+            for (uint32_t i = b; i < e; i++) {
+                const uint32_t r = idx[i];
+                const int k = std::min(kBins - 1, (int)(((double)cen[r][a] - clo[a]) * sc));
+                cnt[k]++;
+                for (int c = 0; c < 3; c++) {
+                    blo[k][c] = std::min(blo[k][c], box[r][c]);
+                    bhi[k][c] = std::max(bhi[k][c], box[r][3 + c]);
+                }
+            }
+            double lcost[kBins];  // cost of bins [0, k] on the left
+            float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            uint32_t n = 0;
+            for (int k = 0; k < kBins - 1; k++) {
+                n += cnt[k];
+                for (int c = 0; c < 3; c++) {
+                    lo[c] = std::min(lo[c], blo[k][c]);
+                    hi[c] = std::max(hi[c], bhi[k][c]);
+                }
+                lcost[k] = n ? half_area(lo, hi) * n : 0.0;
+            }
+            for (int c = 0; c < 3; c++) {
+                lo[c] = INFINITY;
+                hi[c] = -INFINITY;
+            }
+            n = 0;
+            for (int k = kBins - 1; k > 0; k--) {
+                n += cnt[k];
+                for (int c = 0; c < 3; c++) {
+                    lo[c] = std::min(lo[c], blo[k][c]);
+                    hi[c] = std::max(hi[c], bhi[k][c]);
+                }
+                const uint32_t nl = (e - b) - n;
+                if (n == 0 || nl == 0) continue;
+                const double cost = lcost[k - 1] + half_area(lo, hi) * n;
+                if (cost < best) {
+                    best = cost;
+                    best_axis = a;
+                    best_bin = k;
+                }
+            }
+        }
+        if (best_axis < 0) return b + (e - b) / 2;  // coincident centres: split by order
+        const double ext = (double)chi[best_axis] - clo[best_axis];
+        const double sc = kBins / ext;
+        const int a = best_axis;
+        const float c0 = clo[a];
+        auto mid = std::stable_partition(idx.begin() + b, idx.begin() + e, [&](uint32_t r) {
+            return std::min(kBins - 1, (int)(((double)cen[r][a] - c0) * sc)) < best_bin;
+        });
+        const uint32_t m = (uint32_t)(mid - idx.begin());
+        if (m == b || m == e) return b + (e - b) / 2;
+        return m;
+    }
+
+    // Wide subtree over the records idx[b, e) (at least two).
+    uint32_t build(uint32_t b, uint32_t e) {
+        uint32_t cb[4], ce[4];
+        int n = 0;
+        if (e - b <= 4) {
+            for (uint32_t i = b; i < e; i++) {
+                cb[n] = i;
+                ce[n] = i + 1;
+                n++;
+            }
+        } else {
+            const uint32_t m = split(b, e);
+            const uint32_t halves[2][2] = {{b, m}, {m, e}};
+            for (const auto& hv : halves) {
+                if (hv[1] - hv[0] == 1) {
+                    cb[n] = hv[0];
+                    ce[n] = hv[1];
+                    n++;
+                } else {
+                    const uint32_t mm = split(hv[0], hv[1]);
+                    cb[n] = hv[0];
+                    ce[n] = mm;
+                    n++;
+                    cb[n] = mm;
+                    ce[n] = hv[1];
+                    n++;
+                }
+            }
+        }
+        float cbox[4][6];
+        for (int c = 0; c < n; c++) range_box(cb[c], ce[c], cbox[c]);
+        return make_node(n, cbox, [&](int c) {
+            return ce[c] - cb[c] == 1 ? tri_ref(idx[cb[c]]) : build(cb[c], ce[c]);
+        });
     }
 
     // One wide node over n <= 4 children; child_ref(c) is called after the
     // node's slot is taken (preorder) and returns the child's final reference.
     template <class ChildRef>
     uint32_t make_node(int n, const float (*cbox)[6], ChildRef child_ref) {
-        const uint32_t idx = (uint32_t)out.size();
+        const uint32_t node = (uint32_t)out.size();
         out.push_back(wr_wnode{});
         float origin[3], scale[3];
         for (int a = 0; a < 3; a++) {
@@ -271,19 +417,19 @@ struct Collapser {
             }
             origin[a] = lo;
             const double ext = (double)hi - (double)lo;
-            int e = 0;
+            int ex = 0;
             if (ext > 0) {
-                std::frexp(ext / 255.0, &e);  // ext / 255 < 2^e
+                std::frexp(ext / 255.0, &ex);  // ext / 255 < 2^ex
             } else {
-                e = -100;
+                ex = -100;
             }
-            float s = std::ldexp(1.0f, std::max(e, -126));
-            while (std::fmaf(255.0f, s, origin[a]) < hi) s *= 2.0f;
-            scale[a] = s;
+            float sc = std::ldexp(1.0f, std::max(ex, -126));
+            while (std::fmaf(255.0f, sc, origin[a]) < hi) sc *= 2.0f;
+            scale[a] = sc;
         }
         uint32_t wref[4];
         for (int c = 0; c < n; c++) wref[c] = child_ref(c);
-        wr_wnode& W = out[idx];
+        wr_wnode& W = out[node];
         for (int a = 0; a < 3; a++) {
             W.origin[a] = origin[a];
             W.scale[a] = scale[a];
@@ -300,90 +446,34 @@ struct Collapser {
                 W.ref[c] = WR_NONE;
             }
         }
-        return idx;
+        return node;
     }
 
-    // Wide subtree over the triangles recs (at least two): up to four per
-    // node, larger sets split into four runs along the widest centroid axis.
-    uint32_t tri_tree(std::vector<uint32_t> recs) {
-        const int n = (int)recs.size();
-        float cbox[4][6];
-        if (n <= 4) {
-            for (int c = 0; c < n; c++) tri_box(recs[c], cbox[c]);
-            return make_node(n, cbox, [&](int c) { return tri_ref(recs[c]); });
-        }
-        std::vector<std::array<float, 3>> cen(n);
-        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (int i = 0; i < n; i++) {
-            float b[6];
-            tri_box(recs[i], b);
-            for (int a = 0; a < 3; a++) {
-                cen[i][a] = 0.5f * (b[a] + b[3 + a]);
-                lo[a] = std::min(lo[a], cen[i][a]);
-                hi[a] = std::max(hi[a], cen[i][a]);
-            }
-        }
-        int ax = 0;
-        for (int a = 1; a < 3; a++)
-            if (hi[a] - lo[a] > hi[ax] - lo[ax]) ax = a;
-        std::vector<int> ord(n);
-        for (int i = 0; i < n; i++) ord[i] = i;
-        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cen[x][ax] < cen[y][ax]; });
-        std::vector<uint32_t> part[4];
-        for (int k = 0; k < n; k++) part[(size_t)k * 4 / n].push_back(recs[ord[k]]);
-        for (int c = 0; c < 4; c++) {
-            tri_box(part[c][0], cbox[c]);
-            for (size_t i = 1; i < part[c].size(); i++) {
-                float b[6];
-                tri_box(part[c][i], b);
-                for (int a = 0; a < 3; a++) {
-                    cbox[c][a] = std::min(cbox[c][a], b[a]);
-                    cbox[c][3 + a] = std::max(cbox[c][3 + a], b[3 + a]);
-                }
-            }
-        }
-        return make_node(4, cbox, [&](int c) {
-            return part[c].size() == 1 ? tri_ref(part[c][0]) : tri_tree(part[c]);
-        });
-    }
-
-    uint32_t collapse(uint32_t ref) {
-        if (ref & WR_LEAF_BIT) {
-            const uint32_t first = ref & WR_FIRST_MASK;
-            uint32_t cnt = (ref >> WR_CNT_SHIFT) & WR_CNT_MASK;
-            if (cnt == 0) cnt = F.leaf_cnt[first];
-            for (uint32_t k = 0; k < cnt; k++) F.rec_first[first + k] = first;
-            if (cnt <= 1) return ref;
-            std::vector<uint32_t> recs(cnt);
-            for (uint32_t k = 0; k < cnt; k++) recs[k] = first + k;
-            return tri_tree(std::move(recs));
-        }
-        uint32_t cref[4];
-        float cbox[4][6];
-        int n = 0;
-        auto put = [&](uint32_t r, const float* mn, const float* mx) {
-            cref[n] = r;
-            for (int a = 0; a < 3; a++) {
-                cbox[n][a] = mn[a];
-                cbox[n][3 + a] = mx[a];
-            }
-            n++;
-        };
-        auto add_side = [&](uint32_t r, const float* mn, const float* mx) {
-            if (r & WR_LEAF_BIT) {
-                put(r, mn, mx);
-            } else {
-                const wr_node& C = F.nodes[r];
-                put(C.lref, C.lmin, C.lmax);
-                put(C.rref, C.rmin, C.rmax);
-            }
-        };
-        const wr_node& N = F.nodes[ref];
-        add_side(N.lref, N.lmin, N.lmax);
-        add_side(N.rref, N.rmin, N.rmax);
-        return make_node(n, cbox, [&](int c) { return collapse(cref[c]); });
+    uint32_t build_root() {
+        const uint32_t n = (uint32_t)idx.size();
+        if (n == 0) return WR_NONE;
+        if (n == 1) return tri_ref(0);
+        return build(0, n);
     }
 };
+
+// record -> first record of its reference leaf, for every leaf under ref
+void fill_rec_first(FlatTree& F, uint32_t ref) {
+    std::vector<uint32_t> stack{ref};
+    while (!stack.empty()) {
+        const uint32_t r = stack.back();
+        stack.pop_back();
+        if (r & WR_LEAF_BIT) {
+            const uint32_t first = r & WR_FIRST_MASK;
+            uint32_t cnt = (r >> WR_CNT_SHIFT) & WR_CNT_MASK;
+            if (cnt == 0) cnt = F.leaf_cnt[first];
+            for (uint32_t k = 0; k < cnt; k++) F.rec_first[first + k] = first;
+        } else {
+            stack.push_back(F.nodes[r].lref);
+            stack.push_back(F.nodes[r].rref);
+        }
+    }
+}
 
 uint32_t internal_depth(const BNode* n) {
     if (!n->left) return 0;
@@ -455,9 +545,10 @@ FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32
     F.hdr.n_nodes = (uint32_t)F.nodes.size();
     F.hdr.depth = internal_depth(&root);
     F.rec_first.resize(F.tris.size());
-    Collapser col{F, F.wnodes};
-    F.wnodes.reserve(F.nodes.size() / 2 + F.tris.size() / 3 + 1);
-    F.hdr.wide_root = col.collapse(rref);
+    fill_rec_first(F, rref);
+    WideBuilder wb(F, F.wnodes);
+    F.wnodes.reserve(F.tris.size() / 2 + 1);
+    F.hdr.wide_root = wb.build_root();
     F.hdr.n_wide = (uint32_t)F.wnodes.size();
     F.n_leaves = fl.leaves;
     F.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
