@@ -110,6 +110,11 @@ int wr_scene_add_mesh(wr_scene* scene, const float* verts, uint32_t nv, const ui
                       const float* normals, const wr_material* mat, uint32_t* obj_id);
 /* Scene::buildKDTree + buildObjKDTrees (scene.cpp:20-65) with
  * KDTree::build(BEST) (kdtree.cpp:242-258), flattened and uploaded. */
+/* wr_scene_add_mesh with shading normals per triangle corner (nt * 9): the
+ * world normals Mesh::getWorldNormal interpolates (mesh.h:59-72), for meshes
+ * whose normal indices differ from their position indices. */
+int wr_scene_add_mesh_corners(wr_scene* scene, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                              const float* corner_normals, const wr_material* mat, uint32_t* obj_id);
 int wr_scene_build(wr_scene* scene, int strategy);
 int wr_scene_get_info(const wr_scene* scene, wr_scene_info* info);
 /* prim -> (object id, triangle id in the object): the ObjSourceInfo of
@@ -201,6 +206,48 @@ int wr_film_write(wr_film* film, const float* sum, const float* comp, const floa
  * [passes, passes + spp) of the film, accumulated into it.  stats nullable. */
 int wr_render(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, const wr_render_params* params,
               wr_film* film, wr_render_stats* stats);
+
+/* ---- scene files ---------------------------------------------------------- */
+/* Config::load (config.cpp:432-575): reads a reference scene configuration
+ * (Config -> Scene -> GroupObject / Object / Camera, materials by path and
+ * name) with its .obj / .shp shapes, unitized and transformed as the
+ * reference does (mesh.cpp:5-174, mesh.h:106).  Only the four hot object
+ * classes, static transforms and the pinhole camera are accepted; anything
+ * else (other classes, motion, depth of field, refractive decay) fails with
+ * a message.  Host only. */
+typedef struct wr_config wr_config;
+typedef struct {
+    uint32_t n_objects;
+    int32_t spp;          /* <iter> (INT_MAX if absent); 0 when there is no <renderer> */
+    int32_t max_len;      /* the loader's fixed maxPathLen, 20 (config.cpp:523); 0 without <renderer> */
+    wr_camera camera;
+    char renderer[32];    /* <renderer> as written ("PathTracer", "PT", ...), "" if absent */
+    char scene_name[64];
+} wr_config_info;
+int wr_config_load(const char* path, wr_config** out);
+void wr_config_free(wr_config* cfg);
+int wr_config_get_info(const wr_config* cfg, wr_config_info* info);
+/* Object i in Scene::objects order: world-space vertices (nv * 3), triangles
+ * (nt * 3), world shading normals per triangle corner (nt * 9, only when
+ * *has_normals), material.  Pass NULL arrays to get the sizes first. */
+int wr_config_get_object(const wr_config* cfg, uint32_t i, uint32_t* nv, uint32_t* nt, uint32_t* has_normals,
+                         float* verts, uint32_t* tris, float* corner_normals, wr_material* mat);
+/* wr_scene_add_mesh_corners for every object, in order. */
+int wr_config_add_to_scene(const wr_config* cfg, wr_scene* scene);
+
+/* Mesh::loadShape(path, normalize) (mesh.cpp:5-130) for one .obj or .shp file:
+ * positions, faces, vertex normals and per-face normal indices, unitized when
+ * normalize != 0 (mesh.cpp:152-165).  Host only. */
+typedef struct wr_shape wr_shape;
+int wr_shape_load(const char* path, int normalize, wr_shape** out);
+void wr_shape_free(wr_shape* shape);
+int wr_shape_get(const wr_shape* shape, uint32_t* nv, uint32_t* nt, uint32_t* nn, uint32_t* nfn, float* verts,
+                 uint32_t* tris, float* normals, uint32_t* normal_tris);
+
+/* Film::setBuffer + Film::savePFM (film.h:36-42,100-110): rgb is width x
+ * height x 3 floats, row-major, top row first (wr_film_read's mean_rgb); the
+ * file has the reference's header and rows bottom to top.  Host only. */
+int wr_pfm_write(const char* path, const float* rgb, uint32_t width, uint32_t height);
 
 /* ---- serving (no counterpart in the reference: RenderEngine runs one job at
  * a time from a key loop, renderEngine.cpp:72-91) ------------------------- */

@@ -76,6 +76,11 @@ class JobStatus(C.Structure):
                 ("done_s", C.c_double), ("samples", C.c_uint64), ("device_bytes", C.c_uint64)]
 
 
+class ConfigInfo(C.Structure):
+    _fields_ = [("n_objects", C.c_uint32), ("spp", C.c_int32), ("max_len", C.c_int32), ("camera", CameraDesc),
+                ("renderer", C.c_char * 32), ("scene_name", C.c_char * 64)]
+
+
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 
 
@@ -137,6 +142,16 @@ def lib():
         "wr_job_read": (i32, [vp, u64, vp, vp, vp, vp, P(u32)]),
         "wr_job_release": (i32, [vp, u64]),
         "wr_server_clock": (C.c_double, [vp]),
+        "wr_scene_add_mesh_corners": (i32, [vp, vp, u32, vp, u32, vp, P(Material), P(u32)]),
+        "wr_config_load": (i32, [C.c_char_p, P(vp)]),
+        "wr_config_free": (None, [vp]),
+        "wr_config_get_info": (i32, [vp, P(ConfigInfo)]),
+        "wr_config_get_object": (i32, [vp, u32, P(u32), P(u32), P(u32), vp, vp, vp, P(Material)]),
+        "wr_config_add_to_scene": (i32, [vp, vp]),
+        "wr_shape_load": (i32, [C.c_char_p, i32, P(vp)]),
+        "wr_shape_free": (None, [vp]),
+        "wr_shape_get": (i32, [vp, P(u32), P(u32), P(u32), P(u32), vp, vp, vp, vp]),
+        "wr_pfm_write": (i32, [C.c_char_p, vp, u32, u32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -201,6 +216,94 @@ def build_tree_wide(verts, tris, nthreads=0):
                 "record_tri": rt[:nr.value]}
     finally:
         L.wr_tree_free(h)
+
+
+def load_shape(path, normalize=True):
+    """Mesh::loadShape of one .obj / .shp file (host only): dict of numpy arrays."""
+    L = lib()
+    h = C.c_void_p()
+    _check(L.wr_shape_load(os.fsencode(path), int(bool(normalize)), C.byref(h)), None, "wr_shape_load")
+    try:
+        nv, nt, nn, nfn = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(L.wr_shape_get(h, C.byref(nv), C.byref(nt), C.byref(nn), C.byref(nfn), None, None, None, None), None,
+               "wr_shape_get")
+        v = np.zeros((nv.value, 3), np.float32)
+        f = np.zeros((nt.value, 3), np.uint32)
+        vn = np.zeros((nn.value, 3), np.float32)
+        fn = np.zeros((nfn.value, 3), np.uint32)
+        _check(L.wr_shape_get(h, C.byref(nv), C.byref(nt), C.byref(nn), C.byref(nfn), _p(v), _p(f), _p(vn), _p(fn)),
+               None, "wr_shape_get")
+        return {"verts": v, "tris": f, "normals": vn, "normal_tris": fn}
+    finally:
+        L.wr_shape_free(h)
+
+
+def write_pfm(path, rgb, width=None, height=None):
+    """Film::setBuffer + savePFM of an image, top row first: H x W x 3, or flat with width and height."""
+    a = np.ascontiguousarray(rgb, dtype=np.float32)
+    if width is None or height is None:
+        height, width = a.shape[0], a.shape[1]
+    if a.size != 3 * width * height:
+        raise ValueError(f"write_pfm: {a.size} values for a {width} x {height} RGB image")
+    _check(lib().wr_pfm_write(os.fsencode(path), _p(a), width, height), None, "wr_pfm_write")
+
+
+class SceneConfig:
+    """Config::load of a reference scene configuration (host only)."""
+
+    def __init__(self, path):
+        self.h = C.c_void_p()
+        _check(lib().wr_config_load(os.fsencode(path), C.byref(self.h)), None, "wr_config_load")
+        info = ConfigInfo()
+        _check(lib().wr_config_get_info(self.h, C.byref(info)), None, "wr_config_get_info")
+        c = info.camera
+        self.camera = {"pos": list(c.pos), "focus": list(c.focus), "up": list(c.up), "width": c.width,
+                       "height": c.height, "sight_dist": c.sight_dist}
+        self.spp, self.max_len = info.spp, info.max_len
+        self.renderer = info.renderer.decode()
+        self.name = info.scene_name.decode()
+        self.objects = [self._object(i) for i in range(info.n_objects)]
+
+    def _object(self, i):
+        L = lib()
+        nv, nt, hn = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        m = Material()
+        _check(L.wr_config_get_object(self.h, i, C.byref(nv), C.byref(nt), C.byref(hn), None, None, None, None),
+               None, "wr_config_get_object")
+        v = np.zeros((nv.value, 3), np.float32)
+        t = np.zeros((nt.value, 3), np.uint32)
+        cn = np.zeros((nt.value, 3, 3), np.float32) if hn.value else None
+        _check(L.wr_config_get_object(self.h, i, C.byref(nv), C.byref(nt), C.byref(hn), _p(v), _p(t), _p(cn),
+                                      C.byref(m)), None, "wr_config_get_object")
+        if m.type == WR_REFRACTIVE:
+            params = (m.ior,)
+        elif m.type == WR_GLOSSY:
+            params = tuple(m.color) + (m.power,)
+        else:
+            params = tuple(m.color)
+        # per-vertex normals when every corner of a vertex carries the same normal
+        normals = None
+        if cn is not None:
+            normals = np.zeros_like(v)
+            normals[t.reshape(-1)] = cn.reshape(-1, 3)
+            if not np.array_equal(normals[t.reshape(-1)], cn.reshape(-1, 3)):
+                normals = None
+        return {"verts": v, "tris": t, "normals": normals, "corner_normals": cn, "mat": m.type,
+                "params": tuple(float(x) for x in params)}
+
+    def add_to(self, scene):
+        _check(lib().wr_config_add_to_scene(self.h, scene.h), scene.ctx.h, "wr_config_add_to_scene")
+
+    def close(self):
+        if self.h:
+            lib().wr_config_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceBuffer:

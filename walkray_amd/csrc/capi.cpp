@@ -181,11 +181,23 @@ int wr_scene_add_mesh(wr_scene* s, const float* verts, uint32_t nv, const uint32
     HostObject o;
     o.verts.assign(verts, verts + 3 * (size_t)nv);
     o.idx.assign(idx, idx + 3 * (size_t)nt);
-    if (normals) o.normals.assign(normals, normals + 3 * (size_t)nv);
+    if (normals) {  // kept per triangle corner
+        o.normals.resize(9 * (size_t)nt);
+        for (size_t c = 0; c < 3 * (size_t)nt; c++)
+            for (int a = 0; a < 3; a++) o.normals[3 * c + a] = normals[3 * (size_t)idx[c] + a];
+    }
     o.mat = *mat;
     s->objs.push_back(std::move(o));
     s->built = false;
     if (obj_id) *obj_id = (uint32_t)(s->objs.size() - 1);
+    return WR_OK;
+}
+
+int wr_scene_add_mesh_corners(wr_scene* s, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                              const float* corner_normals, const wr_material* mat, uint32_t* obj_id) {
+    const int rc = wr_scene_add_mesh(s, verts, nv, idx, nt, nullptr, mat, obj_id);
+    if (rc != WR_OK || !corner_normals) return rc;
+    s->objs.back().normals.assign(corner_normals, corner_normals + 9 * (size_t)nt);
     return WR_OK;
 }
 
@@ -295,7 +307,7 @@ int wr_scene_build(wr_scene* s, int strategy) {
                     const HostObject& o = s->objs[oi];
                     for (int k = 0; k < 3; k++)
                         for (int a = 0; a < 3; a++)
-                            rec_nrm.push_back(o.normals.empty() ? 0.f : o.normals[3 * (size_t)o.idx[3 * (size_t)local + k] + a]);
+                            rec_nrm.push_back(o.normals.empty() ? 0.f : o.normals[9 * (size_t)local + 3 * k + a]);
                 }
             }
         }

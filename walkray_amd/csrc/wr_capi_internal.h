@@ -21,7 +21,7 @@ struct wr_ctx {
 
 namespace wr {
 struct HostObject {
-    std::vector<float> verts, normals;
+    std::vector<float> verts, normals;  // normals: 9 per triangle (corners), empty for flat shading
     std::vector<uint32_t> idx;
     wr_material mat;
 };

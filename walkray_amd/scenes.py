@@ -302,3 +302,69 @@ def incoherent_rays(n, lo, hi, seed=11):
     rays[:, :3] = o
     rays[:, 3:] = d
     return rays
+
+
+# ---- reference scene files ----------------------------------------------------
+_MAT_TYPES = {DIFFUSE: "Diffuse", GLOSSY: "Glossy", REFRACTIVE: "Refractive", EMISSIVE: "Emissive"}
+
+
+def _unitize_frame(verts):
+    """Centre and half extent the reference loader computes (mesh.cpp:152-165), in float32."""
+    v = np.asarray(verts, np.float32)
+    lo, hi = v.min(axis=0), v.max(axis=0)
+    c = (lo + hi) / np.float32(2)
+    d = hi - lo
+    m = max(d[0], d[1])
+    m = max(m, d[2])
+    return c, np.float32(m) / np.float32(2)
+
+
+def write_reference_scene(directory, objs, cam, spp=16, name="Scene"):
+    """Writes objs and cam as a reference scene configuration that Config::load
+    (config.cpp:432-575) reads: config.xml, scene.xml, materials.xml and one
+    .obj per object.  The loader unitizes every shape; each object's transform
+    scales and translates it back, so the loaded vertices equal the originals
+    up to float rounding.  Returns the path of config.xml."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    f9 = "{:.9g}".format
+    mats, objx = [], []
+    for i, o in enumerate(objs):
+        v = np.asarray(o["verts"], np.float32)
+        t = np.asarray(o["tris"], np.int64) + 1
+        lines = [f"v {f9(a)} {f9(b)} {f9(c)}" for a, b, c in v]
+        if o["normals"] is not None:
+            lines += [f"vn {f9(a)} {f9(b)} {f9(c)}" for a, b, c in np.asarray(o["normals"], np.float32)]
+            lines += [f"f {a}//{a} {b}//{b} {c}//{c}" for a, b, c in t]
+        else:
+            lines += [f"f {a} {b} {c}" for a, b, c in t]
+        with open(os.path.join(directory, f"object{i}.obj"), "w") as fh:
+            fh.write("\n".join(lines) + "\n")
+        p = o["params"]
+        body = f"<type>{_MAT_TYPES[o['mat']]}</type>"
+        if o["mat"] == REFRACTIVE:
+            body += f"<IOR>{f9(p[0])}</IOR>"
+        else:
+            body += f"<color>({f9(p[0])}, {f9(p[1])}, {f9(p[2])})</color>"
+            if o["mat"] == GLOSSY:
+                body += f"<power>{f9(p[3])}</power>"
+        mats.append(f'<Material Name="m{i}">{body}</Material>')
+        c, s = _unitize_frame(v)
+        tr = (f"[{f9(s)}, 0, 0, {f9(c[0])}][0, {f9(s)}, 0, {f9(c[1])}]"
+              f"[0, 0, {f9(s)}, {f9(c[2])}][0, 0, 0, 1]")
+        objx.append(f"<Object><filePath>object{i}.obj</filePath>"
+                    f"<Material><path>materials.xml</path><name>m{i}</name></Material>"
+                    f"<transform>{tr}</transform></Object>")
+    vec = lambda x: f"({f9(x[0])}, {f9(x[1])}, {f9(x[2])})"  # noqa: E731
+    camx = (f'<Camera Name="main"><position>{vec(cam["pos"])}</position><focus>{vec(cam["focus"])}</focus>'
+            f'<up>{vec(cam["up"])}</up><width>{cam["width"]}</width><height>{cam["height"]}</height>'
+            f'<sightDist>{f9(cam["sight_dist"])}</sightDist></Camera>')
+    with open(os.path.join(directory, "materials.xml"), "w") as fh:
+        fh.write("\n".join(mats) + "\n")
+    with open(os.path.join(directory, "scene.xml"), "w") as fh:
+        fh.write(f'<Scene Name="{name}">\n' + "\n".join(objx) + "\n" + camx + "\n</Scene>\n")
+    path = os.path.join(directory, "config.xml")
+    with open(path, "w") as fh:
+        fh.write(f"<Config>\n<Scene><path>scene.xml</path><name>{name}</name><camera>main</camera></Scene>\n"
+                 f"<renderer>PathTracer</renderer><iter>{spp}</iter>\n</Config>\n")
+    return path
