@@ -2,6 +2,8 @@
 // code would drive KDTree / Scene / PathTracer, for tests/test_gpu_mirror.py.
 //
 // usage: mirror_test <in.bin> <out.bin>
+//        mirror_test --config <config.xml> <out.pfm> <spp>
+//   (Config::load, Scene::buildKDTree, PathTracer::renderPixels, savePFM)
 // in : u32 nv, nt, nr, W, H, spp, depth; f32 verts[3nv]; u32 tris[3nt];
 //      f32 rays[6nr]; f32 cam[10] (pos, focus, up, sight_dist)
 // out: u32 prim[nr]; f32 t[nr]            KDTree batch intersect
@@ -11,6 +13,7 @@
 //      f32 rgb[3 W H]                      PathTracer::renderPixels
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "walkray.hpp"
@@ -29,7 +32,28 @@ static void wr(FILE* f, const T* p, size_t n) {
     if (n) fwrite(p, sizeof(T), n, f);
 }
 
+static int render_config(const char* cfg_path, const char* pfm_path, unsigned spp) {
+    try {
+        Context ctx(0);
+        Config cfg(cfg_path);
+        Scene scene(ctx);
+        cfg.load(scene);
+        scene.buildKDTree();
+        PathTracer pt(scene, spp, (unsigned)cfg.maxPathLen(), 7);
+        const wr_camera& cam = cfg.camera();
+        std::vector<vec3f> px = pt.renderPixels(cam);
+        savePFM(pfm_path, px, cam.width, cam.height);
+        printf("mirror_test: %s (%s, %s), %ux%u, %u spp, maxPathLen %d\n", cfg_path, cfg.sceneName().c_str(),
+               cfg.renderer().c_str(), cam.width, cam.height, spp, cfg.maxPathLen());
+    } catch (const std::exception& e) {
+        fprintf(stderr, "mirror_test: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc == 5 && std::string(argv[1]) == "--config") return render_config(argv[2], argv[3], (unsigned)atoi(argv[4]));
     if (argc != 3) {
         fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]);
         return 2;

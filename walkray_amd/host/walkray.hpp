@@ -7,6 +7,8 @@
 //   Scene       scene.h:10-45   (ObjSourceInfo, buildKDTree, intersect,
 //                                findInsideObject)
 //   PathTracer  pathTracer.h:7-18, mcRenderer.h:14-36 (spp, maxLen, renderPixels)
+//   Config      config.h        (load: scene objects, camera, renderer settings)
+//   savePFM     film.h:100-110
 // Differences, all forced by the GPU boundary:
 //   * a Condition cannot run arbitrary host code inside the GPU traversal:
 //     KDTree::intersect accepts only a NULL condition, and the reference's one
@@ -247,5 +249,37 @@ class PathTracer {
     uint64_t seed_;
     uint32_t mode_;
 };
+
+// Config (config.h): load reads a reference scene configuration into a Scene
+// (objects in Scene::objects order, config.cpp:432-575) and keeps the camera
+// and the renderer settings it names.
+class Config {
+  public:
+    explicit Config(const std::string& path) {
+        check(wr_config_load(path.c_str(), &cfg_), nullptr, "Config::load");
+        check(wr_config_get_info(cfg_, &info_), nullptr, "Config::load");
+    }
+    ~Config() { wr_config_free(cfg_); }
+    Config(const Config&) = delete;
+    Config& operator=(const Config&) = delete;
+
+    // adds every object to scene; scene.buildKDTree() follows, as in Config::load
+    void load(Scene& scene) const { check(wr_config_add_to_scene(cfg_, scene.get()), scene.ctx(), "Config::load"); }
+    const wr_camera& camera() const { return info_.camera; }
+    int spp() const { return info_.spp; }
+    int maxPathLen() const { return info_.max_len; }
+    std::string renderer() const { return info_.renderer; }
+    std::string sceneName() const { return info_.scene_name; }
+
+  private:
+    wr_config* cfg_ = nullptr;
+    wr_config_info info_{};
+};
+
+// Film::setBuffer + Film::savePFM (film.h:36-42,100-110) of renderPixels' output.
+inline void savePFM(const std::string& filename, const std::vector<vec3f>& pixels, unsigned width, unsigned height) {
+    if (pixels.size() != (size_t)width * height) throw std::runtime_error("savePFM: pixel count != width * height");
+    check(wr_pfm_write(filename.c_str(), pixels.empty() ? nullptr : &pixels[0].x, width, height), nullptr, "savePFM");
+}
 
 }  // namespace walkray
