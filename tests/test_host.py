@@ -169,3 +169,12 @@ def test_wide_hierarchy_covers_every_record_once(case):
     leaves = ex["right"] == 0xFFFFFFFF
     for f, c in zip(ex["leaf_first"][leaves], ex["leaf_count"][leaves]):
         assert np.all(wide["record_first"][f:f + c] == f)
+
+
+def test_wide_hierarchy_threads_change_speed_not_result():
+    o = S.heightfield(300, seed=6)  # 178,802 triangles: the parallel build path
+    a = W.build_tree_wide(o["verts"], o["tris"], nthreads=1)
+    b = W.build_tree_wide(o["verts"], o["tris"], nthreads=8)
+    assert a["root"] == b["root"]
+    assert np.array_equal(a["nodes"].view(np.uint8), b["nodes"].view(np.uint8))
+    assert np.array_equal(a["record_first"], b["record_first"])

@@ -366,9 +366,9 @@ struct WideBuilder {
         return m;
     }
 
-    // Wide subtree over the records idx[b, e) (at least two).
-    uint32_t build(uint32_t b, uint32_t e) {
-        uint32_t cb[4], ce[4];
+    // Child record ranges of a wide node over idx[b, e) (at least two
+    // records): single records for four or fewer, else two SAH splits.
+    int child_ranges(uint32_t b, uint32_t e, uint32_t* cb, uint32_t* ce) {
         int n = 0;
         if (e - b <= 4) {
             for (uint32_t i = b; i < e; i++) {
@@ -376,38 +376,46 @@ struct WideBuilder {
                 ce[n] = i + 1;
                 n++;
             }
-        } else {
-            const uint32_t m = split(b, e);
-            const uint32_t halves[2][2] = {{b, m}, {m, e}};
-            for (const auto& hv : halves) {
-                if (hv[1] - hv[0] == 1) {
-                    cb[n] = hv[0];
-                    ce[n] = hv[1];
-                    n++;
-                } else {
-                    const uint32_t mm = split(hv[0], hv[1]);
-                    cb[n] = hv[0];
-                    ce[n] = mm;
-                    n++;
-                    cb[n] = mm;
-                    ce[n] = hv[1];
-                    n++;
-                }
+            return n;
+        }
+        const uint32_t m = split(b, e);
+        const uint32_t halves[2][2] = {{b, m}, {m, e}};
+        for (const auto& hv : halves) {
+            if (hv[1] - hv[0] == 1) {
+                cb[n] = hv[0];
+                ce[n] = hv[1];
+                n++;
+            } else {
+                const uint32_t mm = split(hv[0], hv[1]);
+                cb[n] = hv[0];
+                ce[n] = mm;
+                n++;
+                cb[n] = mm;
+                ce[n] = hv[1];
+                n++;
             }
         }
+        return n;
+    }
+
+    // Wide subtree over the records idx[b, e) (at least two), appended to o.
+    uint32_t build(std::vector<wr_wnode>& o, uint32_t b, uint32_t e) {
+        uint32_t cb[4], ce[4];
+        const int n = child_ranges(b, e, cb, ce);
         float cbox[4][6];
         for (int c = 0; c < n; c++) range_box(cb[c], ce[c], cbox[c]);
-        return make_node(n, cbox, [&](int c) {
-            return ce[c] - cb[c] == 1 ? tri_ref(idx[cb[c]]) : build(cb[c], ce[c]);
+        return make_node(o, n, cbox, [&](int c) {
+            return ce[c] - cb[c] == 1 ? tri_ref(idx[cb[c]]) : build(o, cb[c], ce[c]);
         });
     }
 
-    // One wide node over n <= 4 children; child_ref(c) is called after the
-    // node's slot is taken (preorder) and returns the child's final reference.
+    // One wide node over n <= 4 children, appended to o; child_ref(c) is called
+    // after the node's slot is taken (preorder) and returns the child's final
+    // reference.
     template <class ChildRef>
-    uint32_t make_node(int n, const float (*cbox)[6], ChildRef child_ref) {
-        const uint32_t node = (uint32_t)out.size();
-        out.push_back(wr_wnode{});
+    uint32_t make_node(std::vector<wr_wnode>& o, int n, const float (*cbox)[6], ChildRef child_ref) {
+        const uint32_t node = (uint32_t)o.size();
+        o.push_back(wr_wnode{});
         float origin[3], scale[3];
         for (int a = 0; a < 3; a++) {
             float lo = cbox[0][a], hi = cbox[0][3 + a];
@@ -429,7 +437,7 @@ struct WideBuilder {
         }
         uint32_t wref[4];
         for (int c = 0; c < n; c++) wref[c] = child_ref(c);
-        wr_wnode& W = out[node];
+        wr_wnode& W = o[node];
         for (int a = 0; a < 3; a++) {
             W.origin[a] = origin[a];
             W.scale[a] = scale[a];
@@ -449,11 +457,82 @@ struct WideBuilder {
         return node;
     }
 
-    uint32_t build_root() {
+    // Parallel build: the top levels are split here, the subtrees below them
+    // are built by worker threads into their own arrays and spliced in
+    // preorder, so the result is the sequential build's, node for node.
+    struct PlanNode {
+        int n = 0;
+        uint32_t cb[4], ce[4];
+        int kind[4];  // 0: one record, 1: task, 2: plan node
+        uint32_t id[4];
+    };
+    struct Task {
+        uint32_t b, e;
+        std::vector<wr_wnode> nodes;
+        uint32_t root;
+    };
+    std::vector<PlanNode> plans;
+    std::vector<Task> tasks;
+
+    uint32_t plan(uint32_t b, uint32_t e, int level, uint32_t grain) {
+        PlanNode P;
+        P.n = child_ranges(b, e, P.cb, P.ce);
+        for (int c = 0; c < P.n; c++) {
+            const uint32_t sz = P.ce[c] - P.cb[c];
+            if (sz == 1) {
+                P.kind[c] = 0;
+            } else if (level >= 3 || sz <= grain) {
+                P.kind[c] = 1;
+                P.id[c] = (uint32_t)tasks.size();
+                tasks.push_back(Task{P.cb[c], P.ce[c], {}, 0});
+            } else {
+                P.kind[c] = 2;
+                P.id[c] = plan(P.cb[c], P.ce[c], level + 1, grain);
+            }
+        }
+        plans.push_back(P);
+        return (uint32_t)plans.size() - 1;
+    }
+
+    uint32_t emit(uint32_t pid) {
+        const PlanNode P = plans[pid];
+        float cbox[4][6];
+        for (int c = 0; c < P.n; c++) range_box(P.cb[c], P.ce[c], cbox[c]);
+        return make_node(out, P.n, cbox, [&](int c) -> uint32_t {
+            if (P.kind[c] == 0) return tri_ref(idx[P.cb[c]]);
+            if (P.kind[c] == 2) return emit(P.id[c]);
+            Task& T = tasks[P.id[c]];
+            const uint32_t off = (uint32_t)out.size();
+            for (wr_wnode w : T.nodes) {
+                for (int k = 0; k < 4; k++)
+                    if (w.ref[k] != WR_NONE && !(w.ref[k] & WR_LEAF_BIT)) w.ref[k] += off;
+                out.push_back(w);
+            }
+            std::vector<wr_wnode>().swap(T.nodes);
+            return T.root + off;
+        });
+    }
+
+    uint32_t build_root(int nthreads) {
         const uint32_t n = (uint32_t)idx.size();
         if (n == 0) return WR_NONE;
         if (n == 1) return tri_ref(0);
-        return build(0, n);
+        if (nthreads <= 1 || n < (1u << 16)) return build(out, 0, n);
+        const uint32_t grain = std::max<uint32_t>(4096, n / (uint32_t)(8 * nthreads));
+        const uint32_t root_plan = plan(0, n, 0, grain);
+        std::atomic<size_t> next{0};
+        auto worker = [&]() {
+            for (size_t t; (t = next.fetch_add(1)) < tasks.size();) {
+                Task& T = tasks[t];
+                T.nodes.reserve((T.e - T.b) / 2 + 1);
+                T.root = build(T.nodes, T.b, T.e);
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int i = 1; i < nthreads; i++) pool.emplace_back(worker);
+        worker();
+        for (auto& th : pool) th.join();
+        return emit(root_plan);
     }
 };
 
@@ -548,7 +627,7 @@ FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32
     fill_rec_first(F, rref);
     WideBuilder wb(F, F.wnodes);
     F.wnodes.reserve(F.tris.size() / 2 + 1);
-    F.hdr.wide_root = wb.build_root();
+    F.hdr.wide_root = wb.build_root(nthreads);
     F.hdr.n_wide = (uint32_t)F.wnodes.size();
     F.n_leaves = fl.leaves;
     F.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
