@@ -57,6 +57,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def rng_pick(n, k, seed):
+    """k distinct indices of range(n), sorted (a spread-out sample of a ray set)."""
+    return np.sort(np.random.default_rng(seed).choice(n, k, replace=False))
+
+
 def closest_hit_bench(W, S, ctx, steps, warmup, n_side=4096, hf_n=1024):
     t0 = time.time()
     objs = S.heightfield_scene(hf_n, seed=7)
@@ -73,6 +78,14 @@ def closest_hit_bench(W, S, ctx, steps, warmup, n_side=4096, hf_n=1024):
     d_inc = ctx.to_device(inc)
     d_hits = ctx.alloc(16 * n)
     log(f"[bench] rays resident: 2 x {n} ({time.time() - t0:.1f}s)")
+
+    # autotune the persistent kernel's launch knobs on a sample of both sets (results never depend on them)
+    t1 = time.time()
+    k = min(n, 1 << 22)
+    sample = ctx.to_device(np.concatenate([coh[rng_pick(n, k, 1)], inc[:k]]))
+    tuning = sc.autotune_closest_hit(sample, 2 * k)
+    del sample
+    log(f"[bench] closest-hit autotune: {tuning} ({time.time() - t1:.1f}s)")
 
     # counter-enabled run on the same rays: box records and triangle records fetched
     counters = {}
@@ -138,6 +151,7 @@ def closest_hit_bench(W, S, ctx, steps, warmup, n_side=4096, hf_n=1024):
         "avg_launch_ms": round(avg_launch_ms, 4),
         "alg_bytes_per_launch": int(total_b / 2),
     }
+    roofline["launch_tuning"] = {kk: tuning[kk] for kk in ("refill_min", "node_phase_min", "blocks_per_cu")}
     return {
         "scene": sc, "objs": objs, "coh": coh, "inc": inc, "n": n, "mrays": mrays, "ms_per_step": ms_per_step,
         "wall_s": wall, "per_set": per_set, "roofline": roofline, "info": info, "hit_frac_inc": hit_frac_inc,

@@ -152,6 +152,22 @@ int wr_tree_wide(const wr_tree* tree, uint32_t* n_wide, uint32_t* n_records, uin
  * accumulated): box records fetched, triangle records tested, fallback rays. */
 int wr_closest_hit(wr_ctx* ctx, const wr_scene* scene, const wr_ray* d_rays, uint64_t n, wr_hit* d_hits,
                    uint64_t* d_counters);
+/* Autotuner (no counterpart in the reference).  Times wr_closest_hit on the
+ * given rays (a representative sample: every call is traced in full) under
+ * candidate launch configurations of the persistent kernel (refill threshold,
+ * node-phase threshold, workgroups per CU), by coordinate descent from the
+ * scene's current choice, and keeps the fastest for the scene; later
+ * wr_closest_hit calls use it.  Results never depend on the configuration.
+ * Synchronous; allocates 16 * n bytes of scratch while it runs. */
+typedef struct {
+    uint32_t refill_min;     /* idle lanes of a wave before it takes new rays (1..64) */
+    uint32_t node_phase_min; /* lanes at wide nodes for a node-only step (1..65; 65 = never) */
+    uint32_t blocks_per_cu;  /* resident 256-lane workgroups per CU, 0 = as many as fit */
+    float ms;                /* one call on the rays with this configuration */
+} wr_closest_hit_tuning;
+int wr_autotune_closest_hit(wr_ctx* ctx, wr_scene* scene, const wr_ray* d_rays, uint64_t n,
+                            wr_closest_hit_tuning* best);
+int wr_scene_set_closest_hit_tuning(wr_scene* scene, const wr_closest_hit_tuning* tuning);
 /* Scene::intersect with NoSelfCondition(scene, ray) (mcRenderer.cpp:42-44,
  * noself.cpp:13-20) for n path rays. */
 int wr_closest_hit_noself(wr_ctx* ctx, const wr_scene* scene, const wr_path_ray* d_rays, uint64_t n,
@@ -248,6 +264,19 @@ int wr_shape_get(const wr_shape* shape, uint32_t* nv, uint32_t* nt, uint32_t* nn
  * height x 3 floats, row-major, top row first (wr_film_read's mean_rgb); the
  * file has the reference's header and rows bottom to top.  Host only. */
 int wr_pfm_write(const char* path, const float* rgb, uint32_t width, uint32_t height);
+
+/* Autotuner for the wavefront engine (no counterpart in the reference): times
+ * spp-pass renders of the camera under candidate path-slot pool sizes (queue
+ * capacity) and stage grids (workgroups per stage launch) and keeps the
+ * fastest in the context; wr_render calls whose wave_slots / wave_grid are 0
+ * use it.  Films never depend on the choice.  Synchronous. */
+typedef struct {
+    uint32_t wave_slots;
+    uint32_t wave_grid;
+    float ms; /* the spp-pass render with this configuration */
+} wr_render_tuning;
+int wr_autotune_render(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, uint32_t max_depth, uint32_t spp,
+                       wr_render_tuning* best);
 
 /* ---- serving (no counterpart in the reference: RenderEngine runs one job at
  * a time from a key loop, renderEngine.cpp:72-91) ------------------------- */

@@ -1,0 +1,39 @@
+"""The autotuners change speed, never results: closest-hit launch knobs at their
+extremes and after tuning, and wavefront queue sizes / stage grids after
+tuning, all give bit-identical outputs."""
+import numpy as np
+import pytest
+
+from walkray_amd import _lib as W
+from walkray_amd import render as R
+from walkray_amd import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+
+def test_closest_hit_tuning_keeps_results(ctx):
+    objs = [S.heightfield(257, seed=9)]
+    sc = W.Scene(ctx, objs)
+    lo, hi = S.scene_bounds(objs)
+    rays = np.concatenate([S.pinhole_rays(S.heightfield_camera(192, 192)),
+                           S.incoherent_rays(40000, lo, hi, seed=2)]).astype(np.float32)
+    base = sc.closest_hit(rays)
+    for refill, node, blocks in ((1, 65, 1), (64, 1, 0), (7, 13, 3)):
+        sc.set_closest_hit_tuning(refill, node, blocks)
+        h = sc.closest_hit(rays)
+        assert np.array_equal(h.view(np.uint32), base.view(np.uint32)), (refill, node, blocks)
+    d = ctx.to_device(rays)
+    t = sc.autotune_closest_hit(d, len(rays))
+    assert 1 <= t["refill_min"] <= 64 and 1 <= t["node_phase_min"] <= 65 and t["ms"] > 0
+    assert np.array_equal(sc.closest_hit(rays).view(np.uint32), base.view(np.uint32))
+
+
+def test_render_tuning_keeps_films(ctx):
+    objs, cam = S.cornell(k=2, width=48, height=40)
+    sc = W.Scene(ctx, objs)
+    a = R.render_scene(ctx, objs, cam, 6, 8, 3, W.WR_WAVEFRONT, scene=sc)
+    t = W.autotune_render(sc, cam, 8, spp=1)
+    assert t["wave_slots"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23) and t["wave_grid"] in (1024, 2048, 4096)
+    b = R.render_scene(ctx, objs, cam, 6, 8, 3, W.WR_WAVEFRONT, scene=sc)
+    for k in ("sum", "comp", "sumsq"):
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k

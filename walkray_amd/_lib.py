@@ -76,6 +76,23 @@ class JobStatus(C.Structure):
                 ("done_s", C.c_double), ("samples", C.c_uint64), ("device_bytes", C.c_uint64)]
 
 
+class ChTuning(C.Structure):
+    _fields_ = [("refill_min", C.c_uint32), ("node_phase_min", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+                ("ms", C.c_float)]
+
+
+class RenderTuning(C.Structure):
+    _fields_ = [("wave_slots", C.c_uint32), ("wave_grid", C.c_uint32), ("ms", C.c_float)]
+
+
+def autotune_render(scene, cam, max_depth, spp=2):
+    """wr_autotune_render: picks the wavefront queue size and stage grid for this context."""
+    t = RenderTuning()
+    _check(lib().wr_autotune_render(scene.ctx.h, scene.h, C.byref(CameraDesc.from_dict(cam)), max_depth, spp,
+                                    C.byref(t)), scene.ctx.h, "wr_autotune_render")
+    return {"wave_slots": t.wave_slots, "wave_grid": t.wave_grid, "ms": t.ms}
+
+
 class ConfigInfo(C.Structure):
     _fields_ = [("n_objects", C.c_uint32), ("spp", C.c_int32), ("max_len", C.c_int32), ("camera", CameraDesc),
                 ("renderer", C.c_char * 32), ("scene_name", C.c_char * 64)]
@@ -142,6 +159,9 @@ def lib():
         "wr_job_read": (i32, [vp, u64, vp, vp, vp, vp, P(u32)]),
         "wr_job_release": (i32, [vp, u64]),
         "wr_server_clock": (C.c_double, [vp]),
+        "wr_autotune_closest_hit": (i32, [vp, vp, vp, u64, P(ChTuning)]),
+        "wr_scene_set_closest_hit_tuning": (i32, [vp, P(ChTuning)]),
+        "wr_autotune_render": (i32, [vp, vp, P(CameraDesc), u32, u32, P(RenderTuning)]),
         "wr_scene_add_mesh_corners": (i32, [vp, vp, u32, vp, u32, vp, P(Material), P(u32)]),
         "wr_config_load": (i32, [C.c_char_p, P(vp)]),
         "wr_config_free": (None, [vp]),
@@ -493,6 +513,19 @@ class Scene:
         _check(lib().wr_closest_hit(self.ctx.h, self.h, d_rays.ptr, n, d_hits.ptr,
                                     d_counters.ptr if d_counters is not None else None),
                self.ctx.h, "wr_closest_hit")
+
+    def autotune_closest_hit(self, d_rays, n):
+        """wr_autotune_closest_hit on resident rays: the chosen launch configuration and its time."""
+        t = ChTuning()
+        _check(lib().wr_autotune_closest_hit(self.ctx.h, self.h, d_rays.ptr, n, C.byref(t)), self.ctx.h,
+               "wr_autotune_closest_hit")
+        return {"refill_min": t.refill_min, "node_phase_min": t.node_phase_min, "blocks_per_cu": t.blocks_per_cu,
+                "ms": t.ms}
+
+    def set_closest_hit_tuning(self, refill_min, node_phase_min, blocks_per_cu=0):
+        t = ChTuning(refill_min, node_phase_min, blocks_per_cu, 0.0)
+        _check(lib().wr_scene_set_closest_hit_tuning(self.h, C.byref(t)), self.ctx.h,
+               "wr_scene_set_closest_hit_tuning")
 
     # -- host convenience wrappers ------------------------------------------
     def closest_hit(self, rays, counters=False):

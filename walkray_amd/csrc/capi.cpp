@@ -560,8 +560,81 @@ int wr_closest_hit(wr_ctx* ctx, const wr_scene* s, const wr_ray* d_rays, uint64_
     if (!ctx || !s || (n && (!d_rays || !d_hits))) return fail(ctx, WR_EINVAL, "wr_closest_hit: bad argument");
     if (!s->built) return fail(ctx, WR_ESTATE, "wr_closest_hit: scene not built");
     hipError_t e = launch_closest_hit(s->dev, s->hdrs[0], (const float*)d_rays, n, d_hits,
-                                      (unsigned long long*)d_counters, ctx->d_work, ctx->stream);
+                                      (unsigned long long*)d_counters, ctx->d_work, s->ch_tune, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "k_closest_hit launch");
+    return WR_OK;
+}
+
+int wr_autotune_closest_hit(wr_ctx* ctx, wr_scene* s, const wr_ray* d_rays, uint64_t n,
+                            wr_closest_hit_tuning* best) {
+    if (!ctx || !s || !d_rays || n == 0) return fail(ctx, WR_EINVAL, "wr_autotune_closest_hit: bad argument");
+    if (!s->built) return fail(ctx, WR_ESTATE, "wr_autotune_closest_hit: scene not built");
+    (void)hipSetDevice(ctx->device);
+    void* d_hits = nullptr;
+    hipError_t e = hipMalloc(&d_hits, 16 * (size_t)n);
+    if (e != hipSuccess) return fail(ctx, WR_ENOMEM, std::string("wr_autotune_closest_hit: ") + hipGetErrorString(e));
+    // time of one call under t: best of three after one warm-up call
+    auto time_of = [&](const ChTune& t, float& ms) -> hipError_t {
+        ms = 1e30f;
+        for (int rep = 0; rep < 4; rep++) {
+            hipError_t r = hipEventRecord(ctx->ev0, ctx->stream);
+            if (r == hipSuccess)
+                r = launch_closest_hit(s->dev, s->hdrs[0], (const float*)d_rays, n, d_hits, nullptr, ctx->d_work, t,
+                                       ctx->stream);
+            if (r == hipSuccess) r = hipEventRecord(ctx->ev1, ctx->stream);
+            if (r == hipSuccess) r = hipEventSynchronize(ctx->ev1);
+            float m = 0.f;
+            if (r == hipSuccess) r = hipEventElapsedTime(&m, ctx->ev0, ctx->ev1);
+            if (r != hipSuccess) return r;
+            if (rep > 0) ms = std::min(ms, m);
+        }
+        return hipSuccess;
+    };
+    // coordinate descent over the three knobs, starting from the current choice;
+    // a candidate must be 1% faster to replace it (timing noise stays put)
+    ChTune cur = s->ch_tune;
+    float cur_ms = 0.f;
+    e = time_of(cur, cur_ms);
+    const uint32_t nodes[] = {16, 24, 32, 40, 48, 65};
+    const uint32_t refills[] = {4, 8, 16, 24, 32};
+    const uint32_t blocks[] = {0, 7, 6, 5};
+    for (int pass = 0; pass < 2 && e == hipSuccess; pass++) {
+        for (uint32_t v : nodes) {
+            ChTune t = cur;
+            t.node_phase_min = v;
+            float m;
+            if ((e = time_of(t, m)) != hipSuccess) break;
+            if (m < 0.99f * cur_ms) cur = t, cur_ms = m;
+        }
+        for (uint32_t v : refills) {
+            if (e != hipSuccess) break;
+            ChTune t = cur;
+            t.refill_min = v;
+            float m;
+            if ((e = time_of(t, m)) != hipSuccess) break;
+            if (m < 0.99f * cur_ms) cur = t, cur_ms = m;
+        }
+        for (uint32_t v : blocks) {
+            if (e != hipSuccess) break;
+            ChTune t = cur;
+            t.blocks_per_cu = v;
+            float m;
+            if ((e = time_of(t, m)) != hipSuccess) break;
+            if (m < 0.99f * cur_ms) cur = t, cur_ms = m;
+        }
+    }
+    (void)hipFree(d_hits);
+    if (e != hipSuccess) return hip_fail(ctx, e, "wr_autotune_closest_hit");
+    s->ch_tune = cur;
+    if (best) *best = wr_closest_hit_tuning{cur.refill_min, cur.node_phase_min, cur.blocks_per_cu, cur_ms};
+    return WR_OK;
+}
+
+int wr_scene_set_closest_hit_tuning(wr_scene* s, const wr_closest_hit_tuning* t) {
+    if (!s || !t) return fail(nullptr, WR_EINVAL, "wr_scene_set_closest_hit_tuning: bad argument");
+    s->ch_tune.refill_min = t->refill_min;
+    s->ch_tune.node_phase_min = t->node_phase_min;
+    s->ch_tune.blocks_per_cu = t->blocks_per_cu;
     return WR_OK;
 }
 

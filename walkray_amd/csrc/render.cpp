@@ -159,7 +159,8 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
                 e = launch_pt_mega(s->dev, ch, prm->max_depth, prm->seed, film->passes + k, fb, ctx->stream);
         } else {
             e = run_wavefront(s->dev, ch, prm->max_depth, prm->seed, film->passes, prm->spp, fb, ctx->engine,
-                              prm->wave_slots, prm->wave_grid, &ws, ctx->stream);
+                              prm->wave_slots ? prm->wave_slots : ctx->tuned_slots,
+                              prm->wave_grid ? prm->wave_grid : ctx->tuned_grid, &ws, ctx->stream);
         }
     }
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
@@ -177,3 +178,37 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
 }
 
 }  // extern "C"
+
+extern "C" int wr_autotune_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, uint32_t max_depth,
+                                  uint32_t spp, wr_render_tuning* best) {
+    if (!ctx || !s || !cam || spp == 0) return fail(ctx, WR_EINVAL, "wr_autotune_render: bad argument");
+    if (!s->built) return fail(ctx, WR_ESTATE, "wr_autotune_render: scene not built");
+    wr_film* film = nullptr;
+    int rc = wr_film_create(ctx, cam->width, cam->height, &film);
+    if (rc != WR_OK) return rc;
+    const uint32_t slots[] = {1u << 20, 1u << 21, 1u << 22, 1u << 23};
+    const uint32_t grids[] = {1024, 2048, 4096};
+    uint32_t best_slots = 0, best_grid = 0;
+    float best_ms = 1e30f;
+    for (uint32_t sl : slots) {
+        for (uint32_t g : grids) {
+            // the first render of a configuration also builds the engine: time the second
+            wr_render_params p{spp, max_depth, 1, WR_WAVEFRONT, sl, g, 0};
+            wr_render_stats st{};
+            for (int rep = 0; rep < 2 && rc == WR_OK; rep++) rc = wr_render(ctx, s, cam, &p, film, &st);
+            if (rc != WR_OK) break;
+            if (st.kernel_ms < best_ms) {
+                best_ms = st.kernel_ms;
+                best_slots = sl;
+                best_grid = g;
+            }
+        }
+        if (rc != WR_OK) break;
+    }
+    wr_film_destroy(film);
+    if (rc != WR_OK) return rc;
+    ctx->tuned_slots = best_slots;
+    ctx->tuned_grid = best_grid;
+    if (best) *best = wr_render_tuning{best_slots, best_grid, best_ms};
+    return WR_OK;
+}

@@ -53,12 +53,6 @@ __device__ __forceinline__ void add_counters(unsigned long long* out, const Coun
     }
 }
 
-// Idle lanes of a wave are refilled once at least this many are idle.
-constexpr uint32_t kRefillMin = 16;
-// A wave runs node-only steps while at least this many lanes are at a wide
-// node: a mixed step pays for both the node code and the triangle code.
-constexpr uint32_t kNodePhaseMin = 32;
-
 // Fast pass of the batch closest-hit query as a persistent kernel: each wave
 // takes rays in batches of 64 from a global counter and refills a lane with
 // the next ray as soon as its traversal ends, so a wave never waits for its
@@ -67,10 +61,14 @@ constexpr uint32_t kNodePhaseMin = 32;
 // work[0] hands out batches, work[1] counts finished waves; the last wave
 // out zeroes both for the next launch.  Compiled for 8 waves per SIMD (64
 // VGPRs): more waves in flight hide more of the node-fetch latency.
+// Scheduling knobs (ChTune, chosen by the autotuner): idle lanes are refilled
+// once at least refill_min are idle; a wave runs node-only steps while at
+// least node_phase_min lanes sit at wide nodes (a mixed step pays for both the
+// node code and the triangle code).
 template <bool COUNT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_closest_hit(SceneDev S, wr_tree_hdr H, const float* __restrict__ rays,
-                                                     uint64_t n, uint4* __restrict__ hits,
-                                                     unsigned long long* counters, unsigned long long* work) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_closest_hit(
+    SceneDev S, wr_tree_hdr H, const float* __restrict__ rays, uint64_t n, uint4* __restrict__ hits,
+    unsigned long long* counters, unsigned long long* work, uint32_t refill_min, uint32_t node_phase_min) {
     const Tree tr = make_tree(H, S);
     LaneStack st;
     st.lane = threadIdx.x;
@@ -87,7 +85,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     HitState h{};
     for (;;) {
         uint64_t idle = __ballot(!active);
-        if (!exhausted && (uint32_t)__popcll(idle) >= kRefillMin) {
+        if (!exhausted && (uint32_t)__popcll(idle) >= refill_min) {
             while (idle != 0) {
                 if (bnext == bend) {
                     unsigned long long base = 0;
@@ -116,11 +114,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
             }
         }
         if (__ballot(active) == 0) break;  // only once the counter is exhausted
-        // node phase: while at least kNodePhaseMin lanes sit at wide nodes, only
-        // those lanes step (no leaf code runs); then one step for every lane
+        // node phase: while at least node_phase_min lanes sit at wide nodes, only
+        // those lanes step (no triangle code runs); then one step for every lane
         for (;;) {
             const bool at_node = active && !(ref & WR_LEAF_BIT);
-            if ((uint32_t)__popcll(__ballot(at_node)) < kNodePhaseMin) break;
+            if ((uint32_t)__popcll(__ballot(at_node)) < node_phase_min) break;
             if (at_node && !wide_step<false, COUNT>(tr, r, ns, st, h, ref, overflow, cnt)) {
                 hits[rid] = make_uint4(h.rec, __float_as_uint(h.t), h.first, overflow ? 1u : 0u);
                 active = false;
@@ -225,35 +223,42 @@ __global__ __launch_bounds__(256) void k_inside_object(SceneDev S, const uint32_
 
 namespace wr {
 
-// Workgroups of the persistent kernel: as many as can be resident at once.
-static uint32_t persistent_blocks(uint64_t n) {
-    static uint32_t resident = 0;
-    if (!resident) {
-        int dev = 0, cus = 0, per_cu = 0;
+// Workgroups of the persistent kernel: per_cu per CU (0: as many as fit).
+static uint32_t persistent_blocks(uint64_t n, uint32_t per_cu_want) {
+    static int cus = 0, per_cu_max = 0;
+    if (!cus) {
+        int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wrd::k_closest_hit<false>, 256,
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_max, wrd::k_closest_hit<false>, 256,
                                                          wrd::kStackLdsBytes) != hipSuccess ||
-            cus <= 0 || per_cu <= 0) {
+            cus <= 0 || per_cu_max <= 0) {
             cus = 256;
-            per_cu = 1;
+            per_cu_max = 1;
         }
-        resident = (uint32_t)(cus * per_cu);
     }
+    const uint32_t per_cu = per_cu_want && per_cu_want < (uint32_t)per_cu_max ? per_cu_want : (uint32_t)per_cu_max;
+    const uint64_t resident = (uint64_t)cus * per_cu;
     const uint64_t need = (n + 255) / 256;
     return (uint32_t)(need < resident ? need : resident);
 }
 
 hipError_t launch_closest_hit(const SceneDev& S, const wr_tree_hdr& H, const float* rays, uint64_t n, void* hits,
-                              unsigned long long* counters, unsigned long long* work, hipStream_t st) {
+                              unsigned long long* counters, unsigned long long* work, const ChTune& tune,
+                              hipStream_t st) {
     if (n == 0) return hipSuccess;
-    const uint32_t pgrid = persistent_blocks(n);
+    const uint32_t pgrid = persistent_blocks(n, tune.blocks_per_cu);
     const uint32_t grid = (uint32_t)((n + 255) / 256);
+    // refill_min in [1, 64]; node_phase_min in [1, 65] (0 would never leave the node phase; 65 disables it)
+    const uint32_t rmin = tune.refill_min < 1 ? 1 : (tune.refill_min > 64 ? 64 : tune.refill_min);
+    const uint32_t nmin = tune.node_phase_min < 1 ? 1 : (tune.node_phase_min > 65 ? 65 : tune.node_phase_min);
     if (counters) {
-        wrd::k_closest_hit<true><<<pgrid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, counters, work);
+        wrd::k_closest_hit<true><<<pgrid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, counters, work,
+                                                                          rmin, nmin);
         wrd::k_closest_hit_finish<true><<<grid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, counters);
     } else {
-        wrd::k_closest_hit<false><<<pgrid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, nullptr, work);
+        wrd::k_closest_hit<false><<<pgrid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, nullptr, work,
+                                                                           rmin, nmin);
         wrd::k_closest_hit_finish<false><<<grid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, nullptr);
     }
     return hipGetLastError();

@@ -17,6 +17,7 @@ struct wr_ctx {
     std::string err;
     unsigned long long* d_work = nullptr;  // batch counter of the persistent closest-hit kernel (self-resetting)
     wr::Engine engine;  // wavefront engine of wr_render, reused across calls
+    uint32_t tuned_slots = 0, tuned_grid = 0;  // wr_autotune_render's choice (0 = defaults)
 };
 
 namespace wr {
@@ -39,6 +40,7 @@ struct wr_scene {
     std::vector<wr_object_gpu> host_objs;  // host copy of the device object table
     std::vector<void*> dbufs;
     SceneDev dev{};
+    wr::ChTune ch_tune;                    // closest-hit launch configuration (wr_autotune_closest_hit)
     uint64_t device_bytes = 0;
     uint32_t max_obj_depth = 0;
     double build_seconds = 0;

@@ -47,10 +47,18 @@ inline uint32_t wr_grid_cell(float x, float gmin, float inv, uint32_t dim) {
 }
 
 namespace wr {
+// Launch configuration of the persistent closest-hit kernel.  Scheduling
+// only: results never depend on it.
+struct ChTune {
+    uint32_t refill_min = 16;      // idle lanes of a wave before it takes new rays (>= 1)
+    uint32_t node_phase_min = 32;  // lanes at wide nodes for a node-only step (0..64)
+    uint32_t blocks_per_cu = 0;    // resident workgroups per CU, 0 = as many as fit
+};
 // work: two zeroed 64-bit words owned by the caller's stream; the kernel
 // leaves them zeroed again when it ends.
 hipError_t launch_closest_hit(const SceneDev& S, const wr_tree_hdr& H, const float* rays, uint64_t n, void* hits,
-                              unsigned long long* counters, unsigned long long* work, hipStream_t st);
+                              unsigned long long* counters, unsigned long long* work, const ChTune& tune,
+                              hipStream_t st);
 hipError_t launch_closest_hit_path(const SceneDev& S, const wr_tree_hdr& H, const uint32_t* rays, uint64_t n,
                                    void* hits, hipStream_t st);
 hipError_t launch_inside_object(const SceneDev& S, uint32_t max_obj_depth, const uint32_t* rays, uint64_t n,

@@ -70,7 +70,10 @@ def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
     out = {}
     objs, cam = S.cornell(k=5, width=256, height=256)
     sc = W.Scene(ctx, objs)
+    tune = W.autotune_render(sc, cam, 8, spp=4)
+    log(f"[bench] C1 wavefront autotune: {tune}")
     r, film = _compare(ctx, sc, cam, 16, 8, 1234, log, "C1 cornell")
+    r["wavefront_tuning"] = tune
     r["workload"] = ("Cornell box + 2 icospheres k=5 (40,974 triangles), 256x256, 16 spp, max depth 8, "
                      "Russian roulette at every diffuse/Phong bounce (reference rule)")
     if os.path.exists(C1_REFERENCE):
@@ -80,7 +83,10 @@ def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
     out["c1"] = r
     objs, cam = S.random_field()
     sc = W.Scene(ctx, objs)
+    tune = W.autotune_render(sc, cam, 16, spp=2)
+    log(f"[bench] C3 wavefront autotune: {tune}")
     r, film = _compare(ctx, sc, cam, c3_spp, 16, 77, log, "C3 field", repeats=1)
+    r["wavefront_tuning"] = tune
     r["workload"] = (f"512 random icospheres/boxes on a ground plane ({sc.info()['n_triangles']} triangles), "
                      f"diffuse/Phong/refractive, 2 area lights, 1920x1080, {c3_spp} spp, max depth 16")
     out["c3"] = r
