@@ -15,8 +15,9 @@ latency and aggregate samples/s under Poisson arrivals).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pt] [--legs ch,pt,serve]
 
---legs picks the workloads (closest hit, path tracing, serving); a run
-without the closest-hit leg reports value null (used to profile one leg).
+--legs picks the workloads (closest hit, path tracing, serving, and c5: the
+43.8M-triangle large scene at 3840x2160 x 256 spp, not in the default run); a
+run without the closest-hit leg reports value null (used to profile one leg).
 """
 from __future__ import annotations
 
@@ -216,6 +217,9 @@ def main():
     if "pt" in legs:
         from walkray_amd import render as R
         extra["path_tracing"] = R.bench_path_tracing(W, S, ctx, log)
+    if "c5" in legs:  # the large scene: minutes of host build and rendering, not in the default run
+        from walkray_amd import render as R
+        extra["large_scene"] = R.bench_large(W, S, ctx, log)
     if "serve" in legs:
         from walkray_amd import serve as V
         extra["serving"] = V.bench_serving(ctx, log)

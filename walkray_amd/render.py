@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import os
+import time
 
 import numpy as np
 
@@ -91,6 +92,30 @@ def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
                      f"diffuse/Phong/refractive, 2 area lights, 1920x1080, {c3_spp} spp, max depth 16")
     out["c3"] = r
     return out
+
+
+def bench_large(W_, S_, ctx, log, spp=256, depth=32):
+    """Large scene: 43.8M triangles, 3840x2160, 256 spp, depth 32; build time, both engines."""
+    t0 = time.time()
+    objs, cam = S.large_scene()
+    ntri = sum(len(o["tris"]) for o in objs)
+    log(f"[bench] C5 large: {len(objs)} objects, {ntri} triangles generated ({time.time() - t0:.1f}s)")
+    t1 = time.time()
+    sc = W.Scene(ctx, objs)
+    del objs
+    info = sc.info()
+    log(f"[bench] C5 large: scene built and uploaded in {time.time() - t1:.1f}s (host build {info['build_seconds']:.1f}s, "
+        f"{info['device_bytes'] / 2**30:.2f} GiB on the device, depth {info['depth']})")
+    tune = W.autotune_render(sc, cam, depth, spp=1)
+    log(f"[bench] C5 wavefront autotune: {tune}")
+    r, film = _compare(ctx, sc, cam, spp, depth, 2024, log, "C5 large", repeats=1)
+    r["wavefront_tuning"] = tune
+    r["workload"] = (f"4096^2 fBm heightfield + 2,000 Phong icospheres k=4 + sun disk + sky dome ({ntri} triangles), "
+                     f"3840x2160, {spp} spp, max depth {depth}")
+    r["build_seconds"] = round(info["build_seconds"], 2)
+    r["device_gib"] = round(info["device_bytes"] / 2**30, 3)
+    r["mean_radiance"] = round(float(film["mean"].mean()), 5)
+    return r
 
 
 def smoke_check(ctx):

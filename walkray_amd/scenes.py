@@ -104,6 +104,19 @@ def heightfield(n, seed=1, amplitude=0.2, octaves=6, mat=DIFFUSE, params=(0.7, 0
 def icosphere(k, center=(0.0, 0.0, 0.0), radius=1.0, mat=DIFFUSE, params=(0.8, 0.8, 0.8),
               smooth=True, inward=False):
     """Icosahedron subdivided k times: 20*4^k triangles, 10*4^k+2 vertices, outward winding."""
+    unit, tris = _unit_icosphere(k, inward)
+    pos = np.asarray(center, dtype=np.float64) + radius * unit
+    normals = (-unit if inward else unit) if smooth else None
+    return _obj(pos, tris.copy(), normals, mat, params)
+
+
+_ICO_CACHE = {}
+
+
+def _unit_icosphere(k, inward):
+    key = (k, inward)
+    if key in _ICO_CACHE:
+        return _ICO_CACHE[key]
     t = (1.0 + math.sqrt(5.0)) / 2.0
     v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
          (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
@@ -134,9 +147,8 @@ def icosphere(k, center=(0.0, 0.0, 0.0), radius=1.0, mat=DIFFUSE, params=(0.8, 0
     out = np.einsum("ij,ij->i", np.cross(p1 - p0, p2 - p0), p0 + p1 + p2) > 0
     flip = out == inward
     tris[flip] = tris[flip][:, [0, 2, 1]]
-    pos = np.asarray(center, dtype=np.float64) + radius * unit
-    normals = (-unit if inward else unit) if smooth else None
-    return _obj(pos, tris, normals, mat, params)
+    _ICO_CACHE[key] = (unit, tris)
+    return unit, tris
 
 
 def box(lo, hi, mat=DIFFUSE, params=(0.7, 0.7, 0.7), inward=False):
@@ -242,6 +254,49 @@ def random_field(n_objects=512, seed=3, width=1920, height=1080, sphere_k=3, sph
         objs.append(quad((x, 6.0, 0.0), (6.0, 0, 0), (0, 0, 6.0), (0, -1, 0), EMISSIVE, (10.0, 10.0, 10.0)))
     cam = camera((0.0, 5.0, 18.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), width, height,
                  sight_dist=960.0 / math.tan(math.radians(30.0)) * width / 1920.0)
+    return objs, cam
+
+
+def disk(center, normal, radius, segments, mat, params):
+    """A flat disk as a triangle fan, front face along normal."""
+    nrm = np.asarray(normal, np.float64)
+    nrm = nrm / np.linalg.norm(nrm)
+    a = np.cross(nrm, (0.0, 0.0, 1.0) if abs(nrm[2]) < 0.9 else (1.0, 0.0, 0.0))
+    a /= np.linalg.norm(a)
+    b = np.cross(nrm, a)
+    ang = 2 * math.pi * np.arange(segments) / segments
+    ring = np.asarray(center) + radius * (np.cos(ang)[:, None] * a + np.sin(ang)[:, None] * b)
+    verts = np.concatenate([np.asarray(center, np.float64)[None], ring])
+    i = np.arange(segments, dtype=np.uint32)
+    tris = np.stack([np.zeros(segments, np.uint32), 1 + i, 1 + (i + 1) % segments], axis=-1)
+    # (v1-v0)x(v2-v0) must point along normal
+    if np.dot(np.cross(verts[1] - verts[0], verts[2] - verts[0]), nrm) < 0:
+        tris = tris[:, [0, 2, 1]]
+    return _obj(verts, tris, None, mat, params)
+
+
+def large_scene(n=4096, n_spheres=2000, width=3840, height=2160, seed=5):
+    """Large scene: an n x n fBm heightfield on the unit square (2 (n-1)^2 triangles; 33,538,050
+    at n = 4096), n_spheres Phong icospheres k=4 (5,120 triangles each, albedo U[0.1, 0.9],
+    exponent log-uniform 10-1000) resting on it, a sun-like emissive disk 20 units away
+    (0.5 degree radius) and an emissive sky dome (an inward icosphere k=3 of radius 30)."""
+    rng = np.random.default_rng(seed)
+    objs = [heightfield(n, seed=seed, amplitude=0.2, params=(0.5, 0.45, 0.4))]
+    xy = rng.uniform(0.05, 0.95, (n_spheres, 2))
+    r = rng.uniform(0.003, 0.01, n_spheres)
+    z = 0.2 * fbm(xy[:, 0], xy[:, 1], seed, 6) + r
+    alb = rng.uniform(0.1, 0.9, (n_spheres, 3))
+    power = np.exp(rng.uniform(np.log(10.0), np.log(1000.0), n_spheres))
+    for k in range(n_spheres):
+        objs.append(icosphere(4, (xy[k, 0], xy[k, 1], z[k]), r[k], GLOSSY, tuple(alb[k]) + (power[k],)))
+    sun_dir = np.array([math.cos(math.radians(40)) * math.cos(math.radians(30)),
+                        math.cos(math.radians(40)) * math.sin(math.radians(30)), math.sin(math.radians(40))])
+    centre = np.array([0.5, 0.5, 0.0])
+    objs.append(disk(centre + 20.0 * sun_dir, -sun_dir, 20.0 * math.tan(math.radians(0.5)), 32, EMISSIVE,
+                     (20000.0, 18000.0, 15000.0)))
+    objs.append(icosphere(3, tuple(centre), 30.0, EMISSIVE, (0.4, 0.55, 0.8), smooth=False, inward=True))
+    cam = camera((0.5, -0.25, 0.45), (0.5, 0.55, 0.05), (0.0, 0.0, 1.0), width, height,
+                 sight_dist=(width / 2) / math.tan(math.radians(30.0)))
     return objs, cam
 
 
