@@ -11,6 +11,7 @@ from . import scenes as S
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 C1_REFERENCE = os.path.join(ROOT, "tests", "golden", "cornell256_ref.npz")
+C3_REFERENCE = os.path.join(ROOT, "tests", "golden", "field1080_ref_sub8.npz")
 
 
 def film_stats(state):
@@ -90,6 +91,13 @@ def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
     r["wavefront_tuning"] = tune
     r["workload"] = (f"512 random icospheres/boxes on a ground plane ({sc.info()['n_triangles']} triangles), "
                      f"diffuse/Phong/refractive, 2 area lights, 1920x1080, {c3_spp} spp, max depth 16")
+    if os.path.exists(C3_REFERENCE):
+        ref = np.load(C3_REFERENCE)
+        st, off = int(ref["stride"]), int(ref["offset"])
+        img = np.asarray(film["mean"]).reshape(cam["height"], cam["width"], 3)[off::st, off::st]
+        r["rel_rmse_vs_reference"] = round(rel_rmse(img, ref["mean"]), 5)
+        r["reference"] = (f"{int(ref['spp'])}-spp render, FP64 accumulation of per-batch Kahan films, "
+                          f"compared on every {st}th pixel in x and y ({img.shape[1]} x {img.shape[0]})")
     out["c3"] = r
     return out
 
