@@ -190,23 +190,30 @@ extern "C" int wr_autotune_render(wr_ctx* ctx, const wr_scene* s, const wr_camer
     const uint32_t grids[] = {1024, 2048, 4096};
     uint32_t best_slots = 0, best_grid = 0;
     float best_ms = 1e30f;
+    int last_rc = WR_OK;
     for (uint32_t sl : slots) {
         for (uint32_t g : grids) {
-            // the first render of a configuration also builds the engine: time the second
+            // the first render of a configuration also builds the engine: time the second;
+            // a configuration that cannot be set up (device memory) is skipped
             wr_render_params p{spp, max_depth, 1, WR_WAVEFRONT, sl, g, 0};
             wr_render_stats st{};
-            for (int rep = 0; rep < 2 && rc == WR_OK; rep++) rc = wr_render(ctx, s, cam, &p, film, &st);
-            if (rc != WR_OK) break;
+            int r = WR_OK;
+            for (int rep = 0; rep < 2 && r == WR_OK; rep++) r = wr_render(ctx, s, cam, &p, film, &st);
+            if (r != WR_OK) {
+                last_rc = r;
+                if (r != WR_ENOMEM && r != WR_EHIP) break;
+                continue;
+            }
             if (st.kernel_ms < best_ms) {
                 best_ms = st.kernel_ms;
                 best_slots = sl;
                 best_grid = g;
             }
         }
-        if (rc != WR_OK) break;
+        if (last_rc != WR_OK && last_rc != WR_ENOMEM && last_rc != WR_EHIP) break;
     }
     wr_film_destroy(film);
-    if (rc != WR_OK) return rc;
+    if (best_slots == 0) return last_rc != WR_OK ? last_rc : fail(ctx, WR_EINVAL, "wr_autotune_render: no configuration ran");
     ctx->tuned_slots = best_slots;
     ctx->tuned_grid = best_grid;
     if (best) *best = wr_render_tuning{best_slots, best_grid, best_ms};
