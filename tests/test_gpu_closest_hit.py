@@ -179,3 +179,58 @@ def test_full_size_heightfield_sampled(ctx):
     assert np.all(hits["t"][hit] >= 0)
     eps = 1e-3
     assert np.all((hits["b1"][hit] > -eps) & (hits["b2"][hit] > -eps) & (hits["b1"][hit] + hits["b2"][hit] < 1 + eps))
+
+
+def test_field_plain_noself_inside(ctx):
+    """The object-field workload (smaller): plain rays, NoSelf path rays and findInsideObject."""
+    objs, cam = S.random_field(n_objects=48, seed=5, width=96, height=54)
+    sc, osc = _scene_pair(ctx, objs)
+    lo, hi = S.scene_bounds(objs)
+    rays = np.concatenate([S.pinhole_rays(cam), S.incoherent_rays(16384, lo, hi, seed=3)])
+    _assert_same(sc.closest_hit(rays), osc.intersect(rays), "field")
+    rng = np.random.default_rng(8)
+    probe = S.incoherent_rays(40000, (-16, 0.0, -8), (16, 1.0, 8), seed=6)
+    h = osc.intersect(probe)
+    ok = h["prim"] != O.NONE
+    o = (probe[ok, :3] + probe[ok, 3:] * h["t"][ok, None]).astype(np.float32)[:16384]
+    contact = h["obj"][ok][:16384].astype(np.uint32)
+    d = rng.normal(size=(len(o), 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    prays = np.concatenate([o, d.astype(np.float32)], axis=1).astype(np.float32)
+    glass = np.array([i for i, ob in enumerate(objs) if ob["mat"] == S.REFRACTIVE], np.uint32)
+    inside = np.where(rng.random(len(o)) < 0.3, rng.choice(glass, len(o)), O.NONE).astype(np.uint32)
+    _assert_same(sc.closest_hit_noself(prays, inside, contact), osc.intersect(prays, inside, contact, noself=True),
+                 "field-noself")
+    # origins inside objects (sphere and box centres, jittered) and outside
+    cen = np.array([ob["verts"].mean(axis=0) for ob in objs[1:-2]], np.float32)
+    oi = cen[rng.integers(0, len(cen), 4096)] + rng.normal(scale=0.05, size=(4096, 3)).astype(np.float32)
+    di = rng.normal(size=(4096, 3))
+    di /= np.linalg.norm(di, axis=1, keepdims=True)
+    irays = np.concatenate([oi, di.astype(np.float32)], axis=1).astype(np.float32)
+    cur = np.full(len(irays), O.NONE, np.uint32)
+    got = sc.inside_object(irays, cur)
+    want = osc.find_inside(irays, cur)
+    assert np.array_equal(got, want), f"{(got != want).sum()} inside-object mismatches"
+    assert (want != O.NONE).mean() > 0.3
+
+
+def test_ties_on_coincident_and_coplanar_triangles(ctx):
+    """Equal t across leaves and inside a leaf: a soup with repeated and coincident triangles, and
+    vertical rays through the vertices and edges of a flat grid (up to six triangles at the same t)."""
+    rng = np.random.default_rng(2)
+    v = np.round(rng.normal(size=(300, 3)), 1).astype(np.float32)
+    t = rng.integers(0, 300, (500, 3)).astype(np.uint32)
+    t = np.concatenate([t, t[:100], t[50:150, [1, 2, 0]]])  # repeats, and repeats with rotated corners
+    soup = {"verts": v, "tris": t, "normals": None, "mat": S.DIFFUSE, "params": (0.5, 0.5, 0.5)}
+    sc, osc = _scene_pair(ctx, [soup])
+    lo, hi = S.scene_bounds([soup])
+    rays = S.incoherent_rays(20000, lo, hi, seed=12)
+    _assert_same(sc.closest_hit(rays), osc.intersect(rays), "soup")
+    flat = S.heightfield(33, seed=1, amplitude=0.0)
+    sc, osc = _scene_pair(ctx, [flat])
+    g = np.arange(0, 65, dtype=np.float32) / 64.0  # vertices (even steps) and edge midpoints (odd steps)
+    xx, yy = np.meshgrid(g, g)
+    n = xx.size
+    vert = np.stack([xx.ravel(), yy.ravel(), np.ones(n, np.float32), np.zeros(n, np.float32),
+                     np.zeros(n, np.float32), -np.ones(n, np.float32)], axis=1).astype(np.float32)
+    _assert_same(sc.closest_hit(vert), osc.intersect(vert), "flat-grid ties")
