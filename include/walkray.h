@@ -189,6 +189,13 @@ typedef struct {
 } wr_camera;
 
 enum { WR_MEGAKERNEL = 0, WR_WAVEFRONT = 1 };
+/* Render flags.  WR_RENDER_NEE: PathTracer's useNextEventEstimation branch
+ * (pathTracer.cpp:39-66; the reference constructs it false, pathTracer.h:14):
+ * a path that escapes or reaches the depth limit connects the origin of its
+ * last traced ray, if diffuse, to a light sample (Scene::genLightSample,
+ * scene.cpp:16-18) through MCRenderer::visibilityTest (mcRenderer.cpp:59-65).
+ * The light sample's uniforms use vertex counter 0xFFFFFFFF. */
+enum { WR_RENDER_NEE = 1 };
 typedef struct {
     uint32_t spp;        /* passes to add, one sample per pixel each (PathTracer spp) */
     uint32_t max_depth;  /* MCRenderer maxPathLen (mcRenderer.h:16) */
@@ -196,7 +203,7 @@ typedef struct {
     uint32_t mode;       /* WR_MEGAKERNEL (one lane per pixel) or WR_WAVEFRONT */
     uint32_t wave_slots; /* wavefront path-slot pool size, 0 = automatic */
     uint32_t wave_grid;  /* wavefront workgroups per stage launch, 0 = automatic */
-    uint32_t reserved;
+    uint32_t flags;      /* WR_RENDER_* */
 } wr_render_params;
 
 typedef struct {
@@ -297,6 +304,7 @@ typedef struct {
     wr_camera cam;
     uint32_t spp;
     uint32_t max_depth;
+    uint32_t flags; /* WR_RENDER_* */
     uint64_t seed;
 } wr_job_desc;
 

@@ -309,6 +309,7 @@ struct Scene {
             off += (unsigned)o.verts.size();
         }
         tree.build();
+        prepareLights();
         for (auto* t : objTrees) delete t;
         objTrees.clear();
         for (unsigned oi = 0; oi < objs.size(); oi++) {
@@ -318,6 +319,47 @@ struct Scene {
             t->build();
             objTrees.push_back(t);
         }
+    }
+
+    /* Light sampling tables (Scene::prepareForLightSampling, scene.cpp:5-14;
+     * SurfaceSampler::prepareForSampling, surfaceSampler.cpp:7-15;
+     * AbstractObject::prepareForSurfaceSampling, abstractObj.cpp:9-16). */
+    std::vector<unsigned> lightObj;           /* emissive objects, in object order */
+    std::vector<float> lightCum;              /* SurfaceSampler::areaThreshold */
+    std::vector<std::vector<float>> triCum;   /* per light: AbstractObject::areaThreshold */
+    std::vector<float> lightArea, pickProb;   /* per light: boundArea, pickProb */
+    float lightTotal = 0;                     /* SurfaceSampler::boundArea */
+
+    /* Mesh::getTriangleArea (mesh.h:80-88) */
+    float triangleArea(unsigned oi, unsigned ti) const {
+        const Object& o = objs[oi];
+        V vps[3];
+        for (int i = 0; i < 3; i++) vps[i] = o.verts[o.idx[3 * ti + i]];
+        V l0 = sub(vps[1], vps[0]);
+        V l1 = sub(vps[2], vps[1]);
+        float c = length(cross(l0, l1));
+        return c / 2;
+    }
+
+    void prepareLights() {
+        lightObj.clear(); lightCum.clear(); triCum.clear(); lightArea.clear(); pickProb.clear();
+        lightTotal = 0;
+        for (unsigned oi = 0; oi < objs.size(); oi++) {
+            if (objs[oi].mat != ORC_EMISSIVE) continue;
+            lightObj.push_back(oi);
+            std::vector<float> cum;
+            float areaValue = 0;
+            const unsigned nt = (unsigned)(objs[oi].idx.size() / 3);
+            for (unsigned i = 0; i < nt; i++) {
+                areaValue += triangleArea(oi, i);
+                cum.push_back(areaValue);
+            }
+            triCum.push_back(cum);
+            lightArea.push_back(areaValue);
+            lightTotal += areaValue;
+            lightCum.push_back(lightTotal);
+        }
+        for (size_t i = 0; i < lightObj.size(); i++) pickProb.push_back(lightArea[i] / lightTotal);
     }
 
     /* Mesh::getWorldNormal (mesh.h:47-77) with an identity transform. */
@@ -414,6 +456,10 @@ static inline void rng4(uint64_t seed, uint32_t pixel, uint32_t pass, uint32_t v
     philox(c, k, o);
     for (int i = 0; i < 4; i++) f[i] = (float)(o[i] >> 8) * (1.0f / 16777216.0f);
 }
+
+/* Vertex counter of the light sample's uniforms (next-event estimation); path
+ * vertices use 0 .. maxPathLen + 1. */
+static const uint32_t kNeeVertex = 0xFFFFFFFFu;
 
 /* ---- Camera (camera.cpp) -------------------------------------------------- */
 struct Cam {
@@ -676,6 +722,198 @@ struct Tracer {
         if (!legal(color)) color = V(0, 0, 0);
         return cam.fixVignetting(color, p);
     }
+
+    /* ---- The same estimator restated over the explicit Path of Rays ----------
+     * samplePath (mcRenderer.cpp:16-57) builds the vector of rays, terminateRay
+     * entries included; the pixel loop body (pathTracer.cpp:27-68) then forms
+     * the sample with either branch.  This is the form the next-event branch
+     * needs (it indexes the path from its end); with nee == false it must
+     * equal sample() bit for bit, which the tests check. */
+    struct PRay {               /* the Ray fields the estimator reads (ray.h:8-37) */
+        V o, d, rad = V(1, 1, 1);
+        V n;                    /* contactNormal() at o; for the camera ray normalize(focus - o) */
+        float op = 1.f, dp = 1.f;
+        unsigned inside = ORC_NONE, contact = ORC_NONE, ctri = 0;
+        bool camera = false;    /* contactObj is the Camera (camera.cpp:54) */
+        Hit hit;                /* intersectObj / intersectDist / intersectTriangleID */
+    };
+
+    /* Ray::cosineTerm (ray.cpp:52-58); every contact object here has a cosine. */
+    static float cosTerm(const PRay& r) {
+        float c = 1;
+        if ((r.camera || r.contact != ORC_NONE) && length(r.d) > 0.5) c = std::fabs(dot(r.n, r.d));
+        return c > EPS_D ? c : (float)EPS_D;
+    }
+
+    void samplePath(std::vector<PRay>& path, const PRay& prev, unsigned depth, unsigned p, unsigned pass) const {
+        path.push_back(prev);
+        PRay term;                                  /* terminateRay, mcRenderer.cpp:20-27 */
+        term.o = prev.o;
+        term.rad = V(0, 0, 0);
+        term.d = V(0, 0, 0);
+        unsigned self;
+        if (prev.inside != ORC_NONE) self = prev.inside;
+        else if (prev.hit.valid) self = prev.hit.obj;
+        else { path.push_back(term); return; }
+        if (!prev.hit.valid) { path.push_back(term); return; }  /* inside but no hit: as in sample() */
+        float rn[4];
+        rng4(seed, p, pass, depth + 1, rn);
+        Ray in;
+        in.o = prev.o;
+        in.d = prev.d;
+        const Out out = scatter(self, in, prev.hit, prev.inside, rn);
+        PRay next;
+        next.o = out.o; next.d = out.d; next.rad = out.rad; next.n = out.n;
+        next.op = out.op; next.dp = out.dp;
+        next.inside = out.inside; next.contact = out.contact; next.ctri = out.ctri;
+        if (length(next.d) < 0.5) { path.push_back(next); return; }
+        if (depth + 1 > maxDepth) { path.push_back(term); return; }
+        Ray nr;
+        nr.o = next.o;
+        nr.d = next.d;
+        next.hit = trace(nr, true, next.inside, next.contact);
+        if (!next.hit.valid) {
+            path.push_back(next);
+            path.push_back(term);
+            return;
+        }
+        samplePath(path, next, depth + 1, p, pass);
+    }
+
+    /* Scene::genLightSample -> SurfaceSampler::genSample (surfaceSampler.cpp:17-23)
+     * -> AbstractObject::genSurfaceSample (abstractObj.cpp:18-52) with
+     * Mesh::genRandTrianglePosition (mesh.h:89-101).  The four uniforms are
+     * the object pick, the triangle pick and (u1, u2).  The hemisphere
+     * direction, findInsideObject and the NoSelf intersect of the light ray
+     * are not formed: the path tracer overwrites the direction and reads
+     * neither of the others.  valid == false: the scene has no emissive
+     * object (the reference indexes an empty vector there) or the picked one
+     * has no triangles (contactObj stays NULL). */
+    struct LightSample {
+        bool valid = false;
+        V o, n, rad;
+        float op = 1.f;
+        unsigned obj = ORC_NONE, tri = 0;
+    };
+    LightSample genLightSample(const float* rn) const {
+        LightSample L;
+        if (s->lightObj.empty()) return L;
+        const float chosen = rn[0] * s->lightTotal;
+        size_t li = std::lower_bound(s->lightCum.begin(), s->lightCum.end(), chosen) - s->lightCum.begin();
+        if (li == s->lightCum.size()) li--;
+        const std::vector<float>& cum = s->triCum[li];
+        if (cum.empty()) return L;
+        const unsigned oi = s->lightObj[li];
+        const float randArea = rn[1] * s->lightArea[li];
+        size_t ti = std::lower_bound(cum.begin(), cum.end(), randArea) - cum.begin();
+        if (ti == cum.size()) ti--;
+        const Object& o = s->objs[oi];
+        V vps[3];
+        for (int i = 0; i < 3; i++) vps[i] = o.verts[o.idx[3 * ti + i]];
+        V b1 = sub(vps[1], vps[0]);
+        V b2 = sub(vps[2], vps[0]);
+        float u1 = rn[2], u2 = rn[3];
+        if (u1 + u2 > 1) {
+            u1 = 1 - u1;
+            u2 = 1 - u2;
+        }
+        L.o = add(add(vps[0], mul(b1, u1)), mul(b2, u2));
+        L.n = s->worldNormal(oi, (unsigned)ti, L.o);
+        L.rad = o.color;                                  /* EmissiveObject::evaluateBSDF */
+        L.op = 1 / s->lightArea[li] * s->pickProb[li];    /* abstractObj.cpp:41 */
+        L.obj = oi;
+        L.tri = (unsigned)ti;
+        L.valid = true;
+        return L;
+    }
+
+    /* MCRenderer::visibilityTest (mcRenderer.cpp:59-65) */
+    bool visible(const PRay& in, const V& lightOrigin) const {
+        Ray r;
+        r.o = in.o;
+        r.d = in.d;
+        unsigned g = 0;
+        const float dist = s->intersect(r, g, true, in.inside, in.contact);
+        return dist < 0 || (dist - length(sub(in.o, lightOrigin)) >= -EPS_D);
+    }
+
+    /* One pixel sample of PathTracer::renderPixels (pathTracer.cpp:22-69).
+     * The branch's `continue` leaves the pixel's running mean scaled by
+     * s/(s+1) with nothing added, i.e. a sample of 0.  The light sample's
+     * uniforms come from the vertex counter kNeeVertex. */
+    V samplePathVector(unsigned p, unsigned pass, bool nee, unsigned* nv) const {
+        float rn[4];
+        rng4(seed, p, pass, 0, rn);
+        PRay r0;                                          /* Camera::generateRay (camera.cpp:6-60) */
+        r0.o = cam.pos;
+        r0.d = cam.dir(p, rn[0], rn[1]);
+        r0.camera = true;
+        r0.n = normalize(sub(cam.focus, r0.o));            /* Camera::getWorldNormal (camera.cpp:182-186) */
+        Ray cr;
+        cr.o = r0.o;
+        cr.d = r0.d;
+        r0.hit = trace(cr, false, 0, 0);
+        r0.inside = s->findInside(cr, ORC_NONE);
+        std::vector<PRay> eyePath;
+        samplePath(eyePath, r0, 0, p, pass);
+        if (nv) *nv = (unsigned)eyePath.size();
+        const size_t n = eyePath.size();
+        V color(1, 1, 1);
+        const bool hasToConnect = length(eyePath[n - 1].rad) <= 0;
+        if (!nee || !hasToConnect) {
+            for (size_t i = 0; i < n; i++) {
+                if (i != n - 1) {
+                    color = mul(color, cosTerm(eyePath[i]));
+                    color = mulv(color, V(1, 1, 1));      /* radianceDecay: no volumes */
+                }
+                color = mulv(color, divs(divs(eyePath[i].rad, eyePath[i].op), eyePath[i].dp));
+            }
+        } else {
+            const PRay& back = eyePath[n - 1];
+            const int endIndex = (back.contact != ORC_NONE || back.camera || back.inside != ORC_NONE) ? -1 : (int)n - 2;
+            if (endIndex <= 0) return V(0, 0, 0);
+            PRay endRay = eyePath[endIndex];
+            rng4(seed, p, pass, kNeeVertex, rn);
+            const LightSample L = genLightSample(rn);
+            if (endRay.contact != ORC_NONE && s->objs[endRay.contact].mat == ORC_EMISSIVE) return V(0, 0, 0);
+            /* nonSpecular() is false for the glossy (MERGE_GLOSSY unset) and refractive classes */
+            if (endRay.contact != ORC_NONE && s->objs[endRay.contact].mat != ORC_DIFFUSE) return V(0, 0, 0);
+            if (!L.valid) return V(0, 0, 0);              /* contactNormal() of a NULL contact is 0: dot >= 0 */
+            endRay.d = normalize(sub(L.o, endRay.o));
+            PRay lightRay;
+            lightRay.o = L.o;
+            lightRay.d = neg(endRay.d);
+            lightRay.n = L.n;
+            lightRay.contact = L.obj;
+            lightRay.ctri = L.tri;
+            lightRay.rad = L.rad;
+            lightRay.op = L.op;
+            const float len = length(sub(L.o, endRay.o));
+            const float connectDist = len > EPS_D ? len : (float)EPS_D;
+            if (dot(endRay.d, L.n) >= 0) return V(0, 0, 0);
+            if (!visible(endRay, L.o)) return V(0, 0, 0);
+            for (int i = 0; i < endIndex; i++) {
+                color = mulv(color, divs(divs(eyePath[i].rad, eyePath[i].op), eyePath[i].dp));
+                color = mul(color, cosTerm(eyePath[i]));
+                color = mulv(color, V(1, 1, 1));
+            }
+            /* eyePath[endIndex-1].evalBSDF(endRay) -> DiffuseObject::evaluateBSDF
+             * (diffuseObj.h:17-21) -> DiffuseBSDF::evalBSDF (diffuseBsdf.h:29-34) */
+            const V nE = endRay.n;
+            const V inD = eyePath[endIndex - 1].d;
+            V f = (dot(inD, nE) <= 0 && dot(endRay.d, nE) >= 0) ? mul(s->objs[endRay.contact].color, (float)INV_PI_D)
+                                                                   : V(0, 0, 0);
+            f = mulv(f, lightRay.rad);
+            f = mulv(f, V(1, 1, 1));
+            f = mul(f, cosTerm(lightRay));
+            f = mul(f, cosTerm(endRay));
+            f = divs(f, connectDist * connectDist);
+            color = mulv(color, f);
+            color = divs(color, endRay.op * lightRay.op);
+        }
+        if (!legal(color)) color = V(0, 0, 0);
+        return cam.fixVignetting(color, p);
+    }
 };
 
 }  // namespace orc
@@ -868,6 +1106,53 @@ void orc_render_sample(void* sp, const orc_camera* c, uint32_t max_depth, uint64
     Tracer tr{s, Cam(c), max_depth, seed};
     V v = tr.sample(pixel, pass, n_vertices);
     rgb[0] = v.x; rgb[1] = v.y; rgb[2] = v.z;
+}
+
+void orc_render2(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t seed, uint32_t pass0,
+                 uint32_t npass, uint32_t flags, double* sum, double* sumsq, int nthreads) {
+    Scene* s = (Scene*)sp;
+    Tracer tr{s, Cam(c), max_depth, seed};
+    const int64_t np = (int64_t)c->width * c->height;
+    const bool nee = (flags & ORC_NEE) != 0, pv = (flags & (ORC_NEE | ORC_PATH_VECTOR)) != 0;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    for (uint32_t pass = pass0; pass < pass0 + npass; pass++) {
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+        for (int64_t p = 0; p < np; p++) {
+            V v = pv ? tr.samplePathVector((unsigned)p, pass, nee, nullptr) : tr.sample((unsigned)p, pass, nullptr);
+            for (int k = 0; k < 3; k++) {
+                sum[3 * p + k] += v[k];
+                sumsq[3 * p + k] += (double)v[k] * v[k];
+            }
+        }
+    }
+}
+
+void orc_render_sample2(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t seed, uint32_t flags,
+                        uint32_t pixel, uint32_t pass, float* rgb, uint32_t* n_vertices) {
+    Scene* s = (Scene*)sp;
+    Tracer tr{s, Cam(c), max_depth, seed};
+    const bool nee = (flags & ORC_NEE) != 0, pv = (flags & (ORC_NEE | ORC_PATH_VECTOR)) != 0;
+    V v = pv ? tr.samplePathVector(pixel, pass, nee, n_vertices) : tr.sample(pixel, pass, n_vertices);
+    rgb[0] = v.x; rgb[1] = v.y; rgb[2] = v.z;
+}
+
+int orc_light_sample(void* sp, const float rn[4], float* origin, float* normal, float* radiance, float* origin_prob,
+                     uint32_t* obj, uint32_t* tri) {
+    Scene* s = (Scene*)sp;
+    orc_camera c{};
+    c.width = c.height = 1;
+    Tracer tr{s, Cam(&c), 0, 0};
+    const Tracer::LightSample L = tr.genLightSample(rn);
+    if (!L.valid) return 0;
+    for (int k = 0; k < 3; k++) {
+        origin[k] = L.o[k];
+        normal[k] = L.n[k];
+        radiance[k] = L.rad[k];
+    }
+    *origin_prob = L.op;
+    *obj = L.obj;
+    *tri = L.tri;
+    return 1;
 }
 
 void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox(ctr, key, out); }

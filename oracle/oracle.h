@@ -79,6 +79,22 @@ void     orc_render(void* s, const orc_camera* cam, uint32_t max_depth, uint64_t
 void     orc_render_sample(void* s, const orc_camera* cam, uint32_t max_depth, uint64_t seed,
                            uint32_t pixel, uint32_t pass, float* rgb, uint32_t* n_vertices);
 
+/* flags of orc_render2 / orc_render_sample2 */
+enum {
+    ORC_NEE = 1,          /* PathTracer's useNextEventEstimation branch (pathTracer.cpp:39-66) */
+    ORC_PATH_VECTOR = 2   /* form the sample from the explicit Path (implied by ORC_NEE) */
+};
+void     orc_render2(void* s, const orc_camera* cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
+                     uint32_t npass, uint32_t flags, double* sum, double* sumsq, int nthreads);
+void     orc_render_sample2(void* s, const orc_camera* cam, uint32_t max_depth, uint64_t seed, uint32_t flags,
+                            uint32_t pixel, uint32_t pass, float* rgb, uint32_t* n_vertices);
+/* Scene::genLightSample (scene.cpp:16-18, surfaceSampler.cpp:17-23,
+ * abstractObj.cpp:18-52) from four uniforms (object pick, triangle pick, u1,
+ * u2).  Returns 0 when there is nothing to sample (no emissive object, or the
+ * picked one has no triangles). */
+int      orc_light_sample(void* s, const float rn[4], float* origin, float* normal, float* radiance,
+                          float* origin_prob, uint32_t* obj, uint32_t* tri);
+
 /* ---- Counter RNG (Philox-4x32-10, Salmon et al., SC'11) ---------------- */
 void     orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* 4 uniform floats in [0,1) for (seed, pixel, pass, vertex). */

@@ -51,6 +51,10 @@ def lib():
         L.orc_world_normal.argtypes = [vp, u32, u32, vp, vp, vp, vp]
         L.orc_render.argtypes = [vp, vp, u32, u64, u32, u32, vp, vp, C.c_int]
         L.orc_render_sample.argtypes = [vp, vp, u32, u64, u32, u32, vp, vp]
+        L.orc_render2.argtypes = [vp, vp, u32, u64, u32, u32, u32, vp, vp, C.c_int]
+        L.orc_render_sample2.argtypes = [vp, vp, u32, u64, u32, u32, u32, vp, vp]
+        L.orc_light_sample.restype = C.c_int
+        L.orc_light_sample.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_philox4x32.argtypes = [vp, vp, vp]
         L.orc_rng4.argtypes = [u64, u32, u32, u32, vp]
         _lib = L
@@ -173,20 +177,47 @@ class Scene:
         lib().orc_world_normal(self.h, obj, tri, _p(p), _p(n), C.byref(u1), C.byref(u2))
         return n, u1.value, u2.value
 
-    def render(self, cam, spp, max_depth, seed, pass0=0, nthreads=0, sums=None):
+    def render(self, cam, spp, max_depth, seed, pass0=0, nthreads=0, sums=None, nee=False, path_vector=False):
+        """nee: PathTracer's next-event branch (pathTracer.cpp:39-66); path_vector:
+        form the sample from the explicit Path (the restatement the branch uses)."""
         c = Camera.from_dict(cam)
         npx = cam["width"] * cam["height"]
         if sums is None:
             sums = (np.zeros(3 * npx, np.float64), np.zeros(3 * npx, np.float64))
-        lib().orc_render(self.h, C.byref(c), max_depth, seed, pass0, spp, _p(sums[0]), _p(sums[1]), nthreads)
+        flags = (NEE if nee else 0) | (PATH_VECTOR if path_vector else 0)
+        if flags:
+            lib().orc_render2(self.h, C.byref(c), max_depth, seed, pass0, spp, flags, _p(sums[0]), _p(sums[1]),
+                              nthreads)
+        else:
+            lib().orc_render(self.h, C.byref(c), max_depth, seed, pass0, spp, _p(sums[0]), _p(sums[1]), nthreads)
         return sums
 
-    def render_sample(self, cam, max_depth, seed, pixel, pss):
+    def render_sample(self, cam, max_depth, seed, pixel, pss, nee=False, path_vector=False):
         c = Camera.from_dict(cam)
         rgb = np.zeros(3, np.float32)
         nv = C.c_uint32()
-        lib().orc_render_sample(self.h, C.byref(c), max_depth, seed, pixel, pss, _p(rgb), C.byref(nv))
+        flags = (NEE if nee else 0) | (PATH_VECTOR if path_vector else 0)
+        if flags:
+            lib().orc_render_sample2(self.h, C.byref(c), max_depth, seed, flags, pixel, pss, _p(rgb), C.byref(nv))
+        else:
+            lib().orc_render_sample(self.h, C.byref(c), max_depth, seed, pixel, pss, _p(rgb), C.byref(nv))
         return rgb, nv.value
+
+    def light_sample(self, u):
+        """Scene::genLightSample from four uniforms; None when nothing can be sampled."""
+        u = np.ascontiguousarray(u, dtype=np.float32)
+        o, n, rad = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(3, np.float32)
+        op, obj, tri = C.c_float(), C.c_uint32(), C.c_uint32()
+        ok = lib().orc_light_sample(self.h, _p(u), _p(o), _p(n), _p(rad), C.byref(op), C.byref(obj), C.byref(tri))
+        if not ok:
+            return None
+        return {"origin": o, "normal": n, "radiance": rad, "origin_prob": op.value, "obj": obj.value,
+                "tri": tri.value}
+
+
+NEE = 1          # orc_render2 flags
+PATH_VECTOR = 2
+NEE_VERTEX = 0xFFFFFFFF  # vertex counter of the light sample's uniforms
 
 
 def philox4x32(ctr, key):

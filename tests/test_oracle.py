@@ -12,6 +12,7 @@ import math
 import numpy as np
 import pytest
 
+from nee_cases import expected_escape, expected_max_depth, form_factor_square, nee_scene
 from oracle import oracle as O
 from walkray_amd import scenes as S
 
@@ -412,3 +413,138 @@ def test_render_deterministic_and_pass_additive():
     b = sc.render(cam, 3, 8, 3)
     b = sc.render(cam, 3, 8, 3, pass0=3, sums=b)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+# ------------------------------------------------- next-event estimation ----
+def test_path_vector_estimator_equals_loop():
+    """The explicit-Path restatement (samplePath's vector plus the pixel loop's
+    product, pathTracer.cpp:30-37) is the same estimator as the incremental
+    loop: bit-identical films on scenes with every material and both ends."""
+    objs, cam = S.cornell(k=1, width=12, height=12)
+    sc = O.Scene(objs)
+    a = sc.render(cam, 4, 6, 9)
+    b = sc.render(cam, 4, 6, 9, path_vector=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    objs, cam = S.random_field(n_objects=24, width=16, height=9, sphere_k=1)
+    sc = O.Scene(objs)
+    for depth in (0, 1, 5):
+        a = sc.render(cam, 3, depth, 4)
+        b = sc.render(cam, 3, depth, 4, path_vector=True)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def _two_lights():
+    floor = S.quad((0, 0, 0), (40, 0, 0), (0, 0, 40), (0, 1, 0), S.DIFFUSE, (0.5, 0.5, 0.5))
+    small = S.quad((-3, 2, 0), (1, 0, 0), (0, 0, 1), (0, -1, 0), S.EMISSIVE, (1.0, 2.0, 3.0))
+    # one object, two triangles of areas 1 and 2 (a trapezoid split on a diagonal)
+    v = np.array([[2, 2, 0], [4, 2, 0], [4, 2, 2], [2, 2, 1]], np.float32)
+    big = S._obj(v, np.array([[0, 2, 3], [0, 1, 2]], np.uint32), None, S.EMISSIVE, (4.0, 4.0, 4.0))
+    return [floor, small, big]
+
+
+def test_light_sampling_areas_and_probabilities():
+    """SurfaceSampler / genSurfaceSample: objects picked in proportion to their
+    area, triangles within an object likewise, points uniform on the triangle,
+    originProb = (1/A_obj) * (A_obj/A_total) = 1/A_total, radiance = emission."""
+    sc = O.Scene(_two_lights())
+    g = (np.arange(64) + 0.5) / 64
+    picks, tris, pts = {}, {}, {1: [], 2: []}
+    for u0 in g:
+        for u1 in g[::8]:
+            for u2, u3 in ((0.2, 0.3), (0.7, 0.6), (0.45, 0.1)):
+                L = sc.light_sample([u0, u1, u2, u3])
+                picks[L["obj"]] = picks.get(L["obj"], 0) + 1
+                tris[(L["obj"], L["tri"])] = tris.get((L["obj"], L["tri"]), 0) + 1
+                pts[L["obj"]].append(L["origin"])
+                assert L["origin_prob"] == pytest.approx(1 / 4.0, rel=1e-6)
+                want_rad = (1.0, 2.0, 3.0) if L["obj"] == 1 else (4.0, 4.0, 4.0)
+                assert tuple(L["radiance"]) == want_rad
+                assert np.allclose(L["normal"], (0, -1, 0), atol=1e-6)
+    total = sum(picks.values())
+    assert picks[1] / total == pytest.approx(0.25, abs=1 / 64)
+    # big object: triangle 0 (area 1) and triangle 1 (area 2)
+    n_big = picks[2]
+    assert tris[(2, 0)] / n_big == pytest.approx(1 / 3, abs=1 / 8)
+    # every point lies on its light's surface
+    p1 = np.array(pts[1])
+    assert np.all(np.abs(p1[:, 0] + 3) <= 0.5 + 1e-6) and np.all(np.abs(p1[:, 2]) <= 0.5 + 1e-6)
+    assert np.all(p1[:, 1] == np.float32(2))
+    # uniform density: the mean of a fine grid of (u1, u2) is the triangle centroid
+    sc1 = O.Scene([_two_lights()[1]])
+    uu = (np.arange(48) + 0.5) / 48
+    P = np.array([sc1.light_sample([0.5, 0.1, a, b])["origin"] for a in uu for b in uu])
+    tri0 = _two_lights()[1]["verts"][_two_lights()[1]["tris"][0]]
+    assert np.allclose(P.mean(0), tri0.mean(0), atol=2e-3)
+    # nothing to sample: no emissive object
+    assert O.Scene([_two_lights()[0]]).light_sample([0.5, 0.5, 0.5, 0.5]) is None
+
+
+def _mean_se(sc, cam, spp, depth, nee, seed=5):
+    s, q = sc.render(cam, spp, depth, seed, nee=nee)
+    m = s.reshape(-1, 3).sum(0) / spp
+    var = q.reshape(-1, 3).sum(0) / spp - m ** 2
+    return m, np.sqrt(np.maximum(var, 0) / spp)
+
+
+@pytest.mark.parametrize("albedo", [(0.5, 0.5, 0.5), (0.3, 0.7, 0.5)])
+def test_nee_closed_form_escape(albedo):
+    """Floor point (albedo a, roulette p = max a) under a square light (form factor F).
+    Without the branch the pixel is a*Le*F.  With it, paths whose bounce escapes
+    (probability p(1-F)) add the light sample a/pi*Le*cosL*cosE/d^2*A, whose mean is
+    a*Le*F again: the pixel is a*Le*F*(1 + p(1-F)).  A dropped cosine, a missing area
+    pdf or a wrong BSDF fails this."""
+    objs, cam, h = nee_scene(albedo)
+    sc = O.Scene(objs)
+    F = form_factor_square(h, h)
+    a = np.array(albedo)
+    spp = 60000
+    m0, se0 = _mean_se(sc, cam, spp, 1, nee=False)
+    assert np.all(np.abs(m0 - a * F) <= 4 * se0 + 1e-4), (m0, a * F, se0)
+    m1, se1 = _mean_se(sc, cam, spp, 1, nee=True)
+    want = expected_escape(albedo, h)
+    assert np.all(np.abs(m1 - want) <= 4 * se1 + 1e-4), (m1, want, se1)
+    # the depth cut: maxPathLen 0 leaves endIndex 0, so nothing is connected
+    m, _ = _mean_se(sc, cam, 200, 0, nee=True)
+    assert np.all(m == 0)
+
+
+def test_nee_closed_form_max_depth():
+    """maxPathLen 1 with a diffuse ceiling (albedo c) above the light: a bounce that hits
+    the ceiling and survives its roulette ends at the depth limit, and the branch connects
+    from the ray's origin (the floor), not from the ceiling: a*Le*F*(1 + p(1-F)c)."""
+    a, c = 0.6, 0.5
+    objs, cam, h = nee_scene((a, a, a), ceiling=c)
+    sc = O.Scene(objs)
+    spp = 60000
+    m, se = _mean_se(sc, cam, spp, 1, nee=True)
+    want = expected_max_depth(a, c, h)
+    assert np.all(np.abs(m - want) <= 4 * se + 1e-4), (m, want, se)
+
+
+def test_nee_facing_and_visibility():
+    """A light facing away is neither hit (NoSelf drops back faces seen from outside)
+    nor connected (dir . N_light >= 0); a black blocker between floor and light stops
+    both the bounce (roulette with p = 0) and the connection (visibilityTest)."""
+    for kw in ({"light_up": True}, {"blocker": True}):
+        objs, cam, _ = nee_scene((0.5, 0.5, 0.5), **kw)
+        sc = O.Scene(objs)
+        for nee in (False, True):
+            m, _ = _mean_se(sc, cam, 3000, 3, nee=nee)
+            assert np.all(m == 0), (kw, nee, m)
+
+
+def test_nee_changes_only_connected_paths():
+    """Per sample: paths that end on an emitter keep their value bit for bit, and every
+    value the branch adds comes from a path whose plain sample is 0."""
+    objs, cam = S.cornell(k=1, width=16, height=16)
+    sc = O.Scene(objs)
+    changed = 0
+    for px in range(0, 256, 3):
+        for ps in range(4):
+            a, _ = sc.render_sample(cam, 5, 17, px, ps)
+            b, _ = sc.render_sample(cam, 5, 17, px, ps, nee=True)
+            if np.any(a != 0):
+                assert np.array_equal(a, b)
+            elif np.any(b != 0):
+                changed += 1
+    assert changed > 20

@@ -55,7 +55,7 @@ class CameraDesc(C.Structure):
 
 class RenderParams(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("seed", C.c_uint64), ("mode", C.c_uint32),
-                ("wave_slots", C.c_uint32), ("wave_grid", C.c_uint32), ("reserved", C.c_uint32)]
+                ("wave_slots", C.c_uint32), ("wave_grid", C.c_uint32), ("flags", C.c_uint32)]
 
 
 class RenderStats(C.Structure):
@@ -68,7 +68,7 @@ class ServerConfig(C.Structure):
 
 class JobDescC(C.Structure):
     _fields_ = [("scene", C.c_uint32), ("cam", CameraDesc), ("spp", C.c_uint32), ("max_depth", C.c_uint32),
-                ("seed", C.c_uint64)]
+                ("flags", C.c_uint32), ("seed", C.c_uint64)]
 
 
 class JobStatus(C.Structure):
@@ -400,6 +400,7 @@ class Context:
 
 
 WR_MEGAKERNEL, WR_WAVEFRONT = 0, 1
+WR_RENDER_NEE = 1
 
 
 class Film:
@@ -440,10 +441,11 @@ class Film:
             pass
 
 
-def render(scene, cam, film, spp, max_depth, seed, mode=WR_WAVEFRONT, wave_slots=0, wave_grid=0):
-    """PathTracer::renderPixels for `spp` more passes into `film`; returns stats."""
+def render(scene, cam, film, spp, max_depth, seed, mode=WR_WAVEFRONT, wave_slots=0, wave_grid=0, nee=False):
+    """PathTracer::renderPixels for `spp` more passes into `film`; returns stats.
+    nee: the useNextEventEstimation branch (pathTracer.cpp:39-66)."""
     cd = CameraDesc.from_dict(cam)
-    prm = RenderParams(spp, max_depth, seed, mode, wave_slots, wave_grid, 0)
+    prm = RenderParams(spp, max_depth, seed, mode, wave_slots, wave_grid, WR_RENDER_NEE if nee else 0)
     st = RenderStats()
     _check(lib().wr_render(scene.ctx.h, scene.h, C.byref(cd), C.byref(prm), film.h, C.byref(st)), scene.ctx.h,
            "wr_render")
@@ -598,8 +600,8 @@ class Server:
         self._scenes.append(scene)
         return sid.value
 
-    def submit(self, scene_id, cam, spp, max_depth, seed):
-        d = JobDescC(scene_id, CameraDesc.from_dict(cam), spp, max_depth, seed)
+    def submit(self, scene_id, cam, spp, max_depth, seed, nee=False):
+        d = JobDescC(scene_id, CameraDesc.from_dict(cam), spp, max_depth, WR_RENDER_NEE if nee else 0, seed)
         jid = C.c_uint64()
         rc = lib().wr_job_submit(self.h, C.byref(d), C.byref(jid))
         if rc != 0 and jid.value == 0:

@@ -201,6 +201,25 @@ int wr_scene_add_mesh_corners(wr_scene* s, const float* verts, uint32_t nv, cons
     return WR_OK;
 }
 
+// Mesh::getTriangleArea (mesh.h:80-88) in float: |(v1 - v0) x (v2 - v1)| / 2 with
+// nvVector's cross and length (nvVector.h:490-525).
+static float triangle_area(const HostObject& o, uint32_t t) {
+    float v[3][3];
+    for (int k = 0; k < 3; k++)
+        for (int a = 0; a < 3; a++) v[k][a] = o.verts[3 * (size_t)o.idx[3 * (size_t)t + k] + a];
+    float l0[3], l1[3];
+    for (int a = 0; a < 3; a++) {
+        l0[a] = v[1][a] - v[0][a];
+        l1[a] = v[2][a] - v[1][a];
+    }
+    const float c[3] = {l0[1] * l1[2] - l0[2] * l1[1], l0[2] * l1[0] - l0[0] * l1[2], l0[0] * l1[1] - l0[1] * l1[0]};
+    float r = 0.f;
+    r += c[0] * c[0];
+    r += c[1] * c[1];
+    r += c[2] * c[2];
+    return std::sqrt(r) / 2;
+}
+
 int wr_scene_build(wr_scene* s, int strategy) {
     if (!s) return fail(nullptr, WR_EINVAL, "wr_scene_build: scene is NULL");
     wr_ctx* ctx = s->ctx;
@@ -381,6 +400,48 @@ int wr_scene_build(wr_scene* s, int strategy) {
         }
         cell_start[ncells] = (uint32_t)cell_objs.size();
 
+        // light sampler (Scene::prepareForLightSampling, scene.cpp:5-14;
+        // SurfaceSampler::prepareForSampling, surfaceSampler.cpp:7-15;
+        // AbstractObject::prepareForSurfaceSampling, abstractObj.cpp:9-16): running
+        // float sums of Mesh::getTriangleArea in object and triangle order
+        std::vector<wr_light> lights;
+        std::vector<float> light_cum, light_tri_cum;
+        std::vector<uint32_t> light_of(s->objs.size(), WR_NONE);
+        float light_total = 0.f;
+        for (size_t oi = 0; oi < s->objs.size(); oi++) {
+            const HostObject& o = s->objs[oi];
+            if (o.mat.type != WR_EMISSIVE) continue;
+            wr_light L{};
+            L.obj = (uint32_t)oi;
+            L.tri_base = (uint32_t)light_tri_cum.size();
+            L.n_tris = (uint32_t)(o.idx.size() / 3);
+            float area = 0.f;
+            for (uint32_t t = 0; t < L.n_tris; t++) {
+                area += triangle_area(o, t);
+                light_tri_cum.push_back(area);
+            }
+            L.area = area;
+            light_total += area;
+            light_cum.push_back(light_total);
+            light_of[oi] = (uint32_t)lights.size();
+            lights.push_back(L);
+        }
+        for (wr_light& L : lights) {
+            const float pick = L.area / light_total;  // pickProb (surfaceSampler.cpp:13-14)
+            L.oprob = 1.f / L.area * pick;
+        }
+        std::vector<uint32_t> light_tri_rec(light_tri_cum.size(), WR_NONE);
+        {
+            const FlatTree& G = s->trees[0];  // its records come first in the record arrays
+            for (size_t r = 0; r < G.rec_prim.size(); r++) {
+                const uint32_t oi = rec_obj[r];
+                if (light_of[oi] == WR_NONE) continue;
+                light_tri_rec[lights[light_of[oi]].tri_base + (G.rec_prim[r] - s->prim_base[oi])] = (uint32_t)r;
+            }
+        }
+        for (uint32_t r : light_tri_rec)
+            if (r == WR_NONE) return fail(ctx, WR_EINVAL, "wr_scene_build: an emissive triangle has no tree record");
+
         uint64_t bytes = 0;
         uint32_t *d_cs, *d_co;
         wr_node* d_nodes;
@@ -426,6 +487,17 @@ int wr_scene_build(wr_scene* s, int strategy) {
         s->dbufs.push_back(d_cs);
         if ((rc = upload(ctx, cell_objs, &d_co, bytes))) return rc;
         s->dbufs.push_back(d_co);
+        wr_light* d_li;
+        float *d_lcum, *d_ltcum;
+        uint32_t* d_ltrec;
+        if ((rc = upload(ctx, lights, &d_li, bytes))) return rc;
+        s->dbufs.push_back(d_li);
+        if ((rc = upload(ctx, light_cum, &d_lcum, bytes))) return rc;
+        s->dbufs.push_back(d_lcum);
+        if ((rc = upload(ctx, light_tri_cum, &d_ltcum, bytes))) return rc;
+        s->dbufs.push_back(d_ltcum);
+        if ((rc = upload(ctx, light_tri_rec, &d_ltrec, bytes))) return rc;
+        s->dbufs.push_back(d_ltrec);
         SceneDev& D = s->dev;
         D.nodes = d_nodes;
         D.wnodes = d_wnodes;
@@ -446,6 +518,12 @@ int wr_scene_build(wr_scene* s, int strategy) {
         D.max_obj_depth = max_obj_depth;
         D.cell_start = d_cs;
         D.cell_objs = d_co;
+        D.lights = d_li;
+        D.light_cum = d_lcum;
+        D.light_tri_cum = d_ltcum;
+        D.light_tri_rec = d_ltrec;
+        D.n_lights = (uint32_t)lights.size();
+        D.light_total = light_total;
         for (int a = 0; a < 3; a++) {
             D.grid_min[a] = gmin[a];
             D.grid_max[a] = gmax[a];

@@ -24,6 +24,7 @@
 
 namespace wrd {
 
+template <bool NEE>
 __global__ __launch_bounds__(256) void k_pt_mega(SceneDev S, CamDev cam, uint32_t max_depth, uint64_t seed,
                                                  uint32_t pass, FilmDev film, uint32_t npix) {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
@@ -31,7 +32,7 @@ __global__ __launch_bounds__(256) void k_pt_mega(SceneDev S, CamDev cam, uint32_
     LaneStack st;
     st.lane = threadIdx.x;
     const SceneCtx sc = make_scene_ctx(S);
-    const f3 v = sample_path(sc, cam, max_depth, seed, p, pass, st);
+    const f3 v = sample_path<NEE>(sc, cam, max_depth, seed, p, pass, st);
     film_add(film, p, v);
 }
 
@@ -46,13 +47,25 @@ __global__ __launch_bounds__(256) void k_pt_mega(SceneDev S, CamDev cam, uint32_
 // advance retires finished jobs.  Per-path randomness is keyed by the job's
 // seed and the (pixel, pass, vertex) of the path, so a job's film does not
 // depend on which other jobs share the pool.
+//
+// Jobs with next-event estimation (pathTracer.cpp:39-66) keep three more
+// fields per path: U and P, the branch-order products with and without the
+// path's current ray, and crec, the triangle under the ray's origin (kFront
+// holds the incoming side there).  A path whose ray escapes (k_wf_finish) or
+// that reaches the depth limit (k_wf_shade) connects that origin to a light
+// sample: it becomes a connection entry (kShadow) of the next iteration's
+// buffer, traced by the same extend stage with the same NoSelf condition, and
+// k_wf_finish ends it with the visibility test.
 constexpr uint32_t kPrimary = 1u;
 constexpr uint32_t kDead = 2u;
+constexpr uint32_t kShadow = 8u;   // connection entry: T = value if visible, ux = distance to the light point
+constexpr uint32_t kFront = 16u;   // (incoming direction . N) <= 0 at the ray's origin
 
 struct PathBuf {
     float *ox, *oy, *oz, *dx, *dy, *dz;
     float *tx, *ty, *tz, *ht;
-    uint32_t *pixel, *pass, *depth, *inside, *contact, *hrec, *hobj, *flags, *job;
+    float *ux, *uy, *uz, *px, *py, *pz;
+    uint32_t *pixel, *pass, *depth, *inside, *contact, *hrec, *hobj, *flags, *job, *crec;
 };
 
 struct EngineDev {
@@ -103,6 +116,24 @@ __device__ __forceinline__ void count_done(uint32_t* addr) {
             return;
         }
     }
+}
+
+// Appends a connection entry (one atomic per wave) to the next iteration's buffer.
+__device__ __forceinline__ void push_connection(const PathBuf& B, uint32_t* counter, f3 o, const NeeConn& c,
+                                                uint32_t inside, uint32_t contact, uint32_t pixel, uint32_t pass,
+                                                uint32_t job) {
+    const uint32_t j = wave_append(counter);
+    B.ox[j] = o.x; B.oy[j] = o.y; B.oz[j] = o.z;
+    B.dx[j] = c.dir.x; B.dy[j] = c.dir.y; B.dz[j] = c.dir.z;
+    B.tx[j] = c.C.x; B.ty[j] = c.C.y; B.tz[j] = c.C.z;
+    B.ux[j] = c.dist;
+    B.pixel[j] = pixel;
+    B.pass[j] = pass;
+    B.depth[j] = 0;
+    B.inside[j] = inside;
+    B.contact[j] = contact;
+    B.flags[j] = kShadow;
+    B.job[j] = job;
 }
 
 __device__ __forceinline__ void finish(const wr::JobDesc& J, const CamDev& cam, uint32_t p, uint32_t ps, f3 color) {
@@ -215,10 +246,12 @@ __global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
 
 // extend, part 2 (one lane per path): binary fast pass after an overflow,
 // chain verification and exact fallback (trace_dev.h: finish_hit),
-// findInsideObject for camera rays (camera.cpp:52-56), and the end of paths
-// that escape.
+// findInsideObject for camera rays (camera.cpp:52-56), the end of paths that
+// escape (or their next-event connection), and the visibility test of
+// connection entries.
 __global__ __launch_bounds__(256) void k_wf_finish(EngineDev E, uint32_t par) {
     const PathBuf& A = E.buf[par];
+    const PathBuf& B = E.buf[par ^ 1];
     const uint32_t n = E.cnt->n[par];
     LaneStack st;
     st.lane = threadIdx.x;
@@ -239,8 +272,26 @@ __global__ __launch_bounds__(256) void k_wf_finish(EngineDev E, uint32_t par) {
             bool failed;
             finish_hit<true, false>(sc.tr, r, ns, st, cnt, h, (fl & kOverflow) != 0, failed);
         }
+        if (fl & kShadow) {  // MCRenderer::visibilityTest (mcRenderer.cpp:59-65)
+            const bool vis = nee_visible(h.rec, h.t, A.ux[i]);
+            finish(J, job_cam(J), A.pixel[i], A.pass[i], vis ? mk(A.tx[i], A.ty[i], A.tz[i]) : mk(0.f, 0.f, 0.f));
+            A.flags[i] = kDead;
+            continue;
+        }
         if (primary) A.inside[i] = find_inside(S, o, d, WR_NONE, st);
         if (h.rec == WR_NONE) {  // escaped (no environment light) or nothing to scatter: sample 0
+            if ((J.flags & wr::kJobNee) && !primary) {  // endIndex = this ray: connect from its origin
+                float rn[4];
+                rng4(J.seed, A.pixel[i], J.pass0 + A.pass[i], kNeeVertex, rn);
+                const NeeConn c = nee_connect(sc, o, A.contact[i], A.crec[i], (fl & kFront) != 0,
+                                              mk(A.px[i], A.py[i], A.pz[i]), rn);
+                if (c.ok) {
+                    push_connection(B, &E.cnt->n[par ^ 1], o, c, A.inside[i], A.contact[i], A.pixel[i], A.pass[i],
+                                    A.job[i]);
+                    A.flags[i] = kDead;
+                    continue;
+                }
+            }
             finish(J, job_cam(J), A.pixel[i], A.pass[i], mk(0.f, 0.f, 0.f));
             A.flags[i] = kDead;
             continue;
@@ -248,7 +299,7 @@ __global__ __launch_bounds__(256) void k_wf_finish(EngineDev E, uint32_t par) {
         A.ht[i] = h.t;
         A.hrec[i] = h.rec;
         A.hobj[i] = sc.gobj[h.rec];
-        A.flags[i] = 0;
+        A.flags[i] = fl & kFront;
     }
 }
 
@@ -279,7 +330,17 @@ __global__ __launch_bounds__(256) void k_wf_shade(EngineDev E, uint32_t par) {
             finish(J, job_cam(J), p, ps, mulv(T, divs(divs(out.rad, out.op), out.dp)));
             continue;
         }
+        const bool nee = (J.flags & wr::kJobNee) != 0;
         if (depth + 1 > J.max_depth) {
+            if (nee && depth >= 1) {  // endIndex = this ray: connect from its origin
+                rng4(J.seed, p, J.pass0 + ps, kNeeVertex, rn);
+                const NeeConn c = nee_connect(sc, o, A.contact[i], A.crec[i], (A.flags[i] & kFront) != 0,
+                                              mk(A.px[i], A.py[i], A.pz[i]), rn);
+                if (c.ok) {
+                    push_connection(B, &E.cnt->n[par ^ 1], o, c, inside, A.contact[i], p, ps, jb);
+                    continue;
+                }
+            }
             finish(J, job_cam(J), p, ps, mk(0.f, 0.f, 0.f));
             continue;
         }
@@ -288,6 +349,14 @@ __global__ __launch_bounds__(256) void k_wf_shade(EngineDev E, uint32_t par) {
         T = muls(T, cosine_term(out));
         T = mulv(T, divs(divs(out.rad, out.op), out.dp));
         const uint32_t j = wave_append(&E.cnt->n[par ^ 1]);
+        if (nee) {  // the next ray's products (branch order) and its origin's triangle and side
+            const f3 U = mk(A.ux[i], A.uy[i], A.uz[i]);
+            f3 U2 = mulv(U, divs(divs(out.rad, out.op), out.dp));
+            U2 = muls(U2, cosine_term(out));
+            B.px[j] = U.x; B.py[j] = U.y; B.pz[j] = U.z;
+            B.ux[j] = U2.x; B.uy[j] = U2.y; B.uz[j] = U2.z;
+            B.crec[j] = h.rec;
+        }
         B.ox[j] = out.o.x; B.oy[j] = out.o.y; B.oz[j] = out.o.z;
         B.dx[j] = out.d.x; B.dy[j] = out.d.y; B.dz[j] = out.d.z;
         B.tx[j] = T.x; B.ty[j] = T.y; B.tz[j] = T.z;
@@ -296,7 +365,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(EngineDev E, uint32_t par) {
         B.depth[j] = depth + 1;
         B.inside[j] = out.inside;
         B.contact[j] = out.contact;
-        B.flags[j] = 0;
+        B.flags[j] = nee && dot(d, out.n) <= 0.f ? kFront : 0u;
         B.job[j] = jb;
     }
 }
@@ -355,6 +424,9 @@ __global__ __launch_bounds__(256) void k_wf_generate(EngineDev E, uint32_t par) 
         B.ox[j] = cam.pos.x; B.oy[j] = cam.pos.y; B.oz[j] = cam.pos.z;
         B.dx[j] = d.x; B.dy[j] = d.y; B.dz[j] = d.z;
         B.tx[j] = c0; B.ty[j] = c0; B.tz[j] = c0;
+        if (J.flags & wr::kJobNee) {  // branch-order product of the camera ray: (1,1,1) * 1/1/1 * cos
+            B.ux[j] = c0; B.uy[j] = c0; B.uz[j] = c0;
+        }
         B.pixel[j] = p;
         B.pass[j] = ps;
         B.depth[j] = 0;
@@ -429,10 +501,13 @@ static CamDev to_cam(const CamHost& c) {
 }
 
 hipError_t launch_pt_mega(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass,
-                          const FilmBufs& film, hipStream_t st) {
+                          bool nee, const FilmBufs& film, hipStream_t st) {
     const uint32_t npix = cam.W * cam.H;
-    k_pt_mega<<<(npix + 255) / 256, 256, kStackLdsBytes, st>>>(S, to_cam(cam), max_depth, seed, pass,
-                                                                FilmDev{film.sum, film.comp, film.sumsq}, npix);
+    const FilmDev f{film.sum, film.comp, film.sumsq};
+    if (nee)
+        k_pt_mega<true><<<(npix + 255) / 256, 256, kStackLdsBytes, st>>>(S, to_cam(cam), max_depth, seed, pass, f, npix);
+    else
+        k_pt_mega<false><<<(npix + 255) / 256, 256, kStackLdsBytes, st>>>(S, to_cam(cam), max_depth, seed, pass, f, npix);
     return hipGetLastError();
 }
 
@@ -457,8 +532,10 @@ void Engine::release() {
 
 static PathBuf to_pathbuf(void* const* w) {
     PathBuf b;
-    float** f[] = {&b.ox, &b.oy, &b.oz, &b.dx, &b.dy, &b.dz, &b.tx, &b.ty, &b.tz, &b.ht};
-    uint32_t** u[] = {&b.pixel, &b.pass, &b.depth, &b.inside, &b.contact, &b.hrec, &b.hobj, &b.flags, &b.job};
+    float** f[] = {&b.ox, &b.oy, &b.oz, &b.dx, &b.dy, &b.dz, &b.tx, &b.ty, &b.tz, &b.ht,
+                   &b.ux, &b.uy, &b.uz, &b.px, &b.py, &b.pz};
+    uint32_t** u[] = {&b.pixel, &b.pass, &b.depth, &b.inside, &b.contact, &b.hrec, &b.hobj, &b.flags, &b.job, &b.crec};
+    static_assert(sizeof(f) / sizeof(f[0]) + sizeof(u) / sizeof(u[0]) == kPathWords, "path state words");
     int k = 0;
     for (float** p : f) *p = (float*)w[k++];
     for (uint32_t** p : u) *p = (uint32_t*)w[k++];
@@ -565,7 +642,7 @@ void fill_job_camera(JobDesc& J, const CamHost& c) {
 
 // Renders passes [pass0, pass0 + npass) of one film on the engine, alone.
 hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
-                         uint32_t npass, const FilmBufs& film, Engine& E, uint32_t cap, uint32_t grid,
+                         uint32_t npass, bool nee, const FilmBufs& film, Engine& E, uint32_t cap, uint32_t grid,
                          WaveStats* stats, hipStream_t st) {
     const uint32_t npix = cam.W * cam.H;
     if (npass == 0 || npix == 0) return hipSuccess;
@@ -576,6 +653,7 @@ hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_dep
     fill_job_camera(J, cam);
     J.scene = 0;
     J.max_depth = max_depth;
+    J.flags = nee ? kJobNee : 0u;
     J.seed = seed;
     J.pass0 = pass0;
     J.npass = npass;
@@ -601,7 +679,7 @@ hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_dep
         e = E.step(st, hc.data(), &hcnt);
         replays++;
         if (e != hipSuccess || hc[0].state == kJobDone) break;
-        if (replays > 4 * (uint64_t)(max_depth + 3) * (total / E.cap + 2) + 16) {  // cannot happen
+        if (replays > 4 * (uint64_t)(max_depth + 4) * (total / E.cap + 2) + 16) {  // cannot happen
             e = hipErrorUnknown;
             break;
         }

@@ -10,7 +10,8 @@
 // direction.  Randomness comes from Philox-4x32-10 keyed by (seed, pixel, pass,
 // vertex) instead of the reference's one shared mt19937_64 (rng.h:6-59); the
 // draws of one vertex are: camera (u, v); diffuse/glossy (roulette, s.x, s.y);
-// refractive (Fresnel choice).
+// refractive (Fresnel choice); the next-event light sample (object pick,
+// triangle pick, u1, u2) under vertex kNeeVertex.
 #pragma once
 #include "trace_dev.h"
 #include "wr_kernels.h"
@@ -269,6 +270,124 @@ __device__ Out scatter(const SceneCtx& sc, uint32_t self, f3 in_o, f3 in_d, cons
     return out;
 }
 
+// ---- next-event estimation (PathTracer's useNextEventEstimation branch) -------
+// Vertex counter of the light sample's uniforms; path vertices use 0 .. max_depth + 1.
+constexpr uint32_t kNeeVertex = 0xFFFFFFFFu;
+
+// std::lower_bound over n ascending floats: the first index whose value is not
+// less than v (n when there is none).
+__device__ __forceinline__ uint32_t lower_bound_f(const float* a, uint32_t n, float v) {
+    uint32_t lo = 0;
+    while (n > 0) {
+        const uint32_t half = n >> 1;
+        if (a[lo + half] < v) {
+            lo += half + 1;
+            n -= half + 1;
+        } else {
+            n = half;
+        }
+    }
+    return lo;
+}
+
+struct LightSample {
+    f3 o, n, rad;
+    float op;
+    bool ok;
+};
+
+// Scene::genLightSample -> SurfaceSampler::genSample (surfaceSampler.cpp:17-23)
+// -> AbstractObject::genSurfaceSample (abstractObj.cpp:18-52) with
+// Mesh::genRandTrianglePosition (mesh.h:89-101).  rn: object pick, triangle
+// pick, u1, u2.  The hemisphere direction, findInsideObject and the NoSelf
+// intersect of the light ray are not formed: the path tracer overwrites the
+// direction and reads neither of the others.  ok == false: no emissive object,
+// or the picked one has no triangles (its sample has no contact object).
+__device__ __forceinline__ LightSample sample_light(const SceneCtx& sc, const float* rn) {
+    const SceneDev& S = sc.S;
+    LightSample L;
+    L.ok = false;
+    if (S.n_lights == 0) return L;
+    uint32_t li = lower_bound_f(S.light_cum, S.n_lights, rn[0] * S.light_total);
+    if (li == S.n_lights) li--;
+    const wr_light lt = S.lights[li];
+    if (lt.n_tris == 0) return L;
+    uint32_t ti = lower_bound_f(S.light_tri_cum + lt.tri_base, lt.n_tris, rn[1] * lt.area);
+    if (ti == lt.n_tris) ti--;
+    const uint32_t rec = S.light_tri_rec[lt.tri_base + ti];
+    const wr_tri& T = sc.tr.tris[rec];
+    const f3 v0 = mk(T.v0[0], T.v0[1], T.v0[2]);
+    const f3 b1 = sub(mk(T.v1[0], T.v1[1], T.v1[2]), v0);
+    const f3 b2 = sub(mk(T.v2[0], T.v2[1], T.v2[2]), v0);
+    float u1 = rn[2], u2 = rn[3];
+    if (u1 + u2 > 1.f) {
+        u1 = 1.f - u1;
+        u2 = 1.f - u2;
+    }
+    L.o = add(add(v0, muls(b1, u1)), muls(b2, u2));
+    L.n = rec_normal(sc, rec, lt.obj, L.o);
+    const wr_object_gpu& ob = S.objs[lt.obj];
+    L.rad = mk(ob.color[0], ob.color[1], ob.color[2]);  // EmissiveObject::evaluateBSDF
+    L.op = lt.oprob;
+    L.ok = true;
+    return L;
+}
+
+// Ray::cosineTerm (ray.cpp:52-58) of a ray with a contact object.
+__device__ __forceinline__ float cos_term(f3 n, f3 d) {
+    float c = 1.f;
+    if ((double)len(d) > 0.5) c = fabsf(dot(n, d));
+    return (double)c > kEps ? c : (float)kEps;
+}
+
+struct NeeConn {
+    f3 dir, C;  // connection direction; the sample's value if the light point is visible
+    float dist; // |light point - o|
+    bool ok;
+};
+
+// The branch up to its visibility test (pathTracer.cpp:42-52, 58-65), at the
+// origin o of the path's ray endRay: contact = its contactObj, crec = the
+// global record of the triangle under o (contactTriangleID), front = (the
+// incoming direction . N) <= 0 (the first condition of DiffuseBSDF::evalBSDF,
+// diffuseBsdf.h:29-34), P = the product over the rays before endRay in the
+// branch's order (radiance / originProb / directionProb, cosine, decay).
+// ok == false: the branch adds nothing.
+__device__ __forceinline__ NeeConn nee_connect(const SceneCtx& sc, f3 o, uint32_t contact, uint32_t crec, bool front,
+                                               f3 P, const float* rn) {
+    NeeConn c;
+    c.ok = false;
+    const wr_object_gpu& ob = sc.S.objs[contact];
+    // isEmissive / !nonSpecular: only diffuse vertices connect (glossy is
+    // specular without MERGE_GLOSSY, glossyObj.h:32-38; refractiveObj.h:26)
+    if (ob.mat != WR_MAT_DIFFUSE) return c;
+    const LightSample L = sample_light(sc, rn);
+    if (!L.ok) return c;
+    const f3 dir = normalize(sub(L.o, o));
+    const float dist = len(sub(L.o, o));
+    const float cd = (double)dist > kEps ? dist : (float)kEps;  // MAX(dist, EPSILON)
+    if (dot(dir, L.n) >= 0.f) return c;
+    const f3 nE = rec_normal(sc, crec, sc.gobj[crec], o);
+    f3 f = (front && dot(dir, nE) >= 0.f) ? muls(mk(ob.color[0], ob.color[1], ob.color[2]), (float)kInvPi)
+                                          : mk(0.f, 0.f, 0.f);
+    f = mulv(f, L.rad);  // radianceDecay is (1,1,1) here: multiplying by it is exact
+    f = muls(f, cos_term(L.n, mk(-dir.x, -dir.y, -dir.z)));
+    f = muls(f, cos_term(nE, dir));
+    f = divs(f, cd * cd);
+    // endRay's originProb is 1 for every scatter here, so the divisor is the light's
+    c.C = divs(mulv(P, f), L.op);
+    c.dir = dir;
+    c.dist = dist;
+    c.ok = true;
+    return c;
+}
+
+// MCRenderer::visibilityTest (mcRenderer.cpp:59-65) from the closest hit of the
+// connection ray.
+__device__ __forceinline__ bool nee_visible(uint32_t rec, float t, float dist) {
+    return rec == WR_NONE || (double)(t - dist) >= -kEps;
+}
+
 // ---- Kahan film ---------------------------------------------------------------
 struct FilmDev {
     float* sum;    // 3 per pixel
@@ -295,7 +414,23 @@ __device__ __forceinline__ void film_add(const FilmDev& f, uint32_t p, f3 v) {
     }
 }
 
+// The next-event branch for one path, connection and visibility test included.
+__device__ __forceinline__ f3 nee_sample(const SceneCtx& sc, f3 o, uint32_t contact, uint32_t crec, bool front, f3 P,
+                                         uint32_t inside, uint64_t seed, uint32_t p, uint32_t pass, LaneStack& st) {
+    float rn[4];
+    rng4(seed, p, pass, kNeeVertex, rn);
+    const NeeConn c = nee_connect(sc, o, contact, crec, front, P, rn);
+    if (!c.ok) return mk(0.f, 0.f, 0.f);
+    const PHit h = trace_scene<true>(sc, o, c.dir, inside, contact, st);
+    return nee_visible(h.rec, h.t, c.dist) ? c.C : mk(0.f, 0.f, 0.f);
+}
+
 // ---- one eye path, megakernel style (mcRenderer.cpp:16-57, pathTracer.cpp:22-69) ----
+// NEE: PathTracer's next-event branch (pathTracer.cpp:39-66).  It only changes
+// paths whose last entry is terminateRay after an escape or at the depth limit
+// (hasToConnect with endIndex > 0): they connect the origin of their last
+// traced ray to a light sample.  Every other path keeps the plain value.
+template <bool NEE>
 __device__ f3 sample_path(const SceneCtx& sc, const CamDev& cam, uint32_t max_depth, uint64_t seed, uint32_t p,
                           uint32_t pass, LaneStack& st) {
     float rn[4];
@@ -307,6 +442,12 @@ __device__ f3 sample_path(const SceneCtx& sc, const CamDev& cam, uint32_t max_de
     float c0 = fabsf(dot(cam.front, d));
     c0 = (double)c0 > kEps ? c0 : (float)kEps;
     f3 T = mk(c0, c0, c0);
+    // NEE state of the current ray (index depth): U = the branch-order product up
+    // to it, P = without it; contact object, triangle record and incoming side
+    // at its origin (valid for depth >= 1)
+    f3 U = T, P = T;
+    uint32_t c_contact = WR_NONE, c_rec = WR_NONE;
+    bool c_front = false;
     f3 color = mk(0.f, 0.f, 0.f);
     uint32_t depth = 0;
     for (;;) {
@@ -318,11 +459,27 @@ __device__ f3 sample_path(const SceneCtx& sc, const CamDev& cam, uint32_t max_de
             color = mulv(T, divs(divs(out.rad, out.op), out.dp));
             break;
         }
-        if (depth + 1 > max_depth) break;
+        if (depth + 1 > max_depth) {  // endIndex = depth: connect from this ray's origin
+            if (NEE && depth >= 1) color = nee_sample(sc, o, c_contact, c_rec, c_front, P, inside, seed, p, pass, st);
+            break;
+        }
         const PHit nh = trace_scene<true>(sc, out.o, out.d, out.inside, out.contact, st);
-        if (nh.rec == WR_NONE) break;
+        if (nh.rec == WR_NONE) {  // escaped: endIndex = depth + 1, connect from out's origin
+            if (NEE)
+                color = nee_sample(sc, out.o, out.contact, hit.rec, dot(d, out.n) <= 0.f, U, out.inside, seed, p,
+                                   pass, st);
+            break;
+        }
         T = muls(T, cosine_term(out));
         T = mulv(T, divs(divs(out.rad, out.op), out.dp));
+        if (NEE) {
+            P = U;
+            U = mulv(U, divs(divs(out.rad, out.op), out.dp));
+            U = muls(U, cosine_term(out));
+            c_contact = out.contact;
+            c_rec = hit.rec;
+            c_front = dot(d, out.n) <= 0.f;
+        }
         o = out.o;
         d = out.d;
         hit = nh;

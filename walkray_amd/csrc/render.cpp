@@ -147,6 +147,8 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
     if (film->W != cam->width || film->H != cam->height)
         return fail(ctx, WR_EINVAL, "wr_render: film size differs from the camera resolution");
     if (prm->mode != WR_MEGAKERNEL && prm->mode != WR_WAVEFRONT) return fail(ctx, WR_EINVAL, "wr_render: unknown mode");
+    if (prm->flags & ~(uint32_t)WR_RENDER_NEE) return fail(ctx, WR_EINVAL, "wr_render: unknown flags");
+    const bool nee = (prm->flags & WR_RENDER_NEE) != 0;
     if ((uint64_t)cam->width * cam->height >= (1ull << 31)) return fail(ctx, WR_EINVAL, "wr_render: image too large");
     (void)hipSetDevice(ctx->device);
     const CamHost ch = make_cam(*cam);
@@ -156,9 +158,9 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
     if (e == hipSuccess) {
         if (prm->mode == WR_MEGAKERNEL) {
             for (uint32_t k = 0; k < prm->spp && e == hipSuccess; k++)
-                e = launch_pt_mega(s->dev, ch, prm->max_depth, prm->seed, film->passes + k, fb, ctx->stream);
+                e = launch_pt_mega(s->dev, ch, prm->max_depth, prm->seed, film->passes + k, nee, fb, ctx->stream);
         } else {
-            e = run_wavefront(s->dev, ch, prm->max_depth, prm->seed, film->passes, prm->spp, fb, ctx->engine,
+            e = run_wavefront(s->dev, ch, prm->max_depth, prm->seed, film->passes, prm->spp, nee, fb, ctx->engine,
                               prm->wave_slots ? prm->wave_slots : ctx->tuned_slots,
                               prm->wave_grid ? prm->wave_grid : ctx->tuned_grid, &ws, ctx->stream);
         }

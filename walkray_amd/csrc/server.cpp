@@ -188,6 +188,7 @@ static hipError_t admit_one(wr_server* s, Job& j, int slot) {
     fill_job_camera(J, ch);
     J.scene = j.desc.scene;
     J.max_depth = j.desc.max_depth;
+    J.flags = (j.desc.flags & WR_RENDER_NEE) ? kJobNee : 0u;
     J.seed = j.desc.seed;
     J.pass0 = 0;
     J.npass = j.desc.spp;

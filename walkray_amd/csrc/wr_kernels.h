@@ -31,6 +31,13 @@ struct SceneDev {
     const uint32_t* cell_objs;
     float grid_min[3], grid_max[3], grid_inv[3];
     uint32_t grid_dim[3];
+    // light sampler (next-event estimation): emissive objects in object order
+    const wr_light* lights;
+    const float* light_cum;          // SurfaceSampler::areaThreshold (n_lights)
+    const float* light_tri_cum;      // per light triangle: AbstractObject::areaThreshold entry
+    const uint32_t* light_tri_rec;   // per light triangle: its global-tree record
+    uint32_t n_lights;
+    float light_total;               // SurfaceSampler::boundArea
 };
 
 // Cell coordinate of x on one grid axis: monotone in x, so a point inside a

@@ -101,3 +101,17 @@ struct wr_object_gpu {
     uint32_t tree;        // index of the object's own tree header (findInsideObject)
     uint32_t pad[3];
 };
+
+// One emissive object of the light sampler (Scene::prepareForLightSampling,
+// scene.cpp:5-14): its area (AbstractObject::boundArea), originProb = 1 /
+// boundArea * pickProb (abstractObj.cpp:41) and its triangles' entries in the
+// per-triangle tables (cumulative areas, global-tree records).
+struct wr_light {
+    float area;
+    float oprob;
+    uint32_t obj;
+    uint32_t tri_base;
+    uint32_t n_tris;
+    uint32_t pad[3];
+};
+static_assert(sizeof(wr_light) == 32, "light record must be 32 bytes");

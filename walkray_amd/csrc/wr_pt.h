@@ -24,14 +24,15 @@ struct FilmBufs {
     float* sumsq;
 };
 
-constexpr int kPathWords = 19;           // 32-bit words of path state per slot (SoA arrays)
+constexpr int kPathWords = 26;           // 32-bit words of path state per slot (SoA arrays)
 constexpr uint32_t kMaxPassWindow = 64;  // passes of one job whose samples may be in flight
 constexpr uint32_t kMaxJobs = 64;        // concurrently admitted jobs
 constexpr uint32_t kMaxScenes = 64;      // resident scenes of one engine
 constexpr int kGraphIters = 8;           // wavefront iterations per graph replay (even)
-constexpr uint32_t kDefaultPathSlots = 1u << 22;  // 4M paths in flight: 2 x 304 MB of path state
+constexpr uint32_t kDefaultPathSlots = 1u << 22;  // 4M paths in flight: 2 x 416 MB of path state
 
 enum : uint32_t { kJobFree = 0, kJobRunning = 1, kJobDone = 2 };
+enum : uint32_t { kJobNee = 1 };  // JobDesc::flags: PathTracer's next-event branch
 
 // Device record of one admitted job.
 struct JobDesc {
@@ -41,6 +42,7 @@ struct JobDesc {
     uint32_t scene, max_depth;
     uint64_t seed;
     uint32_t pass0, npass, npix, R;  // R: passes in flight (sample window)
+    uint32_t flags, pad;             // kJobNee
     float* contrib;                  // R * npix * 3 samples of the passes in flight
     uint32_t* pass_done;             // R counters: finished paths per window slot
     float *fsum, *fcomp, *fsumsq;    // the job's film
@@ -92,9 +94,9 @@ struct Engine {
 void fill_job_camera(JobDesc& J, const CamHost& c);
 
 hipError_t launch_pt_mega(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass,
-                          const FilmBufs& film, hipStream_t st);
+                          bool nee, const FilmBufs& film, hipStream_t st);
 hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
-                         uint32_t npass, const FilmBufs& film, Engine& engine, uint32_t cap, uint32_t grid,
+                         uint32_t npass, bool nee, const FilmBufs& film, Engine& engine, uint32_t cap, uint32_t grid,
                          WaveStats* stats, hipStream_t st);
 
 }  // namespace wr

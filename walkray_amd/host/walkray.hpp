@@ -225,16 +225,20 @@ class Scene {
 
 // PathTracer (pathTracer.h:7-18): renderPixels returns the per-pixel mean
 // radiance, row-major, top row first (pathTracer.cpp:10,75).
+// useNextEventEstimation starts false as in the reference's constructor
+// (pathTracer.h:14); the setter is this mirror's addition.
 class PathTracer {
   public:
     PathTracer(const Scene& scene, unsigned spp, unsigned maxLen, uint64_t seed = 1,
                uint32_t mode = WR_WAVEFRONT)
         : scene_(scene), spp_(spp), maxLen_(maxLen), seed_(seed), mode_(mode) {}
 
+    void setNextEventEstimation(bool on) { useNextEventEstimation_ = on; }
+
     std::vector<vec3f> renderPixels(const wr_camera& cam) const {
         wr_film* film = nullptr;
         check(wr_film_create(scene_.ctx(), cam.width, cam.height, &film), scene_.ctx(), "wr_film_create");
-        wr_render_params p{spp_, maxLen_, seed_, mode_, 0, 0, 0};
+        wr_render_params p{spp_, maxLen_, seed_, mode_, 0, 0, useNextEventEstimation_ ? (uint32_t)WR_RENDER_NEE : 0u};
         int rc = wr_render(scene_.ctx(), scene_.get(), &cam, &p, film, nullptr);
         std::vector<vec3f> px((size_t)cam.width * cam.height);
         if (rc == WR_OK) rc = wr_film_read(film, &px[0].x, nullptr, nullptr, nullptr, nullptr);
@@ -248,6 +252,7 @@ class PathTracer {
     unsigned spp_, maxLen_;
     uint64_t seed_;
     uint32_t mode_;
+    bool useNextEventEstimation_ = false;
 };
 
 // Config (config.h): load reads a reference scene configuration into a Scene

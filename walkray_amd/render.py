@@ -37,10 +37,10 @@ def expected_rel_rmse(state):
     return float(np.sqrt((se ** 2).sum()) / max(np.linalg.norm(mean), 1e-300))
 
 
-def render_scene(ctx, objs, cam, spp, max_depth, seed, mode, scene=None, wave_slots=0):
+def render_scene(ctx, objs, cam, spp, max_depth, seed, mode, scene=None, wave_slots=0, nee=False):
     sc = scene or W.Scene(ctx, objs)
     film = W.Film(ctx, cam["width"], cam["height"])
-    st = W.render(sc, cam, film, spp, max_depth, seed, mode, wave_slots=wave_slots)
+    st = W.render(sc, cam, film, spp, max_depth, seed, mode, wave_slots=wave_slots, nee=nee)
     out = film.read()
     out["stats"] = st
     return out
