@@ -46,14 +46,14 @@ def render_scene(ctx, objs, cam, spp, max_depth, seed, mode, scene=None, wave_sl
     return out
 
 
-def _compare(ctx, sc, cam, spp, depth, seed, log, label, repeats=2):
+def _compare(ctx, sc, cam, spp, depth, seed, log, label, repeats=2, nee=False):
     res = {}
     films = {}
     for name, mode in (("megakernel", W.WR_MEGAKERNEL), ("wavefront", W.WR_WAVEFRONT)):
         best = None
         for _ in range(repeats):
             film = W.Film(ctx, cam["width"], cam["height"])
-            st = W.render(sc, cam, film, spp, depth, seed, mode)
+            st = W.render(sc, cam, film, spp, depth, seed, mode, nee=nee)
             best = st["kernel_ms"] if best is None else min(best, st["kernel_ms"])
             films[name] = film.read()
         samples = cam["width"] * cam["height"] * spp
@@ -82,6 +82,7 @@ def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
         ref = np.load(C1_REFERENCE)
         r["rel_rmse_vs_reference"] = round(rel_rmse(film["mean"], ref["mean"]), 5)
         r["reference"] = f"{int(ref['spp'])}-spp render, FP64 accumulation of per-batch Kahan films"
+    r["next_event"], _ = _compare(ctx, sc, cam, 16, 8, 1234, log, "C1 cornell +NEE", nee=True)
     out["c1"] = r
     objs, cam = S.random_field()
     sc = W.Scene(ctx, objs)
@@ -98,6 +99,7 @@ def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
         r["rel_rmse_vs_reference"] = round(rel_rmse(img, ref["mean"]), 5)
         r["reference"] = (f"{int(ref['spp'])}-spp render, FP64 accumulation of per-batch Kahan films, "
                           f"compared on every {st}th pixel in x and y ({img.shape[1]} x {img.shape[0]})")
+    r["next_event"], _ = _compare(ctx, sc, cam, c3_spp, 16, 77, log, "C3 field +NEE", repeats=1, nee=True)
     out["c3"] = r
     return out
 
