@@ -230,6 +230,60 @@ int wr_film_write(wr_film* film, const float* sum, const float* comp, const floa
 int wr_render(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, const wr_render_params* params,
               wr_film* film, wr_render_stats* stats);
 
+/* ---- light sub-paths and the photon hash grid ------------------------------ */
+/* One Ray of a Path (ray.h:8-37) as the light-path estimators read it. */
+typedef struct {
+    float origin[3];
+    float direction[3];
+    float radiance[3];
+    float origin_prob;
+    float direction_prob;
+    uint32_t contact_obj; /* contactObj, WR_NONE = NULL */
+    uint32_t contact_tri; /* contactTriangleID (the object's own triangle index), 0 without a contact */
+    uint32_t inside_obj;  /* insideObj, WR_NONE = NULL */
+    uint32_t flags;       /* WR_VERTEX_DELTA: isDeltaDirection */
+    uint32_t pad;
+} wr_path_vertex;
+enum { WR_VERTEX_DELTA = 1 };
+
+/* The light sub-paths of BDPT, VCM and PPM: `Ray lightRay = genLightSample();
+ * samplePath(lightPath, lightRay)` (bidirPathTracer.cpp:31-33,
+ * vcmTracer.cpp:37-41, photonMap.cpp:61-65), for paths path0 .. path0 +
+ * n_paths - 1 of one pass.  The light ray is AbstractObject::genSurfaceSample
+ * (abstractObj.cpp:18-52); randomness is the counter RNG keyed by (seed, path,
+ * pass, vertex) with vertices 0x80000000 (light pick and point), 0x80000001
+ * (direction) and 0x80000002 + d (the scatter at depth d).  d_vertices holds
+ * max_vertices records per path (max_vertices >= max_depth + 3, the longest
+ * possible path); d_counts each path's vertex count.  Device pointers. */
+int wr_light_paths(wr_ctx* ctx, const wr_scene* scene, uint32_t max_depth, uint64_t seed, uint32_t pass,
+                   uint32_t path0, uint32_t n_paths, uint32_t max_vertices, wr_path_vertex* d_vertices,
+                   uint32_t* d_counts);
+/* The light points VCM / PPM keep from those paths (vcmTracer.cpp:42-55,
+ * photonMap.cpp:66-86): per path in order, vertices 1.. up to the first whose
+ * contact object is emissive, skipping isDeltaDirection ones; after n_leading
+ * points at the origin (the reference's vector starts with one
+ * value-initialised point per pixel, vcmTracer.cpp:31).  Writes at most cap
+ * positions (d_pos, 3 floats each) and sources (d_src: path, vertex;
+ * WR_NONE for the leading points); *n_out = the number there are.  Device
+ * pointers except n_out. */
+int wr_photons_collect(wr_ctx* ctx, const wr_scene* scene, const wr_path_vertex* d_vertices, const uint32_t* d_counts,
+                       uint32_t n_paths, uint32_t max_vertices, uint64_t n_leading, float* d_pos, uint32_t* d_src,
+                       uint64_t cap, uint64_t* n_out);
+/* HashGrid (hashgrid.hxx:15-203): Reserve(n_cells) + Build(points, radius)
+ * over n points (d_pos, device, read again by queries: keep it alive), and
+ * Process as a range query: for each query point (device, 3 floats), the
+ * points handed to the query in the reference's order (8 cells, a cell
+ * reached twice through a hash collision visited twice), the first cap of
+ * them in d_idx (cap per query), their number in d_counts. */
+typedef struct wr_hashgrid wr_hashgrid;
+int wr_hashgrid_build(wr_ctx* ctx, const float* d_pos, uint64_t n, float radius, uint32_t n_cells,
+                      wr_hashgrid** out);
+/* Host copies of the grid: bbox (min xyz, max xyz), mCellEnds (n_cells), mIndices (n). */
+int wr_hashgrid_export(const wr_hashgrid* grid, float* bbox, int32_t* cell_ends, int32_t* indices);
+int wr_hashgrid_query(wr_hashgrid* grid, const float* d_queries, uint64_t n_queries, uint32_t cap, uint32_t* d_counts,
+                      uint32_t* d_idx);
+void wr_hashgrid_destroy(wr_hashgrid* grid);
+
 /* ---- scene files ---------------------------------------------------------- */
 /* Config::load (config.cpp:432-575): reads a reference scene configuration
  * (Config -> Scene -> GroupObject / Object / Camera, materials by path and

@@ -460,6 +460,8 @@ static inline void rng4(uint64_t seed, uint32_t pixel, uint32_t pass, uint32_t v
 /* Vertex counter of the light sample's uniforms (next-event estimation); path
  * vertices use 0 .. maxPathLen + 1. */
 static const uint32_t kNeeVertex = 0xFFFFFFFFu;
+/* First vertex counter of light sub-paths (counters kLightVertex + 0, 1, 2 + d). */
+static const uint32_t kLightVertex = 0x80000000u;
 
 /* ---- Camera (camera.cpp) -------------------------------------------------- */
 struct Cam {
@@ -497,6 +499,7 @@ struct Out {
     V o, d, rad, n;  /* n: contact normal, == contactObj->getWorldNormal(contactTri, o) */
     float op = 1.f, dp = 1.f;
     unsigned inside = ORC_NONE, contact = ORC_NONE, ctri = 0;
+    bool delta = false;  /* isDeltaDirection: set by the refractive scatter (refractiveObj.h:85) */
 };
 
 /* Ray::cosineTerm (ray.cpp:52-58) for a scatter vertex (contactObj set, hasCosine true). */
@@ -648,6 +651,7 @@ struct Tracer {
                 }
             }
             out.d = normalize(out.d);
+            out.delta = true;
             break;
         }
         default: { /* ORC_EMISSIVE, emissiveObj.h:15-28 */
@@ -735,6 +739,7 @@ struct Tracer {
         float op = 1.f, dp = 1.f;
         unsigned inside = ORC_NONE, contact = ORC_NONE, ctri = 0;
         bool camera = false;    /* contactObj is the Camera (camera.cpp:54) */
+        bool delta = false;     /* isDeltaDirection */
         Hit hit;                /* intersectObj / intersectDist / intersectTriangleID */
     };
 
@@ -745,19 +750,22 @@ struct Tracer {
         return c > EPS_D ? c : (float)EPS_D;
     }
 
-    void samplePath(std::vector<PRay>& path, const PRay& prev, unsigned depth, unsigned p, unsigned pass) const {
+    /* vbase: the vertex counter of the scatter at depth d is vbase + d + 1. */
+    void samplePath(std::vector<PRay>& path, const PRay& prev, unsigned depth, unsigned p, unsigned pass,
+                    uint32_t vbase = 0) const {
         path.push_back(prev);
         PRay term;                                  /* terminateRay, mcRenderer.cpp:20-27 */
         term.o = prev.o;
         term.rad = V(0, 0, 0);
         term.d = V(0, 0, 0);
+        term.delta = true;
         unsigned self;
         if (prev.inside != ORC_NONE) self = prev.inside;
         else if (prev.hit.valid) self = prev.hit.obj;
         else { path.push_back(term); return; }
         if (!prev.hit.valid) { path.push_back(term); return; }  /* inside but no hit: as in sample() */
         float rn[4];
-        rng4(seed, p, pass, depth + 1, rn);
+        rng4(seed, p, pass, vbase + depth + 1, rn);
         Ray in;
         in.o = prev.o;
         in.d = prev.d;
@@ -766,6 +774,7 @@ struct Tracer {
         next.o = out.o; next.d = out.d; next.rad = out.rad; next.n = out.n;
         next.op = out.op; next.dp = out.dp;
         next.inside = out.inside; next.contact = out.contact; next.ctri = out.ctri;
+        next.delta = out.delta;
         if (length(next.d) < 0.5) { path.push_back(next); return; }
         if (depth + 1 > maxDepth) { path.push_back(term); return; }
         Ray nr;
@@ -777,7 +786,7 @@ struct Tracer {
             path.push_back(term);
             return;
         }
-        samplePath(path, next, depth + 1, p, pass);
+        samplePath(path, next, depth + 1, p, pass, vbase);
     }
 
     /* Scene::genLightSample -> SurfaceSampler::genSample (surfaceSampler.cpp:17-23)
@@ -835,6 +844,52 @@ struct Tracer {
         unsigned g = 0;
         const float dist = s->intersect(r, g, true, in.inside, in.contact);
         return dist < 0 || (dist - length(sub(in.o, lightOrigin)) >= -EPS_D);
+    }
+
+    /* Light sub-path of BDPT / VCM / PPM (bidirPathTracer.cpp:31-33,
+     * vcmTracer.cpp:37-41, photonMap.cpp:61-65): Ray lightRay = genLightSample();
+     * samplePath(lightPath, lightRay).  The light ray is the full
+     * genSurfaceSample (abstractObj.cpp:18-52): the light point as in
+     * genLightSample, a uniform-sphere direction (uniformSphereSampler.h:6-20)
+     * flipped into the normal's hemisphere, directionProb = 2 * 1/(4 pi),
+     * findInsideObject, and a NoSelf intersect whose hit counts only at
+     * dist > 0.  Uniforms: vertex kLightVertex (object pick, triangle pick,
+     * u1, u2), kLightVertex + 1 (the sphere), kLightVertex + 2 + d (the scatter
+     * at depth d).  With nothing to sample the reference's empty-mesh ray
+     * (direction and radiance 0, no contact) starts the path. */
+    std::vector<PRay> lightPath(unsigned path, unsigned pass) const {
+        float rn[4];
+        rng4(seed, path, pass, kLightVertex, rn);
+        const LightSample L = genLightSample(rn);
+        PRay r;
+        if (!L.valid) {
+            r.d = V(0, 0, 0);
+            r.rad = V(0, 0, 0);
+        } else {
+            r.o = L.o;
+            r.n = L.n;
+            r.contact = L.obj;
+            r.ctri = L.tri;
+            rng4(seed, path, pass, kLightVertex + 1, rn);
+            const float term1 = (float)(2.0 * PI_D * (double)rn[0]);
+            const float term2 = 2.f * std::sqrt(rn[1] - rn[1] * rn[1]);
+            V dir(std::cos(term1) * term2, 1.f - 2.f * rn[1], std::sin(term1) * term2);
+            r.dp = (float)(INV_PI_D * 0.25f);
+            if (dot(dir, L.n) < 0) dir = neg(dir);
+            r.d = dir;
+            r.rad = L.rad;                                /* ray.evalBSDF(ray): the emission */
+            Ray q;
+            q.o = r.o;
+            q.d = r.d;
+            r.inside = s->findInside(q, r.contact);
+            r.dp *= 2;
+            r.op = L.op;
+            const Hit h = trace(q, true, r.inside, r.contact);
+            if (h.valid && h.dist > 0) r.hit = h;
+        }
+        std::vector<PRay> out;
+        samplePath(out, r, 0, path, pass, kLightVertex + 1);
+        return out;
     }
 
     /* One pixel sample of PathTracer::renderPixels (pathTracer.cpp:22-69).
@@ -913,6 +968,76 @@ struct Tracer {
         }
         if (!legal(color)) color = V(0, 0, 0);
         return cam.fixVignetting(color, p);
+    }
+};
+
+/* ---- HashGrid (hashgrid.hxx:15-203) ---------------------------------------- */
+struct HashGrid {
+    V bmin, bmax;
+    std::vector<int> cellEnds, indices;
+    float radius = 0, cellSize = 0, invCellSize = 0;
+
+    int cellIndex(int cx, int cy, int cz) const { /* hashgrid.hxx:170-178 */
+        const uint32_t x = (uint32_t)cx, y = (uint32_t)cy, z = (uint32_t)cz;
+        return (int)(((x * 73856093u) ^ (y * 19349663u) ^ (z * 83492791u)) % (uint32_t)cellEnds.size());
+    }
+    int cellIndex(V p) const { /* hashgrid.hxx:180-190 */
+        const V dm = sub(p, bmin);
+        return cellIndex((int)std::floor(invCellSize * dm.x), (int)std::floor(invCellSize * dm.y),
+                         (int)std::floor(invCellSize * dm.z));
+    }
+    /* Reserve(nCells) + Build(particles, radius) with aMaxDist = 0 (hashgrid.hxx:18-75) */
+    void build(const std::vector<V>& pts, float r, size_t nCells) {
+        cellEnds.assign(nCells, 0);
+        float maxDist = 0.f;
+        if (std::fabs(maxDist) < EPS_D) maxDist = r;
+        radius = r;
+        cellSize = maxDist * 2.f;
+        invCellSize = 1.f / cellSize;
+        bmin = V(1e36f, 1e36f, 1e36f);
+        bmax = V(-1e36f, -1e36f, -1e36f);
+        for (const V& p : pts)
+            for (int j = 0; j < 3; j++) {
+                bmax[j] = std::max(bmax[j], p[j]);
+                bmin[j] = std::min(bmin[j], p[j]);
+            }
+        indices.assign(pts.size(), 0);
+        for (const V& p : pts) cellEnds[cellIndex(p)]++;
+        int sum = 0;
+        for (size_t i = 0; i < cellEnds.size(); i++) {
+            const int t = cellEnds[i];
+            cellEnds[i] = sum;
+            sum += t;
+        }
+        for (size_t i = 0; i < pts.size(); i++) indices[cellEnds[cellIndex(pts[i])]++] = (int)i;
+    }
+    /* Process(particles, query) (hashgrid.hxx:77-158): the particles it hands to
+     * the query, in order (a cell reached twice through a hash collision is
+     * visited twice, as there). */
+    void query(const std::vector<V>& pts, V q, std::vector<int>& out) const {
+        out.clear();
+        const V dmin = sub(q, bmin), dmax = sub(bmax, q);
+        for (int i = 0; i < 3; i++)
+            if (dmin[i] < 0.f || dmax[i] < 0.f) return;
+        const V cp = V(invCellSize * dmin.x, invCellSize * dmin.y, invCellSize * dmin.z);
+        const V cf = V(std::floor(cp.x), std::floor(cp.y), std::floor(cp.z));
+        const int px = (int)cf.x, py = (int)cf.y, pz = (int)cf.z;
+        const V fr = sub(cp, cf);
+        const int pxo = px + (fr.x < 0.5f ? -1 : +1);
+        const int pyo = py + (fr.y < 0.5f ? -1 : +1);
+        const int pzo = pz + (fr.z < 0.5f ? -1 : +1);
+        const int cells[8][3] = {{px, py, pz},   {px, py, pzo},   {px, pyo, pz},   {px, pyo, pzo},
+                                 {pxo, py, pz},  {pxo, py, pzo},  {pxo, pyo, pz},  {pxo, pyo, pzo}};
+        for (int j = 0; j < 8; j++) {
+            const int c = cellIndex(cells[j][0], cells[j][1], cells[j][2]);
+            int a = c == 0 ? 0 : cellEnds[c - 1], b = cellEnds[c];
+            for (; a < b; a++) {
+                const int pi = indices[a];
+                const float dist = length(sub(q, pts[pi]));
+                const float d2 = dist * dist;
+                if (d2 <= radius * radius) out.push_back(pi);
+            }
+        }
     }
 };
 
@@ -1154,6 +1279,74 @@ int orc_light_sample(void* sp, const float rn[4], float* origin, float* normal, 
     *tri = L.tri;
     return 1;
 }
+
+uint32_t orc_light_paths(void* sp, uint32_t max_depth, uint64_t seed, uint32_t pass, uint32_t path0,
+                         uint32_t n_paths, uint32_t max_vertices, float* rec, uint32_t* counts, int nthreads) {
+    Scene* s = (Scene*)sp;
+    orc_camera c{};
+    c.width = c.height = 1;
+    Tracer tr{s, Cam(&c), max_depth, seed};
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    uint32_t longest = 0;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads) reduction(max : longest)
+    for (int64_t k = 0; k < (int64_t)n_paths; k++) {
+        const std::vector<Tracer::PRay> P = tr.lightPath(path0 + (uint32_t)k, pass);
+        const uint32_t n = (uint32_t)P.size();
+        longest = std::max(longest, n);
+        counts[k] = n;
+        for (uint32_t i = 0; i < std::min(n, max_vertices); i++) {
+            float* o = rec + 16 * ((size_t)k * max_vertices + i);
+            const Tracer::PRay& r = P[i];
+            for (int a = 0; a < 3; a++) {
+                o[a] = r.o[a];
+                o[3 + a] = r.d[a];
+                o[6 + a] = r.rad[a];
+            }
+            o[9] = r.op;
+            o[10] = r.dp;
+            uint32_t* u = (uint32_t*)o;
+            u[11] = r.contact;
+            u[12] = r.contact == ORC_NONE ? 0u : r.ctri;
+            u[13] = r.inside;
+            u[14] = r.delta ? 1u : 0u;
+            u[15] = 0u;
+        }
+    }
+    return longest;
+}
+
+void* orc_hashgrid_build(const float* pos, uint64_t n, float radius, uint32_t n_cells) {
+    HashGrid* g = new HashGrid;
+    std::vector<V> pts(n);
+    for (uint64_t i = 0; i < n; i++) pts[i] = V(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    g->build(pts, radius, n_cells);
+    return g;
+}
+
+void orc_hashgrid_export(void* gp, float* bbox, int32_t* cell_ends, int32_t* indices) {
+    HashGrid* g = (HashGrid*)gp;
+    for (int a = 0; a < 3; a++) {
+        bbox[a] = g->bmin[a];
+        bbox[3 + a] = g->bmax[a];
+    }
+    std::copy(g->cellEnds.begin(), g->cellEnds.end(), cell_ends);
+    std::copy(g->indices.begin(), g->indices.end(), indices);
+}
+
+void orc_hashgrid_query(void* gp, const float* pos, uint64_t n, const float* q, uint64_t nq, uint32_t cap,
+                        uint32_t* counts, int32_t* idx) {
+    HashGrid* g = (HashGrid*)gp;
+    std::vector<V> pts(n);
+    for (uint64_t i = 0; i < n; i++) pts[i] = V(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    std::vector<int> out;
+    for (uint64_t k = 0; k < nq; k++) {
+        g->query(pts, V(q[3 * k], q[3 * k + 1], q[3 * k + 2]), out);
+        counts[k] = (uint32_t)out.size();
+        for (uint32_t j = 0; j < cap && j < out.size(); j++) idx[(size_t)k * cap + j] = out[j];
+    }
+}
+
+void orc_hashgrid_free(void* gp) { delete (HashGrid*)gp; }
 
 void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox(ctr, key, out); }
 

@@ -95,6 +95,24 @@ void     orc_render_sample2(void* s, const orc_camera* cam, uint32_t max_depth, 
 int      orc_light_sample(void* s, const float rn[4], float* origin, float* normal, float* radiance,
                           float* origin_prob, uint32_t* obj, uint32_t* tri);
 
+/* Light sub-paths (genLightSample + samplePath, bidirPathTracer.cpp:31-33):
+ * paths path0 .. path0 + n_paths - 1 of one pass.  rec: 16 words per vertex,
+ * max_vertices per path: origin, direction, radiance (3 floats each),
+ * originProb, directionProb, contact object, contact triangle (0 without a
+ * contact), inside object, isDeltaDirection, 0.  counts: each path's vertex
+ * count (at most max_depth + 3; vertices past max_vertices are not stored).
+ * Returns the longest path's vertex count. */
+uint32_t orc_light_paths(void* s, uint32_t max_depth, uint64_t seed, uint32_t pass, uint32_t path0,
+                         uint32_t n_paths, uint32_t max_vertices, float* rec, uint32_t* counts, int nthreads);
+/* HashGrid::Reserve(n_cells) + Build(points, radius) (hashgrid.hxx:18-75) and
+ * Process (hashgrid.hxx:77-158): per query, the points handed to the query in
+ * order (first cap of them stored). */
+void*    orc_hashgrid_build(const float* pos, uint64_t n, float radius, uint32_t n_cells);
+void     orc_hashgrid_export(void* g, float* bbox, int32_t* cell_ends, int32_t* indices);
+void     orc_hashgrid_query(void* g, const float* pos, uint64_t n, const float* q, uint64_t nq, uint32_t cap,
+                            uint32_t* counts, int32_t* idx);
+void     orc_hashgrid_free(void* g);
+
 /* ---- Counter RNG (Philox-4x32-10, Salmon et al., SC'11) ---------------- */
 void     orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* 4 uniform floats in [0,1) for (seed, pixel, pass, vertex). */

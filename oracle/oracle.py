@@ -55,6 +55,13 @@ def lib():
         L.orc_render_sample2.argtypes = [vp, vp, u32, u64, u32, u32, u32, vp, vp]
         L.orc_light_sample.restype = C.c_int
         L.orc_light_sample.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_light_paths.restype = u32
+        L.orc_light_paths.argtypes = [vp, u32, u64, u32, u32, u32, u32, vp, vp, C.c_int]
+        L.orc_hashgrid_build.restype = vp
+        L.orc_hashgrid_build.argtypes = [vp, u64, f32, u32]
+        L.orc_hashgrid_export.argtypes = [vp, vp, vp, vp]
+        L.orc_hashgrid_query.argtypes = [vp, vp, u64, vp, u64, u32, vp, vp]
+        L.orc_hashgrid_free.argtypes = [vp]
         L.orc_philox4x32.argtypes = [vp, vp, vp]
         L.orc_rng4.argtypes = [u64, u32, u32, u32, vp]
         _lib = L
@@ -203,6 +210,15 @@ class Scene:
             lib().orc_render_sample(self.h, C.byref(c), max_depth, seed, pixel, pss, _p(rgb), C.byref(nv))
         return rgb, nv.value
 
+    def light_paths(self, max_depth, seed, pss, path0, n_paths, max_vertices, nthreads=0):
+        """Light sub-paths (genLightSample + samplePath): (records, counts, longest).
+        records: (n_paths, max_vertices) of PATH_VERTEX_DTYPE."""
+        rec = np.zeros((n_paths, max_vertices), PATH_VERTEX_DTYPE)
+        counts = np.zeros(n_paths, np.uint32)
+        longest = lib().orc_light_paths(self.h, max_depth, seed, pss, path0, n_paths, max_vertices, _p(rec),
+                                        _p(counts), nthreads)
+        return rec, counts, longest
+
     def light_sample(self, u):
         """Scene::genLightSample from four uniforms; None when nothing can be sampled."""
         u = np.ascontiguousarray(u, dtype=np.float32)
@@ -213,6 +229,64 @@ class Scene:
             return None
         return {"origin": o, "normal": n, "radiance": rad, "origin_prob": op.value, "obj": obj.value,
                 "tri": tri.value}
+
+
+PATH_VERTEX_DTYPE = np.dtype([("origin", np.float32, 3), ("direction", np.float32, 3),
+                              ("radiance", np.float32, 3), ("origin_prob", np.float32),
+                              ("direction_prob", np.float32), ("contact", np.uint32), ("contact_tri", np.uint32),
+                              ("inside", np.uint32), ("delta", np.uint32), ("pad", np.uint32)])
+EMISSIVE = 3
+
+
+def photons(scene_mats, rec, counts, n_leading=0):
+    """The light points VCM / PPM store (vcmTracer.cpp:42-55, photonMap.cpp:66-86): for
+    each path in order, vertices 1.. until one whose contact object is emissive,
+    skipping isDeltaDirection ones.  The reference's vector starts with one
+    value-initialised point per pixel (vcmTracer.cpp:31), at the origin and
+    without a path: n_leading of them come first.  Returns (positions (n, 3),
+    source (n, 2) = (path, vertex), NONE for the leading points)."""
+    pos = [np.zeros((n_leading, 3), np.float32)]
+    src = [np.full((n_leading, 2), NONE, np.uint32)]
+    mats = np.asarray(scene_mats)
+    for k in range(len(counts)):
+        for i in range(1, int(counts[k])):
+            v = rec[k, i]
+            c = int(v["contact"])
+            if c != NONE and mats[c] == EMISSIVE:
+                break
+            if v["delta"]:
+                continue
+            pos.append(v["origin"][None, :].astype(np.float32))
+            src.append(np.array([[k, i]], np.uint32))
+    return np.concatenate(pos), np.concatenate(src)
+
+
+class HashGrid:
+    """HashGrid (hashgrid.hxx): Reserve(n_cells) + Build(points, radius); query = Process."""
+
+    def __init__(self, pos, radius, n_cells):
+        self.pos = np.ascontiguousarray(pos, dtype=np.float32)
+        self.n, self.n_cells = len(self.pos), int(n_cells)
+        self.h = lib().orc_hashgrid_build(_p(self.pos), self.n, radius, self.n_cells)
+
+    def export(self):
+        bbox = np.zeros(6, np.float32)
+        ends = np.zeros(self.n_cells, np.int32)
+        idx = np.zeros(max(self.n, 1), np.int32)
+        lib().orc_hashgrid_export(self.h, _p(bbox), _p(ends), _p(idx))
+        return bbox, ends, idx[:self.n]
+
+    def query(self, q, cap):
+        q = np.ascontiguousarray(q, dtype=np.float32)
+        counts = np.zeros(len(q), np.uint32)
+        idx = np.full((len(q), cap), -1, np.int32)
+        lib().orc_hashgrid_query(self.h, _p(self.pos), self.n, _p(q), len(q), cap, _p(counts), _p(idx))
+        return counts, idx
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_hashgrid_free(self.h)
+            self.h = None
 
 
 NEE = 1          # orc_render2 flags

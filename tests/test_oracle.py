@@ -548,3 +548,116 @@ def test_nee_changes_only_connected_paths():
             elif np.any(b != 0):
                 changed += 1
     assert changed > 20
+
+
+# ------------------------------------------------ light sub-paths, hash grid ----
+def test_light_paths_first_vertex_and_structure():
+    """genSurfaceSample as the light ray: a point on an emitter, a direction uniform
+    over the normal's hemisphere (E[cos] = 1/2, E[cos^2] = 1/3), directionProb =
+    2/(4 pi), originProb = 1/A_total, radiance = emission; every later vertex starts
+    where the previous ray's closest hit is; terminal entries are delta; refractive
+    scatters are delta, diffuse/glossy ones are not."""
+    objs, cam = S.cornell(k=2, width=8, height=8)
+    sc = O.Scene(objs)
+    n = 20000
+    rec, counts, longest = sc.light_paths(8, 5, 0, 0, n, 12)
+    assert longest <= 12 and np.all(counts >= 2)
+    v0 = rec[:, 0]
+    light = 3  # the emissive quad at y = 1 - 1e-3, 0.25 wide, facing -y
+    assert np.all(v0["contact"] == light)
+    assert np.allclose(v0["origin"][:, 1], 1 - 1e-3) and np.all(np.abs(v0["origin"][:, 0] - 0.5) <= 0.125 + 1e-6)
+    cos = -v0["direction"][:, 1]
+    assert np.all(cos >= 0)
+    assert abs(cos.mean() - 0.5) < 0.01 and abs((cos ** 2).mean() - 1 / 3) < 0.01
+    assert np.all(v0["direction_prob"] == np.float32(np.float32(1 / (4 * np.pi)) * 2))
+    assert np.allclose(v0["origin_prob"], 1 / 0.0625, rtol=1e-6)
+    assert np.all(v0["radiance"] == np.float32(17.0))
+    sc_rays = []
+    for k in range(0, n, 97):
+        for i in range(int(counts[k]) - 1):
+            a, b = rec[k, i], rec[k, i + 1]
+            if np.linalg.norm(b["direction"]) < 0.5 and b["delta"]:  # terminateRay
+                assert i + 1 == counts[k] - 1
+                continue
+            sc_rays.append((a, b))
+    assert len(sc_rays) > 300
+    mats = [o["mat"] for o in objs]
+    for a, b in sc_rays:
+        # b starts on a's ray, at a's NoSelf closest hit
+        ray = np.concatenate([a["origin"], a["direction"]])[None, :].astype(np.float32)
+        h = sc.intersect(ray, inside=[a["inside"]], contact=[a["contact"]], noself=True)
+        assert h["prim"][0] != O.NONE
+        assert np.allclose(b["origin"], a["origin"] + h["t"][0] * a["direction"], atol=1e-5)
+        assert bool(b["delta"]) == (mats[int(b["contact"])] == S.REFRACTIVE)
+
+
+def test_light_paths_deterministic_and_sliced():
+    objs, _ = S.cornell(k=1, width=8, height=8)
+    sc = O.Scene(objs)
+    a, ca, _ = sc.light_paths(6, 9, 3, 0, 400, 10)
+    b, cb, _ = sc.light_paths(6, 9, 3, 100, 300, 10)
+    assert np.array_equal(ca[100:], cb) and np.array_equal(a[100:], b)
+    c, _, _ = sc.light_paths(6, 9, 4, 0, 400, 10)
+    assert not np.array_equal(a["origin"], c["origin"])
+
+
+def test_photons_selection_rules():
+    objs, _ = S.random_field(n_objects=24, width=16, height=9, sphere_k=1)
+    sc = O.Scene(objs)
+    rec, counts, _ = sc.light_paths(6, 3, 0, 0, 2000, 10)
+    mats = [o["mat"] for o in objs]
+    pos, src = O.photons(mats, rec, counts, n_leading=7)
+    assert np.all(pos[:7] == 0) and np.all(src[:7] == O.NONE)
+    for (k, i), p in zip(src[7:], pos[7:]):
+        v = rec[k, i]
+        assert i >= 1 and not v["delta"] and np.array_equal(v["origin"], p)
+        for j in range(1, i):  # nothing emissive before it on its path
+            c = int(rec[k, j]["contact"])
+            assert c == O.NONE or mats[c] != S.EMISSIVE
+    assert len(pos) > 7 + 500
+
+
+def _brute_neighbors(pos, q, r):
+    d = pos[None, :, :].astype(np.float32) - q[:, None, :].astype(np.float32)
+    dist = np.sqrt((d.astype(np.float32) ** 2).sum(-1, dtype=np.float32), dtype=np.float32)
+    return (dist * dist) <= np.float32(r) * np.float32(r)
+
+
+def test_hashgrid_structure_and_queries_vs_brute_force():
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(-2, 3, (3000, 3)).astype(np.float32)
+    r, ncells = 0.3, 1024
+    g = O.HashGrid(pos, r, ncells)
+    bbox, ends, idx = g.export()
+    assert np.array_equal(bbox[:3], pos.min(0)) and np.array_equal(bbox[3:], pos.max(0))
+    assert ends[-1] == len(pos) and np.all(np.diff(ends) >= 0)
+    assert np.array_equal(np.sort(idx), np.arange(len(pos)))
+    # within a cell the points keep their input order (the serial scatter)
+    starts = np.concatenate([[0], ends[:-1]])
+    for c in range(ncells):
+        seg = idx[starts[c]:ends[c]]
+        assert np.all(np.diff(seg) > 0)
+    q = rng.uniform(-2.5, 3.5, (500, 3)).astype(np.float32)
+    counts, got = g.query(q, 256)
+    want = _brute_neighbors(pos, q, r)
+    inside = np.all((q >= bbox[:3]) & (q <= bbox[3:]), axis=1)
+    for k in range(len(q)):
+        s = set(got[k, :counts[k]].tolist())
+        if not inside[k]:
+            assert counts[k] == 0
+        else:
+            assert s == set(np.nonzero(want[k])[0].tolist()), k
+    assert counts[inside].mean() > 2
+
+
+def test_hashgrid_lattice_closed_form():
+    """Points on a lattice of spacing 0.5, radius 0.5 (exact in float): an interior
+    lattice point sees itself and its 6 axis neighbours; a query outside the box
+    sees nothing; collisions only ever repeat a point."""
+    ax = np.arange(8, dtype=np.float32) * 0.5
+    pos = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    g = O.HashGrid(pos, 0.5, 97)
+    q = np.array([[1.5, 1.5, 1.5], [2.0, 1.0, 1.5], [-0.1, 1.0, 1.0], [3.6, 1.0, 1.0]], np.float32)
+    counts, got = g.query(q, 64)
+    assert len(set(got[0, :counts[0]].tolist())) == 7 and len(set(got[1, :counts[1]].tolist())) == 7
+    assert counts[2] == 0 and counts[3] == 0

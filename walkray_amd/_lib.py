@@ -172,6 +172,12 @@ def lib():
         "wr_shape_free": (None, [vp]),
         "wr_shape_get": (i32, [vp, P(u32), P(u32), P(u32), P(u32), vp, vp, vp, vp]),
         "wr_pfm_write": (i32, [C.c_char_p, vp, u32, u32]),
+        "wr_light_paths": (i32, [vp, vp, u32, u64, u32, u32, u32, u32, vp, vp]),
+        "wr_photons_collect": (i32, [vp, vp, vp, vp, u32, u32, u64, vp, vp, u64, P(u64)]),
+        "wr_hashgrid_build": (i32, [vp, vp, u64, C.c_float, u32, P(vp)]),
+        "wr_hashgrid_export": (i32, [vp, vp, vp, vp]),
+        "wr_hashgrid_query": (i32, [vp, vp, u64, u32, vp, vp]),
+        "wr_hashgrid_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -453,6 +459,60 @@ def render(scene, cam, film, spp, max_depth, seed, mode=WR_WAVEFRONT, wave_slots
 
 
 HIT_DTYPE = np.dtype([("prim", np.uint32), ("t", np.float32), ("b1", np.float32), ("b2", np.float32)])
+# wr_path_vertex: one Ray of a light path
+PATH_VERTEX_DTYPE = np.dtype([("origin", np.float32, 3), ("direction", np.float32, 3),
+                              ("radiance", np.float32, 3), ("origin_prob", np.float32),
+                              ("direction_prob", np.float32), ("contact", np.uint32), ("contact_tri", np.uint32),
+                              ("inside", np.uint32), ("delta", np.uint32), ("pad", np.uint32)])
+
+
+class HashGrid:
+    """wr_hashgrid: HashGrid::Build over device points (kept alive here) and Process as a range query."""
+
+    def __init__(self, ctx, d_pos, n, radius, n_cells):
+        self.ctx, self.n, self.n_cells = ctx, int(n), int(n_cells)
+        self._pos = d_pos
+        self.h = C.c_void_p()
+        _check(lib().wr_hashgrid_build(ctx.h, d_pos.ptr, self.n, radius, self.n_cells, C.byref(self.h)), ctx.h,
+               "wr_hashgrid_build")
+
+    @classmethod
+    def from_points(cls, ctx, pos, radius, n_cells):
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        return cls(ctx, ctx.to_device(pos), len(pos), radius, n_cells)
+
+    def export(self):
+        bbox = np.zeros(6, np.float32)
+        ends = np.zeros(self.n_cells, np.int32)
+        idx = np.zeros(max(self.n, 1), np.int32)
+        _check(lib().wr_hashgrid_export(self.h, _p(bbox), _p(ends), _p(idx)), self.ctx.h, "wr_hashgrid_export")
+        return bbox, ends, idx[:self.n]
+
+    def query_device(self, d_q, nq, cap, d_counts, d_idx):
+        _check(lib().wr_hashgrid_query(self.h, d_q.ptr, nq, cap, d_counts.ptr, d_idx.ptr if d_idx else None),
+               self.ctx.h, "wr_hashgrid_query")
+
+    def query(self, q, cap):
+        q = np.ascontiguousarray(q, dtype=np.float32).reshape(-1, 3)
+        nq = len(q)
+        dq = self.ctx.to_device(q)
+        dc = self.ctx.alloc(4 * max(nq, 1))
+        di = self.ctx.alloc(4 * max(nq * cap, 1))
+        self.query_device(dq, nq, cap, dc, di)
+        counts = dc.download(np.uint32, nq)
+        idx = di.download(np.int32, nq * cap).reshape(nq, cap)
+        return counts, idx
+
+    def close(self):
+        if self.h and self.h.value:
+            lib().wr_hashgrid_destroy(self.h)
+        self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Scene:
@@ -544,6 +604,33 @@ class Scene:
         if counters:
             return hits, dc.download(np.uint64, 3)
         return hits
+
+    def light_paths_device(self, max_depth, seed, pss, path0, n_paths, max_vertices):
+        """wr_light_paths into new device buffers: (vertices, counts)."""
+        dv = self.ctx.alloc(PATH_VERTEX_DTYPE.itemsize * max(n_paths * max_vertices, 1))
+        dc = self.ctx.alloc(4 * max(n_paths, 1))
+        _check(lib().wr_light_paths(self.ctx.h, self.h, max_depth, seed, pss, path0, n_paths, max_vertices, dv.ptr,
+                                    dc.ptr), self.ctx.h, "wr_light_paths")
+        return dv, dc
+
+    def light_paths(self, max_depth, seed, pss, path0, n_paths, max_vertices):
+        """Light sub-paths on the GPU: (records (n_paths, max_vertices) PATH_VERTEX_DTYPE, counts)."""
+        dv, dc = self.light_paths_device(max_depth, seed, pss, path0, n_paths, max_vertices)
+        rec = dv.download(PATH_VERTEX_DTYPE, n_paths * max_vertices).reshape(n_paths, max_vertices)
+        return rec, dc.download(np.uint32, n_paths)
+
+    def photons_device(self, dv, dc, n_paths, max_vertices, n_leading=0, cap=None):
+        """wr_photons_collect: (device positions, device sources, n).  cap None: sized by a first call."""
+        n = C.c_uint64()
+        if cap is None:
+            _check(lib().wr_photons_collect(self.ctx.h, self.h, dv.ptr, dc.ptr, n_paths, max_vertices, n_leading,
+                                            None, None, 0, C.byref(n)), self.ctx.h, "wr_photons_collect")
+            cap = n.value
+        dp = self.ctx.alloc(12 * max(cap, 1))
+        ds = self.ctx.alloc(8 * max(cap, 1))
+        _check(lib().wr_photons_collect(self.ctx.h, self.h, dv.ptr, dc.ptr, n_paths, max_vertices, n_leading, dp.ptr,
+                                        ds.ptr, cap, C.byref(n)), self.ctx.h, "wr_photons_collect")
+        return dp, ds, n.value
 
     def closest_hit_noself(self, rays, inside, contact):
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)

@@ -354,6 +354,7 @@ int wr_scene_build(wr_scene* s, int strategy) {
             g.mat = o.mat.type;
             g.has_normals = o.normals.empty() ? 0u : 1u;
             g.tree = (uint32_t)(oi + 1);
+            g.prim_base = s->prim_base[oi];
             objs.push_back(g);
             has_n.push_back(g.has_normals);
         }

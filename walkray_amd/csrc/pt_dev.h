@@ -132,6 +132,7 @@ struct Out {
     f3 o, d, rad, n;  // n: contactObj->getWorldNormal(contactTriangleID, o)
     float op, dp;
     uint32_t inside, contact;
+    bool delta;       // isDeltaDirection (the refractive scatter, refractiveObj.h:85)
 };
 
 // Ray::cosineTerm (ray.cpp:52-58) of a scatter vertex.
@@ -173,6 +174,7 @@ __device__ Out scatter(const SceneCtx& sc, uint32_t self, f3 in_o, f3 in_d, cons
     out.dp = 1.f;
     out.rad = mk(1.f, 1.f, 1.f);
     out.inside = in_inside;
+    out.delta = false;
     const f3 color = mk(ob.color[0], ob.color[1], ob.color[2]);
     if (ob.mat == WR_MAT_DIFFUSE) {  // diffuseObj.h:34-57, diffuseBsdf.h:17-26, cosHemiSphereSampler.h:6-18
         const float p = ob.pmax;
@@ -262,6 +264,7 @@ __device__ Out scatter(const SceneCtx& sc, uint32_t self, f3 in_o, f3 in_d, cons
             }
         }
         out.d = normalize(out.d);
+        out.delta = true;
     } else {  // emissive, emissiveObj.h:15-28
         out.d = mk(0.f, 0.f, 0.f);
         out.inside = WR_NONE;
@@ -293,6 +296,7 @@ __device__ __forceinline__ uint32_t lower_bound_f(const float* a, uint32_t n, fl
 struct LightSample {
     f3 o, n, rad;
     float op;
+    uint32_t obj, tri;  // the light object and its (local) triangle
     bool ok;
 };
 
@@ -329,6 +333,8 @@ __device__ __forceinline__ LightSample sample_light(const SceneCtx& sc, const fl
     const wr_object_gpu& ob = S.objs[lt.obj];
     L.rad = mk(ob.color[0], ob.color[1], ob.color[2]);  // EmissiveObject::evaluateBSDF
     L.op = lt.oprob;
+    L.obj = lt.obj;
+    L.tri = ti;
     L.ok = true;
     return L;
 }

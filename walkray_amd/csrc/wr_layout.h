@@ -99,7 +99,8 @@ struct wr_object_gpu {
     int32_t mat;
     uint32_t has_normals;
     uint32_t tree;        // index of the object's own tree header (findInsideObject)
-    uint32_t pad[3];
+    uint32_t prim_base;   // global triangle id of the object's first triangle
+    uint32_t pad[2];
 };
 
 // One emissive object of the light sampler (Scene::prepareForLightSampling,
