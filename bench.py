@@ -10,14 +10,17 @@ measured with HIP events on the stream the kernels run on.
 
 The same JSON line also reports, outside `value`, the path-tracing workloads
 (Cornell box 256^2 x 16 spp and the 512-object field at 1920x1080 x 64 spp,
-megakernel vs wavefront samples/s) and the served-jobs workload (p50/p99
-latency and aggregate samples/s under Poisson arrivals).
+megakernel vs wavefront samples/s, with and without next-event estimation),
+the light sub-paths and photon hash grid of BDPT / VCM / PPM on the field, and
+the served-jobs workload (p50/p99 latency and aggregate samples/s under
+Poisson arrivals).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pt] [--legs ch,pt,serve]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pt] [--legs ch,pt,lp,serve]
 
---legs picks the workloads (closest hit, path tracing, serving, and c5: the
-43.8M-triangle large scene at 3840x2160 x 256 spp, not in the default run); a
-run without the closest-hit leg reports value null (used to profile one leg).
+--legs picks the workloads (closest hit, path tracing, light paths + photon
+grid, serving, and c5: the 43.8M-triangle large scene at 3840x2160 x 256 spp,
+not in the default run); a run without the closest-hit leg reports value null
+(used to profile one leg).
 """
 from __future__ import annotations
 
@@ -190,7 +193,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pt", action="store_true", help="same as --legs ch")
-    ap.add_argument("--legs", default="ch,pt,serve")
+    ap.add_argument("--legs", default="ch,pt,lp,serve")
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (counter calibration)")
@@ -217,6 +220,9 @@ def main():
     if "pt" in legs:
         from walkray_amd import render as R
         extra["path_tracing"] = R.bench_path_tracing(W, S, ctx, log)
+    if "lp" in legs:
+        from walkray_amd import render as R
+        extra["light_paths"] = R.bench_light_paths(ctx, log)
     if "c5" in legs:  # the large scene: minutes of host build and rendering, not in the default run
         from walkray_amd import render as R
         extra["large_scene"] = R.bench_large(W, S, ctx, log)

@@ -362,6 +362,14 @@ class DeviceBuffer:
             pass
 
 
+class DeviceView:
+    """A byte offset into a DeviceBuffer (keeps the buffer alive)."""
+
+    def __init__(self, buf, offset):
+        self.buf = buf
+        self.ptr = C.c_void_p(buf.ptr.value + int(offset))
+
+
 class Context:
     """One HIP device + stream (wr_ctx)."""
 

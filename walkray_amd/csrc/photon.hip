@@ -246,41 +246,110 @@ __global__ __launch_bounds__(256) void k_grid_keys(GridDev g, const float* pos, 
     const uint32_t c = grid_hash(g, (int)floorf(g.inv * dx), (int)floorf(g.inv * dy), (int)floorf(g.inv * dz));
     keys[i] = c;
     vals[i] = i;
-    atomicAdd(cell_count + c, 1u);
+    // one atomic for a wave whose points share a cell (many coincident points,
+    // such as VCM's leading ones), else one per point
+    const uint64_t act = __ballot(1);
+    const uint32_t leader = __ffsll((unsigned long long)act) - 1;
+    const uint64_t same = __ballot(c == __shfl(c, leader));
+    if (same == act) {
+        if ((threadIdx.x & 63) == leader) atomicAdd(cell_count + c, (uint32_t)__popcll(act));
+    } else {
+        atomicAdd(cell_count + c, 1u);
+    }
+}
+
+// Ranges longer than this are scanned by the whole wave, 64 points a step, one
+// query at a time (a crowded cell, or one that shares a hash bucket with it).
+constexpr int kLongRange = 64;
+
+__device__ __forceinline__ bool grid_near(const GridDev& g, const float* pos, f3 qp, uint32_t pi) {
+    const f3 dv = mk(qp.x - pos[3 * pi], qp.y - pos[3 * pi + 1], qp.z - pos[3 * pi + 2]);
+    const float dist = len(dv);
+    return dist * dist <= g.r2;
 }
 
 __global__ __launch_bounds__(256) void k_grid_query(GridDev g, const float* pos, const int* ends, const uint32_t* idx,
                                                     const float* q, uint32_t nq, uint32_t cap, uint32_t* counts,
                                                     uint32_t* out) {
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= nq) return;
-    const f3 qp = mk(q[3 * k], q[3 * k + 1], q[3 * k + 2]);
-    const f3 dmin = mk(qp.x - g.bmin[0], qp.y - g.bmin[1], qp.z - g.bmin[2]);
-    const f3 dmax = mk(g.bmax[0] - qp.x, g.bmax[1] - qp.y, g.bmax[2] - qp.z);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    // every lane stays to the end: the long ranges are scanned by the whole wave
+    bool valid = k < nq;
+    f3 qp = mk(0.f, 0.f, 0.f);
+    int px = 0, py = 0, pz = 0, pxo = 0, pyo = 0, pzo = 0;
+    if (valid) {
+        qp = mk(q[3 * k], q[3 * k + 1], q[3 * k + 2]);
+        const f3 dmin = mk(qp.x - g.bmin[0], qp.y - g.bmin[1], qp.z - g.bmin[2]);
+        const f3 dmax = mk(g.bmax[0] - qp.x, g.bmax[1] - qp.y, g.bmax[2] - qp.z);
+        const bool inside =
+            !(dmin.x < 0.f || dmin.y < 0.f || dmin.z < 0.f || dmax.x < 0.f || dmax.y < 0.f || dmax.z < 0.f);
+        if (!inside) {
+            counts[k] = 0;
+            valid = false;
+        } else {
+            const f3 cp = mk(g.inv * dmin.x, g.inv * dmin.y, g.inv * dmin.z);
+            const f3 cf = mk(floorf(cp.x), floorf(cp.y), floorf(cp.z));
+            px = (int)cf.x;
+            py = (int)cf.y;
+            pz = (int)cf.z;
+            pxo = px + (cp.x - cf.x < 0.5f ? -1 : +1);
+            pyo = py + (cp.y - cf.y < 0.5f ? -1 : +1);
+            pzo = pz + (cp.z - cf.z < 0.5f ? -1 : +1);
+        }
+    }
     uint32_t m = 0;
-    if (!(dmin.x < 0.f || dmin.y < 0.f || dmin.z < 0.f || dmax.x < 0.f || dmax.y < 0.f || dmax.z < 0.f)) {
-        const f3 cp = mk(g.inv * dmin.x, g.inv * dmin.y, g.inv * dmin.z);
-        const f3 cf = mk(floorf(cp.x), floorf(cp.y), floorf(cp.z));
-        const int px = (int)cf.x, py = (int)cf.y, pz = (int)cf.z;
-        const int pxo = px + (cp.x - cf.x < 0.5f ? -1 : +1);
-        const int pyo = py + (cp.y - cf.y < 0.5f ? -1 : +1);
-        const int pzo = pz + (cp.z - cf.z < 0.5f ? -1 : +1);
-        for (int j = 0; j < 8; j++) {  // the reference's cell order
+    for (int j = 0; j < 8; j++) {  // the reference's cell order
+        int a = 0, b = 0;
+        if (valid) {
             const uint32_t c = grid_hash(g, (j & 4) ? pxo : px, (j & 2) ? pyo : py, (j & 1) ? pzo : pz);
-            int a = c == 0 ? 0 : ends[c - 1];
-            const int b = ends[c];
+            a = c == 0 ? 0 : ends[c - 1];
+            b = ends[c];
+        }
+        const bool long_range = b - a > kLongRange;
+        if (!long_range) {
             for (; a < b; a++) {
                 const uint32_t pi = idx[a];
-                const f3 dv = mk(qp.x - pos[3 * pi], qp.y - pos[3 * pi + 1], qp.z - pos[3 * pi + 2]);
-                const float dist = len(dv);
-                if (dist * dist <= g.r2) {
+                if (grid_near(g, pos, qp, pi)) {
                     if (m < cap) out[(size_t)k * cap + m] = pi;
                     m++;
                 }
             }
         }
+        // long ranges: the wave scans them one query at a time, in order, so the
+        // points keep the serial loop's order in the output
+        for (uint64_t pend = __ballot(long_range); pend != 0; pend &= pend - 1) {
+            const uint32_t L = __ffsll((unsigned long long)pend) - 1;
+            const f3 lq = mk(__shfl(qp.x, L), __shfl(qp.y, L), __shfl(qp.z, L));
+            const int la = __shfl(a, L), lb = __shfl(b, L);
+            const uint32_t lk = __shfl(k, L);
+            uint32_t lm = __shfl(m, L);
+            // four 64-point steps per iteration: their loads are in flight together
+            for (int base = la; base < lb; base += 4 * 64) {
+                uint32_t pi[4];
+                bool hit[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int ai = base + 64 * u + (int)lane;
+                    pi[u] = ai < lb ? idx[ai] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    hit[u] = base + 64 * u + (int)lane < lb && grid_near(g, pos, lq, pi[u]);
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const uint64_t hm = __ballot(hit[u]);
+                    if (hit[u]) {
+                        const uint32_t slot = lm + (uint32_t)__popcll(hm & below);
+                        if (slot < cap) out[(size_t)lk * cap + slot] = pi[u];
+                    }
+                    lm += (uint32_t)__popcll(hm);
+                }
+            }
+            if (lane == L) m = lm;
+        }
     }
-    counts[k] = m;
+    if (valid) counts[k] = m;
 }
 
 }  // namespace wrd

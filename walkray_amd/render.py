@@ -104,6 +104,57 @@ def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
     return out
 
 
+def bench_light_paths(ctx, log, depth=16, radius=0.01, repeats=3):
+    """The front end of BDPT / VCM / PPM on the 512-object field: one light sub-path per
+    1080p pixel (maxPathLen 16), the photons VCM keeps (after one leading point per
+    pixel, as its vector starts), the hash grid (one cell per pixel, VCM's base radius)
+    and one range query per photon.  Device times from HIP events, best of `repeats`."""
+    objs, cam = S.random_field()
+    sc = W.Scene(ctx, objs)
+    npix = cam["width"] * cam["height"]
+    maxv = depth + 3
+    best = {}
+
+    def timed(name, fn):
+        ctx.synchronize()
+        ctx.timer_start()
+        out = fn()
+        ms = ctx.timer_stop()
+        best[name] = min(best.get(name, ms), ms)
+        return out
+
+    for _ in range(repeats):
+        dv, dc = timed("light_paths", lambda: sc.light_paths_device(depth, 11, 0, 0, npix, maxv))
+        dp, ds, m = timed("photons", lambda: sc.photons_device(dv, dc, npix, maxv, n_leading=npix, cap=maxv * npix))
+        g = timed("grid_build", lambda: W.HashGrid(ctx, dp, m, radius, npix))
+        # one query at every stored light vertex (the leading points are not queried: VCM's
+        # queries are eye-path vertices)
+        nq = m - npix
+        dq = W.DeviceView(dp, 12 * npix)
+        dq_counts = ctx.alloc(4 * max(nq, 1))
+        timed("query", lambda: g.query_device(dq, nq, 0, dq_counts, None))
+        counts = dq_counts.download(np.uint32, nq)
+        g.close()
+        del dv, dc
+    r = {
+        "workload": (f"512-object field ({sc.info()['n_triangles']} triangles), {npix} light sub-paths "
+                     f"(one per 1920x1080 pixel), maxPathLen {depth}; photons after {npix} leading points; "
+                     f"hash grid of {npix} cells, radius {radius}; one range query per point"),
+        "light_paths_ms": round(best["light_paths"], 3),
+        "light_paths_per_s": round(npix / (best["light_paths"] * 1e-3), 1),
+        "photons_ms": round(best["photons"], 3),
+        "points": int(m),
+        "grid_build_ms": round(best["grid_build"], 3),
+        "queries": int(nq),
+        "query_ms": round(best["query"], 3),
+        "queries_per_s": round(nq / (best["query"] * 1e-3), 1),
+        "mean_neighbours": round(float(counts.mean()), 3),
+    }
+    log(f"[bench] light paths: {r['light_paths_ms']} ms ({r['light_paths_per_s'] / 1e6:.1f} M paths/s), photons "
+        f"{r['photons_ms']} ms ({m} points), grid build {r['grid_build_ms']} ms, {nq} queries {r['query_ms']} ms")
+    return r
+
+
 def bench_large(W_, S_, ctx, log, spp=256, depth=32):
     """Large scene: 43.8M triangles, 3840x2160, 256 spp, depth 32; build time, both engines."""
     t0 = time.time()
