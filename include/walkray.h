@@ -168,6 +168,23 @@ typedef struct {
 int wr_autotune_closest_hit(wr_ctx* ctx, wr_scene* scene, const wr_ray* d_rays, uint64_t n,
                             wr_closest_hit_tuning* best);
 int wr_scene_set_closest_hit_tuning(wr_scene* scene, const wr_closest_hit_tuning* tuning);
+/* Motion blur (AnimatedTransform, motionBlur.h:6-34, motionBlur.cpp:6-87; set
+ * by the loader at config.cpp:28-45): object obj's mesh, as added, is in its
+ * own space and moves from transform `start` at start_time to `end` at
+ * end_time (4x4, row-major: element (row, col) at [4 * row + col]).  Before
+ * every pass wr_render draws the object's time uniformly in [0, 1) (counter
+ * RNG: seed, pixel slot = obj, the pass, vertex 0xFFFFFFFE), places it with
+ * AnimatedTransform::Interpolate, and rebuilds the trees
+ * (Scene::updateSceneForMotionBlur, scene.cpp:67-80).  The built scene, and
+ * every other query on it, shows the object at start_time.  Call before
+ * wr_scene_build; not for served scenes. */
+int wr_scene_set_motion(wr_scene* scene, uint32_t obj, float start_time, float end_time, const float* start,
+                        const float* end);
+/* AnimatedTransform(start, start_time, end, end_time).Interpolate(time) as a
+ * host utility: out = the transform at `time`, row-major.  Host only. */
+int wr_motion_interpolate(float start_time, float end_time, const float* start, const float* end, float time,
+                          float* out);
+
 /* Scene::intersect with NoSelfCondition(scene, ray) (mcRenderer.cpp:42-44,
  * noself.cpp:13-20) for n path rays. */
 int wr_closest_hit_noself(wr_ctx* ctx, const wr_scene* scene, const wr_path_ray* d_rays, uint64_t n,

@@ -277,6 +277,205 @@ struct NoCond {
 };
 
 /* ---- Scene --------------------------------------------------------------- */
+/* ---- 4x4 matrices with nv::matrix4 semantics (nvMatrix.h), element(row, col) ---- */
+struct M4 {
+    float a[4][4];
+    static M4 ident() {
+        M4 m;
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) m.a[i][j] = i == j ? 1.f : 0.f;
+        return m;
+    }
+};
+static M4 m4mul(const M4& l, const M4& r) { /* nvMatrix.h:409-421: r(i,j) = 0; += l(i,c) r(c,j) */
+    M4 o;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            float x = 0.f;
+            for (int c = 0; c < 4; c++) x += l.a[i][c] * r.a[c][j];
+            o.a[i][j] = x;
+        }
+    return o;
+}
+static M4 m4transpose(const M4& m) {
+    M4 o;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) o.a[i][j] = m.a[j][i];
+    return o;
+}
+/* inverse (nvMatrix.h:242-378): Gauss-Jordan with scaled partial pivoting; a
+ * singular input returns the default-constructed (identity) matrix. */
+static M4 m4inverse(const M4& m) {
+    float r[4][8];
+    float* row[4] = {r[0], r[1], r[2], r[3]};
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            row[i][j] = m.a[i][j];
+            row[i][j + 4] = i == j ? 1.f : 0.f;
+        }
+    float scp[4];
+    for (int i = 0; i < 4; i++) {
+        scp[i] = std::fabs(row[i][0]);
+        for (int j = 1; j < 4; j++)
+            if (std::fabs(row[i][j]) > scp[i]) scp[i] = std::fabs(row[i][j]);
+        if (scp[i] == 0.f) return M4::ident();
+    }
+    for (int i = 0; i < 4; i++) {
+        int to = i;
+        float best = std::fabs(row[i][i] / scp[i]);
+        for (int p = i + 1; p < 4; p++)
+            if (std::fabs(row[p][i] / scp[p]) > best) {
+                best = std::fabs(row[p][i] / scp[p]);
+                to = p;
+            }
+        if (to != i) {
+            std::swap(row[i], row[to]);
+            std::swap(scp[i], scp[to]);
+        }
+        for (int j = i + 1; j < 4; j++) {
+            const float mji = row[j][i] / row[i][i];
+            row[j][i] = 0.f;
+            for (int jj = i + 1; jj < 8; jj++) row[j][jj] -= mji * row[i][jj];
+        }
+    }
+    if (row[3][3] == 0.f) return M4::ident();
+    for (int i = 3; i > 0; i--)
+        for (int j = i - 1; j > -1; j--) {
+            const float mij = row[j][i] / row[i][i];
+            for (int jj = j + 1; jj < 8; jj++) row[j][jj] -= mij * row[i][jj];
+        }
+    M4 o;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) o.a[i][j] = row[i][j + 4] / row[i][i];
+    return o;
+}
+/* vec3(M * vec4(v, w)) (nvMatrix.h:424-433) */
+static V m4apply(const M4& m, V v, float w) {
+    V o;
+    for (int i = 0; i < 3; i++) o[i] = v.x * m.a[i][0] + v.y * m.a[i][1] + v.z * m.a[i][2] + w * m.a[i][3];
+    return o;
+}
+
+/* ---- Quaternion (quaternion.h, quaternion.cpp) ---- */
+struct Q {
+    V v;
+    float w = 1.f;
+};
+static float qDot(const Q& a, const Q& b) { return dot(a.v, b.v) + a.w * b.w; }
+static Q qScale(const Q& q, float f) { return Q{mul(q.v, f), q.w * f}; }
+static Q qPlus(const Q& a, const Q& b) { return Q{add(a.v, b.v), a.w + b.w}; }
+static Q qMinus(const Q& a, const Q& b) { return Q{sub(a.v, b.v), a.w - b.w}; }
+static Q qDivide(const Q& q, float f) { return Q{divs(q.v, f), q.w / f}; }
+static Q qNormalize(const Q& q) { return qDivide(q, std::sqrt(qDot(q, q))); }
+static M4 qToTransform(const Q& q) { /* quaternion.cpp:6-31 */
+    const float xx = q.v.x * q.v.x, yy = q.v.y * q.v.y, zz = q.v.z * q.v.z;
+    const float xy = q.v.x * q.v.y, xz = q.v.x * q.v.z, yz = q.v.y * q.v.z;
+    const float wx = q.v.x * q.w, wy = q.v.y * q.w, wz = q.v.z * q.w;
+    M4 m = M4::ident();
+    m.a[0][0] = 1.f - 2.f * (yy + zz);
+    m.a[0][1] = 2.f * (xy + wz);
+    m.a[0][2] = 2.f * (xz - wy);
+    m.a[1][0] = 2.f * (xy - wz);
+    m.a[1][1] = 1.f - 2.f * (xx + zz);
+    m.a[1][2] = 2.f * (yz + wx);
+    m.a[2][0] = 2.f * (xz + wy);
+    m.a[2][1] = 2.f * (yz - wx);
+    m.a[2][2] = 1.f - 2.f * (xx + yy);
+    return m;
+}
+static Q qFromTransform(const M4& m) { /* quaternion.cpp:34-70 */
+    Q q;
+    const float trace = m.a[0][0] + m.a[1][1] + m.a[2][2];
+    if (trace > 0.f) {
+        float s = sqrtf(trace + 1.0);
+        q.w = s / 2.0f;
+        s = 0.5f / s;
+        q.v.x = (m.a[2][1] - m.a[1][2]) * s;
+        q.v.y = (m.a[0][2] - m.a[2][0]) * s;
+        q.v.z = (m.a[1][0] - m.a[0][1]) * s;
+    } else {
+        const int nxt[3] = {1, 2, 0};
+        float c[3];
+        int i = 0;
+        if (m.a[1][1] > m.a[0][0]) i = 1;
+        if (m.a[2][2] > m.a[i][i]) i = 2;
+        const int j = nxt[i], k = nxt[j];
+        float s = sqrtf((m.a[i][i] - (m.a[j][j] + m.a[k][k])) + 1.0);
+        c[i] = s * 0.5f;
+        if (s != 0.f) s = 0.5f / s;
+        q.w = (m.a[k][j] - m.a[j][k]) * s;
+        c[j] = (m.a[j][i] + m.a[i][j]) * s;
+        c[k] = (m.a[k][i] + m.a[i][k]) * s;
+        q.v = V(c[0], c[1], c[2]);
+    }
+    return q;
+}
+static Q qSlerp(float t, const Q& q1, const Q& q2) { /* quaternion.cpp:73-84 */
+    const float cosTheta = qDot(q1, q2);
+    if (cosTheta > .9995f) return qNormalize(qPlus(qScale(q1, 1.f - t), qScale(q2, t)));
+    const float theta = acosf(cosTheta < -1.f ? -1.f : cosTheta > 1.f ? 1.f : cosTheta);
+    const float thetap = theta * t;
+    const Q qperp = qNormalize(qMinus(q2, qScale(q1, cosTheta)));
+    return qPlus(qScale(q1, cosf(thetap)), qScale(qperp, sinf(thetap)));
+}
+
+/* ---- AnimatedTransform (motionBlur.h, motionBlur.cpp) ---- */
+struct Animated {
+    float t0 = 0, t1 = 1;
+    M4 start = M4::ident(), end = M4::ident();
+    V T[2];
+    Q R[2];
+    M4 S[2];
+    bool actually = false;
+    static void decompose(const M4& m, V& T, Q& Rq, M4& S) { /* motionBlur.cpp:48-87 */
+        T = V(m.a[0][3], m.a[1][3], m.a[2][3]);
+        M4 M = m;
+        for (int i = 0; i < 3; ++i) M.a[i][3] = M.a[3][i] = 0.f;
+        M.a[3][3] = 1.f;
+        float norm;
+        int count = 0;
+        M4 R = M;
+        do {
+            M4 Rnext;
+            const M4 Rit = m4inverse(m4transpose(R));
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) Rnext.a[i][j] = 0.5f * (R.a[i][j] + Rit.a[i][j]);
+            norm = 0.f;
+            for (int i = 0; i < 3; ++i) {
+                const float n = fabsf(R.a[i][0] - Rnext.a[i][0]) + fabsf(R.a[i][1] - Rnext.a[i][1]) +
+                                fabsf(R.a[i][2] - Rnext.a[i][2]);
+                norm = std::max(norm, n);
+            }
+            R = Rnext;
+        } while (++count < 100 && norm > .0001f);
+        Rq = qFromTransform(R);
+        S = m4mul(m4inverse(R), M);
+    }
+    Animated() = default;
+    Animated(const M4& a, float ta, const M4& b, float tb) : t0(ta), t1(tb), start(a), end(b) {
+        decompose(start, T[0], R[0], S[0]);
+        decompose(end, T[1], R[1], S[1]);
+        bool same = true;
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) same = same && start.a[i][j] == end.a[i][j];
+        actually = !same;
+    }
+    M4 interpolate(float time) const { /* motionBlur.cpp:6-45 */
+        if (!actually) return start;
+        if (time <= t0) return start;
+        if (time >= t1) return end;
+        const float dt = (time - t0) / (t1 - t0);
+        const V trans = add(mul(T[0], 1.f - dt), mul(T[1], dt));
+        const Q rotate = qSlerp(dt, R[0], R[1]);
+        M4 scale = M4::ident();
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) scale.a[i][j] = (1.f - dt) * S[0].a[i][j] + dt * S[1].a[i][j];
+        M4 I = M4::ident();
+        for (int i = 0; i < 3; i++) I.a[i][3] = trans[i];
+        return m4mul(m4mul(I, qToTransform(rotate)), scale);
+    }
+};
+
 struct Object {
     std::vector<V> verts;
     std::vector<unsigned> idx;
@@ -285,6 +484,18 @@ struct Object {
     V color;      /* albedo or emitted radiance */
     float power;  /* Phong exponent */
     float ior;
+    /* motion blur: the mesh in its own space and its AnimatedTransform */
+    bool moving = false;
+    Animated anim;
+    std::vector<V> localVerts, localNormals;
+    /* getWorldVertexPosition / getWorldNormal with transform M (mesh.h:47-77,106) */
+    void place(const M4& M) {
+        for (size_t i = 0; i < localVerts.size(); i++) verts[i] = m4apply(M, localVerts[i], 1.f);
+        if (!localNormals.empty()) {
+            const M4 nm = m4transpose(m4inverse(M));
+            for (size_t i = 0; i < localNormals.size(); i++) normals[i] = m4apply(nm, localNormals[i], 0.f);
+        }
+    }
 };
 
 struct Scene {
@@ -329,6 +540,12 @@ struct Scene {
     std::vector<std::vector<float>> triCum;   /* per light: AbstractObject::areaThreshold */
     std::vector<float> lightArea, pickProb;   /* per light: boundArea, pickProb */
     float lightTotal = 0;                     /* SurfaceSampler::boundArea */
+
+    /* Scene::updateSceneForMotionBlur (scene.cpp:67-80) before pass `pass`: every
+     * moving object at its time (uniform, counter RNG with the object in the
+     * pixel slot and vertex 0xFFFFFFFE), then every tree rebuilt.  Returns
+     * false (and changes nothing) when no object moves. */
+    bool updateForMotion(uint64_t seed, unsigned pass);
 
     /* Mesh::getTriangleArea (mesh.h:80-88) */
     float triangleArea(unsigned oi, unsigned ti) const {
@@ -455,6 +672,31 @@ static inline void rng4(uint64_t seed, uint32_t pixel, uint32_t pass, uint32_t v
     uint32_t o[4];
     philox(c, k, o);
     for (int i = 0; i < 4; i++) f[i] = (float)(o[i] >> 8) * (1.0f / 16777216.0f);
+}
+
+/* Back to every moving object's start time, as the scene was before a render. */
+static void restoreMotion(Scene* s) {
+    bool any = false;
+    for (Object& o : s->objs)
+        if (o.moving) {
+            o.place(o.anim.interpolate(o.anim.t0));
+            any = true;
+        }
+    if (any) s->build();
+}
+
+bool Scene::updateForMotion(uint64_t seed, unsigned pass) {
+    bool any = false;
+    for (unsigned oi = 0; oi < objs.size(); oi++) {
+        Object& o = objs[oi];
+        if (!o.moving) continue;
+        float u[4];
+        rng4(seed, oi, pass, 0xFFFFFFFEu, u);
+        o.place(o.anim.interpolate(u[0]));
+        any = true;
+    }
+    if (any) build();
+    return any;
 }
 
 /* Vertex counter of the light sample's uniforms (next-event estimation); path
@@ -1214,6 +1456,7 @@ void orc_render(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t seed
     const int64_t np = (int64_t)c->width * c->height;
     if (nthreads <= 0) nthreads = omp_get_max_threads();
     for (uint32_t pass = pass0; pass < pass0 + npass; pass++) {
+        s->updateForMotion(seed, pass);
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
         for (int64_t p = 0; p < np; p++) {
             V v = tr.sample((unsigned)p, pass, nullptr);
@@ -1223,6 +1466,7 @@ void orc_render(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t seed
             }
         }
     }
+    restoreMotion(s);
 }
 
 void orc_render_sample(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t seed,
@@ -1241,6 +1485,7 @@ void orc_render2(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t see
     const bool nee = (flags & ORC_NEE) != 0, pv = (flags & (ORC_NEE | ORC_PATH_VECTOR)) != 0;
     if (nthreads <= 0) nthreads = omp_get_max_threads();
     for (uint32_t pass = pass0; pass < pass0 + npass; pass++) {
+        s->updateForMotion(seed, pass);
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
         for (int64_t p = 0; p < np; p++) {
             V v = pv ? tr.samplePathVector((unsigned)p, pass, nee, nullptr) : tr.sample((unsigned)p, pass, nullptr);
@@ -1250,6 +1495,7 @@ void orc_render2(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t see
             }
         }
     }
+    restoreMotion(s);
 }
 
 void orc_render_sample2(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t seed, uint32_t flags,
@@ -1347,6 +1593,37 @@ void orc_hashgrid_query(void* gp, const float* pos, uint64_t n, const float* q, 
 }
 
 void orc_hashgrid_free(void* gp) { delete (HashGrid*)gp; }
+
+void orc_scene_set_motion(void* sp, uint32_t obj, float t0, float t1, const float* start, const float* end) {
+    Scene* s = (Scene*)sp;
+    Object& o = s->objs[obj];
+    M4 a, b;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            a.a[i][j] = start[4 * i + j];
+            b.a[i][j] = end[4 * i + j];
+        }
+    if (!o.moving) {
+        o.localVerts = o.verts;
+        o.localNormals = o.normals;
+        o.moving = true;
+    }
+    o.anim = Animated(a, t0, b, t1);
+    o.place(o.anim.interpolate(t0));
+    s->build();
+}
+
+void orc_motion_interpolate(float t0, float t1, const float* start, const float* end, float time, float* out) {
+    M4 a, b;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            a.a[i][j] = start[4 * i + j];
+            b.a[i][j] = end[4 * i + j];
+        }
+    const M4 t = Animated(a, t0, b, t1).interpolate(time);
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) out[4 * i + j] = t.a[i][j];
+}
 
 void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox(ctr, key, out); }
 

@@ -113,6 +113,14 @@ void     orc_hashgrid_query(void* g, const float* pos, uint64_t n, const float* 
                             uint32_t* counts, int32_t* idx);
 void     orc_hashgrid_free(void* g);
 
+/* Motion blur: object obj's mesh (as added) is in its own space and moves
+ * with AnimatedTransform(start, t0, end, t1) (motionBlur.cpp; matrices
+ * row-major, element (row, col) at [4 * row + col]); the scene shows it at t0
+ * until a render, which places every moving object at its time for each pass
+ * and rebuilds (Scene::updateSceneForMotionBlur, scene.cpp:67-80). */
+void     orc_scene_set_motion(void* s, uint32_t obj, float t0, float t1, const float* start, const float* end);
+void     orc_motion_interpolate(float t0, float t1, const float* start, const float* end, float time, float* out);
+
 /* ---- Counter RNG (Philox-4x32-10, Salmon et al., SC'11) ---------------- */
 void     orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* 4 uniform floats in [0,1) for (seed, pixel, pass, vertex). */

@@ -62,6 +62,8 @@ def lib():
         L.orc_hashgrid_export.argtypes = [vp, vp, vp, vp]
         L.orc_hashgrid_query.argtypes = [vp, vp, u64, vp, u64, u32, vp, vp]
         L.orc_hashgrid_free.argtypes = [vp]
+        L.orc_scene_set_motion.argtypes = [vp, u32, f32, f32, vp, vp]
+        L.orc_motion_interpolate.argtypes = [f32, f32, vp, vp, f32, vp]
         L.orc_philox4x32.argtypes = [vp, vp, vp]
         L.orc_rng4.argtypes = [u64, u32, u32, u32, vp]
         _lib = L
@@ -210,6 +212,12 @@ class Scene:
             lib().orc_render_sample(self.h, C.byref(c), max_depth, seed, pixel, pss, _p(rgb), C.byref(nv))
         return rgb, nv.value
 
+    def set_motion(self, obj, t0, t1, start, end):
+        """AnimatedTransform(start, t0, end, t1) for object obj (its mesh in its own space)."""
+        a = np.ascontiguousarray(start, dtype=np.float32).reshape(16)
+        b = np.ascontiguousarray(end, dtype=np.float32).reshape(16)
+        lib().orc_scene_set_motion(self.h, obj, t0, t1, _p(a), _p(b))
+
     def light_paths(self, max_depth, seed, pss, path0, n_paths, max_vertices, nthreads=0):
         """Light sub-paths (genLightSample + samplePath): (records, counts, longest).
         records: (n_paths, max_vertices) of PATH_VERTEX_DTYPE."""
@@ -287,6 +295,15 @@ class HashGrid:
         if getattr(self, "h", None) and _lib is not None:
             _lib.orc_hashgrid_free(self.h)
             self.h = None
+
+
+def motion_interpolate(t0, t1, start, end, time):
+    """AnimatedTransform::Interpolate (row-major 4x4 in and out)."""
+    a = np.ascontiguousarray(start, dtype=np.float32).reshape(16)
+    b = np.ascontiguousarray(end, dtype=np.float32).reshape(16)
+    out = np.zeros(16, np.float32)
+    lib().orc_motion_interpolate(t0, t1, _p(a), _p(b), time, _p(out))
+    return out.reshape(4, 4)
 
 
 NEE = 1          # orc_render2 flags

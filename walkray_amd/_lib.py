@@ -173,6 +173,8 @@ def lib():
         "wr_shape_get": (i32, [vp, P(u32), P(u32), P(u32), P(u32), vp, vp, vp, vp]),
         "wr_pfm_write": (i32, [C.c_char_p, vp, u32, u32]),
         "wr_light_paths": (i32, [vp, vp, u32, u64, u32, u32, u32, u32, vp, vp]),
+        "wr_scene_set_motion": (i32, [vp, u32, C.c_float, C.c_float, vp, vp]),
+        "wr_motion_interpolate": (i32, [C.c_float, C.c_float, vp, vp, C.c_float, vp]),
         "wr_photons_collect": (i32, [vp, vp, vp, vp, u32, u32, u64, vp, vp, u64, P(u64)]),
         "wr_hashgrid_build": (i32, [vp, vp, u64, C.c_float, u32, P(vp)]),
         "wr_hashgrid_export": (i32, [vp, vp, vp, vp]),
@@ -455,6 +457,15 @@ class Film:
             pass
 
 
+def motion_interpolate(t0, t1, start, end, time):
+    """wr_motion_interpolate: AnimatedTransform::Interpolate on the host (row-major 4x4)."""
+    a = np.ascontiguousarray(start, dtype=np.float32).reshape(16)
+    b = np.ascontiguousarray(end, dtype=np.float32).reshape(16)
+    out = np.zeros(16, np.float32)
+    _check(lib().wr_motion_interpolate(t0, t1, _p(a), _p(b), time, _p(out)), None, "wr_motion_interpolate")
+    return out.reshape(4, 4)
+
+
 def render(scene, cam, film, spp, max_depth, seed, mode=WR_WAVEFRONT, wave_slots=0, wave_grid=0, nee=False):
     """PathTracer::renderPixels for `spp` more passes into `film`; returns stats.
     nee: the useNextEventEstimation branch (pathTracer.cpp:39-66)."""
@@ -612,6 +623,12 @@ class Scene:
         if counters:
             return hits, dc.download(np.uint64, 3)
         return hits
+
+    def set_motion(self, obj, t0, t1, start, end):
+        """wr_scene_set_motion (before build): object obj moves from `start` (at t0) to `end` (at t1)."""
+        a = np.ascontiguousarray(start, dtype=np.float32).reshape(16)
+        b = np.ascontiguousarray(end, dtype=np.float32).reshape(16)
+        _check(lib().wr_scene_set_motion(self.h, obj, t0, t1, _p(a), _p(b)), self.ctx.h, "wr_scene_set_motion")
 
     def light_paths_device(self, max_depth, seed, pss, path0, n_paths, max_vertices):
         """wr_light_paths into new device buffers: (vertices, counts)."""

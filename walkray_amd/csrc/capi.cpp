@@ -53,6 +53,34 @@ void scene_free_device(wr_scene* s) {
     s->device_bytes = 0;
 }
 
+void place_object(HostObject& o, const io::Mat4& M) {
+    o.verts.resize(o.local_verts.size());
+    for (size_t i = 0; i + 2 < o.local_verts.size(); i += 3) M.apply(&o.local_verts[i], 1.f, &o.verts[i]);
+    o.normals.resize(o.local_normals.size());
+    if (!o.local_normals.empty()) {
+        const io::Mat4 nm = io::transpose(io::inverse(M));
+        for (size_t i = 0; i + 2 < o.local_normals.size(); i += 3) nm.apply(&o.local_normals[i], 0.f, &o.normals[i]);
+    }
+}
+
+// Philox-4x32-10 on the host (Salmon, Moraes, Dror, Shaw, SC'11), the counter
+// RNG of the kernels: the first of its four 24-bit uniforms.
+float motion_time(uint64_t seed, uint32_t obj, uint32_t pass) {
+    uint32_t c[4] = {obj, pass, 0xFFFFFFFEu, 0x57414C4Bu};
+    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return (float)(c[0] >> 8) * (1.0f / 16777216.0f);
+}
+
 }  // namespace wr
 
 using namespace wr;
@@ -190,6 +218,29 @@ int wr_scene_add_mesh(wr_scene* s, const float* verts, uint32_t nv, const uint32
     s->objs.push_back(std::move(o));
     s->built = false;
     if (obj_id) *obj_id = (uint32_t)(s->objs.size() - 1);
+    return WR_OK;
+}
+
+int wr_scene_set_motion(wr_scene* s, uint32_t obj, float start_time, float end_time, const float* start,
+                        const float* end) {
+    if (!s) return fail(nullptr, WR_EINVAL, "wr_scene_set_motion: scene is NULL");
+    wr_ctx* ctx = s->ctx;
+    if (obj >= s->objs.size() || !start || !end) return fail(ctx, WR_EINVAL, "wr_scene_set_motion: bad argument");
+    io::Mat4 a, b;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            a.e[i][j] = start[4 * i + j];
+            b.e[i][j] = end[4 * i + j];
+        }
+    HostObject& o = s->objs[obj];
+    if (!o.moving) {
+        o.local_verts = o.verts;
+        o.local_normals = o.normals;
+        o.moving = true;
+    }
+    o.motion = io::make_motion(start_time, end_time, a, b);
+    place_object(o, io::interpolate(o.motion, start_time));  // as built: the start transform
+    s->built = false;
     return WR_OK;
 }
 

@@ -153,9 +153,36 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
     (void)hipSetDevice(ctx->device);
     const CamHost ch = make_cam(*cam);
     FilmBufs fb{film->sum, film->comp, film->sumsq};
+    bool moving = false;
+    for (const HostObject& o : s->objs) moving = moving || o.moving;
     hipError_t e = hipEventRecord(ctx->ev0, ctx->stream);
     WaveStats ws;
-    if (e == hipSuccess) {
+    if (e == hipSuccess && moving) {
+        // Scene::updateSceneForMotionBlur before every pass (pathTracer.cpp:18,
+        // scene.cpp:67-80): each moving object placed at its time for the pass,
+        // then the trees rebuilt, so every pass sees the reference's trees
+        for (uint32_t k = 0; k < prm->spp && e == hipSuccess; k++) {
+            const uint32_t pass = film->passes + k;
+            wr_scene tmp;
+            tmp.ctx = ctx;
+            tmp.objs = s->objs;
+            tmp.ch_tune = s->ch_tune;
+            for (uint32_t oi = 0; oi < tmp.objs.size(); oi++) {
+                HostObject& o = tmp.objs[oi];
+                if (o.moving) place_object(o, io::interpolate(o.motion, motion_time(prm->seed, oi, pass)));
+            }
+            const int rc = wr_scene_build(&tmp, WR_BEST);
+            if (rc != WR_OK) return rc;
+            if (prm->mode == WR_MEGAKERNEL)
+                e = launch_pt_mega(tmp.dev, ch, prm->max_depth, prm->seed, pass, nee, fb, ctx->stream);
+            else
+                e = run_wavefront(tmp.dev, ch, prm->max_depth, prm->seed, pass, 1, nee, fb, ctx->engine,
+                                  prm->wave_slots ? prm->wave_slots : ctx->tuned_slots,
+                                  prm->wave_grid ? prm->wave_grid : ctx->tuned_grid, &ws, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            scene_free_device(&tmp);
+        }
+    } else if (e == hipSuccess) {
         if (prm->mode == WR_MEGAKERNEL) {
             for (uint32_t k = 0; k < prm->spp && e == hipSuccess; k++)
                 e = launch_pt_mega(s->dev, ch, prm->max_depth, prm->seed, film->passes + k, nee, fb, ctx->stream);

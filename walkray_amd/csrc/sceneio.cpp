@@ -108,6 +108,166 @@ Mat4 inverse(const Mat4& m) {
     return r;
 }
 
+// matrix * matrix, each element summed from 0 over c in order (nvMatrix.h:409-421)
+Mat4 mul(const Mat4& a, const Mat4& b) {
+    Mat4 r;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            float acc = 0.f;
+            for (int c = 0; c < 4; c++) acc += a.e[i][c] * b.e[c][j];
+            r.e[i][j] = acc;
+        }
+    return r;
+}
+
+// ---- AnimatedTransform (motionBlur.h:6-34, motionBlur.cpp:6-87) with the
+// Quaternion it uses (quaternion.h:9-75, quaternion.cpp:5-85) --------------
+static float qdot(const Quat& a, const Quat& b) {  // Dot: v.dot(v) (from 0) + w * w
+    float r = 0.f;
+    r += a.v[0] * b.v[0];
+    r += a.v[1] * b.v[1];
+    r += a.v[2] * b.v[2];
+    return r + a.w * b.w;
+}
+static Quat qmul(const Quat& q, float f) { return Quat{{q.v[0] * f, q.v[1] * f, q.v[2] * f}, q.w * f}; }
+static Quat qadd(const Quat& a, const Quat& b) {
+    return Quat{{a.v[0] + b.v[0], a.v[1] + b.v[1], a.v[2] + b.v[2]}, a.w + b.w};
+}
+static Quat qsub(const Quat& a, const Quat& b) {
+    return Quat{{a.v[0] - b.v[0], a.v[1] - b.v[1], a.v[2] - b.v[2]}, a.w - b.w};
+}
+// operator/(float): the vector part divides as vec3 (a zero divisor leaves it), w as a float
+static Quat qdiv(const Quat& q, float f) {
+    Quat r = q;
+    if (f != 0.f)
+        for (int k = 0; k < 3; k++) r.v[k] = q.v[k] / f;
+    r.w = q.w / f;
+    return r;
+}
+static Quat qnormalize(const Quat& q) { return qdiv(q, std::sqrt(qdot(q, q))); }
+
+// Quaternion::ToTransform (quaternion.cpp:6-31): the rotation block of an identity matrix
+static Mat4 quat_to_matrix(const Quat& q) {
+    const float x = q.v[0], y = q.v[1], z = q.v[2], w = q.w;
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float xy = x * y, xz = x * z, yz = y * z;
+    const float wx = x * w, wy = y * w, wz = z * w;
+    Mat4 m = Mat4::identity();
+    m.e[0][0] = 1.f - 2.f * (yy + zz);
+    m.e[0][1] = 2.f * (xy + wz);
+    m.e[0][2] = 2.f * (xz - wy);
+    m.e[1][0] = 2.f * (xy - wz);
+    m.e[1][1] = 1.f - 2.f * (xx + zz);
+    m.e[1][2] = 2.f * (yz + wx);
+    m.e[2][0] = 2.f * (xz + wy);
+    m.e[2][1] = 2.f * (yz - wx);
+    m.e[2][2] = 1.f - 2.f * (xx + yy);
+    return m;
+}
+
+// Quaternion::Quaternion(const Transform&) (quaternion.cpp:34-70)
+static Quat matrix_to_quat(const Mat4& m) {
+    Quat q{{0.f, 0.f, 0.f}, 1.f};
+    const float trace = m.e[0][0] + m.e[1][1] + m.e[2][2];
+    if (trace > 0.f) {
+        float s = std::sqrt((float)((double)trace + 1.0));
+        q.w = s / 2.0f;
+        s = 0.5f / s;
+        q.v[0] = (m.e[2][1] - m.e[1][2]) * s;
+        q.v[1] = (m.e[0][2] - m.e[2][0]) * s;
+        q.v[2] = (m.e[1][0] - m.e[0][1]) * s;
+    } else {
+        const int nxt[3] = {1, 2, 0};
+        float qq[3];
+        int i = 0;
+        if (m.e[1][1] > m.e[0][0]) i = 1;
+        if (m.e[2][2] > m.e[i][i]) i = 2;
+        const int j = nxt[i], k = nxt[j];
+        float s = std::sqrt((float)((double)(m.e[i][i] - (m.e[j][j] + m.e[k][k])) + 1.0));
+        qq[i] = s * 0.5f;
+        if (s != 0.f) s = 0.5f / s;
+        q.w = (m.e[k][j] - m.e[j][k]) * s;
+        qq[j] = (m.e[j][i] + m.e[i][j]) * s;
+        qq[k] = (m.e[k][i] + m.e[i][k]) * s;
+        q.v[0] = qq[0];
+        q.v[1] = qq[1];
+        q.v[2] = qq[2];
+    }
+    return q;
+}
+
+// Slerp (quaternion.cpp:73-84)
+static Quat slerp(float t, const Quat& q1, const Quat& q2) {
+    const float cos_theta = qdot(q1, q2);
+    if (cos_theta > .9995f) return qnormalize(qadd(qmul(q1, 1.f - t), qmul(q2, t)));
+    const float c = cos_theta < -1.f ? -1.f : cos_theta > 1.f ? 1.f : cos_theta;
+    const float theta = std::acos(c);
+    const float thetap = theta * t;
+    const Quat qperp = qnormalize(qsub(q2, qmul(q1, cos_theta)));
+    return qadd(qmul(q1, std::cos(thetap)), qmul(qperp, std::sin(thetap)));
+}
+
+// AnimatedTransform::Decompose (motionBlur.cpp:48-87): translation, rotation by
+// polar decomposition (averaging with the inverse transpose until the change is
+// at most 1e-4 or 100 steps), scale = inverse(R) * M
+static void decompose(const Mat4& m, float T[3], Quat& Rq, Mat4& S) {
+    T[0] = m.e[0][3];
+    T[1] = m.e[1][3];
+    T[2] = m.e[2][3];
+    Mat4 M = m;
+    for (int i = 0; i < 3; i++) M.e[i][3] = M.e[3][i] = 0.f;
+    M.e[3][3] = 1.f;
+    float norm;
+    int count = 0;
+    Mat4 R = M;
+    do {
+        Mat4 Rnext;
+        const Mat4 Rit = inverse(transpose(R));
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) Rnext.e[i][j] = 0.5f * (R.e[i][j] + Rit.e[i][j]);
+        norm = 0.f;
+        for (int i = 0; i < 3; i++) {
+            const float n = std::fabs(R.e[i][0] - Rnext.e[i][0]) + std::fabs(R.e[i][1] - Rnext.e[i][1]) +
+                            std::fabs(R.e[i][2] - Rnext.e[i][2]);
+            norm = std::max(norm, n);
+        }
+        R = Rnext;
+    } while (++count < 100 && norm > .0001f);
+    Rq = matrix_to_quat(R);
+    S = mul(inverse(R), M);
+}
+
+Motion make_motion(float t0, float t1, const Mat4& a, const Mat4& b) {
+    Motion m;
+    m.t0 = t0;
+    m.t1 = t1;
+    m.start = a;
+    m.end = b;
+    decompose(a, m.T[0], m.R[0], m.S[0]);
+    decompose(b, m.T[1], m.R[1], m.S[1]);
+    bool equal = true;  // operator== over the 16 elements
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) equal = equal && a.e[i][j] == b.e[i][j];
+    m.animated = !equal;
+    return m;
+}
+
+// AnimatedTransform::Interpolate (motionBlur.cpp:6-45)
+Mat4 interpolate(const Motion& m, float time) {
+    if (!m.animated || time <= m.t0) return m.start;
+    if (time >= m.t1) return m.end;
+    const float dt = (time - m.t0) / (m.t1 - m.t0);
+    float trans[3];
+    for (int k = 0; k < 3; k++) trans[k] = (1.f - dt) * m.T[0][k] + dt * m.T[1][k];
+    const Quat rotate = slerp(dt, m.R[0], m.R[1]);
+    Mat4 scale = Mat4::identity();
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) scale.e[i][j] = (1.f - dt) * m.S[0].e[i][j] + dt * m.S[1].e[i][j];  // Lerp
+    Mat4 I = Mat4::identity();
+    for (int k = 0; k < 3; k++) I.e[k][3] = trans[k];  // set_translate
+    return mul(mul(I, quat_to_matrix(rotate)), scale);
+}
+
 // ---- a small XML reader (the subset rapidxml's default parse gives the loader)
 // Element names, attributes, and an element's value: the text of its first
 // non-blank data run, untrimmed, with the five predefined entities and
@@ -744,6 +904,21 @@ int wr_shape_get(const wr_shape* s, uint32_t* nv, uint32_t* nt, uint32_t* nn, ui
     if (tris) memcpy(tris, sh.f.data(), sh.f.size() * sizeof(uint32_t));
     if (normals) memcpy(normals, sh.vn.data(), sh.vn.size() * sizeof(float));
     if (normal_tris) memcpy(normal_tris, sh.fn.data(), sh.fn.size() * sizeof(uint32_t));
+    return WR_OK;
+}
+
+int wr_motion_interpolate(float start_time, float end_time, const float* start, const float* end, float time,
+                          float* out) {
+    if (!start || !end || !out) return wr::fail(nullptr, WR_EINVAL, "wr_motion_interpolate: bad argument");
+    wr::io::Mat4 a, b;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            a.e[i][j] = start[4 * i + j];
+            b.e[i][j] = end[4 * i + j];
+        }
+    const wr::io::Mat4 t = wr::io::interpolate(wr::io::make_motion(start_time, end_time, a, b), time);
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) out[4 * i + j] = t.e[i][j];
     return WR_OK;
 }
 

@@ -113,6 +113,8 @@ double wr_server_clock(const wr_server* s) { return s ? s->now() : 0.0; }
 int wr_server_add_scene(wr_server* s, const wr_scene* scene, uint32_t* scene_id) {
     if (!s || !scene || !scene_id) return fail(s ? s->ctx : nullptr, WR_EINVAL, "wr_server_add_scene: bad argument");
     if (!scene->built) return fail(s->ctx, WR_ESTATE, "wr_server_add_scene: scene not built");
+    for (const HostObject& o : scene->objs)
+        if (o.moving) return fail(s->ctx, WR_EINVAL, "wr_server_add_scene: moving objects are rendered by wr_render only");
     if (s->scenes.size() >= kMaxScenes) return fail(s->ctx, WR_EBUSY, "wr_server_add_scene: too many scenes");
     const uint32_t id = (uint32_t)s->scenes.size();
     hipError_t e = s->engine.set_scene(id, scene->dev, s->ctx->stream);

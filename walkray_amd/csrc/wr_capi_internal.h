@@ -9,6 +9,7 @@
 #include "wr_host.h"
 #include "wr_kernels.h"
 #include "wr_pt.h"
+#include "wr_sceneio.h"
 
 struct wr_ctx {
     int device = 0;
@@ -25,9 +26,22 @@ struct HostObject {
     std::vector<float> verts, normals;  // normals: 9 per triangle (corners), empty for flat shading
     std::vector<uint32_t> idx;
     wr_material mat;
+    // a moving object (wr_scene_set_motion): verts / normals are placed by the
+    // transform of the moment, local_* keep the mesh in its own space
+    bool moving = false;
+    io::Motion motion;
+    std::vector<float> local_verts, local_normals;
 };
 int fail(wr_ctx* ctx, int code, const std::string& msg);
 int hip_fail(wr_ctx* ctx, hipError_t e, const char* what);
+void scene_free_device(wr_scene* s);
+// Places a moving object with transform M: vertices M (v, 1), corner normals
+// transpose(inverse(M)) (n, 0) (mesh.h:47-77,106).
+void place_object(HostObject& o, const io::Mat4& M);
+// Uniform in [0, 1): the time of object obj in pass `pass` (AbstractObject::
+// updateTransform, abstractObj.cpp:70-76), from the counter RNG at vertex
+// 0xFFFFFFFE with the object id in the pixel slot.
+float motion_time(uint64_t seed, uint32_t obj, uint32_t pass);
 }  // namespace wr
 
 struct wr_scene {

@@ -19,6 +19,25 @@ struct Mat4 {
 };
 Mat4 transpose(const Mat4& m);
 Mat4 inverse(const Mat4& m);
+Mat4 mul(const Mat4& a, const Mat4& b);
+
+// AnimatedTransform (motionBlur.h:6-34): the two transforms, their times and
+// their decompositions (translation, rotation quaternion, scale matrix).
+struct Quat {
+    float v[3];
+    float w;
+};
+struct Motion {
+    float t0 = 0.f, t1 = 1.f;
+    Mat4 start = Mat4::identity(), end = Mat4::identity();
+    bool animated = false;  // actuallyAnimated
+    float T[2][3] = {};
+    Quat R[2] = {};
+    Mat4 S[2] = {Mat4::identity(), Mat4::identity()};
+};
+Motion make_motion(float t0, float t1, const Mat4& start, const Mat4& end);
+// AnimatedTransform::Interpolate (motionBlur.cpp:6-45)
+Mat4 interpolate(const Motion& m, float time);
 
 // Mesh's lists (mesh.h:13-19): positions, vertex normals, and per-face
 // position / normal indices (3 per face), plus the mesh transform.
