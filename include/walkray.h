@@ -306,9 +306,9 @@ void wr_hashgrid_destroy(wr_hashgrid* grid);
  * (Config -> Scene -> GroupObject / Object / Camera, materials by path and
  * name) with its .obj / .shp shapes, unitized and transformed as the
  * reference does (mesh.cpp:5-174, mesh.h:106).  Only the four hot object
- * classes, static transforms and the pinhole camera are accepted; anything
- * else (other classes, motion, depth of field, refractive decay) fails with
- * a message.  Host only. */
+ * classes, static transforms, <motion> (wr_config_get_motion) and the pinhole
+ * camera are accepted; anything else (other classes, depth of field,
+ * refractive decay) fails with a message.  Host only. */
 typedef struct wr_config wr_config;
 typedef struct {
     uint32_t n_objects;
@@ -326,8 +326,15 @@ int wr_config_get_info(const wr_config* cfg, wr_config_info* info);
  * *has_normals), material.  Pass NULL arrays to get the sizes first. */
 int wr_config_get_object(const wr_config* cfg, uint32_t i, uint32_t* nv, uint32_t* nt, uint32_t* has_normals,
                          float* verts, uint32_t* tris, float* corner_normals, wr_material* mat);
-/* wr_scene_add_mesh_corners for every object, in order. */
+/* wr_scene_add_mesh_corners for every object, in order, and
+ * wr_scene_set_motion for the moving ones. */
 int wr_config_add_to_scene(const wr_config* cfg, wr_scene* scene);
+/* Object i's <motion> (config.cpp:28-45): *moving = 0 for a static object;
+ * otherwise its vertices and normals from wr_config_get_object are in its own
+ * space and it moves from `start` at start_time to `end` at end_time
+ * (row-major 4x4).  Any output but moving may be NULL. */
+int wr_config_get_motion(const wr_config* cfg, uint32_t i, int* moving, float* start_time, float* end_time,
+                         float* start, float* end);
 
 /* Mesh::loadShape(path, normalize) (mesh.cpp:5-130) for one .obj or .shp file:
  * positions, faces, vertex normals and per-face normal indices, unitized when

@@ -82,3 +82,33 @@ def test_motion_matches_oracle(ctx):
     # the moving sphere does change the image
     still = R.render_scene(ctx, objs, cam, spp, depth, seed, W.WR_WAVEFRONT)
     assert not np.array_equal(still["sum"], film["sum"])
+
+
+def test_motion_from_scene_file(ctx, tmp_path):
+    """A <motion> object read by the loader renders as the same object given to
+    wr_scene_set_motion by hand (wr_config_add_to_scene wires the motion)."""
+    (tmp_path / "strip.obj").write_text("v -0.25 -2 0\nv 0.25 -2 0\nv 0.25 2 0\nv -0.25 2 0\nf 1 2 3\nf 1 3 4\n")
+    (tmp_path / "mats.xml").write_text('<Material Name="lamp"><type>Emissive</type><color>(1, 1, 1)</color>'
+                                       "</Material>")
+    m0 = "[1, 0, 0, -1][0, 1, 0, 0][0, 0, 1, 0][0, 0, 0, 1]"
+    m1 = "[0.5, 0, 0, 1][0, 0.5, 0, 0][0, 0, 0.5, 0][0, 0, 0, 1]"
+    (tmp_path / "scene.xml").write_text(f"""<Scene Name="M">
+<Object><filePath>strip.obj</filePath><Material><path>mats.xml</path><name>lamp</name></Material>
+<motion><startTime>0</startTime><endTime>1</endTime><startTransform>{m0}</startTransform>
+<endTransform>{m1}</endTransform></motion></Object>
+<Camera Name="c"><position>(0, 0, 5)</position><focus>(0, 0, 0)</focus><up>(0, 1, 0)</up>
+<width>16</width><height>4</height><sightDist>128</sightDist></Camera></Scene>""")
+    (tmp_path / "config.xml").write_text("<Config><Scene><path>scene.xml</path><name>M</name><camera>c</camera>"
+                                         "</Scene></Config>")
+    cfg = W.SceneConfig(str(tmp_path / "config.xml"))
+    a = W.Scene(ctx)
+    cfg.add_to(a)
+    a.build()
+    o = cfg.objects[0]
+    t0, t1, ma, mb = o["motion"]
+    b = _scene(ctx, [o], {0: (t0, t1, ma, mb)})
+    print("[motion] from a scene file", flush=True)
+    fa = R.render_scene(ctx, None, cfg.camera, 64, 3, 9, W.WR_WAVEFRONT, scene=a)
+    fb = R.render_scene(ctx, None, cfg.camera, 64, 3, 9, W.WR_WAVEFRONT, scene=b)
+    assert np.array_equal(fa["sum"].view(np.uint32), fb["sum"].view(np.uint32))
+    assert fa["mean"].max() > 0

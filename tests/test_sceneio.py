@@ -169,7 +169,7 @@ def test_config_refuses_off_path_features(tmp_path):
     with pytest.raises(W.WalkRayError, match="Reflective"):
         W.SceneConfig(_scene(tmp_path, obj))
     obj = f"<Object><filePath>cube.obj</filePath>{_mat('white')}<motion><startTime>0</startTime></motion></Object>"
-    with pytest.raises(W.WalkRayError, match="motion"):
+    with pytest.raises(W.WalkRayError, match="endTime"):  # an incomplete <motion>
         W.SceneConfig(_scene(tmp_path, obj))
     obj = f"<Object><filePath>cube.obj</filePath>{_mat('white')}</Object>"
     with pytest.raises(W.WalkRayError, match="depth of field"):
@@ -212,3 +212,25 @@ def test_committed_cornell_files_load():
         assert np.allclose(a["verts"], b["verts"], rtol=0, atol=2e-6)
         assert np.allclose(a["params"], np.float32(b["params"]), rtol=0, atol=0)
         assert (a["normals"] is None) == (b["normals"] is None)
+
+
+def test_config_motion_kept_in_own_space(tmp_path):
+    """<motion> (config.cpp:28-45): the object's mesh stays in its own (unitized) space,
+    whatever static or group transform it has, and its two transforms and times are kept."""
+    m0 = "[1, 0, 0, -1][0, 1, 0, 0][0, 0, 1, 0][0, 0, 0, 1]"
+    m1 = "[0, -1, 0, 2][1, 0, 0, 0][0, 0, 2, 0.5][0, 0, 0, 1]"
+    mo = (f"<motion><startTime>0.25</startTime><endTime>0.75</endTime><startTransform>{m0}</startTransform>"
+          f"<endTransform>{m1}</endTransform></motion>")
+    objs = f"""<Object><filePath>cube.obj</filePath>{_mat("white")}{mo}
+  <transform>[2, 0, 0, 1][0, 2, 0, 0][0, 0, 2, 0][0, 0, 0, 1]</transform></Object>
+<Object><filePath>cube.obj</filePath>{_mat("glass")}</Object>"""
+    cfg = W.SceneConfig(_scene(tmp_path, objs))
+    moving, still = cfg.objects
+    assert still["motion"] is None
+    t0, t1, a, b = moving["motion"]
+    assert (t0, t1) == (0.25, 0.75)
+    assert np.array_equal(a, np.array([[1, 0, 0, -1], [0, 1, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1]], np.float32))
+    assert np.array_equal(b, np.array([[0, -1, 0, 2], [1, 0, 0, 0], [0, 0, 2, 0.5], [0, 0, 0, 1]], np.float32))
+    # own space: the unitized cube, the static <transform> not applied
+    assert sorted(set(moving["verts"][:, 0])) == [-1.0, 1.0]
+    assert np.array_equal(moving["verts"], still["verts"])

@@ -168,6 +168,7 @@ def lib():
         "wr_config_get_info": (i32, [vp, P(ConfigInfo)]),
         "wr_config_get_object": (i32, [vp, u32, P(u32), P(u32), P(u32), vp, vp, vp, P(Material)]),
         "wr_config_add_to_scene": (i32, [vp, vp]),
+        "wr_config_get_motion": (i32, [vp, u32, P(i32), P(C.c_float), P(C.c_float), vp, vp]),
         "wr_shape_load": (i32, [C.c_char_p, i32, P(vp)]),
         "wr_shape_free": (None, [vp]),
         "wr_shape_get": (i32, [vp, P(u32), P(u32), P(u32), P(u32), vp, vp, vp, vp]),
@@ -316,8 +317,14 @@ class SceneConfig:
             normals[t.reshape(-1)] = cn.reshape(-1, 3)
             if not np.array_equal(normals[t.reshape(-1)], cn.reshape(-1, 3)):
                 normals = None
+        moving = C.c_int32()
+        t0, t1 = C.c_float(), C.c_float()
+        a, b = np.zeros(16, np.float32), np.zeros(16, np.float32)
+        _check(L.wr_config_get_motion(self.h, i, C.byref(moving), C.byref(t0), C.byref(t1), _p(a), _p(b)), None,
+               "wr_config_get_motion")
+        motion = (t0.value, t1.value, a.reshape(4, 4), b.reshape(4, 4)) if moving.value else None
         return {"verts": v, "tris": t, "normals": normals, "corner_normals": cn, "mat": m.type,
-                "params": tuple(float(x) for x in params)}
+                "params": tuple(float(x) for x in params), "motion": motion}
 
     def add_to(self, scene):
         _check(lib().wr_config_add_to_scene(self.h, scene.h), scene.ctx.h, "wr_config_add_to_scene")

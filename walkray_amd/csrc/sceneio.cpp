@@ -4,7 +4,7 @@
 // (Film::savePFM after Film::setBuffer, film.h:36-42,100-110).
 //
 // Only what feeds the GPU path is kept: the four hot object classes, static
-// transforms, the pinhole camera and the path tracer's settings.  Motion,
+// transforms, motion, the pinhole camera and the path tracer's settings;
 // depth of field, textures, volumes and the other classes are refused with an
 // error naming them.  World-space vertices and vertex normals are computed on
 // the host with the reference's float operation order (matrix * vec4,
@@ -686,7 +686,13 @@ struct ConfigLoader {
         } else {
             throw std::runtime_error(ctx + ": object class '" + type + "' is not on the GPU path");
         }
-        if (obj->first("motion")) throw std::runtime_error(ctx + ": motion blur is not supported");
+        if (const XmlNode* mo = obj->first("motion")) {  // config.cpp:28-45
+            o.moving = true;
+            o.t0 = atof_f(need(mo, "startTime", ctx));
+            o.t1 = atof_f(need(mo, "endTime", ctx));
+            o.m0 = read_matrix(need(mo, "startTransform", ctx));
+            o.m1 = read_matrix(need(mo, "endTransform", ctx));
+        }
         Mat4 transform = Mat4::identity();
         if (const char* t = obj->val("transform")) transform = read_matrix(t);
         Shape sh;
@@ -699,6 +705,9 @@ struct ConfigLoader {
         const size_t nv = sh.v.size() / 3, nt = sh.f.size() / 3;
         for (uint32_t x : sh.f)
             if (x >= nv) throw std::runtime_error(ctx + ": face vertex index out of range");
+        // a moving object's own transform is replaced before every pass
+        // (AbstractObject::updateTransform): keep its mesh in its own space
+        if (o.moving) transform = Mat4::identity();
         // world vertices: getWorldVertexPosition (mesh.h:106)
         o.verts.resize(3 * nv);
         for (size_t i = 0; i < nv; i++) transform.apply(&sh.v[3 * i], 1.f, &o.verts[3 * i]);
@@ -868,12 +877,38 @@ int wr_config_get_object(const wr_config* cfg, uint32_t i, uint32_t* nv, uint32_
 int wr_config_add_to_scene(const wr_config* cfg, wr_scene* scene) {
     if (!cfg || !scene) return wr::fail(nullptr, WR_EINVAL, "wr_config_add_to_scene: bad argument");
     for (const wr::io::SceneObject& o : cfg->scene.objects) {
-        const int rc = wr_scene_add_mesh_corners(scene, o.verts.data(), (uint32_t)(o.verts.size() / 3), o.tris.data(),
-                                                 (uint32_t)(o.tris.size() / 3),
-                                                 o.corner_normals.empty() ? nullptr : o.corner_normals.data(), &o.mat,
-                                                 nullptr);
+        uint32_t id = 0;
+        int rc = wr_scene_add_mesh_corners(scene, o.verts.data(), (uint32_t)(o.verts.size() / 3), o.tris.data(),
+                                           (uint32_t)(o.tris.size() / 3),
+                                           o.corner_normals.empty() ? nullptr : o.corner_normals.data(), &o.mat, &id);
         if (rc != WR_OK) return rc;
+        if (o.moving) {
+            float a[16], b[16];
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) {
+                    a[4 * i + j] = o.m0.e[i][j];
+                    b[4 * i + j] = o.m1.e[i][j];
+                }
+            rc = wr_scene_set_motion(scene, id, o.t0, o.t1, a, b);
+            if (rc != WR_OK) return rc;
+        }
     }
+    return WR_OK;
+}
+
+int wr_config_get_motion(const wr_config* cfg, uint32_t i, int* moving, float* start_time, float* end_time,
+                         float* start, float* end) {
+    if (!cfg || !moving || i >= cfg->scene.objects.size())
+        return wr::fail(nullptr, WR_EINVAL, "wr_config_get_motion: bad argument");
+    const wr::io::SceneObject& o = cfg->scene.objects[i];
+    *moving = o.moving ? 1 : 0;
+    if (start_time) *start_time = o.t0;
+    if (end_time) *end_time = o.t1;
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) {
+            if (start) start[4 * r + c] = o.m0.e[r][c];
+            if (end) end[4 * r + c] = o.m1.e[r][c];
+        }
     return WR_OK;
 }
 

@@ -60,6 +60,11 @@ struct SceneObject {
     std::vector<uint32_t> tris;
     std::vector<float> corner_normals;
     wr_material mat;
+    // <motion> (config.cpp:28-45): verts / corner_normals are then the mesh's
+    // own space, placed per pass by the AnimatedTransform of these
+    bool moving = false;
+    float t0 = 0.f, t1 = 0.f;
+    Mat4 m0 = Mat4::identity(), m1 = Mat4::identity();
 };
 
 struct Scene {
