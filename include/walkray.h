@@ -226,7 +226,8 @@ typedef struct {
 typedef struct {
     uint64_t paths;       /* paths traced by this call */
     uint64_t iterations;  /* wavefront iterations (0 for the megakernel) */
-    float kernel_ms;      /* device time of this call (HIP events on the context stream) */
+    float kernel_ms;      /* device time of this call (HIP events on the context stream; with moving
+                             objects it spans the per-pass host rebuilds too) */
 } wr_render_stats;
 
 /* Film: per pixel and channel a Kahan-compensated FP32 sum of the per-pass

@@ -172,7 +172,10 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
                 if (o.moving) place_object(o, io::interpolate(o.motion, motion_time(prm->seed, oi, pass)));
             }
             const int rc = wr_scene_build(&tmp, WR_BEST);
-            if (rc != WR_OK) return rc;
+            if (rc != WR_OK) {
+                film->passes += k;  // the passes before this one are in the film
+                return rc;
+            }
             if (prm->mode == WR_MEGAKERNEL)
                 e = launch_pt_mega(tmp.dev, ch, prm->max_depth, prm->seed, pass, nee, fb, ctx->stream);
             else
