@@ -215,11 +215,25 @@ __global__ __launch_bounds__(256) void k_grid_bbox(const float* pos, uint64_t n,
             mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off));
         }
     }
-    if ((threadIdx.x & 63) == 0) {
+    // the block's four waves meet in LDS: one set of atomics per block
+    __shared__ float part[4][6];
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int a = 0; a < 3; a++) {
+            part[w][a] = mn[a];
+            part[w][3 + a] = mx[a];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            atomicMin(keys + a, fkey(mn[a]));
-            atomicMax(keys + 3 + a, fkey(mx[a]));
+            float lo = part[0][a], hi = part[0][3 + a];
+            for (int k = 1; k < 4; k++) {
+                lo = fminf(lo, part[k][a]);
+                hi = fmaxf(hi, part[k][3 + a]);
+            }
+            atomicMin(keys + a, fkey(lo));
+            atomicMax(keys + 3 + a, fkey(hi));
         }
     }
 }
