@@ -68,6 +68,16 @@ def _compare(ctx, sc, cam, spp, depth, seed, log, label, repeats=2, nee=False):
     return res, films["wavefront"]
 
 
+def _at_fixed_rmse(r, spp, cam, target=0.01):
+    """Seconds each engine needs for `target` relative RMSE against the reference: the error
+    falls as 1/sqrt(spp) (no bias: checked from 16 to 4,096 spp on the Cornell box), so
+    spp_needed = spp * (rmse / target)^2 at the measured samples/s."""
+    need = spp * (r["rel_rmse_vs_reference"] / target) ** 2
+    samples = need * cam["width"] * cam["height"]
+    r["at_1pct_rmse"] = {"spp": round(need), **{
+        f"{m}_s": round(samples / r[m]["samples_per_s"], 2) for m in ("megakernel", "wavefront")}}
+
+
 def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
     out = {}
     objs, cam = S.cornell(k=5, width=256, height=256)
@@ -82,6 +92,7 @@ def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
         ref = np.load(C1_REFERENCE)
         r["rel_rmse_vs_reference"] = round(rel_rmse(film["mean"], ref["mean"]), 5)
         r["reference"] = f"{int(ref['spp'])}-spp render, FP64 accumulation of per-batch Kahan films"
+        _at_fixed_rmse(r, 16, cam)
     r["next_event"], _ = _compare(ctx, sc, cam, 16, 8, 1234, log, "C1 cornell +NEE", nee=True)
     out["c1"] = r
     objs, cam = S.random_field()
@@ -99,6 +110,7 @@ def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
         r["rel_rmse_vs_reference"] = round(rel_rmse(img, ref["mean"]), 5)
         r["reference"] = (f"{int(ref['spp'])}-spp render, FP64 accumulation of per-batch Kahan films, "
                           f"compared on every {st}th pixel in x and y ({img.shape[1]} x {img.shape[0]})")
+        _at_fixed_rmse(r, c3_spp, cam)
     r["next_event"], _ = _compare(ctx, sc, cam, c3_spp, 16, 77, log, "C3 field +NEE", repeats=1, nee=True)
     out["c3"] = r
     return out
