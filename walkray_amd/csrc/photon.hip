@@ -61,8 +61,8 @@ __global__ __launch_bounds__(256) void k_light_paths(SceneDev S, uint32_t max_de
                                                      uint32_t* counts) {
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
-    LaneStack st;
-    st.lane = threadIdx.x;
+    StackSpill spill;
+    LaneStack st(&spill);
     const SceneCtx sc = make_scene_ctx(S);
     const uint32_t path = path0 + k;
     wr_path_vertex* V = out + (size_t)k * maxv;

@@ -29,8 +29,8 @@ __global__ __launch_bounds__(256) void k_pt_mega(SceneDev S, CamDev cam, uint32_
                                                  uint32_t pass, FilmDev film, uint32_t npix) {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     if (p >= npix) return;
-    LaneStack st;
-    st.lane = threadIdx.x;
+    StackSpill spill;
+    LaneStack st(&spill);
     const SceneCtx sc = make_scene_ctx(S);
     const f3 v = sample_path<NEE>(sc, cam, max_depth, seed, p, pass, st);
     film_add(film, p, v);
@@ -164,8 +164,8 @@ __global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
     // workgroups beyond what n paths need leave at once (and are not counted below)
     const uint32_t used = (uint32_t)(((uint64_t)n + 255) / 256 < gridDim.x ? ((uint64_t)n + 255) / 256 : gridDim.x);
     if (blockIdx.x >= used) return;
-    LaneStack st;
-    st.lane = threadIdx.x;
+    StackSpill spill;
+    LaneStack st(&spill);
     Counters cnt{0, 0, 0};
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -253,8 +253,8 @@ __global__ __launch_bounds__(256) void k_wf_finish(EngineDev E, uint32_t par) {
     const PathBuf& A = E.buf[par];
     const PathBuf& B = E.buf[par ^ 1];
     const uint32_t n = E.cnt->n[par];
-    LaneStack st;
-    st.lane = threadIdx.x;
+    StackSpill spill;
+    LaneStack st(&spill);
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         const wr::JobDesc& J = E.jobs[A.job[i]];
         const SceneDev& S = E.scenes[J.scene];
@@ -307,8 +307,8 @@ __global__ __launch_bounds__(256) void k_wf_shade(EngineDev E, uint32_t par) {
     const PathBuf& A = E.buf[par];
     const PathBuf& B = E.buf[par ^ 1];
     const uint32_t n = E.cnt->n[par];
-    LaneStack st;
-    st.lane = threadIdx.x;
+    StackSpill spill;
+    LaneStack st(&spill);
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         if (A.flags[i] & kDead) continue;
         const uint32_t jb = A.job[i];

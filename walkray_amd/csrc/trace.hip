@@ -70,8 +70,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     SceneDev S, wr_tree_hdr H, const float* __restrict__ rays, uint64_t n, uint4* __restrict__ hits,
     unsigned long long* counters, unsigned long long* work, uint32_t refill_min, uint32_t node_phase_min) {
     const Tree tr = make_tree(H, S);
-    LaneStack st;
-    st.lane = threadIdx.x;
+    StackSpill spill;
+    LaneStack st(&spill);
     Counters cnt{0, 0, 0};
     const NoSelf ns{};
     const uint32_t lane = threadIdx.x & 63u;
@@ -149,8 +149,8 @@ __global__ __launch_bounds__(256) void k_closest_hit_finish(SceneDev S, wr_tree_
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     const uint64_t i = (uint64_t)b * 256 + threadIdx.x;
     Counters cnt{0, 0, 0};
-    LaneStack st;
-    st.lane = threadIdx.x;
+    StackSpill spill;
+    LaneStack st(&spill);
     if (i < n) {
         const uint4 f = hits[i];
         HitState h{__uint_as_float(f.y), f.x, f.z};
@@ -183,8 +183,8 @@ __global__ __launch_bounds__(256) void k_closest_hit_path(SceneDev S, wr_tree_hd
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     const uint64_t i = (uint64_t)b * 256 + threadIdx.x;
     if (i >= n) return;
-    LaneStack st;
-    st.lane = threadIdx.x;
+    StackSpill spill;
+    LaneStack st(&spill);
     const uint32_t* rp = rays + 8 * i;
     const f3 o = mk(__uint_as_float(rp[0]), __uint_as_float(rp[1]), __uint_as_float(rp[2]));
     const f3 d = mk(__uint_as_float(rp[3]), __uint_as_float(rp[4]), __uint_as_float(rp[5]));
@@ -211,8 +211,8 @@ __global__ __launch_bounds__(256) void k_inside_object(SceneDev S, const uint32_
                                                        uint32_t* __restrict__ out) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    LaneStack st;
-    st.lane = threadIdx.x;
+    StackSpill spill;
+    LaneStack st(&spill);
     const uint32_t* rp = rays + 8 * i;
     const f3 o = mk(__uint_as_float(rp[0]), __uint_as_float(rp[1]), __uint_as_float(rp[2]));
     const f3 d = mk(__uint_as_float(rp[3]), __uint_as_float(rp[4]), __uint_as_float(rp[5]));

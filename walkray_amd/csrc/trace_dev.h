@@ -293,19 +293,28 @@ constexpr int kBlock = 256;  // every traversal kernel runs 256-lane workgroups
 // (references, then entry distances).
 extern __shared__ uint32_t wr_lds_stack[];
 
+// The spill levels are a separate private array of the launching kernel: kept
+// inside LaneStack, the dynamically indexed arrays would put the whole struct,
+// sp and lane included, in scratch, and every traversal step would start with
+// a scratch load of sp before its node fetch.
+struct StackSpill {
+    uint32_t ref[kMaxDepth - kLdsDepth];
+    float t[kMaxDepth - kLdsDepth];
+};
+
 struct LaneStack {
+    StackSpill* spill;
     uint32_t lane;  // threadIdx.x
     int sp;
-    uint32_t spill_ref[kMaxDepth - kLdsDepth];
-    float spill_t[kMaxDepth - kLdsDepth];
+    __device__ explicit LaneStack(StackSpill* s) : spill(s), lane(threadIdx.x), sp(0) {}
     // The LDS accesses are unconditional (a wrapped slot when spilling) so they
     // stay ds_read/ds_write instead of being merged with the scratch access
     // into a generic (flat) one.
     __device__ __forceinline__ void push(uint32_t v, float t) {
         const uint32_t slot = (sp & (kLdsDepth - 1)) * kBlock + lane;
         if (sp >= kLdsDepth) {
-            spill_ref[sp - kLdsDepth] = wr_lds_stack[slot];
-            spill_t[sp - kLdsDepth] = __uint_as_float(wr_lds_stack[kLdsDepth * kBlock + slot]);
+            spill->ref[sp - kLdsDepth] = wr_lds_stack[slot];
+            spill->t[sp - kLdsDepth] = __uint_as_float(wr_lds_stack[kLdsDepth * kBlock + slot]);
         }
         wr_lds_stack[slot] = v;
         wr_lds_stack[kLdsDepth * kBlock + slot] = __float_as_uint(t);
@@ -317,8 +326,8 @@ struct LaneStack {
         const uint32_t v = wr_lds_stack[slot];
         t = __uint_as_float(wr_lds_stack[kLdsDepth * kBlock + slot]);
         if (sp >= kLdsDepth) {
-            wr_lds_stack[slot] = spill_ref[sp - kLdsDepth];
-            wr_lds_stack[kLdsDepth * kBlock + slot] = __float_as_uint(spill_t[sp - kLdsDepth]);
+            wr_lds_stack[slot] = spill->ref[sp - kLdsDepth];
+            wr_lds_stack[kLdsDepth * kBlock + slot] = __float_as_uint(spill->t[sp - kLdsDepth]);
         }
         return v;
     }
