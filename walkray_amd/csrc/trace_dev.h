@@ -104,8 +104,11 @@ __device__ __forceinline__ f3 normalize(f3 a) {
 struct RayPre {
     f3 o, d;
     float ix, iy, iz;     // 1/d (finite stand-in for d == 0)
-    float olx, oly, olz;  // (o + e) / d
-    float ohx, ohy, ohz;  // (o - e) / d
+    // per axis, the offsets of the enlarged box's near and far planes: with
+    // ol = (o + e) / d and oh = (o - e) / d, near = ol and far = oh where
+    // 1/d > 0, swapped where 1/d < 0
+    float onx, ony, onz;
+    float ofx, ofy, ofz;
     float e;              // box enlargement in world units
     float tscale;         // 2^-20 * max(1, |d|_inf): t-proportional part of the face-point error bound
 };
@@ -123,26 +126,41 @@ __device__ __forceinline__ RayPre make_ray(f3 o, f3 d, float scale) {
     r.ix = safe_inv(d.x);
     r.iy = safe_inv(d.y);
     r.iz = safe_inv(d.z);
-    r.olx = (o.x + r.e) * r.ix;
-    r.oly = (o.y + r.e) * r.iy;
-    r.olz = (o.z + r.e) * r.iz;
-    r.ohx = (o.x - r.e) * r.ix;
-    r.ohy = (o.y - r.e) * r.iy;
-    r.ohz = (o.z - r.e) * r.iz;
+    const float olx = (o.x + r.e) * r.ix, ohx = (o.x - r.e) * r.ix;
+    const float oly = (o.y + r.e) * r.iy, ohy = (o.y - r.e) * r.iy;
+    const float olz = (o.z + r.e) * r.iz, ohz = (o.z - r.e) * r.iz;
+    r.onx = r.ix > 0.f ? olx : ohx;
+    r.ofx = r.ix > 0.f ? ohx : olx;
+    r.ony = r.iy > 0.f ? oly : ohy;
+    r.ofy = r.iy > 0.f ? ohy : oly;
+    r.onz = r.iz > 0.f ? olz : ohz;
+    r.ofz = r.iz > 0.f ? ohz : olz;
     r.tscale = fmaxf(1.f, fmaxf(fmaxf(fabsf(d.x), fabsf(d.y)), fabsf(d.z))) * 9.5367431640625e-07f;  // 2^-20
     return r;
 }
 
-// Conservative slab test on the enlarged box, clipped to [0, tcut].
-__device__ __forceinline__ bool slab(const RayPre& r, float lx, float ly, float lz, float hx, float hy,
-                                     float hz, float tcut, float& tn_out) {
-    const float t0x = __builtin_fmaf(lx, r.ix, -r.olx), t1x = __builtin_fmaf(hx, r.ix, -r.ohx);
-    const float t0y = __builtin_fmaf(ly, r.iy, -r.oly), t1y = __builtin_fmaf(hy, r.iy, -r.ohy);
-    const float t0z = __builtin_fmaf(lz, r.iz, -r.olz), t1z = __builtin_fmaf(hz, r.iz, -r.ohz);
-    const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
-    const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tcut));
+// Conservative slab test on the enlarged box, clipped to [0, tcut], from the
+// near-plane and far-plane coordinates of each axis (slab_nf) or from the box
+// (slab).  The plane distances are t0 = fma(lo, 1/d, -(o + e)/d) and t1 =
+// fma(hi, 1/d, -(o - e)/d); rounding is monotone, so t0 <= t1 where 1/d > 0
+// and t0 >= t1 where 1/d < 0: picking the near plane by the sign of 1/d gives
+// exactly min(t0, t1), and the far plane max(t0, t1).
+__device__ __forceinline__ bool slab_nf(const RayPre& r, float nx, float ny, float nz, float fx, float fy, float fz,
+                                        float tcut, float& tn_out) {
+    const float tnx = __builtin_fmaf(nx, r.ix, -r.onx), tfx = __builtin_fmaf(fx, r.ix, -r.ofx);
+    const float tny = __builtin_fmaf(ny, r.iy, -r.ony), tfy = __builtin_fmaf(fy, r.iy, -r.ofy);
+    const float tnz = __builtin_fmaf(nz, r.iz, -r.onz), tfz = __builtin_fmaf(fz, r.iz, -r.ofz);
+    const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.f));
+    const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tcut));
     tn_out = tn;
     return tn <= tf;
+}
+
+__device__ __forceinline__ bool slab(const RayPre& r, float lx, float ly, float lz, float hx, float hy,
+                                     float hz, float tcut, float& tn_out) {
+    const bool px = r.ix > 0.f, py = r.iy > 0.f, pz = r.iz > 0.f;
+    return slab_nf(r, px ? lx : hx, py ? ly : hy, pz ? lz : hz, px ? hx : lx, py ? hy : ly, pz ? hz : lz, tcut,
+                   tn_out);
 }
 
 // Node::BoundingBox::intersect (kdtree.cpp:46-64) reduced to "returns >= 0".
@@ -496,24 +514,36 @@ __device__ __forceinline__ uint32_t next_or_end(LaneStack& st, float tcut) {
 template <bool COUNT>
 __device__ __forceinline__ uint32_t wide_node(const Tree& tr, const RayPre& r, LaneStack& st, float best_t,
                                               uint32_t ref, bool& overflow, Counters& cnt) {
-    const wr_wnode W = tr.wnodes[ref];
+    // the node as four 16 B loads: origin[3] scale[3] | q rows (one dword per
+    // bound, child k in byte k) | ref[4]
+    const uint4* np = reinterpret_cast<const uint4*>(tr.wnodes + ref);
+    const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
     if (COUNT) cnt.nodes += 2;  // node bytes in 32 B units, as for a binary node
     const float tcut = prune_cut(r, best_t);
+    const float ox = __uint_as_float(n0.x), oy = __uint_as_float(n0.y), oz = __uint_as_float(n0.z);
+    const float sx = __uint_as_float(n0.w), sy = __uint_as_float(n1.x), sz = __uint_as_float(n1.y);
+    // rows of the near and far bounds per axis (min row where 1/d > 0, else max)
+    const bool px = r.ix > 0.f, py = r.iy > 0.f, pz = r.iz > 0.f;
+    const uint32_t qnx = px ? n1.z : n2.y, qfx = px ? n2.y : n1.z;
+    const uint32_t qny = py ? n1.w : n2.z, qfy = py ? n2.z : n1.w;
+    const uint32_t qnz = pz ? n2.x : n2.w, qfz = pz ? n2.w : n2.x;
+    const uint32_t cref[4] = {n3.x, n3.y, n3.z, n3.w};
     float t[4];
     uint32_t c[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        const float lx = __builtin_fmaf((float)W.q[0][k], W.scale[0], W.origin[0]);
-        const float ly = __builtin_fmaf((float)W.q[1][k], W.scale[1], W.origin[1]);
-        const float lz = __builtin_fmaf((float)W.q[2][k], W.scale[2], W.origin[2]);
-        const float hx = __builtin_fmaf((float)W.q[3][k], W.scale[0], W.origin[0]);
-        const float hy = __builtin_fmaf((float)W.q[4][k], W.scale[1], W.origin[1]);
-        const float hz = __builtin_fmaf((float)W.q[5][k], W.scale[2], W.origin[2]);
+        const int sh = 8 * k;
+        const float nx = __builtin_fmaf((float)((qnx >> sh) & 0xFFu), sx, ox);
+        const float ny = __builtin_fmaf((float)((qny >> sh) & 0xFFu), sy, oy);
+        const float nz = __builtin_fmaf((float)((qnz >> sh) & 0xFFu), sz, oz);
+        const float fx = __builtin_fmaf((float)((qfx >> sh) & 0xFFu), sx, ox);
+        const float fy = __builtin_fmaf((float)((qfy >> sh) & 0xFFu), sy, oy);
+        const float fz = __builtin_fmaf((float)((qfz >> sh) & 0xFFu), sz, oz);
         float tn;
-        const bool hit = slab(r, lx, ly, lz, hx, hy, hz, tcut, tn) && W.ref[k] != WR_NONE;
+        const bool hit = slab_nf(r, nx, ny, nz, fx, fy, fz, tcut, tn) && cref[k] != WR_NONE;
         // a hit's key is finite, a miss's +inf, so misses sort last
         t[k] = hit ? fminf(tn, 3.0e38f) : __builtin_inff();
-        c[k] = W.ref[k];
+        c[k] = cref[k];
     }
     cswap(t[0], c[0], t[1], c[1]);
     cswap(t[2], c[2], t[3], c[3]);

@@ -58,6 +58,8 @@ struct wr_wnode {
     uint32_t ref[4];
 };
 static_assert(sizeof(wr_wnode) == 64, "wide node must be 64 bytes");
+static_assert(__builtin_offsetof(wr_wnode, q) == 24 && __builtin_offsetof(wr_wnode, ref) == 48,
+              "wide-node fields are read as four 16 B words");
 
 // 128 B chain record of one reference leaf (one L2 line): the boxes that the
 // reference tests on the way to the leaf at its first up-to-five ancestors,
