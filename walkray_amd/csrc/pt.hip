@@ -370,38 +370,77 @@ __global__ __launch_bounds__(256) void k_wf_shade(EngineDev E, uint32_t par) {
     }
 }
 
-// One lane: hand the free slots behind the survivors to the running jobs,
-// round-robin from a rotating start, each within its window of free pass slots.
-__global__ void k_wf_plan(EngineDev E, uint32_t par) {
+// Number of consecutive complete passes of a job starting at its fold cursor.
+__device__ __forceinline__ uint32_t foldable(const wr::JobDesc& J, const wr::JobCtl& C) {
+    uint32_t k = 0;
+    while (k < J.R && C.cursor + k < J.npass && J.pass_done[(C.cursor + k) % J.R] == J.npix) k++;
+    return k;
+}
+
+// One wave (lane k = job k, then lane k = the k-th job of the round-robin
+// order).  First the fold plan: per running job, the complete passes from its
+// fold cursor (shade has finished this iteration's paths, and nothing before
+// k_wf_advance changes the counts).  Then the free slots behind the survivors
+// go to the running jobs round-robin from a rotating start, each within its
+// window of free pass slots: the greedy serial allocation in that order, job
+// i taking min(avail_i, room - sum of the takes before it), is a prefix sum
+// over the wave, clamp(room - sum_{<i} avail, 0, avail_i).
+__global__ __launch_bounds__(64) void k_wf_plan(EngineDev E, uint32_t par) {
+    static_assert(wr::kMaxJobs == 64, "one lane per job");
     wr::EngineCounters& c = *E.cnt;
-    uint32_t base = c.n[par ^ 1];
-    uint32_t room = E.cap - base;
-    uint32_t nseg = 0;
-    for (uint32_t k = 0; k < wr::kMaxJobs && room > 0; k++) {
-        const uint32_t j = (c.rr + k) % wr::kMaxJobs;
-        wr::JobCtl& C = E.ctl[j];
-        if (C.state != wr::kJobRunning) continue;
+    const uint32_t lane = threadIdx.x;
+    const uint64_t below = (1ull << lane) - 1ull;
+    {
+        const wr::JobCtl& C = E.ctl[lane];
+        const uint32_t k = C.state == wr::kJobRunning ? foldable(E.jobs[lane], C) : 0u;
+        const uint64_t m = __ballot(k > 0);
+        c.fold_k[lane] = k;
+        if (k > 0) c.fold_job[__popcll(m & below)] = lane;
+        if (lane == 0) c.nfold = (uint32_t)__popcll(m);
+    }
+    const uint32_t base0 = c.n[par ^ 1];
+    const uint32_t room = E.cap - base0;
+    const uint32_t rr = c.rr;
+    const uint32_t j = (rr + lane) % wr::kMaxJobs;
+    wr::JobCtl& C = E.ctl[j];
+    uint64_t avail = 0, next = 0;
+    if (C.state == wr::kJobRunning) {
         const wr::JobDesc& J = E.jobs[j];
         const uint64_t total = (uint64_t)J.npass * J.npix;
         const uint64_t lim_w = (uint64_t)(C.cursor + J.R) * J.npix;
         const uint64_t limit = lim_w < total ? lim_w : total;
-        const uint64_t avail = limit > C.next_work ? limit - C.next_work : 0;
-        const uint32_t take = (uint32_t)(avail < room ? avail : room);
-        if (take == 0) continue;
-        c.seg_job[nseg] = j;
-        c.seg_base[nseg] = base;
-        c.seg_take[nseg] = take;
-        c.seg_work[nseg] = C.next_work;
-        nseg++;
-        C.next_work += take;
-        base += take;
-        room -= take;
+        next = C.next_work;
+        avail = limit > next ? limit - next : 0;
     }
-    c.nseg = nseg;
-    c.seg_end = base;
-    c.seg_start = c.n[par ^ 1];
-    c.n[par ^ 1] = base;
-    c.rr = (c.rr + 1) % wr::kMaxJobs;
+    if (avail > room) avail = room;  // a take never exceeds room; keeps the sums below 2^39
+    // exclusive prefix sum of avail over the lanes
+    uint64_t incl = avail;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t lo = __shfl_up((uint32_t)incl, off), hi = __shfl_up((uint32_t)(incl >> 32), off);
+        if (lane >= (uint32_t)off) incl += ((uint64_t)hi << 32) | lo;
+    }
+    const uint64_t before = incl - avail;
+    const uint32_t used = (uint32_t)(before < room ? before : room);
+    const uint32_t take = (uint32_t)(avail < (uint64_t)(room - used) ? avail : (uint64_t)(room - used));
+    const uint64_t m = __ballot(take > 0);
+    if (take > 0) {
+        const uint32_t s = (uint32_t)__popcll(m & below);
+        c.seg_job[s] = j;
+        c.seg_base[s] = base0 + used;
+        c.seg_take[s] = take;
+        c.seg_work[s] = next;
+        C.next_work = next + take;
+    }
+    const uint64_t total_take = __shfl(incl, 63);
+    if (lane == 0) {
+        const uint32_t all = (uint32_t)(total_take < room ? total_take : room);
+        c.nseg = (uint32_t)__popcll(m);
+        c.seg_start = base0;
+        c.seg_end = base0 + all;
+        c.n[par ^ 1] = base0 + all;
+        c.rr = (rr + 1) % wr::kMaxJobs;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_wf_generate(EngineDev E, uint32_t par) {
@@ -437,49 +476,47 @@ __global__ __launch_bounds__(256) void k_wf_generate(EngineDev E, uint32_t par) 
     }
 }
 
-// Number of consecutive complete passes of a job starting at its fold cursor.
-__device__ __forceinline__ uint32_t foldable(const wr::JobDesc& J, const wr::JobCtl& C) {
-    uint32_t k = 0;
-    while (k < J.R && C.cursor + k < J.npass && J.pass_done[(C.cursor + k) % J.R] == J.npix) k++;
-    return k;
-}
-
-// Adds every complete pass of every running job to its film, in pass order.
+// Adds every complete pass of every running job to its film, in pass order
+// (the jobs and pass counts of k_wf_plan's fold plan).
 __global__ __launch_bounds__(256) void k_wf_fold(EngineDev E) {
-    for (uint32_t jb = 0; jb < wr::kMaxJobs; jb++) {
+    const wr::EngineCounters& c = *E.cnt;
+    const uint32_t nf = c.nfold;
+    for (uint32_t f = 0; f < nf; f++) {
+        const uint32_t jb = c.fold_job[f];
+        const uint32_t k = c.fold_k[jb];
         const wr::JobCtl& C = E.ctl[jb];
-        if (C.state != wr::kJobRunning) continue;
         const wr::JobDesc& J = E.jobs[jb];
-        const uint32_t k = foldable(J, C);
-        if (k == 0) continue;
         const FilmDev film{J.fsum, J.fcomp, J.fsumsq};
         for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < J.npix; p += gridDim.x * 256) {
             for (uint32_t q = 0; q < k; q++) {
-                const float* c = J.contrib + 3 * ((size_t)((C.cursor + q) % J.R) * J.npix + p);
-                film_add(film, p, mk(c[0], c[1], c[2]));
+                const float* cc = J.contrib + 3 * ((size_t)((C.cursor + q) % J.R) * J.npix + p);
+                film_add(film, p, mk(cc[0], cc[1], cc[2]));
             }
         }
     }
 }
 
-// One lane: advance fold cursors, re-arm folded pass slots, retire finished
-// jobs, empty the consumed path buffer.
-__global__ void k_wf_advance(EngineDev E, uint32_t par) {
+// One wave, lane k = job k: advance fold cursors by the fold plan, re-arm the
+// folded pass slots, retire finished jobs; then empty the consumed path buffer.
+__global__ __launch_bounds__(64) void k_wf_advance(EngineDev E, uint32_t par) {
     wr::EngineCounters& c = *E.cnt;
-    for (uint32_t jb = 0; jb < wr::kMaxJobs; jb++) {
-        wr::JobCtl& C = E.ctl[jb];
-        if (C.state != wr::kJobRunning) continue;
+    const uint32_t jb = threadIdx.x;
+    const uint32_t it = c.iteration;
+    wr::JobCtl& C = E.ctl[jb];
+    if (C.state == wr::kJobRunning) {
         const wr::JobDesc& J = E.jobs[jb];
-        const uint32_t k = foldable(J, C);
+        const uint32_t k = c.fold_k[jb];
         for (uint32_t q = 0; q < k; q++) J.pass_done[(C.cursor + q) % J.R] = 0;
         C.cursor += k;
         if (C.cursor >= J.npass) {
             C.state = wr::kJobDone;
-            C.done_iter = c.iteration;
+            C.done_iter = it;
         }
     }
-    c.n[par] = 0;
-    c.iteration++;
+    if (jb == 0) {
+        c.n[par] = 0;
+        c.iteration = it + 1;
+    }
 }
 
 }  // namespace wrd
@@ -522,8 +559,10 @@ Engine::~Engine() { release(); }
 void Engine::release() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
+    if (tail_exec) (void)hipGraphExecDestroy(tail_exec);
+    if (tail_graph) (void)hipGraphDestroy(tail_graph);
+    exec = tail_exec = nullptr;
+    graph = tail_graph = nullptr;
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     bufs.clear();
@@ -582,21 +621,26 @@ hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st) {
     WR_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     WR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_wf_trace, 256, kStackLdsBytes));
     const uint32_t trace_grid = (uint32_t)std::max(1, std::min<int>(cus * std::max(per_cu, 1), (int)(cap / 256)));
-    // the iteration graph: kGraphIters iterations (even, the buffer parity alternates)
-    WR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int it = 0; it < kGraphIters; it++) {
-        const uint32_t par = (uint32_t)(it & 1);
-        k_wf_trace<<<trace_grid, 256, kStackLdsBytes, st>>>(E, par);
-        k_wf_finish<<<grid, 256, kStackLdsBytes, st>>>(E, par);
-        k_wf_shade<<<grid, 256, kStackLdsBytes, st>>>(E, par);
-        k_wf_plan<<<1, 1, 0, st>>>(E, par);
-        k_wf_generate<<<grid, 256, 0, st>>>(E, par);
-        k_wf_fold<<<grid, 256, 0, st>>>(E);
-        k_wf_advance<<<1, 1, 0, st>>>(E, par);
-    }
-    hipError_t e = hipStreamEndCapture(st, &graph);
-    if (e != hipSuccess) return e;
-    return hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    // the iteration graphs: kGraphIters and kTailIters iterations (even, the
+    // buffer parity alternates)
+    auto capture = [&](int iters, hipGraph_t* g, hipGraphExec_t* x) -> hipError_t {
+        WR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int it = 0; it < iters; it++) {
+            const uint32_t par = (uint32_t)(it & 1);
+            k_wf_trace<<<trace_grid, 256, kStackLdsBytes, st>>>(E, par);
+            k_wf_finish<<<grid, 256, kStackLdsBytes, st>>>(E, par);
+            k_wf_shade<<<grid, 256, kStackLdsBytes, st>>>(E, par);
+            k_wf_plan<<<1, 64, 0, st>>>(E, par);
+            k_wf_generate<<<grid, 256, 0, st>>>(E, par);
+            k_wf_fold<<<grid, 256, 0, st>>>(E);
+            k_wf_advance<<<1, 64, 0, st>>>(E, par);
+        }
+        WR_CHECK(hipStreamEndCapture(st, g));
+        return hipGraphInstantiate(x, *g, nullptr, nullptr, 0);
+    };
+    static_assert(kGraphIters % 2 == 0 && kTailIters % 2 == 0, "replays end on the parity they start on");
+    WR_CHECK(capture(kGraphIters, &graph, &exec));
+    return capture(kTailIters, &tail_graph, &tail_exec);
 }
 
 hipError_t Engine::set_scene(uint32_t slot, const SceneDev& S, hipStream_t st) {
@@ -620,8 +664,8 @@ hipError_t Engine::release_slot(uint32_t slot, hipStream_t st) {
 }
 
 // One replay of the iteration graph, then a read-back of the job states.
-hipError_t Engine::step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt) {
-    WR_CHECK(hipGraphLaunch(exec, st));
+hipError_t Engine::step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt, bool tail) {
+    WR_CHECK(hipGraphLaunch(tail ? tail_exec : exec, st));
     if (host_ctl) WR_CHECK(hipMemcpyAsync(host_ctl, ctl, sizeof(JobCtl) * kMaxJobs, hipMemcpyDeviceToHost, st));
     if (host_cnt) WR_CHECK(hipMemcpyAsync(host_cnt, cnt, sizeof(EngineCounters), hipMemcpyDeviceToHost, st));
     return hipStreamSynchronize(st);
@@ -673,13 +717,19 @@ hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_dep
     if (e == hipSuccess) e = E.admit(0, J, st);
     std::vector<JobCtl> hc(kMaxJobs);
     EngineCounters hcnt{};
-    const uint32_t it0 = 0;
-    uint64_t replays = 0;
+    if (e == hipSuccess) e = hipMemcpy(&hcnt, E.cnt, sizeof(EngineCounters), hipMemcpyDeviceToHost);
+    const uint32_t it0 = hcnt.iteration;
+    // every pass is generated within npass window turns and drains in at most
+    // max_depth + 2 iterations after its last path is issued
+    const uint64_t it_bound = 4 * (uint64_t)(max_depth + 4) * (total / E.cap + npass + 2) + 64;
+    uint64_t iters = 0;
+    bool tail = false;
     while (e == hipSuccess) {
-        e = E.step(st, hc.data(), &hcnt);
-        replays++;
+        e = E.step(st, hc.data(), &hcnt, tail);
+        iters += tail ? kTailIters : kGraphIters;
         if (e != hipSuccess || hc[0].state == kJobDone) break;
-        if (replays > 4 * (uint64_t)(max_depth + 4) * (total / E.cap + 2) + 16) {  // cannot happen
+        tail = hc[0].next_work >= total;  // all camera paths issued: only draining is left
+        if (iters > it_bound) {  // cannot happen
             e = hipErrorUnknown;
             break;
         }

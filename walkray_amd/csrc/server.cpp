@@ -42,6 +42,7 @@ struct Job {
     float *sum = nullptr, *comp = nullptr, *sumsq = nullptr, *contrib = nullptr;
     uint32_t* pass_done = nullptr;
     uint64_t film_bytes = 0, window_bytes = 0;
+    uint64_t total = 0;  // camera paths of the job (spp x pixels)
 };
 
 }  // namespace
@@ -56,6 +57,7 @@ struct wr_server {
     std::deque<uint64_t> queue;
     std::vector<int64_t> slot_job;  // engine job slot -> job id (-1 free)
     uint64_t next_id = 1;
+    bool tail = false;  // every running job had issued all its paths at the last read-back
     std::chrono::steady_clock::time_point t0;
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
@@ -195,6 +197,7 @@ static hipError_t admit_one(wr_server* s, Job& j, int slot) {
     J.pass0 = 0;
     J.npass = j.desc.spp;
     J.npix = npix;
+    j.total = (uint64_t)j.desc.spp * npix;
     J.R = j.R;
     J.contrib = j.contrib;
     J.pass_done = j.pass_done;
@@ -213,6 +216,7 @@ static hipError_t admit_one(wr_server* s, Job& j, int slot) {
 int wr_server_step(wr_server* s, uint32_t* n_active) {
     if (!s) return fail(nullptr, WR_EINVAL, "wr_server_step: server is NULL");
     hipStream_t st = s->ctx->stream;
+    bool admitted = false;
     // admission (first come, first served) against the slot limit and the budget
     while (!s->queue.empty()) {
         Job& j = s->jobs[s->queue.front()];
@@ -227,14 +231,21 @@ int wr_server_step(wr_server* s, uint32_t* n_active) {
         if (e != hipSuccess) return hip_fail(s->ctx, e, "wr_server_step: admit");
         s->slot_job[slot] = (int64_t)s->queue.front();
         s->queue.pop_front();
+        admitted = true;
     }
     uint32_t running = 0;
     for (uint32_t k = 0; k < kMaxJobs; k++) running += s->slot_job[k] >= 0;
     if (running) {
         std::vector<JobCtl> hc(kMaxJobs);
-        hipError_t e = s->engine.step(st, hc.data(), nullptr);
+        // short replays once only draining is left (nothing queued or newly admitted)
+        hipError_t e = s->engine.step(st, hc.data(), nullptr, s->tail && !admitted && s->queue.empty());
         if (e != hipSuccess) return hip_fail(s->ctx, e, "wr_server_step: engine");
         const double t = s->now();
+        s->tail = true;
+        for (uint32_t k = 0; k < kMaxJobs; k++)
+            if (s->slot_job[k] >= 0 && hc[k].state != kJobDone &&
+                hc[k].next_work < s->jobs[(uint64_t)s->slot_job[k]].total)
+                s->tail = false;
         for (uint32_t k = 0; k < kMaxJobs; k++) {
             if (s->slot_job[k] < 0 || hc[k].state != kJobDone) continue;
             Job& j = s->jobs[(uint64_t)s->slot_job[k]];

@@ -29,6 +29,7 @@ constexpr uint32_t kMaxPassWindow = 64;  // passes of one job whose samples may 
 constexpr uint32_t kMaxJobs = 64;        // concurrently admitted jobs
 constexpr uint32_t kMaxScenes = 64;      // resident scenes of one engine
 constexpr int kGraphIters = 8;           // wavefront iterations per graph replay (even)
+constexpr int kTailIters = 2;            // iterations per replay once every job has issued all its paths (even)
 constexpr uint32_t kDefaultPathSlots = 1u << 22;  // 4M paths in flight: 2 x 416 MB of path state
 
 enum : uint32_t { kJobFree = 0, kJobRunning = 1, kJobDone = 2 };
@@ -64,6 +65,9 @@ struct EngineCounters {
     uint32_t seg_job[kMaxJobs], seg_base[kMaxJobs], seg_take[kMaxJobs];
     uint64_t seg_work[kMaxJobs];
     unsigned long long trace_work[2];  // k_wf_trace: path batches handed out, waves finished (self-resetting)
+    uint32_t nfold;                    // jobs with complete passes to fold this iteration
+    uint32_t fold_job[kMaxJobs];       // those jobs, in job order
+    uint32_t fold_k[kMaxJobs];         // per job: complete passes from its fold cursor
 };
 
 struct WaveStats {
@@ -79,8 +83,8 @@ struct Engine {
     JobDesc* jobs = nullptr;
     JobCtl* ctl = nullptr;
     SceneDev* scenes = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr, tail_graph = nullptr;
+    hipGraphExec_t exec = nullptr, tail_exec = nullptr;
     std::vector<void*> bufs;
     ~Engine();
     void release();
@@ -88,7 +92,9 @@ struct Engine {
     hipError_t set_scene(uint32_t slot, const SceneDev& S, hipStream_t st);
     hipError_t admit(uint32_t slot, const JobDesc& J, hipStream_t st);
     hipError_t release_slot(uint32_t slot, hipStream_t st);
-    hipError_t step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt);
+    // One replay: kGraphIters iterations, or kTailIters when `tail` (every
+    // running job has issued all its camera paths, so only draining is left).
+    hipError_t step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt, bool tail = false);
 };
 
 void fill_job_camera(JobDesc& J, const CamHost& c);
