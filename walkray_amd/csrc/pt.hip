@@ -74,6 +74,7 @@ struct EngineDev {
     const wr::JobDesc* jobs;
     wr::JobCtl* ctl;
     const SceneDev* scenes;
+    uint32_t* exit_list;
     uint32_t cap;
 };
 
@@ -303,14 +304,24 @@ __global__ __launch_bounds__(256) void k_wf_finish(EngineDev E, uint32_t par) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_wf_shade(EngineDev E, uint32_t par) {
+// shade: scatter + Russian roulette of every path whose ray hit (MCRenderer::
+// samplePath, mcRenderer.cpp:30-41), survivors appended to the next buffer.
+// EXITS = false shades every live path except refraction exits, whose scatter
+// needs findInsideObject (two closest-hit queries per candidate object): those
+// are listed and shaded by the EXITS = true instance, so the traversal code
+// and its registers stay out of the common case and the queries run on
+// converged waves.  The list order only changes which slots the survivors take.
+template <bool EXITS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXITS ? 1 : 4))) void k_wf_shade(EngineDev E,
+                                                                                                     uint32_t par) {
     const PathBuf& A = E.buf[par];
     const PathBuf& B = E.buf[par ^ 1];
-    const uint32_t n = E.cnt->n[par];
+    const uint32_t n = EXITS ? E.cnt->n_exit : E.cnt->n[par];
     StackSpill spill;
     LaneStack st(&spill);
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        if (A.flags[i] & kDead) continue;
+    for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+        const uint32_t i = EXITS ? E.exit_list[k] : k;
+        if (!EXITS && (A.flags[i] & kDead)) continue;
         const uint32_t jb = A.job[i];
         const wr::JobDesc& J = E.jobs[jb];
         const SceneCtx sc = make_scene_ctx(E.scenes[J.scene]);
@@ -322,9 +333,13 @@ __global__ __launch_bounds__(256) void k_wf_shade(EngineDev E, uint32_t par) {
         h.obj = A.hobj[i];
         const uint32_t inside = A.inside[i], depth = A.depth[i], p = A.pixel[i], ps = A.pass[i];
         const uint32_t self = inside != WR_NONE ? inside : h.obj;
+        if (!EXITS && scatter_queries_inside(sc, self, h, inside)) {
+            E.exit_list[wave_append(&E.cnt->n_exit)] = i;
+            continue;
+        }
         float rn[4];
         rng4(J.seed, p, J.pass0 + ps, depth + 1, rn);
-        const Out out = scatter(sc, self, o, d, h, inside, rn, st);
+        const Out out = scatter<EXITS>(sc, self, o, d, h, inside, rn, st);
         f3 T = mk(A.tx[i], A.ty[i], A.tz[i]);
         if ((double)len(out.d) < 0.5) {  // emission / absorption ends the path
             finish(J, job_cam(J), p, ps, mulv(T, divs(divs(out.rad, out.op), out.dp)));
@@ -440,6 +455,7 @@ __global__ __launch_bounds__(64) void k_wf_plan(EngineDev E, uint32_t par) {
         c.seg_end = base0 + all;
         c.n[par ^ 1] = base0 + all;
         c.rr = (rr + 1) % wr::kMaxJobs;
+        c.n_exit = 0;  // the exit list of this iteration is consumed
     }
 }
 
@@ -602,6 +618,7 @@ hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st) {
     if (!(jobs = (JobDesc*)alloc(sizeof(JobDesc) * kMaxJobs))) return hipErrorOutOfMemory;
     if (!(ctl = (JobCtl*)alloc(sizeof(JobCtl) * kMaxJobs))) return hipErrorOutOfMemory;
     if (!(scenes = (SceneDev*)alloc(sizeof(SceneDev) * kMaxScenes))) return hipErrorOutOfMemory;
+    if (!(exit_list = (uint32_t*)alloc(sizeof(uint32_t) * (size_t)cap))) return hipErrorOutOfMemory;
     WR_CHECK(hipMemsetAsync(cnt, 0, sizeof(EngineCounters), st));
     WR_CHECK(hipMemsetAsync(jobs, 0, sizeof(JobDesc) * kMaxJobs, st));
     WR_CHECK(hipMemsetAsync(ctl, 0, sizeof(JobCtl) * kMaxJobs, st));
@@ -614,6 +631,7 @@ hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st) {
     E.jobs = jobs;
     E.ctl = ctl;
     E.scenes = scenes;
+    E.exit_list = exit_list;
     E.cap = cap;
     // the persistent trace kernel runs as many workgroups as fit at once
     int dev = 0, cus = 0, per_cu = 0;
@@ -629,7 +647,8 @@ hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st) {
             const uint32_t par = (uint32_t)(it & 1);
             k_wf_trace<<<trace_grid, 256, kStackLdsBytes, st>>>(E, par);
             k_wf_finish<<<grid, 256, kStackLdsBytes, st>>>(E, par);
-            k_wf_shade<<<grid, 256, kStackLdsBytes, st>>>(E, par);
+            k_wf_shade<false><<<grid, 256, kStackLdsBytes, st>>>(E, par);
+            k_wf_shade<true><<<grid, 256, kStackLdsBytes, st>>>(E, par);
             k_wf_plan<<<1, 64, 0, st>>>(E, par);
             k_wf_generate<<<grid, 256, 0, st>>>(E, par);
             k_wf_fold<<<grid, 256, 0, st>>>(E);

@@ -159,12 +159,30 @@ __device__ __forceinline__ f3 mirror(f3 N, f3 d) {  // -N.dot(d)*N*2 + d, normal
     return normalize(add(muls(muls(N, -dot(N, d)), 2.f), d));
 }
 
+// The object that scatters a ray from in_inside hitting h (refractiveObj.h:32-33:
+// inside a refractive object and hitting another, that other object scatters).
+__device__ __forceinline__ uint32_t scatter_object(const SceneCtx& sc, uint32_t self, const PHit& h,
+                                                   uint32_t in_inside) {
+    if (sc.S.objs[self].mat == WR_MAT_REFRACTIVE && h.obj != self && in_inside == self) return h.obj;
+    return self;
+}
+
+// Whether scatter needs Scene::findInsideObject: a refractive object's scatter
+// of a ray leaving it (refractiveObj.h:43-44).
+__device__ __forceinline__ bool scatter_queries_inside(const SceneCtx& sc, uint32_t self, const PHit& h,
+                                                       uint32_t in_inside) {
+    const uint32_t s = scatter_object(sc, self, h, in_inside);
+    return sc.S.objs[s].mat == WR_MAT_REFRACTIVE && in_inside == s;
+}
+
+// INSIDE = false compiles the findInsideObject query out, for callers that
+// never pass a ray for which scatter_queries_inside holds.
+template <bool INSIDE = true>
 __device__ Out scatter(const SceneCtx& sc, uint32_t self, f3 in_o, f3 in_d, const PHit& h, uint32_t in_inside,
                        const float* rn, LaneStack& st) {
     const f3 pos = add(in_o, muls(in_d, h.t));
     const f3 N = rec_normal(sc, h.rec, h.obj, pos);
-    // refractiveObj.h:32-33: inside this refractive object and hitting another -> that object scatters
-    if (sc.S.objs[self].mat == WR_MAT_REFRACTIVE && h.obj != self && in_inside == self) self = h.obj;
+    self = scatter_object(sc, self, h, in_inside);
     const wr_object_gpu& ob = sc.S.objs[self];
     Out out;
     out.o = pos;
@@ -224,7 +242,7 @@ __device__ Out scatter(const SceneCtx& sc, uint32_t self, f3 in_o, f3 in_d, cons
         const f3 refl = mirror(N, in_d);
         const float theta = acosf(dot(in_d, N));
         uint32_t cur_out = self;
-        if (in_inside == self) cur_out = find_inside(sc.S, out.o, in_d, self, st);
+        if (INSIDE && in_inside == self) cur_out = find_inside(sc.S, out.o, in_d, self, st);
         const float cn = in_inside != WR_NONE ? sc.S.objs[in_inside].ior : 1.f;
         const float nn = cur_out != WR_NONE ? sc.S.objs[cur_out].ior : 1.f;
         const float sin_phi = cn / nn * sinf(theta);

@@ -65,6 +65,7 @@ struct EngineCounters {
     uint32_t seg_job[kMaxJobs], seg_base[kMaxJobs], seg_take[kMaxJobs];
     uint64_t seg_work[kMaxJobs];
     unsigned long long trace_work[2];  // k_wf_trace: path batches handed out, waves finished (self-resetting)
+    uint32_t n_exit;                   // paths whose shading waits for findInsideObject (k_wf_shade)
     uint32_t nfold;                    // jobs with complete passes to fold this iteration
     uint32_t fold_job[kMaxJobs];       // those jobs, in job order
     uint32_t fold_k[kMaxJobs];         // per job: complete passes from its fold cursor
@@ -83,6 +84,7 @@ struct Engine {
     JobDesc* jobs = nullptr;
     JobCtl* ctl = nullptr;
     SceneDev* scenes = nullptr;
+    uint32_t* exit_list = nullptr;  // path slots of the deferred refraction exits (cap entries)
     hipGraph_t graph = nullptr, tail_graph = nullptr;
     hipGraphExec_t exec = nullptr, tail_exec = nullptr;
     std::vector<void*> bufs;
