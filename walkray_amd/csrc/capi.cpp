@@ -509,6 +509,7 @@ int wr_scene_build(wr_scene* s, int strategy) {
         s->dbufs.push_back(d_nodes);
         if ((rc = upload(ctx, wnodes, &d_wnodes, bytes))) return rc;
         s->dbufs.push_back(d_wnodes);
+        tris.push_back(wr_tri{});  // padding: the fast pass reads 16 B past the record it tests
         if ((rc = upload(ctx, tris, &d_tris, bytes))) return rc;
         s->dbufs.push_back(d_tris);
         if ((rc = upload(ctx, leaf_up, &d_lu, bytes))) return rc;

@@ -400,10 +400,8 @@ __device__ __forceinline__ void visit_leaf(const Tree& tr, const RayPre& r, cons
 // leaf key is the record's reference leaf (rec_first), looked up only for a
 // tie or an accepted hit.
 template <bool NOSELF, bool COUNT>
-__device__ __forceinline__ void visit_tri(const Tree& tr, const RayPre& r, const NoSelf& ns, uint32_t ref,
-                                          HitState& h, Counters& cnt) {
-    const uint32_t rec = ref & WR_FIRST_MASK;
-    const wr_tri T = tr.tris[rec];
+__device__ __forceinline__ void visit_tri_rec(const Tree& tr, const RayPre& r, const NoSelf& ns, uint32_t rec,
+                                              const wr_tri& T, HitState& h, Counters& cnt) {
     if (COUNT) cnt.tris++;
     const float t = ref_tri(T, r.o, r.d, h.t, false);
     if (!(t >= 0.f)) return;
@@ -511,13 +509,12 @@ __device__ __forceinline__ uint32_t next_or_end(LaneStack& st, float tcut) {
 // first, and return the nearest one, else the next stack entry, else WR_NONE
 // (pass ended).  Sets overflow (and returns WR_NONE) when the stack is full:
 // the caller then re-runs the ray on the binary tree.
+// The node is four 16 B words: origin[3] scale[3] | q rows (one dword per
+// bound, child k in byte k) | ref[4].
 template <bool COUNT>
-__device__ __forceinline__ uint32_t wide_node(const Tree& tr, const RayPre& r, LaneStack& st, float best_t,
-                                              uint32_t ref, bool& overflow, Counters& cnt) {
-    // the node as four 16 B loads: origin[3] scale[3] | q rows (one dword per
-    // bound, child k in byte k) | ref[4]
-    const uint4* np = reinterpret_cast<const uint4*>(tr.wnodes + ref);
-    const uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+__device__ __forceinline__ uint32_t wide_node(const RayPre& r, LaneStack& st, float best_t, const uint4& n0,
+                                              const uint4& n1, const uint4& n2, const uint4& n3, bool& overflow,
+                                              Counters& cnt) {
     if (COUNT) cnt.nodes += 2;  // node bytes in 32 B units, as for a binary node
     const float tcut = prune_cut(r, best_t);
     const float ox = __uint_as_float(n0.x), oy = __uint_as_float(n0.y), oz = __uint_as_float(n0.z);
@@ -564,14 +561,32 @@ __device__ __forceinline__ uint32_t wide_node(const Tree& tr, const RayPre& r, L
 // Visit `ref` (a wide node or a leaf) and pick the next reference; false when
 // the pass has ended (stack exhausted) or the stack would overflow (overflow
 // set: the caller re-runs the ray on the binary tree).
+//
+// The record of the step is fetched before the lanes part ways: the wide node
+// (64 B) at a node, the triangle record (48 B, and 16 B beyond it, which the
+// device array pads past its last record) at a leaf, with the same four
+// loads.  A step where some lanes sit at nodes and others at triangles then
+// waits for memory once instead of once per branch.
 template <bool NOSELF, bool COUNT>
 __device__ __forceinline__ bool wide_step(const Tree& tr, const RayPre& r, const NoSelf& ns, LaneStack& st,
                                           HitState& h, uint32_t& ref, bool& overflow, Counters& cnt) {
-    if (ref & WR_LEAF_BIT) {  // always a single triangle in the wide hierarchy
-        visit_tri<NOSELF, COUNT>(tr, r, ns, ref, h, cnt);
+    const bool leaf = (ref & WR_LEAF_BIT) != 0;  // always a single triangle in the wide hierarchy
+    const uint32_t rec = ref & WR_FIRST_MASK;
+    const uint4* p = leaf ? reinterpret_cast<const uint4*>(tr.tris + rec) : reinterpret_cast<const uint4*>(tr.wnodes + ref);
+    const uint4 d0 = p[0], d1 = p[1], d2 = p[2], d3 = p[3];
+    // keeps the fourth load ahead of the branch (it would otherwise sink into
+    // the node branch and wait on its own)
+    asm volatile("" ::"v"(d3.x));
+    if (leaf) {
+        wr_tri T;
+        T.v0[0] = __uint_as_float(d0.x); T.v0[1] = __uint_as_float(d0.y); T.v0[2] = __uint_as_float(d0.z);
+        T.v1[0] = __uint_as_float(d0.w); T.v1[1] = __uint_as_float(d1.x); T.v1[2] = __uint_as_float(d1.y);
+        T.v2[0] = __uint_as_float(d1.z); T.v2[1] = __uint_as_float(d1.w); T.v2[2] = __uint_as_float(d2.x);
+        T.n[0] = __uint_as_float(d2.y); T.n[1] = __uint_as_float(d2.z); T.n[2] = __uint_as_float(d2.w);
+        visit_tri_rec<NOSELF, COUNT>(tr, r, ns, rec, T, h, cnt);
         return st.next(prune_cut(r, h.t), ref);
     }
-    ref = wide_node<COUNT>(tr, r, st, h.t, ref, overflow, cnt);
+    ref = wide_node<COUNT>(r, st, h.t, d0, d1, d2, d3, overflow, cnt);
     return ref != WR_NONE;
 }
 
