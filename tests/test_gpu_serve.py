@@ -56,6 +56,25 @@ def test_alone_equals_mix_equals_render(ctx):
     assert all(s["state"] == W.JOB_DONE and s["done_s"] >= s["admit_s"] >= s["submit_s"] for s in st)
 
 
+def test_small_pool_mix_equals_render(ctx):
+    """A path pool smaller than one pass of most jobs: every iteration's slot
+    allocation across the running jobs is limited by the free room."""
+    scs, built = _scenes(ctx)
+    srv = W.Server(ctx, path_slots=2048)
+    ids = [srv.add_scene(b) for b in built]
+    cams = [V.job_camera(scs[s][1], res) for s, res, *_ in JOBS]
+    mix = [srv.submit(ids[s], cams[k], spp, depth, seed) for k, (s, res, spp, depth, seed) in enumerate(JOBS)]
+    _drain(srv, mix)
+    for k, (s, res, spp, depth, seed) in enumerate(JOBS):
+        got = srv.read(mix[k], res, res)
+        film = W.Film(ctx, res, res)
+        W.render(built[s], cams[k], film, spp, depth, seed, W.WR_MEGAKERNEL)
+        ref = film.read()
+        for key in ("sum", "comp", "sumsq"):
+            assert np.array_equal(ref[key].view(np.uint32), got[key].view(np.uint32)), (k, key)
+    srv.close()
+
+
 def test_budget_and_slots(ctx):
     scs, built = _scenes(ctx)
     cam = V.job_camera(scs[0][1], 64)
