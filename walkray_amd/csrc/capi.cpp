@@ -729,6 +729,7 @@ int wr_autotune_closest_hit(wr_ctx* ctx, wr_scene* s, const wr_ray* d_rays, uint
     const uint32_t nodes[] = {16, 24, 32, 40, 48, 65};
     const uint32_t refills[] = {4, 8, 16, 24, 32};
     const uint32_t blocks[] = {0, 7, 6, 5};
+    const uint32_t levels[] = {4, 8, 16};
     for (int pass = 0; pass < 2 && e == hipSuccess; pass++) {
         for (uint32_t v : nodes) {
             ChTune t = cur;
@@ -753,16 +754,28 @@ int wr_autotune_closest_hit(wr_ctx* ctx, wr_scene* s, const wr_ray* d_rays, uint
             if ((e = time_of(t, m)) != hipSuccess) break;
             if (m < 0.99f * cur_ms) cur = t, cur_ms = m;
         }
+        for (uint32_t v : levels) {
+            if (e != hipSuccess) break;
+            ChTune t = cur;
+            t.lds_levels = v;
+            float m;
+            if ((e = time_of(t, m)) != hipSuccess) break;
+            if (m < 0.99f * cur_ms) cur = t, cur_ms = m;
+        }
     }
     (void)hipFree(d_hits);
     if (e != hipSuccess) return hip_fail(ctx, e, "wr_autotune_closest_hit");
     s->ch_tune = cur;
-    if (best) *best = wr_closest_hit_tuning{cur.refill_min, cur.node_phase_min, cur.blocks_per_cu, cur_ms};
+    if (best) *best = wr_closest_hit_tuning{cur.refill_min, cur.node_phase_min, cur.blocks_per_cu, cur.lds_levels, cur_ms};
     return WR_OK;
 }
 
 int wr_scene_set_closest_hit_tuning(wr_scene* s, const wr_closest_hit_tuning* t) {
     if (!s || !t) return fail(nullptr, WR_EINVAL, "wr_scene_set_closest_hit_tuning: bad argument");
+    const uint32_t lv = t->lds_levels ? t->lds_levels : 8u;
+    if (lv != 4 && lv != 8 && lv != 16)
+        return fail(s->ctx, WR_EINVAL, "wr_scene_set_closest_hit_tuning: lds_levels must be 4, 8 or 16");
+    s->ch_tune.lds_levels = lv;
     s->ch_tune.refill_min = t->refill_min;
     s->ch_tune.node_phase_min = t->node_phase_min;
     s->ch_tune.blocks_per_cu = t->blocks_per_cu;

@@ -65,13 +65,14 @@ __device__ __forceinline__ void add_counters(unsigned long long* out, const Coun
 // once at least refill_min are idle; a wave runs node-only steps while at
 // least node_phase_min lanes sit at wide nodes (a mixed step pays for both the
 // node code and the triangle code).
-template <bool COUNT>
+// D: stack levels in LDS (the autotuner's stack-depth knob).
+template <bool COUNT, int D>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_closest_hit(
     SceneDev S, wr_tree_hdr H, const float* __restrict__ rays, uint64_t n, uint4* __restrict__ hits,
     unsigned long long* counters, unsigned long long* work, uint32_t refill_min, uint32_t node_phase_min) {
     const Tree tr = make_tree(H, S);
-    StackSpill spill;
-    LaneStack st(&spill);
+    StackSpillT<D> spill;
+    LaneStackT<D> st(&spill);
     Counters cnt{0, 0, 0};
     const NoSelf ns{};
     const uint32_t lane = threadIdx.x & 63u;
@@ -142,15 +143,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 // Completion of k_closest_hit's records: binary fast pass after a wide-stack
 // overflow, chain verification, exact fallback, barycentrics, output.  One
 // lane per ray, so the verification walks of a wave run together.
-template <bool COUNT>
+template <bool COUNT, int D>
 __global__ __launch_bounds__(256) void k_closest_hit_finish(SceneDev S, wr_tree_hdr H,
                                                             const float* __restrict__ rays, uint64_t n,
                                                             uint4* __restrict__ hits, unsigned long long* counters) {
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
     const uint64_t i = (uint64_t)b * 256 + threadIdx.x;
     Counters cnt{0, 0, 0};
-    StackSpill spill;
-    LaneStack st(&spill);
+    StackSpillT<D> spill;
+    LaneStackT<D> st(&spill);
     if (i < n) {
         const uint4 f = hits[i];
         HitState h{__uint_as_float(f.y), f.x, f.z};
@@ -223,15 +224,17 @@ __global__ __launch_bounds__(256) void k_inside_object(SceneDev S, const uint32_
 
 namespace wr {
 
-// Workgroups of the persistent kernel: per_cu per CU (0: as many as fit).
+// Workgroups of the persistent kernel with D LDS stack levels: per_cu per CU
+// (0: as many as fit).
+template <int D>
 static uint32_t persistent_blocks(uint64_t n, uint32_t per_cu_want) {
     static int cus = 0, per_cu_max = 0;
     if (!cus) {
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_max, wrd::k_closest_hit<false>, 256,
-                                                         wrd::kStackLdsBytes) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_max, wrd::k_closest_hit<false, D>, 256,
+                                                         wrd::stack_lds_bytes<D>()) != hipSuccess ||
             cus <= 0 || per_cu_max <= 0) {
             cus = 256;
             per_cu_max = 1;
@@ -243,25 +246,38 @@ static uint32_t persistent_blocks(uint64_t n, uint32_t per_cu_want) {
     return (uint32_t)(need < resident ? need : resident);
 }
 
+template <int D>
+static hipError_t launch_closest_hit_d(const SceneDev& S, const wr_tree_hdr& H, const float* rays, uint64_t n,
+                                       void* hits, unsigned long long* counters, unsigned long long* work,
+                                       uint32_t per_cu, uint32_t rmin, uint32_t nmin, hipStream_t st) {
+    const uint32_t pgrid = persistent_blocks<D>(n, per_cu);
+    const uint32_t grid = (uint32_t)((n + 255) / 256);
+    const size_t lds = wrd::stack_lds_bytes<D>();
+    if (counters) {
+        wrd::k_closest_hit<true, D><<<pgrid, 256, lds, st>>>(S, H, rays, n, (uint4*)hits, counters, work, rmin, nmin);
+        wrd::k_closest_hit_finish<true, D><<<grid, 256, lds, st>>>(S, H, rays, n, (uint4*)hits, counters);
+    } else {
+        wrd::k_closest_hit<false, D><<<pgrid, 256, lds, st>>>(S, H, rays, n, (uint4*)hits, nullptr, work, rmin, nmin);
+        wrd::k_closest_hit_finish<false, D><<<grid, 256, lds, st>>>(S, H, rays, n, (uint4*)hits, nullptr);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_closest_hit(const SceneDev& S, const wr_tree_hdr& H, const float* rays, uint64_t n, void* hits,
                               unsigned long long* counters, unsigned long long* work, const ChTune& tune,
                               hipStream_t st) {
     if (n == 0) return hipSuccess;
-    const uint32_t pgrid = persistent_blocks(n, tune.blocks_per_cu);
-    const uint32_t grid = (uint32_t)((n + 255) / 256);
     // refill_min in [1, 64]; node_phase_min in [1, 65] (0 would never leave the node phase; 65 disables it)
     const uint32_t rmin = tune.refill_min < 1 ? 1 : (tune.refill_min > 64 ? 64 : tune.refill_min);
     const uint32_t nmin = tune.node_phase_min < 1 ? 1 : (tune.node_phase_min > 65 ? 65 : tune.node_phase_min);
-    if (counters) {
-        wrd::k_closest_hit<true><<<pgrid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, counters, work,
-                                                                          rmin, nmin);
-        wrd::k_closest_hit_finish<true><<<grid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, counters);
-    } else {
-        wrd::k_closest_hit<false><<<pgrid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, nullptr, work,
-                                                                           rmin, nmin);
-        wrd::k_closest_hit_finish<false><<<grid, 256, wrd::kStackLdsBytes, st>>>(S, H, rays, n, (uint4*)hits, nullptr);
+    switch (tune.lds_levels) {
+        case 4:
+            return launch_closest_hit_d<4>(S, H, rays, n, hits, counters, work, tune.blocks_per_cu, rmin, nmin, st);
+        case 16:
+            return launch_closest_hit_d<16>(S, H, rays, n, hits, counters, work, tune.blocks_per_cu, rmin, nmin, st);
+        default:
+            return launch_closest_hit_d<8>(S, H, rays, n, hits, counters, work, tune.blocks_per_cu, rmin, nmin, st);
     }
-    return hipGetLastError();
 }
 
 hipError_t launch_closest_hit_path(const SceneDev& S, const wr_tree_hdr& H, const uint32_t* rays, uint64_t n,

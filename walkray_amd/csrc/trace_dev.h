@@ -295,57 +295,61 @@ struct HitState {
     uint32_t first;  // first record of the winner's leaf (tie order)
 };
 
-// Per-lane stack of (reference, entry distance) pairs: the first kLdsDepth
-// levels live in LDS (lane-interleaved, stk[level * kBlock + lane]); deeper
-// levels spill to a private array (scratch), which only very deep traversals
-// touch.  An entry whose entry distance exceeds the current cut when it is
-// popped is dropped unvisited: that is the slab test it would fail anyway.
-// Eight LDS levels keep a 256-lane workgroup at 16 KB of LDS, so LDS never
-// holds occupancy below 8 waves per SIMD (traversal is latency-bound: waves
-// in flight matter more than the occasional spilled entry).
+// Per-lane stack of (reference, entry distance) pairs: the first D levels
+// live in LDS (lane-interleaved, stk[level * kBlock + lane]); deeper levels
+// spill to a private array (scratch), which only deep traversals touch.  An
+// entry whose entry distance exceeds the current cut when it is popped is
+// dropped unvisited: that is the slab test it would fail anyway.  D is a
+// launch choice of the closest-hit kernels (the autotuner's stack-depth knob:
+// 4, 8 or 16 levels, 8 KB to 32 KB of LDS per 256-lane workgroup); the other
+// traversal kernels use kLdsDepth = 8, 16 KB, which never holds occupancy
+// below 8 waves per SIMD.  Which entries sit in LDS never changes a result.
 constexpr int kLdsDepth = 8;
 constexpr int kMaxDepth = 64;
 constexpr int kBlock = 256;  // every traversal kernel runs 256-lane workgroups
 
-// Dynamic LDS of the launching kernel: 2 * kLdsDepth * kBlock words
-// (references, then entry distances).
+// Dynamic LDS of the launching kernel: 2 * D * kBlock words (references, then
+// entry distances).
 extern __shared__ uint32_t wr_lds_stack[];
 
 // The spill levels are a separate private array of the launching kernel: kept
 // inside LaneStack, the dynamically indexed arrays would put the whole struct,
 // sp and lane included, in scratch, and every traversal step would start with
 // a scratch load of sp before its node fetch.
-struct StackSpill {
-    uint32_t ref[kMaxDepth - kLdsDepth];
-    float t[kMaxDepth - kLdsDepth];
+template <int D>
+struct StackSpillT {
+    uint32_t ref[kMaxDepth - D];
+    float t[kMaxDepth - D];
 };
 
-struct LaneStack {
-    StackSpill* spill;
+template <int D>
+struct LaneStackT {
+    static_assert(D > 0 && (D & (D - 1)) == 0 && D < kMaxDepth, "LDS levels: a power of two below kMaxDepth");
+    StackSpillT<D>* spill;
     uint32_t lane;  // threadIdx.x
     int sp;
-    __device__ explicit LaneStack(StackSpill* s) : spill(s), lane(threadIdx.x), sp(0) {}
+    __device__ explicit LaneStackT(StackSpillT<D>* s) : spill(s), lane(threadIdx.x), sp(0) {}
     // The LDS accesses are unconditional (a wrapped slot when spilling) so they
     // stay ds_read/ds_write instead of being merged with the scratch access
     // into a generic (flat) one.
     __device__ __forceinline__ void push(uint32_t v, float t) {
-        const uint32_t slot = (sp & (kLdsDepth - 1)) * kBlock + lane;
-        if (sp >= kLdsDepth) {
-            spill->ref[sp - kLdsDepth] = wr_lds_stack[slot];
-            spill->t[sp - kLdsDepth] = __uint_as_float(wr_lds_stack[kLdsDepth * kBlock + slot]);
+        const uint32_t slot = (sp & (D - 1)) * kBlock + lane;
+        if (sp >= D) {
+            spill->ref[sp - D] = wr_lds_stack[slot];
+            spill->t[sp - D] = __uint_as_float(wr_lds_stack[D * kBlock + slot]);
         }
         wr_lds_stack[slot] = v;
-        wr_lds_stack[kLdsDepth * kBlock + slot] = __float_as_uint(t);
+        wr_lds_stack[D * kBlock + slot] = __float_as_uint(t);
         sp++;
     }
     __device__ __forceinline__ uint32_t pop(float& t) {
         --sp;
-        const uint32_t slot = (sp & (kLdsDepth - 1)) * kBlock + lane;
+        const uint32_t slot = (sp & (D - 1)) * kBlock + lane;
         const uint32_t v = wr_lds_stack[slot];
-        t = __uint_as_float(wr_lds_stack[kLdsDepth * kBlock + slot]);
-        if (sp >= kLdsDepth) {
-            wr_lds_stack[slot] = spill->ref[sp - kLdsDepth];
-            wr_lds_stack[kLdsDepth * kBlock + slot] = __float_as_uint(spill->t[sp - kLdsDepth]);
+        t = __uint_as_float(wr_lds_stack[D * kBlock + slot]);
+        if (sp >= D) {
+            wr_lds_stack[slot] = spill->ref[sp - D];
+            wr_lds_stack[D * kBlock + slot] = __float_as_uint(spill->t[sp - D]);
         }
         return v;
     }
@@ -363,7 +367,13 @@ struct LaneStack {
         return false;
     }
 };
-constexpr size_t kStackLdsBytes = 2 * sizeof(uint32_t) * kLdsDepth * kBlock;
+using StackSpill = StackSpillT<kLdsDepth>;
+using LaneStack = LaneStackT<kLdsDepth>;
+template <int D>
+constexpr size_t stack_lds_bytes() {
+    return 2 * sizeof(uint32_t) * D * kBlock;
+}
+constexpr size_t kStackLdsBytes = stack_lds_bytes<kLdsDepth>();
 
 // Distance cut of the t-pruning: boxes entered beyond it cannot hold a
 // triangle whose reference t is <= best t (relative margin 2^-10 plus 4 e).
@@ -417,8 +427,8 @@ __device__ __forceinline__ void visit_tri_rec(const Tree& tr, const RayPre& r, c
 
 // Ordered traversal of the reference binary tree.  EXACT = false: the fast
 // conservative pass; EXACT = true: every box also passes the reference test.
-template <bool NOSELF, bool EXACT, bool COUNT>
-__device__ __forceinline__ void traverse(const Tree& tr, const RayPre& r, const NoSelf& ns, LaneStack& st,
+template <bool NOSELF, bool EXACT, bool COUNT, class Stack>
+__device__ __forceinline__ void traverse(const Tree& tr, const RayPre& r, const NoSelf& ns, Stack& st,
                                          HitState& h, bool& overflow, Counters& cnt) {
     h.t = __builtin_inff();
     h.rec = WR_NONE;
@@ -487,7 +497,8 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ra, float& tb, uint32
 //
 // The pass is split into wide_begin / wide_step so a persistent kernel can
 // interleave the steps of many rays on one lane.
-__device__ __forceinline__ bool wide_begin(const Tree& tr, const RayPre& r, LaneStack& st, HitState& h,
+template <class Stack>
+__device__ __forceinline__ bool wide_begin(const Tree& tr, const RayPre& r, Stack& st, HitState& h,
                                            uint32_t& ref) {
     h.t = __builtin_inff();
     h.rec = WR_NONE;
@@ -500,7 +511,8 @@ __device__ __forceinline__ bool wide_begin(const Tree& tr, const RayPre& r, Lane
 }
 
 // Next stack entry that can still hold a hit, WR_NONE when there is none.
-__device__ __forceinline__ uint32_t next_or_end(LaneStack& st, float tcut) {
+template <class Stack>
+__device__ __forceinline__ uint32_t next_or_end(Stack& st, float tcut) {
     uint32_t ref;
     return st.next(tcut, ref) ? ref : WR_NONE;
 }
@@ -511,8 +523,8 @@ __device__ __forceinline__ uint32_t next_or_end(LaneStack& st, float tcut) {
 // the caller then re-runs the ray on the binary tree.
 // The node is four 16 B words: origin[3] scale[3] | q rows (one dword per
 // bound, child k in byte k) | ref[4].
-template <bool COUNT>
-__device__ __forceinline__ uint32_t wide_node(const RayPre& r, LaneStack& st, float best_t, const uint4& n0,
+template <bool COUNT, class Stack>
+__device__ __forceinline__ uint32_t wide_node(const RayPre& r, Stack& st, float best_t, const uint4& n0,
                                               const uint4& n1, const uint4& n2, const uint4& n3, bool& overflow,
                                               Counters& cnt) {
     if (COUNT) cnt.nodes += 2;  // node bytes in 32 B units, as for a binary node
@@ -567,8 +579,8 @@ __device__ __forceinline__ uint32_t wide_node(const RayPre& r, LaneStack& st, fl
 // device array pads past its last record) at a leaf, with the same four
 // loads.  A step where some lanes sit at nodes and others at triangles then
 // waits for memory once instead of once per branch.
-template <bool NOSELF, bool COUNT>
-__device__ __forceinline__ bool wide_step(const Tree& tr, const RayPre& r, const NoSelf& ns, LaneStack& st,
+template <bool NOSELF, bool COUNT, class Stack>
+__device__ __forceinline__ bool wide_step(const Tree& tr, const RayPre& r, const NoSelf& ns, Stack& st,
                                           HitState& h, uint32_t& ref, bool& overflow, Counters& cnt) {
     const bool leaf = (ref & WR_LEAF_BIT) != 0;  // always a single triangle in the wide hierarchy
     const uint32_t rec = ref & WR_FIRST_MASK;
@@ -658,8 +670,8 @@ __device__ __forceinline__ bool verify_chain(const Tree& tr, const RayPre& r, ui
 
 // Completion of a fast-pass result: the binary fast pass when the wide stack
 // overflowed, chain verification, and the exact fallback when it fails.
-template <bool NOSELF, bool COUNT>
-__device__ __forceinline__ void finish_hit(const Tree& tr, const RayPre& r, const NoSelf& ns, LaneStack& st,
+template <bool NOSELF, bool COUNT, class Stack>
+__device__ __forceinline__ void finish_hit(const Tree& tr, const RayPre& r, const NoSelf& ns, Stack& st,
                                            Counters& cnt, HitState& h, bool overflow, bool& failed) {
     failed = false;
     if (overflow) traverse<NOSELF, false, COUNT>(tr, r, ns, st, h, overflow, cnt);
