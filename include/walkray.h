@@ -354,16 +354,23 @@ int wr_pfm_write(const char* path, const float* rgb, uint32_t width, uint32_t he
 
 /* Autotuner for the wavefront engine (no counterpart in the reference): times
  * spp-pass renders of the camera under candidate path-slot pool sizes (queue
- * capacity) and stage grids (workgroups per stage launch) and keeps the
- * fastest in the context; wr_render calls whose wave_slots / wave_grid are 0
- * use it.  Films never depend on the choice.  Synchronous. */
+ * capacity) and stage grids (workgroups per stage launch), then, at the best of
+ * those, the traversal stack levels kept in LDS by the trace stage, and keeps
+ * the fastest in the context; wr_render calls whose wave_slots / wave_grid are
+ * 0 use it (the stack levels always).  Films never depend on the choice.
+ * Synchronous. */
 typedef struct {
     uint32_t wave_slots;
     uint32_t wave_grid;
-    float ms; /* the spp-pass render with this configuration */
+    uint32_t lds_levels; /* 4, 8 or 16 */
+    float ms;            /* the spp-pass render with this configuration */
 } wr_render_tuning;
 int wr_autotune_render(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, uint32_t max_depth, uint32_t spp,
                        wr_render_tuning* best);
+/* Sets the context's wavefront configuration directly (for example one tuned
+ * earlier): wave_slots / wave_grid 0 keep the defaults, lds_levels must be 4,
+ * 8 or 16 (0 = 8); ms is ignored.  No counterpart in the reference. */
+int wr_set_render_tuning(wr_ctx* ctx, const wr_render_tuning* tuning);
 
 /* ---- serving (no counterpart in the reference: RenderEngine runs one job at
  * a time from a key loop, renderEngine.cpp:72-91) ------------------------- */

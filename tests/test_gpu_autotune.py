@@ -35,6 +35,14 @@ def test_render_tuning_keeps_films(ctx):
     a = R.render_scene(ctx, objs, cam, 6, 8, 3, W.WR_WAVEFRONT, scene=sc)
     t = W.autotune_render(sc, cam, 8, spp=1)
     assert t["wave_slots"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23) and t["wave_grid"] in (1024, 2048, 4096)
+    assert t["lds_levels"] in (4, 8, 16)
     b = R.render_scene(ctx, objs, cam, 6, 8, 3, W.WR_WAVEFRONT, scene=sc)
     for k in ("sum", "comp", "sumsq"):
         assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
+    # every stack depth of the trace stage, on a small pool and grid
+    for levels in (4, 16, 8):
+        W.set_render_tuning(ctx, 1 << 12, 1024, levels)
+        c = R.render_scene(ctx, objs, cam, 6, 8, 3, W.WR_WAVEFRONT, scene=sc)
+        for k in ("sum", "comp", "sumsq"):
+            assert np.array_equal(a[k].view(np.uint32), c[k].view(np.uint32)), (levels, k)
+    W.set_render_tuning(ctx)

@@ -82,7 +82,7 @@ class ChTuning(C.Structure):
 
 
 class RenderTuning(C.Structure):
-    _fields_ = [("wave_slots", C.c_uint32), ("wave_grid", C.c_uint32), ("ms", C.c_float)]
+    _fields_ = [("wave_slots", C.c_uint32), ("wave_grid", C.c_uint32), ("lds_levels", C.c_uint32), ("ms", C.c_float)]
 
 
 def autotune_render(scene, cam, max_depth, spp=2):
@@ -90,7 +90,13 @@ def autotune_render(scene, cam, max_depth, spp=2):
     t = RenderTuning()
     _check(lib().wr_autotune_render(scene.ctx.h, scene.h, C.byref(CameraDesc.from_dict(cam)), max_depth, spp,
                                     C.byref(t)), scene.ctx.h, "wr_autotune_render")
-    return {"wave_slots": t.wave_slots, "wave_grid": t.wave_grid, "ms": t.ms}
+    return {"wave_slots": t.wave_slots, "wave_grid": t.wave_grid, "lds_levels": t.lds_levels, "ms": t.ms}
+
+
+def set_render_tuning(ctx, wave_slots=0, wave_grid=0, lds_levels=8):
+    """wr_set_render_tuning: the context's wavefront configuration (0 = defaults)."""
+    t = RenderTuning(wave_slots, wave_grid, lds_levels, 0.0)
+    _check(lib().wr_set_render_tuning(ctx.h, C.byref(t)), ctx.h, "wr_set_render_tuning")
 
 
 class ConfigInfo(C.Structure):
@@ -162,6 +168,7 @@ def lib():
         "wr_autotune_closest_hit": (i32, [vp, vp, vp, u64, P(ChTuning)]),
         "wr_scene_set_closest_hit_tuning": (i32, [vp, P(ChTuning)]),
         "wr_autotune_render": (i32, [vp, vp, P(CameraDesc), u32, u32, P(RenderTuning)]),
+        "wr_set_render_tuning": (i32, [vp, P(RenderTuning)]),
         "wr_scene_add_mesh_corners": (i32, [vp, vp, u32, vp, u32, vp, P(Material), P(u32)]),
         "wr_config_load": (i32, [C.c_char_p, P(vp)]),
         "wr_config_free": (None, [vp]),

@@ -158,6 +158,8 @@ constexpr uint32_t kOverflow = 4u;
 constexpr uint32_t kTraceRefillMin = 16;
 constexpr uint32_t kTraceNodePhaseMin = 32;
 
+// D: traversal stack levels in LDS (4, 8 or 16; the wavefront autotuner's stack-depth knob).
+template <int D>
 __global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
     const PathBuf& A = E.buf[par];
     const uint32_t n = E.cnt->n[par];
@@ -165,8 +167,8 @@ __global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
     // workgroups beyond what n paths need leave at once (and are not counted below)
     const uint32_t used = (uint32_t)(((uint64_t)n + 255) / 256 < gridDim.x ? ((uint64_t)n + 255) / 256 : gridDim.x);
     if (blockIdx.x >= used) return;
-    StackSpill spill;
-    LaneStack st(&spill);
+    StackSpillT<D> spill;
+    LaneStackT<D> st(&spill);
     Counters cnt{0, 0, 0};
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -597,13 +599,15 @@ static PathBuf to_pathbuf(void* const* w) {
     return b;
 }
 
-hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st) {
+hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st, uint32_t want_lds) {
     if (want_grid == 0) want_grid = 2048;
+    if (want_lds != 4 && want_lds != 16) want_lds = 8;
     want_cap = std::max<uint32_t>(256, (want_cap + 255) / 256 * 256);
-    if (cap == want_cap && grid == want_grid && exec) return hipSuccess;
+    if (cap == want_cap && grid == want_grid && lds_levels == want_lds && exec) return hipSuccess;
     release();
     cap = want_cap;
     grid = want_grid;
+    lds_levels = want_lds;
     auto alloc = [&](size_t bytes) -> void* {
         void* p = nullptr;
         if (hipMalloc(&p, std::max<size_t>(bytes, 4)) != hipSuccess) return nullptr;
@@ -637,7 +641,12 @@ hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st) {
     int dev = 0, cus = 0, per_cu = 0;
     WR_CHECK(hipGetDevice(&dev));
     WR_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    WR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_wf_trace, 256, kStackLdsBytes));
+    const void* trace_fn = lds_levels == 4    ? (const void*)k_wf_trace<4>
+                           : lds_levels == 16 ? (const void*)k_wf_trace<16>
+                                              : (const void*)k_wf_trace<8>;
+    const size_t trace_lds = lds_levels == 4 ? stack_lds_bytes<4>() : lds_levels == 16 ? stack_lds_bytes<16>()
+                                                                                       : stack_lds_bytes<8>();
+    WR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_fn, 256, trace_lds));
     const uint32_t trace_grid = (uint32_t)std::max(1, std::min<int>(cus * std::max(per_cu, 1), (int)(cap / 256)));
     // the iteration graphs: kGraphIters and kTailIters iterations (even, the
     // buffer parity alternates)
@@ -645,7 +654,12 @@ hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st) {
         WR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         for (int it = 0; it < iters; it++) {
             const uint32_t par = (uint32_t)(it & 1);
-            k_wf_trace<<<trace_grid, 256, kStackLdsBytes, st>>>(E, par);
+            if (lds_levels == 4)
+                k_wf_trace<4><<<trace_grid, 256, trace_lds, st>>>(E, par);
+            else if (lds_levels == 16)
+                k_wf_trace<16><<<trace_grid, 256, trace_lds, st>>>(E, par);
+            else
+                k_wf_trace<8><<<trace_grid, 256, trace_lds, st>>>(E, par);
             k_wf_finish<<<grid, 256, kStackLdsBytes, st>>>(E, par);
             k_wf_shade<false><<<grid, 256, kStackLdsBytes, st>>>(E, par);
             k_wf_shade<true><<<grid, 256, kStackLdsBytes, st>>>(E, par);
@@ -706,12 +720,12 @@ void fill_job_camera(JobDesc& J, const CamHost& c) {
 // Renders passes [pass0, pass0 + npass) of one film on the engine, alone.
 hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
                          uint32_t npass, bool nee, const FilmBufs& film, Engine& E, uint32_t cap, uint32_t grid,
-                         WaveStats* stats, hipStream_t st) {
+                         WaveStats* stats, hipStream_t st, uint32_t lds_levels) {
     const uint32_t npix = cam.W * cam.H;
     if (npass == 0 || npix == 0) return hipSuccess;
     const uint64_t total = (uint64_t)npass * npix;
     if (cap == 0) cap = kDefaultPathSlots;
-    WR_CHECK(E.init(cap, grid, st));
+    WR_CHECK(E.init(cap, grid, st, lds_levels));
     JobDesc J{};
     fill_job_camera(J, cam);
     J.scene = 0;

@@ -181,7 +181,8 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
             else
                 e = run_wavefront(tmp.dev, ch, prm->max_depth, prm->seed, pass, 1, nee, fb, ctx->engine,
                                   prm->wave_slots ? prm->wave_slots : ctx->tuned_slots,
-                                  prm->wave_grid ? prm->wave_grid : ctx->tuned_grid, &ws, ctx->stream);
+                                  prm->wave_grid ? prm->wave_grid : ctx->tuned_grid, &ws, ctx->stream,
+                                  ctx->tuned_lds);
             if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
             scene_free_device(&tmp);
         }
@@ -192,7 +193,8 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
         } else {
             e = run_wavefront(s->dev, ch, prm->max_depth, prm->seed, film->passes, prm->spp, nee, fb, ctx->engine,
                               prm->wave_slots ? prm->wave_slots : ctx->tuned_slots,
-                              prm->wave_grid ? prm->wave_grid : ctx->tuned_grid, &ws, ctx->stream);
+                              prm->wave_grid ? prm->wave_grid : ctx->tuned_grid, &ws, ctx->stream,
+                              ctx->tuned_lds);
         }
     }
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
@@ -211,6 +213,16 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
 
 }  // extern "C"
 
+extern "C" int wr_set_render_tuning(wr_ctx* ctx, const wr_render_tuning* t) {
+    if (!ctx || !t) return fail(ctx, WR_EINVAL, "wr_set_render_tuning: bad argument");
+    const uint32_t lv = t->lds_levels ? t->lds_levels : 8u;
+    if (lv != 4 && lv != 8 && lv != 16) return fail(ctx, WR_EINVAL, "wr_set_render_tuning: lds_levels must be 4, 8 or 16");
+    ctx->tuned_slots = t->wave_slots;
+    ctx->tuned_grid = t->wave_grid;
+    ctx->tuned_lds = lv;
+    return WR_OK;
+}
+
 extern "C" int wr_autotune_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, uint32_t max_depth,
                                   uint32_t spp, wr_render_tuning* best) {
     if (!ctx || !s || !cam || spp == 0) return fail(ctx, WR_EINVAL, "wr_autotune_render: bad argument");
@@ -223,6 +235,8 @@ extern "C" int wr_autotune_render(wr_ctx* ctx, const wr_scene* s, const wr_camer
     uint32_t best_slots = 0, best_grid = 0;
     float best_ms = 1e30f;
     int last_rc = WR_OK;
+    const uint32_t lds0 = ctx->tuned_lds;
+    ctx->tuned_lds = 8;  // the pool and grid are chosen at 8 LDS levels
     for (uint32_t sl : slots) {
         for (uint32_t g : grids) {
             // the first render of a configuration also builds the engine: time the second;
@@ -244,10 +258,34 @@ extern "C" int wr_autotune_render(wr_ctx* ctx, const wr_scene* s, const wr_camer
         }
         if (last_rc != WR_OK && last_rc != WR_ENOMEM && last_rc != WR_EHIP) break;
     }
+    // then the trace stage's stack levels in LDS at that pool and grid (a move needs a 1% gain)
+    uint32_t best_lds = 8;
+    if (best_slots != 0) {
+        for (uint32_t lv : {4u, 16u}) {
+            ctx->tuned_lds = lv;
+            wr_render_params p{spp, max_depth, 1, WR_WAVEFRONT, best_slots, best_grid, 0};
+            wr_render_stats st{};
+            int r = WR_OK;
+            for (int rep = 0; rep < 2 && r == WR_OK; rep++) r = wr_render(ctx, s, cam, &p, film, &st);
+            if (r != WR_OK) {
+                last_rc = r;
+                if (r != WR_ENOMEM && r != WR_EHIP) break;
+                continue;
+            }
+            if (st.kernel_ms < 0.99f * best_ms) {
+                best_ms = st.kernel_ms;
+                best_lds = lv;
+            }
+        }
+    }
     wr_film_destroy(film);
-    if (best_slots == 0) return last_rc != WR_OK ? last_rc : fail(ctx, WR_EINVAL, "wr_autotune_render: no configuration ran");
+    if (best_slots == 0) {
+        ctx->tuned_lds = lds0;
+        return last_rc != WR_OK ? last_rc : fail(ctx, WR_EINVAL, "wr_autotune_render: no configuration ran");
+    }
     ctx->tuned_slots = best_slots;
     ctx->tuned_grid = best_grid;
-    if (best) *best = wr_render_tuning{best_slots, best_grid, best_ms};
+    ctx->tuned_lds = best_lds;
+    if (best) *best = wr_render_tuning{best_slots, best_grid, best_lds, best_ms};
     return WR_OK;
 }

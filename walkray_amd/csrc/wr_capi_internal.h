@@ -19,6 +19,7 @@ struct wr_ctx {
     unsigned long long* d_work = nullptr;  // batch counter of the persistent closest-hit kernel (self-resetting)
     wr::Engine engine;  // wavefront engine of wr_render, reused across calls
     uint32_t tuned_slots = 0, tuned_grid = 0;  // wr_autotune_render's choice (0 = defaults)
+    uint32_t tuned_lds = 8;                    // wr_autotune_render's traversal stack levels in LDS
 };
 
 namespace wr {

@@ -79,7 +79,7 @@ struct WaveStats {
 // The shared wavefront: path pool, job and scene tables, and the captured
 // iteration graph.  One per context (wr_render) or per server.
 struct Engine {
-    uint32_t cap = 0, grid = 0;
+    uint32_t cap = 0, grid = 0, lds_levels = 0;  // lds_levels: k_wf_trace's traversal stack levels in LDS
     EngineCounters* cnt = nullptr;
     JobDesc* jobs = nullptr;
     JobCtl* ctl = nullptr;
@@ -90,7 +90,7 @@ struct Engine {
     std::vector<void*> bufs;
     ~Engine();
     void release();
-    hipError_t init(uint32_t cap, uint32_t grid, hipStream_t st);
+    hipError_t init(uint32_t cap, uint32_t grid, hipStream_t st, uint32_t lds_levels = 8);
     hipError_t set_scene(uint32_t slot, const SceneDev& S, hipStream_t st);
     hipError_t admit(uint32_t slot, const JobDesc& J, hipStream_t st);
     hipError_t release_slot(uint32_t slot, hipStream_t st);
@@ -105,6 +105,6 @@ hipError_t launch_pt_mega(const SceneDev& S, const CamHost& cam, uint32_t max_de
                           bool nee, const FilmBufs& film, hipStream_t st);
 hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
                          uint32_t npass, bool nee, const FilmBufs& film, Engine& engine, uint32_t cap, uint32_t grid,
-                         WaveStats* stats, hipStream_t st);
+                         WaveStats* stats, hipStream_t st, uint32_t lds_levels = 8);
 
 }  // namespace wr
