@@ -32,6 +32,13 @@
  *   - The per-pass tree rebuild (pathTracer.cpp:18 -> scene.cpp:67-80) is not
  *     repeated: the builder is deterministic, so every pass would rebuild the
  *     identical tree.
+ *   - BidirectionalPathTracer (bidirPathTracer.cpp:10-313: eye and light
+ *     sub-paths, every connection with its MIS weight, light-tracing splats):
+ *     restated as the checker of a later GPU estimator.  Pinned as a whole by
+ *     its limit: at a depth where both estimators have converged it agrees
+ *     with the PathTracer image within the combined standard error (the two
+ *     differ at small depths because BDPT's maxPathLen bounds each sub-path,
+ *     not the joined path).  The individual MIS terms are parity unpinned.
  */
 #include "oracle.h"
 
@@ -725,6 +732,19 @@ struct Cam {
                   mul(top, ((H / 2.f) - (float)y) - v));
         return normalize(d);
     }
+    float pixelSolidAngle(unsigned p) const { /* Camera::getPixelSolidAngle (camera.cpp:124-131) */
+        unsigned x = p % W, y = p / W;
+        float xc = (float)(x + 0.5 - (W / 2.0)), yc = (float)((H / 2.0) - 0.5 - y);
+        float pixelDist = std::sqrt(xc * xc + yc * yc + dist * dist);
+        return dist / powf(pixelDist, 3);
+    }
+    void toRaster(V p, float& rx, float& ry) const { /* Camera::convToRaster (camera.cpp:148-162) */
+        V v = sub(p, pos);
+        float vFrontDist = dot(v, front);
+        float scale = dist / vFrontDist;
+        rx = (float)(W / 2.0 + dot(v, right) * scale);
+        ry = (float)(H / 2.0 - dot(v, top) * scale);
+    }
     V fixVignetting(V color, unsigned p) const { /* camera.cpp:107-122 */
         unsigned x = p % W, y = p / W;
         V d = add(add(mul(front, dist), mul(right, (float)(x + 0.5 - (W / 2.0)))),
@@ -982,6 +1002,7 @@ struct Tracer {
         unsigned inside = ORC_NONE, contact = ORC_NONE, ctri = 0;
         bool camera = false;    /* contactObj is the Camera (camera.cpp:54) */
         bool delta = false;     /* isDeltaDirection */
+        unsigned pixel = 0;     /* pixelID of the camera ray */
         Hit hit;                /* intersectObj / intersectDist / intersectTriangleID */
     };
 
@@ -1134,6 +1155,235 @@ struct Tracer {
         return out;
     }
 
+    /* ---- BidirectionalPathTracer (bidirPathTracer.cpp) ------------------------ */
+    /* The camera ray's contactObj as a NoSelf contact: not an object, not NULL. */
+    static constexpr unsigned kCameraObj = 0xFFFFFFFEu;
+
+    PRay cameraRay(unsigned p, unsigned pass) const {
+        float rn[4];
+        rng4(seed, p, pass, 0, rn);
+        PRay r0;
+        r0.o = cam.pos;
+        r0.d = cam.dir(p, rn[0], rn[1]);
+        r0.camera = true;
+        r0.pixel = p;
+        r0.n = normalize(sub(cam.focus, r0.o));
+        Ray cr;
+        cr.o = r0.o;
+        cr.d = r0.d;
+        r0.hit = trace(cr, false, 0, 0);
+        r0.inside = s->findInside(cr, ORC_NONE);
+        return r0;
+    }
+
+    /* in.evalBSDF(out) (ray.cpp:15-22): out's contact (else inside) object's evaluateBSDF */
+    V evalBSDF(const PRay& in, const PRay& out) const {
+        if (out.camera) return V(1, 1, 1);  /* AbstractObject::evaluateBSDF */
+        const unsigned obj = out.contact != ORC_NONE ? out.contact : out.inside;
+        if (obj == ORC_NONE) return V(0, 0, 0);
+        const Object& ob = s->objs[obj];
+        if (ob.mat == ORC_REFRACTIVE) return V(0, 0, 0);            /* refractiveObj.h:13-15 */
+        if (ob.mat == ORC_EMISSIVE) return ob.color;                /* emissiveObj.h:13 */
+        if (out.contact == ORC_NONE) return V(0, 0, 0);             /* diffuseObj.h:18, glossyObj.h:21 */
+        const V n = out.n;
+        if (!(dot(in.d, n) <= 0 && dot(out.d, n) >= 0)) return V(0, 0, 0);
+        if (ob.mat == ORC_DIFFUSE) return mul(ob.color, (float)INV_PI_D);  /* diffuseBsdf.h:29-34 */
+        V refl = normalize(add(mul(mul(n, -dot(n, in.d)), 2.f), in.d));     /* isotropicPhong.h:41-48 */
+        V rho = mul(mul(mul(ob.color, (float)(ob.power + 2.0)), 0.5f), (float)INV_PI_D);
+        const float dRW = dot(refl, out.d);
+        return mul(rho, std::pow(dRW > 0 ? dRW : 0.f, ob.power));
+    }
+
+    /* in.evalDirectionProbability(out) (ray.cpp:31-38); same: `in` and `out` are one
+     * object (the "emitted" branch of AbstractObject::evalDirectionProbability) */
+    float evalDirProb(const PRay& in, const PRay& out, bool same) const {
+        if (out.camera) return (float)(1.0 / cam.pixelSolidAngle(out.pixel));  /* camera.cpp:144-146 */
+        const unsigned obj = out.contact != ORC_NONE ? out.contact : out.inside;
+        if (obj == ORC_NONE) return 0.f;
+        const Object& ob = s->objs[obj];
+        if (ob.mat == ORC_REFRACTIVE || ob.mat == ORC_EMISSIVE)  /* abstractObj.cpp:61-68 */
+            return same ? (float)(INV_PI_D * 0.25f) * 2 : 0.f;
+        if (out.contact == ORC_NONE) return 0.f;
+        const float p = maxc(ob.color);
+        const float c = dot(out.n, out.d);
+        if (ob.mat == ORC_DIFFUSE) return p * (c > 0 ? (float)((double)c * INV_PI_D) : 0.f);  /* diffuseBsdf.h:12-14 */
+        const float cosTheta = c > EPS_D ? c : (float)EPS_D;                                   /* isotropicPhong.h:13-18 */
+        return p * (float)((ob.power + 1.0) * (double)std::pow(cosTheta, ob.power) * (INV_PI_D * 0.5));
+    }
+
+    static float cosTermB(const PRay& r) { return cosTerm(r); }
+
+    /* BidirectionalPathTracer::linkVertex (bidirPathTracer.cpp:228-243) */
+    static PRay linkVertex(const std::vector<PRay>& path, int cl, int startID, bool reverse) {
+        const int endID = reverse ? startID - 1 : startID + 1;
+        const int last = (int)path.size() - 1;
+        const PRay& startRay = path[startID < 0 ? 0 : startID > last ? last : startID];
+        const PRay& endRay = path[endID < 0 ? 0 : endID > last ? last : endID];
+        PRay linkRay = startRay;
+        if ((reverse && startID > cl) || (!reverse && startID <= cl)) return linkRay;
+        linkRay.d = normalize(sub(endRay.o, startRay.o));
+        linkRay.inside = endRay.inside;
+        return linkRay;
+    }
+
+    static bool pathCanNotConnect(const std::vector<PRay>& path) { /* bidirPathTracer.cpp:251-257 */
+        for (size_t i = 0; i + 1 < path.size(); i++)
+            if (!path[i].delta && !path[i + 1].delta) return false;
+        return true;
+    }
+
+    /* connectingColorProb (bidirPathTracer.cpp:266-291): color and probability */
+    static void colorProb(const std::vector<PRay>& path, int cl, V& color, float& prob) {
+        color = V(1, 1, 1);
+        prob = 1;
+        const int n = (int)path.size();
+        for (int i = 0; i < n; i++) {
+            color = mulv(color, path[i].rad);
+            float dist = 0.f;
+            if (i <= cl) {
+                const float l = length(sub(path[i].o, path[i + 1].o));
+                dist = l > EPS_D ? l : (float)EPS_D;
+                color = mulv(color, V(1, 1, 1));  /* radianceDecay: no volumes */
+            } else if (i > cl + 1) {
+                const float l = length(sub(path[i - 1].o, path[i].o));
+                dist = l > EPS_D ? l : (float)EPS_D;
+                color = mulv(color, V(1, 1, 1));
+            }
+            if (i == cl && i < n - 1) color = mul(color, cosTerm(path[i]) * cosTerm(path[i + 1]) / (dist * dist));
+            if (i != cl && i != cl + 1) color = mul(color, cosTerm(path[i]));
+            prob *= path[i].dp * path[i].op;
+        }
+    }
+
+    /* connectingMisWeight (bidirPathTracer.cpp:142-221), expTerm 1; every
+     * evalOriginProbability here compares two temporaries, so it is 1 */
+    float misWeight(const std::vector<PRay>& path, int cl) const {
+        const int n = (int)path.size();
+        std::vector<double> direct(n), reverse(n);
+        std::vector<float> distRecord;
+        direct[0] = path[0].op;
+        for (int i = 1; i < n; i++) {
+            const float l = length(sub(path[i].o, path[i - 1].o));
+            const float dist = l > EPS_D ? l : (float)EPS_D;
+            distRecord.push_back(dist);
+            const float cosThere = cosTerm(linkVertex(path, cl, i, true));
+            direct[i] = direct[i - 1] * (double)(1.0f * std::fabs(cosThere) / (dist * dist));  /* PdfWtoA */
+            if (path[i - 1].delta) continue;
+            float linkDirProb;
+            if (i > 1)
+                linkDirProb = evalDirProb(linkVertex(path, cl, i - 2, false), linkVertex(path, cl, i - 1, false), false);
+            else {
+                const PRay r = linkVertex(path, cl, i - 1, false);
+                linkDirProb = evalDirProb(r, r, true);
+            }
+            direct[i] *= linkDirProb * 1.f;
+        }
+        reverse[n - 1] = path[n - 1].op;
+        for (int i = n - 2; i >= 0; i--) {
+            const float dist = distRecord[i];
+            const float cosThere = cosTerm(linkVertex(path, cl, i, false));
+            reverse[i] = reverse[i + 1] * (double)(1.0f * std::fabs(cosThere) / (dist * dist));
+            if (path[i + 1].delta) continue;
+            float linkDirProb;
+            if (i < n - 2)
+                linkDirProb = evalDirProb(linkVertex(path, cl, i + 2, true), linkVertex(path, cl, i + 1, true), false);
+            else {
+                const PRay r = linkVertex(path, cl, i + 1, true);
+                linkDirProb = evalDirProb(r, r, true);
+            }
+            reverse[i] *= linkDirProb * 1.f;
+        }
+        double all = 0;
+        for (int i = 0; i < n - 1; i++)
+            if (!path[i].delta && !path[i + 1].delta) all += std::pow(direct[i] * reverse[i + 1], 1.0);
+        if (pathCanNotConnect(path)) all += std::pow(reverse[0], 1.0);
+        const double self = cl == -1 ? reverse[0] : direct[cl] * reverse[cl + 1];
+        const double w = std::pow(self, 1.0) / all;
+        return (float)(w > 0 ? w : 0);
+    }
+
+    /* connectEdge (bidirPathTracer.cpp:294-324) */
+    bool connectEdge(std::vector<PRay>& path, int cl) const {
+        PRay& lightRay = path[cl];
+        PRay& eyeRay = path[cl + 1];
+        if (lightRay.delta || eyeRay.delta) return false;
+        const int last = (int)path.size() - 1;
+        /* references, read after the edits: for cl = 0 (and for a camera-only eye
+         * side) the previous ray is the connection ray itself */
+        const PRay& preLight = path[cl - 1 > 0 ? cl - 1 : 0];
+        const PRay& preEye = path[cl + 2 < last ? cl + 2 : last];
+        lightRay.d = normalize(sub(eyeRay.o, lightRay.o));
+        eyeRay.d = neg(lightRay.d);
+        lightRay.dp = 1;
+        lightRay.delta = false;
+        const V lr = evalBSDF(preLight, lightRay);
+        lightRay.rad = lr;
+        if (lightRay.contact != ORC_NONE && cl > 0)
+            lightRay.inside = dot(lightRay.n, lightRay.d) > 0 ? preLight.inside : lightRay.contact;
+        eyeRay.dp = 1;
+        eyeRay.delta = false;
+        const V er = evalBSDF(preEye, eyeRay);
+        eyeRay.rad = er;
+        if (eyeRay.contact != ORC_NONE && cl < last - 1)
+            eyeRay.inside = dot(eyeRay.n, eyeRay.d) > 0 ? preEye.inside : eyeRay.contact;
+        return true;
+    }
+
+    /* One pixel's share of a BidirectionalPathTracer pass (bidirPathTracer.cpp:28-36,
+     * 53-140): its eye path and its light path, every connection; contributions to
+     * its own pixel in `own`, light-tracing splats to other pixels in `splats`. */
+    void bdptPixel(unsigned p, unsigned pass, V& own, std::vector<std::pair<unsigned, V>>& splats) const {
+        std::vector<PRay> eyePath;
+        samplePath(eyePath, cameraRay(p, pass), 0, p, pass);
+        const std::vector<PRay> lightP = lightPath(p, pass);
+        own = V(0, 0, 0);
+        const int maxEye = (int)eyePath.size(), maxLight = (int)lightP.size();
+        const unsigned npix = cam.W * cam.H;
+        for (int whole = 2; whole <= maxEye + maxLight; whole++) {
+            const int minL = whole - maxEye > 0 ? whole - maxEye : 0;
+            const int maxL = whole - 1 >= maxLight ? maxLight : whole - 1;
+            for (int lightLen = minL; lightLen <= maxL; lightLen++) {
+                const int eyeLen = whole - lightLen;
+                std::vector<PRay> path(whole);
+                for (int i = 0; i < lightLen; i++) path[i] = lightP[i];
+                for (int i = 0; i < eyeLen; i++) path[lightLen + i] = eyePath[eyeLen - i - 1];
+                const int cl = lightLen - 1;
+                const PRay& front = path[0];
+                if (!(front.contact != ORC_NONE && s->objs[front.contact].mat == ORC_EMISSIVE)) continue;
+                if (lightLen == 0 && !pathCanNotConnect(path)) continue;
+                if (eyeLen > 0 && lightLen > 0) {
+                    const PRay& lc = path[cl];
+                    const PRay& ec = path[cl + 1];
+                    if (lightLen > 1 && lc.contact != ORC_NONE && s->objs[lc.contact].mat == ORC_EMISSIVE) continue;
+                    if (ec.contact != ORC_NONE && s->objs[ec.contact].mat == ORC_EMISSIVE) continue;
+                    if (!connectEdge(path, cl)) continue;
+                    PRay vis = path[cl + 1];
+                    if (vis.camera) vis.contact = kCameraObj;
+                    if (!visible(vis, path[cl].o)) continue;
+                }
+                V color;
+                float prob;
+                colorProb(path, cl, color, prob);
+                if (prob == 0) continue;
+                if (length(color) == 0) continue;
+                const float w = misWeight(path, cl);
+                const V contrib = mul(divs(color, prob), w);
+                if (eyeLen > 1) {
+                    own = add(own, contrib);
+                } else {
+                    float rx, ry;
+                    cam.toRaster(path[whole - 2].o, rx, ry);
+                    const int x = (int)rx, y = (int)ry;  /* checkPixelBound truncates (camera.cpp:164-170) */
+                    if (x >= 0 && y >= 0 && x < (int)cam.W && y < (int)cam.H) {
+                        const float sa = cam.pixelSolidAngle((unsigned)(y * (int)cam.W + x));
+                        const float factor = (float)(1.0 / sa);
+                        splats.push_back({(unsigned)(y * (int)cam.W + x), mul(contrib, factor / (float)npix)});
+                    }
+                }
+            }
+        }
+    }
+
     /* One pixel sample of PathTracer::renderPixels (pathTracer.cpp:22-69).
      * The branch's `continue` leaves the pixel's running mean scaled by
      * s/(s+1) with nothing added, i.e. a sample of 0.  The light sample's
@@ -1145,6 +1395,7 @@ struct Tracer {
         r0.o = cam.pos;
         r0.d = cam.dir(p, rn[0], rn[1]);
         r0.camera = true;
+        r0.pixel = p;
         r0.n = normalize(sub(cam.focus, r0.o));            /* Camera::getWorldNormal (camera.cpp:182-186) */
         Ray cr;
         cr.o = r0.o;
@@ -1623,6 +1874,36 @@ void orc_motion_interpolate(float t0, float t1, const float* start, const float*
     const M4 t = Animated(a, t0, b, t1).interpolate(time);
     for (int i = 0; i < 4; i++)
         for (int j = 0; j < 4; j++) out[4 * i + j] = t.a[i][j];
+}
+
+void orc_render_bdpt(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t seed, uint32_t pass0, uint32_t npass,
+                     double* sum, double* sumsq, int nthreads) {
+    Scene* s = (Scene*)sp;
+    Tracer tr{s, Cam(c), max_depth, seed};
+    const int64_t np = (int64_t)c->width * c->height;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    std::vector<V> iter(np);
+    std::vector<std::vector<std::pair<unsigned, V>>> splats(np);
+    for (uint32_t pass = pass0; pass < pass0 + npass; pass++) {
+        s->updateForMotion(seed, pass);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+        for (int64_t p = 0; p < np; p++) {
+            splats[p].clear();
+            tr.bdptPixel((unsigned)p, pass, iter[p], splats[p]);
+        }
+        for (int64_t p = 0; p < np; p++)
+            for (const auto& sp2 : splats[p]) iter[sp2.first] = add(iter[sp2.first], sp2.second);
+        for (int64_t p = 0; p < np; p++) {
+            V v = iter[p];
+            if (!legal(v)) v = V(0, 0, 0);
+            v = tr.cam.fixVignetting(v, (unsigned)p);
+            for (int k = 0; k < 3; k++) {
+                sum[3 * p + k] += v[k];
+                sumsq[3 * p + k] += (double)v[k] * v[k];
+            }
+        }
+    }
+    restoreMotion(s);
 }
 
 void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox(ctr, key, out); }

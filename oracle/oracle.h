@@ -121,6 +121,14 @@ void     orc_hashgrid_free(void* g);
 void     orc_scene_set_motion(void* s, uint32_t obj, float t0, float t1, const float* start, const float* end);
 void     orc_motion_interpolate(float t0, float t1, const float* start, const float* end, float time, float* out);
 
+/* BidirectionalPathTracer::renderPixels (bidirPathTracer.cpp:10-51): per pass,
+ * every pixel's eye path and light path (the light path with the light-path
+ * vertex counters), every connection, light-tracing splats; the pass's pixel
+ * sums checked (isLegalColor) and vignetting-fixed, then accumulated as in
+ * orc_render. */
+void     orc_render_bdpt(void* s, const orc_camera* cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
+                         uint32_t npass, double* sum, double* sumsq, int nthreads);
+
 /* ---- Counter RNG (Philox-4x32-10, Salmon et al., SC'11) ---------------- */
 void     orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* 4 uniform floats in [0,1) for (seed, pixel, pass, vertex). */

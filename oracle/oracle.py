@@ -64,6 +64,7 @@ def lib():
         L.orc_hashgrid_free.argtypes = [vp]
         L.orc_scene_set_motion.argtypes = [vp, u32, f32, f32, vp, vp]
         L.orc_motion_interpolate.argtypes = [f32, f32, vp, vp, f32, vp]
+        L.orc_render_bdpt.argtypes = [vp, vp, u32, u64, u32, u32, vp, vp, C.c_int]
         L.orc_philox4x32.argtypes = [vp, vp, vp]
         L.orc_rng4.argtypes = [u64, u32, u32, u32, vp]
         _lib = L
@@ -226,6 +227,15 @@ class Scene:
         longest = lib().orc_light_paths(self.h, max_depth, seed, pss, path0, n_paths, max_vertices, _p(rec),
                                         _p(counts), nthreads)
         return rec, counts, longest
+
+    def render_bdpt(self, cam, spp, max_depth, seed, pass0=0, nthreads=0, sums=None):
+        """BidirectionalPathTracer passes [pass0, pass0 + spp)."""
+        c = Camera.from_dict(cam)
+        npx = cam["width"] * cam["height"]
+        if sums is None:
+            sums = (np.zeros(3 * npx, np.float64), np.zeros(3 * npx, np.float64))
+        lib().orc_render_bdpt(self.h, C.byref(c), max_depth, seed, pass0, spp, _p(sums[0]), _p(sums[1]), nthreads)
+        return sums
 
     def light_sample(self, u):
         """Scene::genLightSample from four uniforms; None when nothing can be sampled."""

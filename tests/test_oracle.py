@@ -661,3 +661,27 @@ def test_hashgrid_lattice_closed_form():
     counts, got = g.query(q, 64)
     assert len(set(got[0, :counts[0]].tolist())) == 7 and len(set(got[1, :counts[1]].tolist())) == 7
     assert counts[2] == 0 and counts[3] == 0
+
+
+def test_bdpt_converges_to_path_tracer():
+    """BidirectionalPathTracer and PathTracer estimate the same image once both
+    have converged in depth (BDPT's maxPathLen bounds each sub-path, so the two
+    differ at small depths): Cornell box with both spheres diffuse, BDPT at
+    depth 12 against PathTracer at depth 40, total radiance within 3.5 combined
+    standard errors."""
+    objs, cam = S.cornell(k=1, width=16, height=16)
+    objs = [dict(o) for o in objs]
+    objs[5] = dict(objs[5], mat=S.DIFFUSE, params=(0.8, 0.8, 0.8))
+    sc = O.Scene(objs)
+    n1, n2 = 600, 2400
+    s1, q1 = sc.render_bdpt(cam, n1, 12, 11)
+    s2, q2 = sc.render(cam, n2, 40, 12)
+    m1, m2 = s1 / n1, s2 / n2
+    v1 = np.maximum(q1 / n1 - m1 * m1, 0) / n1
+    v2 = np.maximum(q2 / n2 - m2 * m2, 0) / n2
+    diff = m1.sum() - m2.sum()
+    se = np.sqrt(v1.sum() + v2.sum())
+    assert abs(diff) < 3.5 * se, (m1.sum(), m2.sum(), se)
+    # and the per-pixel images agree within their own errors on most pixels
+    z = np.abs(m1 - m2) / np.sqrt(v1 + v2 + 1e-30)
+    assert (z < 3.5).mean() > 0.97
