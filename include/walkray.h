@@ -249,6 +249,19 @@ int wr_film_write(wr_film* film, const float* sum, const float* comp, const floa
 int wr_render(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, const wr_render_params* params,
               wr_film* film, wr_render_stats* stats);
 
+/* BidirectionalPathTracer::renderPixels (bidirPathTracer.cpp:10-51) for
+ * passes [passes, passes + spp) of the film: per pixel and pass an eye
+ * sub-path and a light sub-path (the light-path counters of wr_light_paths,
+ * keyed by the pixel), every connection with its MIS weight
+ * (bidirPathTracer.cpp:53-313), light-tracing contributions splatted to the
+ * pixel the camera sees them in; the pass's pixel values checked
+ * (isLegalColor), vignetting-fixed and added to the film.  params: spp,
+ * max_depth (maxPathLen, at most 16) and seed; mode, wave_* and flags must
+ * be 0.  The splats are summed with float atomics, so the film is
+ * reproducible up to their summation order.  Static scenes only. */
+int wr_render_bdpt(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, const wr_render_params* params,
+                   wr_film* film, wr_render_stats* stats);
+
 /* ---- light sub-paths and the photon hash grid ------------------------------ */
 /* One Ray of a Path (ray.h:8-37) as the light-path estimators read it. */
 typedef struct {

@@ -156,6 +156,7 @@ def lib():
         "wr_film_read": (i32, [vp, vp, vp, vp, vp, P(u32)]),
         "wr_film_write": (i32, [vp, vp, vp, vp, u32]),
         "wr_render": (i32, [vp, vp, P(CameraDesc), P(RenderParams), vp, P(RenderStats)]),
+        "wr_render_bdpt": (i32, [vp, vp, P(CameraDesc), P(RenderParams), vp, P(RenderStats)]),
         "wr_server_create": (i32, [vp, P(ServerConfig), P(vp)]),
         "wr_server_destroy": (None, [vp]),
         "wr_server_add_scene": (i32, [vp, vp, P(u32)]),
@@ -489,6 +490,16 @@ def render(scene, cam, film, spp, max_depth, seed, mode=WR_WAVEFRONT, wave_slots
     _check(lib().wr_render(scene.ctx.h, scene.h, C.byref(cd), C.byref(prm), film.h, C.byref(st)), scene.ctx.h,
            "wr_render")
     return {"paths": st.paths, "iterations": st.iterations, "kernel_ms": st.kernel_ms}
+
+
+def render_bdpt(scene, cam, film, spp, max_depth, seed):
+    """BidirectionalPathTracer::renderPixels for `spp` more passes into `film`; returns stats."""
+    cd = CameraDesc.from_dict(cam)
+    prm = RenderParams(spp, max_depth, seed, 0, 0, 0, 0)
+    st = RenderStats()
+    _check(lib().wr_render_bdpt(scene.ctx.h, scene.h, C.byref(cd), C.byref(prm), film.h, C.byref(st)), scene.ctx.h,
+           "wr_render_bdpt")
+    return {"paths": st.paths, "kernel_ms": st.kernel_ms}
 
 
 HIT_DTYPE = np.dtype([("prim", np.uint32), ("t", np.float32), ("b1", np.float32), ("b2", np.float32)])

@@ -1,0 +1,464 @@
+// bdpt.hip -- BidirectionalPathTracer (bidirPathTracer.cpp:10-313) on gfx950.
+//
+// One lane per pixel and pass (the reference's per-pixel loop), megakernel
+// style:
+//   1. the eye sub-path: the camera ray (camera.cpp:6-60) and
+//      MCRenderer::samplePath (mcRenderer.cpp:16-57) from it;
+//   2. the light sub-path: Scene::genLightSample as
+//      AbstractObject::genSurfaceSample (abstractObj.cpp:18-52) and samplePath
+//      from it, with the light-path vertex counters;
+//   3. every (light length, eye length) split of every joined length
+//      (bidirPathTracer.cpp:53-140): connectEdge (294-324), the visibility
+//      test (mcRenderer.cpp:59-65), connectingColorProb (266-291) and
+//      connectingMisWeight (142-221) with linkVertex (228-243);
+//   4. contributions with an eye side of two or more vertices go to the
+//      lane's pixel, light-tracing ones (the camera vertex alone on the eye
+//      side) are splatted to the pixel Camera::convToRaster names
+//      (camera.cpp:148-170), scaled by 1 / pixel solid angle / pixel count.
+// The sub-paths live in device memory (one block of records per lane).  The
+// pixel values of a pass are summed with float atomics (the splats of other
+// lanes land on any pixel), then k_bdpt_fold applies isLegalColor and
+// fixVignetting and adds them to the Kahan film.
+#include <hip/hip_runtime.h>
+
+#include "pt_dev.h"
+#include "wr_pt.h"
+
+#pragma clang fp contract(off)
+
+namespace wrd {
+
+constexpr uint32_t kBdptLightVertex = 0x80000000u;  // as the light sub-paths of photon.hip
+constexpr uint32_t kCameraObj = 0xFFFFFFFEu;        // the camera as a NoSelf contact: not an object, not NULL
+constexpr uint32_t kVDelta = 1u, kVCamera = 2u;
+constexpr int kBdptMaxJoined = 2 * (wr::kBdptMaxDepth + 3);
+
+// One Ray of a Path as the estimator reads it (ray.h:8-37).
+struct BVert {
+    f3 o, d, rad, n;  // n: the contact normal at o (the camera's: front)
+    float op, dp;
+    uint32_t inside, contact, flags, pad;
+};
+
+__device__ __forceinline__ void put(BVert* V, uint32_t& nv, uint32_t maxv, const BVert& v) {
+    if (nv < maxv) V[nv] = v;
+    nv++;
+}
+
+// terminateRay (mcRenderer.cpp:20-27) after a ray from o.
+__device__ __forceinline__ BVert terminate_at(f3 o) {
+    BVert t;
+    t.o = o;
+    t.d = mk(0.f, 0.f, 0.f);
+    t.rad = t.d;
+    t.n = t.d;
+    t.op = 1.f;
+    t.dp = 1.f;
+    t.inside = WR_NONE;
+    t.contact = WR_NONE;
+    t.flags = kVDelta;
+    t.pad = 0;
+    return t;
+}
+
+// samplePath from v0 (already traced: hit) with vertex counters vbase + depth + 1.
+__device__ void walk(const SceneCtx& sc, const BVert& v0, PHit hit, uint32_t max_depth, uint64_t seed, uint32_t p,
+                     uint32_t pass, uint32_t vbase, BVert* V, uint32_t& nv, uint32_t maxv, LaneStack& st) {
+    put(V, nv, maxv, v0);
+    f3 o = v0.o, d = v0.d;
+    uint32_t inside = v0.inside;
+    for (uint32_t depth = 0;; depth++) {
+        const uint32_t self = inside != WR_NONE ? inside : hit.obj;
+        if (self == WR_NONE || hit.rec == WR_NONE) {
+            put(V, nv, maxv, terminate_at(o));
+            return;
+        }
+        float rn[4];
+        rng4(seed, p, pass, vbase + depth + 1, rn);
+        const Out r = scatter(sc, self, o, d, hit, inside, rn, st);
+        BVert nx;
+        nx.o = r.o;
+        nx.d = r.d;
+        nx.rad = r.rad;
+        nx.n = r.n;
+        nx.op = r.op;
+        nx.dp = r.dp;
+        nx.inside = r.inside;
+        nx.contact = r.contact;
+        nx.flags = r.delta ? kVDelta : 0u;
+        nx.pad = 0;
+        if ((double)len(r.d) < 0.5) {
+            put(V, nv, maxv, nx);
+            return;
+        }
+        if (depth + 1 > max_depth) {
+            put(V, nv, maxv, terminate_at(o));
+            return;
+        }
+        const PHit nh = trace_scene<true>(sc, r.o, r.d, r.inside, r.contact, st);
+        put(V, nv, maxv, nx);
+        if (nh.rec == WR_NONE) {
+            put(V, nv, maxv, terminate_at(o));
+            return;
+        }
+        o = r.o;
+        d = r.d;
+        inside = r.inside;
+        hit = nh;
+    }
+}
+
+// Camera::getPixelSolidAngle (camera.cpp:124-131).
+__device__ __forceinline__ float pixel_solid_angle(const CamDev& c, uint32_t p) {
+    const uint32_t x = p % c.W, y = p / c.W;
+    const float xc = (float)((double)x + 0.5 - (c.W / 2.0)), yc = (float)((c.H / 2.0) - 0.5 - (double)y);
+    const float pd = sqrtf(xc * xc + yc * yc + c.dist * c.dist);
+    return c.dist / powf(pd, 3.f);
+}
+
+// Ray::cosineTerm (ray.cpp:52-58).
+__device__ __forceinline__ float cos_term_v(const BVert& r) {
+    float c = 1.f;
+    if (((r.flags & kVCamera) || r.contact != WR_NONE) && (double)len(r.d) > 0.5) c = fabsf(dot(r.n, r.d));
+    return (double)c > kEps ? c : (float)kEps;
+}
+
+// in.evalBSDF(out) (ray.cpp:15-22): the object of out's contact (else inside).
+__device__ f3 eval_bsdf(const SceneDev& S, const BVert& in, const BVert& out) {
+    if (out.flags & kVCamera) return mk(1.f, 1.f, 1.f);  // AbstractObject::evaluateBSDF
+    const uint32_t obj = out.contact != WR_NONE ? out.contact : out.inside;
+    if (obj == WR_NONE) return mk(0.f, 0.f, 0.f);
+    const wr_object_gpu& ob = S.objs[obj];
+    const f3 color = mk(ob.color[0], ob.color[1], ob.color[2]);
+    if (ob.mat == WR_MAT_REFRACTIVE) return mk(0.f, 0.f, 0.f);  // refractiveObj.h:13-15
+    if (ob.mat == WR_MAT_EMISSIVE) return color;                // emissiveObj.h:13
+    if (out.contact == WR_NONE) return mk(0.f, 0.f, 0.f);       // diffuseObj.h:18, glossyObj.h:21
+    const f3 n = out.n;
+    if (!(dot(in.d, n) <= 0.f && dot(out.d, n) >= 0.f)) return mk(0.f, 0.f, 0.f);
+    if (ob.mat == WR_MAT_DIFFUSE) return muls(color, (float)kInvPi);  // diffuseBsdf.h:29-34
+    const f3 refl = normalize(add(muls(muls(n, -dot(n, in.d)), 2.f), in.d));  // isotropicPhong.h:41-48
+    const f3 rho = muls(muls(muls(color, (float)((double)ob.power + 2.0)), 0.5f), (float)kInvPi);
+    const float dRW = dot(refl, out.d);
+    return muls(rho, powf(dRW > 0.f ? dRW : 0.f, ob.power));
+}
+
+// in.evalDirectionProbability(out) (ray.cpp:31-38); same: the emitted branch of
+// AbstractObject::evalDirectionProbability (in and out one object).
+__device__ float eval_dir_prob(const SceneDev& S, const CamDev& cam, uint32_t pixel, const BVert& in,
+                               const BVert& out, bool same) {
+    (void)in;
+    if (out.flags & kVCamera) return (float)(1.0 / (double)pixel_solid_angle(cam, pixel));  // camera.cpp:144-146
+    const uint32_t obj = out.contact != WR_NONE ? out.contact : out.inside;
+    if (obj == WR_NONE) return 0.f;
+    const wr_object_gpu& ob = S.objs[obj];
+    if (ob.mat == WR_MAT_REFRACTIVE || ob.mat == WR_MAT_EMISSIVE)  // abstractObj.cpp:61-68
+        return same ? (float)(kInvPi * 0.25f) * 2.f : 0.f;
+    if (out.contact == WR_NONE) return 0.f;
+    const float p = ob.pmax;
+    const float c = dot(out.n, out.d);
+    if (ob.mat == WR_MAT_DIFFUSE) return p * (c > 0.f ? (float)((double)c * kInvPi) : 0.f);  // diffuseBsdf.h:12-14
+    const float cos_theta = (double)c > kEps ? c : (float)kEps;                                // isotropicPhong.h:13-18
+    return p * (float)(((double)ob.power + 1.0) * (double)powf(cos_theta, ob.power) * (kInvPi * 0.5));
+}
+
+// The joined path light[0..nl) + reversed eye[0..ne), with the two connection
+// records (indices cl, cl + 1) replaced by their edited copies once connected.
+struct Joined {
+    const BVert* L;
+    const BVert* E;
+    int nl, ne, n, cl;
+    bool edited;
+    BVert lc, ec;
+    __device__ __forceinline__ BVert at(int i) const {
+        if (edited && i == cl) return lc;
+        if (edited && i == cl + 1) return ec;
+        return i < nl ? L[i] : E[ne - 1 - (i - nl)];
+    }
+    __device__ __forceinline__ bool delta(int i) const { return (at(i).flags & kVDelta) != 0; }
+};
+
+// linkVertex (bidirPathTracer.cpp:228-243)
+__device__ __forceinline__ BVert link_vertex(const Joined& P, int start, bool reverse) {
+    const int end = reverse ? start - 1 : start + 1;
+    const int last = P.n - 1;
+    const BVert s = P.at(start < 0 ? 0 : start > last ? last : start);
+    BVert r = s;
+    if ((reverse && start > P.cl) || (!reverse && start <= P.cl)) return r;
+    const BVert e = P.at(end < 0 ? 0 : end > last ? last : end);
+    r.d = normalize(sub(e.o, s.o));
+    r.inside = e.inside;
+    return r;
+}
+
+// pathCanNotConnect (bidirPathTracer.cpp:251-257)
+__device__ __forceinline__ bool cannot_connect(const Joined& P) {
+    for (int i = 0; i + 1 < P.n; i++)
+        if (!P.delta(i) && !P.delta(i + 1)) return false;
+    return true;
+}
+
+// connectEdge (bidirPathTracer.cpp:294-324): edits the copies lc / ec; the
+// previous records are read after the edits (for cl = 0, and for an eye side
+// of one record, the previous record is the connection record itself).
+__device__ bool connect_edge(const SceneDev& S, Joined& P) {
+    P.lc = P.at(P.cl);
+    P.ec = P.at(P.cl + 1);
+    if ((P.lc.flags & kVDelta) || (P.ec.flags & kVDelta)) return false;
+    P.edited = true;
+    const int last = P.n - 1;
+    const int il = P.cl - 1 > 0 ? P.cl - 1 : 0;
+    const int ie = P.cl + 2 < last ? P.cl + 2 : last;
+    P.lc.d = normalize(sub(P.ec.o, P.lc.o));
+    P.ec.d = mk(-P.lc.d.x, -P.lc.d.y, -P.lc.d.z);
+    P.lc.dp = 1.f;
+    P.lc.flags &= ~kVDelta;
+    {
+        const BVert pre = P.at(il);  // lc itself when il == cl
+        P.lc.rad = eval_bsdf(S, pre, P.lc);
+        if (P.lc.contact != WR_NONE && P.cl > 0)
+            P.lc.inside = dot(P.lc.n, P.lc.d) > 0.f ? pre.inside : P.lc.contact;
+    }
+    P.ec.dp = 1.f;
+    P.ec.flags &= ~kVDelta;
+    {
+        const BVert pre = P.at(ie);  // ec itself when ie == cl + 1
+        P.ec.rad = eval_bsdf(S, pre, P.ec);
+        if (P.ec.contact != WR_NONE && P.cl < last - 1)
+            P.ec.inside = dot(P.ec.n, P.ec.d) > 0.f ? pre.inside : P.ec.contact;
+    }
+    return true;
+}
+
+// connectingColorProb (bidirPathTracer.cpp:266-291); no participating media.
+__device__ void color_prob(const Joined& P, f3& color, float& prob) {
+    color = mk(1.f, 1.f, 1.f);
+    prob = 1.f;
+    for (int i = 0; i < P.n; i++) {
+        const BVert v = P.at(i);
+        color = mulv(color, v.rad);
+        float dist = 0.f;
+        if (i <= P.cl) {
+            const float l = len(sub(v.o, P.at(i + 1).o));
+            dist = (double)l > kEps ? l : (float)kEps;
+        } else if (i > P.cl + 1) {
+            const float l = len(sub(P.at(i - 1).o, v.o));
+            dist = (double)l > kEps ? l : (float)kEps;
+        }
+        if (i == P.cl && i < P.n - 1) color = muls(color, cos_term_v(v) * cos_term_v(P.at(i + 1)) / (dist * dist));
+        if (i != P.cl && i != P.cl + 1) color = muls(color, cos_term_v(v));
+        prob *= v.dp * v.op;
+    }
+}
+
+// connectingMisWeight (bidirPathTracer.cpp:142-221) with expTerm 1; every
+// evalOriginProbability there compares two temporaries, so it is 1.
+__device__ float mis_weight(const SceneDev& S, const CamDev& cam, uint32_t pixel, const Joined& P) {
+    double direct[kBdptMaxJoined], reverse[kBdptMaxJoined];
+    float dist_rec[kBdptMaxJoined];
+    const int n = P.n;
+    direct[0] = P.at(0).op;
+    for (int i = 1; i < n; i++) {
+        const float l = len(sub(P.at(i).o, P.at(i - 1).o));
+        const float dist = (double)l > kEps ? l : (float)kEps;
+        dist_rec[i - 1] = dist;
+        const float cos_there = cos_term_v(link_vertex(P, i, true));
+        direct[i] = direct[i - 1] * (double)(1.0f * fabsf(cos_there) / (dist * dist));  // PdfWtoA
+        if (P.delta(i - 1)) continue;
+        float link_dir;
+        if (i > 1) {
+            link_dir = eval_dir_prob(S, cam, pixel, link_vertex(P, i - 2, false), link_vertex(P, i - 1, false), false);
+        } else {
+            const BVert r = link_vertex(P, i - 1, false);
+            link_dir = eval_dir_prob(S, cam, pixel, r, r, true);
+        }
+        direct[i] *= link_dir * 1.f;
+    }
+    reverse[n - 1] = P.at(n - 1).op;
+    for (int i = n - 2; i >= 0; i--) {
+        const float dist = dist_rec[i];
+        const float cos_there = cos_term_v(link_vertex(P, i, false));
+        reverse[i] = reverse[i + 1] * (double)(1.0f * fabsf(cos_there) / (dist * dist));
+        if (P.delta(i + 1)) continue;
+        float link_dir;
+        if (i < n - 2) {
+            link_dir = eval_dir_prob(S, cam, pixel, link_vertex(P, i + 2, true), link_vertex(P, i + 1, true), false);
+        } else {
+            const BVert r = link_vertex(P, i + 1, true);
+            link_dir = eval_dir_prob(S, cam, pixel, r, r, true);
+        }
+        reverse[i] *= link_dir * 1.f;
+    }
+    double all = 0.0;
+    for (int i = 0; i < n - 1; i++)
+        if (!P.delta(i) && !P.delta(i + 1)) all += direct[i] * reverse[i + 1];
+    if (cannot_connect(P)) all += reverse[0];
+    const double self = P.cl == -1 ? reverse[0] : direct[P.cl] * reverse[P.cl + 1];
+    const double w = self / all;
+    return (float)(w > 0.0 ? w : 0.0);
+}
+
+__device__ __forceinline__ bool emissive(const SceneDev& S, uint32_t obj) {
+    return obj != WR_NONE && S.objs[obj].mat == WR_MAT_EMISSIVE;
+}
+
+__device__ __forceinline__ void atomic_add3(float* buf, uint32_t p, f3 v) {
+    atomicAdd(buf + 3 * p + 0, v.x);
+    atomicAdd(buf + 3 * p + 1, v.y);
+    atomicAdd(buf + 3 * p + 2, v.z);
+}
+
+__global__ __launch_bounds__(256) void k_bdpt(SceneDev S, CamDev cam, uint32_t max_depth, uint64_t seed,
+                                              uint32_t pass, uint32_t npix, uint32_t maxv, BVert* verts, float* pix) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= npix) return;
+    StackSpill spill;
+    LaneStack st(&spill);
+    const SceneCtx sc = make_scene_ctx(S);
+    BVert* E = verts + (size_t)p * 2 * maxv;
+    BVert* L = E + maxv;
+    uint32_t ne = 0, nl = 0;
+    float rn[4];
+    // eye sub-path: the camera ray (camera.cpp:6-60) then samplePath
+    {
+        rng4(seed, p, pass, 0, rn);
+        BVert c;
+        c.o = cam.pos;
+        c.d = cam_dir(cam, p, rn[0], rn[1]);
+        c.rad = mk(1.f, 1.f, 1.f);
+        c.n = cam.front;
+        c.op = 1.f;
+        c.dp = 1.f;
+        c.contact = WR_NONE;
+        c.flags = kVCamera;
+        c.pad = 0;
+        const PHit h = trace_scene<false>(sc, c.o, c.d, WR_NONE, WR_NONE, st);
+        c.inside = find_inside(S, c.o, c.d, WR_NONE, st);
+        walk(sc, c, h, max_depth, seed, p, pass, 0u, E, ne, maxv, st);
+    }
+    // light sub-path (as k_light_paths)
+    {
+        rng4(seed, p, pass, kBdptLightVertex, rn);
+        const LightSample ls = sample_light(sc, rn);
+        BVert r;
+        r.o = mk(0.f, 0.f, 0.f);
+        r.d = r.o;
+        r.rad = r.o;
+        r.n = r.o;
+        r.op = 1.f;
+        r.dp = 1.f;
+        r.inside = WR_NONE;
+        r.contact = WR_NONE;
+        r.flags = 0;
+        r.pad = 0;
+        PHit hit;
+        hit.t = -1.f;
+        hit.rec = WR_NONE;
+        hit.obj = WR_NONE;
+        if (ls.ok) {
+            rng4(seed, p, pass, kBdptLightVertex + 1, rn);
+            const float term1 = (float)(2.0 * kPi * (double)rn[0]);  // uniformSphereSampler.h:6-20
+            const float term2 = 2.f * sqrtf(rn[1] - rn[1] * rn[1]);
+            f3 d = mk(cosf(term1) * term2, 1.f - 2.f * rn[1], sinf(term1) * term2);
+            r.dp = (float)(kInvPi * 0.25f);
+            if (dot(d, ls.n) < 0.f) d = mk(-d.x, -d.y, -d.z);
+            r.o = ls.o;
+            r.n = ls.n;
+            r.d = d;
+            r.rad = ls.rad;
+            r.contact = ls.obj;
+            r.inside = find_inside(S, r.o, r.d, ls.obj, st);
+            r.dp *= 2.f;
+            r.op = ls.op;
+            const PHit h = trace_scene<true>(sc, r.o, r.d, r.inside, r.contact, st);
+            if (h.rec != WR_NONE && h.t > 0.f) hit = h;
+        }
+        walk(sc, r, hit, max_depth, seed, p, pass, kBdptLightVertex + 1, L, nl, maxv, st);
+    }
+    const int max_eye = (int)(ne < maxv ? ne : maxv), max_light = (int)(nl < maxv ? nl : maxv);
+    f3 own = mk(0.f, 0.f, 0.f);
+    for (int whole = 2; whole <= max_eye + max_light; whole++) {
+        const int min_l = whole - max_eye > 0 ? whole - max_eye : 0;
+        const int max_l = whole - 1 >= max_light ? max_light : whole - 1;
+        for (int light_len = min_l; light_len <= max_l; light_len++) {
+            Joined P;
+            P.L = L;
+            P.E = E;
+            P.nl = light_len;
+            P.ne = whole - light_len;
+            P.n = whole;
+            P.cl = light_len - 1;
+            P.edited = false;
+            if (!emissive(S, P.at(0).contact)) continue;
+            if (light_len == 0 && !cannot_connect(P)) continue;
+            if (P.ne > 0 && light_len > 0) {
+                if (light_len > 1 && emissive(S, P.at(P.cl).contact)) continue;
+                if (emissive(S, P.at(P.cl + 1).contact)) continue;
+                if (!connect_edge(S, P)) continue;
+                // MCRenderer::visibilityTest from the eye-side connection record
+                const uint32_t contact = (P.ec.flags & kVCamera) ? kCameraObj : P.ec.contact;
+                const PHit h = trace_scene<true>(sc, P.ec.o, P.ec.d, P.ec.inside, contact, st);
+                if (!nee_visible(h.rec, h.t, len(sub(P.ec.o, P.lc.o)))) continue;
+            }
+            f3 color;
+            float prob;
+            color_prob(P, color, prob);
+            if (prob == 0.f) continue;
+            if (len(color) == 0.f) continue;
+            const float w = mis_weight(S, cam, p, P);
+            const f3 contrib = muls(divs(color, prob), w);
+            if (P.ne > 1) {
+                own = add(own, contrib);
+            } else {  // light tracing: splat to the pixel the camera sees the point in
+                const f3 v = sub(P.at(whole - 2).o, cam.pos);
+                const float vfd = dot(v, cam.front);
+                const float scale = cam.dist / vfd;
+                const float rx = (float)(cam.W / 2.0 + (double)(dot(v, cam.right) * scale));
+                const float ry = (float)(cam.H / 2.0 - (double)(dot(v, cam.top) * scale));
+                const int x = (int)rx, y = (int)ry;  // checkPixelBound truncates (camera.cpp:164-170)
+                if (x >= 0 && y >= 0 && x < (int)cam.W && y < (int)cam.H) {
+                    const uint32_t q = (uint32_t)(y * (int)cam.W + x);
+                    const float factor = (float)(1.0 / (double)pixel_solid_angle(cam, q));
+                    atomic_add3(pix, q, muls(contrib, factor / (float)npix));
+                }
+            }
+        }
+    }
+    atomic_add3(pix, p, own);
+}
+
+// The pass's pixel values: isLegalColor, fixVignetting, the Kahan film.
+__global__ __launch_bounds__(256) void k_bdpt_fold(CamDev cam, uint32_t npix, const float* pix, FilmDev film) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= npix) return;
+    f3 v = mk(pix[3 * p], pix[3 * p + 1], pix[3 * p + 2]);
+    if (!legal(v)) v = mk(0.f, 0.f, 0.f);
+    film_add(film, p, fix_vignetting(cam, v, p));
+}
+
+}  // namespace wrd
+
+namespace wr {
+
+size_t bdpt_vertex_bytes() { return sizeof(wrd::BVert); }
+
+hipError_t launch_bdpt_pass(const SceneDev& S, const CamHost& ch, uint32_t max_depth, uint64_t seed, uint32_t pass,
+                            uint32_t maxv, void* verts, float* pix, const FilmBufs& film, hipStream_t st) {
+    wrd::CamDev cam;
+    cam.pos = wrd::f3{ch.pos[0], ch.pos[1], ch.pos[2]};
+    cam.front = wrd::f3{ch.front[0], ch.front[1], ch.front[2]};
+    cam.right = wrd::f3{ch.right[0], ch.right[1], ch.right[2]};
+    cam.top = wrd::f3{ch.top[0], ch.top[1], ch.top[2]};
+    cam.dist = ch.dist;
+    cam.W = ch.W;
+    cam.H = ch.H;
+    const uint32_t npix = ch.W * ch.H;
+    const uint32_t grid = (npix + 255) / 256;
+    hipError_t e = hipMemsetAsync(pix, 0, sizeof(float) * 3 * (size_t)npix, st);
+    if (e != hipSuccess) return e;
+    wrd::k_bdpt<<<grid, 256, wrd::kStackLdsBytes, st>>>(S, cam, max_depth, seed, pass, npix, maxv,
+                                                       (wrd::BVert*)verts, pix);
+    wrd::k_bdpt_fold<<<grid, 256, 0, st>>>(cam, npix, pix, wrd::FilmDev{film.sum, film.comp, film.sumsq});
+    return hipGetLastError();
+}
+
+}  // namespace wr

@@ -213,6 +213,47 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
 
 }  // extern "C"
 
+extern "C" int wr_render_bdpt(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_render_params* prm,
+                              wr_film* film, wr_render_stats* stats) {
+    if (!ctx || !s || !cam || !prm || !film) return fail(ctx, WR_EINVAL, "wr_render_bdpt: bad argument");
+    if (!s->built) return fail(ctx, WR_ESTATE, "wr_render_bdpt: scene not built");
+    if (film->W != cam->width || film->H != cam->height)
+        return fail(ctx, WR_EINVAL, "wr_render_bdpt: film size differs from the camera resolution");
+    if (prm->mode != 0 || prm->wave_slots || prm->wave_grid || prm->flags)
+        return fail(ctx, WR_EINVAL, "wr_render_bdpt: mode, wave_slots, wave_grid and flags must be 0");
+    if (prm->max_depth > kBdptMaxDepth) return fail(ctx, WR_EINVAL, "wr_render_bdpt: max_depth above 16");
+    for (const HostObject& o : s->objs)
+        if (o.moving) return fail(ctx, WR_EINVAL, "wr_render_bdpt: moving objects are not supported");
+    const uint64_t npix = (uint64_t)cam->width * cam->height;
+    if (npix == 0 || npix >= (1ull << 31)) return fail(ctx, WR_EINVAL, "wr_render_bdpt: bad image size");
+    (void)hipSetDevice(ctx->device);
+    const CamHost ch = make_cam(*cam);
+    const uint32_t maxv = prm->max_depth + 3;  // the longest sub-path: first ray, max_depth scatters, the end
+    void* verts = nullptr;
+    float* pix = nullptr;
+    hipError_t e = hipMalloc(&verts, bdpt_vertex_bytes() * 2 * maxv * npix);
+    if (e == hipSuccess) e = hipMalloc((void**)&pix, sizeof(float) * 3 * npix);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+    const FilmBufs fb{film->sum, film->comp, film->sumsq};
+    for (uint32_t k = 0; k < prm->spp && e == hipSuccess; k++)
+        e = launch_bdpt_pass(s->dev, ch, prm->max_depth, prm->seed, film->passes + k, maxv, verts, pix, fb,
+                             ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    (void)hipFree(verts);
+    (void)hipFree(pix);
+    if (e != hipSuccess) return hip_fail(ctx, e, "wr_render_bdpt");
+    film->passes += prm->spp;
+    if (stats) {
+        stats->paths = npix * prm->spp;
+        stats->iterations = 0;
+        stats->kernel_ms = ms;
+    }
+    return WR_OK;
+}
+
 extern "C" int wr_set_render_tuning(wr_ctx* ctx, const wr_render_tuning* t) {
     if (!ctx || !t) return fail(ctx, WR_EINVAL, "wr_set_render_tuning: bad argument");
     const uint32_t lv = t->lds_levels ? t->lds_levels : 8u;

@@ -31,6 +31,7 @@ constexpr uint32_t kMaxScenes = 64;      // resident scenes of one engine
 constexpr int kGraphIters = 8;           // wavefront iterations per graph replay (even)
 constexpr int kTailIters = 2;            // iterations per replay once every job has issued all its paths (even)
 constexpr uint32_t kDefaultPathSlots = 1u << 22;  // 4M paths in flight: 2 x 416 MB of path state
+constexpr uint32_t kBdptMaxDepth = 16;           // BidirectionalPathTracer: maxPathLen limit of wr_render_bdpt
 
 enum : uint32_t { kJobFree = 0, kJobRunning = 1, kJobDone = 2 };
 enum : uint32_t { kJobNee = 1 };  // JobDesc::flags: PathTracer's next-event branch
@@ -103,6 +104,12 @@ void fill_job_camera(JobDesc& J, const CamHost& c);
 
 hipError_t launch_pt_mega(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass,
                           bool nee, const FilmBufs& film, hipStream_t st);
+// One BidirectionalPathTracer pass (bdpt.hip): verts holds 2 * maxv vertex
+// records (bdpt_vertex_bytes() each) per pixel, pix 3 floats per pixel.
+size_t bdpt_vertex_bytes();
+hipError_t launch_bdpt_pass(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass,
+                            uint32_t maxv, void* verts, float* pix, const FilmBufs& film, hipStream_t st);
+
 hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
                          uint32_t npass, bool nee, const FilmBufs& film, Engine& engine, uint32_t cap, uint32_t grid,
                          WaveStats* stats, hipStream_t st, uint32_t lds_levels = 8);
