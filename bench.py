@@ -11,16 +11,16 @@ measured with HIP events on the stream the kernels run on.
 The same JSON line also reports, outside `value`, the path-tracing workloads
 (Cornell box 256^2 x 16 spp and the 512-object field at 1920x1080 x 64 spp,
 megakernel vs wavefront samples/s, with and without next-event estimation),
-the light sub-paths and photon hash grid of BDPT / VCM / PPM on the field, and
-the served-jobs workload (p50/p99 latency and aggregate samples/s under
+the light sub-paths and photon hash grid of BDPT / VCM / PPM on the field, the
+BidirectionalPathTracer on the Cornell box, and the served-jobs workload (p50/p99 latency and aggregate samples/s under
 Poisson arrivals).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pt] [--legs ch,pt,lp,serve]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pt] [--legs ch,pt,lp,bdpt,serve]
 
 --legs picks the workloads (closest hit, path tracing, light paths + photon
-grid, serving, and c5: the 43.8M-triangle large scene at 3840x2160 x 256 spp,
-not in the default run); a run without the closest-hit leg reports value null
-(used to profile one leg).
+grid, BidirectionalPathTracer on the Cornell box, serving, and c5: the
+43.8M-triangle large scene at 3840x2160 x 256 spp, not in the default run); a
+run without the closest-hit leg reports value null (used to profile one leg).
 """
 from __future__ import annotations
 
@@ -186,6 +186,21 @@ def cpu_baseline(S, objs, coh, inc, sample=1 << 21):
     }
 
 
+def cpu_baseline_bdpt(S, passes=2, depth=8):
+    """The oracle's BidirectionalPathTracer restatement on the host cores, over a bounded
+    sample of the BDPT leg's workload (2 passes of the same 256x256 Cornell image)."""
+    from oracle import oracle as O
+    threads = min(os.cpu_count() or 1, 16)
+    objs, cam = S.cornell(k=5, width=256, height=256)
+    osc = O.Scene(objs)
+    t0 = time.perf_counter()
+    osc.render_bdpt(cam, passes, depth, 1234, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(cam["width"] * cam["height"] * passes / dt, 1), "unit": "samples/s", "cores": threads,
+            "kind": "port", "sample": f"{passes} passes of the same 256x256 image, oracle BidirectionalPathTracer "
+                                      "restatement, OpenMP over pixels"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,7 +208,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pt", action="store_true", help="same as --legs ch")
-    ap.add_argument("--legs", default="ch,pt,lp,serve")
+    ap.add_argument("--legs", default="ch,pt,lp,bdpt,serve")
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (counter calibration)")
@@ -223,6 +238,13 @@ def main():
     if "lp" in legs:
         from walkray_amd import render as R
         extra["light_paths"] = R.bench_light_paths(ctx, log)
+    if "bdpt" in legs:
+        from walkray_amd import render as R
+        extra["bdpt"] = R.bench_bdpt(ctx, log)
+        if not args.no_cpu:
+            extra["bdpt"]["cpu_baseline"] = cpu_baseline_bdpt(S)
+            log(f"[bench] BDPT oracle: {extra['bdpt']['cpu_baseline']['value'] / 1e3:.1f} Ksamples/s on "
+                f"{extra['bdpt']['cpu_baseline']['cores']} cores")
     if "c5" in legs:  # the large scene: minutes of host build and rendering, not in the default run
         from walkray_amd import render as R
         extra["large_scene"] = R.bench_large(W, S, ctx, log)

@@ -15,10 +15,13 @@
 //      lane's pixel, light-tracing ones (the camera vertex alone on the eye
 //      side) are splatted to the pixel Camera::convToRaster names
 //      (camera.cpp:148-170), scaled by 1 / pixel solid angle / pixel count.
-// The sub-paths live in device memory (one block of records per lane).  The
-// pixel values of a pass are summed with float atomics (the splats of other
-// lanes land on any pixel), then k_bdpt_fold applies isLegalColor and
-// fixVignetting and adds them to the Kahan film.
+// The sub-paths live in device memory (one block of records per lane).  A
+// launch runs several passes side by side (lane = pass x pixel: one pass of a
+// 256^2 image is only 1,024 waves of long, serial lanes).  The pixel values
+// of each pass are summed with float atomics in a buffer of their own (the
+// splats of other lanes land on any pixel), then k_bdpt_fold applies
+// isLegalColor and fixVignetting and adds them to the Kahan film in pass
+// order.
 #include <hip/hip_runtime.h>
 
 #include "pt_dev.h"
@@ -308,13 +311,17 @@ __device__ __forceinline__ void atomic_add3(float* buf, uint32_t p, f3 v) {
 }
 
 __global__ __launch_bounds__(256) void k_bdpt(SceneDev S, CamDev cam, uint32_t max_depth, uint64_t seed,
-                                              uint32_t pass, uint32_t npix, uint32_t maxv, BVert* verts, float* pix) {
-    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= npix) return;
+                                              uint32_t pass0, uint32_t npass, uint32_t npix, uint32_t maxv,
+                                              BVert* verts, float* pix_all) {
+    const uint64_t lane = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (lane >= (uint64_t)npass * npix) return;
+    const uint32_t k = (uint32_t)(lane / npix), p = (uint32_t)(lane % npix);
+    const uint32_t pass = pass0 + k;
+    float* pix = pix_all + 3 * (size_t)k * npix;
     StackSpill spill;
     LaneStack st(&spill);
     const SceneCtx sc = make_scene_ctx(S);
-    BVert* E = verts + (size_t)p * 2 * maxv;
+    BVert* E = verts + lane * 2 * maxv;
     BVert* L = E + maxv;
     uint32_t ne = 0, nl = 0;
     float rn[4];
@@ -426,13 +433,17 @@ __global__ __launch_bounds__(256) void k_bdpt(SceneDev S, CamDev cam, uint32_t m
     atomic_add3(pix, p, own);
 }
 
-// The pass's pixel values: isLegalColor, fixVignetting, the Kahan film.
-__global__ __launch_bounds__(256) void k_bdpt_fold(CamDev cam, uint32_t npix, const float* pix, FilmDev film) {
+// Each pass's pixel values, in pass order: isLegalColor, fixVignetting, the Kahan film.
+__global__ __launch_bounds__(256) void k_bdpt_fold(CamDev cam, uint32_t npass, uint32_t npix, const float* pix,
+                                                   FilmDev film) {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
     if (p >= npix) return;
-    f3 v = mk(pix[3 * p], pix[3 * p + 1], pix[3 * p + 2]);
-    if (!legal(v)) v = mk(0.f, 0.f, 0.f);
-    film_add(film, p, fix_vignetting(cam, v, p));
+    for (uint32_t k = 0; k < npass; k++) {
+        const float* c = pix + 3 * ((size_t)k * npix + p);
+        f3 v = mk(c[0], c[1], c[2]);
+        if (!legal(v)) v = mk(0.f, 0.f, 0.f);
+        film_add(film, p, fix_vignetting(cam, v, p));
+    }
 }
 
 }  // namespace wrd
@@ -441,8 +452,9 @@ namespace wr {
 
 size_t bdpt_vertex_bytes() { return sizeof(wrd::BVert); }
 
-hipError_t launch_bdpt_pass(const SceneDev& S, const CamHost& ch, uint32_t max_depth, uint64_t seed, uint32_t pass,
-                            uint32_t maxv, void* verts, float* pix, const FilmBufs& film, hipStream_t st) {
+hipError_t launch_bdpt_passes(const SceneDev& S, const CamHost& ch, uint32_t max_depth, uint64_t seed, uint32_t pass0,
+                              uint32_t npass, uint32_t maxv, void* verts, float* pix, const FilmBufs& film,
+                              hipStream_t st) {
     wrd::CamDev cam;
     cam.pos = wrd::f3{ch.pos[0], ch.pos[1], ch.pos[2]};
     cam.front = wrd::f3{ch.front[0], ch.front[1], ch.front[2]};
@@ -452,12 +464,13 @@ hipError_t launch_bdpt_pass(const SceneDev& S, const CamHost& ch, uint32_t max_d
     cam.W = ch.W;
     cam.H = ch.H;
     const uint32_t npix = ch.W * ch.H;
-    const uint32_t grid = (npix + 255) / 256;
-    hipError_t e = hipMemsetAsync(pix, 0, sizeof(float) * 3 * (size_t)npix, st);
+    const uint64_t lanes = (uint64_t)npass * npix;
+    hipError_t e = hipMemsetAsync(pix, 0, sizeof(float) * 3 * (size_t)lanes, st);
     if (e != hipSuccess) return e;
-    wrd::k_bdpt<<<grid, 256, wrd::kStackLdsBytes, st>>>(S, cam, max_depth, seed, pass, npix, maxv,
-                                                       (wrd::BVert*)verts, pix);
-    wrd::k_bdpt_fold<<<grid, 256, 0, st>>>(cam, npix, pix, wrd::FilmDev{film.sum, film.comp, film.sumsq});
+    wrd::k_bdpt<<<(uint32_t)((lanes + 255) / 256), 256, wrd::kStackLdsBytes, st>>>(S, cam, max_depth, seed, pass0, npass,
+                                                                                   npix, maxv, (wrd::BVert*)verts, pix);
+    wrd::k_bdpt_fold<<<(npix + 255) / 256, 256, 0, st>>>(cam, npass, npix, pix,
+                                                         wrd::FilmDev{film.sum, film.comp, film.sumsq});
     return hipGetLastError();
 }
 

@@ -1,6 +1,7 @@
 // render.cpp -- extern "C" film and render entry points (include/walkray.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -229,15 +230,21 @@ extern "C" int wr_render_bdpt(wr_ctx* ctx, const wr_scene* s, const wr_camera* c
     (void)hipSetDevice(ctx->device);
     const CamHost ch = make_cam(*cam);
     const uint32_t maxv = prm->max_depth + 3;  // the longest sub-path: first ray, max_depth scatters, the end
+    // passes per launch: as many as fill the device, within 4 GiB of sub-path records
+    const uint64_t per_pass = bdpt_vertex_bytes() * 2 * maxv * npix;
+    uint64_t batch = std::max<uint64_t>(1, (4ull << 30) / per_pass);
+    batch = std::min<uint64_t>(std::min<uint64_t>(batch, prm->spp ? prm->spp : 1), std::max<uint64_t>(1, (1ull << 22) / npix));
     void* verts = nullptr;
     float* pix = nullptr;
-    hipError_t e = hipMalloc(&verts, bdpt_vertex_bytes() * 2 * maxv * npix);
-    if (e == hipSuccess) e = hipMalloc((void**)&pix, sizeof(float) * 3 * npix);
+    hipError_t e = hipMalloc(&verts, per_pass * batch);
+    if (e == hipSuccess) e = hipMalloc((void**)&pix, sizeof(float) * 3 * npix * batch);
     if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
     const FilmBufs fb{film->sum, film->comp, film->sumsq};
-    for (uint32_t k = 0; k < prm->spp && e == hipSuccess; k++)
-        e = launch_bdpt_pass(s->dev, ch, prm->max_depth, prm->seed, film->passes + k, maxv, verts, pix, fb,
-                             ctx->stream);
+    for (uint32_t k = 0; k < prm->spp && e == hipSuccess; k += (uint32_t)batch) {
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(batch, prm->spp - k);
+        e = launch_bdpt_passes(s->dev, ch, prm->max_depth, prm->seed, film->passes + k, nb, maxv, verts, pix, fb,
+                               ctx->stream);
+    }
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
     if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
     float ms = 0.f;

@@ -104,11 +104,13 @@ void fill_job_camera(JobDesc& J, const CamHost& c);
 
 hipError_t launch_pt_mega(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass,
                           bool nee, const FilmBufs& film, hipStream_t st);
-// One BidirectionalPathTracer pass (bdpt.hip): verts holds 2 * maxv vertex
-// records (bdpt_vertex_bytes() each) per pixel, pix 3 floats per pixel.
+// BidirectionalPathTracer passes [pass0, pass0 + npass) side by side (bdpt.hip):
+// verts holds 2 * maxv vertex records (bdpt_vertex_bytes() each) per pixel and
+// pass, pix 3 floats per pixel and pass.
 size_t bdpt_vertex_bytes();
-hipError_t launch_bdpt_pass(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass,
-                            uint32_t maxv, void* verts, float* pix, const FilmBufs& film, hipStream_t st);
+hipError_t launch_bdpt_passes(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
+                              uint32_t npass, uint32_t maxv, void* verts, float* pix, const FilmBufs& film,
+                              hipStream_t st);
 
 hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
                          uint32_t npass, bool nee, const FilmBufs& film, Engine& engine, uint32_t cap, uint32_t grid,
