@@ -256,7 +256,7 @@ int wr_render(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, const wr
  * (bidirPathTracer.cpp:53-313), light-tracing contributions splatted to the
  * pixel the camera sees them in; the pass's pixel values checked
  * (isLegalColor), vignetting-fixed and added to the film.  params: spp,
- * max_depth (maxPathLen, at most 16) and seed; mode, wave_* and flags must
+ * max_depth (maxPathLen, at most 32) and seed; mode, wave_* and flags must
  * be 0.  The splats are summed with float atomics, so the film is
  * reproducible up to their summation order.  Static scenes only. */
 int wr_render_bdpt(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, const wr_render_params* params,

@@ -57,7 +57,7 @@ def test_bdpt_arguments(ctx):
     sc = W.Scene(ctx, objs)
     film = W.Film(ctx, 8, 8)
     with pytest.raises(W.WalkRayError):
-        W.render_bdpt(sc, cam, film, 1, 17, 1)  # maxPathLen above 16
+        W.render_bdpt(sc, cam, film, 1, 33, 1)  # maxPathLen above 32
     wrong = W.Film(ctx, 4, 8)
     with pytest.raises(W.WalkRayError):
         W.render_bdpt(sc, cam, wrong, 1, 4, 1)

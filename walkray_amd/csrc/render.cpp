@@ -222,7 +222,7 @@ extern "C" int wr_render_bdpt(wr_ctx* ctx, const wr_scene* s, const wr_camera* c
         return fail(ctx, WR_EINVAL, "wr_render_bdpt: film size differs from the camera resolution");
     if (prm->mode != 0 || prm->wave_slots || prm->wave_grid || prm->flags)
         return fail(ctx, WR_EINVAL, "wr_render_bdpt: mode, wave_slots, wave_grid and flags must be 0");
-    if (prm->max_depth > kBdptMaxDepth) return fail(ctx, WR_EINVAL, "wr_render_bdpt: max_depth above 16");
+    if (prm->max_depth > kBdptMaxDepth) return fail(ctx, WR_EINVAL, "wr_render_bdpt: max_depth above 32");
     for (const HostObject& o : s->objs)
         if (o.moving) return fail(ctx, WR_EINVAL, "wr_render_bdpt: moving objects are not supported");
     const uint64_t npix = (uint64_t)cam->width * cam->height;

@@ -31,7 +31,7 @@ constexpr uint32_t kMaxScenes = 64;      // resident scenes of one engine
 constexpr int kGraphIters = 8;           // wavefront iterations per graph replay (even)
 constexpr int kTailIters = 2;            // iterations per replay once every job has issued all its paths (even)
 constexpr uint32_t kDefaultPathSlots = 1u << 22;  // 4M paths in flight: 2 x 416 MB of path state
-constexpr uint32_t kBdptMaxDepth = 16;           // BidirectionalPathTracer: maxPathLen limit of wr_render_bdpt
+constexpr uint32_t kBdptMaxDepth = 32;           // BidirectionalPathTracer: maxPathLen limit of wr_render_bdpt
 
 enum : uint32_t { kJobFree = 0, kJobRunning = 1, kJobDone = 2 };
 enum : uint32_t { kJobNee = 1 };  // JobDesc::flags: PathTracer's next-event branch

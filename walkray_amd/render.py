@@ -97,7 +97,8 @@ def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
     out["c1"] = r
     objs, cam = S.random_field()
     sc = W.Scene(ctx, objs)
-    tune = W.autotune_render(sc, cam, 16, spp=2)
+    # 8 passes (16.6M paths): enough for the 1M-8M path pools to differ as they do over the full render
+    tune = W.autotune_render(sc, cam, 16, spp=8)
     log(f"[bench] C3 wavefront autotune: {tune}")
     r, film = _compare(ctx, sc, cam, c3_spp, 16, 77, log, "C3 field", repeats=1)
     r["wavefront_tuning"] = tune
