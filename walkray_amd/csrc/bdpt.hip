@@ -10,7 +10,7 @@
 //   3. every (light length, eye length) split of every joined length
 //      (bidirPathTracer.cpp:53-140): connectEdge (294-324), the visibility
 //      test (mcRenderer.cpp:59-65), connectingColorProb (266-291) and
-//      connectingMisWeight (142-221) with linkVertex (228-243);
+//      connectingMisWeight (142-221) with linkVertex (228-243), in one sweep;
 //   4. contributions with an eye side of two or more vertices go to the
 //      lane's pixel, light-tracing ones (the camera vertex alone on the eye
 //      side) are splatted to the pixel Camera::convToRaster names
@@ -180,19 +180,6 @@ struct Joined {
     __device__ __forceinline__ bool delta(int i) const { return (at(i).flags & kVDelta) != 0; }
 };
 
-// linkVertex (bidirPathTracer.cpp:228-243)
-__device__ __forceinline__ BVert link_vertex(const Joined& P, int start, bool reverse) {
-    const int end = reverse ? start - 1 : start + 1;
-    const int last = P.n - 1;
-    const BVert s = P.at(start < 0 ? 0 : start > last ? last : start);
-    BVert r = s;
-    if ((reverse && start > P.cl) || (!reverse && start <= P.cl)) return r;
-    const BVert e = P.at(end < 0 ? 0 : end > last ? last : end);
-    r.d = normalize(sub(e.o, s.o));
-    r.inside = e.inside;
-    return r;
-}
-
 // pathCanNotConnect (bidirPathTracer.cpp:251-257)
 __device__ __forceinline__ bool cannot_connect(const Joined& P) {
     for (int i = 0; i + 1 < P.n; i++)
@@ -232,70 +219,85 @@ __device__ bool connect_edge(const SceneDev& S, Joined& P) {
     return true;
 }
 
-// connectingColorProb (bidirPathTracer.cpp:266-291); no participating media.
-__device__ void color_prob(const Joined& P, f3& color, float& prob) {
+// connectingColorProb (bidirPathTracer.cpp:266-291; no participating media)
+// and connectingMisWeight (142-221, expTerm 1; every evalOriginProbability
+// there compares two temporaries, so it is 1) in one sweep over the joined
+// path, each record read once.  linkVertex(j, reverse = false) is record j
+// re-aimed at j + 1 when j > cl, linkVertex(j, true) record j re-aimed at
+// j - 1 when j <= cl; the weight reads only the cosine term and the
+// direction probability of those links (evalDirectionProbability ignores its
+// incoming ray), so the sweep keeps those per record and the two recurrences
+// run over them afterwards, in the reference's order.
+__device__ float color_prob_weight(const SceneDev& S, const CamDev& cam, uint32_t pixel, const Joined& P, f3& color,
+                                   float& prob) {
+    float cos_f[kBdptMaxJoined], cos_b[kBdptMaxJoined], dp_f[kBdptMaxJoined], dp_b[kBdptMaxJoined];
+    float dist[kBdptMaxJoined], op[kBdptMaxJoined];
+    uint8_t delta[kBdptMaxJoined];
+    const int n = P.n, cl = P.cl;
     color = mk(1.f, 1.f, 1.f);
     prob = 1.f;
-    for (int i = 0; i < P.n; i++) {
-        const BVert v = P.at(i);
-        color = mulv(color, v.rad);
-        float dist = 0.f;
-        if (i <= P.cl) {
-            const float l = len(sub(v.o, P.at(i + 1).o));
-            dist = (double)l > kEps ? l : (float)kEps;
-        } else if (i > P.cl + 1) {
-            const float l = len(sub(P.at(i - 1).o, v.o));
-            dist = (double)l > kEps ? l : (float)kEps;
+    BVert prev = P.at(0), cur = prev, next = prev;
+    for (int j = 0; j < n; j++) {
+        if (j + 1 < n) next = P.at(j + 1);
+        BVert f = cur, b = cur;
+        if (j > cl) {  // linkVertex(j, false)
+            const BVert& e = j + 1 < n ? next : cur;
+            f.d = normalize(sub(e.o, cur.o));
+            f.inside = e.inside;
         }
-        if (i == P.cl && i < P.n - 1) color = muls(color, cos_term_v(v) * cos_term_v(P.at(i + 1)) / (dist * dist));
-        if (i != P.cl && i != P.cl + 1) color = muls(color, cos_term_v(v));
-        prob *= v.dp * v.op;
+        if (j <= cl) {  // linkVertex(j, true)
+            const BVert& e = j > 0 ? prev : cur;
+            b.d = normalize(sub(e.o, cur.o));
+            b.inside = e.inside;
+        }
+        cos_f[j] = cos_term_v(f);
+        cos_b[j] = cos_term_v(b);
+        dp_f[j] = eval_dir_prob(S, cam, pixel, f, f, j == 0);
+        dp_b[j] = eval_dir_prob(S, cam, pixel, b, b, j == n - 1);
+        delta[j] = (cur.flags & kVDelta) ? 1 : 0;
+        op[j] = cur.op;
+        if (j > 0) {
+            const float l = len(sub(cur.o, prev.o));
+            dist[j] = (double)l > kEps ? l : (float)kEps;
+        }
+        // connectingColorProb
+        color = mulv(color, cur.rad);
+        if (j == cl && j < n - 1) {
+            const float l = len(sub(cur.o, next.o));
+            const float d = (double)l > kEps ? l : (float)kEps;
+            color = muls(color, cos_term_v(cur) * cos_term_v(next) / (d * d));
+        }
+        if (j != cl && j != cl + 1) color = muls(color, cos_term_v(cur));
+        prob *= cur.dp * cur.op;
+        prev = cur;
+        cur = next;
     }
-}
-
-// connectingMisWeight (bidirPathTracer.cpp:142-221) with expTerm 1; every
-// evalOriginProbability there compares two temporaries, so it is 1.
-__device__ float mis_weight(const SceneDev& S, const CamDev& cam, uint32_t pixel, const Joined& P) {
+    if (prob == 0.f || len(color) == 0.f) return 0.f;  // the caller skips these
     double direct[kBdptMaxJoined], reverse[kBdptMaxJoined];
-    float dist_rec[kBdptMaxJoined];
-    const int n = P.n;
-    direct[0] = P.at(0).op;
+    direct[0] = op[0];
     for (int i = 1; i < n; i++) {
-        const float l = len(sub(P.at(i).o, P.at(i - 1).o));
-        const float dist = (double)l > kEps ? l : (float)kEps;
-        dist_rec[i - 1] = dist;
-        const float cos_there = cos_term_v(link_vertex(P, i, true));
-        direct[i] = direct[i - 1] * (double)(1.0f * fabsf(cos_there) / (dist * dist));  // PdfWtoA
-        if (P.delta(i - 1)) continue;
-        float link_dir;
-        if (i > 1) {
-            link_dir = eval_dir_prob(S, cam, pixel, link_vertex(P, i - 2, false), link_vertex(P, i - 1, false), false);
-        } else {
-            const BVert r = link_vertex(P, i - 1, false);
-            link_dir = eval_dir_prob(S, cam, pixel, r, r, true);
-        }
-        direct[i] *= link_dir * 1.f;
+        const float d = dist[i];
+        direct[i] = direct[i - 1] * (double)(1.0f * fabsf(cos_b[i]) / (d * d));  // PdfWtoA
+        if (delta[i - 1]) continue;
+        direct[i] *= dp_f[i - 1] * 1.f;
     }
-    reverse[n - 1] = P.at(n - 1).op;
+    reverse[n - 1] = op[n - 1];
     for (int i = n - 2; i >= 0; i--) {
-        const float dist = dist_rec[i];
-        const float cos_there = cos_term_v(link_vertex(P, i, false));
-        reverse[i] = reverse[i + 1] * (double)(1.0f * fabsf(cos_there) / (dist * dist));
-        if (P.delta(i + 1)) continue;
-        float link_dir;
-        if (i < n - 2) {
-            link_dir = eval_dir_prob(S, cam, pixel, link_vertex(P, i + 2, true), link_vertex(P, i + 1, true), false);
-        } else {
-            const BVert r = link_vertex(P, i + 1, true);
-            link_dir = eval_dir_prob(S, cam, pixel, r, r, true);
-        }
-        reverse[i] *= link_dir * 1.f;
+        const float d = dist[i + 1];
+        reverse[i] = reverse[i + 1] * (double)(1.0f * fabsf(cos_f[i]) / (d * d));
+        if (delta[i + 1]) continue;
+        reverse[i] *= dp_b[i + 1] * 1.f;
     }
     double all = 0.0;
-    for (int i = 0; i < n - 1; i++)
-        if (!P.delta(i) && !P.delta(i + 1)) all += direct[i] * reverse[i + 1];
-    if (cannot_connect(P)) all += reverse[0];
-    const double self = P.cl == -1 ? reverse[0] : direct[P.cl] * reverse[P.cl + 1];
+    bool cannot = true;  // pathCanNotConnect (bidirPathTracer.cpp:251-257)
+    for (int i = 0; i < n - 1; i++) {
+        if (!delta[i] && !delta[i + 1]) {
+            all += direct[i] * reverse[i + 1];
+            cannot = false;
+        }
+    }
+    if (cannot) all += reverse[0];
+    const double self = cl == -1 ? reverse[0] : direct[cl] * reverse[cl + 1];
     const double w = self / all;
     return (float)(w > 0.0 ? w : 0.0);
 }
@@ -408,10 +410,9 @@ __global__ __launch_bounds__(256) void k_bdpt(SceneDev S, CamDev cam, uint32_t m
             }
             f3 color;
             float prob;
-            color_prob(P, color, prob);
+            const float w = color_prob_weight(S, cam, p, P, color, prob);
             if (prob == 0.f) continue;
             if (len(color) == 0.f) continue;
-            const float w = mis_weight(S, cam, p, P);
             const f3 contrib = muls(divs(color, prob), w);
             if (P.ne > 1) {
                 own = add(own, contrib);
