@@ -256,8 +256,11 @@ int wr_render(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, const wr
  * (bidirPathTracer.cpp:53-313), light-tracing contributions splatted to the
  * pixel the camera sees them in; the pass's pixel values checked
  * (isLegalColor), vignetting-fixed and added to the film.  params: spp,
- * max_depth (maxPathLen, at most 32) and seed; mode, wave_* and flags must
- * be 0.  The splats are summed with float atomics, so the film is
+ * max_depth (maxPathLen, at most 32), seed, and wave_slots: the entries of
+ * the visibility queue (connection rays traced in a batch after the
+ * connections are formed; 0 = automatic, a full queue makes the remaining
+ * connections trace in place); mode, wave_grid and flags must be 0.  The
+ * contributions of a pixel are summed with float atomics, so the film is
  * reproducible up to their summation order.  Static scenes only. */
 int wr_render_bdpt(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, const wr_render_params* params,
                    wr_film* film, wr_render_stats* stats);

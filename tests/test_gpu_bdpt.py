@@ -52,6 +52,22 @@ def test_bdpt_matches_oracle(ctx, glass, depth):
     assert g["mean"].mean() > 0
 
 
+def test_bdpt_small_queue_traces_in_place(ctx):
+    """A 64-entry visibility queue: nearly every connection is traced in place
+    by the lane that forms it; the film equals the queued one up to the float
+    atomics' summation order."""
+    objs, cam = S.cornell(k=2, width=24, height=24)
+    sc = W.Scene(ctx, objs)
+    films = []
+    for queue in (0, 64):
+        film = W.Film(ctx, cam["width"], cam["height"])
+        W.render_bdpt(sc, cam, film, 8, 6, 5, queue=queue)
+        films.append(film.read()["sum"].astype(np.float64))
+    a, b = films
+    assert np.allclose(a, b, rtol=1e-4, atol=1e-6), np.abs(a - b).max()
+    assert a.sum() > 0
+
+
 def test_bdpt_arguments(ctx):
     objs, cam = S.cornell(k=1, width=8, height=8)
     sc = W.Scene(ctx, objs)

@@ -492,10 +492,11 @@ def render(scene, cam, film, spp, max_depth, seed, mode=WR_WAVEFRONT, wave_slots
     return {"paths": st.paths, "iterations": st.iterations, "kernel_ms": st.kernel_ms}
 
 
-def render_bdpt(scene, cam, film, spp, max_depth, seed):
-    """BidirectionalPathTracer::renderPixels for `spp` more passes into `film`; returns stats."""
+def render_bdpt(scene, cam, film, spp, max_depth, seed, queue=0):
+    """BidirectionalPathTracer::renderPixels for `spp` more passes into `film`; returns stats.
+    queue: visibility-queue entries (0 = automatic)."""
     cd = CameraDesc.from_dict(cam)
-    prm = RenderParams(spp, max_depth, seed, 0, 0, 0, 0)
+    prm = RenderParams(spp, max_depth, seed, 0, queue, 0, 0)
     st = RenderStats()
     _check(lib().wr_render_bdpt(scene.ctx.h, scene.h, C.byref(cd), C.byref(prm), film.h, C.byref(st)), scene.ctx.h,
            "wr_render_bdpt")

@@ -15,6 +15,12 @@
 //      lane's pixel, light-tracing ones (the camera vertex alone on the eye
 //      side) are splatted to the pixel Camera::convToRaster names
 //      (camera.cpp:148-170), scaled by 1 / pixel solid angle / pixel count.
+// The visibility tests of the connections are not traced by the lane that
+// forms them: each connection that passes every other test appends its
+// visibility ray (and its contribution and target pixel) to a queue, traced
+// afterwards one ray per lane by the batch NoSelf query (k_closest_hit_path:
+// the same traversal, so the same answers) and folded in by k_bdpt_accum; a
+// lane whose queue is full traces in place.
 // The sub-paths live in device memory (one block of records per lane).  A
 // launch runs several passes side by side (lane = pass x pixel: one pass of a
 // 256^2 image is only 1,024 waves of long, serial lanes).  The pixel values
@@ -306,6 +312,29 @@ __device__ __forceinline__ bool emissive(const SceneDev& S, uint32_t obj) {
     return obj != WR_NONE && S.objs[obj].mat == WR_MAT_EMISSIVE;
 }
 
+// One slot of a queue per active lane, one atomic per wave.
+__device__ __forceinline__ uint32_t wave_slot(uint32_t* counter) {
+    const uint64_t mask = __ballot(1);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+    const uint32_t rank = __popcll(mask & ((1ull << lane) - 1));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + rank;
+}
+
+// The visibility queue: rays in the batch NoSelf layout (o, d, inside,
+// contact), per entry the contribution and the distance to the light-side
+// record, and the target (pass buffer offset + pixel).
+struct VisQueue {
+    uint32_t* rays;   // 8 words per entry
+    float* pay;       // 4 floats per entry: contribution, distance
+    uint32_t* dst;    // pixel index into the pass buffers (k * npix + q)
+    uint32_t* count;
+    uint32_t cap;
+};
+
 __device__ __forceinline__ void atomic_add3(float* buf, uint32_t p, f3 v) {
     atomicAdd(buf + 3 * p + 0, v.x);
     atomicAdd(buf + 3 * p + 1, v.y);
@@ -314,7 +343,7 @@ __device__ __forceinline__ void atomic_add3(float* buf, uint32_t p, f3 v) {
 
 __global__ __launch_bounds__(256) void k_bdpt(SceneDev S, CamDev cam, uint32_t max_depth, uint64_t seed,
                                               uint32_t pass0, uint32_t npass, uint32_t npix, uint32_t maxv,
-                                              BVert* verts, float* pix_all) {
+                                              BVert* verts, float* pix_all, VisQueue Q) {
     const uint64_t lane = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (lane >= (uint64_t)npass * npix) return;
     const uint32_t k = (uint32_t)(lane / npix), p = (uint32_t)(lane % npix);
@@ -399,39 +428,70 @@ __global__ __launch_bounds__(256) void k_bdpt(SceneDev S, CamDev cam, uint32_t m
             P.edited = false;
             if (!emissive(S, P.at(0).contact)) continue;
             if (light_len == 0 && !cannot_connect(P)) continue;
-            if (P.ne > 0 && light_len > 0) {
+            const bool connected = P.ne > 0 && light_len > 0;
+            if (connected) {
                 if (light_len > 1 && emissive(S, P.at(P.cl).contact)) continue;
                 if (emissive(S, P.at(P.cl + 1).contact)) continue;
                 if (!connect_edge(S, P)) continue;
-                // MCRenderer::visibilityTest from the eye-side connection record
-                const uint32_t contact = (P.ec.flags & kVCamera) ? kCameraObj : P.ec.contact;
-                const PHit h = trace_scene<true>(sc, P.ec.o, P.ec.d, P.ec.inside, contact, st);
-                if (!nee_visible(h.rec, h.t, len(sub(P.ec.o, P.lc.o)))) continue;
             }
             f3 color;
             float prob;
             const float w = color_prob_weight(S, cam, p, P, color, prob);
             if (prob == 0.f) continue;
             if (len(color) == 0.f) continue;
-            const f3 contrib = muls(divs(color, prob), w);
-            if (P.ne > 1) {
-                own = add(own, contrib);
-            } else {  // light tracing: splat to the pixel the camera sees the point in
+            f3 contrib = muls(divs(color, prob), w);
+            uint32_t target = p;
+            if (P.ne <= 1) {  // light tracing: splat to the pixel the camera sees the point in
                 const f3 v = sub(P.at(whole - 2).o, cam.pos);
                 const float vfd = dot(v, cam.front);
                 const float scale = cam.dist / vfd;
                 const float rx = (float)(cam.W / 2.0 + (double)(dot(v, cam.right) * scale));
                 const float ry = (float)(cam.H / 2.0 - (double)(dot(v, cam.top) * scale));
                 const int x = (int)rx, y = (int)ry;  // checkPixelBound truncates (camera.cpp:164-170)
-                if (x >= 0 && y >= 0 && x < (int)cam.W && y < (int)cam.H) {
-                    const uint32_t q = (uint32_t)(y * (int)cam.W + x);
-                    const float factor = (float)(1.0 / (double)pixel_solid_angle(cam, q));
-                    atomic_add3(pix, q, muls(contrib, factor / (float)npix));
-                }
+                if (!(x >= 0 && y >= 0 && x < (int)cam.W && y < (int)cam.H)) continue;
+                target = (uint32_t)(y * (int)cam.W + x);
+                const float factor = (float)(1.0 / (double)pixel_solid_angle(cam, target));
+                contrib = muls(contrib, factor / (float)npix);
+            }
+            if (!connected) {  // no visibility test (a path found by the eye side alone)
+                if (target == p) own = add(own, contrib);
+                else atomic_add3(pix, target, contrib);
+                continue;
+            }
+            // MCRenderer::visibilityTest from the eye-side connection record
+            const uint32_t contact = (P.ec.flags & kVCamera) ? kCameraObj : P.ec.contact;
+            const float dist = len(sub(P.ec.o, P.lc.o));
+            const uint32_t slot = wave_slot(Q.count);
+            if (slot < Q.cap) {
+                uint32_t* r = Q.rays + 8 * (size_t)slot;
+                r[0] = __float_as_uint(P.ec.o.x); r[1] = __float_as_uint(P.ec.o.y); r[2] = __float_as_uint(P.ec.o.z);
+                r[3] = __float_as_uint(P.ec.d.x); r[4] = __float_as_uint(P.ec.d.y); r[5] = __float_as_uint(P.ec.d.z);
+                r[6] = P.ec.inside;
+                r[7] = contact;
+                float* y = Q.pay + 4 * (size_t)slot;
+                y[0] = contrib.x; y[1] = contrib.y; y[2] = contrib.z; y[3] = dist;
+                Q.dst[slot] = k * npix + target;
+            } else {  // queue full: trace here
+                const PHit h = trace_scene<true>(sc, P.ec.o, P.ec.d, P.ec.inside, contact, st);
+                if (!nee_visible(h.rec, h.t, dist)) continue;
+                if (target == p) own = add(own, contrib);
+                else atomic_add3(pix, target, contrib);
             }
         }
     }
     atomic_add3(pix, p, own);
+}
+
+// The queued visibility tests, answered by the batch NoSelf query (hits:
+// prim WR_NONE on a miss, t): visible connections add their contribution.
+__global__ __launch_bounds__(256) void k_bdpt_accum(uint32_t n, const float* pay, const uint32_t* dst,
+                                                    const uint4* hits, float* pix_all) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint4 h = hits[i];
+    const float* y = pay + 4 * (size_t)i;
+    if (!nee_visible(h.x == WR_NONE ? WR_NONE : 0u, __uint_as_float(h.y), y[3])) return;
+    atomic_add3(pix_all, dst[i], mk(y[0], y[1], y[2]));
 }
 
 // Each pass's pixel values, in pass order: isLegalColor, fixVignetting, the Kahan film.
@@ -453,9 +513,9 @@ namespace wr {
 
 size_t bdpt_vertex_bytes() { return sizeof(wrd::BVert); }
 
-hipError_t launch_bdpt_passes(const SceneDev& S, const CamHost& ch, uint32_t max_depth, uint64_t seed, uint32_t pass0,
-                              uint32_t npass, uint32_t maxv, void* verts, float* pix, const FilmBufs& film,
-                              hipStream_t st) {
+hipError_t launch_bdpt_passes(const SceneDev& S, const wr_tree_hdr& H, const CamHost& ch, uint32_t max_depth,
+                              uint64_t seed, uint32_t pass0, uint32_t npass, uint32_t maxv, void* verts, float* pix,
+                              const BdptQueue& q, const FilmBufs& film, hipStream_t st) {
     wrd::CamDev cam;
     cam.pos = wrd::f3{ch.pos[0], ch.pos[1], ch.pos[2]};
     cam.front = wrd::f3{ch.front[0], ch.front[1], ch.front[2]};
@@ -467,9 +527,21 @@ hipError_t launch_bdpt_passes(const SceneDev& S, const CamHost& ch, uint32_t max
     const uint32_t npix = ch.W * ch.H;
     const uint64_t lanes = (uint64_t)npass * npix;
     hipError_t e = hipMemsetAsync(pix, 0, sizeof(float) * 3 * (size_t)lanes, st);
+    if (e == hipSuccess) e = hipMemsetAsync(q.count, 0, sizeof(uint32_t), st);
     if (e != hipSuccess) return e;
+    const wrd::VisQueue Q{q.rays, q.pay, q.dst, q.count, q.cap};
     wrd::k_bdpt<<<(uint32_t)((lanes + 255) / 256), 256, wrd::kStackLdsBytes, st>>>(S, cam, max_depth, seed, pass0, npass,
-                                                                                   npix, maxv, (wrd::BVert*)verts, pix);
+                                                                                   npix, maxv, (wrd::BVert*)verts, pix, Q);
+    uint32_t nq = 0;
+    e = hipMemcpyAsync(&nq, q.count, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    if (nq > q.cap) nq = q.cap;
+    if (nq) {
+        e = launch_closest_hit_path(S, H, q.rays, nq, q.hits, st);
+        if (e != hipSuccess) return e;
+        wrd::k_bdpt_accum<<<(nq + 255) / 256, 256, 0, st>>>(nq, q.pay, q.dst, (const uint4*)q.hits, pix);
+    }
     wrd::k_bdpt_fold<<<(npix + 255) / 256, 256, 0, st>>>(cam, npass, npix, pix,
                                                          wrd::FilmDev{film.sum, film.comp, film.sumsq});
     return hipGetLastError();

@@ -220,8 +220,8 @@ extern "C" int wr_render_bdpt(wr_ctx* ctx, const wr_scene* s, const wr_camera* c
     if (!s->built) return fail(ctx, WR_ESTATE, "wr_render_bdpt: scene not built");
     if (film->W != cam->width || film->H != cam->height)
         return fail(ctx, WR_EINVAL, "wr_render_bdpt: film size differs from the camera resolution");
-    if (prm->mode != 0 || prm->wave_slots || prm->wave_grid || prm->flags)
-        return fail(ctx, WR_EINVAL, "wr_render_bdpt: mode, wave_slots, wave_grid and flags must be 0");
+    if (prm->mode != 0 || prm->wave_grid || prm->flags)
+        return fail(ctx, WR_EINVAL, "wr_render_bdpt: mode, wave_grid and flags must be 0");
     if (prm->max_depth > kBdptMaxDepth) return fail(ctx, WR_EINVAL, "wr_render_bdpt: max_depth above 32");
     for (const HostObject& o : s->objs)
         if (o.moving) return fail(ctx, WR_EINVAL, "wr_render_bdpt: moving objects are not supported");
@@ -236,14 +236,28 @@ extern "C" int wr_render_bdpt(wr_ctx* ctx, const wr_scene* s, const wr_camera* c
     batch = std::min<uint64_t>(std::min<uint64_t>(batch, prm->spp ? prm->spp : 1), std::max<uint64_t>(1, (1ull << 22) / npix));
     void* verts = nullptr;
     float* pix = nullptr;
+    // the visibility queue: 32 entries per lane, at most 24M (1.6 GB); a full queue
+    // makes the remaining connections trace in place
+    BdptQueue q{};
+    q.cap = prm->wave_slots ? prm->wave_slots : (uint32_t)std::min<uint64_t>(32ull * npix * batch, 24ull << 20);
+    void* qbuf = nullptr;
     hipError_t e = hipMalloc(&verts, per_pass * batch);
     if (e == hipSuccess) e = hipMalloc((void**)&pix, sizeof(float) * 3 * npix * batch);
+    if (e == hipSuccess) e = hipMalloc(&qbuf, kBdptQueueEntryBytes * (size_t)q.cap + 256);
+    if (e == hipSuccess) {
+        char* b = (char*)qbuf;
+        q.rays = (uint32_t*)b;
+        q.hits = b + 32 * (size_t)q.cap;
+        q.pay = (float*)(b + 48 * (size_t)q.cap);
+        q.dst = (uint32_t*)(b + 64 * (size_t)q.cap);
+        q.count = (uint32_t*)(b + 68 * (size_t)q.cap + 128);
+    }
     if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
     const FilmBufs fb{film->sum, film->comp, film->sumsq};
     for (uint32_t k = 0; k < prm->spp && e == hipSuccess; k += (uint32_t)batch) {
         const uint32_t nb = (uint32_t)std::min<uint64_t>(batch, prm->spp - k);
-        e = launch_bdpt_passes(s->dev, ch, prm->max_depth, prm->seed, film->passes + k, nb, maxv, verts, pix, fb,
-                               ctx->stream);
+        e = launch_bdpt_passes(s->dev, s->hdrs[0], ch, prm->max_depth, prm->seed, film->passes + k, nb, maxv, verts,
+                               pix, q, fb, ctx->stream);
     }
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
     if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
@@ -251,6 +265,7 @@ extern "C" int wr_render_bdpt(wr_ctx* ctx, const wr_scene* s, const wr_camera* c
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     (void)hipFree(verts);
     (void)hipFree(pix);
+    (void)hipFree(qbuf);
     if (e != hipSuccess) return hip_fail(ctx, e, "wr_render_bdpt");
     film->passes += prm->spp;
     if (stats) {

@@ -107,10 +107,21 @@ hipError_t launch_pt_mega(const SceneDev& S, const CamHost& cam, uint32_t max_de
 // BidirectionalPathTracer passes [pass0, pass0 + npass) side by side (bdpt.hip):
 // verts holds 2 * maxv vertex records (bdpt_vertex_bytes() each) per pixel and
 // pass, pix 3 floats per pixel and pass.
+// q: the visibility queue (cap entries: rays 32 B, pay 16 B, dst 4 B, hits
+// 16 B each) and its counter.
+struct BdptQueue {
+    uint32_t* rays;
+    float* pay;
+    uint32_t* dst;
+    void* hits;
+    uint32_t* count;
+    uint32_t cap;
+};
+constexpr size_t kBdptQueueEntryBytes = 32 + 16 + 4 + 16;
 size_t bdpt_vertex_bytes();
-hipError_t launch_bdpt_passes(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
-                              uint32_t npass, uint32_t maxv, void* verts, float* pix, const FilmBufs& film,
-                              hipStream_t st);
+hipError_t launch_bdpt_passes(const SceneDev& S, const wr_tree_hdr& H, const CamHost& cam, uint32_t max_depth,
+                              uint64_t seed, uint32_t pass0, uint32_t npass, uint32_t maxv, void* verts, float* pix,
+                              const BdptQueue& q, const FilmBufs& film, hipStream_t st);
 
 hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
                          uint32_t npass, bool nee, const FilmBufs& film, Engine& engine, uint32_t cap, uint32_t grid,
