@@ -341,14 +341,16 @@ __device__ __forceinline__ void atomic_add3(float* buf, uint32_t p, f3 v) {
     atomicAdd(buf + 3 * p + 2, v.z);
 }
 
-__global__ __launch_bounds__(256) void k_bdpt(SceneDev S, CamDev cam, uint32_t max_depth, uint64_t seed,
-                                              uint32_t pass0, uint32_t npass, uint32_t npix, uint32_t maxv,
-                                              BVert* verts, float* pix_all, VisQueue Q) {
+// The two sub-paths of one pixel and pass (kept apart from the connections:
+// this half traverses at a register count that lets several waves share a
+// SIMD); counts[2 * lane] = eye records, counts[2 * lane + 1] = light records.
+__global__ __launch_bounds__(256) void k_bdpt_subpaths(SceneDev S, CamDev cam, uint32_t max_depth, uint64_t seed,
+                                                       uint32_t pass0, uint32_t npass, uint32_t npix, uint32_t maxv,
+                                                       BVert* verts, uint32_t* counts) {
     const uint64_t lane = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (lane >= (uint64_t)npass * npix) return;
     const uint32_t k = (uint32_t)(lane / npix), p = (uint32_t)(lane % npix);
     const uint32_t pass = pass0 + k;
-    float* pix = pix_all + 3 * (size_t)k * npix;
     StackSpill spill;
     LaneStack st(&spill);
     const SceneCtx sc = make_scene_ctx(S);
@@ -412,6 +414,26 @@ __global__ __launch_bounds__(256) void k_bdpt(SceneDev S, CamDev cam, uint32_t m
         }
         walk(sc, r, hit, max_depth, seed, p, pass, kBdptLightVertex + 1, L, nl, maxv, st);
     }
+    counts[2 * lane] = ne;
+    counts[2 * lane + 1] = nl;
+}
+
+__global__ __launch_bounds__(256) void k_bdpt(SceneDev S, CamDev cam, uint32_t max_depth, uint64_t seed,
+                                              uint32_t pass0, uint32_t npass, uint32_t npix, uint32_t maxv,
+                                              BVert* verts, const uint32_t* counts, float* pix_all, VisQueue Q) {
+    (void)max_depth;
+    (void)seed;
+    (void)pass0;
+    const uint64_t lane = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (lane >= (uint64_t)npass * npix) return;
+    const uint32_t k = (uint32_t)(lane / npix), p = (uint32_t)(lane % npix);
+    float* pix = pix_all + 3 * (size_t)k * npix;
+    StackSpill spill;
+    LaneStack st(&spill);
+    const SceneCtx sc = make_scene_ctx(S);
+    BVert* E = verts + lane * 2 * maxv;
+    BVert* L = E + maxv;
+    const uint32_t ne = counts[2 * lane], nl = counts[2 * lane + 1];
     const int max_eye = (int)(ne < maxv ? ne : maxv), max_light = (int)(nl < maxv ? nl : maxv);
     f3 own = mk(0.f, 0.f, 0.f);
     for (int whole = 2; whole <= max_eye + max_light; whole++) {
@@ -530,8 +552,11 @@ hipError_t launch_bdpt_passes(const SceneDev& S, const wr_tree_hdr& H, const Cam
     if (e == hipSuccess) e = hipMemsetAsync(q.count, 0, sizeof(uint32_t), st);
     if (e != hipSuccess) return e;
     const wrd::VisQueue Q{q.rays, q.pay, q.dst, q.count, q.cap};
-    wrd::k_bdpt<<<(uint32_t)((lanes + 255) / 256), 256, wrd::kStackLdsBytes, st>>>(S, cam, max_depth, seed, pass0, npass,
-                                                                                   npix, maxv, (wrd::BVert*)verts, pix, Q);
+    const uint32_t blocks = (uint32_t)((lanes + 255) / 256);
+    wrd::k_bdpt_subpaths<<<blocks, 256, wrd::kStackLdsBytes, st>>>(S, cam, max_depth, seed, pass0, npass, npix, maxv,
+                                                                  (wrd::BVert*)verts, q.counts);
+    wrd::k_bdpt<<<blocks, 256, wrd::kStackLdsBytes, st>>>(S, cam, max_depth, seed, pass0, npass, npix, maxv,
+                                                         (wrd::BVert*)verts, q.counts, pix, Q);
     uint32_t nq = 0;
     e = hipMemcpyAsync(&nq, q.count, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);

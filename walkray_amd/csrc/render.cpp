@@ -244,6 +244,7 @@ extern "C" int wr_render_bdpt(wr_ctx* ctx, const wr_scene* s, const wr_camera* c
     hipError_t e = hipMalloc(&verts, per_pass * batch);
     if (e == hipSuccess) e = hipMalloc((void**)&pix, sizeof(float) * 3 * npix * batch);
     if (e == hipSuccess) e = hipMalloc(&qbuf, kBdptQueueEntryBytes * (size_t)q.cap + 256);
+    if (e == hipSuccess) e = hipMalloc((void**)&q.counts, sizeof(uint32_t) * 2 * npix * batch);
     if (e == hipSuccess) {
         char* b = (char*)qbuf;
         q.rays = (uint32_t*)b;
@@ -266,6 +267,7 @@ extern "C" int wr_render_bdpt(wr_ctx* ctx, const wr_scene* s, const wr_camera* c
     (void)hipFree(verts);
     (void)hipFree(pix);
     (void)hipFree(qbuf);
+    (void)hipFree(q.counts);
     if (e != hipSuccess) return hip_fail(ctx, e, "wr_render_bdpt");
     film->passes += prm->spp;
     if (stats) {

@@ -116,6 +116,7 @@ struct BdptQueue {
     void* hits;
     uint32_t* count;
     uint32_t cap;
+    uint32_t* counts;  // 2 per pixel and pass: the eye and light sub-path record counts
 };
 constexpr size_t kBdptQueueEntryBytes = 32 + 16 + 4 + 16;
 size_t bdpt_vertex_bytes();
