@@ -344,7 +344,9 @@ __device__ __forceinline__ void atomic_add3(float* buf, uint32_t p, f3 v) {
 // The two sub-paths of one pixel and pass (kept apart from the connections:
 // this half traverses at a register count that lets several waves share a
 // SIMD); counts[2 * lane] = eye records, counts[2 * lane + 1] = light records.
-__global__ __launch_bounds__(256) void k_bdpt_subpaths(SceneDev S, CamDev cam, uint32_t max_depth, uint64_t seed,
+// 4 waves per SIMD (128 VGPRs, the rest spilled) for both kernels: 80 -> 43 ms on the
+// Cornell box against the unconstrained 264 / 190 registers.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bdpt_subpaths(SceneDev S, CamDev cam, uint32_t max_depth, uint64_t seed,
                                                        uint32_t pass0, uint32_t npass, uint32_t npix, uint32_t maxv,
                                                        BVert* verts, uint32_t* counts) {
     const uint64_t lane = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -418,7 +420,7 @@ __global__ __launch_bounds__(256) void k_bdpt_subpaths(SceneDev S, CamDev cam, u
     counts[2 * lane + 1] = nl;
 }
 
-__global__ __launch_bounds__(256) void k_bdpt(SceneDev S, CamDev cam, uint32_t max_depth, uint64_t seed,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bdpt(SceneDev S, CamDev cam, uint32_t max_depth, uint64_t seed,
                                               uint32_t pass0, uint32_t npass, uint32_t npix, uint32_t maxv,
                                               BVert* verts, const uint32_t* counts, float* pix_all, VisQueue Q) {
     (void)max_depth;
