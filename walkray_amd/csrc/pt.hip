@@ -585,6 +585,13 @@ void Engine::release() {
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     bufs.clear();
+    if (contrib) (void)hipFree(contrib);
+    if (pass_done) (void)hipFree(pass_done);
+    if (stage) (void)hipHostFree(stage);
+    contrib = nullptr;
+    pass_done = nullptr;
+    contrib_floats = pass_slots = 0;
+    stage = nullptr;
     cap = 0;
 }
 
@@ -737,44 +744,66 @@ hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_dep
     J.npass = npass;
     J.npix = npix;
     J.R = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(npass, E.cap / npix + 2), kMaxPassWindow);
-    float* contrib = nullptr;
-    uint32_t* pass_done = nullptr;
-    WR_CHECK(hipMalloc((void**)&contrib, sizeof(float) * 3 * (size_t)J.R * npix));
-    hipError_t e = hipMalloc((void**)&pass_done, sizeof(uint32_t) * J.R);
-    if (e == hipSuccess) e = hipMemsetAsync(pass_done, 0, sizeof(uint32_t) * J.R, st);
-    J.contrib = contrib;
-    J.pass_done = pass_done;
+    // the job's pass window, the engine's buffers grown to fit it
+    const size_t want_floats = 3 * (size_t)J.R * npix;
+    if (E.contrib_floats < want_floats) {
+        if (E.contrib) WR_CHECK(hipFree(E.contrib));
+        E.contrib = nullptr;
+        E.contrib_floats = 0;
+        WR_CHECK(hipMalloc((void**)&E.contrib, sizeof(float) * want_floats));
+        E.contrib_floats = want_floats;
+    }
+    if (E.pass_slots < J.R) {
+        if (E.pass_done) WR_CHECK(hipFree(E.pass_done));
+        E.pass_done = nullptr;
+        E.pass_slots = 0;
+        WR_CHECK(hipMalloc((void**)&E.pass_done, sizeof(uint32_t) * J.R));
+        E.pass_slots = J.R;
+    }
+    if (!E.stage) WR_CHECK(hipHostMalloc((void**)&E.stage, sizeof(Engine::Stage), hipHostMallocDefault));
+    J.contrib = E.contrib;
+    J.pass_done = E.pass_done;
     J.fsum = film.sum;
     J.fcomp = film.comp;
     J.fsumsq = film.sumsq;
-    if (e == hipSuccess) e = E.set_scene(0, S, st);
-    if (e == hipSuccess) e = E.admit(0, J, st);
-    std::vector<JobCtl> hc(kMaxJobs);
-    EngineCounters hcnt{};
-    if (e == hipSuccess) e = hipMemcpy(&hcnt, E.cnt, sizeof(EngineCounters), hipMemcpyDeviceToHost);
-    const uint32_t it0 = hcnt.iteration;
+    // scene slot 0, job slot 0 and the iteration counter before the first
+    // replay, all ordered on the stream ahead of it (the previous call ended
+    // with a synchronisation, so the staging is free)
+    Engine::Stage& sg = *E.stage;
+    sg.scene = S;
+    sg.job = J;
+    sg.ctl = JobCtl{};
+    sg.ctl.state = kJobRunning;
+    sg.ctl_free = JobCtl{};
+    WR_CHECK(hipMemsetAsync(E.pass_done, 0, sizeof(uint32_t) * J.R, st));
+    WR_CHECK(hipMemcpyAsync(E.scenes, &sg.scene, sizeof(SceneDev), hipMemcpyHostToDevice, st));
+    WR_CHECK(hipMemcpyAsync(E.jobs, &sg.job, sizeof(JobDesc), hipMemcpyHostToDevice, st));
+    WR_CHECK(hipMemcpyAsync(E.ctl, &sg.ctl, sizeof(JobCtl), hipMemcpyHostToDevice, st));
+    WR_CHECK(hipMemcpyAsync(&sg.cnt0, E.cnt, sizeof(EngineCounters), hipMemcpyDeviceToHost, st));
     // every pass is generated within npass window turns and drains in at most
     // max_depth + 2 iterations after its last path is issued
     const uint64_t it_bound = 4 * (uint64_t)(max_depth + 4) * (total / E.cap + npass + 2) + 64;
     uint64_t iters = 0;
     bool tail = false;
-    while (e == hipSuccess) {
-        e = E.step(st, hc.data(), &hcnt, tail);
+    hipError_t e = hipSuccess;
+    for (;;) {
+        e = hipGraphLaunch(tail ? E.tail_exec : E.exec, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(&sg.ctl_back, E.ctl, sizeof(JobCtl), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         iters += tail ? kTailIters : kGraphIters;
-        if (e != hipSuccess || hc[0].state == kJobDone) break;
-        tail = hc[0].next_work >= total;  // all camera paths issued: only draining is left
+        if (e != hipSuccess || sg.ctl_back.state == kJobDone) break;
+        tail = sg.ctl_back.next_work >= total;  // all camera paths issued: only draining is left
         if (iters > it_bound) {  // cannot happen
             e = hipErrorUnknown;
             break;
         }
     }
-    if (e == hipSuccess) e = E.release_slot(0, st);
+    // job slot 0 freed for the next call
+    if (e == hipSuccess) e = hipMemcpyAsync(E.ctl, &sg.ctl_free, sizeof(JobCtl), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(contrib);
-    (void)hipFree(pass_done);
     if (e != hipSuccess) return e;
     if (stats) {
-        stats->iterations = hc[0].done_iter + 1 - it0;
+        stats->iterations = sg.ctl_back.done_iter + 1 - sg.cnt0.iteration;
         stats->paths = total;
     }
     return hipSuccess;

@@ -89,6 +89,20 @@ struct Engine {
     hipGraph_t graph = nullptr, tail_graph = nullptr;
     hipGraphExec_t exec = nullptr, tail_exec = nullptr;
     std::vector<void*> bufs;
+    // run_wavefront's per-render buffers, kept between renders (grow-only):
+    // the per-pass contributions and completion counts of its one job
+    float* contrib = nullptr;
+    uint32_t* pass_done = nullptr;
+    size_t contrib_floats = 0, pass_slots = 0;
+    // pinned host staging of run_wavefront's copies, so that they need no
+    // synchronisation of their own
+    struct Stage {
+        SceneDev scene;
+        JobDesc job;
+        JobCtl ctl, ctl_free, ctl_back;
+        EngineCounters cnt0, cnt_back;
+    };
+    Stage* stage = nullptr;
     ~Engine();
     void release();
     hipError_t init(uint32_t cap, uint32_t grid, hipStream_t st, uint32_t lds_levels = 8);
