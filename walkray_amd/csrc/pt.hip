@@ -538,6 +538,23 @@ __global__ __launch_bounds__(64) void k_wf_advance(EngineDev E, uint32_t par) {
     }
 }
 
+// run_wavefront's admission in one launch: scene slot 0, job slot 0 (running,
+// with the engine iteration it starts at) and the job's pass counters zeroed.
+// Nothing else runs on the engine meanwhile (the previous call ended with a
+// synchronisation).
+__global__ __launch_bounds__(256) void k_wf_admit(SceneDev* scenes, wr::JobDesc* jobs, wr::JobCtl* ctl,
+                                                  const wr::EngineCounters* cnt, SceneDev S, wr::JobDesc J) {
+    for (uint32_t k = threadIdx.x; k < J.R; k += 256) J.pass_done[k] = 0;
+    if (threadIdx.x == 0) {
+        scenes[0] = S;
+        jobs[0] = J;
+        wr::JobCtl c{};
+        c.state = wr::kJobRunning;
+        c.start_iter = cnt->iteration;
+        ctl[0] = c;
+    }
+}
+
 }  // namespace wrd
 
 namespace wr {
@@ -587,11 +604,11 @@ void Engine::release() {
     bufs.clear();
     if (contrib) (void)hipFree(contrib);
     if (pass_done) (void)hipFree(pass_done);
-    if (stage) (void)hipHostFree(stage);
+    if (ctl_back) (void)hipHostFree(ctl_back);
     contrib = nullptr;
     pass_done = nullptr;
     contrib_floats = pass_slots = 0;
-    stage = nullptr;
+    ctl_back = nullptr;
     cap = 0;
 }
 
@@ -760,50 +777,45 @@ hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_dep
         WR_CHECK(hipMalloc((void**)&E.pass_done, sizeof(uint32_t) * J.R));
         E.pass_slots = J.R;
     }
-    if (!E.stage) WR_CHECK(hipHostMalloc((void**)&E.stage, sizeof(Engine::Stage), hipHostMallocDefault));
+    if (!E.ctl_back) WR_CHECK(hipHostMalloc((void**)&E.ctl_back, sizeof(JobCtl), hipHostMallocDefault));
     J.contrib = E.contrib;
     J.pass_done = E.pass_done;
     J.fsum = film.sum;
     J.fcomp = film.comp;
     J.fsumsq = film.sumsq;
-    // scene slot 0, job slot 0 and the iteration counter before the first
-    // replay, all ordered on the stream ahead of it (the previous call ended
-    // with a synchronisation, so the staging is free)
-    Engine::Stage& sg = *E.stage;
-    sg.scene = S;
-    sg.job = J;
-    sg.ctl = JobCtl{};
-    sg.ctl.state = kJobRunning;
-    sg.ctl_free = JobCtl{};
-    WR_CHECK(hipMemsetAsync(E.pass_done, 0, sizeof(uint32_t) * J.R, st));
-    WR_CHECK(hipMemcpyAsync(E.scenes, &sg.scene, sizeof(SceneDev), hipMemcpyHostToDevice, st));
-    WR_CHECK(hipMemcpyAsync(E.jobs, &sg.job, sizeof(JobDesc), hipMemcpyHostToDevice, st));
-    WR_CHECK(hipMemcpyAsync(E.ctl, &sg.ctl, sizeof(JobCtl), hipMemcpyHostToDevice, st));
-    WR_CHECK(hipMemcpyAsync(&sg.cnt0, E.cnt, sizeof(EngineCounters), hipMemcpyDeviceToHost, st));
+    k_wf_admit<<<1, 256, 0, st>>>(E.scenes, E.jobs, E.ctl, E.cnt, S, J);
+    WR_CHECK(hipGetLastError());
     // every pass is generated within npass window turns and drains in at most
     // max_depth + 2 iterations after its last path is issued
     const uint64_t it_bound = 4 * (uint64_t)(max_depth + 4) * (total / E.cap + npass + 2) + 64;
+    // every camera path fits the pool and the window: all are issued in the
+    // first iteration, so the first replay is followed by a tail replay at once
+    bool chain_tail = total <= E.cap && npass <= J.R;
     uint64_t iters = 0;
     bool tail = false;
+    JobCtl& back = *E.ctl_back;
     hipError_t e = hipSuccess;
     for (;;) {
         e = hipGraphLaunch(tail ? E.tail_exec : E.exec, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(&sg.ctl_back, E.ctl, sizeof(JobCtl), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
         iters += tail ? kTailIters : kGraphIters;
-        if (e != hipSuccess || sg.ctl_back.state == kJobDone) break;
-        tail = sg.ctl_back.next_work >= total;  // all camera paths issued: only draining is left
+        if (e == hipSuccess && chain_tail) {
+            e = hipGraphLaunch(E.tail_exec, st);
+            iters += kTailIters;
+            chain_tail = false;
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(&back, E.ctl, sizeof(JobCtl), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess || back.state == kJobDone) break;
+        tail = back.next_work >= total;  // all camera paths issued: only draining is left
         if (iters > it_bound) {  // cannot happen
             e = hipErrorUnknown;
             break;
         }
     }
-    // job slot 0 freed for the next call
-    if (e == hipSuccess) e = hipMemcpyAsync(E.ctl, &sg.ctl_free, sizeof(JobCtl), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    // job slot 0 stays done until the next call's admission overwrites it
     if (e != hipSuccess) return e;
     if (stats) {
-        stats->iterations = sg.ctl_back.done_iter + 1 - sg.cnt0.iteration;
+        stats->iterations = back.done_iter + 1 - back.start_iter;
         stats->paths = total;
     }
     return hipSuccess;

@@ -55,7 +55,7 @@ struct JobCtl {
     uint32_t cursor;     // passes folded into the film
     uint32_t state;      // kJobFree / kJobRunning / kJobDone
     uint32_t done_iter;  // engine iteration that folded the last pass
-    uint32_t pad;
+    uint32_t start_iter; // engine iteration at admission (set by run_wavefront's admission kernel)
 };
 
 struct EngineCounters {
@@ -94,15 +94,8 @@ struct Engine {
     float* contrib = nullptr;
     uint32_t* pass_done = nullptr;
     size_t contrib_floats = 0, pass_slots = 0;
-    // pinned host staging of run_wavefront's copies, so that they need no
-    // synchronisation of their own
-    struct Stage {
-        SceneDev scene;
-        JobDesc job;
-        JobCtl ctl, ctl_free, ctl_back;
-        EngineCounters cnt0, cnt_back;
-    };
-    Stage* stage = nullptr;
+    // pinned host buffer of run_wavefront's job-state read-back
+    JobCtl* ctl_back = nullptr;
     ~Engine();
     void release();
     hipError_t init(uint32_t cap, uint32_t grid, hipStream_t st, uint32_t lds_levels = 8);
