@@ -32,6 +32,22 @@ def test_megakernel_equals_wavefront(ctx):
     assert a["mean"].mean() > 0.05
 
 
+def test_wavefront_renders_in_sequence(ctx):
+    # one context, one path pool, renders whose pass windows and images differ:
+    # the per-render buffers kept between calls grow (contributions, then pass
+    # counters, then contributions again) and each film still equals the megakernel's
+    cases = [(40, 32, 7), (16, 16, 24), (40, 32, 3), (64, 48, 5), (16, 16, 2)]
+    for w, h, spp in cases:
+        objs, cam = S.cornell(k=3, width=w, height=h)
+        sc = W.Scene(ctx, objs)
+        a = R.render_scene(ctx, objs, cam, spp, 6, 7 + w, W.WR_MEGAKERNEL, scene=sc)
+        b = R.render_scene(ctx, objs, cam, spp, 6, 7 + w, W.WR_WAVEFRONT, scene=sc, wave_slots=4096)
+        for k in ("sum", "comp", "sumsq"):
+            assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), f"{w}x{h} {spp} spp: film {k} differs"
+        assert b["passes"] == spp and a["mean"].mean() > 0.02
+        assert 2 <= b["stats"]["iterations"] < 500
+
+
 def _zcheck(gpu, orc_sum, orc_sq, n):
     mg, seg = R.film_stats(gpu)
     mo = orc_sum / n
