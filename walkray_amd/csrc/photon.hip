@@ -339,20 +339,21 @@ __global__ __launch_bounds__(256) void k_grid_query(GridDev g, const float* pos,
             const int la = __shfl(a, L), lb = __shfl(b, L);
             const uint32_t lk = __shfl(k, L);
             uint32_t lm = __shfl(m, L);
-            // four 64-point steps per iteration: their loads are in flight together
-            for (int base = la; base < lb; base += 4 * 64) {
-                uint32_t pi[4];
-                bool hit[4];
+            // sixteen 64-point steps per iteration: their loads are in flight together
+            // (a single wave walks the whole range, so its loads in flight set the pace)
+            for (int base = la; base < lb; base += 16 * 64) {
+                uint32_t pi[16];
+                bool hit[16];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < 16; u++) {
                     const int ai = base + 64 * u + (int)lane;
                     pi[u] = ai < lb ? idx[ai] : 0u;
                 }
 #pragma unroll
-                for (int u = 0; u < 4; u++)
+                for (int u = 0; u < 16; u++)
                     hit[u] = base + 64 * u + (int)lane < lb && grid_near(g, pos, lq, pi[u]);
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < 16; u++) {
                     const uint64_t hm = __ballot(hit[u]);
                     if (hit[u]) {
                         const uint32_t slot = lm + (uint32_t)__popcll(hm & below);
