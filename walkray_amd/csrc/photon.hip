@@ -283,41 +283,65 @@ __device__ __forceinline__ bool grid_near(const GridDev& g, const float* pos, f3
     return dist * dist <= g.r2;
 }
 
+// Ranges longer than this hand their whole query to k_grid_query_wide (one
+// workgroup per query), so that no wave walks such a range alone.
+constexpr int kHugeRange = 2048;
+
+// The query's cell corner and its nearer neighbours (HashGrid::Process); false
+// outside the grid's box.
+__device__ __forceinline__ bool query_corner(const GridDev& g, f3 qp, int p[6]) {
+    const f3 dmin = mk(qp.x - g.bmin[0], qp.y - g.bmin[1], qp.z - g.bmin[2]);
+    const f3 dmax = mk(g.bmax[0] - qp.x, g.bmax[1] - qp.y, g.bmax[2] - qp.z);
+    if (dmin.x < 0.f || dmin.y < 0.f || dmin.z < 0.f || dmax.x < 0.f || dmax.y < 0.f || dmax.z < 0.f) return false;
+    const f3 cp = mk(g.inv * dmin.x, g.inv * dmin.y, g.inv * dmin.z);
+    const f3 cf = mk(floorf(cp.x), floorf(cp.y), floorf(cp.z));
+    p[0] = (int)cf.x;
+    p[1] = (int)cf.y;
+    p[2] = (int)cf.z;
+    p[3] = p[0] + (cp.x - cf.x < 0.5f ? -1 : +1);
+    p[4] = p[1] + (cp.y - cf.y < 0.5f ? -1 : +1);
+    p[5] = p[2] + (cp.z - cf.z < 0.5f ? -1 : +1);
+    return true;
+}
+
+// Cell j of the query's 8, in the reference's order.
+__device__ __forceinline__ uint32_t query_cell(const GridDev& g, const int p[6], int j) {
+    return grid_hash(g, (j & 4) ? p[3] : p[0], (j & 2) ? p[4] : p[1], (j & 1) ? p[5] : p[2]);
+}
+
+// defer: [0] the number of deferred queries, then their indices.
 __global__ __launch_bounds__(256) void k_grid_query(GridDev g, const float* pos, const int* ends, const uint32_t* idx,
                                                     const float* q, uint32_t nq, uint32_t cap, uint32_t* counts,
-                                                    uint32_t* out) {
+                                                    uint32_t* out, uint32_t* defer) {
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (1ull << lane) - 1ull;
     // every lane stays to the end: the long ranges are scanned by the whole wave
     bool valid = k < nq;
     f3 qp = mk(0.f, 0.f, 0.f);
-    int px = 0, py = 0, pz = 0, pxo = 0, pyo = 0, pzo = 0;
+    int p[6] = {0, 0, 0, 0, 0, 0};
     if (valid) {
         qp = mk(q[3 * k], q[3 * k + 1], q[3 * k + 2]);
-        const f3 dmin = mk(qp.x - g.bmin[0], qp.y - g.bmin[1], qp.z - g.bmin[2]);
-        const f3 dmax = mk(g.bmax[0] - qp.x, g.bmax[1] - qp.y, g.bmax[2] - qp.z);
-        const bool inside =
-            !(dmin.x < 0.f || dmin.y < 0.f || dmin.z < 0.f || dmax.x < 0.f || dmax.y < 0.f || dmax.z < 0.f);
-        if (!inside) {
+        if (!query_corner(g, qp, p)) {
             counts[k] = 0;
             valid = false;
         } else {
-            const f3 cp = mk(g.inv * dmin.x, g.inv * dmin.y, g.inv * dmin.z);
-            const f3 cf = mk(floorf(cp.x), floorf(cp.y), floorf(cp.z));
-            px = (int)cf.x;
-            py = (int)cf.y;
-            pz = (int)cf.z;
-            pxo = px + (cp.x - cf.x < 0.5f ? -1 : +1);
-            pyo = py + (cp.y - cf.y < 0.5f ? -1 : +1);
-            pzo = pz + (cp.z - cf.z < 0.5f ? -1 : +1);
+            bool huge = false;
+            for (int j = 0; j < 8; j++) {
+                const uint32_t c = query_cell(g, p, j);
+                huge = huge || ends[c] - (c == 0 ? 0 : ends[c - 1]) > kHugeRange;
+            }
+            if (huge) {  // k_grid_query_wide writes this query's list and count
+                defer[1 + atomicAdd(defer, 1u)] = k;
+                valid = false;
+            }
         }
     }
     uint32_t m = 0;
     for (int j = 0; j < 8; j++) {  // the reference's cell order
         int a = 0, b = 0;
         if (valid) {
-            const uint32_t c = grid_hash(g, (j & 4) ? pxo : px, (j & 2) ? pyo : py, (j & 1) ? pzo : pz);
+            const uint32_t c = query_cell(g, p, j);
             a = c == 0 ? 0 : ends[c - 1];
             b = ends[c];
         }
@@ -368,6 +392,71 @@ __global__ __launch_bounds__(256) void k_grid_query(GridDev g, const float* pos,
     if (valid) counts[k] = m;
 }
 
+// The deferred queries, one workgroup per query: the workgroup walks the
+// query's 8 cells in the reference's order, 256 points a step and kWideSteps
+// steps per iteration.  Step u, wave w, lane l holds point base + 256u + 64w + l,
+// so a hit's slot is the hits before it in that order: the query's count so
+// far, the wave counts of the earlier steps and waves (LDS), and its rank in
+// its wave's ballot.  Every loop bound is the same for the whole workgroup.
+constexpr int kWideSteps = 4;
+
+__global__ __launch_bounds__(256) void k_grid_query_wide(GridDev g, const float* pos, const int* ends,
+                                                         const uint32_t* idx, const float* q, uint32_t cap,
+                                                         uint32_t* counts, uint32_t* out, const uint32_t* defer) {
+    __shared__ uint32_t wcnt[kWideSteps][4];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t nd = defer[0];
+    for (uint32_t d = blockIdx.x; d < nd; d += gridDim.x) {
+        const uint32_t k = defer[1 + d];
+        const f3 qp = mk(q[3 * k], q[3 * k + 1], q[3 * k + 2]);
+        int p[6];
+        (void)query_corner(g, qp, p);  // inside the box: only such queries are deferred
+        uint32_t m = 0;
+        for (int j = 0; j < 8; j++) {
+            const uint32_t c = query_cell(g, p, j);
+            const int a = c == 0 ? 0 : ends[c - 1];
+            const int b = ends[c];
+            for (int base = a; base < b; base += kWideSteps * 256) {
+                uint32_t pi[kWideSteps];
+                bool hit[kWideSteps];
+#pragma unroll
+                for (int u = 0; u < kWideSteps; u++) {
+                    const int ai = base + 256 * u + (int)threadIdx.x;
+                    pi[u] = ai < b ? idx[ai] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < kWideSteps; u++)
+                    hit[u] = base + 256 * u + (int)threadIdx.x < b && grid_near(g, pos, qp, pi[u]);
+                uint64_t hm[kWideSteps];
+#pragma unroll
+                for (int u = 0; u < kWideSteps; u++) {
+                    hm[u] = __ballot(hit[u]);
+                    if (lane == 0) wcnt[u][wave] = (uint32_t)__popcll(hm[u]);
+                }
+                __syncthreads();
+                uint32_t off = m;
+#pragma unroll
+                for (int u = 0; u < kWideSteps; u++) {
+                    uint32_t before = 0, all = 0;
+                    for (uint32_t w = 0; w < 4; w++) {
+                        before += w < wave ? wcnt[u][w] : 0u;
+                        all += wcnt[u][w];
+                    }
+                    if (hit[u]) {
+                        const uint32_t slot = off + before + (uint32_t)__popcll(hm[u] & below);
+                        if (slot < cap) out[(size_t)k * cap + slot] = pi[u];
+                    }
+                    off += all;
+                }
+                m = off;
+                __syncthreads();  // wcnt is rewritten by the next iteration
+            }
+        }
+        if (threadIdx.x == 0) counts[k] = m;
+    }
+}
+
 }  // namespace wrd
 
 struct wr_hashgrid {
@@ -377,6 +466,8 @@ struct wr_hashgrid {
     uint64_t n = 0;
     int* ends = nullptr;         // n_cells
     uint32_t* idx = nullptr;     // n
+    uint32_t* defer = nullptr;   // query calls: deferred-query counter and list (grown on demand)
+    uint64_t defer_cap = 0;      // entries of defer
 };
 
 using namespace wrd;
@@ -575,8 +666,21 @@ int wr_hashgrid_query(wr_hashgrid* g, const float* d_queries, uint64_t nq, uint3
     if (nq >= (1ull << 32)) return fail(ctx, WR_EINVAL, "wr_hashgrid_query: too many queries");
     if (nq == 0) return WR_OK;
     (void)hipSetDevice(ctx->device);
+    if (g->defer_cap < nq + 1) {
+        if (g->defer) PH_CHECK(hipFree(g->defer), "wr_hashgrid_query");
+        g->defer = nullptr;
+        g->defer_cap = 0;
+        PH_CHECK(hipMalloc((void**)&g->defer, sizeof(uint32_t) * (nq + 1)), "hipMalloc(hashgrid query)");
+        g->defer_cap = nq + 1;
+    }
+    PH_CHECK(hipMemsetAsync(g->defer, 0, sizeof(uint32_t), ctx->stream), "wr_hashgrid_query");
     k_grid_query<<<(unsigned)((nq + 255) / 256), 256, 0, ctx->stream>>>(g->g, g->pos, g->ends, g->idx, d_queries,
-                                                                         (uint32_t)nq, cap, d_counts, d_idx);
+                                                                         (uint32_t)nq, cap, d_counts, d_idx, g->defer);
+    PH_CHECK(hipGetLastError(), "wr_hashgrid_query");
+    // the deferred queries: a workgroup each, as many workgroups as fill the device twice
+    const unsigned wide_grid = (unsigned)std::min<uint64_t>(nq, 2048);
+    k_grid_query_wide<<<wide_grid, 256, 0, ctx->stream>>>(g->g, g->pos, g->ends, g->idx, d_queries, cap, d_counts,
+                                                          d_idx, g->defer);
     PH_CHECK(hipGetLastError(), "wr_hashgrid_query");
     PH_CHECK(hipStreamSynchronize(ctx->stream), "wr_hashgrid_query");
     return WR_OK;
@@ -587,6 +691,7 @@ void wr_hashgrid_destroy(wr_hashgrid* g) {
     (void)hipSetDevice(g->ctx->device);
     if (g->ends) (void)hipFree(g->ends);
     if (g->idx) (void)hipFree(g->idx);
+    if (g->defer) (void)hipFree(g->defer);
     delete g;
 }
 
