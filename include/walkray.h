@@ -309,7 +309,10 @@ int wr_photons_collect(wr_ctx* ctx, const wr_scene* scene, const wr_path_vertex*
  * Process as a range query: for each query point (device, 3 floats), the
  * points handed to the query in the reference's order (8 cells, a cell
  * reached twice through a hash collision visited twice), the first cap of
- * them in d_idx (cap per query), their number in d_counts. */
+ * them in d_idx (cap per query), their number in d_counts.  A query call
+ * keeps a device work list of n_queries + 1 words in the grid (allocated
+ * when a call needs more than the last), so calls on one grid must not
+ * run concurrently. */
 typedef struct wr_hashgrid wr_hashgrid;
 int wr_hashgrid_build(wr_ctx* ctx, const float* d_pos, uint64_t n, float radius, uint32_t n_cells,
                       wr_hashgrid** out);
