@@ -398,7 +398,7 @@ __global__ __launch_bounds__(256) void k_grid_query(GridDev g, const float* pos,
 // so a hit's slot is the hits before it in that order: the query's count so
 // far, the wave counts of the earlier steps and waves (LDS), and its rank in
 // its wave's ballot.  Every loop bound is the same for the whole workgroup.
-constexpr int kWideSteps = 4;
+constexpr int kWideSteps = 16;
 
 __global__ __launch_bounds__(256) void k_grid_query_wide(GridDev g, const float* pos, const int* ends,
                                                          const uint32_t* idx, const float* q, uint32_t cap,
