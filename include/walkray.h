@@ -320,6 +320,20 @@ int wr_hashgrid_build(wr_ctx* ctx, const float* d_pos, uint64_t n, float radius,
 int wr_hashgrid_export(const wr_hashgrid* grid, float* bbox, int32_t* cell_ends, int32_t* indices);
 int wr_hashgrid_query(wr_hashgrid* grid, const float* d_queries, uint64_t n_queries, uint32_t cap, uint32_t* d_counts,
                       uint32_t* d_idx);
+/* wr_hashgrid_query without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same results as wr_hashgrid_query.  The deferred-query work list is a fixed
+ * array of the library instead of the grid's: WR_HASHGRID_NOALLOC_MAX_QUERIES
+ * + 1 words (16 MiB) on every GPU the library runs on, held until the process
+ * ends; more queries fail with WR_EINVAL.  Asynchronous on the context stream
+ * (no synchronisation and no memset: the list's counter is cleared by a
+ * kernel).  Every call on one GPU shares that array, so calls that overlap on
+ * one GPU (from two contexts, or while another context's call is in flight)
+ * race; calls on one context are ordered by its stream. */
+#define WR_HASHGRID_NOALLOC_MAX_QUERIES (1u << 22)
+int wr_hashgrid_query_noalloc(wr_hashgrid* grid, const float* d_queries, uint64_t n_queries, uint32_t cap,
+                              uint32_t* d_counts, uint32_t* d_idx);
 void wr_hashgrid_destroy(wr_hashgrid* grid);
 
 /* ---- scene files ---------------------------------------------------------- */

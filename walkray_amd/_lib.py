@@ -19,6 +19,7 @@ OVERFLOW = 0xFFFFFFFE
 
 WR_DIFFUSE, WR_GLOSSY, WR_REFRACTIVE, WR_EMISSIVE = 0, 1, 2, 3
 WR_BEST = 1
+WR_HASHGRID_NOALLOC_MAX_QUERIES = 1 << 22  # walkray.h
 
 _lib = None
 
@@ -188,6 +189,7 @@ def lib():
         "wr_hashgrid_build": (i32, [vp, vp, u64, C.c_float, u32, P(vp)]),
         "wr_hashgrid_export": (i32, [vp, vp, vp, vp]),
         "wr_hashgrid_query": (i32, [vp, vp, u64, u32, vp, vp]),
+        "wr_hashgrid_query_noalloc": (i32, [vp, vp, u64, u32, vp, vp]),
         "wr_hashgrid_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
@@ -536,6 +538,11 @@ class HashGrid:
     def query_device(self, d_q, nq, cap, d_counts, d_idx):
         _check(lib().wr_hashgrid_query(self.h, d_q.ptr, nq, cap, d_counts.ptr, d_idx.ptr if d_idx else None),
                self.ctx.h, "wr_hashgrid_query")
+
+    def query_device_noalloc(self, d_q, nq, cap, d_counts, d_idx):
+        """wr_hashgrid_query_noalloc: asynchronous, no device allocation."""
+        _check(lib().wr_hashgrid_query_noalloc(self.h, d_q.ptr, nq, cap, d_counts.ptr, d_idx.ptr if d_idx else None),
+               self.ctx.h, "wr_hashgrid_query_noalloc")
 
     def query(self, q, cap):
         q = np.ascontiguousarray(q, dtype=np.float32).reshape(-1, 3)

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "pt_dev.h"
@@ -457,7 +458,45 @@ __global__ __launch_bounds__(256) void k_grid_query_wide(GridDev g, const float*
     }
 }
 
+// Clears one word with a kernel rather than a memset, so a caller may record
+// the query sequence into a graph.
+__global__ void k_clear_word(uint32_t* p) { *p = 0u; }
+
 }  // namespace wrd
+
+// Deferred-query work list of wr_hashgrid_query_noalloc: the counter, then up
+// to WR_HASHGRID_NOALLOC_MAX_QUERIES query indices (16 MiB per GPU, held until
+// the process ends).  Host code never passes this name to a kernel: it uses
+// the per-device address from hipGetSymbolAddress (noalloc_ptr below).
+static __device__ __align__(256) uint32_t scratch[WR_HASHGRID_NOALLOC_MAX_QUERIES + 1];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxScratchDevices] = {};  // device address of scratch, per device
+}  // namespace
+
+namespace wr {
+// Looks up scratch's address on `device`, which must be current.  Called once
+// per device, from wr_ctx_create.
+hipError_t noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = (uint32_t*)p;
+    return hipSuccess;
+}
+
+static uint32_t* noalloc_ptr(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+}  // namespace wr
 
 struct wr_hashgrid {
     wr_ctx* ctx = nullptr;
@@ -683,6 +722,31 @@ int wr_hashgrid_query(wr_hashgrid* g, const float* d_queries, uint64_t nq, uint3
                                                           d_idx, g->defer);
     PH_CHECK(hipGetLastError(), "wr_hashgrid_query");
     PH_CHECK(hipStreamSynchronize(ctx->stream), "wr_hashgrid_query");
+    return WR_OK;
+}
+
+int wr_hashgrid_query_noalloc(wr_hashgrid* g, const float* d_queries, uint64_t nq, uint32_t cap, uint32_t* d_counts,
+                              uint32_t* d_idx) {
+    if (!g || (nq && (!d_queries || !d_counts)) || (cap && nq && !d_idx))
+        return fail(g ? g->ctx : nullptr, WR_EINVAL, "wr_hashgrid_query_noalloc: bad argument");
+    wr_ctx* ctx = g->ctx;
+    if (nq > WR_HASHGRID_NOALLOC_MAX_QUERIES)
+        return fail(ctx, WR_EINVAL, "wr_hashgrid_query_noalloc: more than WR_HASHGRID_NOALLOC_MAX_QUERIES (4,194,304) "
+                                    "queries; use wr_hashgrid_query or split the call");
+    if (nq == 0) return WR_OK;
+    PH_CHECK(hipSetDevice(ctx->device), "wr_hashgrid_query_noalloc");
+    uint32_t* defer = wr::noalloc_ptr(ctx->device);
+    if (!defer) return fail(ctx, WR_ESTATE, "wr_hashgrid_query_noalloc: work list not set up on this device");
+    // the list keeps the previous call's contents: only its counter must start at zero
+    k_clear_word<<<1, 1, 0, ctx->stream>>>(defer);
+    PH_CHECK(hipGetLastError(), "wr_hashgrid_query_noalloc");
+    k_grid_query<<<(unsigned)((nq + 255) / 256), 256, 0, ctx->stream>>>(g->g, g->pos, g->ends, g->idx, d_queries,
+                                                                         (uint32_t)nq, cap, d_counts, d_idx, defer);
+    PH_CHECK(hipGetLastError(), "wr_hashgrid_query_noalloc");
+    const unsigned wide_grid = (unsigned)std::min<uint64_t>(nq, 2048);
+    k_grid_query_wide<<<wide_grid, 256, 0, ctx->stream>>>(g->g, g->pos, g->ends, g->idx, d_queries, cap, d_counts,
+                                                          d_idx, defer);
+    PH_CHECK(hipGetLastError(), "wr_hashgrid_query_noalloc");
     return WR_OK;
 }
 

@@ -43,6 +43,9 @@ void place_object(HostObject& o, const io::Mat4& M);
 // updateTransform, abstractObj.cpp:70-76), from the counter RNG at vertex
 // 0xFFFFFFFE with the object id in the pixel slot.
 float motion_time(uint64_t seed, uint32_t obj, uint32_t pass);
+// Looks up, on `device` (current), the address of the fixed work list of
+// wr_hashgrid_query_noalloc; once per device, from wr_ctx_create.
+hipError_t noalloc_scratch_init(int device);
 }  // namespace wr
 
 struct wr_scene {
