@@ -1708,7 +1708,7 @@ void orc_render(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t seed
     if (nthreads <= 0) nthreads = omp_get_max_threads();
     for (uint32_t pass = pass0; pass < pass0 + npass; pass++) {
         s->updateForMotion(seed, pass);
-#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads) if(np > 64)
         for (int64_t p = 0; p < np; p++) {
             V v = tr.sample((unsigned)p, pass, nullptr);
             for (int k = 0; k < 3; k++) {
@@ -1718,6 +1718,54 @@ void orc_render(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t seed
         }
     }
     restoreMotion(s);
+}
+
+/* Camera::generateRay's direction (camera.cpp:13-23) for pixel p and jitter
+ * (u, v), and the sample weight the camera gives it: the camera vertex's cosine
+ * term |front . d| (ray.cpp:52-58 with Camera::getWorldNormal, camera.cpp:182-186)
+ * divided by fixVignetting's |front . d_centre| (camera.cpp:107-122). */
+void orc_camera_ray(const orc_camera* c, uint32_t p, float u, float v, float* dir, float* weight) {
+    Cam cam(c);
+    V d = cam.dir(p, u, v);
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+    float c0 = std::fabs(dot(normalize(sub(cam.focus, cam.pos)), d));
+    c0 = c0 > EPS_D ? c0 : (float)EPS_D;
+    *weight = cam.fixVignetting(V(c0, c0, c0), p).x;
+}
+
+/* AbstractObject::scatter of object `self` (dispatched as mcRenderer.cpp:30-31)
+ * for the ray (o, d) whose closest hit is local triangle `tri` of object `obj`
+ * at distance dist, with insideObj `inside`, from the uniforms rn[0..2].
+ * out: origin (3), direction (3), radiance (3), originProb, directionProb,
+ * contact normal (3); ids: insideObj, contactObj.  Returns isDeltaDirection. */
+int orc_scatter(void* sp, uint32_t self, const float* ray6, uint32_t obj, uint32_t tri, float dist, uint32_t inside,
+                const float* rn, float* out, uint32_t* ids) {
+    Scene* s = (Scene*)sp;
+    orc_camera c0{};
+    c0.focus[2] = 1.f;
+    c0.up[1] = 1.f;
+    c0.width = c0.height = 1;
+    c0.sight_dist = 1.f;
+    Tracer tr{s, Cam(&c0), 0, 0};
+    Ray r;
+    r.o = V(ray6[0], ray6[1], ray6[2]);
+    r.d = V(ray6[3], ray6[4], ray6[5]);
+    Tracer::Hit h;
+    h.valid = true;
+    h.dist = dist;
+    h.obj = obj;
+    h.tri = tri;
+    float u[4] = {rn[0], rn[1], rn[2], 0.f};
+    Out o = tr.scatter(self, r, h, inside, u);
+    const V* vs[3] = {&o.o, &o.d, &o.rad};
+    for (int k = 0; k < 3; k++)
+        for (int a = 0; a < 3; a++) out[3 * k + a] = (*vs[k])[a];
+    out[9] = o.op;
+    out[10] = o.dp;
+    out[11] = o.n.x; out[12] = o.n.y; out[13] = o.n.z;
+    ids[0] = o.inside;
+    ids[1] = o.contact;
+    return o.delta ? 1 : 0;
 }
 
 void orc_render_sample(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t seed,
@@ -1737,7 +1785,7 @@ void orc_render2(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t see
     if (nthreads <= 0) nthreads = omp_get_max_threads();
     for (uint32_t pass = pass0; pass < pass0 + npass; pass++) {
         s->updateForMotion(seed, pass);
-#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads) if(np > 64)
         for (int64_t p = 0; p < np; p++) {
             V v = pv ? tr.samplePathVector((unsigned)p, pass, nee, nullptr) : tr.sample((unsigned)p, pass, nullptr);
             for (int k = 0; k < 3; k++) {

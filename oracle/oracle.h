@@ -79,6 +79,19 @@ void     orc_render(void* s, const orc_camera* cam, uint32_t max_depth, uint64_t
 void     orc_render_sample(void* s, const orc_camera* cam, uint32_t max_depth, uint64_t seed,
                            uint32_t pixel, uint32_t pass, float* rgb, uint32_t* n_vertices);
 
+/* Camera::generateRay's direction (camera.cpp:13-23) for pixel p and jitter
+ * (u, v), and the weight the camera gives the sample: |front . d| (ray.cpp:52-58,
+ * camera.cpp:182-186) / |front . d_centre| (fixVignetting, camera.cpp:107-122). */
+void     orc_camera_ray(const orc_camera* cam, uint32_t pixel, float u, float v, float* dir, float* weight);
+/* AbstractObject::scatter of object self (mcRenderer.cpp:30-31 dispatch) for the
+ * ray (o, d) that hits local triangle tri of object obj at distance dist, with
+ * insideObj inside, from three uniforms: out = origin, direction, radiance (3
+ * each), originProb, directionProb, contact normal (3); ids = insideObj,
+ * contactObj.  Returns isDeltaDirection.  diffuseObj.h:34-57, glossyObj.h:39-62,
+ * refractiveObj.h:27-88, emissiveObj.h:15-28. */
+int      orc_scatter(void* s, uint32_t self, const float* ray6, uint32_t obj, uint32_t tri, float dist,
+                     uint32_t inside, const float* rn, float* out, uint32_t* ids);
+
 /* flags of orc_render2 / orc_render_sample2 */
 enum {
     ORC_NEE = 1,          /* PathTracer's useNextEventEstimation branch (pathTracer.cpp:39-66) */

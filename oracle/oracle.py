@@ -53,6 +53,9 @@ def lib():
         L.orc_render_sample.argtypes = [vp, vp, u32, u64, u32, u32, vp, vp]
         L.orc_render2.argtypes = [vp, vp, u32, u64, u32, u32, u32, vp, vp, C.c_int]
         L.orc_render_sample2.argtypes = [vp, vp, u32, u64, u32, u32, u32, vp, vp]
+        L.orc_camera_ray.argtypes = [vp, u32, f32, f32, vp, P(f32)]
+        L.orc_scatter.restype = C.c_int
+        L.orc_scatter.argtypes = [vp, u32, vp, u32, u32, f32, u32, vp, vp, vp]
         L.orc_light_sample.restype = C.c_int
         L.orc_light_sample.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_light_paths.restype = u32
@@ -89,6 +92,16 @@ class Camera(C.Structure):
         c.height = cam["height"]
         c.sight_dist = cam["sight_dist"]
         return c
+
+
+def camera_ray(cam, pixel, u, v):
+    """Camera::generateRay's direction for pixel and jitter (u, v), and the sample weight
+    |front . d| / |front . d_centre| the camera gives it."""
+    c = Camera.from_dict(cam)
+    d = np.zeros(3, np.float32)
+    w = C.c_float()
+    lib().orc_camera_ray(C.byref(c), pixel, u, v, _p(d), C.byref(w))
+    return d, w.value
 
 
 class Tree:
@@ -201,6 +214,18 @@ class Scene:
         else:
             lib().orc_render(self.h, C.byref(c), max_depth, seed, pass0, spp, _p(sums[0]), _p(sums[1]), nthreads)
         return sums
+
+    def scatter(self, self_obj, ray, obj, tri, dist, inside, rn):
+        """AbstractObject::scatter of object self_obj for the ray (o, d) hitting local triangle
+        tri of object obj at distance dist (insideObj = inside), from three uniforms."""
+        r = np.ascontiguousarray(ray, dtype=np.float32).reshape(6)
+        u = np.ascontiguousarray(rn, dtype=np.float32).reshape(3)
+        out = np.zeros(14, np.float32)
+        ids = np.zeros(2, np.uint32)
+        delta = lib().orc_scatter(self.h, self_obj, _p(r), obj, tri, dist, inside, _p(u), _p(out), _p(ids))
+        return {"origin": out[0:3], "direction": out[3:6], "radiance": out[6:9], "origin_prob": out[9],
+                "direction_prob": out[10], "normal": out[11:14], "inside": int(ids[0]), "contact": int(ids[1]),
+                "delta": bool(delta)}
 
     def render_sample(self, cam, max_depth, seed, pixel, pss, nee=False, path_vector=False):
         c = Camera.from_dict(cam)
