@@ -120,6 +120,12 @@ int wr_scene_get_info(const wr_scene* scene, wr_scene_info* info);
 /* prim -> (object id, triangle id in the object): the ObjSourceInfo of
  * scene.cpp:34-36,110. */
 int wr_scene_prim_source(const wr_scene* scene, uint32_t prim, uint32_t* obj, uint32_t* tri);
+/* The same for n hits on the device (Scene::intersect's ObjSourceInfo out
+ * parameter, scene.cpp:110, for a whole batch): d_obj[i], d_tri[i] = the
+ * object and its own triangle index of d_hits[i].prim, WR_NONE for a miss or a
+ * stack-overflow record.  Asynchronous on the context stream; device pointers. */
+int wr_hit_sources(wr_ctx* ctx, const wr_scene* scene, const wr_hit* d_hits, uint64_t n, uint32_t* d_obj,
+                   uint32_t* d_tri);
 /* Canonical preorder export of the global tree (obj = WR_NONE) or of object
  * obj's tree, for comparison with another builder.  Pass NULL arrays to get
  * the sizes only. */
@@ -428,7 +434,7 @@ typedef struct {
     uint64_t seed;
 } wr_job_desc;
 
-enum { WR_JOB_QUEUED = 0, WR_JOB_RUNNING = 1, WR_JOB_DONE = 2, WR_JOB_FAILED = 3 };
+enum { WR_JOB_QUEUED = 0, WR_JOB_RUNNING = 1, WR_JOB_DONE = 2, WR_JOB_FAILED = 3, WR_JOB_CANCELLED = 4 };
 typedef struct {
     int32_t state;
     uint32_t reserved;
@@ -445,8 +451,20 @@ int wr_server_add_scene(wr_server* srv, const wr_scene* scene, uint32_t* scene_i
 int wr_job_submit(wr_server* srv, const wr_job_desc* job, uint64_t* job_id);
 /* Admits queued jobs that fit, runs one batch of wavefront iterations on every
  * admitted job, and retires finished ones.  *n_active (nullable) = admitted +
- * queued jobs left. */
+ * queued jobs left.  A job whose admission fails (device allocation) becomes
+ * WR_JOB_FAILED, its buffers are freed, and admission goes on with the next
+ * queued job: the step itself still succeeds. */
 int wr_server_step(wr_server* srv, uint32_t* n_active);
+/* Cancels a queued or running job: it becomes WR_JOB_CANCELLED at once and its
+ * film and sample window are freed (return to the budget) at once; a running
+ * job's paths leave the shared pool during the next wr_server_step, after
+ * which its job slot is reused.  Every other job's film is unchanged (bit for
+ * bit the same as without the cancelled job).  Cancelling a cancelled job does
+ * nothing; a finished or failed job gives WR_ESTATE. */
+int wr_job_cancel(wr_server* srv, uint64_t job_id);
+/* Test hook for failure isolation: the next n admissions fail as if their
+ * device allocation had failed. */
+int wr_server_inject_admit_failures(wr_server* srv, uint32_t n);
 int wr_job_get_status(wr_server* srv, uint64_t job_id, wr_job_status* status);
 /* Film of a finished job (any pointer may be NULL), as wr_film_read. */
 int wr_job_read(wr_server* srv, uint64_t job_id, float* mean_rgb, float* sum, float* comp, float* sumsq,

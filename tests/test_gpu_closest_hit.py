@@ -234,3 +234,27 @@ def test_ties_on_coincident_and_coplanar_triangles(ctx):
     vert = np.stack([xx.ravel(), yy.ravel(), np.ones(n, np.float32), np.zeros(n, np.float32),
                      np.zeros(n, np.float32), -np.ones(n, np.float32)], axis=1).astype(np.float32)
     _assert_same(sc.closest_hit(vert), osc.intersect(vert), "flat-grid ties")
+
+
+def test_hit_sources_batch(ctx):
+    """wr_hit_sources maps every hit's global triangle id to (object, triangle of the
+    object) as wr_scene_prim_source and the oracle's Scene::intersect do; misses give
+    WR_NONE.  An object without triangles sits between two others."""
+    objs, cam = S.cornell(k=1, width=24, height=24)
+    objs = objs[:3] + [dict(objs[3], verts=objs[3]["verts"][:0], tris=objs[3]["tris"][:0])] + objs[4:]
+    sc = W.Scene(ctx, objs)
+    rays = np.concatenate([S.pinhole_rays(cam), -S.pinhole_rays(cam)])
+    n = len(rays)
+    dr = ctx.to_device(rays)
+    dh = ctx.alloc(16 * n)
+    sc.closest_hit_device(dr, n, dh)
+    obj, tri = sc.hit_sources(dh, n)
+    hits = dh.download(np.uint32, 4 * n).view(W.HIT_DTYPE)
+    o = O.Scene(objs).intersect(rays)
+    assert np.array_equal(hits["prim"], o["prim"])
+    miss = hits["prim"] == W.NONE
+    assert miss.any() and not miss.all()
+    assert np.all(obj[miss] == W.NONE) and np.all(tri[miss] == W.NONE)
+    assert np.array_equal(obj[~miss], o["obj"][~miss]) and np.array_equal(tri[~miss], o["tri"][~miss])
+    for k in np.flatnonzero(~miss)[::97]:
+        assert sc.prim_source(int(hits["prim"][k])) == (obj[k], tri[k])

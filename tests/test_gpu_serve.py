@@ -23,7 +23,7 @@ def _scenes(ctx):
 def _drain(srv, jids, limit=100000):
     for _ in range(limit):
         srv.step()
-        if all(srv.status(j)["state"] in (W.JOB_DONE, W.JOB_FAILED) for j in jids):
+        if all(srv.status(j)["state"] in (W.JOB_DONE, W.JOB_FAILED, W.JOB_CANCELLED) for j in jids):
             return
     raise AssertionError("jobs did not finish")
 
@@ -103,3 +103,71 @@ def test_poisson_run_small(ctx):
     recs = V.run_poisson(srv, ids, [c for _, c in scs], jobs)
     summ = V.summarize(recs)
     assert summ["completed"] == 24 and summ["p99_ms"] >= summ["p50_ms"] > 0
+
+
+def _render_ref(ctx, built, cam, spp, depth, seed):
+    film = W.Film(ctx, cam["width"], cam["height"])
+    W.render(built, cam, film, spp, depth, seed, W.WR_MEGAKERNEL)
+    return film.read()
+
+
+def _same_film(a, b, what):
+    for key in ("sum", "comp", "sumsq"):
+        assert np.array_equal(a[key].view(np.uint32), b[key].view(np.uint32)), (what, key)
+
+
+def test_cancel_running_and_queued(ctx):
+    """A job cancelled while running (its paths in the shared pool) and one cancelled
+    while queued: both end CANCELLED with no film, every other job of the mix equals
+    its wr_render film bit for bit, and a job admitted afterwards into the freed slot
+    renders correctly too."""
+    scs, built = _scenes(ctx)
+    srv = W.Server(ctx, max_jobs=4)
+    ids = [srv.add_scene(b) for b in built]
+    cams = [V.job_camera(scs[s][1], res) for s, res, *_ in JOBS]
+    mix = [srv.submit(ids[s], cams[k], spp, depth, seed) for k, (s, res, spp, depth, seed) in enumerate(JOBS[:3])]
+    big_cam = V.job_camera(scs[6][1], 96)
+    big = srv.submit(ids[6], big_cam, 400, 12, 77)  # long enough to still run after one step
+    queued = srv.submit(ids[1], cams[0], 4, 8, 78)  # a fifth job waits for one of the 4 slots
+    srv.step()
+    assert srv.status(big)["state"] == W.JOB_RUNNING
+    assert srv.status(queued)["state"] == W.JOB_QUEUED
+    srv.cancel(big)
+    srv.cancel(queued)
+    srv.cancel(big)  # cancelling twice does nothing
+    for j in (big, queued):
+        assert srv.status(j)["state"] == W.JOB_CANCELLED
+        with pytest.raises(W.WalkRayError):
+            srv.read(j, 96, 96)
+    late_cam = V.job_camera(scs[3][1], 40)
+    late = srv.submit(ids[3], late_cam, 9, 10, 79)
+    _drain(srv, mix + [late])
+    for k, (s, res, spp, depth, seed) in enumerate(JOBS[:3]):
+        _same_film(srv.read(mix[k], res, res), _render_ref(ctx, built[s], cams[k], spp, depth, seed), k)
+    _same_film(srv.read(late, 40, 40), _render_ref(ctx, built[3], late_cam, 9, 10, 79), "late")
+    for j in mix:
+        with pytest.raises(W.WalkRayError):
+            srv.cancel(j)  # finished jobs cannot be cancelled
+    srv.release(big)
+    srv.release(queued)
+    srv.close()
+
+
+def test_failed_admission_is_isolated(ctx):
+    """An admission that fails (injected as a failed device allocation) fails that job
+    only: it leaves the queue, the step succeeds, and the jobs behind it run and equal
+    their wr_render films bit for bit."""
+    scs, built = _scenes(ctx)
+    srv = W.Server(ctx)
+    ids = [srv.add_scene(b) for b in built]
+    cams = [V.job_camera(scs[s][1], res) for s, res, *_ in JOBS]
+    srv.inject_admit_failures(1)
+    jobs = [srv.submit(ids[s], cams[k], spp, depth, seed) for k, (s, res, spp, depth, seed) in enumerate(JOBS[:3])]
+    _drain(srv, jobs)
+    assert srv.status(jobs[0])["state"] == W.JOB_FAILED
+    for k in (1, 2):
+        s, res, spp, depth, seed = JOBS[k]
+        assert srv.status(jobs[k])["state"] == W.JOB_DONE
+        _same_film(srv.read(jobs[k], res, res), _render_ref(ctx, built[s], cams[k], spp, depth, seed), k)
+    srv.release(jobs[0])
+    srv.close()

@@ -105,7 +105,7 @@ class ConfigInfo(C.Structure):
                 ("renderer", C.c_char * 32), ("scene_name", C.c_char * 64)]
 
 
-JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = 0, 1, 2, 3, 4
 
 
 def header_symbols():
@@ -166,6 +166,9 @@ def lib():
         "wr_job_get_status": (i32, [vp, u64, P(JobStatus)]),
         "wr_job_read": (i32, [vp, u64, vp, vp, vp, vp, P(u32)]),
         "wr_job_release": (i32, [vp, u64]),
+        "wr_job_cancel": (i32, [vp, u64]),
+        "wr_server_inject_admit_failures": (i32, [vp, u32]),
+        "wr_hit_sources": (i32, [vp, vp, vp, u64, vp, vp]),
         "wr_server_clock": (C.c_double, [vp]),
         "wr_autotune_closest_hit": (i32, [vp, vp, vp, u64, P(ChTuning)]),
         "wr_scene_set_closest_hit_tuning": (i32, [vp, P(ChTuning)]),
@@ -610,6 +613,13 @@ class Scene:
         _check(lib().wr_scene_prim_source(self.h, prim, C.byref(o), C.byref(t)), self.ctx.h, "wr_scene_prim_source")
         return o.value, t.value
 
+    def hit_sources(self, d_hits, n):
+        """wr_hit_sources over n device hit records: (obj, tri) arrays on the host."""
+        do, dt = self.ctx.alloc(4 * max(n, 1)), self.ctx.alloc(4 * max(n, 1))
+        _check(lib().wr_hit_sources(self.ctx.h, self.h, d_hits.ptr, n, do.ptr, dt.ptr), self.ctx.h,
+               "wr_hit_sources")
+        return do.download(np.uint32, n), dt.download(np.uint32, n)
+
     def export_tree(self, obj=NONE):
         L = lib()
         n, m = C.c_uint32(), C.c_uint32()
@@ -773,6 +783,13 @@ class Server:
 
     def release(self, jid):
         _check(lib().wr_job_release(self.h, jid), self.ctx.h, "wr_job_release")
+
+    def cancel(self, jid):
+        _check(lib().wr_job_cancel(self.h, jid), self.ctx.h, "wr_job_cancel")
+
+    def inject_admit_failures(self, n):
+        """Test hook: the next n admissions fail as if their device allocation had failed."""
+        _check(lib().wr_server_inject_admit_failures(self.h, n), self.ctx.h, "wr_server_inject_admit_failures")
 
     def clock(self):
         return lib().wr_server_clock(self.h)

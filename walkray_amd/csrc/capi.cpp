@@ -697,6 +697,15 @@ int wr_closest_hit(wr_ctx* ctx, const wr_scene* s, const wr_ray* d_rays, uint64_
     return WR_OK;
 }
 
+int wr_hit_sources(wr_ctx* ctx, const wr_scene* s, const wr_hit* d_hits, uint64_t n, uint32_t* d_obj,
+                   uint32_t* d_tri) {
+    if (!ctx || !s || (n && (!d_hits || !d_obj || !d_tri))) return fail(ctx, WR_EINVAL, "wr_hit_sources: bad argument");
+    if (!s->built) return fail(ctx, WR_ESTATE, "wr_hit_sources: scene not built");
+    hipError_t e = launch_hit_sources(s->dev, s->prim_base.back(), d_hits, n, d_obj, d_tri, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "k_hit_sources launch");
+    return WR_OK;
+}
+
 int wr_autotune_closest_hit(wr_ctx* ctx, wr_scene* s, const wr_ray* d_rays, uint64_t n,
                             wr_closest_hit_tuning* best) {
     if (!ctx || !s || !d_rays || n == 0) return fail(ctx, WR_EINVAL, "wr_autotune_closest_hit: bad argument");

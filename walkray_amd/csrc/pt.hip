@@ -260,6 +260,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     StackSpill spill;
     LaneStack st(&spill);
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        if (E.ctl[A.job[i]].state != wr::kJobRunning) {  // a cancelled job: drop its path, write nothing
+            A.flags[i] = kDead;
+            continue;
+        }
         const wr::JobDesc& J = E.jobs[A.job[i]];
         const SceneDev& S = E.scenes[J.scene];
         const SceneCtx sc = make_scene_ctx(S);
@@ -715,6 +719,12 @@ hipError_t Engine::admit(uint32_t slot, const JobDesc& J, hipStream_t st) {
     return hipStreamSynchronize(st);  // the sources are host stack copies
 }
 
+hipError_t Engine::cancel_slot(uint32_t slot, hipStream_t st) {
+    const uint32_t v = kJobCancelled;
+    WR_CHECK(hipMemcpyAsync(&ctl[slot].state, &v, sizeof(v), hipMemcpyHostToDevice, st));
+    return hipStreamSynchronize(st);
+}
+
 hipError_t Engine::release_slot(uint32_t slot, hipStream_t st) {
     JobCtl c{};
     WR_CHECK(hipMemcpyAsync(ctl + slot, &c, sizeof(JobCtl), hipMemcpyHostToDevice, st));
@@ -742,10 +752,30 @@ void fill_job_camera(JobDesc& J, const CamHost& c) {
     J.H = c.H;
 }
 
-// Renders passes [pass0, pass0 + npass) of one film on the engine, alone.
+static hipError_t run_wavefront_once(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed,
+                                     uint32_t pass0, uint32_t npass, bool nee, const FilmBufs& film, Engine& E,
+                                     uint32_t cap, uint32_t grid, WaveStats* stats, hipStream_t st,
+                                     uint32_t lds_levels);
+
+// Renders passes [pass0, pass0 + npass) of one film on the engine, alone.  On
+// any error the engine is released, so the next call starts from a fresh pool
+// rather than from the live paths and counters of the failed one.
 hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
                          uint32_t npass, bool nee, const FilmBufs& film, Engine& E, uint32_t cap, uint32_t grid,
                          WaveStats* stats, hipStream_t st, uint32_t lds_levels) {
+    const hipError_t e =
+        run_wavefront_once(S, cam, max_depth, seed, pass0, npass, nee, film, E, cap, grid, stats, st, lds_levels);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        E.release();
+    }
+    return e;
+}
+
+static hipError_t run_wavefront_once(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed,
+                                     uint32_t pass0, uint32_t npass, bool nee, const FilmBufs& film, Engine& E,
+                                     uint32_t cap, uint32_t grid, WaveStats* stats, hipStream_t st,
+                                     uint32_t lds_levels) {
     const uint32_t npix = cam.W * cam.H;
     if (npass == 0 || npix == 0) return hipSuccess;
     const uint64_t total = (uint64_t)npass * npix;

@@ -18,6 +18,9 @@ struct wr_film {
     float* comp = nullptr;
     float* sumsq = nullptr;
     uint32_t passes = 0;
+    // set when a render failed part way: the sums no longer match `passes`, so
+    // the film is refused until wr_film_clear or wr_film_write resets it
+    bool broken = false;
 };
 
 namespace {
@@ -103,11 +106,18 @@ int wr_film_clear(wr_film* f) {
     if (e == hipSuccess) e = hipStreamSynchronize(f->ctx->stream);
     if (e != hipSuccess) return hip_fail(f->ctx, e, "wr_film_clear");
     f->passes = 0;
+    f->broken = false;
     return WR_OK;
+}
+
+static int film_broken(wr_film* f, const char* who) {
+    return fail(f->ctx, WR_ESTATE,
+                std::string(who) + ": a render into this film failed part way; clear it or write a checkpoint into it");
 }
 
 int wr_film_read(wr_film* f, float* mean_rgb, float* sum, float* comp, float* sumsq, uint32_t* passes) {
     if (!f) return fail(nullptr, WR_EINVAL, "wr_film_read: film is NULL");
+    if (f->broken) return film_broken(f, "wr_film_read");
     const size_t n = 3 * (size_t)f->W * f->H;
     std::vector<float> tmp;
     float* s = sum;
@@ -138,6 +148,7 @@ int wr_film_write(wr_film* f, const float* sum, const float* comp, const float* 
     if (e == hipSuccess) e = hipStreamSynchronize(f->ctx->stream);
     if (e != hipSuccess) return hip_fail(f->ctx, e, "wr_film_write");
     f->passes = passes;
+    f->broken = false;
     return WR_OK;
 }
 
@@ -151,6 +162,7 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
     if (prm->flags & ~(uint32_t)WR_RENDER_NEE) return fail(ctx, WR_EINVAL, "wr_render: unknown flags");
     const bool nee = (prm->flags & WR_RENDER_NEE) != 0;
     if ((uint64_t)cam->width * cam->height >= (1ull << 31)) return fail(ctx, WR_EINVAL, "wr_render: image too large");
+    if (film->broken) return film_broken(film, "wr_render");
     (void)hipSetDevice(ctx->device);
     const CamHost ch = make_cam(*cam);
     FilmBufs fb{film->sum, film->comp, film->sumsq};
@@ -202,7 +214,10 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
     if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
     float ms = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-    if (e != hipSuccess) return hip_fail(ctx, e, "wr_render");
+    if (e != hipSuccess) {
+        film->broken = true;
+        return hip_fail(ctx, e, "wr_render");
+    }
     film->passes += prm->spp;
     if (stats) {
         stats->paths = (uint64_t)prm->spp * cam->width * cam->height;
@@ -227,6 +242,7 @@ extern "C" int wr_render_bdpt(wr_ctx* ctx, const wr_scene* s, const wr_camera* c
         if (o.moving) return fail(ctx, WR_EINVAL, "wr_render_bdpt: moving objects are not supported");
     const uint64_t npix = (uint64_t)cam->width * cam->height;
     if (npix == 0 || npix >= (1ull << 31)) return fail(ctx, WR_EINVAL, "wr_render_bdpt: bad image size");
+    if (film->broken) return film_broken(film, "wr_render_bdpt");
     (void)hipSetDevice(ctx->device);
     const CamHost ch = make_cam(*cam);
     const uint32_t maxv = prm->max_depth + 3;  // the longest sub-path: first ray, max_depth scatters, the end
@@ -255,6 +271,7 @@ extern "C" int wr_render_bdpt(wr_ctx* ctx, const wr_scene* s, const wr_camera* c
     }
     if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
     const FilmBufs fb{film->sum, film->comp, film->sumsq};
+    const bool launched = e == hipSuccess && prm->spp > 0;  // from here on a failure leaves the film part-written
     for (uint32_t k = 0; k < prm->spp && e == hipSuccess; k += (uint32_t)batch) {
         const uint32_t nb = (uint32_t)std::min<uint64_t>(batch, prm->spp - k);
         e = launch_bdpt_passes(s->dev, s->hdrs[0], ch, prm->max_depth, prm->seed, film->passes + k, nb, maxv, verts,
@@ -268,7 +285,10 @@ extern "C" int wr_render_bdpt(wr_ctx* ctx, const wr_scene* s, const wr_camera* c
     (void)hipFree(pix);
     (void)hipFree(qbuf);
     (void)hipFree(q.counts);
-    if (e != hipSuccess) return hip_fail(ctx, e, "wr_render_bdpt");
+    if (e != hipSuccess) {
+        film->broken = film->broken || launched;
+        return hip_fail(ctx, e, "wr_render_bdpt");
+    }
     film->passes += prm->spp;
     if (stats) {
         stats->paths = npix * prm->spp;

@@ -55,7 +55,8 @@ struct wr_server {
     std::vector<const wr_scene*> scenes;
     std::map<uint64_t, Job> jobs;
     std::deque<uint64_t> queue;
-    std::vector<int64_t> slot_job;  // engine job slot -> job id (-1 free)
+    std::vector<int64_t> slot_job;  // engine job slot -> job id (-1 free, kDraining: a cancelled job's paths)
+    uint32_t inject_admit_failures = 0;  // test hook: admissions left to fail as if allocation had failed
     uint64_t next_id = 1;
     bool tail = false;  // every running job had issued all its paths at the last read-back
     std::chrono::steady_clock::time_point t0;
@@ -63,6 +64,10 @@ struct wr_server {
 };
 
 using namespace wr;
+
+// slot_job value of a slot whose job was cancelled while running: the slot is
+// reused once the next replay has dropped the job's paths from the pool.
+static constexpr int64_t kDraining = -2;
 
 static void free_job_window(Job& j) {
     if (j.contrib) (void)hipFree(j.contrib);
@@ -161,6 +166,10 @@ static hipError_t admit_one(wr_server* s, Job& j, int slot) {
     hipStream_t st = s->ctx->stream;
     const uint32_t npix = j.desc.cam.width * j.desc.cam.height;
     const size_t fb = sizeof(float) * 3 * (size_t)npix;
+    if (s->inject_admit_failures) {
+        s->inject_admit_failures--;
+        return hipErrorOutOfMemory;
+    }
     hipError_t e = hipMalloc((void**)&j.sum, fb);
     if (e == hipSuccess) e = hipMalloc((void**)&j.comp, fb);
     if (e == hipSuccess) e = hipMalloc((void**)&j.sumsq, fb);
@@ -205,7 +214,12 @@ static hipError_t admit_one(wr_server* s, Job& j, int slot) {
     J.fcomp = j.comp;
     J.fsumsq = j.sumsq;
     e = s->engine.admit((uint32_t)slot, J, st);
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        free_job_window(j);
+        free_job_film(j);
+        return e;
+    }
     j.slot = slot;
     j.st.state = WR_JOB_RUNNING;
     j.st.admit_s = s->now();
@@ -224,17 +238,25 @@ int wr_server_step(wr_server* s, uint32_t* n_active) {
         uint32_t running = 0;
         for (uint32_t k = 0; k < kMaxJobs; k++) {
             if (s->slot_job[k] >= 0) running++;
-            else if (slot < 0) slot = (int)k;
+            else if (s->slot_job[k] == -1 && slot < 0) slot = (int)k;  // a draining slot is not free yet
         }
         if (slot < 0 || running >= s->max_jobs || s->used + j.film_bytes + j.window_bytes > s->budget) break;
         hipError_t e = admit_one(s, j, slot);
-        if (e != hipSuccess) return hip_fail(s->ctx, e, "wr_server_step: admit");
+        if (e != hipSuccess) {
+            // the failure stays with this job: its buffers are freed, it leaves the
+            // queue as failed, and admission goes on with the next one
+            (void)hipGetLastError();  // a failed allocation must not surface at a later launch check
+            j.st.state = WR_JOB_FAILED;
+            j.st.done_s = s->now();
+            s->queue.pop_front();
+            continue;
+        }
         s->slot_job[slot] = (int64_t)s->queue.front();
         s->queue.pop_front();
         admitted = true;
     }
-    uint32_t running = 0;
-    for (uint32_t k = 0; k < kMaxJobs; k++) running += s->slot_job[k] >= 0;
+    uint32_t running = 0;  // slots with paths in the pool: admitted jobs and draining cancelled ones
+    for (uint32_t k = 0; k < kMaxJobs; k++) running += s->slot_job[k] >= 0 || s->slot_job[k] == kDraining;
     if (running) {
         std::vector<JobCtl> hc(kMaxJobs);
         // short replays once only draining is left (nothing queued or newly admitted)
@@ -246,6 +268,12 @@ int wr_server_step(wr_server* s, uint32_t* n_active) {
             if (s->slot_job[k] >= 0 && hc[k].state != kJobDone &&
                 hc[k].next_work < s->jobs[(uint64_t)s->slot_job[k]].total)
                 s->tail = false;
+        for (uint32_t k = 0; k < kMaxJobs; k++) {
+            if (s->slot_job[k] != kDraining) continue;  // the replay dropped the cancelled job's paths
+            e = s->engine.release_slot(k, st);
+            if (e != hipSuccess) return hip_fail(s->ctx, e, "wr_server_step: release");
+            s->slot_job[k] = -1;
+        }
         for (uint32_t k = 0; k < kMaxJobs; k++) {
             if (s->slot_job[k] < 0 || hc[k].state != kJobDone) continue;
             Job& j = s->jobs[(uint64_t)s->slot_job[k]];
@@ -259,9 +287,42 @@ int wr_server_step(wr_server* s, uint32_t* n_active) {
             j.slot = -1;
         }
         running = 0;
-        for (uint32_t k = 0; k < kMaxJobs; k++) running += s->slot_job[k] >= 0;
+        for (uint32_t k = 0; k < kMaxJobs; k++) running += s->slot_job[k] >= 0 || s->slot_job[k] == kDraining;
     }
     if (n_active) *n_active = running + (uint32_t)s->queue.size();
+    return WR_OK;
+}
+
+int wr_job_cancel(wr_server* s, uint64_t id) {
+    if (!s) return fail(nullptr, WR_EINVAL, "wr_job_cancel: server is NULL");
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return fail(s->ctx, WR_EINVAL, "wr_job_cancel: unknown job");
+    Job& j = it->second;
+    if (j.st.state == WR_JOB_CANCELLED) return WR_OK;
+    if (j.st.state == WR_JOB_DONE || j.st.state == WR_JOB_FAILED)
+        return fail(s->ctx, WR_ESTATE, "wr_job_cancel: job already finished");
+    if (j.st.state == WR_JOB_QUEUED) {
+        s->queue.erase(std::find(s->queue.begin(), s->queue.end(), id));
+    } else {
+        // running: once the state is on the device no kernel touches the job's
+        // window or film (k_wf_finish drops its paths first), so both are freed now;
+        // the slot waits for the next replay
+        hipError_t e = s->engine.cancel_slot((uint32_t)j.slot, s->ctx->stream);
+        if (e != hipSuccess) return hip_fail(s->ctx, e, "wr_job_cancel");
+        free_job_window(j);
+        free_job_film(j);
+        s->used -= j.film_bytes + j.window_bytes;
+        s->slot_job[j.slot] = kDraining;
+        j.slot = -1;
+    }
+    j.st.state = WR_JOB_CANCELLED;
+    j.st.done_s = s->now();
+    return WR_OK;
+}
+
+int wr_server_inject_admit_failures(wr_server* s, uint32_t n) {
+    if (!s) return fail(nullptr, WR_EINVAL, "wr_server_inject_admit_failures: server is NULL");
+    s->inject_admit_failures = n;
     return WR_OK;
 }
 

@@ -220,9 +220,41 @@ __global__ __launch_bounds__(256) void k_inside_object(SceneDev S, const uint32_
     out[i] = find_inside(S, o, d, rp[7], st);
 }
 
+// Global triangle id -> (object, triangle of the object) for a batch of hits
+// (ObjSourceInfo, scene.cpp:34-36,110): the last object whose first global
+// triangle is <= prim, found by binary search over the object table.
+__global__ __launch_bounds__(256) void k_hit_sources(const wr_object_gpu* __restrict__ objs, uint32_t n_objs,
+                                                     uint32_t n_tris, const uint4* __restrict__ hits, uint64_t n,
+                                                     uint32_t* __restrict__ obj, uint32_t* __restrict__ tri) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t prim = hits[i].x;
+    uint32_t o = WR_NONE, t = WR_NONE;
+    if (prim < n_tris) {
+        uint32_t lo = 0, hi = n_objs;  // first object with prim_base > prim lies in [lo, hi]
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (objs[mid].prim_base <= prim) lo = mid + 1;
+            else hi = mid;
+        }
+        o = lo - 1;
+        t = prim - objs[o].prim_base;
+    }
+    obj[i] = o;
+    tri[i] = t;
+}
+
 }  // namespace wrd
 
 namespace wr {
+
+hipError_t launch_hit_sources(const SceneDev& S, uint32_t n_tris, const void* hits, uint64_t n, uint32_t* obj,
+                              uint32_t* tri, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    wrd::k_hit_sources<<<(uint32_t)((n + 255) / 256), 256, 0, st>>>(S.objs, S.n_objs, n_tris, (const uint4*)hits, n,
+                                                                     obj, tri);
+    return hipGetLastError();
+}
 
 // Workgroups of the persistent kernel with D LDS stack levels: per_cu per CU
 // (0: as many as fit).

@@ -69,6 +69,8 @@ hipError_t launch_closest_hit(const SceneDev& S, const wr_tree_hdr& H, const flo
                               hipStream_t st);
 hipError_t launch_closest_hit_path(const SceneDev& S, const wr_tree_hdr& H, const uint32_t* rays, uint64_t n,
                                    void* hits, hipStream_t st);
+hipError_t launch_hit_sources(const SceneDev& S, uint32_t n_tris, const void* hits, uint64_t n, uint32_t* obj,
+                              uint32_t* tri, hipStream_t st);
 hipError_t launch_inside_object(const SceneDev& S, uint32_t max_obj_depth, const uint32_t* rays, uint64_t n,
                                 uint32_t* out, hipStream_t st);
 }  // namespace wr

@@ -33,7 +33,11 @@ constexpr int kTailIters = 2;            // iterations per replay once every job
 constexpr uint32_t kDefaultPathSlots = 1u << 22;  // 4M paths in flight: 2 x 416 MB of path state
 constexpr uint32_t kBdptMaxDepth = 32;           // BidirectionalPathTracer: maxPathLen limit of wr_render_bdpt
 
-enum : uint32_t { kJobFree = 0, kJobRunning = 1, kJobDone = 2 };
+enum : uint32_t { kJobFree = 0, kJobRunning = 1, kJobDone = 2, kJobCancelled = 3 };
+// A cancelled job's paths still in the pool are dropped by the next extend
+// stage (k_wf_finish reads the job state first), so no kernel touches its
+// sample window or film after the state is written; its slot is reused only
+// after one more replay, once those paths are gone.
 enum : uint32_t { kJobNee = 1 };  // JobDesc::flags: PathTracer's next-event branch
 
 // Device record of one admitted job.
@@ -102,6 +106,7 @@ struct Engine {
     hipError_t set_scene(uint32_t slot, const SceneDev& S, hipStream_t st);
     hipError_t admit(uint32_t slot, const JobDesc& J, hipStream_t st);
     hipError_t release_slot(uint32_t slot, hipStream_t st);
+    hipError_t cancel_slot(uint32_t slot, hipStream_t st);
     // One replay: kGraphIters iterations, or kTailIters when `tail` (every
     // running job has issued all its camera paths, so only draining is left).
     hipError_t step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt, bool tail = false);

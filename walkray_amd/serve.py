@@ -78,7 +78,7 @@ def run_poisson(server, scene_ids, cams, jobs, release=True):
         server.step()
         for jid in list(live):
             st = server.status(jid)
-            if st["state"] in (W.JOB_DONE, W.JOB_FAILED):
+            if st["state"] in (W.JOB_DONE, W.JOB_FAILED, W.JOB_CANCELLED):
                 j = live.pop(jid)
                 rec = dict(j)
                 rec.update(state=st["state"], latency_s=st["done_s"] - (start + j["arrival"]),
