@@ -127,8 +127,11 @@ struct Tree {
         delete n;
     }
 
-    /* kdtree.cpp:141-240 (splitBest) */
+    /* kdtree.cpp:141-240 (splitBest).  The two subtrees of a node are built as
+     * independent OpenMP tasks when the node is large: each subtree depends only
+     * on its own triangle list, so the tree is the serial recursion's. */
     void splitBest(Node* node, unsigned d) {
+#pragma omp critical(orc_depth)
         if (d > depth) depth = d;
         float bestRatio = 10;
         unsigned bestDim = 0;
@@ -188,8 +191,17 @@ struct Tree {
             if (mask == 0) node->right->tris.push_back(node->tris[i]);
             else node->left->tris.push_back(node->tris[i]);
         }
-        splitBest(node->left, d + 1);
-        splitBest(node->right, d + 1);
+        if (n >= 65536) {
+            Node *l = node->left, *r = node->right;
+#pragma omp task firstprivate(l, d)
+            splitBest(l, d + 1);
+#pragma omp task firstprivate(r, d)
+            splitBest(r, d + 1);
+#pragma omp taskwait
+        } else {
+            splitBest(node->left, d + 1);
+            splitBest(node->right, d + 1);
+        }
     }
 
     void build() { /* kdtree.cpp:242-258 */
@@ -198,7 +210,13 @@ struct Tree {
         const unsigned nt = (unsigned)(idx.size() / 3);
         for (unsigned i = 0; i < nt; i++) root->tris.push_back(i);
         depth = 0;
-        splitBest(root, 0);
+        if (nt >= 65536) {
+#pragma omp parallel
+#pragma omp single
+            splitBest(root, 0);
+        } else {
+            splitBest(root, 0);
+        }
     }
 
     /* kdtree.cpp:279-306; cond is evaluated by the caller-supplied functor. */
@@ -276,6 +294,19 @@ struct Tree {
             minDist = (inter && (dist < minDist || minDist < 0)) ? dist : minDist;
         }
         return minDist;
+    }
+};
+
+/* Counts, during an unpruned traversal, the triangles other than `win` that the
+ * triangle test accepts at a distance within tol of t, and accepts none, so the
+ * traversal visits every candidate (the near-ties of a closest hit). */
+struct TieCount {
+    float t, tol;
+    unsigned win;
+    mutable unsigned n = 0;
+    bool operator()(const Ray&, unsigned ti, float dist) const {
+        if (ti != win && std::fabs((double)dist - (double)t) < (double)tol) n++;
+        return false;
     }
 };
 
@@ -1677,6 +1708,24 @@ void orc_scene_intersect(void* sp, const float* rays, const uint32_t* inside, co
             prim[i] = ORC_NONE; obj[i] = ORC_NONE; tri[i] = ORC_NONE;
             t[i] = -1.f; b1[i] = 0.f; b2[i] = 0.f;
         }
+    }
+}
+
+void orc_scene_near_ties(void* sp, const float* rays, uint64_t n, const uint32_t* prim, const float* t, float tol,
+                         uint32_t* count, int nthreads) {
+    Scene* s = (Scene*)sp;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+        count[i] = 0;
+        if (prim[i] == ORC_NONE) continue;
+        Ray r;
+        r.o = V(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]);
+        r.d = V(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
+        TieCount tc{t[i], tol, prim[i]};
+        unsigned tid = 0;
+        (void)s->tree.intersect(r, tid, tc);
+        count[i] = tc.n;
     }
 }
 

@@ -55,6 +55,11 @@ void     orc_scene_intersect(void* s, const float* rays, const uint32_t* inside,
                              const uint32_t* contact, uint64_t n, int noself, uint32_t* prim,
                              uint32_t* obj, uint32_t* tri, float* t, float* b1, float* b2,
                              int nthreads);
+/* Near-ties of closest hits (no Condition): for each ray with a hit (prim, t),
+ * the number of other triangles the unpruned traversal (kdtree.cpp:308-347)
+ * accepts at |dist - t| < tol; 0 for a miss. */
+void     orc_scene_near_ties(void* s, const float* rays, uint64_t n, const uint32_t* prim, const float* t,
+                             float tol, uint32_t* count, int nthreads);
 /* Scene::findInsideObject (scene.cpp:115-144); cur = ORC_NONE for NULL. */
 void     orc_scene_find_inside(void* s, const float* rays, const uint32_t* cur, uint64_t n,
                                uint32_t* out, int nthreads);

@@ -48,6 +48,7 @@ def lib():
         L.orc_scene_build.argtypes = [vp]
         L.orc_scene_intersect.argtypes = [vp, vp, vp, vp, u64, C.c_int, vp, vp, vp, vp, vp, vp, C.c_int]
         L.orc_scene_find_inside.argtypes = [vp, vp, vp, u64, vp, C.c_int]
+        L.orc_scene_near_ties.argtypes = [vp, vp, u64, vp, vp, f32, vp, C.c_int]
         L.orc_world_normal.argtypes = [vp, u32, u32, vp, vp, vp, vp]
         L.orc_render.argtypes = [vp, vp, u32, u64, u32, u32, vp, vp, C.c_int]
         L.orc_render_sample.argtypes = [vp, vp, u32, u64, u32, u32, vp, vp]
@@ -214,6 +215,15 @@ class Scene:
         else:
             lib().orc_render(self.h, C.byref(c), max_depth, seed, pass0, spp, _p(sums[0]), _p(sums[1]), nthreads)
         return sums
+
+    def near_ties(self, rays, prim, t, tol=1e-6, nthreads=0):
+        """Per ray, the other triangles accepted within tol of its hit distance t."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        prim = np.ascontiguousarray(prim, dtype=np.uint32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        out = np.empty(len(rays), np.uint32)
+        lib().orc_scene_near_ties(self.h, _p(rays), len(rays), _p(prim), _p(t), tol, _p(out), nthreads)
+        return out
 
     def scatter(self, self_obj, ray, obj, tri, dist, inside, rn):
         """AbstractObject::scatter of object self_obj for the ray (o, d) hitting local triangle

@@ -258,3 +258,43 @@ def test_hit_sources_batch(ctx):
     assert np.array_equal(obj[~miss], o["obj"][~miss]) and np.array_equal(tri[~miss], o["tri"][~miss])
     for k in np.flatnonzero(~miss)[::97]:
         assert sc.prim_source(int(hits["prim"][k])) == (obj[k], tri[k])
+
+
+def _grazing_rays(objs, n, seed, eps_max=3e-4):
+    """Rays nearly in the plane of a random triangle (|n . d| below eps_max), aimed through
+    its centroid from a random distance: the regime where the reference's t is
+    ill-conditioned, for the fast pass's pruning margin."""
+    rng = np.random.default_rng(seed)
+    V = np.concatenate([o["verts"] for o in objs]).astype(np.float64)
+    T, off = [], 0
+    for o in objs:
+        T.append(o["tris"].astype(np.int64) + off)
+        off += len(o["verts"])
+    T = np.concatenate(T)
+    pick = rng.integers(len(T), size=n)
+    a, b, c = V[T[pick, 0]], V[T[pick, 1]], V[T[pick, 2]]
+    nrm = np.cross(b - a, c - a)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    tan = np.cross(nrm, rng.normal(size=(n, 3)))
+    tan /= np.linalg.norm(tan, axis=1, keepdims=True)
+    eps = rng.uniform(-eps_max, eps_max, n)[:, None]
+    d = tan * np.sqrt(1 - eps * eps) + nrm * eps
+    cen = (a + b + c) / 3
+    o = cen - d * rng.uniform(0.01, 0.5, n)[:, None] + nrm * rng.uniform(-1e-4, 1e-4, n)[:, None]
+    return np.concatenate([o, d], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("which", ["heightfield", "field"])
+def test_near_grazing_rays(ctx, which):
+    """131,072 rays at |n . d| < 3e-4 to a triangle they pass next to: prim, t and
+    barycentrics bit-identical to the oracle's unpruned traversal."""
+    if which == "heightfield":
+        objs = [S.heightfield(257, seed=4)]
+    else:
+        objs, _ = S.random_field(n_objects=48, width=16, height=16, sphere_k=2)
+    sc, osc = _scene_pair(ctx, objs)
+    rays = _grazing_rays(objs, 1 << 17, seed=5)
+    hits = sc.closest_hit(rays)
+    ref = osc.intersect(rays)
+    _assert_same(hits, ref, f"grazing-{which}")
+    assert (ref["prim"] != O.NONE).mean() > 0.3

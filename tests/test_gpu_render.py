@@ -127,3 +127,36 @@ def test_checkpoint_resume(ctx):
     assert a["passes"] == b["passes"] == 10
     for k in ("sum", "comp", "sumsq"):
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_full_c1_matches_oracle(ctx):
+    """The full Cornell workload (two k=5 icospheres, 40,974 triangles, 256^2, 16 spp,
+    max depth 8) on both engines against the oracle's PathTracer, pixel by pixel."""
+    objs, cam = S.cornell(k=5, width=256, height=256)
+    spp, depth, seed = 16, 8, 31
+    sc = W.Scene(ctx, objs)
+    g = R.render_scene(ctx, objs, cam, spp, depth, seed, W.WR_WAVEFRONT, scene=sc)
+    m = R.render_scene(ctx, objs, cam, spp, depth, seed, W.WR_MEGAKERNEL, scene=sc)
+    assert np.array_equal(g["sum"].view(np.uint32), m["sum"].view(np.uint32))
+    s, q = O.Scene(objs).render(cam, spp, depth, seed)
+    z, zero_ok = _zcheck(g, s, q, spp)
+    assert zero_ok
+    assert (z > 3).mean() <= 0.01, f"{(z > 3).mean():.4f} of pixel-channels beyond 3 sigma"
+    assert z.max() <= 6, f"max z {z.max():.2f}"
+    same = np.isclose(g["sum"], s.astype(np.float32), rtol=1e-4, atol=1e-6).mean()
+    assert same > 0.5, f"only {same:.3f} of pixel sums agree closely"
+
+
+def test_field_depth16_plain_matches_oracle(ctx):
+    """The 512-object field (diffuse, Phong and glass objects, two area lights) with the
+    plain estimator at max depth 16, at 160x90 x 32 spp, against the oracle."""
+    objs, cam = S.random_field(n_objects=512, width=160, height=90)
+    spp, depth, seed = 32, 16, 41
+    g = R.render_scene(ctx, objs, cam, spp, depth, seed, W.WR_WAVEFRONT)
+    s, q = O.Scene(objs).render(cam, spp, depth, seed)
+    z, zero_ok = _zcheck(g, s, q, spp)
+    assert zero_ok
+    assert (z > 3).mean() <= 0.01, f"{(z > 3).mean():.4f} of pixel-channels beyond 3 sigma"
+    assert z.max() <= 6, f"max z {z.max():.2f}"
+    same = np.isclose(g["sum"], s.astype(np.float32), rtol=1e-4, atol=1e-6).mean()
+    assert same > 0.5, f"only {same:.3f} of pixel sums agree closely"

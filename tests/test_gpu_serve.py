@@ -171,3 +171,44 @@ def test_failed_admission_is_isolated(ctx):
         _same_film(srv.read(jobs[k], res, res), _render_ref(ctx, built[s], cams[k], spp, depth, seed), k)
     srv.release(jobs[0])
     srv.close()
+
+
+def test_full_size_serving_mix(ctx):
+    """The served-jobs workload at full size: the eight resident scenes (10k to 2M
+    triangles) and a mix with a 1024^2 job on the 2M-triangle heightfield and jobs on
+    the largest fields.  Every film equals its alone-in-a-server film and wr_render's
+    (megakernel) bit for bit; two served jobs at reduced resolution (a heightfield and
+    a 256-object field) agree with the oracle's PathTracer per pixel."""
+    from oracle import oracle as O
+    scs = V.serving_scenes()
+    built = [W.Scene(ctx, objs) for objs, _ in scs]
+    tris = [b.info()["n_triangles"] for b in built]
+    assert min(tris) >= 10000 and max(tris) >= 2000000
+    jobs = [(3, 1024, 4, 8, 201), (7, 256, 16, 16, 202), (2, 512, 8, 12, 203), (5, 128, 64, 10, 204),
+            (1, 64, 16, 8, 205), (6, 64, 16, 12, 206)]
+    srv = W.Server(ctx)
+    ids = [srv.add_scene(b) for b in built]
+    cams = [V.job_camera(scs[s][1], res) for s, res, *_ in jobs]
+    mix = [srv.submit(ids[s], cams[k], spp, depth, seed) for k, (s, res, spp, depth, seed) in enumerate(jobs)]
+    _drain(srv, mix)
+    for k, (s, res, spp, depth, seed) in enumerate(jobs):
+        got = srv.read(mix[k], res, res)
+        _same_film(got, _render_ref(ctx, built[s], cams[k], spp, depth, seed), k)
+        if k in (0, 3):
+            alone_srv = W.Server(ctx)
+            aid = [alone_srv.add_scene(b) for b in built]
+            j = alone_srv.submit(aid[s], cams[k], spp, depth, seed)
+            _drain(alone_srv, [j])
+            _same_film(alone_srv.read(j, res, res), got, f"alone {k}")
+            alone_srv.close()
+        if k in (4, 5):
+            st = {"sum": got["sum"], "sumsq": got["sumsq"], "passes": spp}
+            o_sum, o_sq = O.Scene(scs[s][0]).render(cams[k], spp, depth, seed)
+            mg, seg = R.film_stats(st)
+            mo = o_sum / spp
+            se = np.sqrt(seg ** 2 + np.maximum(o_sq / spp - mo * mo, 0) / spp)
+            nz = se > 0
+            z = np.abs(mg - mo)[nz] / se[nz]
+            assert np.all(np.abs(mg - mo)[~nz] <= 1e-5 * np.maximum(1.0, np.abs(mo[~nz])))
+            assert (z > 3).mean() <= 0.01 and z.max() <= 6, (k, (z > 3).mean(), z.max())
+    srv.close()

@@ -279,6 +279,30 @@ def test_tie_goes_to_right_subtree_and_earlier_slot():
         assert prim[0] == ex["tri_list"][ex["leaf_first"][ex["right"][0]]]
 
 
+def test_near_tie_counts():
+    """Near-ties of a closest hit: a ray through the shared edge of two coplanar
+    triangles has one (the loser at the same t), a ray through their shared vertex
+    region off the edge none, a coincident duplicate always one, two parallel planes
+    1e-7 apart one within 1e-6 and none within 1e-8, and a miss none."""
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], np.float32)
+    quad = {"verts": v, "tris": np.array([[0, 1, 2], [1, 3, 2]], np.uint32), "normals": None,
+            "mat": S.DIFFUSE, "params": (0.5, 0.5, 0.5)}
+    dup = dict(quad, tris=np.array([[0, 1, 2], [0, 1, 2]], np.uint32))
+    lifted = dict(quad, verts=v + np.float32([0, 0, 1e-7]), tris=np.array([[0, 1, 2]], np.uint32))
+    rays = np.array([[0.5, 0.5, 1.0, 0.0, 0.0, -1.0],    # on the shared edge
+                     [0.2, 0.2, 1.0, 0.0, 0.0, -1.0],    # inside the first triangle only
+                     [5.0, 5.0, 1.0, 0.0, 0.0, -1.0]],   # misses
+                    np.float32)
+    for objs, want in (([quad], [1, 0, 0]), ([dup], [1, 1, 0])):
+        sc = O.Scene(objs)
+        h = sc.intersect(rays)
+        assert list(sc.near_ties(rays, h["prim"], h["t"])) == want
+    sc = O.Scene([dict(quad, tris=np.array([[0, 1, 2]], np.uint32)), lifted])
+    h = sc.intersect(rays[1:2])
+    assert sc.near_ties(rays[1:2], h["prim"], h["t"], tol=1e-6)[0] == 1
+    assert sc.near_ties(rays[1:2], h["prim"], h["t"], tol=1e-8)[0] == 0
+
+
 def test_heightfield_vertical_rays_closed_form():
     n = 33
     hf = S.heightfield(n, seed=9)
