@@ -15,12 +15,12 @@ the light sub-paths and photon hash grid of BDPT / VCM / PPM on the field, the
 BidirectionalPathTracer on the Cornell box, and the served-jobs workload (p50/p99 latency and aggregate samples/s under
 Poisson arrivals).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pt] [--legs ch,pt,lp,bdpt,serve]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pt] [--legs ch,pt,lp,bdpt,serve,c5]
 
 --legs picks the workloads (closest hit, path tracing, light paths + photon
 grid, BidirectionalPathTracer on the Cornell box, serving, and c5: the
-43.8M-triangle large scene at 3840x2160 x 256 spp, not in the default run); a
-run without the closest-hit leg reports value null (used to profile one leg).
+43.8M-triangle large scene at 3840x2160 x 256 spp); a run without the
+closest-hit leg reports value null (used to profile one leg).
 """
 from __future__ import annotations
 
@@ -55,6 +55,22 @@ def measured_traffic():
     except OSError:
         pass
     return None
+
+
+def measured_copy_gbs(ctx, nbytes=1 << 30, reps=5):
+    """HBM bandwidth a plain device-to-device copy reaches: bytes read plus written over
+    the best of `reps` copies (HIP events on the context stream)."""
+    a, b = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    ctx.copy(b, a, nbytes)
+    ctx.synchronize()
+    best = None
+    for _ in range(reps):
+        ctx.timer_start()
+        ctx.copy(b, a, nbytes)
+        ms = ctx.timer_stop()
+        best = ms if best is None else min(best, ms)
+    del a, b
+    return round(2 * nbytes / (best * 1e-3) / 1e9, 1)
 
 
 def log(msg):
@@ -215,6 +231,38 @@ def cpu_baseline(W, ctx, r, tie_tol=1e-6):
     }
 
 
+def cpu_baseline_pt(objs, cam, spp, depth, seed, passes=None, rebuild=True):
+    """The oracle's PathTracer on the host cores: the scene build (the reference
+    builder for the global and every object tree), the render including the
+    reference's full rebuild before every pass (pathTracer.cpp:18 -> scene.cpp:67-80),
+    and the render alone.  passes: the bounded sample of passes timed (all by default)."""
+    from oracle import oracle as O
+    threads = min(os.cpu_count() or 1, 16)
+    passes = passes or spp
+    t0 = time.perf_counter()
+    osc = O.Scene(objs)
+    build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    osc.render(cam, passes, depth, seed, nthreads=threads)
+    render_s = time.perf_counter() - t0
+    with_rebuild_s = None
+    if rebuild:
+        t0 = time.perf_counter()
+        for k in range(passes):
+            osc.rebuild()
+            osc.render(cam, 1, depth, seed, pass0=k, nthreads=threads)
+        with_rebuild_s = time.perf_counter() - t0
+    samples = cam["width"] * cam["height"] * passes
+    out = {"value": round(samples / render_s, 1), "unit": "samples/s", "cores": threads, "kind": "port",
+           "sample": f"{passes} of {spp} passes of the {cam['width']}x{cam['height']} image, oracle PathTracer "
+                     "restatement, OpenMP over pixels",
+           "build_seconds": round(build_s, 3), "render_seconds": round(render_s, 3)}
+    if with_rebuild_s is not None:
+        out["render_with_per_pass_rebuild_seconds"] = round(with_rebuild_s, 3)
+        out["samples_per_s_with_rebuild"] = round(samples / with_rebuild_s, 1)
+    return out
+
+
 def cpu_baseline_bdpt(S, passes=2, depth=8):
     """The oracle's BidirectionalPathTracer restatement on the host cores, over a bounded
     sample of the BDPT leg's workload (2 passes of the same 256x256 Cornell image)."""
@@ -237,7 +285,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pt", action="store_true", help="same as --legs ch")
-    ap.add_argument("--legs", default="ch,pt,lp,bdpt,serve")
+    ap.add_argument("--legs", default="ch,pt,lp,bdpt,serve,c5")
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (counter calibration)")
@@ -255,6 +303,8 @@ def main():
         ctx.copy(b, a, args.copy_bytes)
         ctx.synchronize()
         del a, b
+    copy_gbs = measured_copy_gbs(ctx)
+    log(f"[bench] 1 GiB device copy: {copy_gbs} GB/s (read + write) against the {HBM_PEAK_GBS:.0f} GB/s nominal peak")
     legs = {"ch"} if args.no_pt else set(args.legs.split(","))
     r = None
     if "ch" in legs:
@@ -264,6 +314,15 @@ def main():
     if "pt" in legs:
         from walkray_amd import render as R
         extra["path_tracing"] = R.bench_path_tracing(W, S, ctx, log)
+        if not args.no_cpu:
+            objs, cam = S.cornell(k=5, width=256, height=256)
+            c1 = extra["path_tracing"]["c1"]["cpu_baseline"] = cpu_baseline_pt(objs, cam, 16, 8, 1234)
+            log(f"[bench] C1 oracle PathTracer: {c1['value'] / 1e6:.2f} Msamples/s on {c1['cores']} cores, build "
+                f"{c1['build_seconds']}s, 16 passes with the per-pass rebuild {c1['render_with_per_pass_rebuild_seconds']}s")
+            objs, cam = S.random_field()
+            c3 = extra["path_tracing"]["c3"]["cpu_baseline"] = cpu_baseline_pt(objs, cam, 64, 16, 77, passes=1)
+            log(f"[bench] C3 oracle PathTracer: {c3['value'] / 1e6:.2f} Msamples/s on {c3['cores']} cores "
+                f"(1 of 64 passes), build {c3['build_seconds']}s")
     if "lp" in legs:
         from walkray_amd import render as R
         extra["light_paths"] = R.bench_light_paths(ctx, log)
@@ -274,7 +333,7 @@ def main():
             extra["bdpt"]["cpu_baseline"] = cpu_baseline_bdpt(S)
             log(f"[bench] BDPT oracle: {extra['bdpt']['cpu_baseline']['value'] / 1e3:.1f} Ksamples/s on "
                 f"{extra['bdpt']['cpu_baseline']['cores']} cores")
-    if "c5" in legs:  # the large scene: minutes of host build and rendering, not in the default run
+    if "c5" in legs:  # the large scene: about a minute of host build and rendering
         from walkray_amd import render as R
         extra["large_scene"] = R.bench_large(W, S, ctx, log)
     if "serve" in legs:
@@ -303,6 +362,7 @@ def main():
     }
     if r:
         out["roofline"] = r["roofline"]
+        out["roofline"]["measured_copy_gbs"] = copy_gbs
         out["closest_hit"] = {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                               for k, v in r["per_set"].items()}
     out.update(extra)

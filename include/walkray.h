@@ -220,6 +220,13 @@ enum { WR_MEGAKERNEL = 0, WR_WAVEFRONT = 1 };
  * scene.cpp:16-18) through MCRenderer::visibilityTest (mcRenderer.cpp:59-65).
  * The light sample's uniforms use vertex counter 0xFFFFFFFF. */
 enum { WR_RENDER_NEE = 1 };
+/* Wavefront launch modes (no counterpart in the reference; results identical
+ * to the default): WR_RENDER_DIRECT launches the iteration kernels one by one
+ * instead of replaying the captured graphs (for tools that cannot follow graph
+ * launches, such as kernel tracers); WR_RENDER_STAGE_STATS does the same with a
+ * HIP event around every kernel and the counting trace kernel, and keeps the
+ * per-stage times and counts for wr_render_stage_stats.  Wavefront mode only. */
+enum { WR_RENDER_DIRECT = 2, WR_RENDER_STAGE_STATS = 4 };
 typedef struct {
     uint32_t spp;        /* passes to add, one sample per pixel each (PathTracer spp) */
     uint32_t max_depth;  /* MCRenderer maxPathLen (mcRenderer.h:16) */
@@ -254,6 +261,22 @@ int wr_film_write(wr_film* film, const float* sum, const float* comp, const floa
  * [passes, passes + spp) of the film, accumulated into it.  stats nullable. */
 int wr_render(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, const wr_render_params* params,
               wr_film* film, wr_render_stats* stats);
+
+/* Per-stage statistics of the context's last wr_render with
+ * WR_RENDER_STAGE_STATS: device milliseconds summed over the iterations for
+ * each of the eight iteration kernels (extend: trace, finish; shade: common
+ * case, refraction exits; plan; generate; fold; advance), the path segments
+ * traced, the refraction exits, the camera paths generated, and the fast-pass
+ * records the trace stage fetched (nodes in 32 B units, 48 B triangles). */
+typedef struct {
+    float ms[8];
+    uint64_t segments;
+    uint64_t exits;
+    uint64_t generated;
+    uint64_t box_records;
+    uint64_t tri_records;
+} wr_stage_stats;
+int wr_render_stage_stats(wr_ctx* ctx, wr_stage_stats* out);
 
 /* BidirectionalPathTracer::renderPixels (bidirPathTracer.cpp:10-51) for
  * passes [passes, passes + spp) of the film: per pixel and pass an eye

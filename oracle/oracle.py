@@ -216,6 +216,10 @@ class Scene:
             lib().orc_render(self.h, C.byref(c), max_depth, seed, pass0, spp, _p(sums[0]), _p(sums[1]), nthreads)
         return sums
 
+    def rebuild(self):
+        """Scene::updateSceneForMotionBlur's full rebuild (scene.cpp:67-80): every tree again."""
+        lib().orc_scene_build(self.h)
+
     def near_ties(self, rays, prim, t, tol=1e-6, nthreads=0):
         """Per ray, the other triangles accepted within tol of its hit distance t."""
         rays = np.ascontiguousarray(rays, dtype=np.float32)

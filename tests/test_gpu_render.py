@@ -160,3 +160,28 @@ def test_field_depth16_plain_matches_oracle(ctx):
     assert z.max() <= 6, f"max z {z.max():.2f}"
     same = np.isclose(g["sum"], s.astype(np.float32), rtol=1e-4, atol=1e-6).mean()
     assert same > 0.5, f"only {same:.3f} of pixel sums agree closely"
+
+
+@pytest.mark.parametrize("nee", [False, True])
+def test_direct_and_stage_stats_launches_equal_graph(ctx, nee):
+    """The wavefront's iteration kernels launched one by one (WR_RENDER_DIRECT) and with
+    per-stage events and the counting trace kernel (WR_RENDER_STAGE_STATS) render the
+    graph replays' film bit for bit; the statistics count every camera path once and
+    at least one traced segment per camera path."""
+    objs, cam = S.cornell(k=3, width=48, height=40)
+    sc = W.Scene(ctx, objs)
+    spp = 6
+    films = [R.render_scene(ctx, objs, cam, spp, 8, 5, W.WR_WAVEFRONT, scene=sc, nee=nee, launch=lm)
+             for lm in (0, W.WR_RENDER_DIRECT, W.WR_RENDER_STAGE_STATS)]
+    mega = R.render_scene(ctx, objs, cam, spp, 8, 5, W.WR_MEGAKERNEL, scene=sc, nee=nee)
+    for f in films:
+        for k in ("sum", "comp", "sumsq"):
+            assert np.array_equal(f[k].view(np.uint32), mega[k].view(np.uint32)), k
+    st = W.render_stage_stats(ctx)
+    npix = cam["width"] * cam["height"]
+    assert st["generated"] == npix * spp
+    assert st["segments"] >= st["generated"] and st["box_records"] > st["segments"] and st["tri_records"] > 0
+    assert st["ms"]["trace"] > 0 and all(v >= 0 for v in st["ms"].values())
+    assert st["exits"] > 0  # the glass sphere's refraction exits
+    with pytest.raises(W.WalkRayError):
+        W.render(sc, cam, W.Film(ctx, 48, 40), 1, 8, 5, W.WR_MEGAKERNEL, launch=W.WR_RENDER_DIRECT)

@@ -167,6 +167,7 @@ def lib():
         "wr_job_read": (i32, [vp, u64, vp, vp, vp, vp, P(u32)]),
         "wr_job_release": (i32, [vp, u64]),
         "wr_job_cancel": (i32, [vp, u64]),
+        "wr_render_stage_stats": (i32, [vp, vp]),
         "wr_server_inject_admit_failures": (i32, [vp, u32]),
         "wr_hit_sources": (i32, [vp, vp, vp, u64, vp, vp]),
         "wr_server_clock": (C.c_double, [vp]),
@@ -437,6 +438,21 @@ class Context:
 
 WR_MEGAKERNEL, WR_WAVEFRONT = 0, 1
 WR_RENDER_NEE = 1
+WR_RENDER_DIRECT, WR_RENDER_STAGE_STATS = 2, 4
+STAGES = ("trace", "finish", "shade", "shade_exits", "plan", "generate", "fold", "advance")
+
+
+class StageStatsC(C.Structure):
+    _fields_ = [("ms", C.c_float * 8), ("segments", C.c_uint64), ("exits", C.c_uint64), ("generated", C.c_uint64),
+                ("box_records", C.c_uint64), ("tri_records", C.c_uint64)]
+
+
+def render_stage_stats(ctx):
+    """wr_render_stage_stats: per-stage times and counts of the last WR_RENDER_STAGE_STATS render."""
+    s = StageStatsC()
+    _check(lib().wr_render_stage_stats(ctx.h, C.byref(s)), ctx.h, "wr_render_stage_stats")
+    return {"ms": dict(zip(STAGES, (float(x) for x in s.ms))), "segments": s.segments, "exits": s.exits,
+            "generated": s.generated, "box_records": s.box_records, "tri_records": s.tri_records}
 
 
 class Film:
@@ -486,11 +502,13 @@ def motion_interpolate(t0, t1, start, end, time):
     return out.reshape(4, 4)
 
 
-def render(scene, cam, film, spp, max_depth, seed, mode=WR_WAVEFRONT, wave_slots=0, wave_grid=0, nee=False):
+def render(scene, cam, film, spp, max_depth, seed, mode=WR_WAVEFRONT, wave_slots=0, wave_grid=0, nee=False,
+           launch=0):
     """PathTracer::renderPixels for `spp` more passes into `film`; returns stats.
-    nee: the useNextEventEstimation branch (pathTracer.cpp:39-66)."""
+    nee: the useNextEventEstimation branch (pathTracer.cpp:39-66).  launch: 0 (graph
+    replays), WR_RENDER_DIRECT or WR_RENDER_STAGE_STATS (wavefront only)."""
     cd = CameraDesc.from_dict(cam)
-    prm = RenderParams(spp, max_depth, seed, mode, wave_slots, wave_grid, WR_RENDER_NEE if nee else 0)
+    prm = RenderParams(spp, max_depth, seed, mode, wave_slots, wave_grid, (WR_RENDER_NEE if nee else 0) | launch)
     st = RenderStats()
     _check(lib().wr_render(scene.ctx.h, scene.h, C.byref(cd), C.byref(prm), film.h, C.byref(st)), scene.ctx.h,
            "wr_render")

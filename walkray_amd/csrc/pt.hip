@@ -159,7 +159,9 @@ constexpr uint32_t kTraceRefillMin = 16;
 constexpr uint32_t kTraceNodePhaseMin = 32;
 
 // D: traversal stack levels in LDS (4, 8 or 16; the wavefront autotuner's stack-depth knob).
-template <int D>
+// COUNT: add the fast pass's node and triangle records fetched to the engine's
+// totals (the per-stage statistics launches only).
+template <int D, bool COUNT = false>
 __global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
     const PathBuf& A = E.buf[par];
     const uint32_t n = E.cnt->n[par];
@@ -222,7 +224,7 @@ __global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
         for (;;) {
             const bool at_node = active && !(ref & WR_LEAF_BIT);
             if ((uint32_t)__popcll(__ballot(at_node)) < kTraceNodePhaseMin) break;
-            if (at_node && !wide_step<true, false>(tr, r, ns, st, h, ref, overflow, cnt)) {
+            if (at_node && !wide_step<true, COUNT>(tr, r, ns, st, h, ref, overflow, cnt)) {
                 active = false;
                 A.ht[pid] = h.t;
                 A.hrec[pid] = h.rec;
@@ -230,12 +232,24 @@ __global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
                 if (overflow) A.flags[pid] |= kOverflow;
             }
         }
-        if (active && !wide_step<true, false>(tr, r, ns, st, h, ref, overflow, cnt)) {
+        if (active && !wide_step<true, COUNT>(tr, r, ns, st, h, ref, overflow, cnt)) {
             active = false;
             A.ht[pid] = h.t;
             A.hrec[pid] = h.rec;
             A.hobj[pid] = h.first;
             if (overflow) A.flags[pid] |= kOverflow;
+        }
+    }
+    if (COUNT) {
+        unsigned long long a = cnt.nodes, b = cnt.tris;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off);
+            b += __shfl_xor(b, off);
+        }
+        if (lane == 0) {
+            atomicAdd(&E.cnt->trace_nodes, a);
+            atomicAdd(&E.cnt->trace_tris, b);
         }
     }
     if (lane == 0) {
@@ -457,6 +471,9 @@ __global__ __launch_bounds__(64) void k_wf_plan(EngineDev E, uint32_t par) {
     const uint64_t total_take = __shfl(incl, 63);
     if (lane == 0) {
         const uint32_t all = (uint32_t)(total_take < room ? total_take : room);
+        c.traced += c.n[par];
+        c.exits += c.n_exit;
+        c.generated += all;
         c.nseg = (uint32_t)__popcll(m);
         c.seg_start = base0;
         c.seg_end = base0 + all;
@@ -643,7 +660,6 @@ hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st, u
         bufs.push_back(p);
         return p;
     };
-    void* words[2][kPathWords];
     for (int b = 0; b < 2; b++)
         for (int k = 0; k < kPathWords; k++)
             if (!(words[b][k] = alloc(4 * (size_t)cap))) return hipErrorOutOfMemory;
@@ -657,15 +673,6 @@ hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st, u
     WR_CHECK(hipMemsetAsync(ctl, 0, sizeof(JobCtl) * kMaxJobs, st));
     WR_CHECK(hipMemsetAsync(scenes, 0, sizeof(SceneDev) * kMaxScenes, st));
     WR_CHECK(hipStreamSynchronize(st));
-    EngineDev E;
-    E.buf[0] = to_pathbuf(words[0]);
-    E.buf[1] = to_pathbuf(words[1]);
-    E.cnt = cnt;
-    E.jobs = jobs;
-    E.ctl = ctl;
-    E.scenes = scenes;
-    E.exit_list = exit_list;
-    E.cap = cap;
     // the persistent trace kernel runs as many workgroups as fit at once
     int dev = 0, cus = 0, per_cu = 0;
     WR_CHECK(hipGetDevice(&dev));
@@ -676,33 +683,75 @@ hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st, u
     const size_t trace_lds = lds_levels == 4 ? stack_lds_bytes<4>() : lds_levels == 16 ? stack_lds_bytes<16>()
                                                                                        : stack_lds_bytes<8>();
     WR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_fn, 256, trace_lds));
-    const uint32_t trace_grid = (uint32_t)std::max(1, std::min<int>(cus * std::max(per_cu, 1), (int)(cap / 256)));
+    trace_grid = (uint32_t)std::max(1, std::min<int>(cus * std::max(per_cu, 1), (int)(cap / 256)));
     // the iteration graphs: kGraphIters and kTailIters iterations (even, the
-    // buffer parity alternates)
+    // buffer parity alternates), captured from the direct launch sequence
     auto capture = [&](int iters, hipGraph_t* g, hipGraphExec_t* x) -> hipError_t {
         WR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        for (int it = 0; it < iters; it++) {
-            const uint32_t par = (uint32_t)(it & 1);
-            if (lds_levels == 4)
-                k_wf_trace<4><<<trace_grid, 256, trace_lds, st>>>(E, par);
-            else if (lds_levels == 16)
-                k_wf_trace<16><<<trace_grid, 256, trace_lds, st>>>(E, par);
-            else
-                k_wf_trace<8><<<trace_grid, 256, trace_lds, st>>>(E, par);
-            k_wf_finish<<<grid, 256, kStackLdsBytes, st>>>(E, par);
-            k_wf_shade<false><<<grid, 256, kStackLdsBytes, st>>>(E, par);
-            k_wf_shade<true><<<grid, 256, kStackLdsBytes, st>>>(E, par);
-            k_wf_plan<<<1, 64, 0, st>>>(E, par);
-            k_wf_generate<<<grid, 256, 0, st>>>(E, par);
-            k_wf_fold<<<grid, 256, 0, st>>>(E);
-            k_wf_advance<<<1, 64, 0, st>>>(E, par);
+        const hipError_t e = enqueue(iters, st, nullptr);
+        hipGraph_t cg = nullptr;
+        const hipError_t ec = hipStreamEndCapture(st, &cg);
+        if (e != hipSuccess || ec != hipSuccess) {
+            if (cg) (void)hipGraphDestroy(cg);
+            return e != hipSuccess ? e : ec;
         }
-        WR_CHECK(hipStreamEndCapture(st, g));
+        *g = cg;
         return hipGraphInstantiate(x, *g, nullptr, nullptr, 0);
     };
     static_assert(kGraphIters % 2 == 0 && kTailIters % 2 == 0, "replays end on the parity they start on");
     WR_CHECK(capture(kGraphIters, &graph, &exec));
     return capture(kTailIters, &tail_graph, &tail_exec);
+}
+
+static EngineDev engine_dev(const Engine& En) {
+    EngineDev E;
+    E.buf[0] = to_pathbuf(En.words[0]);
+    E.buf[1] = to_pathbuf(En.words[1]);
+    E.cnt = En.cnt;
+    E.jobs = En.jobs;
+    E.ctl = En.ctl;
+    E.scenes = En.scenes;
+    E.exit_list = En.exit_list;
+    E.cap = En.cap;
+    return E;
+}
+
+template <bool COUNT>
+static void launch_trace(const EngineDev& E, uint32_t lds_levels, uint32_t trace_grid, uint32_t par, hipStream_t st) {
+    if (lds_levels == 4)
+        k_wf_trace<4, COUNT><<<trace_grid, 256, stack_lds_bytes<4>(), st>>>(E, par);
+    else if (lds_levels == 16)
+        k_wf_trace<16, COUNT><<<trace_grid, 256, stack_lds_bytes<16>(), st>>>(E, par);
+    else
+        k_wf_trace<8, COUNT><<<trace_grid, 256, stack_lds_bytes<8>(), st>>>(E, par);
+}
+
+hipError_t Engine::enqueue(int iters, hipStream_t st, hipEvent_t* ev) {
+    const EngineDev E = engine_dev(*this);
+    int k = 0;
+    auto mark = [&]() -> hipError_t { return ev ? hipEventRecord(ev[k++], st) : hipSuccess; };
+    WR_CHECK(mark());
+    for (int it = 0; it < iters; it++) {
+        const uint32_t par = (uint32_t)(it & 1);
+        if (ev) launch_trace<true>(E, lds_levels, trace_grid, par, st);
+        else launch_trace<false>(E, lds_levels, trace_grid, par, st);
+        WR_CHECK(mark());
+        k_wf_finish<<<grid, 256, kStackLdsBytes, st>>>(E, par);
+        WR_CHECK(mark());
+        k_wf_shade<false><<<grid, 256, kStackLdsBytes, st>>>(E, par);
+        WR_CHECK(mark());
+        k_wf_shade<true><<<grid, 256, kStackLdsBytes, st>>>(E, par);
+        WR_CHECK(mark());
+        k_wf_plan<<<1, 64, 0, st>>>(E, par);
+        WR_CHECK(mark());
+        k_wf_generate<<<grid, 256, 0, st>>>(E, par);
+        WR_CHECK(mark());
+        k_wf_fold<<<grid, 256, 0, st>>>(E);
+        WR_CHECK(mark());
+        k_wf_advance<<<1, 64, 0, st>>>(E, par);
+        WR_CHECK(mark());
+    }
+    return hipGetLastError();
 }
 
 hipError_t Engine::set_scene(uint32_t slot, const SceneDev& S, hipStream_t st) {
@@ -732,8 +781,9 @@ hipError_t Engine::release_slot(uint32_t slot, hipStream_t st) {
 }
 
 // One replay of the iteration graph, then a read-back of the job states.
-hipError_t Engine::step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt, bool tail) {
-    WR_CHECK(hipGraphLaunch(tail ? tail_exec : exec, st));
+hipError_t Engine::step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt, bool tail, bool direct) {
+    if (direct) WR_CHECK(enqueue(tail ? kTailIters : kGraphIters, st, nullptr));
+    else WR_CHECK(hipGraphLaunch(tail ? tail_exec : exec, st));
     if (host_ctl) WR_CHECK(hipMemcpyAsync(host_ctl, ctl, sizeof(JobCtl) * kMaxJobs, hipMemcpyDeviceToHost, st));
     if (host_cnt) WR_CHECK(hipMemcpyAsync(host_cnt, cnt, sizeof(EngineCounters), hipMemcpyDeviceToHost, st));
     return hipStreamSynchronize(st);
@@ -755,16 +805,27 @@ void fill_job_camera(JobDesc& J, const CamHost& c) {
 static hipError_t run_wavefront_once(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed,
                                      uint32_t pass0, uint32_t npass, bool nee, const FilmBufs& film, Engine& E,
                                      uint32_t cap, uint32_t grid, WaveStats* stats, hipStream_t st,
-                                     uint32_t lds_levels);
+                                     uint32_t lds_levels, int launch, StageStats* stage, hipEvent_t* ev);
 
 // Renders passes [pass0, pass0 + npass) of one film on the engine, alone.  On
 // any error the engine is released, so the next call starts from a fresh pool
 // rather than from the live paths and counters of the failed one.
 hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
                          uint32_t npass, bool nee, const FilmBufs& film, Engine& E, uint32_t cap, uint32_t grid,
-                         WaveStats* stats, hipStream_t st, uint32_t lds_levels) {
-    const hipError_t e =
-        run_wavefront_once(S, cam, max_depth, seed, pass0, npass, nee, film, E, cap, grid, stats, st, lds_levels);
+                         WaveStats* stats, hipStream_t st, uint32_t lds_levels, int launch, StageStats* stage) {
+    // per-stage statistics: one event before the first kernel of a replay and one after each kernel
+    std::vector<hipEvent_t> ev;
+    hipError_t e = hipSuccess;
+    if (launch == kLaunchStageStats) {
+        ev.assign(kStages * kGraphIters + 1, nullptr);
+        for (hipEvent_t& x : ev)
+            if (e == hipSuccess) e = hipEventCreate(&x);
+    }
+    if (e == hipSuccess)
+        e = run_wavefront_once(S, cam, max_depth, seed, pass0, npass, nee, film, E, cap, grid, stats, st, lds_levels,
+                               launch, stage, ev.empty() ? nullptr : ev.data());
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
     if (e != hipSuccess) {
         (void)hipStreamSynchronize(st);
         E.release();
@@ -775,7 +836,7 @@ hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_dep
 static hipError_t run_wavefront_once(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed,
                                      uint32_t pass0, uint32_t npass, bool nee, const FilmBufs& film, Engine& E,
                                      uint32_t cap, uint32_t grid, WaveStats* stats, hipStream_t st,
-                                     uint32_t lds_levels) {
+                                     uint32_t lds_levels, int launch, StageStats* stage, hipEvent_t* ev) {
     const uint32_t npix = cam.W * cam.H;
     if (npass == 0 || npix == 0) return hipSuccess;
     const uint64_t total = (uint64_t)npass * npix;
@@ -813,6 +874,8 @@ static hipError_t run_wavefront_once(const SceneDev& S, const CamHost& cam, uint
     J.fsum = film.sum;
     J.fcomp = film.comp;
     J.fsumsq = film.sumsq;
+    EngineCounters c0{};
+    if (ev) WR_CHECK(hipMemcpy(&c0, E.cnt, sizeof(EngineCounters), hipMemcpyDeviceToHost));
     k_wf_admit<<<1, 256, 0, st>>>(E.scenes, E.jobs, E.ctl, E.cnt, S, J);
     WR_CHECK(hipGetLastError());
     // every pass is generated within npass window turns and drains in at most
@@ -820,21 +883,34 @@ static hipError_t run_wavefront_once(const SceneDev& S, const CamHost& cam, uint
     const uint64_t it_bound = 4 * (uint64_t)(max_depth + 4) * (total / E.cap + npass + 2) + 64;
     // every camera path fits the pool and the window: all are issued in the
     // first iteration, so the first replay is followed by a tail replay at once
-    bool chain_tail = total <= E.cap && npass <= J.R;
+    // (with per-stage events every replay is read back on its own)
+    bool chain_tail = total <= E.cap && npass <= J.R && !ev;
     uint64_t iters = 0;
     bool tail = false;
     JobCtl& back = *E.ctl_back;
     hipError_t e = hipSuccess;
+    auto replay = [&](bool short_one) -> hipError_t {
+        const int n = short_one ? kTailIters : kGraphIters;
+        iters += n;
+        if (launch == kLaunchGraph) return hipGraphLaunch(short_one ? E.tail_exec : E.exec, st);
+        return E.enqueue(n, st, ev);
+    };
     for (;;) {
-        e = hipGraphLaunch(tail ? E.tail_exec : E.exec, st);
-        iters += tail ? kTailIters : kGraphIters;
+        e = replay(tail);
         if (e == hipSuccess && chain_tail) {
-            e = hipGraphLaunch(E.tail_exec, st);
-            iters += kTailIters;
+            e = replay(true);
             chain_tail = false;
         }
         if (e == hipSuccess) e = hipMemcpyAsync(&back, E.ctl, sizeof(JobCtl), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e == hipSuccess && ev && stage) {
+            const int n = tail ? kTailIters : kGraphIters;
+            for (int k = 0; k < kStages * n && e == hipSuccess; k++) {
+                float ms = 0.f;
+                e = hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+                stage->ms[k % kStages] += ms;
+            }
+        }
         if (e != hipSuccess || back.state == kJobDone) break;
         tail = back.next_work >= total;  // all camera paths issued: only draining is left
         if (iters > it_bound) {  // cannot happen
@@ -844,6 +920,15 @@ static hipError_t run_wavefront_once(const SceneDev& S, const CamHost& cam, uint
     }
     // job slot 0 stays done until the next call's admission overwrites it
     if (e != hipSuccess) return e;
+    if (ev && stage) {
+        EngineCounters c1{};
+        WR_CHECK(hipMemcpy(&c1, E.cnt, sizeof(EngineCounters), hipMemcpyDeviceToHost));
+        stage->traced += c1.traced - c0.traced;
+        stage->exits += c1.exits - c0.exits;
+        stage->generated += c1.generated - c0.generated;
+        stage->nodes += c1.trace_nodes - c0.trace_nodes;
+        stage->tris += c1.trace_tris - c0.trace_tris;
+    }
     if (stats) {
         stats->iterations = back.done_iter + 1 - back.start_iter;
         stats->paths = total;

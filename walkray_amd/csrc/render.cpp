@@ -159,8 +159,15 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
     if (film->W != cam->width || film->H != cam->height)
         return fail(ctx, WR_EINVAL, "wr_render: film size differs from the camera resolution");
     if (prm->mode != WR_MEGAKERNEL && prm->mode != WR_WAVEFRONT) return fail(ctx, WR_EINVAL, "wr_render: unknown mode");
-    if (prm->flags & ~(uint32_t)WR_RENDER_NEE) return fail(ctx, WR_EINVAL, "wr_render: unknown flags");
+    if (prm->flags & ~(uint32_t)(WR_RENDER_NEE | WR_RENDER_DIRECT | WR_RENDER_STAGE_STATS))
+        return fail(ctx, WR_EINVAL, "wr_render: unknown flags");
     const bool nee = (prm->flags & WR_RENDER_NEE) != 0;
+    const int launch = (prm->flags & WR_RENDER_STAGE_STATS) ? kLaunchStageStats
+                       : (prm->flags & WR_RENDER_DIRECT)   ? kLaunchDirect
+                                                           : kLaunchGraph;
+    if (launch != kLaunchGraph && prm->mode != WR_WAVEFRONT)
+        return fail(ctx, WR_EINVAL, "wr_render: launch-mode flags are for the wavefront engine");
+    StageStats stage;
     if ((uint64_t)cam->width * cam->height >= (1ull << 31)) return fail(ctx, WR_EINVAL, "wr_render: image too large");
     if (film->broken) return film_broken(film, "wr_render");
     (void)hipSetDevice(ctx->device);
@@ -195,7 +202,7 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
                 e = run_wavefront(tmp.dev, ch, prm->max_depth, prm->seed, pass, 1, nee, fb, ctx->engine,
                                   prm->wave_slots ? prm->wave_slots : ctx->tuned_slots,
                                   prm->wave_grid ? prm->wave_grid : ctx->tuned_grid, &ws, ctx->stream,
-                                  ctx->tuned_lds);
+                                  ctx->tuned_lds, launch, &stage);
             if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
             scene_free_device(&tmp);
         }
@@ -207,7 +214,7 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
             e = run_wavefront(s->dev, ch, prm->max_depth, prm->seed, film->passes, prm->spp, nee, fb, ctx->engine,
                               prm->wave_slots ? prm->wave_slots : ctx->tuned_slots,
                               prm->wave_grid ? prm->wave_grid : ctx->tuned_grid, &ws, ctx->stream,
-                              ctx->tuned_lds);
+                              ctx->tuned_lds, launch, &stage);
         }
     }
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
@@ -219,11 +226,24 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
         return hip_fail(ctx, e, "wr_render");
     }
     film->passes += prm->spp;
+    if (launch == kLaunchStageStats) ctx->last_stage = stage;
     if (stats) {
         stats->paths = (uint64_t)prm->spp * cam->width * cam->height;
         stats->iterations = ws.iterations;
         stats->kernel_ms = ms;
     }
+    return WR_OK;
+}
+
+int wr_render_stage_stats(wr_ctx* ctx, wr_stage_stats* out) {
+    if (!ctx || !out) return fail(ctx, WR_EINVAL, "wr_render_stage_stats: bad argument");
+    const StageStats& s = ctx->last_stage;
+    for (int k = 0; k < kStages; k++) out->ms[k] = (float)s.ms[k];
+    out->segments = s.traced;
+    out->exits = s.exits;
+    out->generated = s.generated;
+    out->box_records = s.nodes;
+    out->tri_records = s.tris;
     return WR_OK;
 }
 

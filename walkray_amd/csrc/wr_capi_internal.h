@@ -20,6 +20,7 @@ struct wr_ctx {
     wr::Engine engine;  // wavefront engine of wr_render, reused across calls
     uint32_t tuned_slots = 0, tuned_grid = 0;  // wr_autotune_render's choice (0 = defaults)
     uint32_t tuned_lds = 8;                    // wr_autotune_render's traversal stack levels in LDS
+    wr::StageStats last_stage;                 // the last WR_RENDER_STAGE_STATS render's per-stage figures
 };
 
 namespace wr {

@@ -74,6 +74,25 @@ struct EngineCounters {
     uint32_t nfold;                    // jobs with complete passes to fold this iteration
     uint32_t fold_job[kMaxJobs];       // those jobs, in job order
     uint32_t fold_k[kMaxJobs];         // per job: complete passes from its fold cursor
+    // running totals since the engine was set up (k_wf_plan adds each iteration's)
+    unsigned long long traced;     // path segments traced (live paths of every iteration)
+    unsigned long long exits;      // of those, refraction exits shaded by the findInsideObject instance
+    unsigned long long generated;  // camera paths generated
+    unsigned long long trace_nodes, trace_tris;  // fast-pass records fetched by the counting trace kernel
+};
+
+// The eight kernels of one wavefront iteration, in launch order.
+enum : int { kStTrace = 0, kStFinish, kStShade, kStShadeExits, kStPlan, kStGenerate, kStFold, kStAdvance, kStages };
+
+// How run_wavefront issues its iterations: replays of the captured graphs, the
+// same kernels launched one by one (identical results; for tools that cannot
+// follow graph launches), or launched one by one with a HIP event around every
+// kernel and the counting trace kernel (per-stage times and traffic counts).
+enum : int { kLaunchGraph = 0, kLaunchDirect = 1, kLaunchStageStats = 2 };
+
+struct StageStats {
+    double ms[kStages] = {};
+    uint64_t traced = 0, exits = 0, generated = 0, nodes = 0, tris = 0;
 };
 
 struct WaveStats {
@@ -100,6 +119,13 @@ struct Engine {
     size_t contrib_floats = 0, pass_slots = 0;
     // pinned host buffer of run_wavefront's job-state read-back
     JobCtl* ctl_back = nullptr;
+    // what the iteration kernels are launched with, kept for direct launches
+    void* words[2][kPathWords] = {};
+    uint32_t trace_grid = 0;
+    // Launches `iters` iterations one kernel at a time (the graphs' sequence).
+    // With ev (kStages * iters + 1 events) an event is recorded before the first
+    // kernel and after every kernel, and the trace stage counts its fetches.
+    hipError_t enqueue(int iters, hipStream_t st, hipEvent_t* ev);
     ~Engine();
     void release();
     hipError_t init(uint32_t cap, uint32_t grid, hipStream_t st, uint32_t lds_levels = 8);
@@ -109,7 +135,8 @@ struct Engine {
     hipError_t cancel_slot(uint32_t slot, hipStream_t st);
     // One replay: kGraphIters iterations, or kTailIters when `tail` (every
     // running job has issued all its camera paths, so only draining is left).
-    hipError_t step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt, bool tail = false);
+    hipError_t step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt, bool tail = false,
+                    bool direct = false);
 };
 
 void fill_job_camera(JobDesc& J, const CamHost& c);
@@ -138,6 +165,7 @@ hipError_t launch_bdpt_passes(const SceneDev& S, const wr_tree_hdr& H, const Cam
 
 hipError_t run_wavefront(const SceneDev& S, const CamHost& cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
                          uint32_t npass, bool nee, const FilmBufs& film, Engine& engine, uint32_t cap, uint32_t grid,
-                         WaveStats* stats, hipStream_t st, uint32_t lds_levels = 8);
+                         WaveStats* stats, hipStream_t st, uint32_t lds_levels = 8, int launch = kLaunchGraph,
+                         StageStats* stage = nullptr);
 
 }  // namespace wr

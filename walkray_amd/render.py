@@ -37,10 +37,10 @@ def expected_rel_rmse(state):
     return float(np.sqrt((se ** 2).sum()) / max(np.linalg.norm(mean), 1e-300))
 
 
-def render_scene(ctx, objs, cam, spp, max_depth, seed, mode, scene=None, wave_slots=0, nee=False):
+def render_scene(ctx, objs, cam, spp, max_depth, seed, mode, scene=None, wave_slots=0, nee=False, launch=0):
     sc = scene or W.Scene(ctx, objs)
     film = W.Film(ctx, cam["width"], cam["height"])
-    st = W.render(sc, cam, film, spp, max_depth, seed, mode, wave_slots=wave_slots, nee=nee)
+    st = W.render(sc, cam, film, spp, max_depth, seed, mode, wave_slots=wave_slots, nee=nee, launch=launch)
     out = film.read()
     out["stats"] = st
     return out
@@ -78,6 +78,56 @@ def _at_fixed_rmse(r, spp, cam, target=0.01):
         f"{m}_s": round(samples / r[m]["samples_per_s"], 2) for m in ("megakernel", "wavefront")}}
 
 
+# Path-state bytes per path segment that each wavefront stage reads or writes
+# (32-bit SoA words of the path buffers, plus the hit's 48 B triangle record in
+# shade); the trace stage adds the fast pass's node (32 B units) and triangle
+# (48 B) records counted by its counting instance.
+STAGE_BYTES = {
+    "trace": 40 + 16,          # o, d, flags, job, inside, contact read; ht, hrec, hobj, flags written
+    "finish": 60 + 8,          # o, d, flags, job, ht, hrec, hobj, inside, contact, pixel, pass read; flags, inside written
+    "shade_read": 76 + 48,     # 19 words of state and the hit's triangle record
+    "shade_write": 52,         # 13 words of a surviving path's next state
+    "sample": 12,              # a finished path's sample
+    "generate": 52,            # a camera path's 13 words
+    "fold": 12 + 72,           # a sample read, the Kahan sum, compensation and square sum read and written
+}
+
+
+def stage_roofline(W_, ctx, sc, cam, spp, depth, seed, log, label, nee=False):
+    """One render with WR_RENDER_STAGE_STATS: per iteration kernel, its device time summed
+    over the render, the algorithmic bytes it moves (STAGE_BYTES per segment, traversal
+    records counted by the trace stage) and the resulting fraction of the HBM peak."""
+    film = W.Film(ctx, cam["width"], cam["height"])
+    W.render(sc, cam, film, spp, depth, seed, W.WR_WAVEFRONT, nee=nee, launch=W.WR_RENDER_STAGE_STATS)
+    st = W.render_stage_stats(ctx)
+    T, X, G = st["segments"], st["exits"], st["generated"]
+    nodes, tris = st["box_records"], st["tri_records"]
+    b = {
+        "trace": T * STAGE_BYTES["trace"] + 32 * nodes + 48 * tris,
+        "finish": T * STAGE_BYTES["finish"] + G * STAGE_BYTES["sample"],
+        "shade": (T - X) * STAGE_BYTES["shade_read"] + (T - G) * STAGE_BYTES["shade_write"] + G * STAGE_BYTES["sample"],
+        "shade_exits": X * (STAGE_BYTES["shade_read"] + STAGE_BYTES["shade_write"]),
+        "generate": G * STAGE_BYTES["generate"],
+        "fold": G * STAGE_BYTES["fold"],
+    }
+    stages = {}
+    for k, ms in st["ms"].items():
+        e = {"ms": round(ms, 3)}
+        if k in b and ms > 0:
+            e["alg_bytes"] = int(b[k])
+            e["achieved_gbs"] = round(b[k] / (ms * 1e-3) / 1e9, 1)
+            e["frac"] = round(b[k] / (ms * 1e-3) / 1e9 / 8000.0, 4)
+        stages[k] = e
+    total = sum(st["ms"].values())
+    out = {"stages": stages, "total_ms": round(total, 3), "segments": T, "exits": X, "camera_paths": G,
+           "box_records_per_segment": round(nodes / max(T, 1), 2), "tri_records_per_segment": round(tris / max(T, 1), 2),
+           "launch": "kernels launched one by one with a HIP event around each (the graph replays' sequence)",
+           "bytes_per_segment": STAGE_BYTES}
+    log(f"[bench] {label} stages (ms): " + ", ".join(f"{k} {v['ms']}" for k, v in stages.items()) +
+        f"; trace {stages['trace'].get('achieved_gbs')} GB/s")
+    return out
+
+
 def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
     out = {}
     objs, cam = S.cornell(k=5, width=256, height=256)
@@ -94,6 +144,18 @@ def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
         r["reference"] = f"{int(ref['spp'])}-spp render, FP64 accumulation of per-batch Kahan films"
         _at_fixed_rmse(r, 16, cam)
     r["next_event"], _ = _compare(ctx, sc, cam, 16, 8, 1234, log, "C1 cornell +NEE", nee=True)
+    r["stage_roofline"] = stage_roofline(W, ctx, sc, cam, 16, 8, 1234, log, "C1")
+    # steady state: enough passes that the pool stays full and the drain tail is a small part
+    ss = {}
+    for name, mode in (("megakernel", W.WR_MEGAKERNEL), ("wavefront", W.WR_WAVEFRONT)):
+        film = W.Film(ctx, cam["width"], cam["height"])
+        st = W.render(sc, cam, film, 1024, 8, 1234, mode)
+        ss[name] = {"ms": round(st["kernel_ms"], 2),
+                    "samples_per_s": round(cam["width"] * cam["height"] * 1024 / (st["kernel_ms"] * 1e-3), 1)}
+    ss["speedup_wavefront_vs_megakernel"] = round(ss["wavefront"]["samples_per_s"] / ss["megakernel"]["samples_per_s"], 3)
+    r["steady_state_1024spp"] = ss
+    log(f"[bench] C1 1024 spp: megakernel {ss['megakernel']['samples_per_s'] / 1e6:.1f}, wavefront "
+        f"{ss['wavefront']['samples_per_s'] / 1e6:.1f} Msamples/s ({ss['speedup_wavefront_vs_megakernel']}x)")
     out["c1"] = r
     objs, cam = S.random_field()
     sc = W.Scene(ctx, objs)
@@ -113,6 +175,7 @@ def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
                           f"compared on every {st}th pixel in x and y ({img.shape[1]} x {img.shape[0]})")
         _at_fixed_rmse(r, c3_spp, cam)
     r["next_event"], _ = _compare(ctx, sc, cam, c3_spp, 16, 77, log, "C3 field +NEE", repeats=1, nee=True)
+    r["stage_roofline"] = stage_roofline(W, ctx, sc, cam, c3_spp, 16, 77, log, "C3")
     out["c3"] = r
     return out
 

@@ -100,6 +100,26 @@ def summarize(records):
             "samples_per_s": round(samples / span, 1), "span_s": round(span, 3)}
 
 
+def bench_serving_sweep(ctx, log, srv, ids, cams, rates=(8.0, 16.0, 32.0, 64.0), n_jobs=40):
+    """Capacity: the same 40-job stream offered at rising Poisson rates, then all at
+    once (a burst, every job queued at t = 0).  Latency grows without bound once the
+    offered samples/s exceed what the engine sustains; the burst's aggregate
+    samples/s is that sustainable rate."""
+    out = []
+    base = make_jobs(n_jobs, len(ids), 1.0, seed=9)
+    for rate in list(rates) + [None]:
+        jobs = [dict(j, arrival=(j["arrival"] / rate if rate else 0.0)) for j in base]
+        recs = run_poisson(srv, ids, cams, jobs)
+        res = summarize(recs)
+        offered = sum(r["samples"] for r in recs) / max(jobs[-1]["arrival"], 1e-9) if rate else None
+        res.update(rate_jobs_per_s=rate if rate else "burst",
+                   offered_samples_per_s=round(offered, 1) if offered else None)
+        out.append(res)
+        log(f"[bench] serving sweep {res['rate_jobs_per_s']}: p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms, "
+            f"{res['samples_per_s'] / 1e6:.1f} Msamples/s")
+    return out
+
+
 def bench_serving(ctx, log, rate=8.0, n_jobs=80, budget_gib=16):
     t0 = time.time()
     scs = serving_scenes()
@@ -116,5 +136,7 @@ def bench_serving(ctx, log, rate=8.0, n_jobs=80, budget_gib=16):
                         "spp log-uniform 4-256, depth 8-16")
     log(f"[bench] serving: p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms, {res['samples_per_s'] / 1e6:.1f} "
         f"Msamples/s over {res['span_s']} s")
+    res["sweep"] = bench_serving_sweep(ctx, log, srv, ids, [c for _, c in scs])
+    res["sustainable_samples_per_s"] = res["sweep"][-1]["samples_per_s"]
     srv.close()
     return res
