@@ -19,8 +19,10 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pt] 
 
 --legs picks the workloads (closest hit, path tracing, light paths + photon
 grid, BidirectionalPathTracer on the Cornell box, serving, and c5: the
-43.8M-triangle large scene at 3840x2160 x 256 spp); a run without the
-closest-hit leg reports value null (used to profile one leg).
+43.8M-triangle large scene at 3840x2160 x 256 spp; ptd, not in the default
+run: C1 and C3 with the wavefront's kernels launched one by one and no graph
+created, for kernel tracers); a run without the closest-hit leg reports value
+null (used to profile one leg).
 """
 from __future__ import annotations
 
@@ -323,6 +325,9 @@ def main():
             c3 = extra["path_tracing"]["c3"]["cpu_baseline"] = cpu_baseline_pt(objs, cam, 64, 16, 77, passes=1)
             log(f"[bench] C3 oracle PathTracer: {c3['value'] / 1e6:.2f} Msamples/s on {c3['cores']} cores "
                 f"(1 of 64 passes), build {c3['build_seconds']}s")
+    if "ptd" in legs:  # for kernel tracers: no graph is created or launched
+        from walkray_amd import render as R
+        extra["path_tracing_direct"] = R.profile_path_tracing(ctx, log)
     if "lp" in legs:
         from walkray_amd import render as R
         extra["light_paths"] = R.bench_light_paths(ctx, log)

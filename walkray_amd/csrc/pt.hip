@@ -620,6 +620,7 @@ void Engine::release() {
     if (tail_graph) (void)hipGraphDestroy(tail_graph);
     exec = tail_exec = nullptr;
     graph = tail_graph = nullptr;
+    ready = false;
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     bufs.clear();
@@ -649,7 +650,7 @@ hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st, u
     if (want_grid == 0) want_grid = 2048;
     if (want_lds != 4 && want_lds != 16) want_lds = 8;
     want_cap = std::max<uint32_t>(256, (want_cap + 255) / 256 * 256);
-    if (cap == want_cap && grid == want_grid && lds_levels == want_lds && exec) return hipSuccess;
+    if (cap == want_cap && grid == want_grid && lds_levels == want_lds && ready) return hipSuccess;
     release();
     cap = want_cap;
     grid = want_grid;
@@ -684,8 +685,15 @@ hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st, u
                                                                                        : stack_lds_bytes<8>();
     WR_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_fn, 256, trace_lds));
     trace_grid = (uint32_t)std::max(1, std::min<int>(cus * std::max(per_cu, 1), (int)(cap / 256)));
-    // the iteration graphs: kGraphIters and kTailIters iterations (even, the
-    // buffer parity alternates), captured from the direct launch sequence
+    ready = true;
+    return hipSuccess;
+}
+
+// The iteration graphs (kGraphIters and kTailIters iterations: even, the buffer
+// parity alternates), captured from the direct launch sequence the first time a
+// replay needs them, so direct launches never create a graph.
+hipError_t Engine::ensure_graphs(hipStream_t st) {
+    if (exec && tail_exec) return hipSuccess;
     auto capture = [&](int iters, hipGraph_t* g, hipGraphExec_t* x) -> hipError_t {
         WR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         const hipError_t e = enqueue(iters, st, nullptr);
@@ -699,8 +707,9 @@ hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st, u
         return hipGraphInstantiate(x, *g, nullptr, nullptr, 0);
     };
     static_assert(kGraphIters % 2 == 0 && kTailIters % 2 == 0, "replays end on the parity they start on");
-    WR_CHECK(capture(kGraphIters, &graph, &exec));
-    return capture(kTailIters, &tail_graph, &tail_exec);
+    if (!exec) WR_CHECK(capture(kGraphIters, &graph, &exec));
+    if (!tail_exec) WR_CHECK(capture(kTailIters, &tail_graph, &tail_exec));
+    return hipSuccess;
 }
 
 static EngineDev engine_dev(const Engine& En) {
@@ -782,8 +791,12 @@ hipError_t Engine::release_slot(uint32_t slot, hipStream_t st) {
 
 // One replay of the iteration graph, then a read-back of the job states.
 hipError_t Engine::step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt, bool tail, bool direct) {
-    if (direct) WR_CHECK(enqueue(tail ? kTailIters : kGraphIters, st, nullptr));
-    else WR_CHECK(hipGraphLaunch(tail ? tail_exec : exec, st));
+    if (direct) {
+        WR_CHECK(enqueue(tail ? kTailIters : kGraphIters, st, nullptr));
+    } else {
+        WR_CHECK(ensure_graphs(st));
+        WR_CHECK(hipGraphLaunch(tail ? tail_exec : exec, st));
+    }
     if (host_ctl) WR_CHECK(hipMemcpyAsync(host_ctl, ctl, sizeof(JobCtl) * kMaxJobs, hipMemcpyDeviceToHost, st));
     if (host_cnt) WR_CHECK(hipMemcpyAsync(host_cnt, cnt, sizeof(EngineCounters), hipMemcpyDeviceToHost, st));
     return hipStreamSynchronize(st);
@@ -842,6 +855,7 @@ static hipError_t run_wavefront_once(const SceneDev& S, const CamHost& cam, uint
     const uint64_t total = (uint64_t)npass * npix;
     if (cap == 0) cap = kDefaultPathSlots;
     WR_CHECK(E.init(cap, grid, st, lds_levels));
+    if (launch == kLaunchGraph) WR_CHECK(E.ensure_graphs(st));
     JobDesc J{};
     fill_job_camera(J, cam);
     J.scene = 0;

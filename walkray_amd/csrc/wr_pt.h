@@ -128,7 +128,9 @@ struct Engine {
     hipError_t enqueue(int iters, hipStream_t st, hipEvent_t* ev);
     ~Engine();
     void release();
+    bool ready = false;  // buffers allocated and zeroed for (cap, grid, lds_levels)
     hipError_t init(uint32_t cap, uint32_t grid, hipStream_t st, uint32_t lds_levels = 8);
+    hipError_t ensure_graphs(hipStream_t st);
     hipError_t set_scene(uint32_t slot, const SceneDev& S, hipStream_t st);
     hipError_t admit(uint32_t slot, const JobDesc& J, hipStream_t st);
     hipError_t release_slot(uint32_t slot, hipStream_t st);

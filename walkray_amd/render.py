@@ -180,6 +180,26 @@ def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
     return out
 
 
+def profile_path_tracing(ctx, log, c3_spp=16):
+    """The C1 and C3 renders for a kernel tracer: both engines, the wavefront's kernels
+    launched one by one (WR_RENDER_DIRECT) and no graph created, the default engine
+    configuration (no autotuning).  Two renders of each, the second the one to read."""
+    out = {}
+    for name, (objs, cam), spp, depth, seed in (("c1", S.cornell(k=5, width=256, height=256), 16, 8, 1234),
+                                                ("c3", S.random_field(), c3_spp, 16, 77)):
+        sc = W.Scene(ctx, objs)
+        r = {}
+        for mode_name, mode, launch in (("megakernel", W.WR_MEGAKERNEL, 0),
+                                        ("wavefront", W.WR_WAVEFRONT, W.WR_RENDER_DIRECT)):
+            for _ in range(2):
+                film = W.Film(ctx, cam["width"], cam["height"])
+                st = W.render(sc, cam, film, spp, depth, seed, mode, launch=launch)
+            r[mode_name] = {"ms": round(st["kernel_ms"], 3), "iterations": st["iterations"]}
+            log(f"[bench] {name} {mode_name} (direct launches): {st['kernel_ms']:.2f} ms")
+        out[name] = r
+    return out
+
+
 def bench_light_paths(ctx, log, depth=16, radius=0.01, repeats=3):
     """The front end of BDPT / VCM / PPM on the 512-object field: one light sub-path per
     1080p pixel (maxPathLen 16), the photons VCM keeps (after one leading point per
