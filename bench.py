@@ -39,7 +39,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 32   # node bytes are counted in 32 B units (a 64 B binary or wide node counts 2)
-TRI_BYTES = 48
+TRI_BYTES = 64   # triangle record: vertices, normal, tie key, triangle id, object, chain record
 RAY_IN_BYTES = 24
 HIT_OUT_BYTES = 16
 

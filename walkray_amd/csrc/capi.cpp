@@ -348,7 +348,7 @@ int wr_scene_build(wr_scene* s, int strategy) {
                 }
                 leaf_chain.push_back((uint32_t)chains.size());
                 wr_chain c{};
-                while (up != WR_NONE && c.count < 5) {
+                while (up != WR_NONE && c.count < kChainLevels) {
                     const wr_node& N = F.nodes[up >> 1];
                     const float* b = (up & 1u) ? N.rmin : N.lmin;  // min then max, contiguous
                     for (int k = 0; k < 3; k++) {
@@ -500,7 +500,7 @@ int wr_scene_build(wr_scene* s, int strategy) {
         wr_node* d_nodes;
         wr_wnode* d_wnodes;
         wr_tri* d_tris;
-        uint32_t *d_lu, *d_lc, *d_rf, *d_lch, *d_nu, *d_rp, *d_ro, *d_hn;
+        uint32_t *d_lu, *d_lc, *d_nu, *d_hn;
         wr_chain* d_ch;
         float* d_rn;
         wr_tree_hdr* d_h;
@@ -510,25 +510,28 @@ int wr_scene_build(wr_scene* s, int strategy) {
         s->dbufs.push_back(d_nodes);
         if ((rc = upload(ctx, wnodes, &d_wnodes, bytes))) return rc;
         s->dbufs.push_back(d_wnodes);
-        tris.push_back(wr_tri{});  // padding: the fast pass reads 16 B past the record it tests
+        // each record carries its tie key, triangle id, object and leaf chain
+        for (size_t ti = 0; ti < hdrs.size(); ti++) {
+            const uint32_t base = hdrs[ti].rec_base;
+            const size_t end = ti + 1 < hdrs.size() ? hdrs[ti + 1].rec_base : tris.size();
+            for (size_t g = base; g < end; g++) {
+                wr_tri& T = tris[g];
+                T.first = rec_first[g];
+                T.prim = rec_prim[g];
+                T.obj = rec_obj[g];
+                T.chain = leaf_chain[base + rec_first[g]];
+            }
+        }
         if ((rc = upload(ctx, tris, &d_tris, bytes))) return rc;
         s->dbufs.push_back(d_tris);
         if ((rc = upload(ctx, leaf_up, &d_lu, bytes))) return rc;
         s->dbufs.push_back(d_lu);
         if ((rc = upload(ctx, leaf_cnt, &d_lc, bytes))) return rc;
         s->dbufs.push_back(d_lc);
-        if ((rc = upload(ctx, rec_first, &d_rf, bytes))) return rc;
-        s->dbufs.push_back(d_rf);
-        if ((rc = upload(ctx, leaf_chain, &d_lch, bytes))) return rc;
-        s->dbufs.push_back(d_lch);
         if ((rc = upload(ctx, chains, &d_ch, bytes))) return rc;
         s->dbufs.push_back(d_ch);
         if ((rc = upload(ctx, node_up, &d_nu, bytes))) return rc;
         s->dbufs.push_back(d_nu);
-        if ((rc = upload(ctx, rec_prim, &d_rp, bytes))) return rc;
-        s->dbufs.push_back(d_rp);
-        if ((rc = upload(ctx, rec_obj, &d_ro, bytes))) return rc;
-        s->dbufs.push_back(d_ro);
         if ((rc = upload(ctx, rec_nrm, &d_rn, bytes))) return rc;
         s->dbufs.push_back(d_rn);
         if ((rc = upload(ctx, hdrs, &d_h, bytes))) return rc;
@@ -558,12 +561,8 @@ int wr_scene_build(wr_scene* s, int strategy) {
         D.tris = d_tris;
         D.leaf_up = d_lu;
         D.leaf_cnt = d_lc;
-        D.rec_first = d_rf;
-        D.leaf_chain = d_lch;
         D.chains = d_ch;
         D.node_anc = (const uint4*)d_nu;
-        D.rec_prim = d_rp;
-        D.rec_obj = d_ro;
         D.rec_nrm = d_rn;
         D.hdrs = d_h;
         D.objs = d_o;

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
         rng4(seed, path, pass, kLightVertex + 2 + depth, rn);
         const Out r = scatter(sc, self, o, d, hit, inside, rn, st);
-        const uint32_t ctri = S.rec_prim[hit.rec] - S.objs[hit.obj].prim_base;  // intersectTriangleID
+        const uint32_t ctri = S.tris[hit.rec].prim - S.objs[hit.obj].prim_base;  // intersectTriangleID
         if ((double)len(r.d) < 0.5) {
             put_vertex(V, nv, maxv, r.o, r.d, r.rad, r.op, r.dp, r.contact, ctri, r.inside, r.delta);
             break;

@@ -207,7 +207,7 @@ __global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
                     r = make_ray(mk(A.ox[pid], A.oy[pid], A.oz[pid]), mk(A.dx[pid], A.dy[pid], A.dz[pid]), tr.scale);
                     const bool primary = A.flags[pid] & kPrimary;
                     ns = NoSelf{primary ? WR_NONE : A.inside[pid], primary ? WR_NONE : A.contact[pid],
-                                S.rec_obj + H.rec_base, S.rec_nrm + 9 * (size_t)H.rec_base, S.obj_has_normals};
+                                S.rec_nrm + 9 * (size_t)H.rec_base, S.obj_has_normals};
                     overflow = false;
                     if (wide_begin(tr, r, st, h, ref)) {
                         active = true;
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         HitState h{A.ht[i], A.hrec[i], A.hobj[i]};
         if (h.rec != WR_NONE || (fl & kOverflow)) {
             const RayPre r = make_ray(o, d, sc.tr.scale);
-            const NoSelf ns{primary ? WR_NONE : A.inside[i], primary ? WR_NONE : A.contact[i], sc.gobj, sc.gnrm,
+            const NoSelf ns{primary ? WR_NONE : A.inside[i], primary ? WR_NONE : A.contact[i], sc.gnrm,
                             S.obj_has_normals};
             Counters cnt{0, 0, 0};
             bool failed;
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
         A.ht[i] = h.t;
         A.hrec[i] = h.rec;
-        A.hobj[i] = sc.gobj[h.rec];
+        A.hobj[i] = sc.tr.tris[h.rec].obj;
         A.flags[i] = fl & kFront;
     }
 }

@@ -84,7 +84,6 @@ __device__ __forceinline__ f3 fix_vignetting(const CamDev& c, f3 color, uint32_t
 struct SceneCtx {
     SceneDev S;
     Tree tr;               // global tree
-    const uint32_t* gobj;  // global record -> object
     const float* gnrm;     // global record -> 9 vertex-normal floats
 };
 
@@ -93,7 +92,6 @@ __device__ __forceinline__ SceneCtx make_scene_ctx(const SceneDev& S) {
     c.S = S;
     const wr_tree_hdr H = S.hdrs[0];
     c.tr = make_tree(H, S);
-    c.gobj = S.rec_obj + H.rec_base;
     c.gnrm = S.rec_nrm + 9 * (size_t)H.rec_base;
     return c;
 }
@@ -109,14 +107,14 @@ template <bool NOSELF>
 __device__ __forceinline__ PHit trace_scene(const SceneCtx& sc, f3 o, f3 d, uint32_t inside, uint32_t contact,
                                             LaneStack& st) {
     const RayPre r = make_ray(o, d, sc.tr.scale);
-    NoSelf ns{inside, contact, sc.gobj, sc.gnrm, sc.S.obj_has_normals};
+    NoSelf ns{inside, contact, sc.gnrm, sc.S.obj_has_normals};
     Counters cnt{0, 0, 0};
     bool failed;
     const HitState h = closest_hit<NOSELF, false>(sc.tr, r, ns, st, cnt, failed);
     PHit ph;
     ph.t = h.t;
     ph.rec = h.rec;
-    ph.obj = h.rec == WR_NONE ? WR_NONE : sc.gobj[h.rec];
+    ph.obj = h.rec == WR_NONE ? WR_NONE : sc.tr.tris[h.rec].obj;
     return ph;
 }
 
@@ -391,7 +389,7 @@ __device__ __forceinline__ NeeConn nee_connect(const SceneCtx& sc, f3 o, uint32_
     const float dist = len(sub(L.o, o));
     const float cd = (double)dist > kEps ? dist : (float)kEps;  // MAX(dist, EPSILON)
     if (dot(dir, L.n) >= 0.f) return c;
-    const f3 nE = rec_normal(sc, crec, sc.gobj[crec], o);
+    const f3 nE = rec_normal(sc, crec, sc.tr.tris[crec].obj, o);
     f3 f = (front && dot(dir, nE) >= 0.f) ? muls(mk(ob.color[0], ob.color[1], ob.color[2]), (float)kInvPi)
                                           : mk(0.f, 0.f, 0.f);
     f = mulv(f, L.rad);  // radianceDecay is (1,1,1) here: multiplying by it is exact

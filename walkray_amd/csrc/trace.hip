@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void k_closest_hit_finish(SceneDev S, wr_tree_
             if (h.rec != WR_NONE) {
                 float u1, u2;
                 bary(tr.tris[h.rec], o, d, h.t, u1, u2);
-                out = make_uint4(S.rec_prim[H.rec_base + h.rec], __float_as_uint(h.t), __float_as_uint(u1),
+                out = make_uint4(tr.tris[h.rec].prim, __float_as_uint(h.t), __float_as_uint(u1),
                                  __float_as_uint(u2));
             }
             if (failed) out.x = 0xFFFFFFFEu;  // stack overflow: reported, never silently wrong
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void k_closest_hit_path(SceneDev S, wr_tree_hd
     const f3 d = mk(__uint_as_float(rp[3]), __uint_as_float(rp[4]), __uint_as_float(rp[5]));
     const Tree tr = make_tree(H, S);
     const RayPre r = make_ray(o, d, tr.scale);
-    NoSelf ns{rp[6], rp[7], S.rec_obj + H.rec_base, S.rec_nrm + 9 * (size_t)H.rec_base, S.obj_has_normals};
+    NoSelf ns{rp[6], rp[7], S.rec_nrm + 9 * (size_t)H.rec_base, S.obj_has_normals};
     Counters cnt{0, 0, 0};
     bool failed;
     const HitState h = closest_hit<true, false>(tr, r, ns, st, cnt, failed);
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_closest_hit_path(SceneDev S, wr_tree_hd
     if (h.rec != WR_NONE) {
         float u1, u2;
         bary(tr.tris[h.rec], o, d, h.t, u1, u2);
-        out = make_uint4(S.rec_prim[H.rec_base + h.rec], __float_as_uint(h.t), __float_as_uint(u1),
+        out = make_uint4(tr.tris[h.rec].prim, __float_as_uint(h.t), __float_as_uint(u1),
                          __float_as_uint(u2));
     } else {
         out = make_uint4(WR_NONE, __float_as_uint(-1.f), 0u, 0u);

@@ -40,8 +40,6 @@ struct Tree {
     const wr_tri* tris;        // already offset to this tree
     const uint32_t* leaf_up;   // offset like tris
     const uint32_t* leaf_cnt;  // offset like tris
-    const uint32_t* rec_first; // offset like tris
-    const uint32_t* leaf_chain;  // offset like tris (values index chains directly)
     const wr_chain* chains;
     const uint4* node_anc;     // offset like nodes
     float rmin[3], rmax[3];
@@ -57,8 +55,6 @@ __device__ __forceinline__ Tree make_tree(const wr_tree_hdr& h, const SceneDev& 
     t.tris = S.tris + h.rec_base;
     t.leaf_up = S.leaf_up + h.rec_base;
     t.leaf_cnt = S.leaf_cnt + h.rec_base;
-    t.rec_first = S.rec_first + h.rec_base;
-    t.leaf_chain = S.leaf_chain + h.rec_base;
     t.chains = S.chains;
     t.node_anc = S.node_anc + h.node_base;
     for (int a = 0; a < 3; a++) {
@@ -216,7 +212,7 @@ __device__ __forceinline__ bool ref_box(const RayPre& r, const float* mn, const 
     return ref_box_exact(r.o.x, r.o.y, r.o.z, r.d.x, r.d.y, r.d.z, mn[0], mn[1], mn[2], mx[0], mx[1], mx[2]);
 }
 
-// Reference triangle test (kdtree.cpp:279-306) on a 48 B record; -1 on a miss.
+// Reference triangle test (kdtree.cpp:279-306) on a triangle record; -1 on a miss.
 // The test is a pure function, so its three conditions may be evaluated in
 // any order: t (exactly the reference's orthoDist / d) comes first, and the
 // three edge-sign tests are skipped when t cannot beat the current best
@@ -265,7 +261,6 @@ __device__ __forceinline__ f3 world_normal(const wr_tri& T, const float* nrm, f3
 // NoSelfCondition::legal (noself.cpp:13-20) on object ids (WR_NONE = NULL).
 struct NoSelf {
     uint32_t inside, contact;
-    const uint32_t* rec_obj;    // record -> object (global tree), already offset
     const float* rec_nrm;       // record -> 9 normals, already offset
     const uint32_t* obj_flags;  // object -> has_normals
 };
@@ -274,7 +269,7 @@ __device__ __forceinline__ bool noself_legal(const NoSelf& ns, const wr_tri& T, 
                                              const f3& d, float t) {
     // both NULL: legal() returns !(in && NULL == obj), true for every object
     if (ns.inside == WR_NONE && ns.contact == WR_NONE) return true;
-    const uint32_t obj = ns.rec_obj[rec];
+    const uint32_t obj = T.obj;
     const float* nrm = ns.obj_flags[obj] ? ns.rec_nrm + 9 * (size_t)rec : nullptr;
     float u1, u2;
     const f3 pos = add(o, muls(d, t));
@@ -407,15 +402,14 @@ __device__ __forceinline__ void visit_leaf(const Tree& tr, const RayPre& r, cons
 
 // Reference triangle test of one record reached through the wide hierarchy
 // (its own box), folded into h with the same total order as visit_leaf: the
-// leaf key is the record's reference leaf (rec_first), looked up only for a
-// tie or an accepted hit.
+// leaf key is the record's reference leaf (carried in the record).
 template <bool NOSELF, bool COUNT>
 __device__ __forceinline__ void visit_tri_rec(const Tree& tr, const RayPre& r, const NoSelf& ns, uint32_t rec,
                                               const wr_tri& T, HitState& h, Counters& cnt) {
     if (COUNT) cnt.tris++;
     const float t = ref_tri(T, r.o, r.d, h.t, false);
     if (!(t >= 0.f)) return;
-    const uint32_t f = tr.rec_first[rec];
+    const uint32_t f = T.first;
     // t <= h.t here; on equality the larger leaf key wins, then the earlier slot
     const bool better = h.rec == WR_NONE || t < h.t || f > h.first || (f == h.first && rec < h.rec);
     if (better && (!NOSELF || noself_legal(ns, T, rec, r.o, r.d, t))) {
@@ -575,8 +569,7 @@ __device__ __forceinline__ uint32_t wide_node(const RayPre& r, Stack& st, float 
 // set: the caller re-runs the ray on the binary tree).
 //
 // The record of the step is fetched before the lanes part ways: the wide node
-// (64 B) at a node, the triangle record (48 B, and 16 B beyond it, which the
-// device array pads past its last record) at a leaf, with the same four
+// (64 B) at a node, the triangle record (64 B) at a leaf, with the same four
 // loads.  A step where some lanes sit at nodes and others at triangles then
 // waits for memory once instead of once per branch.
 template <bool NOSELF, bool COUNT, class Stack>
@@ -595,6 +588,10 @@ __device__ __forceinline__ bool wide_step(const Tree& tr, const RayPre& r, const
         T.v1[0] = __uint_as_float(d0.w); T.v1[1] = __uint_as_float(d1.x); T.v1[2] = __uint_as_float(d1.y);
         T.v2[0] = __uint_as_float(d1.z); T.v2[1] = __uint_as_float(d1.w); T.v2[2] = __uint_as_float(d2.x);
         T.n[0] = __uint_as_float(d2.y); T.n[1] = __uint_as_float(d2.z); T.n[2] = __uint_as_float(d2.w);
+        T.first = d3.x;
+        T.prim = d3.y;
+        T.obj = d3.z;
+        T.chain = d3.w;
         visit_tri_rec<NOSELF, COUNT>(tr, r, ns, rec, T, h, cnt);
         return st.next(prune_cut(r, h.t), ref);
     }
@@ -613,10 +610,11 @@ __device__ __forceinline__ void traverse_wide(const Tree& tr, const RayPre& r, c
 }
 
 // Re-check the winner's root-to-leaf chain with the reference box test.  The
-// five levels nearest the leaf come from the leaf's chain record (one line);
-// above them the links come four at a time (node_anc), so the box fetches of
-// four levels are issued together instead of one dependent fetch per level,
-// each reading only the 24 B box of the chain's side of the node.
+// ten levels nearest the leaf come from the leaf's chain record (two lines,
+// five levels each); above them the links come four at a time (node_anc), so
+// the box fetches of four levels are issued together instead of one dependent
+// fetch per level, each reading only the 24 B box of the chain's side of the
+// node.
 struct Box6 {
     float mn[3], mx[3];
 };
@@ -631,16 +629,27 @@ __device__ __forceinline__ Box6 side_box(const Tree& tr, uint32_t up) {
     return b;
 }
 
-__device__ __forceinline__ bool verify_chain(const Tree& tr, const RayPre& r, uint32_t first) {
-    const uint32_t ci = tr.leaf_chain[first];
-    if (ci != WR_NONE) {
-        // the leaf's first five levels come in one 128 B line
-        const wr_chain C = tr.chains[ci];
-        const uint32_t count = C.count;
-        uint32_t up = C.next;
+// Five boxes of a chain record (one 128 B line's worth), loaded before any is tested.
+__device__ __forceinline__ bool chain_five(const RayPre& r, const float* p, uint32_t n) {
+    float b[5][6];
 #pragma unroll
-        for (int k = 0; k < 5; k++)
-            if ((uint32_t)k < count && !ref_box(r, C.box[k], C.box[k] + 3)) return false;
+    for (int k = 0; k < 5; k++)
+#pragma unroll
+        for (int a = 0; a < 6; a++) b[k][a] = p[6 * k + a];
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+        if ((uint32_t)k < n && !ref_box(r, b[k], b[k] + 3)) return false;
+    return true;
+}
+
+// ci: the chain record of the winner's leaf (wr_tri::chain), WR_NONE for the root leaf.
+__device__ __forceinline__ bool verify_chain(const Tree& tr, const RayPre& r, uint32_t ci) {
+    if (ci != WR_NONE) {
+        const wr_chain* C = tr.chains + ci;
+        const uint32_t count = C->count;
+        uint32_t up = C->next;
+        if (!chain_five(r, &C->box[0][0], count)) return false;
+        if (count > 5 && !chain_five(r, &C->box[5][0], count - 5)) return false;
         if (up != WR_NONE) {
             {
                 const Box6 b = side_box(tr, up);
@@ -675,7 +684,7 @@ __device__ __forceinline__ void finish_hit(const Tree& tr, const RayPre& r, cons
                                            Counters& cnt, HitState& h, bool overflow, bool& failed) {
     failed = false;
     if (overflow) traverse<NOSELF, false, COUNT>(tr, r, ns, st, h, overflow, cnt);
-    if (!overflow && (h.rec == WR_NONE || verify_chain(tr, r, h.first))) return;
+    if (!overflow && (h.rec == WR_NONE || verify_chain(tr, r, tr.tris[h.rec].chain))) return;
     if (COUNT) cnt.fallback++;
     traverse<NOSELF, true, COUNT>(tr, r, ns, st, h, overflow, cnt);
     failed = overflow;

@@ -13,12 +13,8 @@ struct SceneDev {
     const wr_tri* tris;
     const uint32_t* leaf_up;
     const uint32_t* leaf_cnt;
-    const uint32_t* rec_first;       // record -> first record of its reference leaf
-    const uint32_t* leaf_chain;      // first record of a non-root leaf -> its chain record (global index)
     const wr_chain* chains;
     const uint4* node_anc;           // internal node -> its next four ancestor links (parent << 1 | side), WR_NONE past the root
-    const uint32_t* rec_prim;        // record -> triangle id (global id for the global tree, local for object trees)
-    const uint32_t* rec_obj;         // record -> object id
     const float* rec_nrm;            // record -> 9 floats: vertex normals (objects with normals)
     const wr_tree_hdr* hdrs;         // [0] = global tree, [1 + i] = object i
     const wr_object_gpu* objs;

@@ -34,15 +34,25 @@ struct wr_node {
 };
 static_assert(sizeof(wr_node) == 64, "node must be 64 bytes");
 
-// 48 B triangle record: the three vertices in reference order and the
-// precomputed reference normal normalize((v1-v0) x (v2-v1)) (kdtree.cpp:294-295).
+// 64 B triangle record (half an L2 line, never split across lines): the three
+// vertices in reference order, the precomputed reference normal
+// normalize((v1-v0) x (v2-v1)) (kdtree.cpp:294-295), and what the queries need
+// about the record once its triangle test accepts, so no side array is read on
+// the hot path: the first record of its reference leaf (the tie key), the
+// triangle id (global in the global tree, the object's own in an object tree),
+// the object, and the chain record of its reference leaf (WR_NONE when the leaf
+// is the root).
 struct wr_tri {
     float v0[3];
     float v1[3];
     float v2[3];
     float n[3];
+    uint32_t first;
+    uint32_t prim;
+    uint32_t obj;
+    uint32_t chain;
 };
-static_assert(sizeof(wr_tri) == 48, "triangle record must be 48 bytes");
+static_assert(sizeof(wr_tri) == 64, "triangle record must be 64 bytes");
 
 // 64 B wide node of the fast-pass hierarchy: the reference tree collapsed two
 // levels at a time (up to four children: grandchildren, or a child that is a
@@ -61,17 +71,21 @@ static_assert(sizeof(wr_wnode) == 64, "wide node must be 64 bytes");
 static_assert(__builtin_offsetof(wr_wnode, q) == 24 && __builtin_offsetof(wr_wnode, ref) == 48,
               "wide-node fields are read as four 16 B words");
 
-// 128 B chain record of one reference leaf (one L2 line): the boxes that the
-// reference tests on the way to the leaf at its first up-to-five ancestors,
+// 256 B chain record of one reference leaf (two L2 lines): the boxes that the
+// reference tests on the way to the leaf at its first up-to-ten ancestors,
 // leaf-side first (box[k] is the box of the ancestor-k child on the path,
-// stored in ancestor k + 1), and the link (parent << 1 | side) of the next
-// level up, WR_NONE when the chain ends at the root.
+// stored in ancestor k + 1), their number, and the link (parent << 1 | side)
+// of the next level up, WR_NONE when the chain ends at the root.  The first
+// line holds the header and the five levels nearest the leaf.
+constexpr uint32_t kChainLevels = 10;
 struct wr_chain {
-    float box[5][6];
-    uint32_t next;
     uint32_t count;
+    uint32_t next;
+    float box[kChainLevels][6];
+    uint32_t pad[2];
 };
-static_assert(sizeof(wr_chain) == 128, "chain record must be 128 bytes");
+static_assert(sizeof(wr_chain) == 256, "chain record must be 256 bytes");
+static_assert(__builtin_offsetof(wr_chain, box) + 5 * 24 == 128, "the first line holds five levels");
 
 // Per-tree header.  root_ref may itself be a leaf.
 struct wr_tree_hdr {

@@ -79,13 +79,13 @@ def _at_fixed_rmse(r, spp, cam, target=0.01):
 
 
 # Path-state bytes per path segment that each wavefront stage reads or writes
-# (32-bit SoA words of the path buffers, plus the hit's 48 B triangle record in
+# (32-bit SoA words of the path buffers, plus the hit's 64 B triangle record in
 # shade); the trace stage adds the fast pass's node (32 B units) and triangle
-# (48 B) records counted by its counting instance.
+# (64 B) records counted by its counting instance.
 STAGE_BYTES = {
     "trace": 40 + 16,          # o, d, flags, job, inside, contact read; ht, hrec, hobj, flags written
     "finish": 60 + 8,          # o, d, flags, job, ht, hrec, hobj, inside, contact, pixel, pass read; flags, inside written
-    "shade_read": 76 + 48,     # 19 words of state and the hit's triangle record
+    "shade_read": 76 + 64,     # 19 words of state and the hit's triangle record
     "shade_write": 52,         # 13 words of a surviving path's next state
     "sample": 12,              # a finished path's sample
     "generate": 52,            # a camera path's 13 words
@@ -103,7 +103,7 @@ def stage_roofline(W_, ctx, sc, cam, spp, depth, seed, log, label, nee=False):
     T, X, G = st["segments"], st["exits"], st["generated"]
     nodes, tris = st["box_records"], st["tri_records"]
     b = {
-        "trace": T * STAGE_BYTES["trace"] + 32 * nodes + 48 * tris,
+        "trace": T * STAGE_BYTES["trace"] + 32 * nodes + 64 * tris,
         "finish": T * STAGE_BYTES["finish"] + G * STAGE_BYTES["sample"],
         "shade": (T - X) * STAGE_BYTES["shade_read"] + (T - G) * STAGE_BYTES["shade_write"] + G * STAGE_BYTES["sample"],
         "shade_exits": X * (STAGE_BYTES["shade_read"] + STAGE_BYTES["shade_write"]),
