@@ -15,11 +15,12 @@ the light sub-paths and photon hash grid of BDPT / VCM / PPM on the field, the
 BidirectionalPathTracer on the Cornell box, and the served-jobs workload (p50/p99 latency and aggregate samples/s under
 Poisson arrivals).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pt] [--legs ch,pt,lp,bdpt,serve,c5]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pt] [--legs ch,pt,lp,bdpt,motion,serve,c5]
 
 --legs picks the workloads (closest hit, path tracing, light paths + photon
-grid, BidirectionalPathTracer on the Cornell box, serving, and c5: the
-43.8M-triangle large scene at 3840x2160 x 256 spp; ptd, not in the default
+grid, BidirectionalPathTracer on the Cornell box, moving objects (per-pass
+rebuilds), serving, and c5: the 43.8M-triangle large scene at 3840x2160 x
+256 spp; ptd, not in the default
 run: C1 and C3 with the wavefront's kernels launched one by one and no graph
 created, for kernel tracers); a run without the closest-hit leg reports value
 null (used to profile one leg).
@@ -287,7 +288,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pt", action="store_true", help="same as --legs ch")
-    ap.add_argument("--legs", default="ch,pt,lp,bdpt,serve,c5")
+    ap.add_argument("--legs", default="ch,pt,lp,bdpt,motion,serve,c5")
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (counter calibration)")
@@ -338,6 +339,9 @@ def main():
             extra["bdpt"]["cpu_baseline"] = cpu_baseline_bdpt(S)
             log(f"[bench] BDPT oracle: {extra['bdpt']['cpu_baseline']['value'] / 1e3:.1f} Ksamples/s on "
                 f"{extra['bdpt']['cpu_baseline']['cores']} cores")
+    if "motion" in legs:
+        from walkray_amd import render as R
+        extra["motion"] = R.bench_motion(ctx, log)
     if "c5" in legs:  # the large scene: about a minute of host build and rendering
         from walkray_amd import render as R
         extra["large_scene"] = R.bench_large(W, S, ctx, log)

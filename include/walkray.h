@@ -142,6 +142,15 @@ int wr_tree_build(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t
 int wr_tree_export(const wr_tree* tree, uint32_t* n_nodes, uint32_t* n_leaf_tris, uint32_t* depth, float* box,
                    uint32_t* right, uint32_t* leaf_first, uint32_t* leaf_count, uint32_t* tri_list);
 void wr_tree_free(wr_tree* tree);
+/* Host-only: wr_tree_build for the same triangles at other vertex positions
+ * (a moving object's pass), the reference tree built anew by the same rule
+ * and the fast-pass hierarchy of prev refit to the new positions instead of
+ * built anew: what wr_render's per-pass rebuild of moving objects does (no
+ * counterpart in the reference, whose updateSceneForMotionBlur rebuilds every
+ * tree, scene.cpp:67-80).  Exposed so the refit hierarchy can be checked
+ * without a GPU. */
+int wr_tree_build_refit(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, int nthreads,
+                        const wr_tree* prev, wr_tree** out);
 /* Host-only: the fast-pass hierarchy the GPU traverses for this tree (no
  * counterpart in the reference; exposed so its invariants can be checked
  * without a GPU).  wide_nodes receives n_wide 64-byte nodes (wr_wnode of
@@ -242,6 +251,8 @@ typedef struct {
     uint64_t iterations;  /* wavefront iterations (0 for the megakernel) */
     float kernel_ms;      /* device time of this call (HIP events on the context stream; with moving
                              objects it spans the per-pass host rebuilds too) */
+    float build_ms;       /* with moving objects: host time of the per-pass tree rebuilds and their
+                             upload (Scene::updateSceneForMotionBlur); 0 for a static scene */
 } wr_render_stats;
 
 /* Film: per pixel and channel a Kahan-compensated FP32 sum of the per-pass

@@ -178,3 +178,32 @@ def test_wide_hierarchy_threads_change_speed_not_result():
     assert a["root"] == b["root"]
     assert np.array_equal(a["nodes"].view(np.uint8), b["nodes"].view(np.uint8))
     assert np.array_equal(a["record_first"], b["record_first"])
+
+
+@pytest.mark.parametrize("case", ["field", "heightfield"])
+def test_refit_hierarchy_after_motion(case):
+    """The per-pass rebuild of moving objects: the reference tree is built anew for the
+    new positions (records in its leaf order, the same as a fresh build) and the previous
+    fast-pass hierarchy is refit to them: every record still appears exactly once and
+    every child box contains its triangles at their new positions."""
+    rng = np.random.default_rng(3)
+    if case == "field":
+        objs = S.random_field(n_objects=16, seed=2)[0]
+        V, T = _soup(objs)
+        V2 = V.copy()
+        off = np.cumsum([0] + [len(o["verts"]) for o in objs])
+        for i in (1, 4, 9):  # three objects move and turn
+            M = S.trs(rng.uniform(-0.3, 0.3, 3), rng.normal(size=3), rng.uniform(-1, 1), [1.0, 1.0, 1.0])
+            p = V[off[i]:off[i + 1]].astype(np.float64)
+            V2[off[i]:off[i + 1]] = (p @ M[:3, :3].T.astype(np.float64) + M[:3, 3]).astype(np.float32)
+    else:
+        hf = S.heightfield(65, seed=4)
+        V, T = hf["verts"], hf["tris"]
+        V2 = V.copy()
+        V2[:, 2] += rng.uniform(-0.05, 0.05, len(V)).astype(np.float32)
+    refit = W.build_tree_wide(V, T, refit_verts=V2)
+    fresh = W.build_tree_wide(V2, T)
+    assert np.array_equal(refit["record_tri"], fresh["record_tri"])
+    assert np.array_equal(refit["record_first"], fresh["record_first"])
+    seen = _wide_walk(refit, V2, T)
+    assert sorted(seen) == list(range(len(T))) and set(seen.values()) == {1}

@@ -60,7 +60,7 @@ class RenderParams(C.Structure):
 
 
 class RenderStats(C.Structure):
-    _fields_ = [("paths", C.c_uint64), ("iterations", C.c_uint64), ("kernel_ms", C.c_float)]
+    _fields_ = [("paths", C.c_uint64), ("iterations", C.c_uint64), ("kernel_ms", C.c_float), ("build_ms", C.c_float)]
 
 
 class ServerConfig(C.Structure):
@@ -148,6 +148,7 @@ def lib():
         "wr_tree_export": (i32, [vp, P(u32), P(u32), P(u32), vp, vp, vp, vp, vp]),
         "wr_tree_free": (None, [vp]),
         "wr_tree_wide": (i32, [vp, P(u32), P(u32), P(u32), vp, vp, vp]),
+        "wr_tree_build_refit": (i32, [vp, u32, vp, u32, i32, vp, P(vp)]),
         "wr_closest_hit": (i32, [vp, vp, vp, u64, vp, vp]),
         "wr_closest_hit_noself": (i32, [vp, vp, vp, u64, vp]),
         "wr_inside_object": (i32, [vp, vp, vp, u64, vp]),
@@ -240,13 +241,24 @@ WNODE_DTYPE = np.dtype([("origin", np.float32, 3), ("scale", np.float32, 3), ("q
                         ("ref", np.uint32, 4)])
 
 
-def build_tree_wide(verts, tris, nthreads=0):
-    """Host-only builder run; returns the fast-pass hierarchy (wide nodes, root, record tables)."""
+def build_tree_wide(verts, tris, nthreads=0, refit_verts=None):
+    """Host-only builder run; returns the fast-pass hierarchy (wide nodes, root, record tables).
+    refit_verts: the same triangles at these positions instead, with the first build's
+    hierarchy refit to them (the per-pass rebuild of moving objects)."""
     L = lib()
     v = np.ascontiguousarray(verts, dtype=np.float32)
     t = np.ascontiguousarray(tris, dtype=np.uint32)
     h = C.c_void_p()
     _check(L.wr_tree_build(_p(v), len(v), _p(t), len(t), nthreads, C.byref(h)), None, "wr_tree_build")
+    if refit_verts is not None:
+        v2 = np.ascontiguousarray(refit_verts, dtype=np.float32)
+        h2 = C.c_void_p()
+        try:
+            _check(L.wr_tree_build_refit(_p(v2), len(v2), _p(t), len(t), nthreads, h, C.byref(h2)), None,
+                   "wr_tree_build_refit")
+        finally:
+            L.wr_tree_free(h)
+        h = h2
     try:
         nw, nr, root = C.c_uint32(), C.c_uint32(), C.c_uint32()
         _check(L.wr_tree_wide(h, C.byref(nw), C.byref(nr), C.byref(root), None, None, None), None, "wr_tree_wide")
@@ -512,7 +524,7 @@ def render(scene, cam, film, spp, max_depth, seed, mode=WR_WAVEFRONT, wave_slots
     st = RenderStats()
     _check(lib().wr_render(scene.ctx.h, scene.h, C.byref(cd), C.byref(prm), film.h, C.byref(st)), scene.ctx.h,
            "wr_render")
-    return {"paths": st.paths, "iterations": st.iterations, "kernel_ms": st.kernel_ms}
+    return {"paths": st.paths, "iterations": st.iterations, "kernel_ms": st.kernel_ms, "build_ms": st.build_ms}
 
 
 def render_bdpt(scene, cam, film, spp, max_depth, seed, queue=0):

@@ -105,16 +105,38 @@ struct Builder {
         int bestDim = 0;
         uint32_t bestLeft = 0;
         const size_t U = s.uniq.size();
-        s.coords.resize(U);
-        for (int dim = 0; dim < 3; dim++) {
-            for (size_t i = 0; i < U; i++) s.coords[i] = c(s.uniq[i], dim);
-            std::nth_element(s.coords.begin(), s.coords.begin() + (U >> 1), s.coords.end());
-            const float split = s.coords[U >> 1];
+        // per dim: the upper median of the unique vertices' coordinates and the
+        // triangles with a vertex at or below it; a large node's three dims run
+        // side by side (each on its own coordinate copy), then are compared in
+        // dim order as the serial loop does
+        float dsplit[3];
+        uint32_t dleft[3];
+        auto dim_stats = [&](int dim, std::vector<float>& coords) {
+            coords.resize(U);
+            for (size_t i = 0; i < U; i++) coords[i] = c(s.uniq[i], dim);
+            std::nth_element(coords.begin(), coords.begin() + (U >> 1), coords.end());
+            const float split = coords[U >> 1];
             uint32_t left = 0;
             for (uint32_t i = b; i < e; i++) {
                 const uint32_t* t = I + 3 * (size_t)order[i];
                 left += (uint32_t)((int)(c(t[0], dim) <= split) | (int)(c(t[1], dim) <= split) | (int)(c(t[2], dim) <= split));
             }
+            dsplit[dim] = split;
+            dleft[dim] = left;
+        };
+        if (n >= kParallelMin * 4) {
+            std::vector<float> c1, c2;
+            std::thread t1([&]() { dim_stats(1, c1); });
+            std::thread t2([&]() { dim_stats(2, c2); });
+            dim_stats(0, s.coords);
+            t1.join();
+            t2.join();
+        } else {
+            for (int dim = 0; dim < 3; dim++) dim_stats(dim, s.coords);
+        }
+        for (int dim = 0; dim < 3; dim++) {
+            const float split = dsplit[dim];
+            const uint32_t left = dleft[dim];
             const float ratio = left / float(n);
             if (std::fabs((double)ratio - 0.5) < std::fabs((double)bestRatio - 0.5)) {
                 bestRatio = ratio;
@@ -235,6 +257,125 @@ uint8_t quant_hi(float v, float origin, float scale) {
     int qi = (int)std::max(0.0, std::min(255.0, q));
     while (qi < 255 && std::fmaf((float)qi, scale, origin) < v) qi++;
     return (uint8_t)qi;
+}
+
+// Encodes a wide node over n <= 4 children with boxes cbox and references
+// ref: per axis a grid origin (the children's minimum) and a power-of-two
+// scale reaching their maximum within 255 steps, each child box rounded
+// outward onto the grid.
+void encode_wnode(wr_wnode& W, int n, const float (*cbox)[6], const uint32_t* ref) {
+    float origin[3], scale[3];
+    for (int a = 0; a < 3; a++) {
+        float lo = cbox[0][a], hi = cbox[0][3 + a];
+        for (int c = 1; c < n; c++) {
+            lo = std::min(lo, cbox[c][a]);
+            hi = std::max(hi, cbox[c][3 + a]);
+        }
+        origin[a] = lo;
+        const double ext = (double)hi - (double)lo;
+        int ex = 0;
+        if (ext > 0) {
+            std::frexp(ext / 255.0, &ex);  // ext / 255 < 2^ex
+        } else {
+            ex = -100;
+        }
+        float sc = std::ldexp(1.0f, std::max(ex, -126));
+        while (std::fmaf(255.0f, sc, origin[a]) < hi) sc *= 2.0f;
+        scale[a] = sc;
+    }
+    for (int a = 0; a < 3; a++) {
+        W.origin[a] = origin[a];
+        W.scale[a] = scale[a];
+    }
+    for (int c = 0; c < 4; c++) {
+        if (c < n) {
+            for (int a = 0; a < 3; a++) {
+                W.q[a][c] = quant_lo(cbox[c][a], origin[a], scale[a]);
+                W.q[3 + a][c] = quant_hi(cbox[c][3 + a], origin[a], scale[a]);
+            }
+            W.ref[c] = ref[c];
+        } else {
+            for (int k = 0; k < 6; k++) W.q[k][c] = 0;
+            W.ref[c] = WR_NONE;
+        }
+    }
+}
+
+// The fast-pass hierarchy of `prev` (the same triangles at other positions)
+// refit to F: every leaf reference renamed to the record that now holds its
+// triangle, every box recomputed bottom-up from the current vertices and
+// encoded outward again.  The topology is prev's, so its quality is prev's
+// for the positions prev was built at; correctness needs only that every box
+// contains the triangles below it, which the recomputation gives.
+void refit_wide(FlatTree& F, const FlatTree& prev, int nthreads) {
+    const size_t nr = F.tris.size();
+    std::vector<uint32_t> rec_of(nr);  // input triangle -> record
+    for (size_t r = 0; r < nr; r++) rec_of[F.rec_prim[r]] = (uint32_t)r;
+    F.wnodes = prev.wnodes;
+    const size_t nw = F.wnodes.size();
+    auto rename = [&](uint32_t ref) {
+        if (ref == WR_NONE || !(ref & WR_LEAF_BIT)) return ref;
+        const uint32_t old = ref & WR_FIRST_MASK;
+        return (ref & ~WR_FIRST_MASK) | rec_of[prev.rec_prim[old]];
+    };
+    auto tri_box = [&](uint32_t rec, float* b) {
+        const wr_tri& T = F.tris[rec];
+        for (int a = 0; a < 3; a++) {
+            b[a] = std::min(std::min(T.v0[a], T.v1[a]), T.v2[a]);
+            b[3 + a] = std::max(std::max(T.v0[a], T.v1[a]), T.v2[a]);
+        }
+    };
+    // nodes are in preorder (a node's wide children come after it): their depths in
+    // one forward sweep, then the levels from the deepest up, each level's nodes
+    // independent of one another
+    std::vector<uint32_t> depth(nw, 0u);
+    uint32_t max_d = 0;
+    for (size_t i = 0; i < nw; i++)
+        for (int c = 0; c < 4; c++) {
+            const uint32_t r = F.wnodes[i].ref[c];
+            if (r != WR_NONE && !(r & WR_LEAF_BIT)) {
+                depth[r] = depth[i] + 1;
+                max_d = std::max(max_d, depth[r]);
+            }
+        }
+    std::vector<std::vector<uint32_t>> level(max_d + 1);
+    for (size_t i = 0; i < nw; i++) level[depth[i]].push_back((uint32_t)i);
+    std::vector<std::array<float, 6>> nbox(nw);
+    auto refit_node = [&](uint32_t i) {
+        wr_wnode& W = F.wnodes[i];
+        float cbox[4][6];
+        uint32_t ref[4];
+        int n = 0;
+        for (int c = 0; c < 4 && W.ref[c] != WR_NONE; c++, n++) {
+            ref[c] = rename(W.ref[c]);
+            if (ref[c] & WR_LEAF_BIT) tri_box(ref[c] & WR_FIRST_MASK, cbox[c]);
+            else std::copy(nbox[ref[c]].begin(), nbox[ref[c]].end(), cbox[c]);
+        }
+        auto& nb = nbox[i];
+        for (int a = 0; a < 3; a++) {
+            nb[a] = cbox[0][a];
+            nb[3 + a] = cbox[0][3 + a];
+            for (int c = 1; c < n; c++) {
+                nb[a] = std::min(nb[a], cbox[c][a]);
+                nb[3 + a] = std::max(nb[3 + a], cbox[c][3 + a]);
+            }
+        }
+        encode_wnode(W, n, cbox, ref);
+    };
+    for (size_t d = level.size(); d-- > 0;) {
+        const std::vector<uint32_t>& L = level[d];
+        const int nt = (int)std::min<size_t>((size_t)std::max(nthreads, 1), L.size() / 2048 + 1);
+        std::atomic<size_t> next{0};
+        auto worker = [&]() {
+            for (size_t k; (k = next.fetch_add(256)) < L.size();)
+                for (size_t j = k; j < std::min(L.size(), k + 256); j++) refit_node(L[j]);
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; t++) pool.emplace_back(worker);
+        worker();
+        for (std::thread& t : pool) t.join();
+    }
+    F.hdr.wide_root = rename(prev.hdr.wide_root);
 }
 
 // The fast-pass hierarchy: a 4-wide BVH over the triangle records, built
@@ -416,44 +557,9 @@ struct WideBuilder {
     uint32_t make_node(std::vector<wr_wnode>& o, int n, const float (*cbox)[6], ChildRef child_ref) {
         const uint32_t node = (uint32_t)o.size();
         o.push_back(wr_wnode{});
-        float origin[3], scale[3];
-        for (int a = 0; a < 3; a++) {
-            float lo = cbox[0][a], hi = cbox[0][3 + a];
-            for (int c = 1; c < n; c++) {
-                lo = std::min(lo, cbox[c][a]);
-                hi = std::max(hi, cbox[c][3 + a]);
-            }
-            origin[a] = lo;
-            const double ext = (double)hi - (double)lo;
-            int ex = 0;
-            if (ext > 0) {
-                std::frexp(ext / 255.0, &ex);  // ext / 255 < 2^ex
-            } else {
-                ex = -100;
-            }
-            float sc = std::ldexp(1.0f, std::max(ex, -126));
-            while (std::fmaf(255.0f, sc, origin[a]) < hi) sc *= 2.0f;
-            scale[a] = sc;
-        }
         uint32_t wref[4];
         for (int c = 0; c < n; c++) wref[c] = child_ref(c);
-        wr_wnode& W = o[node];
-        for (int a = 0; a < 3; a++) {
-            W.origin[a] = origin[a];
-            W.scale[a] = scale[a];
-        }
-        for (int c = 0; c < 4; c++) {
-            if (c < n) {
-                for (int a = 0; a < 3; a++) {
-                    W.q[a][c] = quant_lo(cbox[c][a], origin[a], scale[a]);
-                    W.q[3 + a][c] = quant_hi(cbox[c][3 + a], origin[a], scale[a]);
-                }
-                W.ref[c] = wref[c];
-            } else {
-                for (int k = 0; k < 6; k++) W.q[k][c] = 0;
-                W.ref[c] = WR_NONE;
-            }
-        }
+        encode_wnode(o[node], n, cbox, wref);
         return node;
     }
 
@@ -561,7 +667,8 @@ uint32_t internal_depth(const BNode* n) {
 
 }  // namespace
 
-FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, int nthreads) {
+FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, int nthreads,
+                    const FlatTree* refit_from) {
     auto t0 = std::chrono::steady_clock::now();
     if (nt > WR_MAX_RECORDS) throw std::runtime_error("too many triangles for one tree (max 2^27)");
     for (size_t i = 0; i < 3 * (size_t)nt; i++)
@@ -602,16 +709,26 @@ FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32
     F.rec_prim.resize(nt);
     F.leaf_up.assign(nt, WR_NONE);
     F.leaf_cnt.assign(nt, 0u);
-    for (uint32_t r = 0; r < nt; r++) {
-        const uint32_t ti = B.order[r];
-        wr_tri& t = F.tris[r];
-        for (int a = 0; a < 3; a++) {
-            t.v0[a] = verts[3 * (size_t)idx[3 * (size_t)ti + 0] + a];
-            t.v1[a] = verts[3 * (size_t)idx[3 * (size_t)ti + 1] + a];
-            t.v2[a] = verts[3 * (size_t)idx[3 * (size_t)ti + 2] + a];
+    auto fill = [&](uint32_t r0, uint32_t r1) {
+        for (uint32_t r = r0; r < r1; r++) {
+            const uint32_t ti = B.order[r];
+            wr_tri& t = F.tris[r];
+            for (int a = 0; a < 3; a++) {
+                t.v0[a] = verts[3 * (size_t)idx[3 * (size_t)ti + 0] + a];
+                t.v1[a] = verts[3 * (size_t)idx[3 * (size_t)ti + 1] + a];
+                t.v2[a] = verts[3 * (size_t)idx[3 * (size_t)ti + 2] + a];
+            }
+            ref_normal(t.v0, t.v1, t.v2, t.n);
+            F.rec_prim[r] = ti;
         }
-        ref_normal(t.v0, t.v1, t.v2, t.n);
-        F.rec_prim[r] = ti;
+    };
+    {
+        const uint32_t nw = nt >= (1u << 16) ? (uint32_t)std::max(1, nthreads) : 1u;
+        std::vector<std::thread> pool;
+        for (uint32_t w = 1; w < nw; w++)
+            pool.emplace_back(fill, (uint32_t)((uint64_t)nt * w / nw), (uint32_t)((uint64_t)nt * (w + 1) / nw));
+        fill(0, (uint32_t)((uint64_t)nt / nw));
+        for (std::thread& t : pool) t.join();
     }
     Flattener fl(F);
     F.nodes.reserve(nt);
@@ -625,9 +742,13 @@ FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32
     F.hdr.depth = internal_depth(&root);
     F.rec_first.resize(F.tris.size());
     fill_rec_first(F, rref);
-    WideBuilder wb(F, F.wnodes);
-    F.wnodes.reserve(F.tris.size() / 2 + 1);
-    F.hdr.wide_root = wb.build_root(nthreads);
+    if (refit_from && refit_from->hdr.n_recs == nt && refit_from->rec_prim.size() == nt) {
+        refit_wide(F, *refit_from, nthreads);
+    } else {
+        WideBuilder wb(F, F.wnodes);
+        F.wnodes.reserve(F.tris.size() / 2 + 1);
+        F.hdr.wide_root = wb.build_root(nthreads);
+    }
     F.hdr.n_wide = (uint32_t)F.wnodes.size();
     F.n_leaves = fl.leaves;
     F.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

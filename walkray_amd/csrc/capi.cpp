@@ -3,11 +3,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <exception>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -274,9 +276,17 @@ static float triangle_area(const HostObject& o, uint32_t t) {
 
 int wr_scene_build(wr_scene* s, int strategy) {
     if (!s) return fail(nullptr, WR_EINVAL, "wr_scene_build: scene is NULL");
-    wr_ctx* ctx = s->ctx;
     if (strategy != WR_BEST)
-        return fail(ctx, WR_EINVAL, "wr_scene_build: only BEST is supported (Scene always builds BEST, scene.cpp:41)");
+        return fail(s->ctx, WR_EINVAL, "wr_scene_build: only BEST is supported (Scene always builds BEST, scene.cpp:41)");
+    return wr::scene_build(s, nullptr);
+}
+
+}  // extern "C"
+
+namespace wr {
+
+int scene_build(wr_scene* s, const wr_scene* reuse) {
+    wr_ctx* ctx = s->ctx;
     try {
         (void)hipSetDevice(ctx->device);
         scene_free_device(s);
@@ -294,12 +304,54 @@ int wr_scene_build(wr_scene* s, int strategy) {
             s->prim_base[oi + 1] = s->prim_base[oi] + (uint32_t)(o.idx.size() / 3);
         }
         const uint32_t nt = (uint32_t)(I.size() / 3);
+        const bool can_reuse = reuse && reuse->built && reuse->trees.size() == 1 + s->objs.size();
         s->trees.clear();
-        s->trees.push_back(build_tree(V.data(), (uint32_t)(V.size() / 3), I.data(), nt, nth));
-        // per-object trees (scene.cpp:46-65)
-        for (const HostObject& o : s->objs)
-            s->trees.push_back(
-                build_tree(o.verts.data(), (uint32_t)(o.verts.size() / 3), o.idx.data(), (uint32_t)(o.idx.size() / 3), nth));
+        // with `reuse`, the global tree's fast-pass hierarchy is refit rather than built
+        s->trees.push_back(build_tree(V.data(), (uint32_t)(V.size() / 3), I.data(), nt, nth,
+                                      can_reuse ? &reuse->trees[0] : nullptr));
+        // per-object trees (scene.cpp:46-65).  A static object's tree from `reuse`
+        // (the same scene before its moving objects were placed) is the tree this
+        // deterministic builder would make again.  The others: large objects one
+        // after another with every thread, the rest one object per thread.
+        const size_t no = s->objs.size();
+        s->trees.resize(1 + no);
+        std::vector<size_t> big, small;
+        for (size_t oi = 0; oi < no; oi++) {
+            if (can_reuse && !s->objs[oi].moving) {
+                s->trees[1 + oi] = reuse->trees[1 + oi];
+                s->trees[1 + oi].build_seconds = 0;  // not built here
+                continue;
+            }
+            (s->objs[oi].idx.size() / 3 >= (1u << 18) ? big : small).push_back(oi);
+        }
+        auto build_obj = [&](size_t oi, int threads) {
+            const HostObject& o = s->objs[oi];
+            s->trees[1 + oi] = build_tree(o.verts.data(), (uint32_t)(o.verts.size() / 3), o.idx.data(),
+                                          (uint32_t)(o.idx.size() / 3), threads,
+                                          can_reuse ? &reuse->trees[1 + oi] : nullptr);
+        };
+        for (size_t oi : big) build_obj(oi, nth);
+        if (!small.empty()) {
+            std::atomic<size_t> next{0};
+            std::exception_ptr err;
+            std::mutex err_mu;
+            auto worker = [&]() {
+                for (size_t k; (k = next.fetch_add(1)) < small.size();) {
+                    try {
+                        build_obj(small[k], 1);
+                    } catch (...) {
+                        std::lock_guard<std::mutex> lk(err_mu);
+                        if (!err) err = std::current_exception();
+                    }
+                }
+            };
+            const int nw = (int)std::min<size_t>((size_t)nth, small.size());
+            std::vector<std::thread> pool;
+            for (int w = 1; w < nw; w++) pool.emplace_back(worker);
+            worker();
+            for (std::thread& t : pool) t.join();
+            if (err) std::rethrow_exception(err);
+        }
         if (s->trees[0].hdr.depth > 64) return fail(ctx, WR_EINVAL, "wr_scene_build: tree deeper than 64 levels");
         uint32_t max_obj_depth = 0;
         for (size_t i = 1; i < s->trees.size(); i++) {
@@ -597,6 +649,10 @@ int wr_scene_build(wr_scene* s, int strategy) {
     }
 }
 
+}  // namespace wr
+
+extern "C" {
+
 int wr_scene_get_info(const wr_scene* s, wr_scene_info* info) {
     if (!s || !info) return fail(nullptr, WR_EINVAL, "wr_scene_get_info: bad argument");
     if (!s->built) return fail(s->ctx, WR_ESTATE, "wr_scene_get_info: scene not built");
@@ -653,6 +709,21 @@ int wr_tree_build(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t
         return WR_OK;
     } catch (const std::exception& ex) {
         return fail(nullptr, WR_EINVAL, std::string("wr_tree_build: ") + ex.what());
+    }
+}
+
+int wr_tree_build_refit(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, int nthreads,
+                        const wr_tree* prev, wr_tree** out) {
+    if (!out || !prev || (!verts && nv) || (!idx && nt)) return fail(nullptr, WR_EINVAL, "wr_tree_build_refit: bad argument");
+    *out = nullptr;
+    if (prev->t.hdr.n_recs != nt) return fail(nullptr, WR_EINVAL, "wr_tree_build_refit: prev has another triangle count");
+    try {
+        std::unique_ptr<wr_tree> t(new wr_tree);
+        t->t = build_tree(verts, nv, idx, nt, nthreads, &prev->t);
+        *out = t.release();
+        return WR_OK;
+    } catch (const std::exception& ex) {
+        return fail(nullptr, WR_EINVAL, std::string("wr_tree_build_refit: ") + ex.what());
     }
 }
 

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -168,6 +169,7 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
     if (launch != kLaunchGraph && prm->mode != WR_WAVEFRONT)
         return fail(ctx, WR_EINVAL, "wr_render: launch-mode flags are for the wavefront engine");
     StageStats stage;
+    double build_ms = 0;  // host time of the per-pass rebuilds
     if ((uint64_t)cam->width * cam->height >= (1ull << 31)) return fail(ctx, WR_EINVAL, "wr_render: image too large");
     if (film->broken) return film_broken(film, "wr_render");
     (void)hipSetDevice(ctx->device);
@@ -191,7 +193,9 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
                 HostObject& o = tmp.objs[oi];
                 if (o.moving) place_object(o, io::interpolate(o.motion, motion_time(prm->seed, oi, pass)));
             }
-            const int rc = wr_scene_build(&tmp, WR_BEST);
+            const auto b0 = std::chrono::steady_clock::now();
+            const int rc = scene_build(&tmp, s);  // static objects keep their trees
+            build_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - b0).count();
             if (rc != WR_OK) {
                 film->passes += k;  // the passes before this one are in the film
                 return rc;
@@ -231,6 +235,7 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
         stats->paths = (uint64_t)prm->spp * cam->width * cam->height;
         stats->iterations = ws.iterations;
         stats->kernel_ms = ms;
+        stats->build_ms = (float)build_ms;
     }
     return WR_OK;
 }
@@ -314,6 +319,7 @@ extern "C" int wr_render_bdpt(wr_ctx* ctx, const wr_scene* s, const wr_camera* c
         stats->paths = npix * prm->spp;
         stats->iterations = 0;
         stats->kernel_ms = ms;
+        stats->build_ms = 0.f;
     }
     return WR_OK;
 }

@@ -47,6 +47,9 @@ float motion_time(uint64_t seed, uint32_t obj, uint32_t pass);
 // Looks up, on `device` (current), the address of the fixed work list of
 // wr_hashgrid_query_noalloc; once per device, from wr_ctx_create.
 hipError_t noalloc_scratch_init(int device);
+// wr_scene_build; the trees of objects that do not move are copied from
+// `reuse` (nullable) when it is the same scene, built.
+int scene_build(wr_scene* s, const wr_scene* reuse);
 }  // namespace wr
 
 struct wr_scene {

@@ -26,7 +26,10 @@ struct FlatTree {
 
 // KDTree::build(BEST) (kdtree.cpp:242-258, splitBest 141-240) over the given
 // triangle soup, then flattened.  Deterministic; threads only change speed.
-FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, int nthreads);
+// refit_from (nullable): a tree over the same triangles (at other positions)
+// whose fast-pass hierarchy is refit instead of built anew (moving objects).
+FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, int nthreads,
+                    const FlatTree* refit_from = nullptr);
 
 // Canonical preorder export (same format as the oracle's): used by tests to
 // compare this builder with the oracle's restatement node by node.

@@ -294,6 +294,42 @@ def bench_large(W_, S_, ctx, log, spp=256, depth=32):
     return r
 
 
+def bench_motion(ctx, log, spp=16):
+    """Moving objects (Scene::updateSceneForMotionBlur before every pass, scene.cpp:67-80):
+    the Cornell box with its diffuse sphere moving (256^2, max depth 8) and the 512-object
+    field with 8 moving objects (480x270, max depth 16), both engines.  Per pass the
+    global tree and the moving objects' trees are rebuilt and the static objects' trees
+    kept; for comparison, the time of building and uploading every tree of the placed
+    scene (what a whole rebuild costs) is measured on the same scene."""
+    out = {}
+    objs, cam, motions = S.cornell_moving(k=5)
+    fobjs, fcam = S.random_field(width=480, height=270)
+    fmot = {}
+    for i in range(1, 9):  # the first eight field objects slide sideways during the exposure
+        fmot[i] = (0.0, 1.0, np.eye(4, dtype=np.float32), S.trs([0.05 * i / 8, 0.0, 0.02], [0, 1, 0], 0.0, [1, 1, 1]))
+    for name, (ob, cm, mot, depth) in (("c1_moving_sphere", (objs, cam, motions, 8)),
+                                       ("field_8_moving", (fobjs, fcam, fmot, 16))):
+        sc = W.Scene(ctx, ob, build=False)
+        for o, (t0, t1, a, b) in mot.items():
+            sc.set_motion(o, t0, t1, a, b)
+        sc.build()
+        r = {"workload": f"{len(ob)} objects ({len(mot)} moving), {cm['width']}x{cm['height']}, {spp} spp, "
+                         f"max depth {depth}"}
+        for mode_name, mode in (("megakernel", W.WR_MEGAKERNEL), ("wavefront", W.WR_WAVEFRONT)):
+            film = W.Film(ctx, cm["width"], cm["height"])
+            st = W.render(sc, cm, film, spp, depth, 5, mode)
+            r[mode_name] = {"ms": round(st["kernel_ms"], 2), "rebuild_ms_per_pass": round(st["build_ms"] / spp, 3),
+                            "samples_per_s": round(cm["width"] * cm["height"] * spp / (st["kernel_ms"] * 1e-3), 1)}
+        t0 = time.perf_counter()
+        for _ in range(3):
+            W.Scene(ctx, ob).close()
+        r["whole_scene_build_ms"] = round((time.perf_counter() - t0) / 3 * 1e3, 3)
+        log(f"[bench] motion {name}: rebuild {r['wavefront']['rebuild_ms_per_pass']} ms per pass (whole scene "
+            f"{r['whole_scene_build_ms']} ms); megakernel {r['megakernel']['ms']} ms, wavefront {r['wavefront']['ms']} ms")
+        out[name] = r
+    return out
+
+
 def smoke_check(ctx):
     """A 24x24 Cornell render: megakernel and wavefront films must be identical."""
     objs, cam = S.cornell(k=2, width=24, height=24)

@@ -300,6 +300,33 @@ def large_scene(n=4096, n_spheres=2000, width=3840, height=2160, seed=5):
     return objs, cam
 
 
+def trs(t, axis, ang, scale):
+    """4x4 row-major transform: translation t, then a rotation by ang about axis, then a
+    per-axis scale (applied to a point as T R S p)."""
+    a = np.asarray(axis, np.float64)
+    x, y, z = a / np.linalg.norm(a)
+    c, s = math.cos(ang), math.sin(ang)
+    M = np.eye(4)
+    M[:3, :3] = np.array([[c + x * x * (1 - c), x * y * (1 - c) - z * s, x * z * (1 - c) + y * s],
+                          [y * x * (1 - c) + z * s, c + y * y * (1 - c), y * z * (1 - c) - x * s],
+                          [z * x * (1 - c) - y * s, z * y * (1 - c) + x * s, c + z * z * (1 - c)]]) @ np.diag(scale)
+    M[:3, 3] = t
+    return M.astype(np.float32)
+
+
+def cornell_moving(k=5, width=256, height=256):
+    """The Cornell box with its diffuse sphere (object 4) moving during the exposure:
+    translated, turned about y and shrunk between its transforms at times 0 and 1.
+    Returns objects (the sphere in its own frame), camera and {object: (t0, t1, start, end)}."""
+    objs, cam = cornell(k=k, width=width, height=height)
+    c = np.array([0.28, 0.201, 0.35], np.float32)
+    objs[4] = dict(objs[4])
+    objs[4]["verts"] = (objs[4]["verts"] - c).astype(np.float32)
+    a = trs(c, [0, 1, 0], 0.0, [1.0, 1.0, 1.0])
+    b = trs(c + np.float32([0.25, 0.1, 0.2]), [0, 1, 0], 1.2, [0.7, 0.9, 0.8])
+    return objs, cam, {4: (0.0, 1.0, a, b)}
+
+
 def heightfield_scene(n=1024, seed=7):
     """Ray-query workload: one fBm heightfield of 2 (n-1)^2 triangles."""
     return [heightfield(n, seed=seed)]
