@@ -180,3 +180,24 @@ def test_full_size_photon_grid(ctx):
         got_k = set(got[k, :counts[k]].tolist())
         assert got_k == (want if inside[k] else set()), k
     assert counts.mean() > 0.5 and counts.max() >= 3
+
+
+@pytest.mark.parametrize("n_cells,cap", [(1, 5000), (3, 20000)])
+def test_hashgrid_long_ranges_in_order(ctx, n_cells, cap):
+    """Buckets far longer than one step of the workgroup query (4,096 points per
+    iteration): 10,000 points, most of them within the radius of the queries, so every
+    query walks several iterations of its ranges; the lists equal the oracle's in order
+    (repeats through hash collisions included) up to the cap, and the counts beyond it."""
+    rng = np.random.default_rng(31 + n_cells)
+    pos = rng.uniform(0.0, 0.2, (10000, 3)).astype(np.float32)
+    pos[::7] = 0.05  # coincident points
+    q = np.concatenate([rng.uniform(0.0, 0.2, (40, 3)), np.full((4, 3), 0.05)]).astype(np.float32)
+    g = W.HashGrid.from_points(ctx, pos, 0.12, n_cells)
+    o = O.HashGrid(pos, 0.12, n_cells)
+    gc, gi = g.query(q, cap)
+    oc, oi = o.query(q, cap)
+    assert np.array_equal(gc, oc) and gc.min() > 4096
+    assert (gc > cap).any()  # some lists are cut at the cap
+    for k in range(len(q)):
+        m = min(int(gc[k]), cap)
+        assert np.array_equal(gi[k, :m], oi[k, :m]), k
