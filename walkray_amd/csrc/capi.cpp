@@ -362,7 +362,7 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
         std::vector<wr_node> nodes;
         std::vector<wr_wnode> wnodes;
         std::vector<wr_tri> tris;
-        std::vector<uint32_t> leaf_up, leaf_cnt, node_up, rec_prim, rec_obj, rec_first, leaf_chain;
+        std::vector<uint32_t> leaf_up, leaf_cnt, node_up, rec_prim, rec_obj, rec_first;
         std::vector<wr_chain> chains;
         std::vector<float> rec_nrm;
         std::vector<wr_tree_hdr> hdrs;
@@ -392,25 +392,25 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
             leaf_up.insert(leaf_up.end(), F.leaf_up.begin(), F.leaf_up.end());
             leaf_cnt.insert(leaf_cnt.end(), F.leaf_cnt.begin(), F.leaf_cnt.end());
             rec_first.insert(rec_first.end(), F.rec_first.begin(), F.rec_first.end());
-            for (size_t rr = 0; rr < F.rec_first.size(); rr++) {  // chain record of every non-root leaf
-                uint32_t up = F.leaf_up[rr];
-                if (F.rec_first[rr] != rr || up == WR_NONE) {
-                    leaf_chain.push_back(WR_NONE);
-                    continue;
-                }
-                leaf_chain.push_back((uint32_t)chains.size());
+            // chain records, one per triangle record: the leaf's at its first record, the
+            // others (records after the first of a leaf) empty and never read
+            for (size_t rr = 0; rr < F.rec_first.size(); rr++) {
                 wr_chain c{};
-                while (up != WR_NONE && c.count < kChainLevels) {
-                    const wr_node& N = F.nodes[up >> 1];
-                    const float* b = (up & 1u) ? N.rmin : N.lmin;  // min then max, contiguous
-                    for (int k = 0; k < 3; k++) {
-                        c.box[c.count][k] = b[k];
-                        c.box[c.count][3 + k] = b[3 + k];
+                c.next = WR_NONE;
+                if (F.rec_first[rr] == rr) {
+                    uint32_t up = F.leaf_up[rr];
+                    while (up != WR_NONE && c.count < kChainLevels) {
+                        const wr_node& N = F.nodes[up >> 1];
+                        const float* b = (up & 1u) ? N.rmin : N.lmin;  // min then max, contiguous
+                        for (int k = 0; k < 3; k++) {
+                            c.box[c.count][k] = b[k];
+                            c.box[c.count][3 + k] = b[3 + k];
+                        }
+                        c.count++;
+                        up = F.node_up[up >> 1];
                     }
-                    c.count++;
-                    up = F.node_up[up >> 1];
+                    c.next = up;
                 }
-                c.next = up;
                 chains.push_back(c);
             }
             rec_prim.insert(rec_prim.end(), F.rec_prim.begin(), F.rec_prim.end());
@@ -571,7 +571,7 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
                 T.first = rec_first[g];
                 T.prim = rec_prim[g];
                 T.obj = rec_obj[g];
-                T.chain = leaf_chain[base + rec_first[g]];
+                T.chain = (uint32_t)(base + rec_first[g]);
             }
         }
         if ((rc = upload(ctx, tris, &d_tris, bytes))) return rc;

@@ -165,12 +165,16 @@ __global__ __launch_bounds__(256) void k_closest_hit_finish(SceneDev S, wr_tree_
             const RayPre r = make_ray(o, d, tr.scale);
             const NoSelf ns{};
             bool failed;
+            // the winner's record is fetched alongside the verification's first fetch
+            const uint32_t rec0 = h.rec;
+            wr_tri T{};
+            if (rec0 != WR_NONE) T = tr.tris[rec0];
             finish_hit<false, COUNT>(tr, r, ns, st, cnt, h, overflow, failed);
             if (h.rec != WR_NONE) {
+                if (h.rec != rec0) T = tr.tris[h.rec];
                 float u1, u2;
-                bary(tr.tris[h.rec], o, d, h.t, u1, u2);
-                out = make_uint4(tr.tris[h.rec].prim, __float_as_uint(h.t), __float_as_uint(u1),
-                                 __float_as_uint(u2));
+                bary(T, o, d, h.t, u1, u2);
+                out = make_uint4(T.prim, __float_as_uint(h.t), __float_as_uint(u1), __float_as_uint(u2));
             }
             if (failed) out.x = 0xFFFFFFFEu;  // stack overflow: reported, never silently wrong
         }

@@ -55,7 +55,7 @@ __device__ __forceinline__ Tree make_tree(const wr_tree_hdr& h, const SceneDev& 
     t.tris = S.tris + h.rec_base;
     t.leaf_up = S.leaf_up + h.rec_base;
     t.leaf_cnt = S.leaf_cnt + h.rec_base;
-    t.chains = S.chains;
+    t.chains = S.chains + h.rec_base;
     t.node_anc = S.node_anc + h.node_base;
     for (int a = 0; a < 3; a++) {
         t.rmin[a] = h.root_min[a];
@@ -611,10 +611,11 @@ __device__ __forceinline__ void traverse_wide(const Tree& tr, const RayPre& r, c
 
 // Re-check the winner's root-to-leaf chain with the reference box test.  The
 // ten levels nearest the leaf come from the leaf's chain record (two lines,
-// five levels each); above them the links come four at a time (node_anc), so
-// the box fetches of four levels are issued together instead of one dependent
-// fetch per level, each reading only the 24 B box of the chain's side of the
-// node.
+// five levels each, at the leaf's first record, so its fetch does not wait
+// for the winner's triangle record); above them the links come four at a
+// time (node_anc), so the box fetches of four levels are issued together
+// instead of one dependent fetch per level, each reading only the 24 B box of
+// the chain's side of the node.
 struct Box6 {
     float mn[3], mx[3];
 };
@@ -642,36 +643,34 @@ __device__ __forceinline__ bool chain_five(const RayPre& r, const float* p, uint
     return true;
 }
 
-// ci: the chain record of the winner's leaf (wr_tri::chain), WR_NONE for the root leaf.
-__device__ __forceinline__ bool verify_chain(const Tree& tr, const RayPre& r, uint32_t ci) {
-    if (ci != WR_NONE) {
-        const wr_chain* C = tr.chains + ci;
-        const uint32_t count = C->count;
-        uint32_t up = C->next;
-        if (!chain_five(r, &C->box[0][0], count)) return false;
-        if (count > 5 && !chain_five(r, &C->box[5][0], count - 5)) return false;
-        if (up != WR_NONE) {
-            {
-                const Box6 b = side_box(tr, up);
-                if (!ref_box(r, b.mn, b.mx)) return false;
-            }
-            uint32_t node = up >> 1;
-            for (;;) {
-                const uint4 a = tr.node_anc[node];
-                if (a.x == WR_NONE) break;
-                const Box6 b0 = side_box(tr, a.x);
-                const Box6 b1 = side_box(tr, a.y == WR_NONE ? a.x : a.y);
-                const Box6 b2 = side_box(tr, a.z == WR_NONE ? a.x : a.z);
-                const Box6 b3 = side_box(tr, a.w == WR_NONE ? a.x : a.w);
-                if (!ref_box(r, b0.mn, b0.mx)) return false;
-                if (a.y == WR_NONE) break;
-                if (!ref_box(r, b1.mn, b1.mx)) return false;
-                if (a.z == WR_NONE) break;
-                if (!ref_box(r, b2.mn, b2.mx)) return false;
-                if (a.w == WR_NONE) break;
-                if (!ref_box(r, b3.mn, b3.mx)) return false;
-                node = a.w >> 1;
-            }
+// first: the first record of the winner's reference leaf.
+__device__ __forceinline__ bool verify_chain(const Tree& tr, const RayPre& r, uint32_t first) {
+    const wr_chain* C = tr.chains + first;
+    const uint32_t count = C->count;
+    const uint32_t up = C->next;
+    if (!chain_five(r, &C->box[0][0], count)) return false;
+    if (count > 5 && !chain_five(r, &C->box[5][0], count - 5)) return false;
+    if (up != WR_NONE) {
+        {
+            const Box6 b = side_box(tr, up);
+            if (!ref_box(r, b.mn, b.mx)) return false;
+        }
+        uint32_t node = up >> 1;
+        for (;;) {
+            const uint4 a = tr.node_anc[node];
+            if (a.x == WR_NONE) break;
+            const Box6 b0 = side_box(tr, a.x);
+            const Box6 b1 = side_box(tr, a.y == WR_NONE ? a.x : a.y);
+            const Box6 b2 = side_box(tr, a.z == WR_NONE ? a.x : a.z);
+            const Box6 b3 = side_box(tr, a.w == WR_NONE ? a.x : a.w);
+            if (!ref_box(r, b0.mn, b0.mx)) return false;
+            if (a.y == WR_NONE) break;
+            if (!ref_box(r, b1.mn, b1.mx)) return false;
+            if (a.z == WR_NONE) break;
+            if (!ref_box(r, b2.mn, b2.mx)) return false;
+            if (a.w == WR_NONE) break;
+            if (!ref_box(r, b3.mn, b3.mx)) return false;
+            node = a.w >> 1;
         }
     }
     return ref_box(r, tr.rmin, tr.rmax);
@@ -684,7 +683,7 @@ __device__ __forceinline__ void finish_hit(const Tree& tr, const RayPre& r, cons
                                            Counters& cnt, HitState& h, bool overflow, bool& failed) {
     failed = false;
     if (overflow) traverse<NOSELF, false, COUNT>(tr, r, ns, st, h, overflow, cnt);
-    if (!overflow && (h.rec == WR_NONE || verify_chain(tr, r, tr.tris[h.rec].chain))) return;
+    if (!overflow && (h.rec == WR_NONE || verify_chain(tr, r, h.first))) return;
     if (COUNT) cnt.fallback++;
     traverse<NOSELF, true, COUNT>(tr, r, ns, st, h, overflow, cnt);
     failed = overflow;

@@ -40,8 +40,7 @@ static_assert(sizeof(wr_node) == 64, "node must be 64 bytes");
 // about the record once its triangle test accepts, so no side array is read on
 // the hot path: the first record of its reference leaf (the tie key), the
 // triangle id (global in the global tree, the object's own in an object tree),
-// the object, and the chain record of its reference leaf (WR_NONE when the leaf
-// is the root).
+// the object, and the global index of its reference leaf's chain record.
 struct wr_tri {
     float v0[3];
     float v1[3];
@@ -71,7 +70,9 @@ static_assert(sizeof(wr_wnode) == 64, "wide node must be 64 bytes");
 static_assert(__builtin_offsetof(wr_wnode, q) == 24 && __builtin_offsetof(wr_wnode, ref) == 48,
               "wide-node fields are read as four 16 B words");
 
-// 256 B chain record of one reference leaf (two L2 lines): the boxes that the
+// 256 B chain record of one reference leaf (two L2 lines), stored at the
+// index of the leaf's first triangle record (the winner's tie key, so it is
+// fetched alongside the winner's triangle record): the boxes that the
 // reference tests on the way to the leaf at its first up-to-ten ancestors,
 // leaf-side first (box[k] is the box of the ancestor-k child on the path,
 // stored in ancestor k + 1), their number, and the link (parent << 1 | side)
