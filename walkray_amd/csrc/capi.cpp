@@ -362,7 +362,7 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
         std::vector<wr_node> nodes;
         std::vector<wr_wnode> wnodes;
         std::vector<wr_tri> tris;
-        std::vector<uint32_t> leaf_up, leaf_cnt, node_up, rec_prim, rec_obj, rec_first;
+        std::vector<uint32_t> leaf_up, leaf_cnt, node_up, rec_prim, rec_obj, rec_first, leaf_chain;
         std::vector<wr_chain> chains;
         std::vector<float> rec_nrm;
         std::vector<wr_tree_hdr> hdrs;
@@ -392,25 +392,25 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
             leaf_up.insert(leaf_up.end(), F.leaf_up.begin(), F.leaf_up.end());
             leaf_cnt.insert(leaf_cnt.end(), F.leaf_cnt.begin(), F.leaf_cnt.end());
             rec_first.insert(rec_first.end(), F.rec_first.begin(), F.rec_first.end());
-            // chain records, one per triangle record: the leaf's at its first record, the
-            // others (records after the first of a leaf) empty and never read
-            for (size_t rr = 0; rr < F.rec_first.size(); rr++) {
-                wr_chain c{};
-                c.next = WR_NONE;
-                if (F.rec_first[rr] == rr) {
-                    uint32_t up = F.leaf_up[rr];
-                    while (up != WR_NONE && c.count < kChainLevels) {
-                        const wr_node& N = F.nodes[up >> 1];
-                        const float* b = (up & 1u) ? N.rmin : N.lmin;  // min then max, contiguous
-                        for (int k = 0; k < 3; k++) {
-                            c.box[c.count][k] = b[k];
-                            c.box[c.count][3 + k] = b[3 + k];
-                        }
-                        c.count++;
-                        up = F.node_up[up >> 1];
-                    }
-                    c.next = up;
+            for (size_t rr = 0; rr < F.rec_first.size(); rr++) {  // chain record of every non-root leaf
+                uint32_t up = F.leaf_up[rr];
+                if (F.rec_first[rr] != rr || up == WR_NONE) {
+                    leaf_chain.push_back(WR_NONE);
+                    continue;
                 }
+                leaf_chain.push_back((uint32_t)chains.size());
+                wr_chain c{};
+                while (up != WR_NONE && c.count < kChainLevels) {
+                    const wr_node& N = F.nodes[up >> 1];
+                    const float* b = (up & 1u) ? N.rmin : N.lmin;  // min then max, contiguous
+                    for (int k = 0; k < 3; k++) {
+                        c.box[c.count][k] = b[k];
+                        c.box[c.count][3 + k] = b[3 + k];
+                    }
+                    c.count++;
+                    up = F.node_up[up >> 1];
+                }
+                c.next = up;
                 chains.push_back(c);
             }
             rec_prim.insert(rec_prim.end(), F.rec_prim.begin(), F.rec_prim.end());
@@ -562,16 +562,16 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
         s->dbufs.push_back(d_nodes);
         if ((rc = upload(ctx, wnodes, &d_wnodes, bytes))) return rc;
         s->dbufs.push_back(d_wnodes);
-        // each record carries its tie key, triangle id, object and leaf chain
+        // each record carries its leaf key (the leaf's chain record), triangle id and object
         for (size_t ti = 0; ti < hdrs.size(); ti++) {
             const uint32_t base = hdrs[ti].rec_base;
             const size_t end = ti + 1 < hdrs.size() ? hdrs[ti + 1].rec_base : tris.size();
             for (size_t g = base; g < end; g++) {
                 wr_tri& T = tris[g];
-                T.first = rec_first[g];
+                T.leaf = leaf_chain[base + rec_first[g]];  // the leaf's chain record: tie key and verification
                 T.prim = rec_prim[g];
                 T.obj = rec_obj[g];
-                T.chain = (uint32_t)(base + rec_first[g]);
+                T.pad = 0;
             }
         }
         if ((rc = upload(ctx, tris, &d_tris, bytes))) return rc;

@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
                 active = false;
                 A.ht[pid] = h.t;
                 A.hrec[pid] = h.rec;
-                A.hobj[pid] = h.first;
+                A.hobj[pid] = h.leaf;
                 if (overflow) A.flags[pid] |= kOverflow;
             }
         }
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
             active = false;
             A.ht[pid] = h.t;
             A.hrec[pid] = h.rec;
-            A.hobj[pid] = h.first;
+            A.hobj[pid] = h.leaf;
             if (overflow) A.flags[pid] |= kOverflow;
         }
     }

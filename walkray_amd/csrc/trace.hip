@@ -121,12 +121,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
             const bool at_node = active && !(ref & WR_LEAF_BIT);
             if ((uint32_t)__popcll(__ballot(at_node)) < node_phase_min) break;
             if (at_node && !wide_step<false, COUNT>(tr, r, ns, st, h, ref, overflow, cnt)) {
-                hits[rid] = make_uint4(h.rec, __float_as_uint(h.t), h.first, overflow ? 1u : 0u);
+                hits[rid] = make_uint4(h.rec, __float_as_uint(h.t), h.leaf, overflow ? 1u : 0u);
                 active = false;
             }
         }
         if (active && !wide_step<false, COUNT>(tr, r, ns, st, h, ref, overflow, cnt)) {
-            hits[rid] = make_uint4(h.rec, __float_as_uint(h.t), h.first, overflow ? 1u : 0u);
+            hits[rid] = make_uint4(h.rec, __float_as_uint(h.t), h.leaf, overflow ? 1u : 0u);
             active = false;
         }
     }

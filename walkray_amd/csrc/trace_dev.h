@@ -55,7 +55,7 @@ __device__ __forceinline__ Tree make_tree(const wr_tree_hdr& h, const SceneDev& 
     t.tris = S.tris + h.rec_base;
     t.leaf_up = S.leaf_up + h.rec_base;
     t.leaf_cnt = S.leaf_cnt + h.rec_base;
-    t.chains = S.chains + h.rec_base;
+    t.chains = S.chains;
     t.node_anc = S.node_anc + h.node_base;
     for (int a = 0; a < 3; a++) {
         t.rmin[a] = h.root_min[a];
@@ -286,8 +286,8 @@ struct Counters {
 
 struct HitState {
     float t;
-    uint32_t rec;    // record index within the tree, WR_NONE on a miss
-    uint32_t first;  // first record of the winner's leaf (tie order)
+    uint32_t rec;   // record index within the tree, WR_NONE on a miss
+    uint32_t leaf;  // the winner's leaf key (wr_tri::leaf): tie order and chain record
 };
 
 // Per-lane stack of (reference, entry distance) pairs: the first D levels
@@ -377,24 +377,27 @@ __device__ __forceinline__ float prune_cut(const RayPre& r, float best_t) {
 }
 
 // Reference triangle tests of one leaf (kdtree.cpp:308-327), folded into h
-// with the total order (t, then larger first record, then earlier slot).
+// with the total order (t, then the larger leaf key, then the earlier slot).
+// The leaf key (wr_tri::leaf, the same in every record of a leaf) orders
+// leaves like their first records, the way kdtree.cpp:331-346 breaks ties.
 template <bool NOSELF, bool COUNT>
 __device__ __forceinline__ void visit_leaf(const Tree& tr, const RayPre& r, const NoSelf& ns, uint32_t ref,
                                            HitState& h, Counters& cnt) {
     const uint32_t first = ref & WR_FIRST_MASK;
     uint32_t c = (ref >> WR_CNT_SHIFT) & WR_CNT_MASK;
     if (c == 0) c = tr.leaf_cnt[first];
-    const bool tie_loses = h.rec != WR_NONE && first <= h.first;
     for (uint32_t k = 0; k < c; k++) {
         const wr_tri T = tr.tris[first + k];
         if (COUNT) cnt.tris++;
-        // tie_loses stays valid inside the leaf: an accepted slot makes h.first == first
-        const float t = ref_tri(T, r.o, r.d, h.t, tie_loses || (h.rec != WR_NONE && h.first == first));
+        // an equal t loses to a leaf with a larger key, and to an earlier slot of this leaf
+        // (once a slot of it is accepted, h.leaf is this leaf's key)
+        const bool tie_loses = h.rec != WR_NONE && T.leaf <= h.leaf;
+        const float t = ref_tri(T, r.o, r.d, h.t, tie_loses);
         if (t >= 0.f) {
             if (!NOSELF || noself_legal(ns, T, first + k, r.o, r.d, t)) {
                 h.t = t;
                 h.rec = first + k;
-                h.first = first;
+                h.leaf = T.leaf;
             }
         }
     }
@@ -409,13 +412,13 @@ __device__ __forceinline__ void visit_tri_rec(const Tree& tr, const RayPre& r, c
     if (COUNT) cnt.tris++;
     const float t = ref_tri(T, r.o, r.d, h.t, false);
     if (!(t >= 0.f)) return;
-    const uint32_t f = T.first;
+    const uint32_t f = T.leaf;
     // t <= h.t here; on equality the larger leaf key wins, then the earlier slot
-    const bool better = h.rec == WR_NONE || t < h.t || f > h.first || (f == h.first && rec < h.rec);
+    const bool better = h.rec == WR_NONE || t < h.t || f > h.leaf || (f == h.leaf && rec < h.rec);
     if (better && (!NOSELF || noself_legal(ns, T, rec, r.o, r.d, t))) {
         h.t = t;
         h.rec = rec;
-        h.first = f;
+        h.leaf = f;
     }
 }
 
@@ -426,7 +429,7 @@ __device__ __forceinline__ void traverse(const Tree& tr, const RayPre& r, const 
                                          HitState& h, bool& overflow, Counters& cnt) {
     h.t = __builtin_inff();
     h.rec = WR_NONE;
-    h.first = 0;
+    h.leaf = 0;
     overflow = false;
     st.sp = 0;
     if (tr.root_ref == WR_NONE) return;
@@ -496,7 +499,7 @@ __device__ __forceinline__ bool wide_begin(const Tree& tr, const RayPre& r, Stac
                                            uint32_t& ref) {
     h.t = __builtin_inff();
     h.rec = WR_NONE;
-    h.first = 0;
+    h.leaf = 0;
     st.sp = 0;
     ref = tr.wide_root;
     if (ref == WR_NONE) return false;
@@ -588,10 +591,10 @@ __device__ __forceinline__ bool wide_step(const Tree& tr, const RayPre& r, const
         T.v1[0] = __uint_as_float(d0.w); T.v1[1] = __uint_as_float(d1.x); T.v1[2] = __uint_as_float(d1.y);
         T.v2[0] = __uint_as_float(d1.z); T.v2[1] = __uint_as_float(d1.w); T.v2[2] = __uint_as_float(d2.x);
         T.n[0] = __uint_as_float(d2.y); T.n[1] = __uint_as_float(d2.z); T.n[2] = __uint_as_float(d2.w);
-        T.first = d3.x;
+        T.leaf = d3.x;
         T.prim = d3.y;
         T.obj = d3.z;
-        T.chain = d3.w;
+        T.pad = d3.w;
         visit_tri_rec<NOSELF, COUNT>(tr, r, ns, rec, T, h, cnt);
         return st.next(prune_cut(r, h.t), ref);
     }
@@ -611,11 +614,11 @@ __device__ __forceinline__ void traverse_wide(const Tree& tr, const RayPre& r, c
 
 // Re-check the winner's root-to-leaf chain with the reference box test.  The
 // ten levels nearest the leaf come from the leaf's chain record (two lines,
-// five levels each, at the leaf's first record, so its fetch does not wait
-// for the winner's triangle record); above them the links come four at a
-// time (node_anc), so the box fetches of four levels are issued together
-// instead of one dependent fetch per level, each reading only the 24 B box of
-// the chain's side of the node.
+// five levels each; the leaf key is its index, so its fetch does not wait for
+// the winner's triangle record); above them the links come four at a time
+// (node_anc), so the box fetches of four levels are issued together instead
+// of one dependent fetch per level, each reading only the 24 B box of the
+// chain's side of the node.
 struct Box6 {
     float mn[3], mx[3];
 };
@@ -643,9 +646,10 @@ __device__ __forceinline__ bool chain_five(const RayPre& r, const float* p, uint
     return true;
 }
 
-// first: the first record of the winner's reference leaf.
-__device__ __forceinline__ bool verify_chain(const Tree& tr, const RayPre& r, uint32_t first) {
-    const wr_chain* C = tr.chains + first;
+// leaf: the winner's leaf key, WR_NONE when the tree is a single leaf.
+__device__ __forceinline__ bool verify_chain(const Tree& tr, const RayPre& r, uint32_t leaf) {
+    if (leaf == WR_NONE) return ref_box(r, tr.rmin, tr.rmax);
+    const wr_chain* C = tr.chains + leaf;
     const uint32_t count = C->count;
     const uint32_t up = C->next;
     if (!chain_five(r, &C->box[0][0], count)) return false;
@@ -683,7 +687,7 @@ __device__ __forceinline__ void finish_hit(const Tree& tr, const RayPre& r, cons
                                            Counters& cnt, HitState& h, bool overflow, bool& failed) {
     failed = false;
     if (overflow) traverse<NOSELF, false, COUNT>(tr, r, ns, st, h, overflow, cnt);
-    if (!overflow && (h.rec == WR_NONE || verify_chain(tr, r, h.first))) return;
+    if (!overflow && (h.rec == WR_NONE || verify_chain(tr, r, h.leaf))) return;
     if (COUNT) cnt.fallback++;
     traverse<NOSELF, true, COUNT>(tr, r, ns, st, h, overflow, cnt);
     failed = overflow;

@@ -10,9 +10,9 @@
 //   * a child reference with bit 31 set is a leaf: bits 27..30 hold the
 //     triangle count (1..15; 0 = long leaf, count in leaf_cnt[first]) and bits
 //     0..26 the first triangle record;
-//   * triangle records (48 B) are stored in left-to-right leaf order, so the
-//     first-record index of a leaf orders leaves the way kdtree.cpp:331-346
-//     breaks ties (equal t -> right subtree -> larger first record).
+//   * triangle records (64 B) are stored in left-to-right leaf order, and
+//     leaf keys follow that order, so they order leaves the way
+//     kdtree.cpp:331-346 breaks ties (equal t -> right subtree -> larger key).
 #pragma once
 #include <stdint.h>
 
@@ -38,18 +38,20 @@ static_assert(sizeof(wr_node) == 64, "node must be 64 bytes");
 // vertices in reference order, the precomputed reference normal
 // normalize((v1-v0) x (v2-v1)) (kdtree.cpp:294-295), and what the queries need
 // about the record once its triangle test accepts, so no side array is read on
-// the hot path: the first record of its reference leaf (the tie key), the
-// triangle id (global in the global tree, the object's own in an object tree),
-// the object, and the global index of its reference leaf's chain record.
+// the hot path: the key of its reference leaf, the triangle id (global in the
+// global tree, the object's own in an object tree) and the object.  The leaf
+// key is the index of the leaf's chain record (WR_NONE when the leaf is the
+// root); chain records are numbered in left-to-right leaf order, so keys order
+// leaves like their first records, the way kdtree.cpp:331-346 breaks ties.
 struct wr_tri {
     float v0[3];
     float v1[3];
     float v2[3];
     float n[3];
-    uint32_t first;
+    uint32_t leaf;
     uint32_t prim;
     uint32_t obj;
-    uint32_t chain;
+    uint32_t pad;
 };
 static_assert(sizeof(wr_tri) == 64, "triangle record must be 64 bytes");
 
@@ -70,14 +72,13 @@ static_assert(sizeof(wr_wnode) == 64, "wide node must be 64 bytes");
 static_assert(__builtin_offsetof(wr_wnode, q) == 24 && __builtin_offsetof(wr_wnode, ref) == 48,
               "wide-node fields are read as four 16 B words");
 
-// 256 B chain record of one reference leaf (two L2 lines), stored at the
-// index of the leaf's first triangle record (the winner's tie key, so it is
-// fetched alongside the winner's triangle record): the boxes that the
-// reference tests on the way to the leaf at its first up-to-ten ancestors,
-// leaf-side first (box[k] is the box of the ancestor-k child on the path,
-// stored in ancestor k + 1), their number, and the link (parent << 1 | side)
-// of the next level up, WR_NONE when the chain ends at the root.  The first
-// line holds the header and the five levels nearest the leaf.
+// 256 B chain record of one non-root reference leaf (two L2 lines), indexed
+// by the leaf key: the boxes that the reference tests on the way to the leaf
+// at its first up-to-ten ancestors, leaf-side first (box[k] is the box of the
+// ancestor-k child on the path, stored in ancestor k + 1), their number, and
+// the link (parent << 1 | side) of the next level up, WR_NONE when the chain
+// ends at the root.  The first line holds the header and the five levels
+// nearest the leaf.
 constexpr uint32_t kChainLevels = 10;
 struct wr_chain {
     uint32_t count;
