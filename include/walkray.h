@@ -305,6 +305,26 @@ int wr_render_stage_stats(wr_ctx* ctx, wr_stage_stats* out);
 int wr_render_bdpt(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, const wr_render_params* params,
                    wr_film* film, wr_render_stats* stats);
 
+/* VCMTracer::renderPixels (vcmTracer.cpp:6-84) for passes [passes, passes +
+ * spp) of the film.  Per pass s: the merging radius wr_vcm_radius(base_radius,
+ * s); one light sub-path per pixel and the pixel's eye sub-path (the sub-paths
+ * of wr_render_bdpt); the light points (every non-delta vertex past the first
+ * of each light path, up to an emitter) after width * height default ones,
+ * in path order, and the hash grid over them (width * height cells,
+ * hashgrid.hxx:18-75); per pixel the vertex connections of wr_render_bdpt
+ * with vcmMisWeight (vcmTracer.cpp:88-173,221-320) and the vertex merges of
+ * every non-delta eye vertex before an emitter with the light points within
+ * the radius (vcmTracer.cpp:175-217), merged contributions divided by the
+ * pixel count; the pass's pixel values checked, vignetting-fixed and added to
+ * the film.  params as for wr_render_bdpt.  base_radius: the configuration's
+ * radius (config.cpp:560).  Reproducible up to the float atomics' summation
+ * order.  Static scenes only. */
+int wr_render_vcm(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, const wr_render_params* params,
+                  float base_radius, wr_film* film, wr_render_stats* stats);
+/* The merging radius of pass s: MAX(base_radius * sqrt(powf(s + 1, alpha - 1)),
+ * EPSILON) with alpha = 2/3 (vcmTracer.cpp:30, vcmTracer.h:39-40). */
+float wr_vcm_radius(float base_radius, uint32_t pass);
+
 /* ---- light sub-paths and the photon hash grid ------------------------------ */
 /* One Ray of a Path (ray.h:8-37) as the light-path estimators read it. */
 typedef struct {

@@ -1565,6 +1565,243 @@ struct HashGrid {
     }
 };
 
+/* ---- VCMTracer (vcmTracer.cpp:6-446) --------------------------------------- */
+/* One pass of VCMTracer::renderPixels with the pass's radius: the sub-paths and
+ * the vertex-connection half are BidirectionalPathTracer's (the same light and
+ * eye sub-paths, the same connectEdge / colour / visibility rules), with
+ * vcmMisWeight in place of connectingMisWeight; the vertex-merging half
+ * queries the light points around every non-delta eye vertex. */
+struct Vcm {
+    using PRay = Tracer::PRay;
+    const Tracer& tr;
+    float radius;
+    unsigned npix;
+
+    bool emissiveObj(unsigned o) const { return o != ORC_NONE && tr.s->objs[o].mat == ORC_EMISSIVE; }
+    /* insideObj && !contactObj (a point inside a medium; the camera is a contact) */
+    static bool inVolume(const PRay& r) { return r.inside != ORC_NONE && r.contact == ORC_NONE && !r.camera; }
+
+    /* connectEdge (vcmTracer.cpp:366-398): both directions re-aimed; for a
+     * connection (vm false) the light-side ray is edited as in BDPT, for a
+     * merge only the eye-side ray is.  The previous rays are references, read
+     * after the edits. */
+    bool connectEdge(std::vector<PRay>& path, int cl, bool vm) const {
+        PRay& lightRay = path[cl];
+        PRay& eyeRay = path[cl + 1];
+        if (!vm && (lightRay.delta || eyeRay.delta)) return false;
+        if (vm && eyeRay.delta) return false;
+        const int last = (int)path.size() - 1;
+        const PRay& preLight = path[cl - 1 > 0 ? cl - 1 : 0];
+        const PRay& preEye = path[cl + 2 < last ? cl + 2 : last];
+        lightRay.d = normalize(sub(eyeRay.o, lightRay.o));
+        eyeRay.d = neg(lightRay.d);
+        if (!vm) {
+            lightRay.dp = 1;
+            lightRay.delta = false;
+            const V lr = tr.evalBSDF(preLight, lightRay);
+            lightRay.rad = lr;
+            if (lightRay.contact != ORC_NONE && cl > 0)
+                lightRay.inside = dot(lightRay.n, lightRay.d) > 0 ? preLight.inside : lightRay.contact;
+        }
+        eyeRay.dp = 1;
+        eyeRay.delta = false;
+        const V er = tr.evalBSDF(preEye, eyeRay);
+        eyeRay.rad = er;
+        if (eyeRay.contact != ORC_NONE && cl < last - 1)
+            eyeRay.inside = dot(eyeRay.n, eyeRay.d) > 0 ? preEye.inside : eyeRay.contact;
+        return true;
+    }
+
+    /* connectingColorProb (vcmTracer.cpp:329-362); a merge (vm) also takes the
+     * merging disc, the area density of the merged vertex and, inside a medium,
+     * the sphere's extra length into the probability */
+    void colorProb(const std::vector<PRay>& path, int cl, bool vm, V& color, float& prob) const {
+        color = V(1, 1, 1);
+        prob = 1;
+        const int n = (int)path.size();
+        for (int i = 0; i < n; i++) {
+            color = mulv(color, path[i].rad);
+            float dist = 0.f;
+            if (i <= cl) {
+                const float l = length(sub(path[i].o, path[i + 1].o));
+                dist = l > EPS_D ? l : (float)EPS_D;
+                color = mulv(color, V(1, 1, 1));  /* radianceDecay: no volumes */
+            } else if (i > cl + 1) {
+                const float l = length(sub(path[i - 1].o, path[i].o));
+                dist = l > EPS_D ? l : (float)EPS_D;
+                color = mulv(color, V(1, 1, 1));
+            }
+            if (i == cl && i < n - 1) color = mul(color, Tracer::cosTerm(path[i]) * Tracer::cosTerm(path[i + 1]) / (dist * dist));
+            if (i != cl && i != cl + 1) color = mul(color, Tracer::cosTerm(path[i]));
+            prob *= path[i].dp * path[i].op;
+            if (vm && i == cl) {
+                prob = (float)((double)prob * (PI_D * (double)radius * (double)radius));
+                const float cosThere = Tracer::cosTerm(Tracer::linkVertex(path, cl, i + 1, true));
+                prob *= 1.0f * std::fabs(cosThere) / (dist * dist);  /* PdfWtoA */
+                prob *= 1.f;                                         /* evalOriginProbability of two temporaries */
+                if (inVolume(path[i + 1])) prob = (float)((double)prob * (4.0 / 3 * (double)radius));
+            }
+        }
+    }
+
+    /* vcmMisWeight (vcmTracer.cpp:221-320), expTerm 1, nVC = 1, nVM = pixels;
+     * every evalOriginProbability there compares two temporaries, so it is 1 */
+    float misWeight(const std::vector<PRay>& path, int cl, bool vm) const {
+        const int n = (int)path.size();
+        std::vector<double> direct(n), reverse(n);
+        std::vector<float> distRecord;
+        direct[0] = path[0].op;
+        for (int i = 1; i < n; i++) {
+            const float l = length(sub(path[i].o, path[i - 1].o));
+            const float dist = l > EPS_D ? l : (float)EPS_D;
+            distRecord.push_back(dist);
+            const float cosThere = Tracer::cosTerm(Tracer::linkVertex(path, cl, i, true));
+            direct[i] = direct[i - 1] * (double)(1.0f * std::fabs(cosThere) / (dist * dist));
+            if (path[i - 1].delta) continue;
+            float linkDirProb;
+            if (i > 1)
+                linkDirProb = tr.evalDirProb(Tracer::linkVertex(path, cl, i - 2, false),
+                                             Tracer::linkVertex(path, cl, i - 1, false), false);
+            else {
+                const PRay r = Tracer::linkVertex(path, cl, i - 1, false);
+                linkDirProb = tr.evalDirProb(r, r, true);
+            }
+            direct[i] *= linkDirProb * 1.f;
+        }
+        reverse[n - 1] = path[n - 1].op;
+        for (int i = n - 2; i >= 0; i--) {
+            const float dist = distRecord[i];
+            const float cosThere = Tracer::cosTerm(Tracer::linkVertex(path, cl, i, false));
+            reverse[i] = reverse[i + 1] * (double)(1.0f * std::fabs(cosThere) / (dist * dist));
+            if (path[i + 1].delta) continue;
+            float linkDirProb;
+            if (i < n - 2)
+                linkDirProb = tr.evalDirProb(Tracer::linkVertex(path, cl, i + 2, true),
+                                             Tracer::linkVertex(path, cl, i + 1, true), false);
+            else {
+                const PRay r = Tracer::linkVertex(path, cl, i + 1, true);
+                linkDirProb = tr.evalDirProb(r, r, true);
+            }
+            reverse[i] *= linkDirProb * 1.f;
+        }
+        const double nVC = 1.0, nVM = (double)npix;
+        double all = 0;
+        double self = cl == -1 ? reverse[0] : direct[cl] * reverse[cl + 1];
+        for (int i = 0; i < n - 1; i++) {
+            if (!path[i].delta && !path[i + 1].delta) all += direct[i] * reverse[i + 1] * nVC;
+            if (i < n - 2 && !path[i + 1].delta) {
+                double p = direct[i] * reverse[i + 1];
+                if (!path[i].delta) {
+                    float dp;
+                    if (i > 0)
+                        dp = tr.evalDirProb(Tracer::linkVertex(path, cl, i - 1, false),
+                                            Tracer::linkVertex(path, cl, i, false), false);
+                    else {
+                        const PRay r = Tracer::linkVertex(path, cl, i, false);
+                        dp = tr.evalDirProb(r, r, true);
+                    }
+                    p *= dp;
+                }
+                p *= 1.f;  /* evalOriginProbability */
+                const float cosThere = Tracer::cosTerm(Tracer::linkVertex(path, cl, i + 1, true));
+                p *= (double)(1.0f * std::fabs(cosThere) / (distRecord[i] * distRecord[i]));
+                p *= PI_D * (double)radius * (double)radius;
+                if (inVolume(path[i + 1])) p *= 4.0 / 3 * (double)radius;
+                if (vm && i == cl) self = p;
+                all += p * nVM;
+            }
+        }
+        if (Tracer::pathCanNotConnect(path)) all += reverse[0];
+        self *= vm ? nVM : nVC;
+        const double w = self / all;
+        if (!(w >= 0)) return 0.f;
+        return (float)w;
+    }
+
+    /* One pixel's share of a pass: throughputByConnecting (vcmTracer.cpp:88-173),
+     * then throughputByMerging (175-217) over the pass's light points (pts, with
+     * their source path and index; path -1 for the leading default points). */
+    void pixel(unsigned p, unsigned pass, const std::vector<std::vector<PRay>>& lightPaths,
+               const std::vector<V>& pts, const std::vector<std::pair<int, int>>& src, const HashGrid& grid,
+               V& own, std::vector<std::pair<unsigned, V>>& splats) const {
+        std::vector<PRay> eyePath;
+        tr.samplePath(eyePath, tr.cameraRay(p, pass), 0, p, pass);
+        const std::vector<PRay>& lightP = lightPaths[p];
+        own = V(0, 0, 0);
+        const int maxEye = (int)eyePath.size(), maxLight = (int)lightP.size();
+        for (int whole = 2; whole <= maxEye + maxLight; whole++) {
+            const int minL = whole - maxEye > 0 ? whole - maxEye : 0;
+            const int maxL = whole - 1 >= maxLight ? maxLight : whole - 1;
+            for (int lightLen = minL; lightLen <= maxL; lightLen++) {
+                const int eyeLen = whole - lightLen;
+                std::vector<PRay> path(whole);
+                for (int i = 0; i < lightLen; i++) path[i] = lightP[i];
+                for (int i = 0; i < eyeLen; i++) path[lightLen + i] = eyePath[eyeLen - i - 1];
+                const int cl = lightLen - 1;
+                if (!emissiveObj(path[0].contact)) continue;
+                if (lightLen == 0 && !Tracer::pathCanNotConnect(path)) continue;
+                if (eyeLen > 0 && lightLen > 0) {
+                    if (lightLen > 1 && emissiveObj(path[cl].contact)) continue;
+                    if (emissiveObj(path[cl + 1].contact)) continue;
+                    if (!connectEdge(path, cl, false)) continue;
+                    PRay vis = path[cl + 1];
+                    if (vis.camera) vis.contact = Tracer::kCameraObj;
+                    if (!tr.visible(vis, path[cl].o)) continue;
+                }
+                V color;
+                float prob;
+                colorProb(path, cl, false, color, prob);
+                if (prob == 0) continue;
+                if (length(color) == 0) continue;
+                const float w = misWeight(path, cl, false);
+                const V contrib = mul(divs(color, prob), w);
+                if (eyeLen > 1) {
+                    own = add(own, contrib);
+                } else {
+                    float rx, ry;
+                    tr.cam.toRaster(path[whole - 2].o, rx, ry);
+                    const int x = (int)rx, y = (int)ry;
+                    if (x >= 0 && y >= 0 && x < (int)tr.cam.W && y < (int)tr.cam.H) {
+                        const float sa = tr.cam.pixelSolidAngle((unsigned)(y * (int)tr.cam.W + x));
+                        const float factor = (float)(1.0 / sa);
+                        splats.push_back({(unsigned)(y * (int)tr.cam.W + x), mul(contrib, factor / (float)npix)});
+                    }
+                }
+            }
+        }
+        std::vector<int> near;
+        for (int i = 1; i < maxEye; i++) {
+            if (emissiveObj(eyePath[i].contact)) break;
+            if (eyePath[i].delta) continue;
+            V contrib(0, 0, 0);
+            grid.query(pts, eyePath[i].o, near);
+            for (int pi : near) {
+                const int k = src[pi].first, j = src[pi].second;
+                if (k < 0 || j < 0) continue;  /* a leading default point: no path */
+                const std::vector<PRay>& L = lightPaths[k];
+                std::vector<PRay> path(L.begin(), L.begin() + j);
+                for (int e = 0; e <= i; e++) path.push_back(eyePath[i - e]);
+                if (!connectEdge(path, j - 1, true)) continue;
+                V color;
+                float prob;
+                colorProb(path, j - 1, true, color, prob);
+                if (length(color) == 0 || prob == 0) continue;
+                const float w = misWeight(path, j - 1, true);
+                contrib = add(contrib, divs(mul(color, w), prob));
+            }
+            own = add(own, divs(contrib, (float)npix));
+        }
+    }
+};
+
+/* VCMTracer's radius of pass s (vcmTracer.cpp:30): MAX(baseRadius *
+ * sqrt(powf(s+1, alpha-1)), EPSILON) with alpha = 2/3 (vcmTracer.h:39). */
+static float vcmRadius(float base, uint32_t s) {
+    const float alpha = (float)(2.0 / 3);
+    const float r = base * std::sqrt(std::pow((float)(s + 1), alpha - 1.f));
+    return r > EPS_D ? r : (float)EPS_D;
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -2002,6 +2239,56 @@ void orc_render_bdpt(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t
     }
     restoreMotion(s);
 }
+
+void orc_render_vcm(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t seed, uint32_t pass0, uint32_t npass,
+                    float base_radius, double* sum, double* sumsq, int nthreads) {
+    Scene* s = (Scene*)sp;
+    Tracer tr{s, Cam(c), max_depth, seed};
+    const int64_t np = (int64_t)c->width * c->height;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    std::vector<V> iter(np);
+    std::vector<std::vector<std::pair<unsigned, V>>> splats(np);
+    std::vector<std::vector<Tracer::PRay>> lightPaths(np);
+    for (uint32_t pass = pass0; pass < pass0 + npass; pass++) {
+        const Vcm vcm{tr, vcmRadius(base_radius, pass), (unsigned)np};
+        /* step 1 (vcmTracer.cpp:36-57): one light path per pixel; its light points
+         * after the np default-constructed ones, in path order */
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+        for (int64_t p = 0; p < np; p++) lightPaths[p] = tr.lightPath((unsigned)p, pass);
+        std::vector<V> pts((size_t)np, V(0, 0, 0));
+        std::vector<std::pair<int, int>> src((size_t)np, {-1, -1});
+        for (int64_t p = 0; p < np; p++) {
+            const auto& L = lightPaths[p];
+            for (size_t i = 1; i < L.size(); i++) {
+                if (vcm.emissiveObj(L[i].contact)) break;
+                if (L[i].delta) continue;
+                pts.push_back(L[i].o);
+                src.push_back({(int)p, (int)i});
+            }
+        }
+        HashGrid grid;
+        grid.build(pts, vcm.radius, (size_t)np);
+        /* step 2 (vcmTracer.cpp:59-70) */
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+        for (int64_t p = 0; p < np; p++) {
+            splats[p].clear();
+            vcm.pixel((unsigned)p, pass, lightPaths, pts, src, grid, iter[p], splats[p]);
+        }
+        for (int64_t p = 0; p < np; p++)
+            for (const auto& sp2 : splats[p]) iter[sp2.first] = add(iter[sp2.first], sp2.second);
+        for (int64_t p = 0; p < np; p++) {
+            V v = iter[p];
+            if (!legal(v)) v = V(0, 0, 0);
+            v = tr.cam.fixVignetting(v, (unsigned)p);
+            for (int k = 0; k < 3; k++) {
+                sum[3 * p + k] += v[k];
+                sumsq[3 * p + k] += (double)v[k] * v[k];
+            }
+        }
+    }
+}
+
+float orc_vcm_radius(float base_radius, uint32_t pass) { return vcmRadius(base_radius, pass); }
 
 void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox(ctr, key, out); }
 

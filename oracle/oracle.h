@@ -146,6 +146,15 @@ void     orc_motion_interpolate(float t0, float t1, const float* start, const fl
  * orc_render. */
 void     orc_render_bdpt(void* s, const orc_camera* cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
                          uint32_t npass, double* sum, double* sumsq, int nthreads);
+/* VCMTracer::renderPixels (vcmTracer.cpp:6-84) for passes [pass0, pass0 + npass):
+ * per pass the radius orc_vcm_radius(base_radius, pass), one light path per
+ * pixel (BDPT's), its light points after width * height default ones, the hash
+ * grid over them (width * height cells), then per pixel the vertex connections
+ * (BDPT's with vcmMisWeight) and the vertex merges; each pass's pixel values
+ * checked, vignetting-fixed and added to sum / sumsq as orc_render_bdpt. */
+void     orc_render_vcm(void* s, const orc_camera* cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
+                        uint32_t npass, float base_radius, double* sum, double* sumsq, int nthreads);
+float    orc_vcm_radius(float base_radius, uint32_t pass);
 
 /* ---- Counter RNG (Philox-4x32-10, Salmon et al., SC'11) ---------------- */
 void     orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -2,8 +2,8 @@
 refraction and total internal reflection (refractiveObj.h:27-88, util.h:40-54),
 the camera's ray mapping and sample weight (camera.cpp:6-23,107-122,182-186,
 ray.cpp:52-58), isotropic Phong off the normal (isotropicPhong.h:20-38,
-glossyObj.h:39-62) and BidirectionalPathTracer with a glass sphere
-(bidirPathTracer.cpp:53-221).  Expected values come from tests/pin_cases.py
+glossyObj.h:39-62), BidirectionalPathTracer with a glass sphere
+(bidirPathTracer.cpp:53-221) and VCMTracer (vcmTracer.cpp:6-446).  Expected values come from tests/pin_cases.py
 (float64 physics and geometry), never from the oracle's own formulas."""
 import math
 
@@ -235,3 +235,48 @@ def test_bdpt_with_glass_converges_to_path_tracer():
     assert abs(diff) < 3.5 * se, (m1.sum(), m2.sum(), se)
     z = np.abs(m1 - m2) / np.sqrt(v1 + v2 + 1e-30)
     assert (z < 3.5).mean() > 0.97
+
+
+# --------------------------------------------------------------------- VCM ---
+def test_vcm_radius_schedule():
+    """radius = MAX(baseRadius * sqrt(powf(s+1, alpha-1)), EPSILON), alpha = 2/3
+    (vcmTracer.cpp:30, vcmTracer.h:39): base * (s+1)^(-1/6), never below 1e-5."""
+    for base in (0.01, 0.05, 2.0):
+        for s_ in (0, 1, 5, 63, 1000):
+            want = base * (s_ + 1) ** (-1.0 / 6.0)
+            assert abs(O.vcm_radius(base, s_) - want) <= 1e-6 * want, (base, s_)
+    assert O.vcm_radius(1e-7, 0) == np.float32(1e-5)
+    assert O.vcm_radius(1e-5, 10) == np.float32(1e-5)
+
+
+def test_vcm_tiny_radius_equals_bdpt():
+    """At the smallest radius the merging strategies weigh nothing (pi r^2 = 3e-10 in a
+    unit box) and no light point lies within reach: VCMTracer is then
+    BidirectionalPathTracer on the same sub-paths, connections and weights, pixel
+    for pixel up to the rounding of the weights' extra (vanishing) terms."""
+    objs, cam = S.cornell(k=1, width=16, height=16)
+    sc = O.Scene(objs)
+    a, _ = sc.render_bdpt(cam, 8, 8, 21)
+    b, _ = sc.render_vcm(cam, 8, 8, 21, 1e-7)
+    assert a.sum() > 0
+    assert np.allclose(a, b, rtol=1e-4, atol=1e-7), np.abs(a - b).max()
+
+
+def test_vcm_converges_to_bdpt():
+    """VCMTracer on the Cornell box with its glass sphere, at a radius where merging
+    carries a real share of the image (dropping the merges, leaving out the merge's
+    pi r^2 in its probability, or the pixel count in its own weight each darkens the
+    image by about 7%), agrees with BidirectionalPathTracer on the whole image: the
+    MIS weights over connections and merges sum to one.  The merge's blur is below
+    the tolerance at this radius (0.05 in a unit box, shrinking with the passes)."""
+    objs, cam = S.cornell(k=1, width=16, height=16)
+    sc = O.Scene(objs)
+    n = 800
+    s1, q1 = sc.render_vcm(cam, n, 12, 23, 0.05)
+    s2, q2 = sc.render_bdpt(cam, n, 12, 24)
+    m1, m2 = s1 / n, s2 / n
+    v1 = np.maximum(q1 / n - m1 * m1, 0) / n
+    v2 = np.maximum(q2 / n - m2 * m2, 0) / n
+    diff = m1.sum() - m2.sum()
+    se = math.sqrt(v1.sum() + v2.sum())
+    assert abs(diff) < 4 * se + 0.005 * m2.sum(), (m1.sum(), m2.sum(), se)

@@ -159,6 +159,8 @@ def lib():
         "wr_film_write": (i32, [vp, vp, vp, vp, u32]),
         "wr_render": (i32, [vp, vp, P(CameraDesc), P(RenderParams), vp, P(RenderStats)]),
         "wr_render_bdpt": (i32, [vp, vp, P(CameraDesc), P(RenderParams), vp, P(RenderStats)]),
+        "wr_render_vcm": (i32, [vp, vp, P(CameraDesc), P(RenderParams), C.c_float, vp, P(RenderStats)]),
+        "wr_vcm_radius": (C.c_float, [C.c_float, u32]),
         "wr_server_create": (i32, [vp, P(ServerConfig), P(vp)]),
         "wr_server_destroy": (None, [vp]),
         "wr_server_add_scene": (i32, [vp, vp, P(u32)]),
@@ -536,6 +538,22 @@ def render_bdpt(scene, cam, film, spp, max_depth, seed, queue=0):
     _check(lib().wr_render_bdpt(scene.ctx.h, scene.h, C.byref(cd), C.byref(prm), film.h, C.byref(st)), scene.ctx.h,
            "wr_render_bdpt")
     return {"paths": st.paths, "kernel_ms": st.kernel_ms}
+
+
+def render_vcm(scene, cam, film, spp, max_depth, seed, base_radius, queue=0):
+    """VCMTracer::renderPixels for `spp` more passes into `film` with the configuration's
+    radius `base_radius`; returns stats.  queue: visibility-queue entries (0 = automatic)."""
+    cd = CameraDesc.from_dict(cam)
+    prm = RenderParams(spp, max_depth, seed, 0, queue, 0, 0)
+    st = RenderStats()
+    _check(lib().wr_render_vcm(scene.ctx.h, scene.h, C.byref(cd), C.byref(prm), base_radius, film.h, C.byref(st)),
+           scene.ctx.h, "wr_render_vcm")
+    return {"paths": st.paths, "kernel_ms": st.kernel_ms}
+
+
+def vcm_radius(base_radius, pss):
+    """wr_vcm_radius: the merging radius of pass pss."""
+    return float(lib().wr_vcm_radius(base_radius, pss))
 
 
 HIT_DTYPE = np.dtype([("prim", np.uint32), ("t", np.float32), ("b1", np.float32), ("b2", np.float32)])

@@ -29,7 +29,9 @@
 // isLegalColor and fixVignetting and adds them to the Kahan film in pass
 // order.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
+#include "grid_dev.h"
 #include "pt_dev.h"
 #include "wr_pt.h"
 
@@ -234,11 +236,28 @@ __device__ bool connect_edge(const SceneDev& S, Joined& P) {
 // direction probability of those links (evalDirectionProbability ignores its
 // incoming ray), so the sweep keeps those per record and the two recurrences
 // run over them afterwards, in the reference's order.
+// With vt (VCMTracer) the colour and probability are connectingColorProb of
+// vcmTracer.cpp:329-362 (a merge also takes the merging disc, the area
+// density of the merged vertex and, inside a medium, the sphere's extra length)
+// and the weight is vcmMisWeight (vcmTracer.cpp:221-320, expTerm 1, nVC = 1):
+// the same two recurrences, with every merging strategy beside every
+// connection in the sum.
+struct VcmTerms {
+    float r;     // the pass's merging radius
+    double nvm;  // light sub-paths of the pass (one per pixel)
+    bool vm;     // a merge (VM), else a connection (VC)
+};
+
+// insideObj && !contactObj: a point inside a medium (the camera is a contact)
+__device__ __forceinline__ bool in_volume(const BVert& v) {
+    return v.inside != WR_NONE && v.contact == WR_NONE && !(v.flags & kVCamera);
+}
+
 __device__ float color_prob_weight(const SceneDev& S, const CamDev& cam, uint32_t pixel, const Joined& P, f3& color,
-                                   float& prob) {
+                                   float& prob, const VcmTerms* vt = nullptr) {
     float cos_f[kBdptMaxJoined], cos_b[kBdptMaxJoined], dp_f[kBdptMaxJoined], dp_b[kBdptMaxJoined];
     float dist[kBdptMaxJoined], op[kBdptMaxJoined];
-    uint8_t delta[kBdptMaxJoined];
+    uint8_t delta[kBdptMaxJoined], vol[kBdptMaxJoined];
     const int n = P.n, cl = P.cl;
     color = mk(1.f, 1.f, 1.f);
     prob = 1.f;
@@ -261,6 +280,7 @@ __device__ float color_prob_weight(const SceneDev& S, const CamDev& cam, uint32_
         dp_f[j] = eval_dir_prob(S, cam, pixel, f, f, j == 0);
         dp_b[j] = eval_dir_prob(S, cam, pixel, b, b, j == n - 1);
         delta[j] = (cur.flags & kVDelta) ? 1 : 0;
+        vol[j] = in_volume(cur) ? 1 : 0;
         op[j] = cur.op;
         if (j > 0) {
             const float l = len(sub(cur.o, prev.o));
@@ -275,6 +295,13 @@ __device__ float color_prob_weight(const SceneDev& S, const CamDev& cam, uint32_
         }
         if (j != cl && j != cl + 1) color = muls(color, cos_term_v(cur));
         prob *= cur.dp * cur.op;
+        if (vt && vt->vm && j == cl) {  // a merge: cl + 1 (the eye-side vertex) always exists
+            const float l = len(sub(cur.o, next.o));
+            const float d = (double)l > kEps ? l : (float)kEps;
+            prob = (float)((double)prob * (kPi * (double)vt->r * (double)vt->r));
+            prob *= 1.0f * fabsf(cos_term_v(next)) / (d * d);  // PdfWtoA at the eye-side vertex
+            if (in_volume(next)) prob = (float)((double)prob * (4.0 / 3 * (double)vt->r));
+        }
         prev = cur;
         cur = next;
     }
@@ -296,16 +323,62 @@ __device__ float color_prob_weight(const SceneDev& S, const CamDev& cam, uint32_
     }
     double all = 0.0;
     bool cannot = true;  // pathCanNotConnect (bidirPathTracer.cpp:251-257)
+    if (!vt) {
+        for (int i = 0; i < n - 1; i++) {
+            if (!delta[i] && !delta[i + 1]) {
+                all += direct[i] * reverse[i + 1];
+                cannot = false;
+            }
+        }
+        if (cannot) all += reverse[0];
+        const double self = cl == -1 ? reverse[0] : direct[cl] * reverse[cl + 1];
+        const double w = self / all;
+        return (float)(w > 0.0 ? w : 0.0);
+    }
+    double self = cl == -1 ? reverse[0] : direct[cl] * reverse[cl + 1];
     for (int i = 0; i < n - 1; i++) {
-        if (!delta[i] && !delta[i + 1]) {
-            all += direct[i] * reverse[i + 1];
+        if (!delta[i] && !delta[i + 1]) {  // VC
+            all += direct[i] * reverse[i + 1] * 1.0;
             cannot = false;
+        }
+        if (i < n - 2 && !delta[i + 1]) {  // VM at vertex i + 1
+            double p = direct[i] * reverse[i + 1];
+            if (!delta[i]) p *= dp_f[i];
+            p *= 1.f;  // evalOriginProbability of two temporaries
+            p *= (double)(1.0f * fabsf(cos_b[i + 1]) / (dist[i + 1] * dist[i + 1]));
+            p *= kPi * (double)vt->r * (double)vt->r;
+            if (vol[i + 1]) p *= 4.0 / 3 * (double)vt->r;
+            if (vt->vm && i == cl) self = p;
+            all += p * vt->nvm;
         }
     }
     if (cannot) all += reverse[0];
-    const double self = cl == -1 ? reverse[0] : direct[cl] * reverse[cl + 1];
+    self *= vt->vm ? vt->nvm : 1.0;
     const double w = self / all;
-    return (float)(w > 0.0 ? w : 0.0);
+    if (!(w >= 0.0)) return 0.f;
+    return (float)w;
+}
+
+// connectEdge for a merge (vcmTracer.cpp:366-398 with VM): both directions are
+// re-aimed, only the eye-side record is edited (radiance, probability, delta,
+// inside); its previous record is an eye record (the merged eye vertex is at
+// least the second of its path).
+__device__ bool connect_edge_vm(const SceneDev& S, Joined& P) {
+    P.lc = P.at(P.cl);
+    P.ec = P.at(P.cl + 1);
+    if (P.ec.flags & kVDelta) return false;
+    P.edited = true;
+    const int last = P.n - 1;
+    const int ie = P.cl + 2 < last ? P.cl + 2 : last;
+    P.lc.d = normalize(sub(P.ec.o, P.lc.o));
+    P.ec.d = mk(-P.lc.d.x, -P.lc.d.y, -P.lc.d.z);
+    P.ec.dp = 1.f;
+    P.ec.flags &= ~kVDelta;
+    const BVert pre = P.at(ie);
+    P.ec.rad = eval_bsdf(S, pre, P.ec);
+    if (P.ec.contact != WR_NONE && P.cl < last - 1)
+        P.ec.inside = dot(P.ec.n, P.ec.d) > 0.f ? pre.inside : P.ec.contact;
+    return true;
 }
 
 __device__ __forceinline__ bool emissive(const SceneDev& S, uint32_t obj) {
@@ -531,15 +604,212 @@ __global__ __launch_bounds__(256) void k_bdpt_fold(CamDev cam, uint32_t npass, u
     }
 }
 
+// ---- VCMTracer (vcmTracer.cpp:6-446) ---------------------------------------
+// One pass at a time (each pass has its own light points, radius and grid):
+// the sub-paths of k_bdpt_subpaths (one light path per pixel); the light
+// points (vcmTracer.cpp:42-55: after the pixel count of default-constructed
+// ones, every non-delta vertex past the first of each light path, up to an
+// emitter), path-major; HashGrid::Build over them; then k_vcm, one lane per
+// pixel: the connections of k_bdpt with vcmMisWeight, and the merges.
+
+// true: vertex i of a light path is a light point; stop: the path's points end there
+__device__ __forceinline__ bool light_point(const SceneDev& S, const BVert& v, bool& stop) {
+    stop = emissive(S, v.contact);
+    return !stop && !(v.flags & kVDelta);
+}
+
+__global__ __launch_bounds__(256) void k_vcm_point_count(SceneDev S, const BVert* verts, const uint32_t* counts,
+                                                         uint32_t npix, uint32_t maxv, uint32_t* pc) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= npix) return;
+    const BVert* L = verts + (size_t)k * 2 * maxv + maxv;
+    const uint32_t nl = min(counts[2 * k + 1], maxv);
+    uint32_t m = 0;
+    for (uint32_t i = 1; i < nl; i++) {
+        bool stop;
+        const bool keep = light_point(S, L[i], stop);
+        if (stop) break;
+        m += keep;
+    }
+    pc[k] = m;
+}
+
+// The points: the npix leading default ones (origin, no path), then each path's.
+__global__ __launch_bounds__(256) void k_vcm_point_write(SceneDev S, const BVert* verts, const uint32_t* counts,
+                                                         uint32_t npix, uint32_t maxv, const uint32_t* offs,
+                                                         float* pos, uint32_t* src) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= npix) return;
+    pos[3 * (size_t)k] = 0.f;
+    pos[3 * (size_t)k + 1] = 0.f;
+    pos[3 * (size_t)k + 2] = 0.f;
+    src[2 * (size_t)k] = WR_NONE;
+    src[2 * (size_t)k + 1] = WR_NONE;
+    const BVert* L = verts + (size_t)k * 2 * maxv + maxv;
+    const uint32_t nl = min(counts[2 * k + 1], maxv);
+    size_t j = (size_t)npix + offs[k];
+    for (uint32_t i = 1; i < nl; i++) {
+        bool stop;
+        const bool keep = light_point(S, L[i], stop);
+        if (stop) break;
+        if (!keep) continue;
+        pos[3 * j] = L[i].o.x;
+        pos[3 * j + 1] = L[i].o.y;
+        pos[3 * j + 2] = L[i].o.z;
+        src[2 * j] = k;
+        src[2 * j + 1] = i;
+        j++;
+    }
+}
+
+struct VcmGrid {
+    GridDev g;
+    const int* ends;
+    const uint32_t* idx;
+    const float* pos;
+    const uint32_t* src;
+    float r;
+};
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_vcm(SceneDev S, CamDev cam, uint32_t npix, uint32_t maxv, const BVert* verts,
+                                             const uint32_t* counts, float* pix, VisQueue Q, VcmGrid G) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= npix) return;
+    StackSpill spill;
+    LaneStack st(&spill);
+    const SceneCtx sc = make_scene_ctx(S);
+    const BVert* E = verts + (size_t)p * 2 * maxv;
+    const BVert* L = E + maxv;
+    const uint32_t ne = counts[2 * p], nl = counts[2 * p + 1];
+    const int max_eye = (int)(ne < maxv ? ne : maxv), max_light = (int)(nl < maxv ? nl : maxv);
+    const VcmTerms vc{G.r, (double)npix, false};
+    f3 own = mk(0.f, 0.f, 0.f);
+    // throughputByConnecting (vcmTracer.cpp:88-173): k_bdpt's loop with vcmMisWeight
+    for (int whole = 2; whole <= max_eye + max_light; whole++) {
+        const int min_l = whole - max_eye > 0 ? whole - max_eye : 0;
+        const int max_l = whole - 1 >= max_light ? max_light : whole - 1;
+        for (int light_len = min_l; light_len <= max_l; light_len++) {
+            Joined P;
+            P.L = L;
+            P.E = E;
+            P.nl = light_len;
+            P.ne = whole - light_len;
+            P.n = whole;
+            P.cl = light_len - 1;
+            P.edited = false;
+            if (!emissive(S, P.at(0).contact)) continue;
+            if (light_len == 0 && !cannot_connect(P)) continue;
+            const bool connected = P.ne > 0 && light_len > 0;
+            if (connected) {
+                if (light_len > 1 && emissive(S, P.at(P.cl).contact)) continue;
+                if (emissive(S, P.at(P.cl + 1).contact)) continue;
+                if (!connect_edge(S, P)) continue;
+            }
+            f3 color;
+            float prob;
+            const float w = color_prob_weight(S, cam, p, P, color, prob, &vc);
+            if (prob == 0.f) continue;
+            if (len(color) == 0.f) continue;
+            f3 contrib = muls(divs(color, prob), w);
+            uint32_t target = p;
+            if (P.ne <= 1) {  // light tracing: splat to the pixel the camera sees the point in
+                const f3 v = sub(P.at(whole - 2).o, cam.pos);
+                const float vfd = dot(v, cam.front);
+                const float scale = cam.dist / vfd;
+                const float rx = (float)(cam.W / 2.0 + (double)(dot(v, cam.right) * scale));
+                const float ry = (float)(cam.H / 2.0 - (double)(dot(v, cam.top) * scale));
+                const int x = (int)rx, y = (int)ry;  // checkPixelBound truncates (camera.cpp:164-170)
+                if (!(x >= 0 && y >= 0 && x < (int)cam.W && y < (int)cam.H)) continue;
+                target = (uint32_t)(y * (int)cam.W + x);
+                const float factor = (float)(1.0 / (double)pixel_solid_angle(cam, target));
+                contrib = muls(contrib, factor / (float)npix);
+            }
+            if (!connected) {
+                if (target == p) own = add(own, contrib);
+                else atomic_add3(pix, target, contrib);
+                continue;
+            }
+            const uint32_t contact = (P.ec.flags & kVCamera) ? kCameraObj : P.ec.contact;
+            const float dist = len(sub(P.ec.o, P.lc.o));
+            const uint32_t slot = wave_slot(Q.count);
+            if (slot < Q.cap) {
+                uint32_t* r = Q.rays + 8 * (size_t)slot;
+                r[0] = __float_as_uint(P.ec.o.x); r[1] = __float_as_uint(P.ec.o.y); r[2] = __float_as_uint(P.ec.o.z);
+                r[3] = __float_as_uint(P.ec.d.x); r[4] = __float_as_uint(P.ec.d.y); r[5] = __float_as_uint(P.ec.d.z);
+                r[6] = P.ec.inside;
+                r[7] = contact;
+                float* y = Q.pay + 4 * (size_t)slot;
+                y[0] = contrib.x; y[1] = contrib.y; y[2] = contrib.z; y[3] = dist;
+                Q.dst[slot] = target;
+            } else {  // queue full: trace here
+                const PHit h = trace_scene<true>(sc, P.ec.o, P.ec.d, P.ec.inside, contact, st);
+                if (!nee_visible(h.rec, h.t, dist)) continue;
+                if (target == p) own = add(own, contrib);
+                else atomic_add3(pix, target, contrib);
+            }
+        }
+    }
+    // throughputByMerging (vcmTracer.cpp:175-217): the light points within the
+    // radius of every non-delta eye vertex before an emitter, in the grid's
+    // order (HashGrid::Process); the default points have no path and give nothing
+    const VcmTerms vm{G.r, (double)npix, true};
+    for (int i = 1; i < max_eye; i++) {
+        const BVert ev = E[i];
+        if (emissive(S, ev.contact)) break;
+        if (ev.flags & kVDelta) continue;
+        f3 contrib = mk(0.f, 0.f, 0.f);
+        int pc[6];
+        if (query_corner(G.g, ev.o, pc)) {
+            for (int c8 = 0; c8 < 8; c8++) {
+                const uint32_t cell = query_cell(G.g, pc, c8);
+                int a = cell == 0 ? 0 : G.ends[cell - 1];
+                const int b = G.ends[cell];
+                // the default points lead their cell's range (the sort keeps input order in a cell)
+                int hi = b;
+                while (a < hi) {
+                    const int mid = (a + hi) >> 1;
+                    if (G.idx[mid] < npix) a = mid + 1;
+                    else hi = mid;
+                }
+                for (; a < b; a++) {
+                    const uint32_t pi = G.idx[a];
+                    if (!grid_near(G.g, G.pos, ev.o, pi)) continue;
+                    const uint32_t k = G.src[2 * (size_t)pi], j = G.src[2 * (size_t)pi + 1];
+                    Joined P;
+                    P.L = verts + (size_t)k * 2 * maxv + maxv;
+                    P.E = E;
+                    P.nl = (int)j;
+                    P.ne = i + 1;
+                    P.n = (int)j + i + 1;
+                    P.cl = (int)j - 1;
+                    P.edited = false;
+                    if (!connect_edge_vm(S, P)) continue;
+                    f3 color;
+                    float prob;
+                    const float w = color_prob_weight(S, cam, p, P, color, prob, &vm);
+                    if (len(color) == 0.f || prob == 0.f) continue;
+                    contrib = add(contrib, divs(muls(color, w), prob));
+                }
+            }
+        }
+        own = add(own, divs(contrib, (float)npix));
+    }
+    atomic_add3(pix, p, own);
+}
+
 }  // namespace wrd
 
 namespace wr {
 
 size_t bdpt_vertex_bytes() { return sizeof(wrd::BVert); }
 
-hipError_t launch_bdpt_passes(const SceneDev& S, const wr_tree_hdr& H, const CamHost& ch, uint32_t max_depth,
-                              uint64_t seed, uint32_t pass0, uint32_t npass, uint32_t maxv, void* verts, float* pix,
-                              const BdptQueue& q, const FilmBufs& film, hipStream_t st) {
+float vcm_radius(float base, uint32_t pass) {
+    const float alpha = (float)(2.0 / 3);  // vcmTracer.h:39
+    const float r = base * std::sqrt(std::pow((float)(pass + 1), alpha - 1.f));
+    return (double)r > 1e-5 ? r : (float)1e-5;
+}
+
+static wrd::CamDev make_cam_dev(const CamHost& ch) {
     wrd::CamDev cam;
     cam.pos = wrd::f3{ch.pos[0], ch.pos[1], ch.pos[2]};
     cam.front = wrd::f3{ch.front[0], ch.front[1], ch.front[2]};
@@ -548,6 +818,70 @@ hipError_t launch_bdpt_passes(const SceneDev& S, const wr_tree_hdr& H, const Cam
     cam.dist = ch.dist;
     cam.W = ch.W;
     cam.H = ch.H;
+    return cam;
+}
+
+int launch_vcm_pass(wr_ctx* ctx, const SceneDev& S, const wr_tree_hdr& H, const CamHost& ch, uint32_t max_depth,
+                    uint64_t seed, uint32_t pass, float radius, uint32_t maxv, void* verts, float* pix,
+                    const BdptQueue& q, const VcmBufs& vb, const FilmBufs& film) {
+    hipStream_t st = ctx->stream;
+    const wrd::CamDev cam = make_cam_dev(ch);
+    const uint32_t npix = ch.W * ch.H;
+    const uint32_t blocks = (npix + 255) / 256;
+    wrd::BVert* V = (wrd::BVert*)verts;
+    hipError_t e = hipMemsetAsync(pix, 0, sizeof(float) * 3 * (size_t)npix, st);
+    if (e == hipSuccess) e = hipMemsetAsync(q.count, 0, sizeof(uint32_t), st);
+    if (e == hipSuccess) e = hipMemsetAsync(vb.pc + npix, 0, sizeof(uint32_t), st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "wr_render_vcm");
+    wrd::k_bdpt_subpaths<<<blocks, 256, wrd::kStackLdsBytes, st>>>(S, cam, max_depth, seed, pass, 1, npix, maxv, V,
+                                                                  q.counts);
+    wrd::k_vcm_point_count<<<blocks, 256, 0, st>>>(S, V, q.counts, npix, maxv, vb.pc);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "wr_render_vcm");
+    size_t tb = vb.scan_bytes;
+    if ((e = hipcub::DeviceScan::ExclusiveSum(vb.scan, tb, vb.pc, vb.offs, (int)npix + 1, st)) != hipSuccess)
+        return hip_fail(ctx, e, "wr_render_vcm: scan");
+    uint32_t total = 0;
+    if ((e = hipMemcpyAsync(&total, vb.offs + npix, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+        (e = hipStreamSynchronize(st)) != hipSuccess)
+        return hip_fail(ctx, e, "wr_render_vcm");
+    const uint64_t npts = (uint64_t)npix + total;
+    if (npts > vb.cap) return fail(ctx, WR_EINVAL, "wr_render_vcm: more light points than the buffer holds");
+    wrd::k_vcm_point_write<<<blocks, 256, 0, st>>>(S, V, q.counts, npix, maxv, vb.offs, vb.pos, vb.src);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "wr_render_vcm");
+    // HashGrid::Reserve(pixels) + Build(light points, radius) (vcmTracer.cpp:33,57)
+    wr_hashgrid* grid = nullptr;
+    int rc = wr_hashgrid_build(ctx, vb.pos, npts, radius, npix, &grid);
+    if (rc != WR_OK) return rc;
+    const wrd::VcmGrid G{grid->g, grid->ends, grid->idx, vb.pos, vb.src, radius};
+    const wrd::VisQueue Q{q.rays, q.pay, q.dst, q.count, q.cap};
+    wrd::k_vcm<<<blocks, 256, wrd::kStackLdsBytes, st>>>(S, cam, npix, maxv, V, q.counts, pix, Q, G);
+    uint32_t nq = 0;
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&nq, q.count, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    wr_hashgrid_destroy(grid);
+    if (e != hipSuccess) return hip_fail(ctx, e, "wr_render_vcm");
+    if (nq > q.cap) nq = q.cap;
+    if (nq) {
+        if ((e = launch_closest_hit_path(S, H, q.rays, nq, q.hits, st)) != hipSuccess)
+            return hip_fail(ctx, e, "wr_render_vcm: visibility");
+        wrd::k_bdpt_accum<<<(nq + 255) / 256, 256, 0, st>>>(nq, q.pay, q.dst, (const uint4*)q.hits, pix);
+    }
+    wrd::k_bdpt_fold<<<blocks, 256, 0, st>>>(cam, 1, npix, pix, wrd::FilmDev{film.sum, film.comp, film.sumsq});
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "wr_render_vcm");
+    return WR_OK;
+}
+
+size_t vcm_scan_bytes(uint32_t npix) {
+    size_t tb = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)npix + 1);
+    return tb;
+}
+
+hipError_t launch_bdpt_passes(const SceneDev& S, const wr_tree_hdr& H, const CamHost& ch, uint32_t max_depth,
+                              uint64_t seed, uint32_t pass0, uint32_t npass, uint32_t maxv, void* verts, float* pix,
+                              const BdptQueue& q, const FilmBufs& film, hipStream_t st) {
+    const wrd::CamDev cam = make_cam_dev(ch);
     const uint32_t npix = ch.W * ch.H;
     const uint64_t lanes = (uint64_t)npass * npix;
     hipError_t e = hipMemsetAsync(pix, 0, sizeof(float) * 3 * (size_t)lanes, st);

@@ -21,6 +21,7 @@
 #include <mutex>
 #include <vector>
 
+#include "grid_dev.h"
 #include "pt_dev.h"
 #include "wr_capi_internal.h"
 
@@ -240,19 +241,6 @@ __global__ __launch_bounds__(256) void k_grid_bbox(const float* pos, uint64_t n,
     }
 }
 
-struct GridDev {
-    float bmin[3], bmax[3];
-    float inv;     // mInvCellSize
-    float r2;      // mRadius * mRadius
-    uint32_t n_cells;
-};
-
-// GetCellIndex(Vec3i) (hashgrid.hxx:170-178)
-__device__ __forceinline__ uint32_t grid_hash(const GridDev& g, int cx, int cy, int cz) {
-    const uint32_t x = (uint32_t)cx, y = (uint32_t)cy, z = (uint32_t)cz;
-    return ((x * 73856093u) ^ (y * 19349663u) ^ (z * 83492791u)) % g.n_cells;
-}
-
 __global__ __launch_bounds__(256) void k_grid_keys(GridDev g, const float* pos, uint32_t n, uint32_t* keys,
                                                    uint32_t* vals, uint32_t* cell_count) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -278,37 +266,9 @@ __global__ __launch_bounds__(256) void k_grid_keys(GridDev g, const float* pos, 
 // query at a time (a crowded cell, or one that shares a hash bucket with it).
 constexpr int kLongRange = 64;
 
-__device__ __forceinline__ bool grid_near(const GridDev& g, const float* pos, f3 qp, uint32_t pi) {
-    const f3 dv = mk(qp.x - pos[3 * pi], qp.y - pos[3 * pi + 1], qp.z - pos[3 * pi + 2]);
-    const float dist = len(dv);
-    return dist * dist <= g.r2;
-}
-
 // Ranges longer than this hand their whole query to k_grid_query_wide (one
 // workgroup per query), so that no wave walks such a range alone.
 constexpr int kHugeRange = 2048;
-
-// The query's cell corner and its nearer neighbours (HashGrid::Process); false
-// outside the grid's box.
-__device__ __forceinline__ bool query_corner(const GridDev& g, f3 qp, int p[6]) {
-    const f3 dmin = mk(qp.x - g.bmin[0], qp.y - g.bmin[1], qp.z - g.bmin[2]);
-    const f3 dmax = mk(g.bmax[0] - qp.x, g.bmax[1] - qp.y, g.bmax[2] - qp.z);
-    if (dmin.x < 0.f || dmin.y < 0.f || dmin.z < 0.f || dmax.x < 0.f || dmax.y < 0.f || dmax.z < 0.f) return false;
-    const f3 cp = mk(g.inv * dmin.x, g.inv * dmin.y, g.inv * dmin.z);
-    const f3 cf = mk(floorf(cp.x), floorf(cp.y), floorf(cp.z));
-    p[0] = (int)cf.x;
-    p[1] = (int)cf.y;
-    p[2] = (int)cf.z;
-    p[3] = p[0] + (cp.x - cf.x < 0.5f ? -1 : +1);
-    p[4] = p[1] + (cp.y - cf.y < 0.5f ? -1 : +1);
-    p[5] = p[2] + (cp.z - cf.z < 0.5f ? -1 : +1);
-    return true;
-}
-
-// Cell j of the query's 8, in the reference's order.
-__device__ __forceinline__ uint32_t query_cell(const GridDev& g, const int p[6], int j) {
-    return grid_hash(g, (j & 4) ? p[3] : p[0], (j & 2) ? p[4] : p[1], (j & 1) ? p[5] : p[2]);
-}
 
 // defer: [0] the number of deferred queries, then their indices.
 __global__ __launch_bounds__(256) void k_grid_query(GridDev g, const float* pos, const int* ends, const uint32_t* idx,
@@ -498,16 +458,6 @@ static uint32_t* noalloc_ptr(int device) {
 }
 }  // namespace wr
 
-struct wr_hashgrid {
-    wr_ctx* ctx = nullptr;
-    wrd::GridDev g{};
-    const float* pos = nullptr;  // the caller's points
-    uint64_t n = 0;
-    int* ends = nullptr;         // n_cells
-    uint32_t* idx = nullptr;     // n
-    uint32_t* defer = nullptr;   // query calls: deferred-query counter and list (grown on demand)
-    uint64_t defer_cap = 0;      // entries of defer
-};
 
 using namespace wrd;
 using wr::fail;

@@ -324,6 +324,101 @@ extern "C" int wr_render_bdpt(wr_ctx* ctx, const wr_scene* s, const wr_camera* c
     return WR_OK;
 }
 
+extern "C" float wr_vcm_radius(float base_radius, uint32_t pass) { return vcm_radius(base_radius, pass); }
+
+extern "C" int wr_render_vcm(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_render_params* prm,
+                             float base_radius, wr_film* film, wr_render_stats* stats) {
+    if (!ctx || !s || !cam || !prm || !film) return fail(ctx, WR_EINVAL, "wr_render_vcm: bad argument");
+    if (!s->built) return fail(ctx, WR_ESTATE, "wr_render_vcm: scene not built");
+    if (film->W != cam->width || film->H != cam->height)
+        return fail(ctx, WR_EINVAL, "wr_render_vcm: film size differs from the camera resolution");
+    if (prm->mode != 0 || prm->wave_grid || prm->flags)
+        return fail(ctx, WR_EINVAL, "wr_render_vcm: mode, wave_grid and flags must be 0");
+    if (prm->max_depth > kBdptMaxDepth) return fail(ctx, WR_EINVAL, "wr_render_vcm: max_depth above 32");
+    if (!(base_radius > 0.f) || !std::isfinite(base_radius))
+        return fail(ctx, WR_EINVAL, "wr_render_vcm: base_radius must be finite and > 0");
+    for (const HostObject& o : s->objs)
+        if (o.moving) return fail(ctx, WR_EINVAL, "wr_render_vcm: moving objects are not supported");
+    const uint64_t npix = (uint64_t)cam->width * cam->height;
+    const uint32_t maxv = prm->max_depth + 3;
+    // light points: npix default ones plus at most maxv - 1 per light path
+    const uint64_t cap = npix * maxv;
+    if (npix == 0 || cap >= (1ull << 31)) return fail(ctx, WR_EINVAL, "wr_render_vcm: bad image size");
+    if (film->broken) return film_broken(film, "wr_render_vcm");
+    (void)hipSetDevice(ctx->device);
+    const CamHost ch = make_cam(*cam);
+    const uint64_t per_pass = bdpt_vertex_bytes() * 2 * maxv * npix;
+    BdptQueue q{};
+    q.cap = prm->wave_slots ? prm->wave_slots : (uint32_t)std::min<uint64_t>(32ull * npix, 24ull << 20);
+    VcmBufs vb{};
+    vb.cap = cap;
+    vb.scan_bytes = vcm_scan_bytes((uint32_t)npix);
+    void *verts = nullptr, *qbuf = nullptr;
+    float* pix = nullptr;
+    hipError_t e = hipMalloc(&verts, per_pass);
+    if (e == hipSuccess) e = hipMalloc((void**)&pix, sizeof(float) * 3 * npix);
+    if (e == hipSuccess) e = hipMalloc(&qbuf, kBdptQueueEntryBytes * (size_t)q.cap + 256);
+    if (e == hipSuccess) e = hipMalloc((void**)&q.counts, sizeof(uint32_t) * 2 * npix);
+    if (e == hipSuccess) e = hipMalloc((void**)&vb.pc, sizeof(uint32_t) * (npix + 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&vb.offs, sizeof(uint32_t) * (npix + 1));
+    if (e == hipSuccess) e = hipMalloc(&vb.scan, std::max<size_t>(vb.scan_bytes, 4));
+    if (e == hipSuccess) e = hipMalloc((void**)&vb.pos, sizeof(float) * 3 * cap);
+    if (e == hipSuccess) e = hipMalloc((void**)&vb.src, sizeof(uint32_t) * 2 * cap);
+    if (e == hipSuccess) {
+        char* b = (char*)qbuf;
+        q.rays = (uint32_t*)b;
+        q.hits = b + 32 * (size_t)q.cap;
+        q.pay = (float*)(b + 48 * (size_t)q.cap);
+        q.dst = (uint32_t*)(b + 64 * (size_t)q.cap);
+        q.count = (uint32_t*)(b + 68 * (size_t)q.cap + 128);
+    }
+    auto release = [&]() {
+        (void)hipFree(verts);
+        (void)hipFree(pix);
+        (void)hipFree(qbuf);
+        (void)hipFree(q.counts);
+        (void)hipFree(vb.pc);
+        (void)hipFree(vb.offs);
+        (void)hipFree(vb.scan);
+        (void)hipFree(vb.pos);
+        (void)hipFree(vb.src);
+    };
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+    if (e != hipSuccess) {
+        release();
+        return hip_fail(ctx, e, "wr_render_vcm");
+    }
+    const FilmBufs fb{film->sum, film->comp, film->sumsq};
+    for (uint32_t k = 0; k < prm->spp; k++) {
+        const uint32_t pass = film->passes + k;
+        const int rc = launch_vcm_pass(ctx, s->dev, s->hdrs[0], ch, prm->max_depth, prm->seed, pass,
+                                       vcm_radius(base_radius, pass), maxv, verts, pix, q, vb, fb);
+        if (rc != WR_OK) {  // the film holds k whole passes and part of one more
+            (void)hipStreamSynchronize(ctx->stream);
+            release();
+            film->broken = true;
+            return rc;
+        }
+    }
+    e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    release();
+    if (e != hipSuccess) {
+        film->broken = film->broken || prm->spp > 0;
+        return hip_fail(ctx, e, "wr_render_vcm");
+    }
+    film->passes += prm->spp;
+    if (stats) {
+        stats->paths = npix * prm->spp;
+        stats->iterations = 0;
+        stats->kernel_ms = ms;
+        stats->build_ms = 0.f;
+    }
+    return WR_OK;
+}
+
 extern "C" int wr_set_render_tuning(wr_ctx* ctx, const wr_render_tuning* t) {
     if (!ctx || !t) return fail(ctx, WR_EINVAL, "wr_set_render_tuning: bad argument");
     const uint32_t lv = t->lds_levels ? t->lds_levels : 8u;

@@ -50,6 +50,26 @@ hipError_t noalloc_scratch_init(int device);
 // wr_scene_build; the trees of objects that do not move are copied from
 // `reuse` (nullable) when it is the same scene, built.
 int scene_build(wr_scene* s, const wr_scene* reuse);
+
+// VCMTracer (bdpt.hip): the merging radius of a pass (vcmTracer.cpp:30), and
+// one pass [sub-paths, light points, hash grid, connections and merges, film]
+// on the context stream.  verts / pix / q as for launch_bdpt_passes with one
+// pass; vb: the light-point buffers (cap points of 3 floats and 2 words), the
+// per-pixel counts and offsets (pixels + 1 each) and the scan's scratch.
+struct VcmBufs {
+    uint32_t* pc;
+    uint32_t* offs;
+    void* scan;
+    size_t scan_bytes;
+    float* pos;
+    uint32_t* src;
+    uint64_t cap;
+};
+float vcm_radius(float base, uint32_t pass);
+size_t vcm_scan_bytes(uint32_t npix);
+int launch_vcm_pass(wr_ctx* ctx, const SceneDev& S, const wr_tree_hdr& H, const CamHost& ch, uint32_t max_depth,
+                    uint64_t seed, uint32_t pass, float radius, uint32_t maxv, void* verts, float* pix,
+                    const BdptQueue& q, const VcmBufs& vb, const FilmBufs& film);
 }  // namespace wr
 
 struct wr_scene {
