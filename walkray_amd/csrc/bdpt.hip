@@ -31,6 +31,8 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <vector>
+
 #include "grid_dev.h"
 #include "pt_dev.h"
 #include "wr_pt.h"
@@ -605,12 +607,13 @@ __global__ __launch_bounds__(256) void k_bdpt_fold(CamDev cam, uint32_t npass, u
 }
 
 // ---- VCMTracer (vcmTracer.cpp:6-446) ---------------------------------------
-// One pass at a time (each pass has its own light points, radius and grid):
-// the sub-paths of k_bdpt_subpaths (one light path per pixel); the light
-// points (vcmTracer.cpp:42-55: after the pixel count of default-constructed
-// ones, every non-delta vertex past the first of each light path, up to an
-// emitter), path-major; HashGrid::Build over them; then k_vcm, one lane per
-// pixel: the connections of k_bdpt with vcmMisWeight, and the merges.
+// Several passes side by side (lane = pass x pixel, as k_bdpt), each with its
+// own light points, radius and grid: the sub-paths of k_bdpt_subpaths (one
+// light path per pixel); per pass the light points (vcmTracer.cpp:42-55: the
+// pixel count of default-constructed ones, then every non-delta vertex past
+// the first of each light path, up to an emitter, path-major) in a block of
+// their own; HashGrid::Build over each block; then k_vcm, one lane per pixel
+// and pass: the connections of k_bdpt with vcmMisWeight, and the merges.
 
 // true: vertex i of a light path is a light point; stop: the path's points end there
 __device__ __forceinline__ bool light_point(const SceneDev& S, const BVert& v, bool& stop) {
@@ -619,9 +622,9 @@ __device__ __forceinline__ bool light_point(const SceneDev& S, const BVert& v, b
 }
 
 __global__ __launch_bounds__(256) void k_vcm_point_count(SceneDev S, const BVert* verts, const uint32_t* counts,
-                                                         uint32_t npix, uint32_t maxv, uint32_t* pc) {
+                                                         uint32_t lanes, uint32_t maxv, uint32_t* pc) {
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= npix) return;
+    if (k >= lanes) return;
     const BVert* L = verts + (size_t)k * 2 * maxv + maxv;
     const uint32_t nl = min(counts[2 * k + 1], maxv);
     uint32_t m = 0;
@@ -634,20 +637,25 @@ __global__ __launch_bounds__(256) void k_vcm_point_count(SceneDev S, const BVert
     pc[k] = m;
 }
 
-// The points: the npix leading default ones (origin, no path), then each path's.
+// Pass b's block of points starts at b * npix + offs[b * npix] (offs: exclusive
+// sums of the counts over all lanes): the npix default points (origin, no
+// path), then the points of the pass's paths; src holds the lane in the batch
+// and the vertex.
 __global__ __launch_bounds__(256) void k_vcm_point_write(SceneDev S, const BVert* verts, const uint32_t* counts,
-                                                         uint32_t npix, uint32_t maxv, const uint32_t* offs,
-                                                         float* pos, uint32_t* src) {
+                                                         uint32_t npix, uint32_t lanes, uint32_t maxv,
+                                                         const uint32_t* offs, float* pos, uint32_t* src) {
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= npix) return;
-    pos[3 * (size_t)k] = 0.f;
-    pos[3 * (size_t)k + 1] = 0.f;
-    pos[3 * (size_t)k + 2] = 0.f;
-    src[2 * (size_t)k] = WR_NONE;
-    src[2 * (size_t)k + 1] = WR_NONE;
+    if (k >= lanes) return;
+    const uint32_t b = k / npix, p = k % npix;
+    const size_t block = (size_t)b * npix + offs[(size_t)b * npix];
+    pos[3 * (block + p)] = 0.f;
+    pos[3 * (block + p) + 1] = 0.f;
+    pos[3 * (block + p) + 2] = 0.f;
+    src[2 * (block + p)] = WR_NONE;
+    src[2 * (block + p) + 1] = WR_NONE;
     const BVert* L = verts + (size_t)k * 2 * maxv + maxv;
     const uint32_t nl = min(counts[2 * k + 1], maxv);
-    size_t j = (size_t)npix + offs[k];
+    size_t j = block + npix + (offs[k] - offs[(size_t)b * npix]);
     for (uint32_t i = 1; i < nl; i++) {
         bool stop;
         const bool keep = light_point(S, L[i], stop);
@@ -662,6 +670,13 @@ __global__ __launch_bounds__(256) void k_vcm_point_write(SceneDev S, const BVert
     }
 }
 
+// The first offset of every pass: starts[b] = offs[b * npix], b = 0..npass.
+__global__ void k_vcm_pass_starts(const uint32_t* offs, uint32_t npix, uint32_t npass, uint32_t* starts) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b <= npass) starts[b] = offs[(size_t)b * npix];
+}
+
+// One pass's grid: pos / src point at the pass's block, idx indexes it.
 struct VcmGrid {
     GridDev g;
     const int* ends;
@@ -671,16 +686,20 @@ struct VcmGrid {
     float r;
 };
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_vcm(SceneDev S, CamDev cam, uint32_t npix, uint32_t maxv, const BVert* verts,
-                                             const uint32_t* counts, float* pix, VisQueue Q, VcmGrid G) {
-    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= npix) return;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_vcm(SceneDev S, CamDev cam, uint32_t npix, uint32_t npass, uint32_t maxv,
+                                             const BVert* verts, const uint32_t* counts, float* pix_all, VisQueue Q,
+                                             const VcmGrid* grids) {
+    const uint64_t lane = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (lane >= (uint64_t)npass * npix) return;
+    const uint32_t kb = (uint32_t)(lane / npix), p = (uint32_t)(lane % npix);
+    const VcmGrid G = grids[kb];
+    float* pix = pix_all + 3 * (size_t)kb * npix;
     StackSpill spill;
     LaneStack st(&spill);
     const SceneCtx sc = make_scene_ctx(S);
-    const BVert* E = verts + (size_t)p * 2 * maxv;
+    const BVert* E = verts + lane * 2 * maxv;
     const BVert* L = E + maxv;
-    const uint32_t ne = counts[2 * p], nl = counts[2 * p + 1];
+    const uint32_t ne = counts[2 * lane], nl = counts[2 * lane + 1];
     const int max_eye = (int)(ne < maxv ? ne : maxv), max_light = (int)(nl < maxv ? nl : maxv);
     const VcmTerms vc{G.r, (double)npix, false};
     f3 own = mk(0.f, 0.f, 0.f);
@@ -740,7 +759,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 r[7] = contact;
                 float* y = Q.pay + 4 * (size_t)slot;
                 y[0] = contrib.x; y[1] = contrib.y; y[2] = contrib.z; y[3] = dist;
-                Q.dst[slot] = target;
+                Q.dst[slot] = kb * npix + target;
             } else {  // queue full: trace here
                 const PHit h = trace_scene<true>(sc, P.ec.o, P.ec.d, P.ec.inside, contact, st);
                 if (!nee_visible(h.rec, h.t, dist)) continue;
@@ -821,45 +840,65 @@ static wrd::CamDev make_cam_dev(const CamHost& ch) {
     return cam;
 }
 
-int launch_vcm_pass(wr_ctx* ctx, const SceneDev& S, const wr_tree_hdr& H, const CamHost& ch, uint32_t max_depth,
-                    uint64_t seed, uint32_t pass, float radius, uint32_t maxv, void* verts, float* pix,
-                    const BdptQueue& q, const VcmBufs& vb, const FilmBufs& film) {
+int launch_vcm_passes(wr_ctx* ctx, const SceneDev& S, const wr_tree_hdr& H, const CamHost& ch, uint32_t max_depth,
+                      uint64_t seed, uint32_t pass0, uint32_t npass, float base_radius, uint32_t maxv, void* verts,
+                      float* pix, const BdptQueue& q, const VcmBufs& vb, const FilmBufs& film) {
     hipStream_t st = ctx->stream;
     const wrd::CamDev cam = make_cam_dev(ch);
     const uint32_t npix = ch.W * ch.H;
-    const uint32_t blocks = (npix + 255) / 256;
+    const uint32_t lanes = npass * npix;  // the caller keeps npass * npix below 2^31
+    const uint32_t blocks = (lanes + 255) / 256;
     wrd::BVert* V = (wrd::BVert*)verts;
-    hipError_t e = hipMemsetAsync(pix, 0, sizeof(float) * 3 * (size_t)npix, st);
+    hipError_t e = hipMemsetAsync(pix, 0, sizeof(float) * 3 * (size_t)lanes, st);
     if (e == hipSuccess) e = hipMemsetAsync(q.count, 0, sizeof(uint32_t), st);
-    if (e == hipSuccess) e = hipMemsetAsync(vb.pc + npix, 0, sizeof(uint32_t), st);
+    if (e == hipSuccess) e = hipMemsetAsync(vb.pc + lanes, 0, sizeof(uint32_t), st);
     if (e != hipSuccess) return hip_fail(ctx, e, "wr_render_vcm");
-    wrd::k_bdpt_subpaths<<<blocks, 256, wrd::kStackLdsBytes, st>>>(S, cam, max_depth, seed, pass, 1, npix, maxv, V,
+    wrd::k_bdpt_subpaths<<<blocks, 256, wrd::kStackLdsBytes, st>>>(S, cam, max_depth, seed, pass0, npass, npix, maxv, V,
                                                                   q.counts);
-    wrd::k_vcm_point_count<<<blocks, 256, 0, st>>>(S, V, q.counts, npix, maxv, vb.pc);
+    wrd::k_vcm_point_count<<<blocks, 256, 0, st>>>(S, V, q.counts, lanes, maxv, vb.pc);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "wr_render_vcm");
     size_t tb = vb.scan_bytes;
-    if ((e = hipcub::DeviceScan::ExclusiveSum(vb.scan, tb, vb.pc, vb.offs, (int)npix + 1, st)) != hipSuccess)
+    if ((e = hipcub::DeviceScan::ExclusiveSum(vb.scan, tb, vb.pc, vb.offs, (int)lanes + 1, st)) != hipSuccess)
         return hip_fail(ctx, e, "wr_render_vcm: scan");
-    uint32_t total = 0;
-    if ((e = hipMemcpyAsync(&total, vb.offs + npix, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+    wrd::k_vcm_pass_starts<<<(npass + 1 + 63) / 64, 64, 0, st>>>(vb.offs, npix, npass, vb.starts);
+    std::vector<uint32_t> starts(npass + 1);
+    if ((e = hipGetLastError()) != hipSuccess ||
+        (e = hipMemcpyAsync(starts.data(), vb.starts, sizeof(uint32_t) * (npass + 1), hipMemcpyDeviceToHost, st)) !=
+            hipSuccess ||
         (e = hipStreamSynchronize(st)) != hipSuccess)
         return hip_fail(ctx, e, "wr_render_vcm");
-    const uint64_t npts = (uint64_t)npix + total;
-    if (npts > vb.cap) return fail(ctx, WR_EINVAL, "wr_render_vcm: more light points than the buffer holds");
-    wrd::k_vcm_point_write<<<blocks, 256, 0, st>>>(S, V, q.counts, npix, maxv, vb.offs, vb.pos, vb.src);
+    if ((uint64_t)lanes + starts[npass] > vb.cap)
+        return fail(ctx, WR_EINVAL, "wr_render_vcm: more light points than the buffer holds");
+    wrd::k_vcm_point_write<<<blocks, 256, 0, st>>>(S, V, q.counts, npix, lanes, maxv, vb.offs, vb.pos, vb.src);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "wr_render_vcm");
-    // HashGrid::Reserve(pixels) + Build(light points, radius) (vcmTracer.cpp:33,57)
-    wr_hashgrid* grid = nullptr;
-    int rc = wr_hashgrid_build(ctx, vb.pos, npts, radius, npix, &grid);
-    if (rc != WR_OK) return rc;
-    const wrd::VcmGrid G{grid->g, grid->ends, grid->idx, vb.pos, vb.src, radius};
+    // per pass: HashGrid::Reserve(pixels) + Build(its light points, its radius) (vcmTracer.cpp:30,33,57)
+    std::vector<wr_hashgrid*> grids(npass, nullptr);
+    std::vector<wrd::VcmGrid> G(npass);
+    auto destroy = [&]() {
+        for (wr_hashgrid* g : grids) wr_hashgrid_destroy(g);
+    };
+    for (uint32_t b = 0; b < npass; b++) {
+        const uint64_t block = (uint64_t)b * npix + starts[b];
+        const uint64_t n = (uint64_t)npix + (starts[b + 1] - starts[b]);
+        const float r = vcm_radius(base_radius, pass0 + b);
+        const int rc = wr_hashgrid_build(ctx, vb.pos + 3 * block, n, r, npix, &grids[b]);
+        if (rc != WR_OK) {
+            destroy();
+            return rc;
+        }
+        G[b] = wrd::VcmGrid{grids[b]->g, grids[b]->ends, grids[b]->idx, vb.pos + 3 * block, vb.src + 2 * block, r};
+    }
     const wrd::VisQueue Q{q.rays, q.pay, q.dst, q.count, q.cap};
-    wrd::k_vcm<<<blocks, 256, wrd::kStackLdsBytes, st>>>(S, cam, npix, maxv, V, q.counts, pix, Q, G);
+    e = hipMemcpyAsync(vb.grids, G.data(), sizeof(wrd::VcmGrid) * npass, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        wrd::k_vcm<<<blocks, 256, wrd::kStackLdsBytes, st>>>(S, cam, npix, npass, maxv, V, q.counts, pix, Q,
+                                                            (const wrd::VcmGrid*)vb.grids);
+        e = hipGetLastError();
+    }
     uint32_t nq = 0;
-    e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(&nq, q.count, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    wr_hashgrid_destroy(grid);
+    destroy();
     if (e != hipSuccess) return hip_fail(ctx, e, "wr_render_vcm");
     if (nq > q.cap) nq = q.cap;
     if (nq) {
@@ -867,14 +906,17 @@ int launch_vcm_pass(wr_ctx* ctx, const SceneDev& S, const wr_tree_hdr& H, const 
             return hip_fail(ctx, e, "wr_render_vcm: visibility");
         wrd::k_bdpt_accum<<<(nq + 255) / 256, 256, 0, st>>>(nq, q.pay, q.dst, (const uint4*)q.hits, pix);
     }
-    wrd::k_bdpt_fold<<<blocks, 256, 0, st>>>(cam, 1, npix, pix, wrd::FilmDev{film.sum, film.comp, film.sumsq});
+    wrd::k_bdpt_fold<<<(npix + 255) / 256, 256, 0, st>>>(cam, npass, npix, pix,
+                                                         wrd::FilmDev{film.sum, film.comp, film.sumsq});
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "wr_render_vcm");
     return WR_OK;
 }
 
-size_t vcm_scan_bytes(uint32_t npix) {
+size_t vcm_grid_bytes() { return sizeof(wrd::VcmGrid); }
+
+size_t vcm_scan_bytes(uint32_t lanes) {
     size_t tb = 0;
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)npix + 1);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)lanes + 1);
     return tb;
 }
 

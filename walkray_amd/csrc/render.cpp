@@ -341,29 +341,35 @@ extern "C" int wr_render_vcm(wr_ctx* ctx, const wr_scene* s, const wr_camera* ca
         if (o.moving) return fail(ctx, WR_EINVAL, "wr_render_vcm: moving objects are not supported");
     const uint64_t npix = (uint64_t)cam->width * cam->height;
     const uint32_t maxv = prm->max_depth + 3;
-    // light points: npix default ones plus at most maxv - 1 per light path
-    const uint64_t cap = npix * maxv;
-    if (npix == 0 || cap >= (1ull << 31)) return fail(ctx, WR_EINVAL, "wr_render_vcm: bad image size");
+    // light points of a pass: npix default ones plus at most maxv - 1 per light path
+    if (npix == 0 || npix * maxv >= (1ull << 31)) return fail(ctx, WR_EINVAL, "wr_render_vcm: bad image size");
     if (film->broken) return film_broken(film, "wr_render_vcm");
     (void)hipSetDevice(ctx->device);
     const CamHost ch = make_cam(*cam);
+    // passes per launch: as many as fill the device, within 4 GiB of sub-path records and 2^31 points
     const uint64_t per_pass = bdpt_vertex_bytes() * 2 * maxv * npix;
+    uint64_t batch = std::max<uint64_t>(1, (4ull << 30) / per_pass);
+    batch = std::min<uint64_t>(std::min<uint64_t>(batch, prm->spp ? prm->spp : 1), std::max<uint64_t>(1, (1ull << 22) / npix));
+    batch = std::min<uint64_t>(batch, std::max<uint64_t>(1, ((1ull << 31) - 1) / (npix * maxv)));
+    const uint64_t lanes = batch * npix;
     BdptQueue q{};
-    q.cap = prm->wave_slots ? prm->wave_slots : (uint32_t)std::min<uint64_t>(32ull * npix, 24ull << 20);
+    q.cap = prm->wave_slots ? prm->wave_slots : (uint32_t)std::min<uint64_t>(32ull * lanes, 24ull << 20);
     VcmBufs vb{};
-    vb.cap = cap;
-    vb.scan_bytes = vcm_scan_bytes((uint32_t)npix);
+    vb.cap = lanes * maxv;
+    vb.scan_bytes = vcm_scan_bytes((uint32_t)lanes);
     void *verts = nullptr, *qbuf = nullptr;
     float* pix = nullptr;
-    hipError_t e = hipMalloc(&verts, per_pass);
-    if (e == hipSuccess) e = hipMalloc((void**)&pix, sizeof(float) * 3 * npix);
+    hipError_t e = hipMalloc(&verts, per_pass * batch);
+    if (e == hipSuccess) e = hipMalloc((void**)&pix, sizeof(float) * 3 * lanes);
     if (e == hipSuccess) e = hipMalloc(&qbuf, kBdptQueueEntryBytes * (size_t)q.cap + 256);
-    if (e == hipSuccess) e = hipMalloc((void**)&q.counts, sizeof(uint32_t) * 2 * npix);
-    if (e == hipSuccess) e = hipMalloc((void**)&vb.pc, sizeof(uint32_t) * (npix + 1));
-    if (e == hipSuccess) e = hipMalloc((void**)&vb.offs, sizeof(uint32_t) * (npix + 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&q.counts, sizeof(uint32_t) * 2 * lanes);
+    if (e == hipSuccess) e = hipMalloc((void**)&vb.pc, sizeof(uint32_t) * (lanes + 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&vb.offs, sizeof(uint32_t) * (lanes + 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&vb.starts, sizeof(uint32_t) * (batch + 1));
     if (e == hipSuccess) e = hipMalloc(&vb.scan, std::max<size_t>(vb.scan_bytes, 4));
-    if (e == hipSuccess) e = hipMalloc((void**)&vb.pos, sizeof(float) * 3 * cap);
-    if (e == hipSuccess) e = hipMalloc((void**)&vb.src, sizeof(uint32_t) * 2 * cap);
+    if (e == hipSuccess) e = hipMalloc((void**)&vb.pos, sizeof(float) * 3 * vb.cap);
+    if (e == hipSuccess) e = hipMalloc((void**)&vb.src, sizeof(uint32_t) * 2 * vb.cap);
+    if (e == hipSuccess) e = hipMalloc(&vb.grids, vcm_grid_bytes() * batch);
     if (e == hipSuccess) {
         char* b = (char*)qbuf;
         q.rays = (uint32_t*)b;
@@ -373,15 +379,9 @@ extern "C" int wr_render_vcm(wr_ctx* ctx, const wr_scene* s, const wr_camera* ca
         q.count = (uint32_t*)(b + 68 * (size_t)q.cap + 128);
     }
     auto release = [&]() {
-        (void)hipFree(verts);
-        (void)hipFree(pix);
-        (void)hipFree(qbuf);
-        (void)hipFree(q.counts);
-        (void)hipFree(vb.pc);
-        (void)hipFree(vb.offs);
-        (void)hipFree(vb.scan);
-        (void)hipFree(vb.pos);
-        (void)hipFree(vb.src);
+        for (void* ptr : {verts, (void*)pix, qbuf, (void*)q.counts, (void*)vb.pc, (void*)vb.offs, (void*)vb.starts,
+                          vb.scan, (void*)vb.pos, (void*)vb.src, vb.grids})
+            (void)hipFree(ptr);
     };
     if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
     if (e != hipSuccess) {
@@ -389,11 +389,11 @@ extern "C" int wr_render_vcm(wr_ctx* ctx, const wr_scene* s, const wr_camera* ca
         return hip_fail(ctx, e, "wr_render_vcm");
     }
     const FilmBufs fb{film->sum, film->comp, film->sumsq};
-    for (uint32_t k = 0; k < prm->spp; k++) {
-        const uint32_t pass = film->passes + k;
-        const int rc = launch_vcm_pass(ctx, s->dev, s->hdrs[0], ch, prm->max_depth, prm->seed, pass,
-                                       vcm_radius(base_radius, pass), maxv, verts, pix, q, vb, fb);
-        if (rc != WR_OK) {  // the film holds k whole passes and part of one more
+    for (uint32_t k = 0; k < prm->spp; k += (uint32_t)batch) {
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(batch, prm->spp - k);
+        const int rc = launch_vcm_passes(ctx, s->dev, s->hdrs[0], ch, prm->max_depth, prm->seed, film->passes + k, nb,
+                                         base_radius, maxv, verts, pix, q, vb, fb);
+        if (rc != WR_OK) {  // the film may hold part of this call's passes
             (void)hipStreamSynchronize(ctx->stream);
             release();
             film->broken = true;

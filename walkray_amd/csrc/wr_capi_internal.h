@@ -52,24 +52,29 @@ hipError_t noalloc_scratch_init(int device);
 int scene_build(wr_scene* s, const wr_scene* reuse);
 
 // VCMTracer (bdpt.hip): the merging radius of a pass (vcmTracer.cpp:30), and
-// one pass [sub-paths, light points, hash grid, connections and merges, film]
-// on the context stream.  verts / pix / q as for launch_bdpt_passes with one
-// pass; vb: the light-point buffers (cap points of 3 floats and 2 words), the
-// per-pixel counts and offsets (pixels + 1 each) and the scan's scratch.
+// passes [pass0, pass0 + npass) side by side [sub-paths, light points, one
+// hash grid per pass, connections and merges, film] on the context stream.
+// verts / pix / q as for launch_bdpt_passes; vb: the light-point buffers (cap
+// points of 3 floats and 2 words), the per-lane counts and offsets (lanes + 1
+// each), the per-pass starts (npass + 1), the scan's scratch, and the device
+// copy of the per-pass grids (npass records of vcm_grid_bytes()).
 struct VcmBufs {
     uint32_t* pc;
     uint32_t* offs;
+    uint32_t* starts;
     void* scan;
     size_t scan_bytes;
     float* pos;
     uint32_t* src;
     uint64_t cap;
+    void* grids;
 };
 float vcm_radius(float base, uint32_t pass);
-size_t vcm_scan_bytes(uint32_t npix);
-int launch_vcm_pass(wr_ctx* ctx, const SceneDev& S, const wr_tree_hdr& H, const CamHost& ch, uint32_t max_depth,
-                    uint64_t seed, uint32_t pass, float radius, uint32_t maxv, void* verts, float* pix,
-                    const BdptQueue& q, const VcmBufs& vb, const FilmBufs& film);
+size_t vcm_scan_bytes(uint32_t lanes);
+size_t vcm_grid_bytes();
+int launch_vcm_passes(wr_ctx* ctx, const SceneDev& S, const wr_tree_hdr& H, const CamHost& ch, uint32_t max_depth,
+                      uint64_t seed, uint32_t pass0, uint32_t npass, float base_radius, uint32_t maxv, void* verts,
+                      float* pix, const BdptQueue& q, const VcmBufs& vb, const FilmBufs& film);
 }  // namespace wr
 
 struct wr_scene {
