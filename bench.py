@@ -12,13 +12,14 @@ The same JSON line also reports, outside `value`, the path-tracing workloads
 (Cornell box 256^2 x 16 spp and the 512-object field at 1920x1080 x 64 spp,
 megakernel vs wavefront samples/s, with and without next-event estimation),
 the light sub-paths and photon hash grid of BDPT / VCM / PPM on the field, the
-BidirectionalPathTracer and the VCMTracer on the Cornell box, and the served-jobs workload (p50/p99 latency and aggregate samples/s under
+BidirectionalPathTracer, the VCMTracer and PhotonMap's progressive photon mapping on the Cornell box, and the
+served-jobs workload (p50/p99 latency and aggregate samples/s under
 Poisson arrivals).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pt] [--legs ch,pt,lp,bdpt,vcm,motion,serve,c5]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-pt] [--legs ch,pt,lp,bdpt,vcm,ppm,motion,serve,c5]
 
 --legs picks the workloads (closest hit, path tracing, light paths + photon
-grid, BidirectionalPathTracer and VCMTracer on the Cornell box, moving objects (per-pass
+grid, BidirectionalPathTracer, VCMTracer and PhotonMap on the Cornell box, moving objects (per-pass
 rebuilds), serving, and c5: the 43.8M-triangle large scene at 3840x2160 x
 256 spp; ptd, not in the default
 run: C1 and C3 with the wavefront's kernels launched one by one and no graph
@@ -281,6 +282,21 @@ def cpu_baseline_vcm(S, passes=2, depth=8, radius=0.01):
                                       "OpenMP over pixels"}
 
 
+def cpu_baseline_ppm(S, passes=4, depth=8, radius=0.01):
+    """The oracle's PhotonMap restatement on the host cores, over a bounded sample of the PPM
+    leg's workload (4 passes of the same 256x256 Cornell image)."""
+    from oracle import oracle as O
+    threads = min(os.cpu_count() or 1, 16)
+    objs, cam = S.cornell(k=5, width=256, height=256)
+    osc = O.Scene(objs)
+    t0 = time.perf_counter()
+    osc.render_ppm(cam, passes, depth, 1234, radius, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(cam["width"] * cam["height"] * passes / dt, 1), "unit": "samples/s", "cores": threads,
+            "kind": "port", "sample": f"{passes} passes of the same 256x256 image, oracle PhotonMap restatement, "
+                                      "OpenMP over pixels"}
+
+
 def cpu_baseline_bdpt(S, passes=2, depth=8):
     """The oracle's BidirectionalPathTracer restatement on the host cores, over a bounded
     sample of the BDPT leg's workload (2 passes of the same 256x256 Cornell image)."""
@@ -303,7 +319,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pt", action="store_true", help="same as --legs ch")
-    ap.add_argument("--legs", default="ch,pt,lp,bdpt,vcm,motion,serve,c5")
+    ap.add_argument("--legs", default="ch,pt,lp,bdpt,vcm,ppm,motion,serve,c5")
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="first run one device-to-device copy of this many bytes (counter calibration)")
@@ -361,6 +377,13 @@ def main():
             extra["vcm"]["cpu_baseline"] = cpu_baseline_vcm(S)
             log(f"[bench] VCM oracle: {extra['vcm']['cpu_baseline']['value'] / 1e3:.1f} Ksamples/s on "
                 f"{extra['vcm']['cpu_baseline']['cores']} cores")
+    if "ppm" in legs:
+        from walkray_amd import render as R
+        extra["ppm"] = R.bench_ppm(ctx, log)
+        if not args.no_cpu:
+            extra["ppm"]["cpu_baseline"] = cpu_baseline_ppm(S)
+            log(f"[bench] PPM oracle: {extra['ppm']['cpu_baseline']['value'] / 1e3:.1f} Ksamples/s on "
+                f"{extra['ppm']['cpu_baseline']['cores']} cores")
     if "motion" in legs:
         from walkray_amd import render as R
         extra["motion"] = R.bench_motion(ctx, log)

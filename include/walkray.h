@@ -325,6 +325,23 @@ int wr_render_vcm(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, cons
  * EPSILON) with alpha = 2/3 (vcmTracer.cpp:30, vcmTracer.h:39-40). */
 float wr_vcm_radius(float base_radius, uint32_t pass);
 
+/* PhotonMap::renderPixels (photonMap.cpp:21-148) with progressive photon
+ * mapping (PPM) for passes [passes, passes + spp) of the film.  Per pass s:
+ * the radius wr_vcm_radius(base_radius, s) (photonMap.cpp:47), the light
+ * sub-paths, light points and hash grid of wr_render_vcm; per pixel the eye
+ * "merge path" (samplePath with maxPathLen 10, photonMap.cpp:150-190) up to
+ * its first emitter, whose emitted radiance it takes, or its first diffuse
+ * vertex, where it takes the kernel density estimate of the photons within the
+ * radius (kernel 3 (1 - d^2/r^2)^2 / pi over photon count x r^2,
+ * photonMap.cpp:198-248,656-668); times the camera vertex's factor
+ * (photonMap.cpp:670-674); the pass's pixel values checked, vignetting-fixed
+ * and added to the film.  Every object here is a surface, and none is an
+ * anisotropic glossy one, so the volume and ellipse branches do not arise.
+ * params: spp, max_depth (of the light sub-paths, at most 32), seed; mode,
+ * wave_slots, wave_grid and flags must be 0.  Static scenes only. */
+int wr_render_ppm(wr_ctx* ctx, const wr_scene* scene, const wr_camera* cam, const wr_render_params* params,
+                  float base_radius, wr_film* film, wr_render_stats* stats);
+
 /* ---- light sub-paths and the photon hash grid ------------------------------ */
 /* One Ray of a Path (ray.h:8-37) as the light-path estimators read it. */
 typedef struct {

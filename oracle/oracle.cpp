@@ -1794,6 +1794,81 @@ struct Vcm {
     }
 };
 
+/* ---- PhotonMap (photonMap.cpp:21-680) ----------------------------------------
+ * Progressive photon mapping on surfaces (photonMap.cpp with PPM defined and
+ * NO_RAY_MARCHING, ANI_PM and RESEARCH undefined): no object here is a volume
+ * or an anisotropic glossy object, so every light point is a surface photon
+ * and every density estimate is the plain range query.  Per pass: the radius
+ * of vcmRadius, the light points of VCMTracer (the same light paths), the
+ * hash grid; per pixel the eye "merge path" (samplePath with maxPathLen
+ * MERGE_LEN = 10, photonMap.cpp:10,150-190), and at its first emitter or
+ * non-specular (diffuse; Phong is specular without MERGE_GLOSSY) vertex the
+ * emitted radiance or the kernel density estimate of the photons around it
+ * (photonMap.cpp:198-248,656-668), times the camera vertex's factor only
+ * (photonMap.cpp:670-674). */
+struct Ppm {
+    using PRay = Tracer::PRay;
+    const Tracer& tr;  /* light paths: maxPathLen of the renderer */
+    const Tracer& te;  /* eye merge paths: maxPathLen MERGE_LEN */
+    float radius;
+    unsigned npix;
+
+    /* Query::Process (photonMap.cpp:219-248) for one light point */
+    V photon(const std::vector<PRay>& L, int index, const PRay& outRay) const {
+        V thr(1, 1, 1);
+        for (int i = 0; i < index; i++) {
+            thr = mulv(thr, divs(divs(L[i].rad, L[i].dp), L[i].op));
+            thr = mul(thr, Tracer::cosTerm(L[i]));
+            thr = mulv(thr, V(1, 1, 1));  /* radianceDecay: no volumes */
+        }
+        thr = divs(thr, L[index].op);
+        PRay photonRay = L[index];
+        photonRay.d = L[index - 1].d;
+        const V color = mulv(thr, tr.evalBSDF(photonRay, outRay));
+        const float l = length(sub(outRay.o, L[index].o));
+        const float distSqr = l * l;  /* powf(l, 2): the correctly rounded square */
+        if (!legal(color)) return V(0, 0, 0);
+        const float radiusSqr = radius * radius;
+        const float q = 1 - distSqr / radiusSqr;
+        const float sk = 0 > q ? 0.f : q;                     /* MAX(0, 1 - distSqr / radiusSqr) */
+        const float kernel = (float)(3 * sk * sk / PI_D);     /* Kernel (photonMap.cpp:209-212) */
+        const float normalization = kernel / ((float)npix * radius * radius);
+        return mul(color, normalization);
+    }
+
+    V pixel(unsigned p, unsigned pass, const std::vector<std::vector<PRay>>& lightPaths, const std::vector<V>& pts,
+            const std::vector<std::pair<int, int>>& src, const HashGrid& grid) const {
+        std::vector<PRay> eye;
+        te.samplePath(eye, te.cameraRay(p, pass), 0, p, pass);
+        V surface(0, 0, 0);
+        std::vector<int> near;
+        for (size_t i = 1; i < eye.size(); i++) {
+            const unsigned c = eye[i].contact;
+            if (c != ORC_NONE && tr.s->objs[c].mat == ORC_EMISSIVE) {  /* the eye path hit a light */
+                surface = eye[i].rad;
+                break;
+            }
+            if (c != ORC_NONE && tr.s->objs[c].mat == ORC_DIFFUSE) {  /* nonSpecular() */
+                PRay outRay = eye[i];
+                outRay.d = neg(eye[i - 1].d);
+                grid.query(pts, eye[i].o, near);
+                V contrib(0, 0, 0);
+                for (int pi : near) {
+                    const int k = src[pi].first, j = src[pi].second;
+                    if (k < 0 || j < 0) continue;  /* a leading default point: no path */
+                    contrib = add(contrib, photon(lightPaths[k], j, outRay));
+                }
+                surface = contrib;
+                break;
+            }
+        }
+        V color = add(mulv(V(1, 1, 1), surface), V(0, 0, 0));  /* Tr * SurfaceColor + VolumeColor */
+        const PRay& e0 = eye[0];
+        color = mulv(color, divs(divs(mul(e0.rad, Tracer::cosTerm(e0)), e0.dp), e0.op));
+        return color;
+    }
+};
+
 /* VCMTracer's radius of pass s (vcmTracer.cpp:30): MAX(baseRadius *
  * sqrt(powf(s+1, alpha-1)), EPSILON) with alpha = 2/3 (vcmTracer.h:39). */
 static float vcmRadius(float base, uint32_t s) {
@@ -2276,6 +2351,48 @@ void orc_render_vcm(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t 
         }
         for (int64_t p = 0; p < np; p++)
             for (const auto& sp2 : splats[p]) iter[sp2.first] = add(iter[sp2.first], sp2.second);
+        for (int64_t p = 0; p < np; p++) {
+            V v = iter[p];
+            if (!legal(v)) v = V(0, 0, 0);
+            v = tr.cam.fixVignetting(v, (unsigned)p);
+            for (int k = 0; k < 3; k++) {
+                sum[3 * p + k] += v[k];
+                sumsq[3 * p + k] += (double)v[k] * v[k];
+            }
+        }
+    }
+}
+
+void orc_render_ppm(void* sp, const orc_camera* c, uint32_t max_depth, uint64_t seed, uint32_t pass0, uint32_t npass,
+                    float base_radius, double* sum, double* sumsq, int nthreads) {
+    Scene* s = (Scene*)sp;
+    Tracer tr{s, Cam(c), max_depth, seed};
+    Tracer te{s, Cam(c), 10, seed};  /* MERGE_LEN (photonMap.cpp:10) */
+    const int64_t np = (int64_t)c->width * c->height;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    std::vector<V> iter(np);
+    std::vector<std::vector<Tracer::PRay>> lightPaths(np);
+    for (uint32_t pass = pass0; pass < pass0 + npass; pass++) {
+        const Ppm ppm{tr, te, vcmRadius(base_radius, pass), (unsigned)np};
+        /* step 1 (photonMap.cpp:59-96) */
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+        for (int64_t p = 0; p < np; p++) lightPaths[p] = tr.lightPath((unsigned)p, pass);
+        std::vector<V> pts((size_t)np, V(0, 0, 0));
+        std::vector<std::pair<int, int>> src((size_t)np, {-1, -1});
+        for (int64_t p = 0; p < np; p++) {
+            const auto& L = lightPaths[p];
+            for (size_t i = 1; i < L.size(); i++) {
+                if (L[i].contact != ORC_NONE && s->objs[L[i].contact].mat == ORC_EMISSIVE) break;
+                if (L[i].delta) continue;
+                pts.push_back(L[i].o);
+                src.push_back({(int)p, (int)i});
+            }
+        }
+        HashGrid grid;
+        grid.build(pts, ppm.radius, (size_t)np);
+        /* step 2 (photonMap.cpp:98-116) */
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+        for (int64_t p = 0; p < np; p++) iter[p] = ppm.pixel((unsigned)p, pass, lightPaths, pts, src, grid);
         for (int64_t p = 0; p < np; p++) {
             V v = iter[p];
             if (!legal(v)) v = V(0, 0, 0);

@@ -155,6 +155,14 @@ void     orc_render_bdpt(void* s, const orc_camera* cam, uint32_t max_depth, uin
 void     orc_render_vcm(void* s, const orc_camera* cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
                         uint32_t npass, float base_radius, double* sum, double* sumsq, int nthreads);
 float    orc_vcm_radius(float base_radius, uint32_t pass);
+/* PhotonMap::renderPixels (photonMap.cpp:21-148) with PPM: per pass the radius
+ * orc_vcm_radius(base_radius, pass), the light points and grid of
+ * orc_render_vcm, per pixel the eye merge path (maxPathLen 10) and at its first
+ * emitter the emitted radiance or at its first diffuse vertex the kernel
+ * density estimate of the photons around it; added to sum / sumsq as
+ * orc_render_bdpt. */
+void     orc_render_ppm(void* s, const orc_camera* cam, uint32_t max_depth, uint64_t seed, uint32_t pass0,
+                        uint32_t npass, float base_radius, double* sum, double* sumsq, int nthreads);
 
 /* ---- Counter RNG (Philox-4x32-10, Salmon et al., SC'11) ---------------- */
 void     orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

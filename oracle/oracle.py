@@ -71,6 +71,7 @@ def lib():
         L.orc_render_bdpt.argtypes = [vp, vp, u32, u64, u32, u32, vp, vp, C.c_int]
         L.orc_render_vcm.argtypes = [vp, vp, u32, u64, u32, u32, f32, vp, vp, C.c_int]
         L.orc_vcm_radius.restype = f32
+        L.orc_render_ppm.argtypes = [vp, vp, u32, u64, u32, u32, f32, vp, vp, C.c_int]
         L.orc_vcm_radius.argtypes = [f32, u32]
         L.orc_philox4x32.argtypes = [vp, vp, vp]
         L.orc_rng4.argtypes = [u64, u32, u32, u32, vp]
@@ -291,6 +292,16 @@ class Scene:
         if sums is None:
             sums = (np.zeros(3 * npx, np.float64), np.zeros(3 * npx, np.float64))
         lib().orc_render_vcm(self.h, C.byref(c), max_depth, seed, pass0, spp, base_radius, _p(sums[0]), _p(sums[1]),
+                             nthreads)
+        return sums
+
+    def render_ppm(self, cam, spp, max_depth, seed, base_radius, pass0=0, nthreads=0, sums=None):
+        """PhotonMap (progressive photon mapping) passes [pass0, pass0 + spp)."""
+        c = Camera.from_dict(cam)
+        npx = cam["width"] * cam["height"]
+        if sums is None:
+            sums = (np.zeros(3 * npx, np.float64), np.zeros(3 * npx, np.float64))
+        lib().orc_render_ppm(self.h, C.byref(c), max_depth, seed, pass0, spp, base_radius, _p(sums[0]), _p(sums[1]),
                              nthreads)
         return sums
 

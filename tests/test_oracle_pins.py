@@ -3,7 +3,8 @@ refraction and total internal reflection (refractiveObj.h:27-88, util.h:40-54),
 the camera's ray mapping and sample weight (camera.cpp:6-23,107-122,182-186,
 ray.cpp:52-58), isotropic Phong off the normal (isotropicPhong.h:20-38,
 glossyObj.h:39-62), BidirectionalPathTracer with a glass sphere
-(bidirPathTracer.cpp:53-221) and VCMTracer (vcmTracer.cpp:6-446).  Expected values come from tests/pin_cases.py
+(bidirPathTracer.cpp:53-221), VCMTracer (vcmTracer.cpp:6-446) and PhotonMap's
+progressive photon mapping (photonMap.cpp:21-248,656-674).  Expected values come from tests/pin_cases.py
 (float64 physics and geometry), never from the oracle's own formulas."""
 import math
 
@@ -280,3 +281,35 @@ def test_vcm_converges_to_bdpt():
     diff = m1.sum() - m2.sum()
     se = math.sqrt(v1.sum() + v2.sum())
     assert abs(diff) < 4 * se + 0.005 * m2.sum(), (m1.sum(), m2.sum(), se)
+
+
+# --------------------------------------------------------------------- PPM ---
+def _ppm_furnace(alb, le, width=64, height=48):
+    """A diffuse floor under an emitter plane (both 4000 wide), a 90-degree camera at half
+    height looking straight down."""
+    floor = S.quad((0, 0, 0), (4000, 0, 0), (0, 0, 4000), (0, 1, 0), S.DIFFUSE, (alb, alb, alb))
+    ceil = S.quad((0, 1, 0), (4000, 0, 0), (0, 0, 4000), (0, -1, 0), S.EMISSIVE, (le, le, le))
+    cam = S.camera((0.0, 0.5, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), width, height, sight_dist=width / 2)
+    return [floor, ceil], cam
+
+
+def test_ppm_furnace_density_estimate():
+    """Under a uniform emitter plane the floor's irradiance is pi Le everywhere within a
+    thousand units of the camera's view, so the kernel density estimate of the photons
+    (each carrying Le cos / (pdf_dir pdf_area), the Epanechnikov-type kernel 3 (1-d^2/r^2)^2
+    / pi normalised by the photon count and r^2, the diffuse BSDF a/pi) is unbiased at any
+    radius: every pixel's expectation is a Le times the camera weight of the pixel (the
+    float64 quadrature of cos theta_jit / cos theta_centre).  All pixels of a pass share its
+    photons, so the error is measured across passes.  A dropped origin probability, a
+    dropped 3/pi or the BSDF's 1/pi missing each moves the ratio by more than 4%."""
+    objs, cam = _ppm_furnace(0.5, 2.0)
+    sc = O.Scene(objs)
+    want = 0.5 * 2.0 * P.wide_camera_expected(cam, m=16)
+    ratios = []
+    for k in range(400):
+        s, _ = sc.render_ppm(cam, 1, 4, 7, 1000.0, pass0=k)
+        ratios.append((s.reshape(-1, 3)[:, 0] / want).mean())
+    ratios = np.asarray(ratios)
+    se = ratios.std() / math.sqrt(len(ratios))
+    assert se < 0.008, se
+    assert abs(ratios.mean() - 1.0) < 4 * se, (ratios.mean(), se)

@@ -161,6 +161,7 @@ def lib():
         "wr_render_bdpt": (i32, [vp, vp, P(CameraDesc), P(RenderParams), vp, P(RenderStats)]),
         "wr_render_vcm": (i32, [vp, vp, P(CameraDesc), P(RenderParams), C.c_float, vp, P(RenderStats)]),
         "wr_vcm_radius": (C.c_float, [C.c_float, u32]),
+        "wr_render_ppm": (i32, [vp, vp, P(CameraDesc), P(RenderParams), C.c_float, vp, P(RenderStats)]),
         "wr_server_create": (i32, [vp, P(ServerConfig), P(vp)]),
         "wr_server_destroy": (None, [vp]),
         "wr_server_add_scene": (i32, [vp, vp, P(u32)]),
@@ -548,6 +549,17 @@ def render_vcm(scene, cam, film, spp, max_depth, seed, base_radius, queue=0):
     st = RenderStats()
     _check(lib().wr_render_vcm(scene.ctx.h, scene.h, C.byref(cd), C.byref(prm), base_radius, film.h, C.byref(st)),
            scene.ctx.h, "wr_render_vcm")
+    return {"paths": st.paths, "kernel_ms": st.kernel_ms}
+
+
+def render_ppm(scene, cam, film, spp, max_depth, seed, base_radius):
+    """PhotonMap::renderPixels (progressive photon mapping) for `spp` more passes into `film`
+    with the configuration's radius `base_radius`; returns stats."""
+    cd = CameraDesc.from_dict(cam)
+    prm = RenderParams(spp, max_depth, seed, 0, 0, 0, 0)
+    st = RenderStats()
+    _check(lib().wr_render_ppm(scene.ctx.h, scene.h, C.byref(cd), C.byref(prm), base_radius, film.h, C.byref(st)),
+           scene.ctx.h, "wr_render_ppm")
     return {"paths": st.paths, "kernel_ms": st.kernel_ms}
 
 

@@ -423,7 +423,7 @@ __device__ __forceinline__ void atomic_add3(float* buf, uint32_t p, f3 v) {
 // Cornell box against the unconstrained 264 / 190 registers.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bdpt_subpaths(SceneDev S, CamDev cam, uint32_t max_depth, uint64_t seed,
                                                        uint32_t pass0, uint32_t npass, uint32_t npix, uint32_t maxv,
-                                                       BVert* verts, uint32_t* counts) {
+                                                       BVert* verts, uint32_t* counts, bool eye = true) {
     const uint64_t lane = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (lane >= (uint64_t)npass * npix) return;
     const uint32_t k = (uint32_t)(lane / npix), p = (uint32_t)(lane % npix);
@@ -435,8 +435,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     BVert* L = E + maxv;
     uint32_t ne = 0, nl = 0;
     float rn[4];
-    // eye sub-path: the camera ray (camera.cpp:6-60) then samplePath
-    {
+    // eye sub-path: the camera ray (camera.cpp:6-60) then samplePath (PhotonMap forms its own)
+    if (eye) {
         rng4(seed, p, pass, 0, rn);
         BVert c;
         c.o = cam.pos;
@@ -816,6 +816,132 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     atomic_add3(pix, p, own);
 }
 
+// ---- PhotonMap (photonMap.cpp:21-680), progressive photon mapping ----------
+// The light points and per-pass grids of VCMTracer (the same light paths);
+// k_ppm, one lane per pixel and pass: the eye "merge path" (samplePath with
+// maxPathLen MERGE_LEN = 10, photonMap.cpp:10,150-190) up to its first
+// emitter (its emitted radiance) or first non-specular, i.e. diffuse, vertex
+// (the kernel density estimate of the photons around it, photonMap.cpp:
+// 198-248,656-668), times the camera vertex's factor only (photonMap.cpp:
+// 670-674).  No object here is a volume or an anisotropic glossy object, so
+// every photon is a surface photon and every estimate the plain range query.
+constexpr uint32_t kMergeLen = 10;
+
+// Query::Process for the photons around outRay (the eye vertex, its direction
+// towards the previous eye vertex).
+__device__ f3 ppm_density(const SceneDev& S, const VcmGrid& G, const BVert* verts, uint32_t maxv, uint32_t npix,
+                          const BVert& outRay) {
+    f3 contrib = mk(0.f, 0.f, 0.f);
+    int pc[6];
+    if (!query_corner(G.g, outRay.o, pc)) return contrib;
+    const float r = G.r;
+    for (int c8 = 0; c8 < 8; c8++) {
+        const uint32_t cell = query_cell(G.g, pc, c8);
+        int a = cell == 0 ? 0 : G.ends[cell - 1];
+        const int b = G.ends[cell];
+        int hi = b;  // the default points lead their cell's range
+        while (a < hi) {
+            const int mid = (a + hi) >> 1;
+            if (G.idx[mid] < npix) a = mid + 1;
+            else hi = mid;
+        }
+        for (; a < b; a++) {
+            const uint32_t pi = G.idx[a];
+            if (!grid_near(G.g, G.pos, outRay.o, pi)) continue;
+            const uint32_t k = G.src[2 * (size_t)pi], j = G.src[2 * (size_t)pi + 1];
+            const BVert* L = verts + (size_t)k * 2 * maxv + maxv;
+            f3 thr = mk(1.f, 1.f, 1.f);
+            for (uint32_t m = 0; m < j; m++) {
+                const BVert lm = L[m];
+                thr = mulv(thr, divs(divs(lm.rad, lm.dp), lm.op));
+                thr = muls(thr, cos_term_v(lm));
+            }
+            BVert ph = L[j];
+            thr = divs(thr, ph.op);
+            ph.d = L[j - 1].d;
+            const f3 col = mulv(thr, eval_bsdf(S, ph, outRay));
+            const float l = len(sub(outRay.o, ph.o));
+            const float dist_sqr = l * l;  // powf(l, 2): the correctly rounded square
+            if (!legal(col)) continue;
+            const float q = 1.f - dist_sqr / (r * r);
+            const float sk = 0.f > q ? 0.f : q;
+            const float kernel = (float)((double)(3.f * sk * sk) / kPi);  // Kernel (photonMap.cpp:209-212)
+            const float norm = kernel / ((float)npix * r * r);
+            contrib = add(contrib, muls(col, norm));
+        }
+    }
+    return contrib;
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ppm(SceneDev S, CamDev cam, uint32_t npix, uint32_t npass, uint32_t maxv,
+                                             const BVert* verts, uint64_t seed, uint32_t pass0, float* pix_all,
+                                             const VcmGrid* grids) {
+    const uint64_t lane = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (lane >= (uint64_t)npass * npix) return;
+    const uint32_t kb = (uint32_t)(lane / npix), p = (uint32_t)(lane % npix);
+    const uint32_t pass = pass0 + kb;
+    const VcmGrid G = grids[kb];
+    StackSpill spill;
+    LaneStack st(&spill);
+    const SceneCtx sc = make_scene_ctx(S);
+    float rn[4];
+    rng4(seed, p, pass, 0, rn);
+    BVert c;
+    c.o = cam.pos;
+    c.d = cam_dir(cam, p, rn[0], rn[1]);
+    c.rad = mk(1.f, 1.f, 1.f);
+    c.n = cam.front;
+    c.op = 1.f;
+    c.dp = 1.f;
+    c.contact = WR_NONE;
+    c.flags = kVCamera;
+    c.pad = 0;
+    PHit hit = trace_scene<false>(sc, c.o, c.d, WR_NONE, WR_NONE, st);
+    c.inside = find_inside(S, c.o, c.d, WR_NONE, st);
+    f3 surface = mk(0.f, 0.f, 0.f);
+    f3 o = c.o, d = c.d;
+    uint32_t inside = c.inside;
+    for (uint32_t depth = 0;; depth++) {
+        const uint32_t self = inside != WR_NONE ? inside : hit.obj;
+        if (self == WR_NONE || hit.rec == WR_NONE) break;  // the terminating record: no contact
+        rng4(seed, p, pass, depth + 1, rn);
+        const Out r = scatter(sc, self, o, d, hit, inside, rn, st);
+        const bool absorbed = (double)len(r.d) < 0.5;
+        if (!absorbed && depth + 1 > kMergeLen) break;  // the terminating record
+        if (emissive(S, r.contact)) {  // the eye path hit a light
+            surface = r.rad;
+            break;
+        }
+        if (r.contact != WR_NONE && S.objs[r.contact].mat == WR_MAT_DIFFUSE) {  // nonSpecular()
+            BVert out;
+            out.o = r.o;
+            out.d = mk(-d.x, -d.y, -d.z);
+            out.rad = r.rad;
+            out.n = r.n;
+            out.op = r.op;
+            out.dp = r.dp;
+            out.inside = r.inside;
+            out.contact = r.contact;
+            out.flags = 0;
+            out.pad = 0;
+            surface = ppm_density(S, G, verts, maxv, npix, out);
+            break;
+        }
+        if (absorbed) break;
+        const PHit nh = trace_scene<true>(sc, r.o, r.d, r.inside, r.contact, st);
+        if (nh.rec == WR_NONE) break;
+        o = r.o;
+        d = r.d;
+        inside = r.inside;
+        hit = nh;
+    }
+    const f3 color = mulv(mulv(mk(1.f, 1.f, 1.f), surface), divs(divs(muls(c.rad, cos_term_v(c)), c.dp), c.op));
+    float* px = pix_all + 3 * lane;
+    px[0] = color.x;
+    px[1] = color.y;
+    px[2] = color.z;
+}
+
 }  // namespace wrd
 
 namespace wr {
@@ -840,61 +966,79 @@ static wrd::CamDev make_cam_dev(const CamHost& ch) {
     return cam;
 }
 
-int launch_vcm_passes(wr_ctx* ctx, const SceneDev& S, const wr_tree_hdr& H, const CamHost& ch, uint32_t max_depth,
-                      uint64_t seed, uint32_t pass0, uint32_t npass, float base_radius, uint32_t maxv, void* verts,
-                      float* pix, const BdptQueue& q, const VcmBufs& vb, const FilmBufs& film) {
+// The light sub-paths of passes [pass0, pass0 + npass) (and their eye sub-paths
+// when eye), their light points and one grid per pass, uploaded to vb.grids;
+// the grids go to `grids` (the caller destroys them).
+static int light_point_grids(wr_ctx* ctx, const SceneDev& S, const wrd::CamDev& cam, uint32_t npix, uint32_t max_depth,
+                             uint64_t seed, uint32_t pass0, uint32_t npass, float base_radius, uint32_t maxv,
+                             wrd::BVert* V, uint32_t* counts, const VcmBufs& vb, bool eye,
+                             std::vector<wr_hashgrid*>& grids, const char* who) {
     hipStream_t st = ctx->stream;
-    const wrd::CamDev cam = make_cam_dev(ch);
-    const uint32_t npix = ch.W * ch.H;
     const uint32_t lanes = npass * npix;  // the caller keeps npass * npix below 2^31
     const uint32_t blocks = (lanes + 255) / 256;
-    wrd::BVert* V = (wrd::BVert*)verts;
-    hipError_t e = hipMemsetAsync(pix, 0, sizeof(float) * 3 * (size_t)lanes, st);
-    if (e == hipSuccess) e = hipMemsetAsync(q.count, 0, sizeof(uint32_t), st);
-    if (e == hipSuccess) e = hipMemsetAsync(vb.pc + lanes, 0, sizeof(uint32_t), st);
-    if (e != hipSuccess) return hip_fail(ctx, e, "wr_render_vcm");
+    hipError_t e = hipMemsetAsync(vb.pc + lanes, 0, sizeof(uint32_t), st);
+    if (e != hipSuccess) return hip_fail(ctx, e, who);
     wrd::k_bdpt_subpaths<<<blocks, 256, wrd::kStackLdsBytes, st>>>(S, cam, max_depth, seed, pass0, npass, npix, maxv, V,
-                                                                  q.counts);
-    wrd::k_vcm_point_count<<<blocks, 256, 0, st>>>(S, V, q.counts, lanes, maxv, vb.pc);
-    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "wr_render_vcm");
+                                                                  counts, eye);
+    wrd::k_vcm_point_count<<<blocks, 256, 0, st>>>(S, V, counts, lanes, maxv, vb.pc);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, who);
     size_t tb = vb.scan_bytes;
     if ((e = hipcub::DeviceScan::ExclusiveSum(vb.scan, tb, vb.pc, vb.offs, (int)lanes + 1, st)) != hipSuccess)
-        return hip_fail(ctx, e, "wr_render_vcm: scan");
+        return hip_fail(ctx, e, who);
     wrd::k_vcm_pass_starts<<<(npass + 1 + 63) / 64, 64, 0, st>>>(vb.offs, npix, npass, vb.starts);
     std::vector<uint32_t> starts(npass + 1);
     if ((e = hipGetLastError()) != hipSuccess ||
         (e = hipMemcpyAsync(starts.data(), vb.starts, sizeof(uint32_t) * (npass + 1), hipMemcpyDeviceToHost, st)) !=
             hipSuccess ||
         (e = hipStreamSynchronize(st)) != hipSuccess)
-        return hip_fail(ctx, e, "wr_render_vcm");
+        return hip_fail(ctx, e, who);
     if ((uint64_t)lanes + starts[npass] > vb.cap)
-        return fail(ctx, WR_EINVAL, "wr_render_vcm: more light points than the buffer holds");
-    wrd::k_vcm_point_write<<<blocks, 256, 0, st>>>(S, V, q.counts, npix, lanes, maxv, vb.offs, vb.pos, vb.src);
-    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "wr_render_vcm");
-    // per pass: HashGrid::Reserve(pixels) + Build(its light points, its radius) (vcmTracer.cpp:30,33,57)
-    std::vector<wr_hashgrid*> grids(npass, nullptr);
+        return fail(ctx, WR_EINVAL, std::string(who) + ": more light points than the buffer holds");
+    wrd::k_vcm_point_write<<<blocks, 256, 0, st>>>(S, V, counts, npix, lanes, maxv, vb.offs, vb.pos, vb.src);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, who);
+    // per pass: HashGrid::Reserve(pixels) + Build(its light points, its radius)
+    // (vcmTracer.cpp:30,33,57; photonMap.cpp:48,56,93)
+    grids.assign(npass, nullptr);
     std::vector<wrd::VcmGrid> G(npass);
-    auto destroy = [&]() {
-        for (wr_hashgrid* g : grids) wr_hashgrid_destroy(g);
-    };
     for (uint32_t b = 0; b < npass; b++) {
         const uint64_t block = (uint64_t)b * npix + starts[b];
         const uint64_t n = (uint64_t)npix + (starts[b + 1] - starts[b]);
         const float r = vcm_radius(base_radius, pass0 + b);
         const int rc = wr_hashgrid_build(ctx, vb.pos + 3 * block, n, r, npix, &grids[b]);
-        if (rc != WR_OK) {
-            destroy();
-            return rc;
-        }
+        if (rc != WR_OK) return rc;
         G[b] = wrd::VcmGrid{grids[b]->g, grids[b]->ends, grids[b]->idx, vb.pos + 3 * block, vb.src + 2 * block, r};
     }
-    const wrd::VisQueue Q{q.rays, q.pay, q.dst, q.count, q.cap};
-    e = hipMemcpyAsync(vb.grids, G.data(), sizeof(wrd::VcmGrid) * npass, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) {
-        wrd::k_vcm<<<blocks, 256, wrd::kStackLdsBytes, st>>>(S, cam, npix, npass, maxv, V, q.counts, pix, Q,
-                                                            (const wrd::VcmGrid*)vb.grids);
-        e = hipGetLastError();
+    if ((e = hipMemcpyAsync(vb.grids, G.data(), sizeof(wrd::VcmGrid) * npass, hipMemcpyHostToDevice, st)) != hipSuccess)
+        return hip_fail(ctx, e, who);
+    return WR_OK;
+}
+
+int launch_vcm_passes(wr_ctx* ctx, const SceneDev& S, const wr_tree_hdr& H, const CamHost& ch, uint32_t max_depth,
+                      uint64_t seed, uint32_t pass0, uint32_t npass, float base_radius, uint32_t maxv, void* verts,
+                      float* pix, const BdptQueue& q, const VcmBufs& vb, const FilmBufs& film) {
+    hipStream_t st = ctx->stream;
+    const wrd::CamDev cam = make_cam_dev(ch);
+    const uint32_t npix = ch.W * ch.H;
+    const uint32_t lanes = npass * npix;
+    const uint32_t blocks = (lanes + 255) / 256;
+    wrd::BVert* V = (wrd::BVert*)verts;
+    hipError_t e = hipMemsetAsync(pix, 0, sizeof(float) * 3 * (size_t)lanes, st);
+    if (e == hipSuccess) e = hipMemsetAsync(q.count, 0, sizeof(uint32_t), st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "wr_render_vcm");
+    std::vector<wr_hashgrid*> grids;
+    auto destroy = [&]() {
+        for (wr_hashgrid* g : grids) wr_hashgrid_destroy(g);
+    };
+    int rc = light_point_grids(ctx, S, cam, npix, max_depth, seed, pass0, npass, base_radius, maxv, V, q.counts, vb,
+                               true, grids, "wr_render_vcm");
+    if (rc != WR_OK) {
+        destroy();
+        return rc;
     }
+    const wrd::VisQueue Q{q.rays, q.pay, q.dst, q.count, q.cap};
+    wrd::k_vcm<<<blocks, 256, wrd::kStackLdsBytes, st>>>(S, cam, npix, npass, maxv, V, q.counts, pix, Q,
+                                                        (const wrd::VcmGrid*)vb.grids);
+    e = hipGetLastError();
     uint32_t nq = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&nq, q.count, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -909,6 +1053,36 @@ int launch_vcm_passes(wr_ctx* ctx, const SceneDev& S, const wr_tree_hdr& H, cons
     wrd::k_bdpt_fold<<<(npix + 255) / 256, 256, 0, st>>>(cam, npass, npix, pix,
                                                          wrd::FilmDev{film.sum, film.comp, film.sumsq});
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(ctx, e, "wr_render_vcm");
+    return WR_OK;
+}
+
+int launch_ppm_passes(wr_ctx* ctx, const SceneDev& S, const CamHost& ch, uint32_t max_depth, uint64_t seed,
+                      uint32_t pass0, uint32_t npass, float base_radius, uint32_t maxv, void* verts, uint32_t* counts,
+                      float* pix, const VcmBufs& vb, const FilmBufs& film) {
+    hipStream_t st = ctx->stream;
+    const wrd::CamDev cam = make_cam_dev(ch);
+    const uint32_t npix = ch.W * ch.H;
+    const uint32_t lanes = npass * npix;
+    const uint32_t blocks = (lanes + 255) / 256;
+    wrd::BVert* V = (wrd::BVert*)verts;
+    std::vector<wr_hashgrid*> grids;
+    auto destroy = [&]() {
+        for (wr_hashgrid* g : grids) wr_hashgrid_destroy(g);
+    };
+    int rc = light_point_grids(ctx, S, cam, npix, max_depth, seed, pass0, npass, base_radius, maxv, V, counts, vb,
+                               false, grids, "wr_render_ppm");
+    if (rc != WR_OK) {
+        destroy();
+        return rc;
+    }
+    wrd::k_ppm<<<blocks, 256, wrd::kStackLdsBytes, st>>>(S, cam, npix, npass, maxv, V, seed, pass0, pix,
+                                                        (const wrd::VcmGrid*)vb.grids);
+    wrd::k_bdpt_fold<<<(npix + 255) / 256, 256, 0, st>>>(cam, npass, npix, pix,
+                                                         wrd::FilmDev{film.sum, film.comp, film.sumsq});
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);  // the grids are read until k_ppm ends
+    destroy();
+    if (e != hipSuccess) return hip_fail(ctx, e, "wr_render_ppm");
     return WR_OK;
 }
 

@@ -419,6 +419,87 @@ extern "C" int wr_render_vcm(wr_ctx* ctx, const wr_scene* s, const wr_camera* ca
     return WR_OK;
 }
 
+extern "C" int wr_render_ppm(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_render_params* prm,
+                             float base_radius, wr_film* film, wr_render_stats* stats) {
+    if (!ctx || !s || !cam || !prm || !film) return fail(ctx, WR_EINVAL, "wr_render_ppm: bad argument");
+    if (!s->built) return fail(ctx, WR_ESTATE, "wr_render_ppm: scene not built");
+    if (film->W != cam->width || film->H != cam->height)
+        return fail(ctx, WR_EINVAL, "wr_render_ppm: film size differs from the camera resolution");
+    if (prm->mode != 0 || prm->wave_grid || prm->flags || prm->wave_slots)
+        return fail(ctx, WR_EINVAL, "wr_render_ppm: mode, wave_slots, wave_grid and flags must be 0");
+    if (prm->max_depth > kBdptMaxDepth) return fail(ctx, WR_EINVAL, "wr_render_ppm: max_depth above 32");
+    if (!(base_radius > 0.f) || !std::isfinite(base_radius))
+        return fail(ctx, WR_EINVAL, "wr_render_ppm: base_radius must be finite and > 0");
+    for (const HostObject& o : s->objs)
+        if (o.moving) return fail(ctx, WR_EINVAL, "wr_render_ppm: moving objects are not supported");
+    const uint64_t npix = (uint64_t)cam->width * cam->height;
+    const uint32_t maxv = prm->max_depth + 3;
+    if (npix == 0 || npix * maxv >= (1ull << 31)) return fail(ctx, WR_EINVAL, "wr_render_ppm: bad image size");
+    if (film->broken) return film_broken(film, "wr_render_ppm");
+    (void)hipSetDevice(ctx->device);
+    const CamHost ch = make_cam(*cam);
+    const uint64_t per_pass = bdpt_vertex_bytes() * 2 * maxv * npix;
+    uint64_t batch = std::max<uint64_t>(1, (4ull << 30) / per_pass);
+    batch = std::min<uint64_t>(std::min<uint64_t>(batch, prm->spp ? prm->spp : 1), std::max<uint64_t>(1, (1ull << 22) / npix));
+    batch = std::min<uint64_t>(batch, std::max<uint64_t>(1, ((1ull << 31) - 1) / (npix * maxv)));
+    const uint64_t lanes = batch * npix;
+    VcmBufs vb{};
+    vb.cap = lanes * maxv;
+    vb.scan_bytes = vcm_scan_bytes((uint32_t)lanes);
+    void* verts = nullptr;
+    float* pix = nullptr;
+    uint32_t* counts = nullptr;
+    hipError_t e = hipMalloc(&verts, per_pass * batch);
+    if (e == hipSuccess) e = hipMalloc((void**)&pix, sizeof(float) * 3 * lanes);
+    if (e == hipSuccess) e = hipMalloc((void**)&counts, sizeof(uint32_t) * 2 * lanes);
+    if (e == hipSuccess) e = hipMalloc((void**)&vb.pc, sizeof(uint32_t) * (lanes + 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&vb.offs, sizeof(uint32_t) * (lanes + 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&vb.starts, sizeof(uint32_t) * (batch + 1));
+    if (e == hipSuccess) e = hipMalloc(&vb.scan, std::max<size_t>(vb.scan_bytes, 4));
+    if (e == hipSuccess) e = hipMalloc((void**)&vb.pos, sizeof(float) * 3 * vb.cap);
+    if (e == hipSuccess) e = hipMalloc((void**)&vb.src, sizeof(uint32_t) * 2 * vb.cap);
+    if (e == hipSuccess) e = hipMalloc(&vb.grids, vcm_grid_bytes() * batch);
+    auto release = [&]() {
+        for (void* ptr : {verts, (void*)pix, (void*)counts, (void*)vb.pc, (void*)vb.offs, (void*)vb.starts, vb.scan,
+                          (void*)vb.pos, (void*)vb.src, vb.grids})
+            (void)hipFree(ptr);
+    };
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+    if (e != hipSuccess) {
+        release();
+        return hip_fail(ctx, e, "wr_render_ppm");
+    }
+    const FilmBufs fb{film->sum, film->comp, film->sumsq};
+    for (uint32_t k = 0; k < prm->spp; k += (uint32_t)batch) {
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(batch, prm->spp - k);
+        const int rc = launch_ppm_passes(ctx, s->dev, ch, prm->max_depth, prm->seed, film->passes + k, nb, base_radius,
+                                         maxv, verts, counts, pix, vb, fb);
+        if (rc != WR_OK) {  // the film may hold part of this call's passes
+            (void)hipStreamSynchronize(ctx->stream);
+            release();
+            film->broken = true;
+            return rc;
+        }
+    }
+    e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    release();
+    if (e != hipSuccess) {
+        film->broken = film->broken || prm->spp > 0;
+        return hip_fail(ctx, e, "wr_render_ppm");
+    }
+    film->passes += prm->spp;
+    if (stats) {
+        stats->paths = npix * prm->spp;
+        stats->iterations = 0;
+        stats->kernel_ms = ms;
+        stats->build_ms = 0.f;
+    }
+    return WR_OK;
+}
+
 extern "C" int wr_set_render_tuning(wr_ctx* ctx, const wr_render_tuning* t) {
     if (!ctx || !t) return fail(ctx, WR_EINVAL, "wr_set_render_tuning: bad argument");
     const uint32_t lv = t->lds_levels ? t->lds_levels : 8u;

@@ -75,6 +75,12 @@ size_t vcm_grid_bytes();
 int launch_vcm_passes(wr_ctx* ctx, const SceneDev& S, const wr_tree_hdr& H, const CamHost& ch, uint32_t max_depth,
                       uint64_t seed, uint32_t pass0, uint32_t npass, float base_radius, uint32_t maxv, void* verts,
                       float* pix, const BdptQueue& q, const VcmBufs& vb, const FilmBufs& film);
+// PhotonMap (bdpt.hip): passes [pass0, pass0 + npass) side by side [light
+// sub-paths, light points, one hash grid per pass, eye merge paths and density
+// estimates, film]; buffers as for launch_vcm_passes (counts: 2 per lane).
+int launch_ppm_passes(wr_ctx* ctx, const SceneDev& S, const CamHost& ch, uint32_t max_depth, uint64_t seed,
+                      uint32_t pass0, uint32_t npass, float base_radius, uint32_t maxv, void* verts, uint32_t* counts,
+                      float* pix, const VcmBufs& vb, const FilmBufs& film);
 }  // namespace wr
 
 struct wr_scene {

@@ -290,6 +290,26 @@ def bench_vcm(ctx, log, spp=16, depth=8, radius=0.01, repeats=2):
     return r
 
 
+def bench_ppm(ctx, log, spp=16, depth=8, radius=0.01, repeats=2):
+    """PhotonMap's progressive photon mapping on the C1 Cornell box (256^2, 16 spp, light
+    paths of maxPathLen 8, base radius 0.01 in the unit box): GPU samples/s, best of `repeats`."""
+    objs, cam = S.cornell(k=5, width=256, height=256)
+    sc = W.Scene(ctx, objs)
+    npix = cam["width"] * cam["height"]
+    best, film = None, None
+    for _ in range(repeats):
+        film = W.Film(ctx, cam["width"], cam["height"])
+        st = W.render_ppm(sc, cam, film, spp, depth, 1234, radius)
+        best = st["kernel_ms"] if best is None else min(best, st["kernel_ms"])
+    f = film.read()
+    r = {"workload": f"PhotonMap (progressive photon mapping), Cornell box + 2 icospheres k=5, 256x256, {spp} spp, "
+                     f"light paths of maxPathLen {depth}, base radius {radius}",
+         "ms": round(best, 3), "samples_per_s": round(npix * spp / (best * 1e-3), 1),
+         "mean_radiance": round(float(f["mean"].mean()), 5)}
+    log(f"[bench] PPM C1: {best:.1f} ms, {r['samples_per_s'] / 1e6:.2f} Msamples/s")
+    return r
+
+
 def bench_large(W_, S_, ctx, log, spp=256, depth=32):
     """Large scene: 43.8M triangles, 3840x2160, 256 spp, depth 32; build time, both engines."""
     t0 = time.time()
