@@ -12,6 +12,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/walkray.h"
@@ -362,8 +363,9 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
         std::vector<wr_node> nodes;
         std::vector<wr_wnode> wnodes;
         std::vector<wr_tri> tris;
-        std::vector<uint32_t> leaf_up, leaf_cnt, node_up, rec_prim, rec_obj, rec_first, leaf_chain;
+        std::vector<uint32_t> leaf_up, leaf_cnt, rec_prim, rec_obj, rec_first, leaf_chain;
         std::vector<wr_chain> chains;
+        std::vector<float> top_boxes;
         std::vector<float> rec_nrm;
         std::vector<wr_tree_hdr> hdrs;
         std::vector<wr_object_gpu> objs;
@@ -381,17 +383,11 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
                 return fail(ctx, WR_EINVAL, "wr_scene_build: scene too large");
             hdrs.push_back(h);
             nodes.insert(nodes.end(), F.nodes.begin(), F.nodes.end());
-            for (size_t nd = 0; nd < F.node_up.size(); nd++) {  // next four ancestor links of every node
-                uint32_t up = F.node_up[nd];
-                for (int k = 0; k < 4; k++) {
-                    node_up.push_back(up);
-                    if (up != WR_NONE) up = F.node_up[up >> 1];
-                }
-            }
             tris.insert(tris.end(), F.tris.begin(), F.tris.end());
             leaf_up.insert(leaf_up.end(), F.leaf_up.begin(), F.leaf_up.end());
             leaf_cnt.insert(leaf_cnt.end(), F.leaf_cnt.begin(), F.leaf_cnt.end());
             rec_first.insert(rec_first.end(), F.rec_first.begin(), F.rec_first.end());
+            std::unordered_map<uint32_t, std::pair<uint32_t, uint32_t>> top_of;  // link -> (first box, boxes)
             for (size_t rr = 0; rr < F.rec_first.size(); rr++) {  // chain record of every non-root leaf
                 uint32_t up = F.leaf_up[rr];
                 if (F.rec_first[rr] != rr || up == WR_NONE) {
@@ -411,6 +407,21 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
                     up = F.node_up[up >> 1];
                 }
                 c.next = up;
+                if (up != WR_NONE) {  // the top record of this link, built once per tree and link
+                    auto it = top_of.find(up);
+                    if (it == top_of.end()) {
+                        const uint32_t first = (uint32_t)(top_boxes.size() / 6);
+                        uint32_t n = 0;
+                        for (uint32_t u = up; u != WR_NONE; u = F.node_up[u >> 1], n++) {
+                            const wr_node& N = F.nodes[u >> 1];
+                            const float* b = (u & 1u) ? N.rmin : N.lmin;
+                            top_boxes.insert(top_boxes.end(), b, b + 6);
+                        }
+                        it = top_of.emplace(up, std::make_pair(first, n)).first;
+                    }
+                    c.top = it->second.first;
+                    c.top_count = it->second.second;
+                }
                 chains.push_back(c);
             }
             rec_prim.insert(rec_prim.end(), F.rec_prim.begin(), F.rec_prim.end());
@@ -552,7 +563,7 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
         wr_node* d_nodes;
         wr_wnode* d_wnodes;
         wr_tri* d_tris;
-        uint32_t *d_lu, *d_lc, *d_nu, *d_hn;
+        uint32_t *d_lu, *d_lc, *d_hn;
         wr_chain* d_ch;
         float* d_rn;
         wr_tree_hdr* d_h;
@@ -582,8 +593,10 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
         s->dbufs.push_back(d_lc);
         if ((rc = upload(ctx, chains, &d_ch, bytes))) return rc;
         s->dbufs.push_back(d_ch);
-        if ((rc = upload(ctx, node_up, &d_nu, bytes))) return rc;
-        s->dbufs.push_back(d_nu);
+        float* d_tb;
+        top_boxes.resize(top_boxes.size() + 5 * 6, 0.f);  // five boxes' room past every record
+        if ((rc = upload(ctx, top_boxes, &d_tb, bytes))) return rc;
+        s->dbufs.push_back(d_tb);
         if ((rc = upload(ctx, rec_nrm, &d_rn, bytes))) return rc;
         s->dbufs.push_back(d_rn);
         if ((rc = upload(ctx, hdrs, &d_h, bytes))) return rc;
@@ -614,7 +627,7 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
         D.leaf_up = d_lu;
         D.leaf_cnt = d_lc;
         D.chains = d_ch;
-        D.node_anc = (const uint4*)d_nu;
+        D.top_boxes = d_tb;
         D.rec_nrm = d_rn;
         D.hdrs = d_h;
         D.objs = d_o;

@@ -41,7 +41,7 @@ struct Tree {
     const uint32_t* leaf_up;   // offset like tris
     const uint32_t* leaf_cnt;  // offset like tris
     const wr_chain* chains;
-    const uint4* node_anc;     // offset like nodes
+    const float* top_boxes;    // the scene's pool (chain records hold global box indices)
     float rmin[3], rmax[3];
     uint32_t root_ref;
     uint32_t wide_root;
@@ -56,7 +56,7 @@ __device__ __forceinline__ Tree make_tree(const wr_tree_hdr& h, const SceneDev& 
     t.leaf_up = S.leaf_up + h.rec_base;
     t.leaf_cnt = S.leaf_cnt + h.rec_base;
     t.chains = S.chains;
-    t.node_anc = S.node_anc + h.node_base;
+    t.top_boxes = S.top_boxes;
     for (int a = 0; a < 3; a++) {
         t.rmin[a] = h.root_min[a];
         t.rmax[a] = h.root_max[a];
@@ -615,25 +615,15 @@ __device__ __forceinline__ void traverse_wide(const Tree& tr, const RayPre& r, c
 // Re-check the winner's root-to-leaf chain with the reference box test.  The
 // ten levels nearest the leaf come from the leaf's chain record (two lines,
 // five levels each; the leaf key is its index, so its fetch does not wait for
-// the winner's triangle record); above them the links come four at a time
-// (node_anc), so the box fetches of four levels are issued together instead
-// of one dependent fetch per level, each reading only the 24 B box of the
-// chain's side of the node.
+// the winner's triangle record); the levels above them from the chain's top
+// record (the boxes up to the root, shared by every chain that ends below the
+// same link), five at a time with their loads issued together.
 struct Box6 {
     float mn[3], mx[3];
 };
 
-__device__ __forceinline__ Box6 side_box(const Tree& tr, uint32_t up) {
-    const float* p = reinterpret_cast<const float*>(tr.nodes + (up >> 1)) + 6 * (up & 1u);
-    Box6 b;
-    for (int k = 0; k < 3; k++) {
-        b.mn[k] = p[k];
-        b.mx[k] = p[3 + k];
-    }
-    return b;
-}
-
-// Five boxes of a chain record (one 128 B line's worth), loaded before any is tested.
+// The first n (at most five) of the five boxes of 6 floats at p, all five
+// loaded before any is tested (every caller's p has five boxes' room).
 __device__ __forceinline__ bool chain_five(const RayPre& r, const float* p, uint32_t n) {
     float b[5][6];
 #pragma unroll
@@ -652,29 +642,14 @@ __device__ __forceinline__ bool verify_chain(const Tree& tr, const RayPre& r, ui
     const wr_chain* C = tr.chains + leaf;
     const uint32_t count = C->count;
     const uint32_t up = C->next;
-    if (!chain_five(r, &C->box[0][0], count)) return false;
+    const uint32_t top = C->top, ntop = C->top_count;
+    if (!chain_five(r, &C->box[0][0], count < 5 ? count : 5)) return false;
     if (count > 5 && !chain_five(r, &C->box[5][0], count - 5)) return false;
     if (up != WR_NONE) {
-        {
-            const Box6 b = side_box(tr, up);
-            if (!ref_box(r, b.mn, b.mx)) return false;
-        }
-        uint32_t node = up >> 1;
-        for (;;) {
-            const uint4 a = tr.node_anc[node];
-            if (a.x == WR_NONE) break;
-            const Box6 b0 = side_box(tr, a.x);
-            const Box6 b1 = side_box(tr, a.y == WR_NONE ? a.x : a.y);
-            const Box6 b2 = side_box(tr, a.z == WR_NONE ? a.x : a.z);
-            const Box6 b3 = side_box(tr, a.w == WR_NONE ? a.x : a.w);
-            if (!ref_box(r, b0.mn, b0.mx)) return false;
-            if (a.y == WR_NONE) break;
-            if (!ref_box(r, b1.mn, b1.mx)) return false;
-            if (a.z == WR_NONE) break;
-            if (!ref_box(r, b2.mn, b2.mx)) return false;
-            if (a.w == WR_NONE) break;
-            if (!ref_box(r, b3.mn, b3.mx)) return false;
-            node = a.w >> 1;
+        const float* T = tr.top_boxes + 6 * (size_t)top;
+        for (uint32_t k = 0; k < ntop; k += 5) {
+            const uint32_t m = ntop - k;
+            if (!chain_five(r, T + 6 * (size_t)k, m < 5 ? m : 5)) return false;
         }
     }
     return ref_box(r, tr.rmin, tr.rmax);

@@ -14,7 +14,7 @@ struct SceneDev {
     const uint32_t* leaf_up;
     const uint32_t* leaf_cnt;
     const wr_chain* chains;
-    const uint4* node_anc;           // internal node -> its next four ancestor links (parent << 1 | side), WR_NONE past the root
+    const float* top_boxes;          // chain top records: 6 floats per box (wr_chain::top)
     const float* rec_nrm;            // record -> 9 floats: vertex normals (objects with normals)
     const wr_tree_hdr* hdrs;         // [0] = global tree, [1 + i] = object i
     const wr_object_gpu* objs;

@@ -78,13 +78,18 @@ static_assert(__builtin_offsetof(wr_wnode, q) == 24 && __builtin_offsetof(wr_wno
 // ancestor-k child on the path, stored in ancestor k + 1), their number, and
 // the link (parent << 1 | side) of the next level up, WR_NONE when the chain
 // ends at the root.  The first line holds the header and the five levels
-// nearest the leaf.
+// nearest the leaf.  The levels above the chain come from its top record:
+// the boxes from `next` up to the root (top_count of them, 6 floats each, at
+// box top of the scene's top-box pool), stored once for all the chains that
+// end below the same link; the upper levels are few and shared by many
+// leaves, so these records stay in L2.
 constexpr uint32_t kChainLevels = 10;
 struct wr_chain {
     uint32_t count;
     uint32_t next;
     float box[kChainLevels][6];
-    uint32_t pad[2];
+    uint32_t top;        // first box of the top record in the pool (when next != WR_NONE)
+    uint32_t top_count;  // its boxes: the levels from next up to the root
 };
 static_assert(sizeof(wr_chain) == 256, "chain record must be 256 bytes");
 static_assert(__builtin_offsetof(wr_chain, box) + 5 * 24 == 128, "the first line holds five levels");
