@@ -429,6 +429,7 @@ typedef struct {
     wr_camera camera;
     char renderer[32];    /* <renderer> as written ("PathTracer", "PT", ...), "" if absent */
     char scene_name[64];
+    float radius;         /* <radius> (VCMTracer / PhotonMap base radius, config.cpp:560,571), 0 if absent */
 } wr_config_info;
 int wr_config_load(const char* path, wr_config** out);
 void wr_config_free(wr_config* cfg);

@@ -176,6 +176,10 @@ def test_config_refuses_off_path_features(tmp_path):
         W.SceneConfig(_scene(tmp_path, obj, extra_cam="<lensRadius>1</lensRadius><focalDist>5</focalDist>"))
     cfg = W.SceneConfig(_scene(tmp_path, obj, renderer=""))
     assert cfg.renderer == "" and cfg.spp == 0 and cfg.max_len == 0
+    assert cfg.radius == 0.0
+    vcm = W.SceneConfig(_scene(tmp_path, obj, renderer="<renderer>VCMTracer</renderer><iter>8</iter><radius>0.004</radius>"))
+    assert (vcm.renderer, vcm.spp, vcm.max_len) == ("VCMTracer", 8, 20)
+    assert vcm.radius == np.float32(0.004)  # setRadius(atof(...)) (config.cpp:560)
 
 
 def test_config_normals_follow_inverse_transpose(tmp_path):

@@ -102,7 +102,7 @@ def set_render_tuning(ctx, wave_slots=0, wave_grid=0, lds_levels=8):
 
 class ConfigInfo(C.Structure):
     _fields_ = [("n_objects", C.c_uint32), ("spp", C.c_int32), ("max_len", C.c_int32), ("camera", CameraDesc),
-                ("renderer", C.c_char * 32), ("scene_name", C.c_char * 64)]
+                ("renderer", C.c_char * 32), ("scene_name", C.c_char * 64), ("radius", C.c_float)]
 
 
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = 0, 1, 2, 3, 4
@@ -319,6 +319,7 @@ class SceneConfig:
                        "height": c.height, "sight_dist": c.sight_dist}
         self.spp, self.max_len = info.spp, info.max_len
         self.renderer = info.renderer.decode()
+        self.radius = float(info.radius)
         self.name = info.scene_name.decode()
         self.objects = [self._object(i) for i in range(info.n_objects)]
 

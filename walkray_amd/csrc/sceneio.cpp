@@ -789,6 +789,7 @@ struct ConfigLoader {
             out.spp = INT_MAX;  // config.cpp:521-527
             if (const char* it = cfg->val("iter")) out.spp = (int)atof(it);
             out.max_len = 20;
+            if (const char* rd = cfg->val("radius")) out.radius = (float)atof(rd);  // setRadius(atof(...))
         }
     }
 };
@@ -852,6 +853,7 @@ int wr_config_get_info(const wr_config* cfg, wr_config_info* info) {
     info->n_objects = (uint32_t)s.objects.size();
     info->spp = s.spp;
     info->max_len = s.max_len;
+    info->radius = s.radius;
     info->camera = s.cam;
     snprintf(info->renderer, sizeof info->renderer, "%s", s.renderer.c_str());
     snprintf(info->scene_name, sizeof info->scene_name, "%s", s.name.c_str());

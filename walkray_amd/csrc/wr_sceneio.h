@@ -74,6 +74,7 @@ struct Scene {
     std::string renderer;  // empty when the configuration names none
     int spp = 0;
     int max_len = 0;
+    float radius = 0.f;  // <radius> of VCMTracer / PhotonMap (config.cpp:560,571), 0 if absent
 };
 
 // Config::load (config.cpp:432-575).  Throws std::runtime_error with a message.
