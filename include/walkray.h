@@ -108,13 +108,15 @@ void wr_scene_destroy(wr_scene* scene);
  * mesh.h:59-65).  *obj_id receives the object id. */
 int wr_scene_add_mesh(wr_scene* scene, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
                       const float* normals, const wr_material* mat, uint32_t* obj_id);
-/* Scene::buildKDTree + buildObjKDTrees (scene.cpp:20-65) with
- * KDTree::build(BEST) (kdtree.cpp:242-258), flattened and uploaded. */
 /* wr_scene_add_mesh with shading normals per triangle corner (nt * 9): the
  * world normals Mesh::getWorldNormal interpolates (mesh.h:59-72), for meshes
  * whose normal indices differ from their position indices. */
 int wr_scene_add_mesh_corners(wr_scene* scene, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
                               const float* corner_normals, const wr_material* mat, uint32_t* obj_id);
+/* Scene::buildKDTree + buildObjKDTrees (scene.cpp:20-65) with
+ * KDTree::build(BEST) (kdtree.cpp:242-258), flattened and uploaded.
+ * WR_EINVAL for a NaN vertex coordinate in a triangle: the split rule's
+ * median (std::nth_element, kdtree.cpp:179) is undefined on it. */
 int wr_scene_build(wr_scene* scene, int strategy);
 int wr_scene_get_info(const wr_scene* scene, wr_scene_info* info);
 /* prim -> (object id, triangle id in the object): the ObjSourceInfo of

@@ -47,7 +47,9 @@ def _soup(objs):
 def _same(a, b):
     for k in ("right", "leaf_first", "leaf_count", "tri_list"):
         assert np.array_equal(a[k], b[k]), k
-    assert np.array_equal(a["box"], b["box"])
+    # bit for bit: the sign of a zero bound is the reference fold's (first of equal values)
+    assert np.array_equal(np.asarray(a["box"], np.float32).view(np.uint32),
+                          np.asarray(b["box"], np.float32).view(np.uint32))
 
 
 @pytest.mark.parametrize("case", ["heightfield", "cornell", "field", "soup", "tiny"])
@@ -75,6 +77,23 @@ def test_product_builder_equals_oracle(case):
 def test_builder_large_heightfield_equals_oracle():
     hf = S.heightfield(400, seed=7)
     _same(O.Tree(hf["verts"], hf["tris"]).export(), W.build_tree_export(hf["verts"], hf["tris"]))
+
+
+def test_builder_signed_zeros_and_nan():
+    """Signed zeros everywhere (the box fold keeps the first of equal values, so the
+    bound's sign is the first vertex's): the oracle's tree bit for bit, whatever the
+    thread count.  A NaN coordinate leaves the split rule's median undefined
+    (std::nth_element needs a strict weak order): the build is refused."""
+    rng = np.random.default_rng(5)
+    v = rng.choice(np.array([0.0, -0.0, 1.0, -1.0, 0.5], np.float32), size=(4000, 3))
+    t = rng.integers(0, 4000, (70000, 3)).astype(np.uint32)
+    for nth in (1, 8):
+        _same(O.Tree(v, t).export(), W.build_tree_export(v, t, nthreads=nth))
+    v2 = v.copy()
+    v2[int(t[5, 2]), 1] = np.nan
+    for nth in (1, 8):
+        with pytest.raises(W.WalkRayError, match="NaN"):
+            W.build_tree_export(v2, t, nthreads=nth)
 
 
 def test_builder_rejects_bad_indices():

@@ -16,12 +16,16 @@
 //   * leaf if the best ratio is 0 or 1 (kdtree.cpp:211-215);
 //   * order-preserving partition, "any vertex <= split" goes left
 //     (kdtree.cpp:220-236).
-// What differs is only how it runs: one shared triangle-index array is
-// partitioned in place (stable), distinct vertices are found with a per-thread
-// stamp array instead of an unordered_set, large subtrees are built on their
-// own threads, and the recursion leaves the triangles of each leaf contiguous
-// and in left-to-right leaf order, which is exactly the record order the GPU
-// layout wants.
+// What differs is only how it runs: one array of triangle records (vertex
+// coordinates and indices) is partitioned in place (stable), distinct vertices
+// are found with a per-thread stamp array instead of an unordered_set, the
+// median is a selection chosen by size (branch-free rank count, branch-free
+// quickselect, radix select) that returns std::nth_element's value, large
+// subtrees are built on their own threads and the scans of a large node are
+// split over workers, and the recursion leaves the triangles of each leaf
+// contiguous and in left-to-right leaf order, which is exactly the record
+// order the GPU layout wants.  A NaN vertex coordinate is refused: the
+// reference's median (std::nth_element, kdtree.cpp:179) is undefined on it.
 #include <algorithm>
 #include <array>
 #include <atomic>
@@ -39,28 +43,181 @@ namespace {
 
 struct BNode {
     uint32_t b = 0, e = 0;
+    uint32_t n_int = 0;  // internal nodes in this subtree (0 for a leaf)
     float box[6];
-    std::unique_ptr<BNode> left, right;
+    std::unique_ptr<BNode[]> kids;  // left, right; null for a leaf
+};
+
+// One triangle as the split sees it: its vertex coordinates (vertex-major,
+// as the reference reads them), its vertex indices and its input index.
+// These records are partitioned in place of the triangle order, so every scan
+// over a node reads them in sequence and the vertex array is read only once.
+struct TriRec {
+    float p[9];
+    uint32_t v[3];
+    uint32_t ti;
 };
 
 struct Scratch {
-    std::vector<uint32_t> stamp;
+    // per vertex: the id of the last node that met it (atomic so the workers
+    // of one large node can share it; relaxed loads and stores elsewhere)
+    std::unique_ptr<std::atomic<uint32_t>[]> stamp;
     uint32_t stamp_id = 0;
-    std::vector<uint32_t> uniq;
-    std::vector<float> coords;
+    std::vector<float> u[3];  // per dim, the coordinates of the node's distinct vertices
 };
+
+
+// Order-preserving map of a non-NaN float to an unsigned key (-0 sorts just
+// below +0, which compare equal as floats).
+inline uint32_t float_key(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
+}
+
+inline float key_float(uint32_t k) {
+    const uint32_t u = (k & 0x80000000u) ? (k ^ 0x80000000u) : ~k;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// The k-th smallest of x (NaN-free, k < x.size()) by a most-significant-digit
+// radix select over float_key: three counting passes of 11, 11 and 10 bits,
+// each keeping only the bucket that holds rank k.  The value equals the one
+// std::nth_element puts at position k, but for the sign of a zero, which no
+// comparison against it can see.  Reorders nothing; x is only read.
+float select_kth(const std::vector<float>& x, size_t k) {
+    uint32_t hist[2048];
+    std::vector<uint32_t> keep;
+    const size_t n = x.size();
+    // first digit: over x directly
+    std::fill(hist, hist + 2048, 0u);
+    for (size_t i = 0; i < n; i++) hist[float_key(x[i]) >> 21]++;
+    uint32_t d = 0;
+    while (k >= hist[d]) k -= hist[d++];
+    uint32_t prefix = d << 21;
+    keep.resize(hist[d] + 1);  // + 1: the branch-free filter writes one past the last kept key
+    size_t m = 0;
+    for (size_t i = 0; i < n; i++) {
+        const uint32_t key = float_key(x[i]);
+        keep[m] = key;
+        m += (key >> 21) == d;
+    }
+    // second and third digits: over the kept keys, compacted in place
+    const int shift[2] = {10, 0};
+    const uint32_t width[2] = {11, 10};
+    for (int p = 0; p < 2; p++) {
+        const uint32_t mask = (1u << width[p]) - 1u;
+        std::fill(hist, hist + (1u << width[p]), 0u);
+        for (size_t i = 0; i < m; i++) hist[(keep[i] >> shift[p]) & mask]++;
+        d = 0;
+        while (k >= hist[d]) k -= hist[d++];
+        prefix |= d << shift[p];
+        size_t w = 0;
+        for (size_t i = 0; i < m; i++) {
+            const uint32_t key = keep[i];
+            keep[w] = key;
+            w += ((key >> shift[p]) & mask) == d;
+        }
+        m = w;
+    }
+    return key_float(prefix);
+}
+
+// The k-th smallest of a[0, n) (NaN-free, k < n), reordering a: quickselect
+// with branch-free partitions (the swap-always form of Lomuto's scheme), one
+// pass splitting off the values below the pivot and, when rank k is not among
+// them, one splitting the rest at the pivot, so runs of equal values end it
+// (std::nth_element takes over after 64 rounds).  Same value as std::nth_element but for the sign of a zero.
+float quickselect(float* a, size_t n, size_t k) {
+    size_t lo = 0, hi = n;
+    for (int budget = 64; hi - lo > 12; budget--) {
+        if (budget == 0) {  // bad pivots all along: the library's introselect
+            std::nth_element(a + lo, a + k, a + hi);
+            return a[k];
+        }
+        const float x = a[lo], y = a[lo + (hi - lo) / 2], z = a[hi - 1];
+        const float p = std::max(std::min(x, y), std::min(std::max(x, y), z));  // median of three
+        size_t j = lo;
+        for (size_t i = lo; i < hi; i++) {
+            const float v = a[i];
+            a[i] = a[j];
+            a[j] = v;
+            j += v < p;
+        }
+        if (k < j) {
+            hi = j;
+            continue;
+        }
+        size_t m = j;
+        for (size_t i = j; i < hi; i++) {
+            const float v = a[i];
+            a[i] = a[m];
+            a[m] = v;
+            m += !(p < v);
+        }
+        if (k < m) return p;
+        lo = m;
+    }
+    for (size_t i = lo + 1; i < hi; i++)  // insertion sort of the last few
+        for (size_t j = i; j > lo && a[j] < a[j - 1]; j--) std::swap(a[j], a[j - 1]);
+    return a[k];
+}
+
+// The k-th smallest of a[0, n) for a small NaN-free n, without branches: the
+// largest value with at most k values below it.
+float select_small(const float* a, size_t n, size_t k) {
+    float best = -INFINITY;
+    for (size_t i = 0; i < n; i++) {
+        uint32_t below = 0;
+        for (size_t j = 0; j < n; j++) below += a[j] < a[i];
+        best = (below <= k && a[i] > best) ? a[i] : best;
+    }
+    return best;
+}
+
+// Runs fn(0) .. fn(n - 1), each on its own thread but the last.
+template <class Fn>
+void run_parallel(int n, Fn fn) {
+    std::vector<std::thread> pool;
+    for (int w = 0; w + 1 < n; w++) pool.emplace_back(fn, w);
+    fn(n - 1);
+    for (std::thread& t : pool) t.join();
+}
+
+// Folds the coordinates of recs[b, e) into a box with the reference's rule,
+// `p < m ? p : m` (and `>` for the maximum) over every vertex in list order,
+// starting from the first vertex.
+inline void fold_box(const TriRec* recs, uint32_t b, uint32_t e, float* mn, float* mx) {
+    for (int a = 0; a < 3; a++) mn[a] = mx[a] = recs[b].p[a];
+    for (uint32_t i = b; i < e; i++)
+        for (int k = 0; k < 3; k++)
+            for (int a = 0; a < 3; a++) {
+                const float p = recs[i].p[3 * k + a];
+                mn[a] = p < mn[a] ? p : mn[a];
+                mx[a] = p > mx[a] ? p : mx[a];
+            }
+}
+
+// "any vertex coordinate <= split" (kdtree.cpp:192-198,224-230)
+inline bool any_le(const TriRec& R, int dim, float split) {
+    return ((int)(R.p[dim] <= split) | (int)(R.p[3 + dim] <= split) | (int)(R.p[6 + dim] <= split)) != 0;
+}
 
 struct Builder {
     const float* V;
     const uint32_t* I;
     uint32_t nv, nt;
-    std::vector<uint32_t> order, tmp;
+    int nthreads = 1;
+    std::vector<TriRec> recs, rtmp;
     std::atomic<int> spare{0};
-    uint32_t max_depth = 0;
     std::atomic<uint32_t> depth_seen{0};
     static constexpr uint32_t kParallelMin = 1u << 14;
-
-    float c(uint32_t vi, int dim) const { return V[3 * (size_t)vi + dim]; }
+    static constexpr uint32_t kChunk = 1u << 15;  // triangles per worker in one node's scans
+    static constexpr int kMaxWorkers = 64;
+    static constexpr size_t kSmallSelect = 24;    // distinct vertices up to which the median is select_small
+    static constexpr size_t kRadixMin = 1024;     // distinct vertices from which the median is a radix select
 
     void note_depth(uint32_t d) {
         uint32_t cur = depth_seen.load(std::memory_order_relaxed);
@@ -68,30 +225,103 @@ struct Builder {
         }
     }
 
+    void init_recs() {
+        recs.resize(nt);
+        rtmp.resize(nt);
+        const int nw = nt >= 4 * kChunk ? std::min(nthreads, kMaxWorkers) : 1;
+        char nan[kMaxWorkers] = {};
+        run_parallel(nw, [&](int w) {
+            const uint32_t r0 = (uint32_t)((uint64_t)nt * w / nw), r1 = (uint32_t)((uint64_t)nt * (w + 1) / nw);
+            char f = 0;
+            for (uint32_t ti = r0; ti < r1; ti++) {
+                TriRec& R = recs[ti];
+                for (int k = 0; k < 3; k++) {
+                    R.v[k] = I[3 * (size_t)ti + k];
+                    for (int a = 0; a < 3; a++) {
+                        const float p = V[3 * (size_t)R.v[k] + a];
+                        R.p[3 * k + a] = p;
+                        f |= (char)(p != p);
+                    }
+                }
+                R.ti = ti;
+            }
+            nan[w] = f;
+        });
+        for (int w = 0; w < nw; w++)
+            if (nan[w]) throw std::runtime_error("a vertex coordinate is NaN (the split rule's median is undefined)");
+    }
+
+    // The upper median of the distinct vertices' coordinates along dim (the
+    // order of u does not change it).
+    float median(std::vector<float>& u) const {
+        const size_t U = u.size();
+        if (U <= kSmallSelect) return select_small(u.data(), U, U >> 1);
+        if (U < kRadixMin) return quickselect(u.data(), U, U >> 1);
+        return select_kth(u, U >> 1);
+    }
+
     void split(BNode* node, Scratch& s, uint32_t depth) {
         const uint32_t b = node->b, e = node->e, n = e - b;
         if (n == 0) return;
         note_depth(depth);
-        float mn[3], mx[3];
-        {
-            const uint32_t v0 = I[3 * (size_t)order[b]];
-            for (int k = 0; k < 3; k++) mn[k] = mx[k] = c(v0, k);
+        // workers for this node's scans: about the threads this level of the
+        // tree has, and one per kChunk triangles
+        int nw = (int)std::min<uint64_t>(std::max(1, nthreads >> std::min(depth, 20u)), n / kChunk);
+        nw = std::min(nw, kMaxWorkers);
+        if (nw < 2) nw = 1;
+        auto lo_of = [&](int w) { return b + (uint32_t)((uint64_t)n * w / nw); };
+
+        if (!s.stamp) {
+            s.stamp.reset(new std::atomic<uint32_t>[nv]);
+            for (uint32_t i = 0; i < nv; i++) s.stamp[i].store(0u, std::memory_order_relaxed);
         }
-        if (s.stamp.size() != nv) s.stamp.assign(nv, 0u);
         const uint32_t sid = ++s.stamp_id;
-        s.uniq.clear();
-        for (uint32_t i = b; i < e; i++) {
-            const uint32_t* t = I + 3 * (size_t)order[i];
-            for (int k = 0; k < 3; k++) {
-                const uint32_t vi = t[k];
-                for (int a = 0; a < 3; a++) {
-                    const float p = c(vi, a);
-                    mn[a] = p < mn[a] ? p : mn[a];
-                    mx[a] = p > mx[a] ? p : mx[a];
+        std::atomic<uint32_t>* st = s.stamp.get();
+        const TriRec* rc = recs.data();
+
+        // node box and distinct vertices (kdtree.cpp:151-165)
+        float mn[3], mx[3];
+        if (nw == 1) {
+            fold_box(rc, b, e, mn, mx);
+            for (int a = 0; a < 3; a++) s.u[a].clear();
+            for (uint32_t i = b; i < e; i++)
+                for (int k = 0; k < 3; k++) {
+                    const uint32_t vi = rc[i].v[k];
+                    if (st[vi].load(std::memory_order_relaxed) != sid) {
+                        st[vi].store(sid, std::memory_order_relaxed);
+                        for (int a = 0; a < 3; a++) s.u[a].push_back(rc[i].p[3 * k + a]);
+                    }
                 }
-                if (s.stamp[vi] != sid) {
-                    s.stamp[vi] = sid;
-                    s.uniq.push_back(vi);
+        } else {
+            // a worker's fold keeps the first of its smallest (largest)
+            // values, and folding those in worker order keeps the first
+            // overall, as the serial fold does (no NaN: init_recs refuses it)
+            float wbox[kMaxWorkers][6];
+            std::vector<float> wu[kMaxWorkers][3];
+            run_parallel(nw, [&](int w) {
+                fold_box(rc, lo_of(w), lo_of(w + 1), wbox[w], wbox[w] + 3);
+                for (uint32_t i = lo_of(w); i < lo_of(w + 1); i++)
+                    for (int k = 0; k < 3; k++) {
+                        const uint32_t vi = rc[i].v[k];
+                        if (st[vi].load(std::memory_order_relaxed) != sid &&
+                            st[vi].exchange(sid, std::memory_order_relaxed) != sid)
+                            for (int a = 0; a < 3; a++) wu[w][a].push_back(rc[i].p[3 * k + a]);
+                    }
+            });
+            for (int a = 0; a < 3; a++) {
+                mn[a] = wbox[0][a];
+                mx[a] = wbox[0][3 + a];
+                size_t tot = 0;
+                for (int w = 0; w < nw; w++) {
+                    mn[a] = wbox[w][a] < mn[a] ? wbox[w][a] : mn[a];
+                    mx[a] = wbox[w][3 + a] > mx[a] ? wbox[w][3 + a] : mx[a];
+                    tot += wu[w][a].size();
+                }
+                s.u[a].resize(tot);
+                size_t o = 0;
+                for (int w = 0; w < nw; w++) {
+                    std::copy(wu[w][a].begin(), wu[w][a].end(), s.u[a].begin() + o);
+                    o += wu[w][a].size();
                 }
             }
         }
@@ -101,74 +331,80 @@ struct Builder {
         }
         if (n <= 1) return;
 
-        float bestRatio = 10.f, bestSplit = 0.f;
-        int bestDim = 0;
-        uint32_t bestLeft = 0;
-        const size_t U = s.uniq.size();
-        // per dim: the upper median of the unique vertices' coordinates and the
-        // triangles with a vertex at or below it; a large node's three dims run
-        // side by side (each on its own coordinate copy), then are compared in
-        // dim order as the serial loop does
+        // per dim: the split and the triangles with a vertex at or below it
+        // (kdtree.cpp:177-208); a large node's three medians run side by side
         float dsplit[3];
-        uint32_t dleft[3];
-        auto dim_stats = [&](int dim, std::vector<float>& coords) {
-            coords.resize(U);
-            for (size_t i = 0; i < U; i++) coords[i] = c(s.uniq[i], dim);
-            std::nth_element(coords.begin(), coords.begin() + (U >> 1), coords.end());
-            const float split = coords[U >> 1];
-            uint32_t left = 0;
-            for (uint32_t i = b; i < e; i++) {
-                const uint32_t* t = I + 3 * (size_t)order[i];
-                left += (uint32_t)((int)(c(t[0], dim) <= split) | (int)(c(t[1], dim) <= split) | (int)(c(t[2], dim) <= split));
-            }
-            dsplit[dim] = split;
-            dleft[dim] = left;
-        };
         if (n >= kParallelMin * 4) {
-            std::vector<float> c1, c2;
-            std::thread t1([&]() { dim_stats(1, c1); });
-            std::thread t2([&]() { dim_stats(2, c2); });
-            dim_stats(0, s.coords);
+            std::thread t1([&]() { dsplit[1] = median(s.u[1]); });
+            std::thread t2([&]() { dsplit[2] = median(s.u[2]); });
+            dsplit[0] = median(s.u[0]);
             t1.join();
             t2.join();
         } else {
-            for (int dim = 0; dim < 3; dim++) dim_stats(dim, s.coords);
+            for (int dim = 0; dim < 3; dim++) dsplit[dim] = median(s.u[dim]);
         }
+        uint32_t wleft[kMaxWorkers][3];
+        run_parallel(nw, [&](int w) {
+            uint32_t l0 = 0, l1 = 0, l2 = 0;
+            for (uint32_t i = lo_of(w); i < lo_of(w + 1); i++) {
+                l0 += (uint32_t)any_le(rc[i], 0, dsplit[0]);
+                l1 += (uint32_t)any_le(rc[i], 1, dsplit[1]);
+                l2 += (uint32_t)any_le(rc[i], 2, dsplit[2]);
+            }
+            wleft[w][0] = l0;
+            wleft[w][1] = l1;
+            wleft[w][2] = l2;
+        });
+        float bestRatio = 10.f, bestSplit = 0.f;
+        int bestDim = 0;
         for (int dim = 0; dim < 3; dim++) {
-            const float split = dsplit[dim];
-            const uint32_t left = dleft[dim];
+            uint32_t left = 0;
+            for (int w = 0; w < nw; w++) left += wleft[w][dim];
             const float ratio = left / float(n);
             if (std::fabs((double)ratio - 0.5) < std::fabs((double)bestRatio - 0.5)) {
                 bestRatio = ratio;
                 bestDim = dim;
-                bestSplit = split;
-                bestLeft = left;
+                bestSplit = dsplit[dim];
             }
         }
-        if (bestRatio == 0 || bestRatio == 1) return;
+        if (bestRatio == 0 || bestRatio == 1) return;  // kdtree.cpp:211-215
 
-        // stable partition of order[b, e): left part first
-        uint32_t wl = b, wr_ = b;
-        for (uint32_t i = b; i < e; i++) {
-            const uint32_t ti = order[i];
-            const uint32_t* t = I + 3 * (size_t)ti;
-            const bool goes_left = ((int)(c(t[0], bestDim) <= bestSplit) | (int)(c(t[1], bestDim) <= bestSplit) |
-                                    (int)(c(t[2], bestDim) <= bestSplit)) != 0;
-            if (goes_left) order[wl++] = ti;
-            else tmp[wr_++] = ti;
+        // stable partition of recs[b, e): left part first (kdtree.cpp:220-236)
+        uint32_t nleft = 0;
+        for (int w = 0; w < nw; w++) nleft += wleft[w][bestDim];
+        const uint32_t mid = b + nleft;
+        if (nw == 1) {
+            uint32_t wl = b, wr_ = b;
+            for (uint32_t i = b; i < e; i++) {
+                if (any_le(recs[i], bestDim, bestSplit)) recs[wl++] = recs[i];
+                else rtmp[wr_++] = recs[i];
+            }
+            std::copy(rtmp.begin() + b, rtmp.begin() + wr_, recs.begin() + wl);
+        } else {
+            uint32_t lpos[kMaxWorkers], rpos[kMaxWorkers];
+            for (uint32_t w = 0, l = b, r = mid; w < (uint32_t)nw; w++) {
+                lpos[w] = l;
+                rpos[w] = r;
+                l += wleft[w][bestDim];
+                r += lo_of(w + 1) - lo_of(w) - wleft[w][bestDim];
+            }
+            run_parallel(nw, [&](int w) {
+                uint32_t l = lpos[w], r = rpos[w];
+                for (uint32_t i = lo_of(w); i < lo_of(w + 1); i++) {
+                    if (any_le(recs[i], bestDim, bestSplit)) rtmp[l++] = recs[i];
+                    else rtmp[r++] = recs[i];
+                }
+            });
+            run_parallel(nw, [&](int w) { std::copy(rtmp.begin() + lo_of(w), rtmp.begin() + lo_of(w + 1), recs.begin() + lo_of(w)); });
         }
-        std::memcpy(order.data() + wl, tmp.data() + b, (size_t)(wr_ - b) * sizeof(uint32_t));
-        (void)bestLeft;
 
-        node->left.reset(new BNode);
-        node->right.reset(new BNode);
-        node->left->b = b;
-        node->left->e = wl;
-        node->right->b = wl;
-        node->right->e = e;
-
-        BNode* L = node->left.get();
-        BNode* R = node->right.get();
+        node->kids.reset(new BNode[2]);
+        BNode* L = &node->kids[0];
+        BNode* R = &node->kids[1];
+        L->b = b;
+        L->e = mid;
+        R->b = mid;
+        R->e = e;
         if (n >= kParallelMin && spare.fetch_sub(1) > 0) {
             std::thread th([this, L, depth]() {
                 Scratch s2;
@@ -182,6 +418,7 @@ struct Builder {
             split(L, s, depth + 1);
             split(R, s, depth + 1);
         }
+        node->n_int = 1 + L->n_int + R->n_int;
     }
 };
 
@@ -206,40 +443,51 @@ void ref_normal(const float* v0, const float* v1, const float* v2, float* out) {
     for (int k = 0; k < 3; k++) out[k] = n[k];
 }
 
+// Writes the internal nodes under n in preorder from index k (a node, then
+// its left subtree, then its right one: the right child's index follows from
+// the left subtree's node count), the links and counts of the leaves and each
+// record's first record; large subtrees run on their own threads.  Every
+// node's children are freed once written.  Returns n's reference.
 struct Flattener {
     FlatTree& F;
-    uint32_t leaves = 0;
-    explicit Flattener(FlatTree& f) : F(f) {}
+    std::atomic<int> spare{0};
+    static constexpr uint32_t kParallelNodes = 1u << 12;
+    explicit Flattener(FlatTree& f, int nthreads) : F(f) { spare = nthreads - 1; }
 
-    uint32_t leaf_ref(const BNode* n) {
-        const uint32_t cnt = n->e - n->b;
-        leaves++;
-        uint32_t field = cnt <= WR_CNT_MASK ? cnt : 0u;
-        if (field == 0) F.leaf_cnt[n->b] = cnt;
-        return WR_LEAF_BIT | (field << WR_CNT_SHIFT) | n->b;
-    }
-
-    // Returns the reference to `n`; `up` is (parent << 1 | side) or WR_NONE.
-    uint32_t visit(const BNode* n, uint32_t up) {
-        if (!n->left) {
-            uint32_t r = leaf_ref(n);
+    uint32_t visit(BNode* n, uint32_t k, uint32_t up) {
+        if (!n->kids) {
+            const uint32_t cnt = n->e - n->b;
+            const uint32_t field = cnt <= WR_CNT_MASK ? cnt : 0u;
+            if (field == 0) F.leaf_cnt[n->b] = cnt;
             F.leaf_up[n->b] = up;
-            return r;
+            for (uint32_t r = n->b; r < n->e; r++) F.rec_first[r] = n->b;
+            return WR_LEAF_BIT | (field << WR_CNT_SHIFT) | n->b;
         }
-        const uint32_t k = (uint32_t)F.nodes.size();
-        F.nodes.push_back(wr_node{});
-        F.node_up.push_back(up);
-        const uint32_t lr = visit(n->left.get(), k << 1);
-        const uint32_t rr = visit(n->right.get(), (k << 1) | 1u);
+        F.node_up[k] = up;
+        BNode* nl = &n->kids[0];
+        BNode* nr = &n->kids[1];
+        const uint32_t kl = k + 1, kr = k + 1 + nl->n_int;
+        uint32_t lr = 0, rr = 0;
+        if (n->n_int >= kParallelNodes && spare.fetch_sub(1) > 0) {
+            std::thread th([&]() { lr = visit(nl, kl, k << 1); });
+            rr = visit(nr, kr, (k << 1) | 1u);
+            th.join();
+            spare.fetch_add(1);
+        } else {
+            if (n->n_int >= kParallelNodes) spare.fetch_add(1);
+            lr = visit(nl, kl, k << 1);
+            rr = visit(nr, kr, (k << 1) | 1u);
+        }
         wr_node& w = F.nodes[k];
         for (int a = 0; a < 3; a++) {
-            w.lmin[a] = n->left->box[a];
-            w.lmax[a] = n->left->box[3 + a];
-            w.rmin[a] = n->right->box[a];
-            w.rmax[a] = n->right->box[3 + a];
+            w.lmin[a] = nl->box[a];
+            w.lmax[a] = nl->box[3 + a];
+            w.rmin[a] = nr->box[a];
+            w.rmax[a] = nr->box[3 + a];
         }
         w.lref = lr;
         w.rref = rr;
+        n->kids.reset();
         return k;
     }
 };
@@ -642,27 +890,78 @@ struct WideBuilder {
     }
 };
 
-// record -> first record of its reference leaf, for every leaf under ref
-void fill_rec_first(FlatTree& F, uint32_t ref) {
-    std::vector<uint32_t> stack{ref};
-    while (!stack.empty()) {
-        const uint32_t r = stack.back();
-        stack.pop_back();
-        if (r & WR_LEAF_BIT) {
-            const uint32_t first = r & WR_FIRST_MASK;
-            uint32_t cnt = (r >> WR_CNT_SHIFT) & WR_CNT_MASK;
-            if (cnt == 0) cnt = F.leaf_cnt[first];
-            for (uint32_t k = 0; k < cnt; k++) F.rec_first[first + k] = first;
-        } else {
-            stack.push_back(F.nodes[r].lref);
-            stack.push_back(F.nodes[r].rref);
+// The chain records of F (the layout is wr_chain's): one per leaf that is not
+// the root, in record order (so left-to-right leaf order), each holding the
+// child boxes of its first kChainLevels ancestors on its path and the link
+// where it stops; the levels above come from the top record of that link
+// (its boxes up to the root), one per distinct link.  rec_chain: record ->
+// its leaf's chain index, WR_NONE under a root leaf.  Leaves and links are
+// split over threads; the result does not depend on their number.
+void build_chains(FlatTree& F, int nthreads) {
+    const size_t nr = F.rec_first.size(), nn = F.nodes.size();
+    std::vector<uint32_t> leaves;  // first record of each non-root leaf
+    for (size_t r = 0; r < nr; r++)
+        if (F.rec_first[r] == r && F.leaf_up[r] != WR_NONE) leaves.push_back((uint32_t)r);
+    const size_t nl = leaves.size();
+    F.chains.assign(nl, wr_chain{});
+    F.rec_chain.assign(nr, WR_NONE);
+    F.top_boxes.clear();
+    if (nl == 0) return;
+    // node depths (nodes are in preorder: a parent comes before its children)
+    std::vector<uint32_t> depth(nn, 0u);
+    for (size_t i = 1; i < nn; i++) depth[i] = depth[F.node_up[i] >> 1] + 1;
+    auto ranges = [&](size_t n, auto fn) {
+        const int nw = n >= 4096 ? std::max(1, nthreads) : 1;
+        run_parallel(nw, [&](int w) { fn(n * w / nw, n * (w + 1) / nw); });
+    };
+    auto child_box = [&](uint32_t link) {
+        const wr_node& N = F.nodes[link >> 1];
+        return (link & 1u) ? N.rmin : N.lmin;  // min then max, contiguous
+    };
+    ranges(nl, [&](size_t a, size_t b) {
+        for (size_t i = a; i < b; i++) {
+            wr_chain& c = F.chains[i];
+            uint32_t up = F.leaf_up[leaves[i]];
+            while (up != WR_NONE && c.count < kChainLevels) {
+                const float* bx = child_box(up);
+                std::copy(bx, bx + 6, c.box[c.count]);
+                c.count++;
+                up = F.node_up[up >> 1];
+            }
+            c.next = up;
         }
-    }
-}
-
-uint32_t internal_depth(const BNode* n) {
-    if (!n->left) return 0;
-    return 1 + std::max(internal_depth(n->left.get()), internal_depth(n->right.get()));
+    });
+    // top records: one per distinct stopping link, in link order
+    std::vector<uint32_t> links;
+    for (const wr_chain& c : F.chains)
+        if (c.next != WR_NONE) links.push_back(c.next);
+    std::sort(links.begin(), links.end());
+    links.erase(std::unique(links.begin(), links.end()), links.end());
+    std::vector<uint32_t> first(links.size() + 1, 0u);
+    for (size_t k = 0; k < links.size(); k++) first[k + 1] = first[k] + depth[links[k] >> 1] + 1;
+    F.top_boxes.resize(6 * (size_t)first.back());
+    ranges(links.size(), [&](size_t a, size_t b) {
+        for (size_t k = a; k < b; k++) {
+            float* out = F.top_boxes.data() + 6 * (size_t)first[k];
+            for (uint32_t u = links[k]; u != WR_NONE; u = F.node_up[u >> 1], out += 6) {
+                const float* bx = child_box(u);
+                std::copy(bx, bx + 6, out);
+            }
+        }
+    });
+    ranges(nl, [&](size_t a, size_t b) {
+        for (size_t i = a; i < b; i++) {
+            wr_chain& c = F.chains[i];
+            if (c.next != WR_NONE) {
+                const size_t k = (size_t)(std::lower_bound(links.begin(), links.end(), c.next) - links.begin());
+                c.top = first[k];
+                c.top_count = first[k + 1] - first[k];
+            }
+            // the leaf's records are contiguous from its first
+            const uint32_t f = leaves[i];
+            for (uint32_t r = f; r < nr && F.rec_first[r] == f; r++) F.rec_chain[r] = (uint32_t)i;
+        }
+    });
 }
 
 }  // namespace
@@ -692,11 +991,10 @@ FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32
     B.I = idx;
     B.nv = nv;
     B.nt = nt;
-    B.order.resize(nt);
-    B.tmp.resize(nt);
-    for (uint32_t i = 0; i < nt; i++) B.order[i] = i;
     if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+    B.nthreads = nthreads;
     B.spare = nthreads - 1;
+    B.init_recs();
     BNode root;
     root.b = 0;
     root.e = nt;
@@ -711,7 +1009,7 @@ FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32
     F.leaf_cnt.assign(nt, 0u);
     auto fill = [&](uint32_t r0, uint32_t r1) {
         for (uint32_t r = r0; r < r1; r++) {
-            const uint32_t ti = B.order[r];
+            const uint32_t ti = B.recs[r].ti;
             wr_tri& t = F.tris[r];
             for (int a = 0; a < 3; a++) {
                 t.v0[a] = verts[3 * (size_t)idx[3 * (size_t)ti + 0] + a];
@@ -730,18 +1028,18 @@ FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32
         fill(0, (uint32_t)((uint64_t)nt / nw));
         for (std::thread& t : pool) t.join();
     }
-    Flattener fl(F);
-    F.nodes.reserve(nt);
-    const uint32_t rref = fl.visit(&root, WR_NONE);
-    F.hdr.root_ref = rref;
+    F.nodes.resize(root.n_int);
+    F.node_up.resize(root.n_int);
+    F.rec_first.resize(nt);
+    Flattener fl(F, nthreads);
+    F.hdr.root_ref = fl.visit(&root, 0, WR_NONE);
     for (int a = 0; a < 3; a++) {
         F.hdr.root_min[a] = root.box[a];
         F.hdr.root_max[a] = root.box[3 + a];
     }
     F.hdr.n_nodes = (uint32_t)F.nodes.size();
-    F.hdr.depth = internal_depth(&root);
-    F.rec_first.resize(F.tris.size());
-    fill_rec_first(F, rref);
+    F.hdr.depth = B.depth_seen.load();  // depth of the deepest leaf
+    build_chains(F, nthreads);
     if (refit_from && refit_from->hdr.n_recs == nt && refit_from->rec_prim.size() == nt) {
         refit_wide(F, *refit_from, nthreads);
     } else {
@@ -750,7 +1048,7 @@ FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32
         F.hdr.wide_root = wb.build_root(nthreads);
     }
     F.hdr.n_wide = (uint32_t)F.wnodes.size();
-    F.n_leaves = fl.leaves;
+    F.n_leaves = root.n_int + 1;
     F.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return F;
 }

@@ -12,7 +12,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
-#include <unordered_map>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/walkray.h"
@@ -35,17 +35,22 @@ int hip_fail(wr_ctx* ctx, hipError_t e, const char* what) {
 }
 
 template <class T>
-static int upload(wr_ctx* ctx, const std::vector<T>& v, T** d, uint64_t& bytes) {
+static int upload(wr_ctx* ctx, const T* v, size_t n, T** d, uint64_t& bytes) {
     *d = nullptr;
-    const size_t b = std::max<size_t>(v.size(), 1) * sizeof(T);
+    const size_t b = std::max<size_t>(n, 1) * sizeof(T);
     hipError_t e = hipMalloc((void**)d, b);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc(scene)");
     bytes += b;
-    if (!v.empty()) {
-        e = hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (n) {
+        e = hipMemcpy(*d, v, n * sizeof(T), hipMemcpyHostToDevice);
         if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpy(scene)");
     }
     return WR_OK;
+}
+
+template <class T>
+static int upload(wr_ctx* ctx, const std::vector<T>& v, T** d, uint64_t& bytes) {
+    return upload(ctx, v.data(), v.size(), d, bytes);
 }
 
 void scene_free_device(wr_scene* s) {
@@ -286,6 +291,28 @@ int wr_scene_build(wr_scene* s, int strategy) {
 
 namespace wr {
 
+void* Staging::get(size_t slot, size_t bytes) {
+    if (bufs.size() <= slot) bufs.resize(slot + 1);
+    Buf& b = bufs[slot];
+    if (b.cap < bytes) {
+        if (b.p) (void)hipHostFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        const size_t cap = bytes + bytes / 4;
+        if (hipHostMalloc(&b.p, cap, hipHostMallocDefault) != hipSuccess) {
+            b.p = nullptr;
+            return nullptr;
+        }
+        b.cap = cap;
+    }
+    return b.p;
+}
+
+Staging::~Staging() {
+    for (Buf& b : bufs)
+        if (b.p) (void)hipHostFree(b.p);
+}
+
 int scene_build(wr_scene* s, const wr_scene* reuse) {
     wr_ctx* ctx = s->ctx;
     try {
@@ -359,92 +386,119 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
             if (s->trees[i].hdr.depth > 64) return fail(ctx, WR_EINVAL, "wr_scene_build: object tree deeper than 64 levels");
             max_obj_depth = std::max(max_obj_depth, s->trees[i].hdr.depth);
         }
-        // concatenate into device arrays
-        std::vector<wr_node> nodes;
-        std::vector<wr_wnode> wnodes;
-        std::vector<wr_tri> tris;
-        std::vector<uint32_t> leaf_up, leaf_cnt, rec_prim, rec_obj, rec_first, leaf_chain;
-        std::vector<wr_chain> chains;
-        std::vector<float> top_boxes;
-        std::vector<float> rec_nrm;
-        std::vector<wr_tree_hdr> hdrs;
-        std::vector<wr_object_gpu> objs;
-        std::vector<uint32_t> has_n;
+        // concatenate into device arrays: the bases of every tree first, then
+        // each tree's part filled in place, its records in chunks over threads
+        const size_t ntree = s->trees.size();
+        std::vector<wr_tree_hdr> hdrs(ntree);
+        std::vector<uint64_t> chain_base(ntree + 1, 0), top_base(ntree + 1, 0);
+        uint64_t n_nodes = 0, n_wide = 0, n_recs = 0;
+        for (size_t ti = 0; ti < ntree; ti++) {
+            const FlatTree& F = s->trees[ti];
+            wr_tree_hdr& h = hdrs[ti];
+            h = F.hdr;
+            h.node_base = (uint32_t)n_nodes;
+            h.wide_base = (uint32_t)n_wide;
+            h.rec_base = (uint32_t)n_recs;
+            n_nodes += F.nodes.size();
+            n_wide += F.wnodes.size();
+            n_recs += F.tris.size();
+            chain_base[ti + 1] = chain_base[ti] + F.chains.size();
+            top_base[ti + 1] = top_base[ti] + F.top_boxes.size() / 6;
+        }
+        if (std::max({n_nodes, n_wide, n_recs, chain_base[ntree], top_base[ntree]}) > 0xFFFFFFFFull)
+            return fail(ctx, WR_EINVAL, "wr_scene_build: scene too large");
         bool any_normals = false;
         for (const HostObject& o : s->objs) any_normals |= !o.normals.empty();
-        for (size_t ti = 0; ti < s->trees.size(); ti++) {
-            FlatTree& F = s->trees[ti];
-            wr_tree_hdr h = F.hdr;
-            h.node_base = (uint32_t)nodes.size();
-            h.wide_base = (uint32_t)wnodes.size();
-            wnodes.insert(wnodes.end(), F.wnodes.begin(), F.wnodes.end());
-            h.rec_base = (uint32_t)tris.size();
-            if ((uint64_t)h.rec_base + F.tris.size() > 0xFFFFFFFFull)
-                return fail(ctx, WR_EINVAL, "wr_scene_build: scene too large");
-            hdrs.push_back(h);
-            nodes.insert(nodes.end(), F.nodes.begin(), F.nodes.end());
-            tris.insert(tris.end(), F.tris.begin(), F.tris.end());
-            leaf_up.insert(leaf_up.end(), F.leaf_up.begin(), F.leaf_up.end());
-            leaf_cnt.insert(leaf_cnt.end(), F.leaf_cnt.begin(), F.leaf_cnt.end());
-            rec_first.insert(rec_first.end(), F.rec_first.begin(), F.rec_first.end());
-            std::unordered_map<uint32_t, std::pair<uint32_t, uint32_t>> top_of;  // link -> (first box, boxes)
-            for (size_t rr = 0; rr < F.rec_first.size(); rr++) {  // chain record of every non-root leaf
-                uint32_t up = F.leaf_up[rr];
-                if (F.rec_first[rr] != rr || up == WR_NONE) {
-                    leaf_chain.push_back(WR_NONE);
-                    continue;
-                }
-                leaf_chain.push_back((uint32_t)chains.size());
-                wr_chain c{};
-                while (up != WR_NONE && c.count < kChainLevels) {
-                    const wr_node& N = F.nodes[up >> 1];
-                    const float* b = (up & 1u) ? N.rmin : N.lmin;  // min then max, contiguous
-                    for (int k = 0; k < 3; k++) {
-                        c.box[c.count][k] = b[k];
-                        c.box[c.count][3 + k] = b[3 + k];
-                    }
-                    c.count++;
-                    up = F.node_up[up >> 1];
-                }
-                c.next = up;
-                if (up != WR_NONE) {  // the top record of this link, built once per tree and link
-                    auto it = top_of.find(up);
-                    if (it == top_of.end()) {
-                        const uint32_t first = (uint32_t)(top_boxes.size() / 6);
-                        uint32_t n = 0;
-                        for (uint32_t u = up; u != WR_NONE; u = F.node_up[u >> 1], n++) {
-                            const wr_node& N = F.nodes[u >> 1];
-                            const float* b = (u & 1u) ? N.rmin : N.lmin;
-                            top_boxes.insert(top_boxes.end(), b, b + 6);
-                        }
-                        it = top_of.emplace(up, std::make_pair(first, n)).first;
-                    }
-                    c.top = it->second.first;
-                    c.top_count = it->second.second;
-                }
-                chains.push_back(c);
+        // host arrays, written in full below: a rebuild's in `reuse`'s pinned
+        // staging, else fresh (uninitialised, first touched by the fill threads)
+        std::vector<std::unique_ptr<char[]>> own;
+        Staging* stg = reuse ? reuse->staging.get() : nullptr;
+        std::unique_lock<std::mutex> staging_lock;
+        if (stg) staging_lock = std::unique_lock<std::mutex>(stg->mu);
+        auto host_array = [&](auto* type, size_t slot, size_t n) {
+            using T = std::remove_pointer_t<decltype(type)>;
+            const size_t b = std::max<size_t>(n, 1) * sizeof(T);
+            void* p = stg ? stg->get(slot, b) : nullptr;
+            if (!p) {  // no staging, or no pinned memory for it
+                own.emplace_back(new char[b]);
+                p = own.back().get();
             }
-            rec_prim.insert(rec_prim.end(), F.rec_prim.begin(), F.rec_prim.end());
-            for (size_t r = 0; r < F.rec_prim.size(); r++) {
-                uint32_t oi, local;
-                if (ti == 0) {
-                    const uint32_t p = F.rec_prim[r];
-                    oi = (uint32_t)(std::upper_bound(s->prim_base.begin(), s->prim_base.end(), p) -
-                                    s->prim_base.begin() - 1);
-                    local = p - s->prim_base[oi];
-                } else {
-                    oi = (uint32_t)(ti - 1);
-                    local = F.rec_prim[r];
-                }
-                rec_obj.push_back(oi);
-                if (any_normals) {
-                    const HostObject& o = s->objs[oi];
-                    for (int k = 0; k < 3; k++)
-                        for (int a = 0; a < 3; a++)
-                            rec_nrm.push_back(o.normals.empty() ? 0.f : o.normals[9 * (size_t)local + 3 * k + a]);
-                }
+            return static_cast<T*>(p);
+        };
+        const size_t n_chains = chain_base[ntree], n_top = 6 * (top_base[ntree] + 5);  // five boxes' room past every record
+        const size_t n_nrm = any_normals ? 9 * n_recs : 0;
+        wr_node* nodes = host_array((wr_node*)nullptr, 0, n_nodes);
+        wr_wnode* wnodes = host_array((wr_wnode*)nullptr, 1, n_wide);
+        wr_tri* tris = host_array((wr_tri*)nullptr, 2, n_recs);
+        uint32_t* leaf_cnt = host_array((uint32_t*)nullptr, 3, n_recs);
+        uint32_t* rec_obj = host_array((uint32_t*)nullptr, 4, n_recs);
+        wr_chain* chains = host_array((wr_chain*)nullptr, 5, n_chains);
+        float* top_boxes = host_array((float*)nullptr, 6, n_top);
+        float* rec_nrm = host_array((float*)nullptr, 7, n_nrm);
+        std::fill(top_boxes + 6 * top_base[ntree], top_boxes + n_top, 0.f);
+        {
+            struct Job {
+                uint32_t ti, r0, r1;
+            };
+            constexpr uint32_t kJobRecs = 1u << 16;
+            std::vector<Job> jobs;
+            for (uint32_t ti = 0; ti < ntree; ti++) {
+                const uint32_t nr = (uint32_t)s->trees[ti].tris.size();
+                for (uint32_t r0 = 0; r0 == 0 || r0 < nr; r0 += kJobRecs) jobs.push_back({ti, r0, std::min(nr, r0 + kJobRecs)});
             }
+            auto run_job = [&](const Job& j) {
+                const FlatTree& F = s->trees[j.ti];
+                const wr_tree_hdr& h = hdrs[j.ti];
+                if (j.r0 == 0) {  // the tree's nodes, wide nodes, chains and top records
+                    std::copy(F.nodes.begin(), F.nodes.end(), nodes + h.node_base);
+                    std::copy(F.wnodes.begin(), F.wnodes.end(), wnodes + h.wide_base);
+                    std::copy(F.top_boxes.begin(), F.top_boxes.end(), top_boxes + 6 * top_base[j.ti]);
+                    for (size_t c = 0; c < F.chains.size(); c++) {
+                        wr_chain ch = F.chains[c];
+                        if (ch.next != WR_NONE) ch.top += (uint32_t)top_base[j.ti];
+                        chains[chain_base[j.ti] + c] = ch;
+                    }
+                }
+                for (uint32_t r = j.r0; r < j.r1; r++) {
+                    const size_t g = (size_t)h.rec_base + r;
+                    uint32_t oi, local;
+                    if (j.ti == 0) {
+                        const uint32_t p = F.rec_prim[r];
+                        oi = (uint32_t)(std::upper_bound(s->prim_base.begin(), s->prim_base.end(), p) -
+                                        s->prim_base.begin() - 1);
+                        local = p - s->prim_base[oi];
+                    } else {
+                        oi = j.ti - 1;
+                        local = F.rec_prim[r];
+                    }
+                    wr_tri T = F.tris[r];
+                    // the leaf's chain record: tie key and verification
+                    T.leaf = F.rec_chain[r] == WR_NONE ? WR_NONE : F.rec_chain[r] + (uint32_t)chain_base[j.ti];
+                    T.prim = F.rec_prim[r];
+                    T.obj = oi;
+                    T.pad = 0;
+                    tris[g] = T;
+                    leaf_cnt[g] = F.leaf_cnt[r];
+                    rec_obj[g] = oi;
+                    if (any_normals) {
+                        const HostObject& o = s->objs[oi];
+                        for (int k = 0; k < 9; k++)
+                            rec_nrm[9 * g + k] = o.normals.empty() ? 0.f : o.normals[9 * (size_t)local + k];
+                    }
+                }
+            };
+            std::atomic<size_t> next{0};
+            auto worker = [&]() {
+                for (size_t k; (k = next.fetch_add(1)) < jobs.size();) run_job(jobs[k]);
+            };
+            const int nw = (int)std::min<size_t>((size_t)nth, jobs.size());
+            std::vector<std::thread> pool;
+            for (int w = 1; w < nw; w++) pool.emplace_back(worker);
+            worker();
+            for (std::thread& t : pool) t.join();
         }
+        std::vector<wr_object_gpu> objs;
+        std::vector<uint32_t> has_n;
         for (size_t oi = 0; oi < s->objs.size(); oi++) {
             const HostObject& o = s->objs[oi];
             wr_object_gpu g{};
@@ -563,41 +617,26 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
         wr_node* d_nodes;
         wr_wnode* d_wnodes;
         wr_tri* d_tris;
-        uint32_t *d_lu, *d_lc, *d_hn;
+        uint32_t *d_lc, *d_hn;
         wr_chain* d_ch;
         float* d_rn;
         wr_tree_hdr* d_h;
         wr_object_gpu* d_o;
         int rc;
-        if ((rc = upload(ctx, nodes, &d_nodes, bytes))) return rc;
+        if ((rc = upload(ctx, nodes, n_nodes, &d_nodes, bytes))) return rc;
         s->dbufs.push_back(d_nodes);
-        if ((rc = upload(ctx, wnodes, &d_wnodes, bytes))) return rc;
+        if ((rc = upload(ctx, wnodes, n_wide, &d_wnodes, bytes))) return rc;
         s->dbufs.push_back(d_wnodes);
-        // each record carries its leaf key (the leaf's chain record), triangle id and object
-        for (size_t ti = 0; ti < hdrs.size(); ti++) {
-            const uint32_t base = hdrs[ti].rec_base;
-            const size_t end = ti + 1 < hdrs.size() ? hdrs[ti + 1].rec_base : tris.size();
-            for (size_t g = base; g < end; g++) {
-                wr_tri& T = tris[g];
-                T.leaf = leaf_chain[base + rec_first[g]];  // the leaf's chain record: tie key and verification
-                T.prim = rec_prim[g];
-                T.obj = rec_obj[g];
-                T.pad = 0;
-            }
-        }
-        if ((rc = upload(ctx, tris, &d_tris, bytes))) return rc;
+        if ((rc = upload(ctx, tris, n_recs, &d_tris, bytes))) return rc;
         s->dbufs.push_back(d_tris);
-        if ((rc = upload(ctx, leaf_up, &d_lu, bytes))) return rc;
-        s->dbufs.push_back(d_lu);
-        if ((rc = upload(ctx, leaf_cnt, &d_lc, bytes))) return rc;
+        if ((rc = upload(ctx, leaf_cnt, n_recs, &d_lc, bytes))) return rc;
         s->dbufs.push_back(d_lc);
-        if ((rc = upload(ctx, chains, &d_ch, bytes))) return rc;
+        if ((rc = upload(ctx, chains, n_chains, &d_ch, bytes))) return rc;
         s->dbufs.push_back(d_ch);
         float* d_tb;
-        top_boxes.resize(top_boxes.size() + 5 * 6, 0.f);  // five boxes' room past every record
-        if ((rc = upload(ctx, top_boxes, &d_tb, bytes))) return rc;
+        if ((rc = upload(ctx, top_boxes, n_top, &d_tb, bytes))) return rc;
         s->dbufs.push_back(d_tb);
-        if ((rc = upload(ctx, rec_nrm, &d_rn, bytes))) return rc;
+        if ((rc = upload(ctx, rec_nrm, n_nrm, &d_rn, bytes))) return rc;
         s->dbufs.push_back(d_rn);
         if ((rc = upload(ctx, hdrs, &d_h, bytes))) return rc;
         s->dbufs.push_back(d_h);
@@ -624,7 +663,6 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
         D.nodes = d_nodes;
         D.wnodes = d_wnodes;
         D.tris = d_tris;
-        D.leaf_up = d_lu;
         D.leaf_cnt = d_lc;
         D.chains = d_ch;
         D.top_boxes = d_tb;

@@ -194,7 +194,7 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
                 if (o.moving) place_object(o, io::interpolate(o.motion, motion_time(prm->seed, oi, pass)));
             }
             const auto b0 = std::chrono::steady_clock::now();
-            const int rc = scene_build(&tmp, s);  // static objects keep their trees
+            const int rc = scene_build(&tmp, s);  // static objects keep their trees; s stages the arrays
             build_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - b0).count();
             if (rc != WR_OK) {
                 film->passes += k;  // the passes before this one are in the film

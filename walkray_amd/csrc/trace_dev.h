@@ -38,7 +38,6 @@ struct Tree {
     const wr_node* nodes;      // already offset to this tree
     const wr_wnode* wnodes;    // already offset to this tree
     const wr_tri* tris;        // already offset to this tree
-    const uint32_t* leaf_up;   // offset like tris
     const uint32_t* leaf_cnt;  // offset like tris
     const wr_chain* chains;
     const float* top_boxes;    // the scene's pool (chain records hold global box indices)
@@ -53,7 +52,6 @@ __device__ __forceinline__ Tree make_tree(const wr_tree_hdr& h, const SceneDev& 
     t.nodes = S.nodes + h.node_base;
     t.wnodes = S.wnodes + h.wide_base;
     t.tris = S.tris + h.rec_base;
-    t.leaf_up = S.leaf_up + h.rec_base;
     t.leaf_cnt = S.leaf_cnt + h.rec_base;
     t.chains = S.chains;
     t.top_boxes = S.top_boxes;

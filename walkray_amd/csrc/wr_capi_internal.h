@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/walkray.h"
@@ -47,8 +49,23 @@ float motion_time(uint64_t seed, uint32_t obj, uint32_t pass);
 // Looks up, on `device` (current), the address of the fixed work list of
 // wr_hashgrid_query_noalloc; once per device, from wr_ctx_create.
 hipError_t noalloc_scratch_init(int device);
+// Pinned host buffers that a scene keeps for its per-pass rebuilds (moving
+// objects): every pass stages the concatenated scene arrays in the same
+// pages, so they are neither faulted in again nor copied through pageable
+// memory.  Slot k grows (with a quarter of headroom) when a pass needs more.
+struct Staging {
+    struct Buf {
+        void* p = nullptr;
+        size_t cap = 0;
+    };
+    std::vector<Buf> bufs;
+    std::mutex mu;  // held by the rebuild that stages in these buffers
+    void* get(size_t slot, size_t bytes);  // nullptr when hipHostMalloc fails
+    ~Staging();
+};
 // wr_scene_build; the trees of objects that do not move are copied from
-// `reuse` (nullable) when it is the same scene, built.
+// `reuse` (nullable) when it is the same scene, built, and the host arrays
+// are staged in reuse's pinned buffers.
 int scene_build(wr_scene* s, const wr_scene* reuse);
 
 // VCMTracer (bdpt.hip): the merging radius of a pass (vcmTracer.cpp:30), and
@@ -97,4 +114,5 @@ struct wr_scene {
     uint64_t device_bytes = 0;
     uint32_t max_obj_depth = 0;
     double build_seconds = 0;
+    mutable std::unique_ptr<wr::Staging> staging = std::make_unique<wr::Staging>();  // for the per-pass rebuilds
 };

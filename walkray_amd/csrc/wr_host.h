@@ -18,7 +18,10 @@ struct FlatTree {
     std::vector<uint32_t> leaf_cnt;    // first record of a long leaf -> triangle count
     std::vector<uint32_t> node_up;     // internal node -> (parent << 1 | side); WR_NONE for the root
     std::vector<wr_wnode> wnodes;      // fast-pass hierarchy: four children per node, single triangles at the bottom
-    std::vector<uint32_t> rec_first;   // record -> first record of its reference leaf (tie order key)
+    std::vector<uint32_t> rec_first;   // record -> first record of its reference leaf
+    std::vector<wr_chain> chains;      // chain record of every non-root leaf, in leaf order
+    std::vector<float> top_boxes;      // top records (6 floats a box), indexed by wr_chain::top
+    std::vector<uint32_t> rec_chain;   // record -> its leaf's chain (the tie key), WR_NONE under a root leaf
     wr_tree_hdr hdr{};
     uint32_t n_leaves = 0;
     double build_seconds = 0.0;

@@ -11,7 +11,6 @@ struct SceneDev {
     const wr_node* nodes;
     const wr_wnode* wnodes;          // fast-pass hierarchy of every tree (offsets in the headers)
     const wr_tri* tris;
-    const uint32_t* leaf_up;
     const uint32_t* leaf_cnt;
     const wr_chain* chains;
     const float* top_boxes;          // chain top records: 6 floats per box (wr_chain::top)
