@@ -931,18 +931,25 @@ void build_chains(FlatTree& F, int nthreads) {
             c.next = up;
         }
     });
-    // top records: one per distinct stopping link, in link order
-    std::vector<uint32_t> links;
+    // top records: one per distinct stopping link, in link order (a link is
+    // parent << 1 | side, so 2 * nn of them)
+    std::vector<uint32_t> first(2 * nn + 1, 0u);  // link -> first box of its top record
     for (const wr_chain& c : F.chains)
-        if (c.next != WR_NONE) links.push_back(c.next);
-    std::sort(links.begin(), links.end());
-    links.erase(std::unique(links.begin(), links.end()), links.end());
-    std::vector<uint32_t> first(links.size() + 1, 0u);
-    for (size_t k = 0; k < links.size(); k++) first[k + 1] = first[k] + depth[links[k] >> 1] + 1;
-    F.top_boxes.resize(6 * (size_t)first.back());
+        if (c.next != WR_NONE) first[c.next] = 1u;
+    std::vector<uint32_t> links;
+    uint32_t boxes = 0;
+    for (uint32_t l = 0; l < 2 * nn; l++) {
+        const bool used = first[l] != 0;
+        first[l] = boxes;
+        if (used) {
+            links.push_back(l);
+            boxes += depth[l >> 1] + 1;
+        }
+    }
+    F.top_boxes.resize(6 * (size_t)boxes);
     ranges(links.size(), [&](size_t a, size_t b) {
         for (size_t k = a; k < b; k++) {
-            float* out = F.top_boxes.data() + 6 * (size_t)first[k];
+            float* out = F.top_boxes.data() + 6 * (size_t)first[links[k]];
             for (uint32_t u = links[k]; u != WR_NONE; u = F.node_up[u >> 1], out += 6) {
                 const float* bx = child_box(u);
                 std::copy(bx, bx + 6, out);
@@ -953,9 +960,8 @@ void build_chains(FlatTree& F, int nthreads) {
         for (size_t i = a; i < b; i++) {
             wr_chain& c = F.chains[i];
             if (c.next != WR_NONE) {
-                const size_t k = (size_t)(std::lower_bound(links.begin(), links.end(), c.next) - links.begin());
-                c.top = first[k];
-                c.top_count = first[k + 1] - first[k];
+                c.top = first[c.next];
+                c.top_count = depth[c.next >> 1] + 1;
             }
             // the leaf's records are contiguous from its first
             const uint32_t f = leaves[i];
