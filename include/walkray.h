@@ -128,6 +128,20 @@ int wr_scene_prim_source(const wr_scene* scene, uint32_t prim, uint32_t* obj, ui
  * stack-overflow record.  Asynchronous on the context stream; device pointers. */
 int wr_hit_sources(wr_ctx* ctx, const wr_scene* scene, const wr_hit* d_hits, uint64_t n, uint32_t* d_obj,
                    uint32_t* d_tri);
+/* The scene as pass `pass` of a render with `seed` places it: what wr_render
+ * does before every pass of a scene with moving objects (Scene::
+ * updateSceneForMotionBlur, scene.cpp:67-80, called at pathTracer.cpp:18):
+ * each moving object at its time for that pass, the reference trees rebuilt
+ * exactly, the trees of objects that do not move kept, and the fast-pass
+ * hierarchies of the rebuilt trees refit on the device.  *out is a built,
+ * static scene of its own (queries, renders, exports; wr_scene_destroy).  No
+ * counterpart in the reference beyond that call, whose scene it exposes. */
+int wr_scene_motion_pass(const wr_scene* scene, uint64_t seed, uint32_t pass, wr_scene** out);
+/* The fast-pass hierarchy of the global tree (obj = WR_NONE) or of object
+ * obj's tree as on the device: n_wide 64-byte nodes (wr_tree_wide's layout)
+ * and the root reference.  Pass NULL nodes to get the sizes only.  No
+ * counterpart in the reference (its tree has no such hierarchy). */
+int wr_scene_export_wide(const wr_scene* scene, uint32_t obj, uint32_t* n_wide, uint32_t* wide_root, void* nodes);
 /* Canonical preorder export of the global tree (obj = WR_NONE) or of object
  * obj's tree, for comparison with another builder.  Pass NULL arrays to get
  * the sizes only. */

@@ -112,3 +112,70 @@ def test_motion_from_scene_file(ctx, tmp_path):
     fb = R.render_scene(ctx, None, cfg.camera, 64, 3, 9, W.WR_WAVEFRONT, scene=b)
     assert np.array_equal(fa["sum"].view(np.uint32), fb["sum"].view(np.uint32))
     assert fa["mean"].max() > 0
+
+
+def _placed(objs, motions, seed, pss):
+    """The objects as pass `pss` of a render with `seed` places them: the time of each
+    moving object from the counter RNG (object id in the pixel slot, vertex 0xFFFFFFFE),
+    Interpolate from the oracle, and Mat4::apply's float operations, in order."""
+    out = []
+    for oi, o in enumerate(objs):
+        v = np.asarray(o["verts"], np.float32)
+        if oi in motions:
+            t0, t1, a, b = motions[oi]
+            tm = float(O.rng4(seed, oi, pss, 0xFFFFFFFE)[0]) if pss is not None else t0
+            M = O.motion_interpolate(t0, t1, a, b, tm)
+            v = ((v[:, 0:1] * M[:3, 0] + v[:, 1:2] * M[:3, 1]) + v[:, 2:3] * M[:3, 2]) + M[:3, 3]
+            v = v.astype(np.float32)
+        out.append(dict(o, verts=v))
+    return out
+
+
+def _soup(objs):
+    V = np.concatenate([o["verts"] for o in objs]).astype(np.float32)
+    off = np.cumsum([0] + [len(o["verts"]) for o in objs])
+    T = np.concatenate([o["tris"].astype(np.int64) + off[i] for i, o in enumerate(objs)]).astype(np.uint32)
+    return V, T
+
+
+@pytest.mark.parametrize("pss", [0, 7])
+def test_motion_pass_scene(ctx, pss):
+    """wr_scene_motion_pass: the scene of one pass.  Its reference trees are the
+    oracle's for the placed objects (box bits included), its fast-pass hierarchies,
+    refit on the device, are the host refit's node for node, and its closest hits are
+    the oracle's on the placed scene, bit for bit."""
+    objs, cam, motions = _cornell_moving()
+    seed = 11
+    sc = _scene(ctx, objs, motions)
+    print(f"[motion] pass scene {pss}", flush=True)
+    ps = sc.motion_pass(seed, pss)
+    built, placed = _placed(objs, motions, seed, None), _placed(objs, motions, seed, pss)
+    Vb, T = _soup(built)
+    Vp, _ = _soup(placed)
+    assert not np.array_equal(Vb, Vp)
+    ref = O.Tree(Vp, T).export()
+    got = ps.export_tree()
+    for k in ("right", "leaf_first", "leaf_count", "tri_list"):
+        assert np.array_equal(got[k], ref[k]), k
+    assert np.array_equal(got["box"].view(np.uint32), np.asarray(ref["box"], np.float32).view(np.uint32))
+    # fast-pass hierarchies: the global tree and the moving object's own tree
+    host = W.build_tree_wide(Vb, T, refit_verts=Vp)
+    dev = ps.export_wide()
+    assert dev["root"] == host["root"]
+    assert dev["nodes"].tobytes() == host["nodes"].tobytes()
+    for oi in motions:
+        host = W.build_tree_wide(built[oi]["verts"], objs[oi]["tris"], refit_verts=placed[oi]["verts"])
+        dev = ps.export_wide(oi)
+        assert dev["root"] == host["root"] and dev["nodes"].tobytes() == host["nodes"].tobytes()
+    # a static object's hierarchy is the built scene's
+    assert ps.export_wide(0)["nodes"].tobytes() == sc.export_wide(0)["nodes"].tobytes()
+    lo, hi = S.scene_bounds(placed)
+    rays = np.concatenate([S.pinhole_rays(cam), S.incoherent_rays(20000, lo, hi, seed=3)])
+    g = ps.closest_hit(rays)
+    o = O.Scene(placed).intersect(rays)
+    assert np.array_equal(g["prim"], o["prim"])
+    assert np.array_equal(g["t"].view(np.uint32), o["t"].view(np.uint32))
+    hit = g["prim"] != O.NONE
+    for k in ("b1", "b2"):
+        assert np.array_equal(g[k][hit].view(np.uint32), o[k][hit].view(np.uint32))
+    ps.close()

@@ -144,6 +144,8 @@ def lib():
         "wr_scene_get_info": (i32, [vp, P(SceneInfo)]),
         "wr_scene_prim_source": (i32, [vp, u32, P(u32), P(u32)]),
         "wr_scene_export_tree": (i32, [vp, u32, P(u32), P(u32), vp, vp, vp, vp, vp]),
+        "wr_scene_motion_pass": (i32, [vp, u64, u32, P(vp)]),
+        "wr_scene_export_wide": (i32, [vp, u32, P(u32), P(u32), vp]),
         "wr_tree_build": (i32, [vp, u32, vp, u32, i32, P(vp)]),
         "wr_tree_export": (i32, [vp, P(u32), P(u32), P(u32), vp, vp, vp, vp, vp]),
         "wr_tree_free": (None, [vp]),
@@ -680,6 +682,27 @@ class Scene:
         _check(lib().wr_hit_sources(self.ctx.h, self.h, d_hits.ptr, n, do.ptr, dt.ptr), self.ctx.h,
                "wr_hit_sources")
         return do.download(np.uint32, n), dt.download(np.uint32, n)
+
+    def motion_pass(self, seed, pass_):
+        """wr_scene_motion_pass: the scene as pass `pass_` of a render with `seed` places
+        it (a static Scene of its own)."""
+        h = C.c_void_p()
+        _check(lib().wr_scene_motion_pass(self.h, seed, pass_, C.byref(h)), self.ctx.h, "wr_scene_motion_pass")
+        sc = Scene.__new__(Scene)
+        sc.ctx = self.ctx
+        sc.h = h
+        return sc
+
+    def export_wide(self, obj=NONE):
+        """wr_scene_export_wide: the device's fast-pass hierarchy of one tree."""
+        L = lib()
+        nw, root = C.c_uint32(), C.c_uint32()
+        _check(L.wr_scene_export_wide(self.h, obj, C.byref(nw), C.byref(root), None), self.ctx.h,
+               "wr_scene_export_wide")
+        nodes = np.zeros(max(nw.value, 1), WNODE_DTYPE)
+        _check(L.wr_scene_export_wide(self.h, obj, C.byref(nw), C.byref(root), _p(nodes)), self.ctx.h,
+               "wr_scene_export_wide")
+        return {"nodes": nodes[:nw.value], "root": root.value}
 
     def export_tree(self, obj=NONE):
         L = lib()

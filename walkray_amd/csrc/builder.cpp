@@ -972,8 +972,30 @@ void build_chains(FlatTree& F, int nthreads) {
 
 }  // namespace
 
+std::vector<uint32_t> wide_levels(const FlatTree& t, std::vector<uint32_t>& level_start) {
+    const size_t nw = t.wnodes.size();
+    std::vector<uint32_t> depth(nw, 0u);
+    uint32_t max_d = 0;
+    for (size_t i = 0; i < nw; i++)  // preorder: a node's wide children come after it
+        for (int c = 0; c < 4; c++) {
+            const uint32_t r = t.wnodes[i].ref[c];
+            if (r != WR_NONE && !(r & WR_LEAF_BIT)) {
+                depth[r] = depth[i] + 1;
+                max_d = std::max(max_d, depth[r]);
+            }
+        }
+    std::vector<uint32_t> count(max_d + 2, 0u);
+    for (size_t i = 0; i < nw; i++) count[max_d - depth[i] + 1]++;
+    for (size_t k = 1; k < count.size(); k++) count[k] += count[k - 1];
+    level_start.assign(count.begin(), count.end());
+    std::vector<uint32_t> order(nw);
+    for (size_t i = 0; i < nw; i++) order[count[max_d - depth[i]]++] = (uint32_t)i;
+    if (nw == 0) level_start.assign(1, 0u);
+    return order;
+}
+
 FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, int nthreads,
-                    const FlatTree* refit_from) {
+                    const FlatTree* refit_from, bool wide_on_device) {
     auto t0 = std::chrono::steady_clock::now();
     if (nt > WR_MAX_RECORDS) throw std::runtime_error("too many triangles for one tree (max 2^27)");
     for (size_t i = 0; i < 3 * (size_t)nt; i++)
@@ -1046,7 +1068,20 @@ FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32
     F.hdr.n_nodes = (uint32_t)F.nodes.size();
     F.hdr.depth = B.depth_seen.load();  // depth of the deepest leaf
     build_chains(F, nthreads);
-    if (refit_from && refit_from->hdr.n_recs == nt && refit_from->rec_prim.size() == nt) {
+    const bool refit = refit_from && refit_from->hdr.n_recs == nt && refit_from->rec_prim.size() == nt;
+    if (refit && wide_on_device) {
+        F.hdr.n_wide = refit_from->hdr.n_wide;
+        F.hdr.wide_root = refit_from->hdr.wide_root;
+        if (F.hdr.wide_root != WR_NONE && (F.hdr.wide_root & WR_LEAF_BIT)) {  // a one-record tree: rename its leaf
+            const uint32_t prim = refit_from->rec_prim[F.hdr.wide_root & WR_FIRST_MASK];
+            const uint32_t rec = (uint32_t)(std::find(F.rec_prim.begin(), F.rec_prim.end(), prim) - F.rec_prim.begin());
+            F.hdr.wide_root = (F.hdr.wide_root & ~WR_FIRST_MASK) | rec;
+        }
+        F.n_leaves = root.n_int + 1;
+        F.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return F;
+    }
+    if (refit) {
         refit_wide(F, *refit_from, nthreads);
     } else {
         WideBuilder wb(F, F.wnodes);

@@ -311,6 +311,68 @@ void* Staging::get(size_t slot, size_t bytes) {
 Staging::~Staging() {
     for (Buf& b : bufs)
         if (b.p) (void)hipHostFree(b.p);
+    for (uint32_t* p : d_order)
+        if (p) (void)hipFree(p);
+    if (d_rec_of) (void)hipFree(d_rec_of);
+    if (d_nbox) (void)hipFree(d_nbox);
+}
+
+// The device refit of tree ti's fast-pass hierarchy for a pass (see
+// WideRefit): its level order from the built scene's host tree on first use,
+// scratch grown as needed.  d_wnodes: the pass's copy of the built scene's
+// nodes; d_tris: the pass's records.
+static hipError_t refit_on_device(Staging& C, const wr_scene* built, size_t ti, wr_wnode* d_wnodes,
+                                  const wr_tree_hdr& h, const wr_tri* d_tris, hipStream_t st) {
+    const FlatTree& prev = built->trees[ti];
+    if (C.d_order.size() < built->trees.size()) {
+        C.d_order.resize(built->trees.size(), nullptr);
+        C.level_start.resize(built->trees.size());
+    }
+    hipError_t e = hipSuccess;
+    if (!C.d_order[ti]) {
+        const std::vector<uint32_t> order = wide_levels(prev, C.level_start[ti]);
+        e = hipMalloc((void**)&C.d_order[ti], std::max<size_t>(order.size(), 1) * sizeof(uint32_t));
+        if (e != hipSuccess) return e;
+        if (!order.empty())
+            e = hipMemcpy(C.d_order[ti], order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return e;
+    }
+    const size_t nr = prev.tris.size(), nw = prev.wnodes.size();
+    if (C.rec_cap < nr) {
+        if (C.d_rec_of) (void)hipFree(C.d_rec_of);
+        C.d_rec_of = nullptr;
+        C.rec_cap = 0;
+        if ((e = hipMalloc((void**)&C.d_rec_of, nr * sizeof(uint32_t))) != hipSuccess) return e;
+        C.rec_cap = nr;
+    }
+    if (C.box_cap < 6 * nw) {
+        if (C.d_nbox) (void)hipFree(C.d_nbox);
+        C.d_nbox = nullptr;
+        C.box_cap = 0;
+        if ((e = hipMalloc((void**)&C.d_nbox, 6 * nw * sizeof(float))) != hipSuccess) return e;
+        C.box_cap = 6 * nw;
+    }
+    WideRefit R;
+    R.wnodes = d_wnodes + h.wide_base;
+    R.old_tris = built->dev.tris + built->hdrs[ti].rec_base;
+    R.new_tris = d_tris + h.rec_base;
+    R.n_recs = (uint32_t)nr;
+    R.order = C.d_order[ti];
+    R.level_start = C.level_start[ti];
+    R.rec_of = C.d_rec_of;
+    R.nbox = C.d_nbox;
+    return launch_wide_refit(R, st);
+}
+
+int scene_pass(const wr_scene* s, uint64_t seed, uint32_t pass, wr_scene* out) {
+    out->ctx = s->ctx;
+    out->objs = s->objs;
+    out->ch_tune = s->ch_tune;
+    for (uint32_t oi = 0; oi < out->objs.size(); oi++) {
+        HostObject& o = out->objs[oi];
+        if (o.moving) place_object(o, io::interpolate(o.motion, motion_time(seed, oi, pass)));
+    }
+    return scene_build(out, s);  // static objects keep their trees; s stages the arrays
 }
 
 int scene_build(wr_scene* s, const wr_scene* reuse) {
@@ -335,8 +397,11 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
         const bool can_reuse = reuse && reuse->built && reuse->trees.size() == 1 + s->objs.size();
         s->trees.clear();
         // with `reuse`, the global tree's fast-pass hierarchy is refit rather than built
+        // a pass of a render with moving objects: the rebuilt trees' fast-pass
+        // hierarchies are refit on the device from the built scene's
+        const bool gpu_refit = can_reuse;
         s->trees.push_back(build_tree(V.data(), (uint32_t)(V.size() / 3), I.data(), nt, nth,
-                                      can_reuse ? &reuse->trees[0] : nullptr));
+                                      can_reuse ? &reuse->trees[0] : nullptr, gpu_refit));
         // per-object trees (scene.cpp:46-65).  A static object's tree from `reuse`
         // (the same scene before its moving objects were placed) is the tree this
         // deterministic builder would make again.  The others: large objects one
@@ -356,7 +421,7 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
             const HostObject& o = s->objs[oi];
             s->trees[1 + oi] = build_tree(o.verts.data(), (uint32_t)(o.verts.size() / 3), o.idx.data(),
                                           (uint32_t)(o.idx.size() / 3), threads,
-                                          can_reuse ? &reuse->trees[1 + oi] : nullptr);
+                                          can_reuse ? &reuse->trees[1 + oi] : nullptr, gpu_refit);
         };
         for (size_t oi : big) build_obj(oi, nth);
         if (!small.empty()) {
@@ -400,7 +465,7 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
             h.wide_base = (uint32_t)n_wide;
             h.rec_base = (uint32_t)n_recs;
             n_nodes += F.nodes.size();
-            n_wide += F.wnodes.size();
+            n_wide += F.hdr.n_wide;
             n_recs += F.tris.size();
             chain_base[ti + 1] = chain_base[ti] + F.chains.size();
             top_base[ti + 1] = top_base[ti] + F.top_boxes.size() / 6;
@@ -428,7 +493,7 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
         const size_t n_chains = chain_base[ntree], n_top = 6 * (top_base[ntree] + 5);  // five boxes' room past every record
         const size_t n_nrm = any_normals ? 9 * n_recs : 0;
         wr_node* nodes = host_array((wr_node*)nullptr, 0, n_nodes);
-        wr_wnode* wnodes = host_array((wr_wnode*)nullptr, 1, n_wide);
+        wr_wnode* wnodes = gpu_refit ? nullptr : host_array((wr_wnode*)nullptr, 1, n_wide);
         wr_tri* tris = host_array((wr_tri*)nullptr, 2, n_recs);
         uint32_t* leaf_cnt = host_array((uint32_t*)nullptr, 3, n_recs);
         uint32_t* rec_obj = host_array((uint32_t*)nullptr, 4, n_recs);
@@ -451,7 +516,7 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
                 const wr_tree_hdr& h = hdrs[j.ti];
                 if (j.r0 == 0) {  // the tree's nodes, wide nodes, chains and top records
                     std::copy(F.nodes.begin(), F.nodes.end(), nodes + h.node_base);
-                    std::copy(F.wnodes.begin(), F.wnodes.end(), wnodes + h.wide_base);
+                    if (wnodes) std::copy(F.wnodes.begin(), F.wnodes.end(), wnodes + h.wide_base);
                     std::copy(F.top_boxes.begin(), F.top_boxes.end(), top_boxes + 6 * top_base[j.ti]);
                     for (size_t c = 0; c < F.chains.size(); c++) {
                         wr_chain ch = F.chains[c];
@@ -625,10 +690,31 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
         int rc;
         if ((rc = upload(ctx, nodes, n_nodes, &d_nodes, bytes))) return rc;
         s->dbufs.push_back(d_nodes);
-        if ((rc = upload(ctx, wnodes, n_wide, &d_wnodes, bytes))) return rc;
-        s->dbufs.push_back(d_wnodes);
         if ((rc = upload(ctx, tris, n_recs, &d_tris, bytes))) return rc;
         s->dbufs.push_back(d_tris);
+        if (!gpu_refit) {
+            if ((rc = upload(ctx, wnodes, n_wide, &d_wnodes, bytes))) return rc;
+            s->dbufs.push_back(d_wnodes);
+        } else {
+            // the built scene's nodes (the same count per tree), then the
+            // rebuilt trees refit in place; synchronised so that the refit's
+            // scratch is free when the staging lock is released
+            uint64_t built_wide = 0;
+            for (const wr_tree_hdr& bh : reuse->hdrs) built_wide += bh.n_wide;
+            if (built_wide != n_wide) return fail(ctx, WR_EINVAL, "wr_scene_build: fast-pass hierarchy size changed");
+            hipError_t e = hipMalloc((void**)&d_wnodes, std::max<uint64_t>(n_wide, 1) * sizeof(wr_wnode));
+            if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc(scene)");
+            s->dbufs.push_back(d_wnodes);
+            bytes += std::max<uint64_t>(n_wide, 1) * sizeof(wr_wnode);
+            if (n_wide)
+                e = hipMemcpyAsync(d_wnodes, reuse->dev.wnodes, n_wide * sizeof(wr_wnode), hipMemcpyDeviceToDevice,
+                                   ctx->stream);
+            for (size_t ti = 0; ti < ntree && e == hipSuccess; ti++)
+                if (ti == 0 || s->objs[ti - 1].moving)
+                    e = refit_on_device(*stg, reuse, ti, d_wnodes, hdrs[ti], d_tris, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) return hip_fail(ctx, e, "wr_scene_build: device refit");
+        }
         if ((rc = upload(ctx, leaf_cnt, n_recs, &d_lc, bytes))) return rc;
         s->dbufs.push_back(d_lc);
         if ((rc = upload(ctx, chains, n_chains, &d_ch, bytes))) return rc;
@@ -726,6 +812,40 @@ int wr_scene_prim_source(const wr_scene* s, uint32_t prim, uint32_t* obj, uint32
         (uint32_t)(std::upper_bound(s->prim_base.begin(), s->prim_base.end(), prim) - s->prim_base.begin() - 1);
     *obj = oi;
     *tri = prim - s->prim_base[oi];
+    return WR_OK;
+}
+
+int wr_scene_motion_pass(const wr_scene* s, uint64_t seed, uint32_t pass, wr_scene** out) {
+    if (!s || !out) return fail(nullptr, WR_EINVAL, "wr_scene_motion_pass: bad argument");
+    *out = nullptr;
+    if (!s->built) return fail(s->ctx, WR_ESTATE, "wr_scene_motion_pass: scene not built");
+    (void)hipSetDevice(s->ctx->device);
+    std::unique_ptr<wr_scene> t(new wr_scene);
+    const int rc = scene_pass(s, seed, pass, t.get());
+    if (rc != WR_OK) {
+        scene_free_device(t.get());
+        return rc;
+    }
+    for (HostObject& o : t->objs) o.moving = false;  // a snapshot: placed once
+    *out = t.release();
+    return WR_OK;
+}
+
+int wr_scene_export_wide(const wr_scene* s, uint32_t obj, uint32_t* n_wide, uint32_t* wide_root, void* nodes) {
+    if (!s || !n_wide || !wide_root) return fail(nullptr, WR_EINVAL, "wr_scene_export_wide: bad argument");
+    if (!s->built) return fail(s->ctx, WR_ESTATE, "wr_scene_export_wide: scene not built");
+    const size_t ti = obj == WR_NONE ? 0 : (size_t)obj + 1;
+    if (ti >= s->hdrs.size()) return fail(s->ctx, WR_EINVAL, "wr_scene_export_wide: no such object");
+    const wr_tree_hdr& h = s->hdrs[ti];
+    *n_wide = h.n_wide;
+    *wide_root = h.wide_root;
+    if (nodes && h.n_wide) {
+        (void)hipSetDevice(s->ctx->device);
+        hipError_t e = hipStreamSynchronize(s->ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpy(nodes, s->dev.wnodes + h.wide_base, (size_t)h.n_wide * sizeof(wr_wnode), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(s->ctx, e, "wr_scene_export_wide");
+    }
     return WR_OK;
 }
 

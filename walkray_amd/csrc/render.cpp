@@ -186,17 +186,11 @@ int wr_render(wr_ctx* ctx, const wr_scene* s, const wr_camera* cam, const wr_ren
         for (uint32_t k = 0; k < prm->spp && e == hipSuccess; k++) {
             const uint32_t pass = film->passes + k;
             wr_scene tmp;
-            tmp.ctx = ctx;
-            tmp.objs = s->objs;
-            tmp.ch_tune = s->ch_tune;
-            for (uint32_t oi = 0; oi < tmp.objs.size(); oi++) {
-                HostObject& o = tmp.objs[oi];
-                if (o.moving) place_object(o, io::interpolate(o.motion, motion_time(prm->seed, oi, pass)));
-            }
             const auto b0 = std::chrono::steady_clock::now();
-            const int rc = scene_build(&tmp, s);  // static objects keep their trees; s stages the arrays
+            const int rc = scene_pass(s, prm->seed, pass, &tmp);
             build_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - b0).count();
             if (rc != WR_OK) {
+                scene_free_device(&tmp);
                 film->passes += k;  // the passes before this one are in the film
                 return rc;
             }

@@ -49,17 +49,26 @@ float motion_time(uint64_t seed, uint32_t obj, uint32_t pass);
 // Looks up, on `device` (current), the address of the fixed work list of
 // wr_hashgrid_query_noalloc; once per device, from wr_ctx_create.
 hipError_t noalloc_scratch_init(int device);
-// Pinned host buffers that a scene keeps for its per-pass rebuilds (moving
-// objects): every pass stages the concatenated scene arrays in the same
-// pages, so they are neither faulted in again nor copied through pageable
-// memory.  Slot k grows (with a quarter of headroom) when a pass needs more.
+// What a scene keeps for its per-pass rebuilds (moving objects).  Pinned host
+// buffers: every pass stages the concatenated scene arrays in the same pages,
+// so they are neither faulted in again nor copied through pageable memory;
+// slot k grows (with a quarter of headroom) when a pass needs more.  The
+// device refit of the fast-pass hierarchies: per tree of the built scene its
+// wide nodes deepest level first (made on the first pass that rebuilds the
+// tree), and the refit's scratch.
 struct Staging {
     struct Buf {
         void* p = nullptr;
         size_t cap = 0;
     };
     std::vector<Buf> bufs;
-    std::mutex mu;  // held by the rebuild that stages in these buffers
+    std::vector<uint32_t*> d_order;                // per tree, nullptr until needed
+    std::vector<std::vector<uint32_t>> level_start;
+    uint32_t* d_rec_of = nullptr;
+    size_t rec_cap = 0;
+    float* d_nbox = nullptr;
+    size_t box_cap = 0;
+    std::mutex mu;  // held by the rebuild that uses all this
     void* get(size_t slot, size_t bytes);  // nullptr when hipHostMalloc fails
     ~Staging();
 };
@@ -67,6 +76,10 @@ struct Staging {
 // `reuse` (nullable) when it is the same scene, built, and the host arrays
 // are staged in reuse's pinned buffers.
 int scene_build(wr_scene* s, const wr_scene* reuse);
+// Scene::updateSceneForMotionBlur (scene.cpp:67-80) for pass `pass` of a
+// render with `seed`: out (empty) receives s's objects, the moving ones placed
+// at their time for the pass, built with scene_build(out, s).
+int scene_pass(const wr_scene* s, uint64_t seed, uint32_t pass, wr_scene* out);
 
 // VCMTracer (bdpt.hip): the merging radius of a pass (vcmTracer.cpp:30), and
 // passes [pass0, pass0 + npass) side by side [sub-paths, light points, one

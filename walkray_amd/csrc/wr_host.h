@@ -30,9 +30,16 @@ struct FlatTree {
 // KDTree::build(BEST) (kdtree.cpp:242-258, splitBest 141-240) over the given
 // triangle soup, then flattened.  Deterministic; threads only change speed.
 // refit_from (nullable): a tree over the same triangles (at other positions)
-// whose fast-pass hierarchy is refit instead of built anew (moving objects).
+// whose fast-pass hierarchy is refit instead of built anew (moving objects);
+// with wide_on_device the refit is left to the caller's device pass: wnodes
+// stays empty, hdr.n_wide is refit_from's and hdr.wide_root its root renamed.
 FlatTree build_tree(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, int nthreads,
-                    const FlatTree* refit_from = nullptr);
+                    const FlatTree* refit_from = nullptr, bool wide_on_device = false);
+
+// The fast-pass hierarchy's node indices grouped by depth, deepest level
+// first (children before parents); level_start[k] .. level_start[k + 1] is
+// level k of that order.
+std::vector<uint32_t> wide_levels(const FlatTree& t, std::vector<uint32_t>& level_start);
 
 // Canonical preorder export (same format as the oracle's): used by tests to
 // compare this builder with the oracle's restatement node by node.

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "wr_layout.h"
 
 // Device pointers of one uploaded scene.  Record arrays cover the global tree
@@ -68,4 +70,23 @@ hipError_t launch_hit_sources(const SceneDev& S, uint32_t n_tris, const void* hi
                               uint32_t* tri, hipStream_t st);
 hipError_t launch_inside_object(const SceneDev& S, uint32_t max_obj_depth, const uint32_t* rays, uint64_t n,
                                 uint32_t* out, hipStream_t st);
+
+// The fast-pass hierarchy of one tree refit on the device (refit.hip) to the
+// records of a pass: wnodes holds the built scene's nodes of this tree (its
+// topology and leaf references) and is overwritten; old_tris / new_tris are
+// the tree's records in the built scene and in this pass (the same
+// triangles, n_recs of them); order / level_start its node indices deepest
+// level first (wide_levels); rec_of (n_recs) and nbox (6 floats per node) are
+// scratch.
+struct WideRefit {
+    wr_wnode* wnodes;
+    const wr_tri* old_tris;
+    const wr_tri* new_tris;
+    uint32_t n_recs;
+    const uint32_t* order;
+    std::vector<uint32_t> level_start;
+    uint32_t* rec_of;
+    float* nbox;
+};
+hipError_t launch_wide_refit(const WideRefit& R, hipStream_t st);
 }  // namespace wr
