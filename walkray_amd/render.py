@@ -145,8 +145,9 @@ def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
         _at_fixed_rmse(r, 16, cam)
     r["next_event"], _ = _compare(ctx, sc, cam, 16, 8, 1234, log, "C1 cornell +NEE", nee=True)
     r["stage_roofline"] = stage_roofline(W, ctx, sc, cam, 16, 8, 1234, log, "C1")
-    # steady state: enough passes that the pool stays full and the drain tail is a small part
-    ss = {}
+    # steady state: enough passes that the pool stays full and the drain tail is a small part;
+    # the pool is tuned again on a workload that can fill it (64 spp: 4M paths)
+    ss = {"wavefront_tuning": W.autotune_render(sc, cam, 8, spp=64)}
     for name, mode in (("megakernel", W.WR_MEGAKERNEL), ("wavefront", W.WR_WAVEFRONT)):
         film = W.Film(ctx, cam["width"], cam["height"])
         st = W.render(sc, cam, film, 1024, 8, 1234, mode)
@@ -155,7 +156,8 @@ def bench_path_tracing(W_, S_, ctx, log, c3_spp=64):
     ss["speedup_wavefront_vs_megakernel"] = round(ss["wavefront"]["samples_per_s"] / ss["megakernel"]["samples_per_s"], 3)
     r["steady_state_1024spp"] = ss
     log(f"[bench] C1 1024 spp: megakernel {ss['megakernel']['samples_per_s'] / 1e6:.1f}, wavefront "
-        f"{ss['wavefront']['samples_per_s'] / 1e6:.1f} Msamples/s ({ss['speedup_wavefront_vs_megakernel']}x)")
+        f"{ss['wavefront']['samples_per_s'] / 1e6:.1f} Msamples/s ({ss['speedup_wavefront_vs_megakernel']}x; "
+        f"tuning {ss['wavefront_tuning']})")
     out["c1"] = r
     objs, cam = S.random_field()
     sc = W.Scene(ctx, objs)
