@@ -179,3 +179,28 @@ def test_motion_pass_scene(ctx, pss):
     for k in ("b1", "b2"):
         assert np.array_equal(g[k][hit].view(np.uint32), o[k][hit].view(np.uint32))
     ps.close()
+
+
+def test_motion_pass_of_static_scene_and_errors(ctx):
+    """A scene without moving objects: its pass scene is the scene (trees, hierarchies,
+    hits).  An unbuilt scene has no pass; a NaN vertex coordinate is refused at build."""
+    objs, cam = S.cornell(k=1, width=16, height=16)
+    sc = W.Scene(ctx, objs)
+    ps = sc.motion_pass(3, 5)
+    for obj in (O.NONE, 4):
+        a, b = sc.export_tree(obj), ps.export_tree(obj)
+        assert all(np.array_equal(a[k], b[k]) for k in ("right", "leaf_first", "leaf_count", "tri_list"))
+        assert np.array_equal(a["box"].view(np.uint32), b["box"].view(np.uint32))
+        assert sc.export_wide(obj)["nodes"].tobytes() == ps.export_wide(obj)["nodes"].tobytes()
+    rays = S.pinhole_rays(cam)
+    g, h = sc.closest_hit(rays), ps.closest_hit(rays)
+    assert np.array_equal(g["prim"], h["prim"]) and np.array_equal(g["t"].view(np.uint32), h["t"].view(np.uint32))
+    ps.close()
+    un = W.Scene(ctx, objs, build=False)
+    with pytest.raises(W.WalkRayError):
+        un.motion_pass(3, 5)
+    bad = [dict(o) for o in objs]
+    bad[2] = dict(bad[2], verts=bad[2]["verts"].copy())
+    bad[2]["verts"][1, 0] = np.nan
+    with pytest.raises(W.WalkRayError, match="NaN"):
+        W.Scene(ctx, bad)
