@@ -291,7 +291,7 @@ int wr_scene_build(wr_scene* s, int strategy) {
 
 namespace wr {
 
-void* Staging::get(size_t slot, size_t bytes) {
+void* RebuildCache::get(size_t slot, size_t bytes) {
     if (bufs.size() <= slot) bufs.resize(slot + 1);
     Buf& b = bufs[slot];
     if (b.cap < bytes) {
@@ -308,7 +308,7 @@ void* Staging::get(size_t slot, size_t bytes) {
     return b.p;
 }
 
-Staging::~Staging() {
+RebuildCache::~RebuildCache() {
     for (Buf& b : bufs)
         if (b.p) (void)hipHostFree(b.p);
     for (uint32_t* p : d_order)
@@ -321,7 +321,7 @@ Staging::~Staging() {
 // WideRefit): its level order from the built scene's host tree on first use,
 // scratch grown as needed.  d_wnodes: the pass's copy of the built scene's
 // nodes; d_tris: the pass's records.
-static hipError_t refit_on_device(Staging& C, const wr_scene* built, size_t ti, wr_wnode* d_wnodes,
+static hipError_t refit_on_device(RebuildCache& C, const wr_scene* built, size_t ti, wr_wnode* d_wnodes,
                                   const wr_tree_hdr& h, const wr_tri* d_tris, hipStream_t st) {
     const FlatTree& prev = built->trees[ti];
     if (C.d_order.size() < built->trees.size()) {
@@ -477,9 +477,9 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
         // host arrays, written in full below: a rebuild's in `reuse`'s pinned
         // staging, else fresh (uninitialised, first touched by the fill threads)
         std::vector<std::unique_ptr<char[]>> own;
-        Staging* stg = reuse ? reuse->staging.get() : nullptr;
-        std::unique_lock<std::mutex> staging_lock;
-        if (stg) staging_lock = std::unique_lock<std::mutex>(stg->mu);
+        RebuildCache* stg = reuse ? reuse->rebuild.get() : nullptr;
+        std::unique_lock<std::mutex> rebuild_lock;
+        if (stg) rebuild_lock = std::unique_lock<std::mutex>(stg->mu);
         auto host_array = [&](auto* type, size_t slot, size_t n) {
             using T = std::remove_pointer_t<decltype(type)>;
             const size_t b = std::max<size_t>(n, 1) * sizeof(T);
@@ -698,7 +698,7 @@ int scene_build(wr_scene* s, const wr_scene* reuse) {
         } else {
             // the built scene's nodes (the same count per tree), then the
             // rebuilt trees refit in place; synchronised so that the refit's
-            // scratch is free when the staging lock is released
+            // scratch is free when the rebuild lock is released
             uint64_t built_wide = 0;
             for (const wr_tree_hdr& bh : reuse->hdrs) built_wide += bh.n_wide;
             if (built_wide != n_wide) return fail(ctx, WR_EINVAL, "wr_scene_build: fast-pass hierarchy size changed");

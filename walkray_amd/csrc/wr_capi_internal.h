@@ -56,7 +56,7 @@ hipError_t noalloc_scratch_init(int device);
 // device refit of the fast-pass hierarchies: per tree of the built scene its
 // wide nodes deepest level first (made on the first pass that rebuilds the
 // tree), and the refit's scratch.
-struct Staging {
+struct RebuildCache {
     struct Buf {
         void* p = nullptr;
         size_t cap = 0;
@@ -70,7 +70,7 @@ struct Staging {
     size_t box_cap = 0;
     std::mutex mu;  // held by the rebuild that uses all this
     void* get(size_t slot, size_t bytes);  // nullptr when hipHostMalloc fails
-    ~Staging();
+    ~RebuildCache();
 };
 // wr_scene_build; the trees of objects that do not move are copied from
 // `reuse` (nullable) when it is the same scene, built, and the host arrays
@@ -127,5 +127,5 @@ struct wr_scene {
     uint64_t device_bytes = 0;
     uint32_t max_obj_depth = 0;
     double build_seconds = 0;
-    mutable std::unique_ptr<wr::Staging> staging = std::make_unique<wr::Staging>();  // for the per-pass rebuilds
+    mutable std::unique_ptr<wr::RebuildCache> rebuild = std::make_unique<wr::RebuildCache>();  // for the per-pass rebuilds
 };
