@@ -686,9 +686,42 @@ struct VcmGrid {
     float r;
 };
 
+// The merges of the VCM lanes, one entry per light point in reach of an eye
+// vertex: word 0 = lane | eye vertex << 24 (lanes < 2^22), word 1 = the light
+// point; count at ent[2 * cap].  A separate kernel takes one merge per lane, so
+// the uneven number of merges per pixel does not leave most lanes of a wave idle.
+struct MergeQueue {
+    uint32_t* ent;
+    uint32_t cap;
+};
+
+// One merge (throughputByMerging, vcmTracer.cpp:196-213): the eye vertex i of
+// lane `lane` with light point pi of the lane's pass.  Returns the contribution
+// (before the division by the pixel count); zero when the merge gives nothing.
+__device__ __forceinline__ f3 vcm_merge(const SceneDev& S, const CamDev& cam, uint32_t npix, uint32_t maxv,
+                                        const BVert* verts, const VcmGrid& G, uint64_t lane, int i, uint32_t pi) {
+    const uint32_t p = (uint32_t)(lane % npix);
+    const uint32_t k = G.src[2 * (size_t)pi], j = G.src[2 * (size_t)pi + 1];
+    Joined P;
+    P.L = verts + (size_t)k * 2 * maxv + maxv;
+    P.E = verts + lane * 2 * maxv;
+    P.nl = (int)j;
+    P.ne = i + 1;
+    P.n = (int)j + i + 1;
+    P.cl = (int)j - 1;
+    P.edited = false;
+    if (!connect_edge_vm(S, P)) return mk(0.f, 0.f, 0.f);
+    const VcmTerms vm{G.r, (double)npix, true};
+    f3 color;
+    float prob;
+    const float w = color_prob_weight(S, cam, p, P, color, prob, &vm);
+    if (len(color) == 0.f || prob == 0.f) return mk(0.f, 0.f, 0.f);
+    return divs(muls(color, w), prob);
+}
+
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_vcm(SceneDev S, CamDev cam, uint32_t npix, uint32_t npass, uint32_t maxv,
                                              const BVert* verts, const uint32_t* counts, float* pix_all, VisQueue Q,
-                                             const VcmGrid* grids) {
+                                             const VcmGrid* grids, MergeQueue M) {
     const uint64_t lane = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (lane >= (uint64_t)npass * npix) return;
     const uint32_t kb = (uint32_t)(lane / npix), p = (uint32_t)(lane % npix);
@@ -771,7 +804,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // throughputByMerging (vcmTracer.cpp:175-217): the light points within the
     // radius of every non-delta eye vertex before an emitter, in the grid's
     // order (HashGrid::Process); the default points have no path and give nothing
-    const VcmTerms vm{G.r, (double)npix, true};
     for (int i = 1; i < max_eye; i++) {
         const BVert ev = E[i];
         if (emissive(S, ev.contact)) break;
@@ -793,27 +825,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 for (; a < b; a++) {
                     const uint32_t pi = G.idx[a];
                     if (!grid_near(G.g, G.pos, ev.o, pi)) continue;
-                    const uint32_t k = G.src[2 * (size_t)pi], j = G.src[2 * (size_t)pi + 1];
-                    Joined P;
-                    P.L = verts + (size_t)k * 2 * maxv + maxv;
-                    P.E = E;
-                    P.nl = (int)j;
-                    P.ne = i + 1;
-                    P.n = (int)j + i + 1;
-                    P.cl = (int)j - 1;
-                    P.edited = false;
-                    if (!connect_edge_vm(S, P)) continue;
-                    f3 color;
-                    float prob;
-                    const float w = color_prob_weight(S, cam, p, P, color, prob, &vm);
-                    if (len(color) == 0.f || prob == 0.f) continue;
-                    contrib = add(contrib, divs(muls(color, w), prob));
+                    const uint32_t slot = wave_slot(M.ent + 2 * (size_t)M.cap);
+                    if (slot < M.cap) {
+                        M.ent[2 * (size_t)slot] = (uint32_t)lane | ((uint32_t)i << 24);
+                        M.ent[2 * (size_t)slot + 1] = pi;
+                    } else {  // queue full: merge here
+                        contrib = add(contrib, vcm_merge(S, cam, npix, maxv, verts, G, lane, i, pi));
+                    }
                 }
             }
         }
         own = add(own, divs(contrib, (float)npix));
     }
     atomic_add3(pix, p, own);
+}
+
+// The queued merges of k_vcm, one per lane, each added to its pixel of its pass.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_vcm_merge(SceneDev S, CamDev cam, uint32_t npix, uint32_t maxv,
+                                                   const BVert* verts, float* pix_all, MergeQueue M,
+                                                   const VcmGrid* grids) {
+    const uint32_t n = min(M.ent[2 * (size_t)M.cap], M.cap);
+    for (uint32_t e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) {
+        const uint32_t w0 = M.ent[2 * (size_t)e], pi = M.ent[2 * (size_t)e + 1];
+        const uint32_t lane = w0 & 0xFFFFFFu;
+        const int i = (int)(w0 >> 24);
+        const uint32_t kb = lane / npix, p = lane % npix;
+        const f3 c = vcm_merge(S, cam, npix, maxv, verts, grids[kb], lane, i, pi);
+        if (c.x != 0.f || c.y != 0.f || c.z != 0.f) atomic_add3(pix_all + 3 * (size_t)kb * npix, p, divs(c, (float)npix));
+    }
 }
 
 // ---- PhotonMap (photonMap.cpp:21-680), progressive photon mapping ----------
@@ -1036,9 +1075,20 @@ int launch_vcm_passes(wr_ctx* ctx, const SceneDev& S, const wr_tree_hdr& H, cons
         return rc;
     }
     const wrd::VisQueue Q{q.rays, q.pay, q.dst, q.count, q.cap};
+    const wrd::MergeQueue M{vb.mq, vb.mq_cap};
+    e = hipMemsetAsync(vb.mq + 2 * (size_t)vb.mq_cap, 0, sizeof(uint32_t), st);
+    if (e != hipSuccess) {
+        destroy();
+        return hip_fail(ctx, e, "wr_render_vcm");
+    }
     wrd::k_vcm<<<blocks, 256, wrd::kStackLdsBytes, st>>>(S, cam, npix, npass, maxv, V, q.counts, pix, Q,
-                                                        (const wrd::VcmGrid*)vb.grids);
+                                                        (const wrd::VcmGrid*)vb.grids, M);
     e = hipGetLastError();
+    if (e == hipSuccess) {
+        wrd::k_vcm_merge<<<4096, 256, 0, st>>>(S, cam, npix, maxv, V, pix, M,
+                                                                (const wrd::VcmGrid*)vb.grids);
+        e = hipGetLastError();
+    }
     uint32_t nq = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&nq, q.count, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);

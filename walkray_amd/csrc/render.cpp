@@ -351,10 +351,12 @@ extern "C" int wr_render_vcm(wr_ctx* ctx, const wr_scene* s, const wr_camera* ca
     VcmBufs vb{};
     vb.cap = lanes * maxv;
     vb.scan_bytes = vcm_scan_bytes((uint32_t)lanes);
+    vb.mq_cap = (uint32_t)std::min<uint64_t>(64ull * lanes, 48ull << 20);  // beyond it a lane merges in place
     void *verts = nullptr, *qbuf = nullptr;
     float* pix = nullptr;
     hipError_t e = hipMalloc(&verts, per_pass * batch);
     if (e == hipSuccess) e = hipMalloc((void**)&pix, sizeof(float) * 3 * lanes);
+    if (e == hipSuccess) e = hipMalloc((void**)&vb.mq, sizeof(uint32_t) * (2 * (size_t)vb.mq_cap + 1));
     if (e == hipSuccess) e = hipMalloc(&qbuf, kBdptQueueEntryBytes * (size_t)q.cap + 256);
     if (e == hipSuccess) e = hipMalloc((void**)&q.counts, sizeof(uint32_t) * 2 * lanes);
     if (e == hipSuccess) e = hipMalloc((void**)&vb.pc, sizeof(uint32_t) * (lanes + 1));
@@ -374,7 +376,7 @@ extern "C" int wr_render_vcm(wr_ctx* ctx, const wr_scene* s, const wr_camera* ca
     }
     auto release = [&]() {
         for (void* ptr : {verts, (void*)pix, qbuf, (void*)q.counts, (void*)vb.pc, (void*)vb.offs, (void*)vb.starts,
-                          vb.scan, (void*)vb.pos, (void*)vb.src, vb.grids})
+                          vb.scan, (void*)vb.pos, (void*)vb.src, vb.grids, (void*)vb.mq})
             (void)hipFree(ptr);
     };
     if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);

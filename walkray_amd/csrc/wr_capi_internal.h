@@ -98,6 +98,8 @@ struct VcmBufs {
     uint32_t* src;
     uint64_t cap;
     void* grids;
+    uint32_t* mq;      // VCM merges queued for k_vcm_merge: 2 words each, then the count
+    uint32_t mq_cap;
 };
 float vcm_radius(float base, uint32_t pass);
 size_t vcm_scan_bytes(uint32_t lanes);
