@@ -722,8 +722,12 @@ __device__ __forceinline__ f3 vcm_merge(const SceneDev& S, const CamDev& cam, ui
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_vcm(SceneDev S, CamDev cam, uint32_t npix, uint32_t npass, uint32_t maxv,
                                              const BVert* verts, const uint32_t* counts, float* pix_all, VisQueue Q,
                                              const VcmGrid* grids, MergeQueue M) {
-    const uint64_t lane = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (lane >= (uint64_t)npass * npix) return;
+    // every lane of a wave stays to the end (the merge reservations below are
+    // wave-wide scans); a lane past the last one stands in for lane 0 and does
+    // nothing
+    const uint64_t lane_id = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool valid = lane_id < (uint64_t)npass * npix;
+    const uint64_t lane = valid ? lane_id : 0;
     const uint32_t kb = (uint32_t)(lane / npix), p = (uint32_t)(lane % npix);
     const VcmGrid G = grids[kb];
     float* pix = pix_all + 3 * (size_t)kb * npix;
@@ -737,7 +741,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const VcmTerms vc{G.r, (double)npix, false};
     f3 own = mk(0.f, 0.f, 0.f);
     // throughputByConnecting (vcmTracer.cpp:88-173): k_bdpt's loop with vcmMisWeight
-    for (int whole = 2; whole <= max_eye + max_light; whole++) {
+    for (int whole = 2; valid && whole <= max_eye + max_light; whole++) {
         const int min_l = whole - max_eye > 0 ? whole - max_eye : 0;
         const int max_l = whole - 1 >= max_light ? max_light : whole - 1;
         for (int light_len = min_l; light_len <= max_l; light_len++) {
@@ -803,15 +807,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
     // throughputByMerging (vcmTracer.cpp:175-217): the light points within the
     // radius of every non-delta eye vertex before an emitter, in the grid's
-    // order (HashGrid::Process); the default points have no path and give nothing
-    for (int i = 1; i < max_eye; i++) {
-        const BVert ev = E[i];
-        if (emissive(S, ev.contact)) break;
-        if (ev.flags & kVDelta) continue;
-        f3 contrib = mk(0.f, 0.f, 0.f);
+    // order (HashGrid::Process); the default points have no path and give
+    // nothing.  Each merge is queued for k_vcm_merge: per eye vertex the lane
+    // counts its points in reach, the wave reserves their slots with one
+    // atomic (an exclusive scan of the counts), and the lane writes them.
+    const uint32_t wl = threadIdx.x & 63u;
+    uint32_t* mcount = M.ent + 2 * (size_t)M.cap;
+    bool alive = valid;
+    for (int i = 1; i < (int)maxv; i++) {
+        bool act = alive && i < max_eye;
+        BVert ev;
+        if (act) {
+            ev = E[i];
+            if (emissive(S, ev.contact)) {
+                alive = false;
+                act = false;
+            } else if (ev.flags & kVDelta) {
+                act = false;
+            }
+        }
+        if (i + 1 >= max_eye) alive = false;
         int pc[6];
-        if (query_corner(G.g, ev.o, pc)) {
-            for (int c8 = 0; c8 < 8; c8++) {
+        const bool q = act && query_corner(G.g, ev.o, pc);
+        uint32_t cnt = 0;
+        for (int pass2 = 0; pass2 < 2; pass2++) {
+            uint32_t k = 0;
+            if (pass2) {  // the slots of this lane's merges: the wave's scan, one atomic
+                uint32_t v = cnt;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const uint32_t t = __shfl_up(v, off, 64);
+                    if (wl >= (uint32_t)off) v += t;
+                }
+                const uint32_t total = __shfl(v, 63, 64);
+                uint32_t base = 0;
+                if (wl == 0 && total) base = atomicAdd(mcount, total);
+                k = __shfl(base, 0, 64) + (v - cnt);
+                if (cnt == 0) break;
+            }
+            f3 contrib = mk(0.f, 0.f, 0.f);
+            for (int c8 = 0; q && c8 < 8; c8++) {
                 const uint32_t cell = query_cell(G.g, pc, c8);
                 int a = cell == 0 ? 0 : G.ends[cell - 1];
                 const int b = G.ends[cell];
@@ -825,19 +859,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 for (; a < b; a++) {
                     const uint32_t pi = G.idx[a];
                     if (!grid_near(G.g, G.pos, ev.o, pi)) continue;
-                    const uint32_t slot = wave_slot(M.ent + 2 * (size_t)M.cap);
-                    if (slot < M.cap) {
-                        M.ent[2 * (size_t)slot] = (uint32_t)lane | ((uint32_t)i << 24);
-                        M.ent[2 * (size_t)slot + 1] = pi;
+                    if (!pass2) {
+                        cnt++;
+                    } else if (k < M.cap) {
+                        M.ent[2 * (size_t)k] = (uint32_t)lane | ((uint32_t)i << 24);
+                        M.ent[2 * (size_t)k + 1] = pi;
+                        k++;
                     } else {  // queue full: merge here
                         contrib = add(contrib, vcm_merge(S, cam, npix, maxv, verts, G, lane, i, pi));
+                        k++;
                     }
                 }
             }
+            if (pass2) own = add(own, divs(contrib, (float)npix));
         }
-        own = add(own, divs(contrib, (float)npix));
+        if (__ballot(alive) == 0) break;
     }
-    atomic_add3(pix, p, own);
+    if (valid) atomic_add3(pix, p, own);
 }
 
 // The queued merges of k_vcm, one per lane, each added to its pixel of its pass.
