@@ -102,6 +102,28 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter) {
     return base + rank;
 }
 
+// One slot per lane that wants one, for the whole 256-lane workgroup with a
+// single atomic; every thread of the workgroup must make the call.  Slots go
+// out in thread order.
+__device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool want) {
+    __shared__ uint32_t s_cnt[4];
+    __shared__ uint32_t s_base;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint64_t mask = __ballot(want);
+    if (lane == 0) s_cnt[w] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+        s_base = t ? atomicAdd(counter, t) : 0u;
+    }
+    __syncthreads();
+    uint32_t off = s_base;
+    for (uint32_t v = 0; v < w; v++) off += s_cnt[v];
+    off += (uint32_t)__popcll(mask & ((1ull << lane) - 1));
+    __syncthreads();  // s_cnt and s_base serve the next call
+    return off;
+}
+
 // *addr += 1 for every active lane, one atomic per distinct address in the wave
 // (lanes of a wave almost always finish paths of the same job and pass).
 __device__ __forceinline__ void count_done(uint32_t* addr) {
@@ -340,69 +362,117 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXITS ? 1 :
     const uint32_t n = EXITS ? E.cnt->n_exit : E.cnt->n[par];
     StackSpill spill;
     LaneStack st(&spill);
-    for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
-        const uint32_t i = EXITS ? E.exit_list[k] : k;
-        if (!EXITS && (A.flags[i] & kDead)) continue;
-        const uint32_t jb = A.job[i];
-        const wr::JobDesc& J = E.jobs[jb];
-        const SceneCtx sc = make_scene_ctx(E.scenes[J.scene]);
-        const f3 o = mk(A.ox[i], A.oy[i], A.oz[i]);
-        const f3 d = mk(A.dx[i], A.dy[i], A.dz[i]);
-        PHit h;
-        h.t = A.ht[i];
-        h.rec = A.hrec[i];
-        h.obj = A.hobj[i];
-        const uint32_t inside = A.inside[i], depth = A.depth[i], p = A.pixel[i], ps = A.pass[i];
-        const uint32_t self = inside != WR_NONE ? inside : h.obj;
-        if (!EXITS && scatter_queries_inside(sc, self, h, inside)) {
-            E.exit_list[wave_append(&E.cnt->n_exit)] = i;
-            continue;
-        }
-        float rn[4];
-        rng4(J.seed, p, J.pass0 + ps, depth + 1, rn);
-        const Out out = scatter<EXITS>(sc, self, o, d, h, inside, rn, st);
-        f3 T = mk(A.tx[i], A.ty[i], A.tz[i]);
-        if ((double)len(out.d) < 0.5) {  // emission / absorption ends the path
-            finish(J, job_cam(J), p, ps, mulv(T, divs(divs(out.rad, out.op), out.dp)));
-            continue;
-        }
-        const bool nee = (J.flags & wr::kJobNee) != 0;
-        if (depth + 1 > J.max_depth) {
-            if (nee && depth >= 1) {  // endIndex = this ray: connect from its origin
-                rng4(J.seed, p, J.pass0 + ps, kNeeVertex, rn);
-                const NeeConn c = nee_connect(sc, o, A.contact[i], A.crec[i], (A.flags[i] & kFront) != 0,
-                                              mk(A.px[i], A.py[i], A.pz[i]), rn);
-                if (c.ok) {
-                    push_connection(B, &E.cnt->n[par ^ 1], o, c, inside, A.contact[i], p, ps, jb);
-                    continue;
+    // the workgroup steps through the slots together: each step's survivors,
+    // connection entries and refraction exits take their slots with one
+    // atomic per workgroup (block_append) instead of one per wave
+    for (uint32_t k0 = blockIdx.x * 256; k0 < n; k0 += gridDim.x * 256) {
+        const uint32_t k = k0 + threadIdx.x;
+        enum { kNone, kSurvive, kConnect, kExit } fate = kNone;
+        uint32_t i = 0, jb = 0, p = 0, ps = 0, depth = 0, inside = 0, contact = 0, crec = 0, flags = 0;
+        f3 o0, d0, t0, u0, u1;  // the next entry: origin, direction, throughput, NEE products
+        float dist = 0.f;
+        if (k < n) {
+            i = EXITS ? E.exit_list[k] : k;
+            if (EXITS || !(A.flags[i] & kDead)) {
+                jb = A.job[i];
+                const wr::JobDesc& J = E.jobs[jb];
+                const SceneCtx sc = make_scene_ctx(E.scenes[J.scene]);
+                const f3 o = mk(A.ox[i], A.oy[i], A.oz[i]);
+                const f3 d = mk(A.dx[i], A.dy[i], A.dz[i]);
+                PHit h;
+                h.t = A.ht[i];
+                h.rec = A.hrec[i];
+                h.obj = A.hobj[i];
+                inside = A.inside[i];
+                depth = A.depth[i];
+                p = A.pixel[i];
+                ps = A.pass[i];
+                const uint32_t self = inside != WR_NONE ? inside : h.obj;
+                if (!EXITS && scatter_queries_inside(sc, self, h, inside)) {
+                    fate = kExit;
+                } else {
+                    float rn[4];
+                    rng4(J.seed, p, J.pass0 + ps, depth + 1, rn);
+                    const Out out = scatter<EXITS>(sc, self, o, d, h, inside, rn, st);
+                    f3 T = mk(A.tx[i], A.ty[i], A.tz[i]);
+                    const bool nee = (J.flags & wr::kJobNee) != 0;
+                    if ((double)len(out.d) < 0.5) {  // emission / absorption ends the path
+                        finish(J, job_cam(J), p, ps, mulv(T, divs(divs(out.rad, out.op), out.dp)));
+                    } else if (depth + 1 > J.max_depth) {
+                        NeeConn c;
+                        c.ok = false;
+                        if (nee && depth >= 1) {  // endIndex = this ray: connect from its origin
+                            rng4(J.seed, p, J.pass0 + ps, kNeeVertex, rn);
+                            c = nee_connect(sc, o, A.contact[i], A.crec[i], (A.flags[i] & kFront) != 0,
+                                            mk(A.px[i], A.py[i], A.pz[i]), rn);
+                        }
+                        if (c.ok) {
+                            fate = kConnect;
+                            o0 = o;
+                            d0 = c.dir;
+                            t0 = c.C;
+                            dist = c.dist;
+                            contact = A.contact[i];
+                        } else {
+                            finish(J, job_cam(J), p, ps, mk(0.f, 0.f, 0.f));
+                        }
+                    } else {
+                        // the weight of this vertex only matters if the next ray hits; applying
+                        // it now keeps the per-pixel loop's multiplication order
+                        T = muls(T, cosine_term(out));
+                        T = mulv(T, divs(divs(out.rad, out.op), out.dp));
+                        fate = kSurvive;
+                        o0 = out.o;
+                        d0 = out.d;
+                        t0 = T;
+                        inside = out.inside;
+                        contact = out.contact;
+                        flags = nee && dot(d, out.n) <= 0.f ? kFront : 0u;
+                        if (nee) {  // the next ray's products (branch order) and its origin's triangle
+                            u0 = mk(A.ux[i], A.uy[i], A.uz[i]);
+                            u1 = mulv(u0, divs(divs(out.rad, out.op), out.dp));
+                            u1 = muls(u1, cosine_term(out));
+                            crec = h.rec;
+                            flags |= 0x80000000u;  // marks the NEE fields for the write below
+                        }
+                    }
                 }
             }
-            finish(J, job_cam(J), p, ps, mk(0.f, 0.f, 0.f));
-            continue;
         }
-        // the weight of this vertex only matters if the next ray hits; applying it
-        // now keeps the per-pixel loop's multiplication order
-        T = muls(T, cosine_term(out));
-        T = mulv(T, divs(divs(out.rad, out.op), out.dp));
-        const uint32_t j = wave_append(&E.cnt->n[par ^ 1]);
-        if (nee) {  // the next ray's products (branch order) and its origin's triangle and side
-            const f3 U = mk(A.ux[i], A.uy[i], A.uz[i]);
-            f3 U2 = mulv(U, divs(divs(out.rad, out.op), out.dp));
-            U2 = muls(U2, cosine_term(out));
-            B.px[j] = U.x; B.py[j] = U.y; B.pz[j] = U.z;
-            B.ux[j] = U2.x; B.uy[j] = U2.y; B.uz[j] = U2.z;
-            B.crec[j] = h.rec;
+        const uint32_t j = block_append(&E.cnt->n[par ^ 1], fate == kSurvive || fate == kConnect);
+        if (!EXITS) {
+            const uint32_t x = block_append(&E.cnt->n_exit, fate == kExit);
+            if (fate == kExit) E.exit_list[x] = i;
         }
-        B.ox[j] = out.o.x; B.oy[j] = out.o.y; B.oz[j] = out.o.z;
-        B.dx[j] = out.d.x; B.dy[j] = out.d.y; B.dz[j] = out.d.z;
-        B.tx[j] = T.x; B.ty[j] = T.y; B.tz[j] = T.z;
-        B.pixel[j] = p;
-        B.pass[j] = ps;
-        B.depth[j] = depth + 1;
-        B.inside[j] = out.inside;
-        B.contact[j] = out.contact;
-        B.flags[j] = nee && dot(d, out.n) <= 0.f ? kFront : 0u;
-        B.job[j] = jb;
+        if (fate == kConnect) {  // a connection entry (MCRenderer::visibilityTest of the NEE branch)
+            B.ox[j] = o0.x; B.oy[j] = o0.y; B.oz[j] = o0.z;
+            B.dx[j] = d0.x; B.dy[j] = d0.y; B.dz[j] = d0.z;
+            B.tx[j] = t0.x; B.ty[j] = t0.y; B.tz[j] = t0.z;
+            B.ux[j] = dist;
+            B.pixel[j] = p;
+            B.pass[j] = ps;
+            B.depth[j] = 0;
+            B.inside[j] = inside;
+            B.contact[j] = contact;
+            B.flags[j] = kShadow;
+            B.job[j] = jb;
+        } else if (fate == kSurvive) {
+            if (flags & 0x80000000u) {
+                B.px[j] = u0.x; B.py[j] = u0.y; B.pz[j] = u0.z;
+                B.ux[j] = u1.x; B.uy[j] = u1.y; B.uz[j] = u1.z;
+                B.crec[j] = crec;
+            }
+            B.ox[j] = o0.x; B.oy[j] = o0.y; B.oz[j] = o0.z;
+            B.dx[j] = d0.x; B.dy[j] = d0.y; B.dz[j] = d0.z;
+            B.tx[j] = t0.x; B.ty[j] = t0.y; B.tz[j] = t0.z;
+            B.pixel[j] = p;
+            B.pass[j] = ps;
+            B.depth[j] = depth + 1;
+            B.inside[j] = inside;
+            B.contact[j] = contact;
+            B.flags[j] = flags & ~0x80000000u;
+            B.job[j] = jb;
+        }
     }
 }
 
