@@ -159,7 +159,10 @@ __device__ __forceinline__ void push_connection(const PathBuf& B, uint32_t* coun
     B.job[j] = job;
 }
 
-__device__ __forceinline__ void finish(const wr::JobDesc& J, const CamDev& cam, uint32_t p, uint32_t ps, f3 color) {
+// The value of a finished path into its pass slot; returns the pass's
+// completion counter, which the caller increments.
+__device__ __forceinline__ uint32_t* finish_value(const wr::JobDesc& J, const CamDev& cam, uint32_t p, uint32_t ps,
+                                                  f3 color) {
     if (!legal(color)) color = mk(0.f, 0.f, 0.f);
     const f3 v = fix_vignetting(cam, color, p);
     const uint32_t key = ps % J.R;
@@ -167,7 +170,34 @@ __device__ __forceinline__ void finish(const wr::JobDesc& J, const CamDev& cam, 
     c[0] = v.x;
     c[1] = v.y;
     c[2] = v.z;
-    count_done(J.pass_done + key);
+    return J.pass_done + key;
+}
+
+__device__ __forceinline__ void finish(const wr::JobDesc& J, const CamDev& cam, uint32_t p, uint32_t ps, f3 color) {
+    count_done(finish_value(J, cam, p, ps, color));
+}
+
+// *addr += 1 for every lane of the workgroup with want set; every thread must
+// make the call.  The lanes whose counter is the first wanting lane's (in a
+// single-job render, almost all of them) add with one atomic for the
+// workgroup, the others per wave as count_done does.
+__device__ __forceinline__ void block_count_done(uint32_t* addr, bool want) {
+    __shared__ unsigned long long s_addr;
+    __shared__ uint32_t s_n;
+    if (threadIdx.x == 0) {
+        s_addr = 0ull;
+        s_n = 0u;
+    }
+    __syncthreads();
+    if (want) atomicCAS(&s_addr, 0ull, (unsigned long long)addr);
+    __syncthreads();
+    const bool same = want && (unsigned long long)addr == s_addr;
+    const uint64_t m = __ballot(same);
+    if ((threadIdx.x & 63u) == 0 && m) atomicAdd(&s_n, (uint32_t)__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0 && s_n) atomicAdd((uint32_t*)s_addr, s_n);
+    if (want && !same) count_done(addr);
+    __syncthreads();  // s_addr and s_n serve the next call
 }
 
 // extend, part 1: the fast pass of every live ray (Scene::intersect; camera
@@ -369,6 +399,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXITS ? 1 :
         const uint32_t k = k0 + threadIdx.x;
         enum { kNone, kSurvive, kConnect, kExit } fate = kNone;
         uint32_t i = 0, jb = 0, p = 0, ps = 0, depth = 0, inside = 0, contact = 0, crec = 0, flags = 0;
+        uint32_t* done = nullptr;  // the completion counter of a path that ends here
         f3 o0, d0, t0, u0, u1;  // the next entry: origin, direction, throughput, NEE products
         float dist = 0.f;
         if (k < n) {
@@ -397,7 +428,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXITS ? 1 :
                     f3 T = mk(A.tx[i], A.ty[i], A.tz[i]);
                     const bool nee = (J.flags & wr::kJobNee) != 0;
                     if ((double)len(out.d) < 0.5) {  // emission / absorption ends the path
-                        finish(J, job_cam(J), p, ps, mulv(T, divs(divs(out.rad, out.op), out.dp)));
+                        done = finish_value(J, job_cam(J), p, ps, mulv(T, divs(divs(out.rad, out.op), out.dp)));
                     } else if (depth + 1 > J.max_depth) {
                         NeeConn c;
                         c.ok = false;
@@ -414,7 +445,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXITS ? 1 :
                             dist = c.dist;
                             contact = A.contact[i];
                         } else {
-                            finish(J, job_cam(J), p, ps, mk(0.f, 0.f, 0.f));
+                            done = finish_value(J, job_cam(J), p, ps, mk(0.f, 0.f, 0.f));
                         }
                     } else {
                         // the weight of this vertex only matters if the next ray hits; applying
@@ -439,6 +470,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXITS ? 1 :
                 }
             }
         }
+        block_count_done(done, done != nullptr);
         const uint32_t j = block_append(&E.cnt->n[par ^ 1], fate == kSurvive || fate == kConnect);
         if (!EXITS) {
             const uint32_t x = block_append(&E.cnt->n_exit, fate == kExit);
