@@ -90,18 +90,6 @@ __device__ __forceinline__ CamDev job_cam(const wr::JobDesc& J) {
     return c;
 }
 
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter) {
-    // one atomic per wave; lanes get consecutive slots in lane order
-    const uint64_t mask = __ballot(1);
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
-    const uint32_t rank = __popcll(mask & ((1ull << lane) - 1));
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    return base + rank;
-}
-
 // One slot per lane that wants one, for the whole 256-lane workgroup with a
 // single atomic; every thread of the workgroup must make the call.  Slots go
 // out in thread order.
@@ -141,11 +129,11 @@ __device__ __forceinline__ void count_done(uint32_t* addr) {
     }
 }
 
-// Appends a connection entry (one atomic per wave) to the next iteration's buffer.
-__device__ __forceinline__ void push_connection(const PathBuf& B, uint32_t* counter, f3 o, const NeeConn& c,
-                                                uint32_t inside, uint32_t contact, uint32_t pixel, uint32_t pass,
-                                                uint32_t job) {
-    const uint32_t j = wave_append(counter);
+// A connection entry (MCRenderer::visibilityTest of the NEE branch) at slot j
+// of the next iteration's buffer.
+__device__ __forceinline__ void write_connection(const PathBuf& B, uint32_t j, f3 o, const NeeConn& c,
+                                                 uint32_t inside, uint32_t contact, uint32_t pixel, uint32_t pass,
+                                                 uint32_t job) {
     B.ox[j] = o.x; B.oy[j] = o.y; B.oz[j] = o.z;
     B.dx[j] = c.dir.x; B.dy[j] = c.dir.y; B.dz[j] = c.dir.z;
     B.tx[j] = c.C.x; B.ty[j] = c.C.y; B.tz[j] = c.C.z;
@@ -171,10 +159,6 @@ __device__ __forceinline__ uint32_t* finish_value(const wr::JobDesc& J, const Ca
     c[1] = v.y;
     c[2] = v.z;
     return J.pass_done + key;
-}
-
-__device__ __forceinline__ void finish(const wr::JobDesc& J, const CamDev& cam, uint32_t p, uint32_t ps, f3 color) {
-    count_done(finish_value(J, cam, p, ps, color));
 }
 
 // *addr += 1 for every lane of the workgroup with want set; every thread must
@@ -325,55 +309,65 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const uint32_t n = E.cnt->n[par];
     StackSpill spill;
     LaneStack st(&spill);
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        if (E.ctl[A.job[i]].state != wr::kJobRunning) {  // a cancelled job: drop its path, write nothing
-            A.flags[i] = kDead;
-            continue;
-        }
-        const wr::JobDesc& J = E.jobs[A.job[i]];
-        const SceneDev& S = E.scenes[J.scene];
-        const SceneCtx sc = make_scene_ctx(S);
-        const f3 o = mk(A.ox[i], A.oy[i], A.oz[i]);
-        const f3 d = mk(A.dx[i], A.dy[i], A.dz[i]);
-        const uint32_t fl = A.flags[i];
-        const bool primary = fl & kPrimary;
-        HitState h{A.ht[i], A.hrec[i], A.hobj[i]};
-        if (h.rec != WR_NONE || (fl & kOverflow)) {
-            const RayPre r = make_ray(o, d, sc.tr.scale);
-            const NoSelf ns{primary ? WR_NONE : A.inside[i], primary ? WR_NONE : A.contact[i], sc.gnrm,
-                            S.obj_has_normals};
-            Counters cnt{0, 0, 0};
-            bool failed;
-            finish_hit<true, false>(sc.tr, r, ns, st, cnt, h, (fl & kOverflow) != 0, failed);
-        }
-        if (fl & kShadow) {  // MCRenderer::visibilityTest (mcRenderer.cpp:59-65)
-            const bool vis = nee_visible(h.rec, h.t, A.ux[i]);
-            finish(J, job_cam(J), A.pixel[i], A.pass[i], vis ? mk(A.tx[i], A.ty[i], A.tz[i]) : mk(0.f, 0.f, 0.f));
-            A.flags[i] = kDead;
-            continue;
-        }
-        if (primary) A.inside[i] = find_inside(S, o, d, WR_NONE, st);
-        if (h.rec == WR_NONE) {  // escaped (no environment light) or nothing to scatter: sample 0
-            if ((J.flags & wr::kJobNee) && !primary) {  // endIndex = this ray: connect from its origin
-                float rn[4];
-                rng4(J.seed, A.pixel[i], J.pass0 + A.pass[i], kNeeVertex, rn);
-                const NeeConn c = nee_connect(sc, o, A.contact[i], A.crec[i], (fl & kFront) != 0,
-                                              mk(A.px[i], A.py[i], A.pz[i]), rn);
-                if (c.ok) {
-                    push_connection(B, &E.cnt->n[par ^ 1], o, c, A.inside[i], A.contact[i], A.pixel[i], A.pass[i],
-                                    A.job[i]);
+    // the workgroup steps through the slots together: the paths that end here
+    // are counted, and the connection entries take their slots, with one
+    // atomic per workgroup
+    for (uint32_t i0 = blockIdx.x * 256; i0 < n; i0 += gridDim.x * 256) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t* done = nullptr;  // the completion counter of a path that ends here
+        bool conn = false;         // a connection entry from an escaped path (NEE)
+        NeeConn nc;
+        f3 co;
+        if (i < n) {
+            if (E.ctl[A.job[i]].state != wr::kJobRunning) {  // a cancelled job: drop its path, write nothing
+                A.flags[i] = kDead;
+            } else {
+                const wr::JobDesc& J = E.jobs[A.job[i]];
+                const SceneDev& S = E.scenes[J.scene];
+                const SceneCtx sc = make_scene_ctx(S);
+                const f3 o = mk(A.ox[i], A.oy[i], A.oz[i]);
+                const f3 d = mk(A.dx[i], A.dy[i], A.dz[i]);
+                const uint32_t fl = A.flags[i];
+                const bool primary = fl & kPrimary;
+                HitState h{A.ht[i], A.hrec[i], A.hobj[i]};
+                if (h.rec != WR_NONE || (fl & kOverflow)) {
+                    const RayPre r = make_ray(o, d, sc.tr.scale);
+                    const NoSelf ns{primary ? WR_NONE : A.inside[i], primary ? WR_NONE : A.contact[i], sc.gnrm,
+                                    S.obj_has_normals};
+                    Counters cnt{0, 0, 0};
+                    bool failed;
+                    finish_hit<true, false>(sc.tr, r, ns, st, cnt, h, (fl & kOverflow) != 0, failed);
+                }
+                if (fl & kShadow) {  // MCRenderer::visibilityTest (mcRenderer.cpp:59-65)
+                    const bool vis = nee_visible(h.rec, h.t, A.ux[i]);
+                    done = finish_value(J, job_cam(J), A.pixel[i], A.pass[i],
+                                        vis ? mk(A.tx[i], A.ty[i], A.tz[i]) : mk(0.f, 0.f, 0.f));
                     A.flags[i] = kDead;
-                    continue;
+                } else {
+                    if (primary) A.inside[i] = find_inside(S, o, d, WR_NONE, st);
+                    if (h.rec == WR_NONE) {  // escaped (no environment light) or nothing to scatter: sample 0
+                        if ((J.flags & wr::kJobNee) && !primary) {  // endIndex = this ray: connect from its origin
+                            float rn[4];
+                            rng4(J.seed, A.pixel[i], J.pass0 + A.pass[i], kNeeVertex, rn);
+                            nc = nee_connect(sc, o, A.contact[i], A.crec[i], (fl & kFront) != 0,
+                                             mk(A.px[i], A.py[i], A.pz[i]), rn);
+                            conn = nc.ok;
+                            co = o;
+                        }
+                        if (!conn) done = finish_value(J, job_cam(J), A.pixel[i], A.pass[i], mk(0.f, 0.f, 0.f));
+                        A.flags[i] = kDead;
+                    } else {
+                        A.ht[i] = h.t;
+                        A.hrec[i] = h.rec;
+                        A.hobj[i] = sc.tr.tris[h.rec].obj;
+                        A.flags[i] = fl & kFront;
+                    }
                 }
             }
-            finish(J, job_cam(J), A.pixel[i], A.pass[i], mk(0.f, 0.f, 0.f));
-            A.flags[i] = kDead;
-            continue;
         }
-        A.ht[i] = h.t;
-        A.hrec[i] = h.rec;
-        A.hobj[i] = sc.tr.tris[h.rec].obj;
-        A.flags[i] = fl & kFront;
+        block_count_done(done, done != nullptr);
+        const uint32_t j = block_append(&E.cnt->n[par ^ 1], conn);
+        if (conn) write_connection(B, j, co, nc, A.inside[i], A.contact[i], A.pixel[i], A.pass[i], A.job[i]);
     }
 }
 
@@ -401,7 +395,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXITS ? 1 :
         uint32_t i = 0, jb = 0, p = 0, ps = 0, depth = 0, inside = 0, contact = 0, crec = 0, flags = 0;
         uint32_t* done = nullptr;  // the completion counter of a path that ends here
         f3 o0, d0, t0, u0, u1;  // the next entry: origin, direction, throughput, NEE products
-        float dist = 0.f;
+        NeeConn nc;             // a connection entry's direction, value and distance
         if (k < n) {
             i = EXITS ? E.exit_list[k] : k;
             if (EXITS || !(A.flags[i] & kDead)) {
@@ -440,9 +434,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXITS ? 1 :
                         if (c.ok) {
                             fate = kConnect;
                             o0 = o;
-                            d0 = c.dir;
-                            t0 = c.C;
-                            dist = c.dist;
+                            nc = c;
                             contact = A.contact[i];
                         } else {
                             done = finish_value(J, job_cam(J), p, ps, mk(0.f, 0.f, 0.f));
@@ -476,18 +468,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXITS ? 1 :
             const uint32_t x = block_append(&E.cnt->n_exit, fate == kExit);
             if (fate == kExit) E.exit_list[x] = i;
         }
-        if (fate == kConnect) {  // a connection entry (MCRenderer::visibilityTest of the NEE branch)
-            B.ox[j] = o0.x; B.oy[j] = o0.y; B.oz[j] = o0.z;
-            B.dx[j] = d0.x; B.dy[j] = d0.y; B.dz[j] = d0.z;
-            B.tx[j] = t0.x; B.ty[j] = t0.y; B.tz[j] = t0.z;
-            B.ux[j] = dist;
-            B.pixel[j] = p;
-            B.pass[j] = ps;
-            B.depth[j] = 0;
-            B.inside[j] = inside;
-            B.contact[j] = contact;
-            B.flags[j] = kShadow;
-            B.job[j] = jb;
+        if (fate == kConnect) {
+            write_connection(B, j, o0, nc, inside, contact, p, ps, jb);
         } else if (fate == kSurvive) {
             if (flags & 0x80000000u) {
                 B.px[j] = u0.x; B.py[j] = u0.y; B.pz[j] = u0.z;
