@@ -175,7 +175,8 @@ def closest_hit_bench(W, S, ctx, steps, warmup, n_side=4096, hf_n=1024):
         "avg_launch_ms": round(avg_launch_ms, 4),
         "alg_bytes_per_launch": int(total_b / 2),
     }
-    roofline["launch_tuning"] = {kk: tuning[kk] for kk in ("refill_min", "node_phase_min", "blocks_per_cu", "lds_levels")}
+    roofline["launch_tuning"] = {kk: tuning[kk] for kk in ("refill_min", "node_phase_min", "blocks_per_cu", "lds_levels",
+                                                           "batch")}
     return {
         "scene": sc, "objs": objs, "coh": coh, "inc": inc, "n": n, "mrays": mrays, "ms_per_step": ms_per_step,
         "wall_s": wall, "per_set": per_set, "roofline": roofline, "info": info, "hit_frac_inc": hit_frac_inc,

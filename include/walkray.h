@@ -186,7 +186,8 @@ int wr_closest_hit(wr_ctx* ctx, const wr_scene* scene, const wr_ray* d_rays, uin
 /* Autotuner (no counterpart in the reference).  Times wr_closest_hit on the
  * given rays (a representative sample: every call is traced in full) under
  * candidate launch configurations of the persistent kernel (refill threshold,
- * node-phase threshold, workgroups per CU, traversal stack levels in LDS), by coordinate descent from the
+ * node-phase threshold, workgroups per CU, traversal stack levels in LDS, rays
+ * per work-counter atomic), by coordinate descent from the
  * scene's current choice, and keeps the fastest for the scene; later
  * wr_closest_hit calls use it.  Results never depend on the configuration.
  * Synchronous; allocates 16 * n bytes of scratch while it runs. */
@@ -195,6 +196,7 @@ typedef struct {
     uint32_t node_phase_min; /* lanes at wide nodes for a node-only step (1..65; 65 = never) */
     uint32_t blocks_per_cu;  /* resident 256-lane workgroups per CU, 0 = as many as fit */
     uint32_t lds_levels;     /* traversal stack levels kept in LDS: 4, 8 or 16 (0 = 8); deeper levels spill */
+    uint32_t batch;          /* rays a wave takes from the shared work counter at a time, 64..4096 (0 = 256) */
     float ms;                /* one call on the rays with this configuration */
 } wr_closest_hit_tuning;
 int wr_autotune_closest_hit(wr_ctx* ctx, wr_scene* scene, const wr_ray* d_rays, uint64_t n,

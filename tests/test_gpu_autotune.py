@@ -18,14 +18,15 @@ def test_closest_hit_tuning_keeps_results(ctx):
     rays = np.concatenate([S.pinhole_rays(S.heightfield_camera(192, 192)),
                            S.incoherent_rays(40000, lo, hi, seed=2)]).astype(np.float32)
     base = sc.closest_hit(rays)
-    for refill, node, blocks, levels in ((1, 65, 1, 4), (64, 1, 0, 16), (7, 13, 3, 8), (16, 32, 0, 4)):
-        sc.set_closest_hit_tuning(refill, node, blocks, levels)
+    for refill, node, blocks, levels, batch in ((1, 65, 1, 4, 64), (64, 1, 0, 16, 4096), (7, 13, 3, 8, 100),
+                                                (16, 32, 0, 4, 1024)):
+        sc.set_closest_hit_tuning(refill, node, blocks, levels, batch)
         h = sc.closest_hit(rays)
-        assert np.array_equal(h.view(np.uint32), base.view(np.uint32)), (refill, node, blocks, levels)
+        assert np.array_equal(h.view(np.uint32), base.view(np.uint32)), (refill, node, blocks, levels, batch)
     d = ctx.to_device(rays)
     t = sc.autotune_closest_hit(d, len(rays))
     assert 1 <= t["refill_min"] <= 64 and 1 <= t["node_phase_min"] <= 65 and t["ms"] > 0
-    assert t["lds_levels"] in (4, 8, 16)
+    assert t["lds_levels"] in (4, 8, 16) and 64 <= t["batch"] <= 4096
     assert np.array_equal(sc.closest_hit(rays).view(np.uint32), base.view(np.uint32))
 
 

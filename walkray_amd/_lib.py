@@ -79,7 +79,7 @@ class JobStatus(C.Structure):
 
 class ChTuning(C.Structure):
     _fields_ = [("refill_min", C.c_uint32), ("node_phase_min", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("lds_levels", C.c_uint32), ("ms", C.c_float)]
+                ("lds_levels", C.c_uint32), ("batch", C.c_uint32), ("ms", C.c_float)]
 
 
 class RenderTuning(C.Structure):
@@ -728,10 +728,10 @@ class Scene:
         _check(lib().wr_autotune_closest_hit(self.ctx.h, self.h, d_rays.ptr, n, C.byref(t)), self.ctx.h,
                "wr_autotune_closest_hit")
         return {"refill_min": t.refill_min, "node_phase_min": t.node_phase_min, "blocks_per_cu": t.blocks_per_cu,
-                "lds_levels": t.lds_levels, "ms": t.ms}
+                "lds_levels": t.lds_levels, "batch": t.batch, "ms": t.ms}
 
-    def set_closest_hit_tuning(self, refill_min, node_phase_min, blocks_per_cu=0, lds_levels=8):
-        t = ChTuning(refill_min, node_phase_min, blocks_per_cu, lds_levels, 0.0)
+    def set_closest_hit_tuning(self, refill_min, node_phase_min, blocks_per_cu=0, lds_levels=8, batch=256):
+        t = ChTuning(refill_min, node_phase_min, blocks_per_cu, lds_levels, batch, 0.0)
         _check(lib().wr_scene_set_closest_hit_tuning(self.h, C.byref(t)), self.ctx.h,
                "wr_scene_set_closest_hit_tuning")
 

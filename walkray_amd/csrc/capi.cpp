@@ -972,7 +972,7 @@ int wr_autotune_closest_hit(wr_ctx* ctx, wr_scene* s, const wr_ray* d_rays, uint
         }
         return hipSuccess;
     };
-    // coordinate descent over the three knobs, starting from the current choice;
+    // coordinate descent over the knobs, starting from the current choice;
     // a candidate must be 1% faster to replace it (timing noise stays put)
     ChTune cur = s->ch_tune;
     float cur_ms = 0.f;
@@ -981,6 +981,7 @@ int wr_autotune_closest_hit(wr_ctx* ctx, wr_scene* s, const wr_ray* d_rays, uint
     const uint32_t refills[] = {4, 8, 16, 24, 32};
     const uint32_t blocks[] = {0, 7, 6, 5};
     const uint32_t levels[] = {4, 8, 16};
+    const uint32_t batches[] = {128, 256, 512, 1024};
     for (int pass = 0; pass < 2 && e == hipSuccess; pass++) {
         for (uint32_t v : nodes) {
             ChTune t = cur;
@@ -1013,11 +1014,21 @@ int wr_autotune_closest_hit(wr_ctx* ctx, wr_scene* s, const wr_ray* d_rays, uint
             if ((e = time_of(t, m)) != hipSuccess) break;
             if (m < 0.99f * cur_ms) cur = t, cur_ms = m;
         }
+        for (uint32_t v : batches) {
+            if (e != hipSuccess) break;
+            ChTune t = cur;
+            t.batch = v;
+            float m;
+            if ((e = time_of(t, m)) != hipSuccess) break;
+            if (m < 0.99f * cur_ms) cur = t, cur_ms = m;
+        }
     }
     (void)hipFree(d_hits);
     if (e != hipSuccess) return hip_fail(ctx, e, "wr_autotune_closest_hit");
     s->ch_tune = cur;
-    if (best) *best = wr_closest_hit_tuning{cur.refill_min, cur.node_phase_min, cur.blocks_per_cu, cur.lds_levels, cur_ms};
+    if (best)
+        *best = wr_closest_hit_tuning{cur.refill_min, cur.node_phase_min, cur.blocks_per_cu, cur.lds_levels, cur.batch,
+                                      cur_ms};
     return WR_OK;
 }
 
@@ -1030,6 +1041,7 @@ int wr_scene_set_closest_hit_tuning(wr_scene* s, const wr_closest_hit_tuning* t)
     s->ch_tune.refill_min = t->refill_min;
     s->ch_tune.node_phase_min = t->node_phase_min;
     s->ch_tune.blocks_per_cu = t->blocks_per_cu;
+    s->ch_tune.batch = t->batch ? t->batch : 256u;
     return WR_OK;
 }
 

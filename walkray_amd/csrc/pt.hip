@@ -192,6 +192,11 @@ __device__ __forceinline__ void block_count_done(uint32_t* addr, bool want) {
 // of the leaf) and the overflow flag; k_wf_finish completes it.
 constexpr uint32_t kOverflow = 4u;
 constexpr uint32_t kTraceRefillMin = 16;
+// Paths a wave takes from the work counter at a time.  An iteration's paths
+// are fewer than a closest-hit call's rays and its tail matters more: the
+// closest-hit call's larger batches (fewer atomics on the one counter) made
+// this kernel slower.
+constexpr uint32_t kTraceBatch = 64;
 constexpr uint32_t kTraceNodePhaseMin = 32;
 
 // D: traversal stack levels in LDS (4, 8 or 16; the wavefront autotuner's stack-depth knob).
@@ -224,14 +229,14 @@ __global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
             while (idle != 0) {
                 if (bnext == bend) {
                     unsigned long long base = 0;
-                    if (lane == 0) base = atomicAdd(work, 64ull);
+                    if (lane == 0) base = atomicAdd(work, (unsigned long long)kTraceBatch);
                     base = __shfl(base, 0);
                     if (base >= n) {
                         exhausted = true;
                         break;
                     }
                     bnext = (uint32_t)base;
-                    bend = base + 64 < n ? (uint32_t)base + 64 : n;
+                    bend = base + kTraceBatch < n ? (uint32_t)base + kTraceBatch : n;
                 }
                 const uint32_t avail = bend - bnext;
                 const uint32_t rank = (uint32_t)__popcll(idle & below);

@@ -54,7 +54,7 @@ __device__ __forceinline__ void add_counters(unsigned long long* out, const Coun
 }
 
 // Fast pass of the batch closest-hit query as a persistent kernel: each wave
-// takes rays in batches of 64 from a global counter and refills a lane with
+// takes rays in batches of `batch` from a global counter and refills a lane with
 // the next ray as soon as its traversal ends, so a wave never waits for its
 // slowest ray.  The result of the fast pass (record, t, first record of the
 // leaf, overflow flag) goes to hits[ray]; k_closest_hit_finish completes it.
@@ -69,7 +69,8 @@ __device__ __forceinline__ void add_counters(unsigned long long* out, const Coun
 template <bool COUNT, int D>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_closest_hit(
     SceneDev S, wr_tree_hdr H, const float* __restrict__ rays, uint64_t n, uint4* __restrict__ hits,
-    unsigned long long* counters, unsigned long long* work, uint32_t refill_min, uint32_t node_phase_min) {
+    unsigned long long* counters, unsigned long long* work, uint32_t refill_min, uint32_t node_phase_min,
+    uint32_t batch) {
     const Tree tr = make_tree(H, S);
     StackSpillT<D> spill;
     LaneStackT<D> st(&spill);
@@ -90,14 +91,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
             while (idle != 0) {
                 if (bnext == bend) {
                     unsigned long long base = 0;
-                    if (lane == 0) base = atomicAdd(work, 64ull);
+                    if (lane == 0) base = atomicAdd(work, (unsigned long long)batch);
                     base = __shfl(base, 0);
                     if (base >= n) {
                         exhausted = true;
                         break;
                     }
                     bnext = base;
-                    bend = base + 64 < n ? base + 64 : n;
+                    bend = base + batch < n ? base + batch : n;
                 }
                 const uint64_t avail = bend - bnext;
                 const uint64_t rank = (uint64_t)__popcll(idle & below);
@@ -285,15 +286,18 @@ static uint32_t persistent_blocks(uint64_t n, uint32_t per_cu_want) {
 template <int D>
 static hipError_t launch_closest_hit_d(const SceneDev& S, const wr_tree_hdr& H, const float* rays, uint64_t n,
                                        void* hits, unsigned long long* counters, unsigned long long* work,
-                                       uint32_t per_cu, uint32_t rmin, uint32_t nmin, hipStream_t st) {
+                                       uint32_t per_cu, uint32_t rmin, uint32_t nmin, uint32_t batch,
+                                       hipStream_t st) {
     const uint32_t pgrid = persistent_blocks<D>(n, per_cu);
     const uint32_t grid = (uint32_t)((n + 255) / 256);
     const size_t lds = wrd::stack_lds_bytes<D>();
     if (counters) {
-        wrd::k_closest_hit<true, D><<<pgrid, 256, lds, st>>>(S, H, rays, n, (uint4*)hits, counters, work, rmin, nmin);
+        wrd::k_closest_hit<true, D><<<pgrid, 256, lds, st>>>(S, H, rays, n, (uint4*)hits, counters, work, rmin, nmin,
+                                                                 batch);
         wrd::k_closest_hit_finish<true, D><<<grid, 256, lds, st>>>(S, H, rays, n, (uint4*)hits, counters);
     } else {
-        wrd::k_closest_hit<false, D><<<pgrid, 256, lds, st>>>(S, H, rays, n, (uint4*)hits, nullptr, work, rmin, nmin);
+        wrd::k_closest_hit<false, D><<<pgrid, 256, lds, st>>>(S, H, rays, n, (uint4*)hits, nullptr, work, rmin, nmin,
+                                                                  batch);
         wrd::k_closest_hit_finish<false, D><<<grid, 256, lds, st>>>(S, H, rays, n, (uint4*)hits, nullptr);
     }
     return hipGetLastError();
@@ -306,13 +310,16 @@ hipError_t launch_closest_hit(const SceneDev& S, const wr_tree_hdr& H, const flo
     // refill_min in [1, 64]; node_phase_min in [1, 65] (0 would never leave the node phase; 65 disables it)
     const uint32_t rmin = tune.refill_min < 1 ? 1 : (tune.refill_min > 64 ? 64 : tune.refill_min);
     const uint32_t nmin = tune.node_phase_min < 1 ? 1 : (tune.node_phase_min > 65 ? 65 : tune.node_phase_min);
+    // rays per work-counter atomic in [64, 4096]
+    const uint32_t batch = tune.batch < 64 ? 64 : (tune.batch > 4096 ? 4096 : tune.batch);
+    const uint32_t pc = tune.blocks_per_cu;
     switch (tune.lds_levels) {
         case 4:
-            return launch_closest_hit_d<4>(S, H, rays, n, hits, counters, work, tune.blocks_per_cu, rmin, nmin, st);
+            return launch_closest_hit_d<4>(S, H, rays, n, hits, counters, work, pc, rmin, nmin, batch, st);
         case 16:
-            return launch_closest_hit_d<16>(S, H, rays, n, hits, counters, work, tune.blocks_per_cu, rmin, nmin, st);
+            return launch_closest_hit_d<16>(S, H, rays, n, hits, counters, work, pc, rmin, nmin, batch, st);
         default:
-            return launch_closest_hit_d<8>(S, H, rays, n, hits, counters, work, tune.blocks_per_cu, rmin, nmin, st);
+            return launch_closest_hit_d<8>(S, H, rays, n, hits, counters, work, pc, rmin, nmin, batch, st);
     }
 }
 

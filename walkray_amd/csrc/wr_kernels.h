@@ -58,6 +58,7 @@ struct ChTune {
     uint32_t node_phase_min = 32;  // lanes at wide nodes for a node-only step (0..64)
     uint32_t blocks_per_cu = 0;    // resident workgroups per CU, 0 = as many as fit
     uint32_t lds_levels = 8;       // traversal stack levels kept in LDS: 4, 8 or 16 (deeper ones spill)
+    uint32_t batch = 256;          // rays a wave takes from the shared work counter at a time (64..4096)
 };
 // work: two zeroed 64-bit words owned by the caller's stream; the kernel
 // leaves them zeroed again when it ends.
