@@ -192,11 +192,16 @@ __device__ __forceinline__ void block_count_done(uint32_t* addr, bool want) {
 // of the leaf) and the overflow flag; k_wf_finish completes it.
 constexpr uint32_t kOverflow = 4u;
 constexpr uint32_t kTraceRefillMin = 16;
-// Paths a wave takes from the work counter at a time.  An iteration's paths
-// are fewer than a closest-hit call's rays and its tail matters more: the
-// closest-hit call's larger batches (fewer atomics on the one counter) made
-// this kernel slower.
-constexpr uint32_t kTraceBatch = 64;
+// Paths a wave takes from the work counter at a time: guided self-scheduling
+// (Polychronopoulos and Kuck, 1987), the paths left after the wave's last
+// batch over four times the waves, clamped to [kTraceBatchMin, kTraceBatchMax]
+// and rounded up to whole waves.  Large batches early keep the waves off the
+// one counter while most paths remain; small ones at the end keep the tail
+// short.  A fixed 64 left the counter contended on large iterations, a fixed
+// 256 stretched the tail of small ones.
+constexpr uint32_t kTraceBatchMin = 64;
+constexpr uint32_t kTraceBatchMax = 1024;
+constexpr uint32_t kTraceBatchShare = 4;
 constexpr uint32_t kTraceNodePhaseMin = 32;
 
 // D: traversal stack levels in LDS (4, 8 or 16; the wavefront autotuner's stack-depth knob).
@@ -229,14 +234,17 @@ __global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
             while (idle != 0) {
                 if (bnext == bend) {
                     unsigned long long base = 0;
-                    if (lane == 0) base = atomicAdd(work, (unsigned long long)kTraceBatch);
+                    uint32_t take = (n > bend ? n - bend : 0u) / (used * 4u * kTraceBatchShare);
+                    take = take < kTraceBatchMin ? kTraceBatchMin : (take > kTraceBatchMax ? kTraceBatchMax : take);
+                    take = (take + 63u) & ~63u;
+                    if (lane == 0) base = atomicAdd(work, (unsigned long long)take);
                     base = __shfl(base, 0);
                     if (base >= n) {
                         exhausted = true;
                         break;
                     }
                     bnext = (uint32_t)base;
-                    bend = base + kTraceBatch < n ? (uint32_t)base + kTraceBatch : n;
+                    bend = base + take < n ? (uint32_t)base + take : n;
                 }
                 const uint32_t avail = bend - bnext;
                 const uint32_t rank = (uint32_t)__popcll(idle & below);
