@@ -112,8 +112,8 @@ int wr_ctx_create(int device, wr_ctx** out) {
         return hip_fail(nullptr, e, "hipStreamCreate");
     if ((e = hipEventCreate(&c->ev0)) != hipSuccess) return hip_fail(nullptr, e, "hipEventCreate");
     if ((e = hipEventCreate(&c->ev1)) != hipSuccess) return hip_fail(nullptr, e, "hipEventCreate");
-    if ((e = hipMalloc(&c->d_work, 4 * sizeof(unsigned long long))) != hipSuccess) return hip_fail(nullptr, e, "hipMalloc");
-    if ((e = hipMemset(c->d_work, 0, 4 * sizeof(unsigned long long))) != hipSuccess) return hip_fail(nullptr, e, "hipMemset");
+    if ((e = hipMalloc(&c->d_work, wr::kWorkWords * sizeof(unsigned long long))) != hipSuccess) return hip_fail(nullptr, e, "hipMalloc");
+    if ((e = hipMemset(c->d_work, 0, wr::kWorkWords * sizeof(unsigned long long))) != hipSuccess) return hip_fail(nullptr, e, "hipMemset");
     if ((e = wr::noalloc_scratch_init(device)) != hipSuccess) return hip_fail(nullptr, e, "hipGetSymbolAddress(scratch)");
     *out = c.release();
     return WR_OK;

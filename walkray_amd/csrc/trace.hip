@@ -18,6 +18,10 @@
 
 namespace wrd {
 
+using wr::kWorkDone;
+using wr::kWorkStride;
+using wr::kXcds;
+
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t G) {
     const uint32_t xcd = b & 7u, local = b >> 3;
     const uint32_t per = G >> 3, rem = G & 7u;
@@ -54,12 +58,18 @@ __device__ __forceinline__ void add_counters(unsigned long long* out, const Coun
 }
 
 // Fast pass of the batch closest-hit query as a persistent kernel: each wave
-// takes rays in batches of `batch` from a global counter and refills a lane with
+// takes rays in batches of `batch` from a work counter and refills a lane with
 // the next ray as soon as its traversal ends, so a wave never waits for its
 // slowest ray.  The result of the fast pass (record, t, first record of the
 // leaf, overflow flag) goes to hits[ray]; k_closest_hit_finish completes it.
-// work[0] hands out batches, work[1] counts finished waves; the last wave
-// out zeroes both for the next launch.  Compiled for 8 waves per SIMD (64
+// Work queues: the rays are cut into kXcds contiguous ranges, range q handed
+// out by the counter work[q * kWorkStride] to the workgroups of XCD q
+// (workgroups are dispatched to the XCDs round-robin), so neighbouring rays
+// share one XCD's L2 and each counter takes an eighth of the atomics.  A wave
+// whose range is used up moves on to the next range (and so on round the
+// ranges); once it has found all of them used up it is done.  work[kWorkDone]
+// counts finished waves; the last wave out zeroes every counter for the next
+// launch.  Compiled for 8 waves per SIMD (64
 // VGPRs): more waves in flight hide more of the node-fetch latency.
 // Scheduling knobs (ChTune, chosen by the autotuner): idle lanes are refilled
 // once at least refill_min are idle; a wave runs node-only steps while at
@@ -81,6 +91,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     uint64_t bnext = 0, bend = 0;  // the wave's current batch [bnext, bend)
     bool exhausted = false;
     bool active = false, overflow = false;
+    uint32_t q = blockIdx.x % kXcds;  // the work range of this workgroup's XCD
+    uint32_t used_up = 0;             // ranges this wave has found used up (each once: they stay so)
     uint64_t rid = 0;
     uint32_t ref = 0;
     RayPre r{};
@@ -90,15 +102,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
         if (!exhausted && (uint32_t)__popcll(idle) >= refill_min) {
             while (idle != 0) {
                 if (bnext == bend) {
-                    unsigned long long base = 0;
-                    if (lane == 0) base = atomicAdd(work, (unsigned long long)batch);
-                    base = __shfl(base, 0);
-                    if (base >= n) {
+                    // next batch: from range q, or the next range once q is used up
+                    bool got = false;
+                    while (used_up < kXcds) {
+                        const uint64_t lo = n * q / kXcds, hi = n * (q + 1) / kXcds;
+                        unsigned long long base = 0;
+                        if (lane == 0) base = atomicAdd(work + kWorkStride * q, (unsigned long long)batch);
+                        base = __shfl(base, 0);
+                        if (lo + base < hi) {
+                            bnext = lo + base;
+                            bend = lo + base + batch < hi ? lo + base + batch : hi;
+                            got = true;
+                            break;
+                        }
+                        q = q + 1 < kXcds ? q + 1 : 0;
+                        used_up++;
+                    }
+                    if (!got) {
                         exhausted = true;
                         break;
                     }
-                    bnext = base;
-                    bend = base + batch < n ? base + batch : n;
                 }
                 const uint64_t avail = bend - bnext;
                 const uint64_t rank = (uint64_t)__popcll(idle & below);
@@ -134,9 +157,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     if (COUNT) add_counters(counters, cnt);
     if (lane == 0) {
         const unsigned long long waves = (unsigned long long)gridDim.x * (blockDim.x / 64);
-        if (atomicAdd(work + 1, 1ull) == waves - 1) {
-            atomicExch(work, 0ull);
-            atomicExch(work + 1, 0ull);
+        if (atomicAdd(work + kWorkDone, 1ull) == waves - 1) {
+            for (uint32_t k = 0; k < kXcds; k++) atomicExch(work + kWorkStride * k, 0ull);
+            atomicExch(work + kWorkDone, 0ull);
         }
     }
 }

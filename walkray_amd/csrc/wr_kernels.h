@@ -58,10 +58,16 @@ struct ChTune {
     uint32_t node_phase_min = 32;  // lanes at wide nodes for a node-only step (0..64)
     uint32_t blocks_per_cu = 0;    // resident workgroups per CU, 0 = as many as fit
     uint32_t lds_levels = 8;       // traversal stack levels kept in LDS: 4, 8 or 16 (deeper ones spill)
-    uint32_t batch = 256;          // rays a wave takes from the shared work counter at a time (64..4096)
+    uint32_t batch = 256;          // rays a wave takes from its XCD's work counter at a time (64..4096)
 };
-// work: two zeroed 64-bit words owned by the caller's stream; the kernel
-// leaves them zeroed again when it ends.
+// The persistent closest-hit kernel's work counters: one per XCD, each on its
+// own 128 B line, and the count of finished waves after them.
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kWorkStride = 16;                    // 64-bit words between counters
+constexpr uint32_t kWorkDone = kXcds * kWorkStride;     // the finished-wave count
+constexpr uint32_t kWorkWords = kWorkDone + kWorkStride;
+// work: kWorkWords zeroed 64-bit words owned by the caller's stream; the
+// kernel leaves them zeroed again when it ends.
 hipError_t launch_closest_hit(const SceneDev& S, const wr_tree_hdr& H, const float* rays, uint64_t n, void* hits,
                               unsigned long long* counters, unsigned long long* work, const ChTune& tune,
                               hipStream_t st);
