@@ -24,6 +24,10 @@
 
 namespace wrd {
 
+using wr::kWorkDone;
+using wr::kWorkStride;
+using wr::kXcds;
+
 template <bool NEE>
 __global__ __launch_bounds__(256) void k_pt_mega(SceneDev S, CamDev cam, uint32_t max_depth, uint64_t seed,
                                                  uint32_t pass, FilmDev film, uint32_t npix) {
@@ -192,13 +196,18 @@ __device__ __forceinline__ void block_count_done(uint32_t* addr, bool want) {
 // of the leaf) and the overflow flag; k_wf_finish completes it.
 constexpr uint32_t kOverflow = 4u;
 constexpr uint32_t kTraceRefillMin = 16;
-// Paths a wave takes from the work counter at a time: guided self-scheduling
-// (Polychronopoulos and Kuck, 1987), the paths left after the wave's last
-// batch over four times the waves, clamped to [kTraceBatchMin, kTraceBatchMax]
-// and rounded up to whole waves.  Large batches early keep the waves off the
-// one counter while most paths remain; small ones at the end keep the tail
-// short.  A fixed 64 left the counter contended on large iterations, a fixed
-// 256 stretched the tail of small ones.
+// Work ranges and batches: the live paths are cut into kXcds contiguous
+// ranges, range q handed out by its own counter to the workgroups of XCD q
+// (workgroups are dispatched to the XCDs round-robin), so neighbouring paths
+// share one XCD's L2 and each counter takes an eighth of the atomics; a wave
+// whose range is used up moves on round the others.  Paths a wave takes at a
+// time: guided self-scheduling (Polychronopoulos and Kuck, 1987), the paths
+// left in the range over four times the XCD's waves, clamped to
+// [kTraceBatchMin, kTraceBatchMax] and rounded up to whole waves.  Large
+// batches early keep the waves off the counters while most paths remain;
+// small ones at the end keep the tail short.  A fixed 64 left a single
+// counter contended on large iterations, a fixed 256 stretched the tail of
+// small ones.
 constexpr uint32_t kTraceBatchMin = 64;
 constexpr uint32_t kTraceBatchMax = 1024;
 constexpr uint32_t kTraceBatchShare = 4;
@@ -221,6 +230,9 @@ __global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (1ull << lane) - 1ull;
     uint32_t bnext = 0, bend = 0;  // the wave's current batch of paths [bnext, bend)
+    uint32_t q = blockIdx.x % kXcds;  // the work range of this workgroup's XCD
+    uint32_t used_up = 0;             // ranges this wave has found used up (each once: they stay so)
+    const uint32_t xcd_waves = used * 4u / kXcds > 0u ? used * 4u / kXcds : 1u;
     bool exhausted = false;
     bool active = false, overflow = false;
     uint32_t pid = 0, ref = 0;
@@ -233,18 +245,31 @@ __global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
         if (!exhausted && (uint32_t)__popcll(idle) >= kTraceRefillMin) {
             while (idle != 0) {
                 if (bnext == bend) {
-                    unsigned long long base = 0;
-                    uint32_t take = (n > bend ? n - bend : 0u) / (used * 4u * kTraceBatchShare);
-                    take = take < kTraceBatchMin ? kTraceBatchMin : (take > kTraceBatchMax ? kTraceBatchMax : take);
-                    take = (take + 63u) & ~63u;
-                    if (lane == 0) base = atomicAdd(work, (unsigned long long)take);
-                    base = __shfl(base, 0);
-                    if (base >= n) {
+                    // next batch: from range q, or the next range once q is used up
+                    bool got = false;
+                    while (used_up < kXcds) {
+                        const uint32_t lo = (uint32_t)((uint64_t)n * q / kXcds);
+                        const uint32_t hi = (uint32_t)((uint64_t)n * (q + 1) / kXcds);
+                        const uint32_t from = bend >= lo && bend < hi ? bend : lo;  // progress estimate
+                        uint32_t take = (hi - from) / (xcd_waves * kTraceBatchShare);
+                        take = take < kTraceBatchMin ? kTraceBatchMin : (take > kTraceBatchMax ? kTraceBatchMax : take);
+                        take = (take + 63u) & ~63u;
+                        unsigned long long base = 0;
+                        if (lane == 0) base = atomicAdd(work + kWorkStride * q, (unsigned long long)take);
+                        base = __shfl(base, 0);
+                        if (lo + base < hi) {
+                            bnext = lo + (uint32_t)base;
+                            bend = lo + base + take < hi ? lo + (uint32_t)base + take : hi;
+                            got = true;
+                            break;
+                        }
+                        q = q + 1 < kXcds ? q + 1 : 0;
+                        used_up++;
+                    }
+                    if (!got) {
                         exhausted = true;
                         break;
                     }
-                    bnext = (uint32_t)base;
-                    bend = base + take < n ? (uint32_t)base + take : n;
                 }
                 const uint32_t avail = bend - bnext;
                 const uint32_t rank = (uint32_t)__popcll(idle & below);
@@ -303,9 +328,9 @@ __global__ __launch_bounds__(256) void k_wf_trace(EngineDev E, uint32_t par) {
     }
     if (lane == 0) {
         const unsigned long long waves = (unsigned long long)used * (blockDim.x / 64);
-        if (atomicAdd(work + 1, 1ull) == waves - 1) {
-            atomicExch(work, 0ull);
-            atomicExch(work + 1, 0ull);
+        if (atomicAdd(work + kWorkDone, 1ull) == waves - 1) {
+            for (uint32_t k = 0; k < kXcds; k++) atomicExch(work + kWorkStride * k, 0ull);
+            atomicExch(work + kWorkDone, 0ull);
         }
     }
 }

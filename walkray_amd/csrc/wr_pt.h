@@ -69,7 +69,7 @@ struct EngineCounters {
     uint32_t nseg, seg_start, seg_end, pad;
     uint32_t seg_job[kMaxJobs], seg_base[kMaxJobs], seg_take[kMaxJobs];
     uint64_t seg_work[kMaxJobs];
-    unsigned long long trace_work[2];  // k_wf_trace: path batches handed out, waves finished (self-resetting)
+    unsigned long long trace_work[kWorkWords];  // k_wf_trace: one path counter per XCD, then finished waves (self-resetting)
     uint32_t n_exit;                   // paths whose shading waits for findInsideObject (k_wf_shade)
     uint32_t nfold;                    // jobs with complete passes to fold this iteration
     uint32_t fold_job[kMaxJobs];       // those jobs, in job order
