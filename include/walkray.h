@@ -196,7 +196,7 @@ typedef struct {
     uint32_t node_phase_min; /* lanes at wide nodes for a node-only step (1..65; 65 = never) */
     uint32_t blocks_per_cu;  /* resident 256-lane workgroups per CU, 0 = as many as fit */
     uint32_t lds_levels;     /* traversal stack levels kept in LDS: 4, 8 or 16 (0 = 8); deeper levels spill */
-    uint32_t batch;          /* rays a wave takes from the shared work counter at a time, 64..4096 (0 = 256) */
+    uint32_t batch;          /* rays a wave takes from its XCD's work counter at a time, 64..4096 (0 = 256) */
     float ms;                /* one call on the rays with this configuration */
 } wr_closest_hit_tuning;
 int wr_autotune_closest_hit(wr_ctx* ctx, wr_scene* scene, const wr_ray* d_rays, uint64_t n,
