@@ -416,7 +416,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // are listed and shaded by the EXITS = true instance, so the traversal code
 // and its registers stay out of the common case and the queries run on
 // converged waves.  The list order only changes which slots the survivors take.
-template <bool EXITS>
+// NEE = false: the instance for pools in which no job asks for next-event
+// estimation; without that code the common-case kernel keeps its registers
+// (4 spilled instead of 62 at 128 VGPRs).
+template <bool EXITS, bool NEE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXITS ? 1 : 4))) void k_wf_shade(EngineDev E,
                                                                                                      uint32_t par) {
     const PathBuf& A = E.buf[par];
@@ -458,7 +461,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXITS ? 1 :
                     rng4(J.seed, p, J.pass0 + ps, depth + 1, rn);
                     const Out out = scatter<EXITS>(sc, self, o, d, h, inside, rn, st);
                     f3 T = mk(A.tx[i], A.ty[i], A.tz[i]);
-                    const bool nee = (J.flags & wr::kJobNee) != 0;
+                    const bool nee = NEE && (J.flags & wr::kJobNee) != 0;
                     if ((double)len(out.d) < 0.5) {  // emission / absorption ends the path
                         done = finish_value(J, job_cam(J), p, ps, mulv(T, divs(divs(out.rad, out.op), out.dp)));
                     } else if (depth + 1 > J.max_depth) {
@@ -736,12 +739,14 @@ hipError_t launch_pt_mega(const SceneDev& S, const CamHost& cam, uint32_t max_de
 Engine::~Engine() { release(); }
 
 void Engine::release() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    if (tail_exec) (void)hipGraphExecDestroy(tail_exec);
-    if (tail_graph) (void)hipGraphDestroy(tail_graph);
-    exec = tail_exec = nullptr;
-    graph = tail_graph = nullptr;
+    for (int m = 0; m < 2; m++) {
+        if (exec[m]) (void)hipGraphExecDestroy(exec[m]);
+        if (graph[m]) (void)hipGraphDestroy(graph[m]);
+        if (tail_exec[m]) (void)hipGraphExecDestroy(tail_exec[m]);
+        if (tail_graph[m]) (void)hipGraphDestroy(tail_graph[m]);
+        exec[m] = tail_exec[m] = nullptr;
+        graph[m] = tail_graph[m] = nullptr;
+    }
     ready = false;
     for (void* p : bufs)
         if (p) (void)hipFree(p);
@@ -814,11 +819,12 @@ hipError_t Engine::init(uint32_t want_cap, uint32_t want_grid, hipStream_t st, u
 // The iteration graphs (kGraphIters and kTailIters iterations: even, the buffer
 // parity alternates), captured from the direct launch sequence the first time a
 // replay needs them, so direct launches never create a graph.
-hipError_t Engine::ensure_graphs(hipStream_t st) {
-    if (exec && tail_exec) return hipSuccess;
+hipError_t Engine::ensure_graphs(hipStream_t st, bool nee) {
+    const int m = nee ? 1 : 0;
+    if (exec[m] && tail_exec[m]) return hipSuccess;
     auto capture = [&](int iters, hipGraph_t* g, hipGraphExec_t* x) -> hipError_t {
         WR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        const hipError_t e = enqueue(iters, st, nullptr);
+        const hipError_t e = enqueue(iters, st, nullptr, nee);
         hipGraph_t cg = nullptr;
         const hipError_t ec = hipStreamEndCapture(st, &cg);
         if (e != hipSuccess || ec != hipSuccess) {
@@ -829,8 +835,8 @@ hipError_t Engine::ensure_graphs(hipStream_t st) {
         return hipGraphInstantiate(x, *g, nullptr, nullptr, 0);
     };
     static_assert(kGraphIters % 2 == 0 && kTailIters % 2 == 0, "replays end on the parity they start on");
-    if (!exec) WR_CHECK(capture(kGraphIters, &graph, &exec));
-    if (!tail_exec) WR_CHECK(capture(kTailIters, &tail_graph, &tail_exec));
+    if (!exec[m]) WR_CHECK(capture(kGraphIters, &graph[m], &exec[m]));
+    if (!tail_exec[m]) WR_CHECK(capture(kTailIters, &tail_graph[m], &tail_exec[m]));
     return hipSuccess;
 }
 
@@ -857,7 +863,7 @@ static void launch_trace(const EngineDev& E, uint32_t lds_levels, uint32_t trace
         k_wf_trace<8, COUNT><<<trace_grid, 256, stack_lds_bytes<8>(), st>>>(E, par);
 }
 
-hipError_t Engine::enqueue(int iters, hipStream_t st, hipEvent_t* ev) {
+hipError_t Engine::enqueue(int iters, hipStream_t st, hipEvent_t* ev, bool nee) {
     const EngineDev E = engine_dev(*this);
     int k = 0;
     auto mark = [&]() -> hipError_t { return ev ? hipEventRecord(ev[k++], st) : hipSuccess; };
@@ -869,9 +875,15 @@ hipError_t Engine::enqueue(int iters, hipStream_t st, hipEvent_t* ev) {
         WR_CHECK(mark());
         k_wf_finish<<<grid, 256, kStackLdsBytes, st>>>(E, par);
         WR_CHECK(mark());
-        k_wf_shade<false><<<grid, 256, kStackLdsBytes, st>>>(E, par);
-        WR_CHECK(mark());
-        k_wf_shade<true><<<grid, 256, kStackLdsBytes, st>>>(E, par);
+        if (nee) {
+            k_wf_shade<false, true><<<grid, 256, kStackLdsBytes, st>>>(E, par);
+            WR_CHECK(mark());
+            k_wf_shade<true, true><<<grid, 256, kStackLdsBytes, st>>>(E, par);
+        } else {
+            k_wf_shade<false, false><<<grid, 256, kStackLdsBytes, st>>>(E, par);
+            WR_CHECK(mark());
+            k_wf_shade<true, false><<<grid, 256, kStackLdsBytes, st>>>(E, par);
+        }
         WR_CHECK(mark());
         k_wf_plan<<<1, 64, 0, st>>>(E, par);
         WR_CHECK(mark());
@@ -912,12 +924,14 @@ hipError_t Engine::release_slot(uint32_t slot, hipStream_t st) {
 }
 
 // One replay of the iteration graph, then a read-back of the job states.
-hipError_t Engine::step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt, bool tail, bool direct) {
+hipError_t Engine::step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt, bool nee, bool tail,
+                        bool direct) {
     if (direct) {
-        WR_CHECK(enqueue(tail ? kTailIters : kGraphIters, st, nullptr));
+        WR_CHECK(enqueue(tail ? kTailIters : kGraphIters, st, nullptr, nee));
     } else {
-        WR_CHECK(ensure_graphs(st));
-        WR_CHECK(hipGraphLaunch(tail ? tail_exec : exec, st));
+        WR_CHECK(ensure_graphs(st, nee));
+        const int m = nee ? 1 : 0;
+        WR_CHECK(hipGraphLaunch(tail ? tail_exec[m] : exec[m], st));
     }
     if (host_ctl) WR_CHECK(hipMemcpyAsync(host_ctl, ctl, sizeof(JobCtl) * kMaxJobs, hipMemcpyDeviceToHost, st));
     if (host_cnt) WR_CHECK(hipMemcpyAsync(host_cnt, cnt, sizeof(EngineCounters), hipMemcpyDeviceToHost, st));
@@ -977,7 +991,7 @@ static hipError_t run_wavefront_once(const SceneDev& S, const CamHost& cam, uint
     const uint64_t total = (uint64_t)npass * npix;
     if (cap == 0) cap = kDefaultPathSlots;
     WR_CHECK(E.init(cap, grid, st, lds_levels));
-    if (launch == kLaunchGraph) WR_CHECK(E.ensure_graphs(st));
+    if (launch == kLaunchGraph) WR_CHECK(E.ensure_graphs(st, nee));
     JobDesc J{};
     fill_job_camera(J, cam);
     J.scene = 0;
@@ -1028,8 +1042,9 @@ static hipError_t run_wavefront_once(const SceneDev& S, const CamHost& cam, uint
     auto replay = [&](bool short_one) -> hipError_t {
         const int n = short_one ? kTailIters : kGraphIters;
         iters += n;
-        if (launch == kLaunchGraph) return hipGraphLaunch(short_one ? E.tail_exec : E.exec, st);
-        return E.enqueue(n, st, ev);
+        const int m = nee ? 1 : 0;
+        if (launch == kLaunchGraph) return hipGraphLaunch(short_one ? E.tail_exec[m] : E.exec[m], st);
+        return E.enqueue(n, st, ev, nee);
     };
     for (;;) {
         e = replay(tail);

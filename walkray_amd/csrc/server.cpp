@@ -56,6 +56,7 @@ struct wr_server {
     std::map<uint64_t, Job> jobs;
     std::deque<uint64_t> queue;
     std::vector<int64_t> slot_job;  // engine job slot -> job id (-1 free, kDraining: a cancelled job's paths)
+    std::vector<uint8_t> slot_nee;  // the slot's job (running or draining) asks for next-event estimation
     uint32_t inject_admit_failures = 0;  // test hook: admissions left to fail as if allocation had failed
     uint64_t next_id = 1;
     bool tail = false;  // every running job had issued all its paths at the last read-back
@@ -95,6 +96,7 @@ int wr_server_create(wr_ctx* ctx, const wr_server_config* cfg, wr_server** out) 
     s->max_jobs = cfg && cfg->max_jobs ? std::min<uint32_t>(cfg->max_jobs, kMaxJobs) : kMaxJobs;
     const uint32_t slots = cfg && cfg->path_slots ? cfg->path_slots : kDefaultPathSlots;
     s->slot_job.assign(kMaxJobs, -1);
+    s->slot_nee.assign(kMaxJobs, 0);
     (void)hipSetDevice(ctx->device);
     hipError_t e = s->engine.init(slots, 0, ctx->stream);
     if (e != hipSuccess) {
@@ -252,6 +254,7 @@ int wr_server_step(wr_server* s, uint32_t* n_active) {
             continue;
         }
         s->slot_job[slot] = (int64_t)s->queue.front();
+        s->slot_nee[slot] = (j.desc.flags & WR_RENDER_NEE) != 0;
         s->queue.pop_front();
         admitted = true;
     }
@@ -259,8 +262,11 @@ int wr_server_step(wr_server* s, uint32_t* n_active) {
     for (uint32_t k = 0; k < kMaxJobs; k++) running += s->slot_job[k] >= 0 || s->slot_job[k] == kDraining;
     if (running) {
         std::vector<JobCtl> hc(kMaxJobs);
+        // the shading instance with next-event estimation while any job in the pool asks for it
+        bool nee = false;
+        for (uint32_t k = 0; k < kMaxJobs; k++) nee = nee || s->slot_nee[k];
         // short replays once only draining is left (nothing queued or newly admitted)
-        hipError_t e = s->engine.step(st, hc.data(), nullptr, s->tail && !admitted && s->queue.empty());
+        hipError_t e = s->engine.step(st, hc.data(), nullptr, nee, s->tail && !admitted && s->queue.empty());
         if (e != hipSuccess) return hip_fail(s->ctx, e, "wr_server_step: engine");
         const double t = s->now();
         s->tail = true;
@@ -273,6 +279,7 @@ int wr_server_step(wr_server* s, uint32_t* n_active) {
             e = s->engine.release_slot(k, st);
             if (e != hipSuccess) return hip_fail(s->ctx, e, "wr_server_step: release");
             s->slot_job[k] = -1;
+            s->slot_nee[k] = 0;
         }
         for (uint32_t k = 0; k < kMaxJobs; k++) {
             if (s->slot_job[k] < 0 || hc[k].state != kJobDone) continue;
@@ -284,6 +291,7 @@ int wr_server_step(wr_server* s, uint32_t* n_active) {
             e = s->engine.release_slot(k, st);
             if (e != hipSuccess) return hip_fail(s->ctx, e, "wr_server_step: release");
             s->slot_job[k] = -1;
+            s->slot_nee[k] = 0;
             j.slot = -1;
         }
         running = 0;

@@ -109,8 +109,10 @@ struct Engine {
     JobCtl* ctl = nullptr;
     SceneDev* scenes = nullptr;
     uint32_t* exit_list = nullptr;  // path slots of the deferred refraction exits (cap entries)
-    hipGraph_t graph = nullptr, tail_graph = nullptr;
-    hipGraphExec_t exec = nullptr, tail_exec = nullptr;
+    // the iteration graphs, per shading instance: [0] without next-event
+    // estimation (no running job asks for it), [1] with it
+    hipGraph_t graph[2] = {}, tail_graph[2] = {};
+    hipGraphExec_t exec[2] = {}, tail_exec[2] = {};
     std::vector<void*> bufs;
     // run_wavefront's per-render buffers, kept between renders (grow-only):
     // the per-pass contributions and completion counts of its one job
@@ -125,19 +127,24 @@ struct Engine {
     // Launches `iters` iterations one kernel at a time (the graphs' sequence).
     // With ev (kStages * iters + 1 events) an event is recorded before the first
     // kernel and after every kernel, and the trace stage counts its fetches.
-    hipError_t enqueue(int iters, hipStream_t st, hipEvent_t* ev);
+    // nee: the shading instance with next-event estimation, needed as soon as
+    // one job in the pool asks for it (the other instance leaves its code and
+    // registers out).
+    hipError_t enqueue(int iters, hipStream_t st, hipEvent_t* ev, bool nee);
     ~Engine();
     void release();
     bool ready = false;  // buffers allocated and zeroed for (cap, grid, lds_levels)
     hipError_t init(uint32_t cap, uint32_t grid, hipStream_t st, uint32_t lds_levels = 8);
-    hipError_t ensure_graphs(hipStream_t st);
+    hipError_t ensure_graphs(hipStream_t st, bool nee);
     hipError_t set_scene(uint32_t slot, const SceneDev& S, hipStream_t st);
     hipError_t admit(uint32_t slot, const JobDesc& J, hipStream_t st);
     hipError_t release_slot(uint32_t slot, hipStream_t st);
     hipError_t cancel_slot(uint32_t slot, hipStream_t st);
     // One replay: kGraphIters iterations, or kTailIters when `tail` (every
     // running job has issued all its camera paths, so only draining is left).
-    hipError_t step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt, bool tail = false,
+    // nee: some job in the pool (running or draining) asks for next-event
+    // estimation.
+    hipError_t step(hipStream_t st, JobCtl* host_ctl, EngineCounters* host_cnt, bool nee, bool tail = false,
                     bool direct = false);
 };
 
