@@ -202,15 +202,16 @@ constexpr uint32_t kTraceRefillMin = 16;
 // share one XCD's L2 and each counter takes an eighth of the atomics; a wave
 // whose range is used up moves on round the others.  Paths a wave takes at a
 // time: guided self-scheduling (Polychronopoulos and Kuck, 1987), the paths
-// left in the range over four times the XCD's waves, clamped to
+// left in the range over sixteen times the XCD's waves, clamped to
 // [kTraceBatchMin, kTraceBatchMax] and rounded up to whole waves.  Large
 // batches early keep the waves off the counters while most paths remain;
 // small ones at the end keep the tail short.  A fixed 64 left a single
 // counter contended on large iterations, a fixed 256 stretched the tail of
-// small ones.
+// small ones; with a counter per XCD a share of 16 beat 8, 4 and 2 (on the
+// 512-object field's iterations of about 8M paths it takes 64-128 paths).
 constexpr uint32_t kTraceBatchMin = 64;
 constexpr uint32_t kTraceBatchMax = 1024;
-constexpr uint32_t kTraceBatchShare = 4;
+constexpr uint32_t kTraceBatchShare = 16;
 constexpr uint32_t kTraceNodePhaseMin = 32;
 
 // D: traversal stack levels in LDS (4, 8 or 16; the wavefront autotuner's stack-depth knob).
