@@ -56,6 +56,31 @@ def test_alone_equals_mix_equals_render(ctx):
     assert all(s["state"] == W.JOB_DONE and s["done_s"] >= s["admit_s"] >= s["submit_s"] for s in st)
 
 
+def test_nee_and_plain_jobs_mixed_equal_render(ctx):
+    """Jobs with and without next-event estimation in one pool: the engine shades
+    with the NEE instance while any job in the pool asks for it and with the plain
+    one otherwise (a plain job admitted after the NEE job finished runs on the
+    plain instance, one admitted alongside it on the NEE one); every film equals
+    wr_render's with the job's own setting (pathTracer.cpp:39-66)."""
+    scs, built = _scenes(ctx)
+    srv = W.Server(ctx, max_jobs=2)
+    ids = [srv.add_scene(b) for b in built]
+    jobs = [(0, 32, 4, 8, 201, True), (2, 48, 12, 10, 202, False), (4, 24, 3, 9, 203, True),
+            (5, 40, 6, 12, 204, False), (7, 32, 5, 8, 205, False)]
+    cams = [V.job_camera(scs[s][1], res) for s, res, *_ in jobs]
+    jids = [srv.submit(ids[s], cams[k], spp, depth, seed, nee=nee)
+            for k, (s, res, spp, depth, seed, nee) in enumerate(jobs)]
+    _drain(srv, jids)
+    for k, (s, res, spp, depth, seed, nee) in enumerate(jobs):
+        got = srv.read(jids[k], res, res)
+        film = W.Film(ctx, res, res)
+        W.render(built[s], cams[k], film, spp, depth, seed, W.WR_MEGAKERNEL, nee=nee)
+        ref = film.read()
+        for key in ("sum", "comp", "sumsq"):
+            assert np.array_equal(ref[key].view(np.uint32), got[key].view(np.uint32)), (k, nee, key)
+    srv.close()
+
+
 def test_small_pool_mix_equals_render(ctx):
     """A path pool smaller than one pass of most jobs: every iteration's slot
     allocation across the running jobs is limited by the free room."""
