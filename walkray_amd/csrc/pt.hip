@@ -609,6 +609,32 @@ __global__ __launch_bounds__(64) void k_wf_plan(EngineDev E, uint32_t par) {
     }
 }
 
+// The pixel of the r-th camera path of a pass: pixels are issued in kCamTile x
+// kCamTile tiles (tiles row-major within bands kCamTile rows high, pixels
+// row-major within a tile; the last band and the last tile of a band are cut
+// to the image), so the paths of one wave and one work range look at one patch
+// of the scene.  A bijection on [0, W*H): each (pixel, pass) is generated once,
+// and since a path's samples depend only on its pixel and pass, the order
+// changes nothing in the film.
+constexpr uint32_t kCamTile = 8;
+__device__ __forceinline__ uint32_t tile_pixel(uint32_t r, uint32_t W, uint32_t H) {
+    const uint32_t band = r / (W * kCamTile);
+    const uint32_t hb = H - band * kCamTile < kCamTile ? H - band * kCamTile : kCamTile;
+    const uint32_t o = r - band * W * kCamTile;  // index within the band
+    const uint32_t nfull = W / kCamTile;         // full-width tiles of the band
+    uint32_t x, y;
+    if (o < nfull * kCamTile * hb) {
+        const uint32_t tc = o / (kCamTile * hb), oo = o - tc * kCamTile * hb;
+        x = tc * kCamTile + oo % kCamTile;
+        y = band * kCamTile + oo / kCamTile;
+    } else {  // the band's last tile, narrower than kCamTile
+        const uint32_t oo = o - nfull * kCamTile * hb, wt = W - nfull * kCamTile;
+        x = nfull * kCamTile + oo % wt;
+        y = band * kCamTile + oo / wt;
+    }
+    return y * W + x;
+}
+
 __global__ __launch_bounds__(256) void k_wf_generate(EngineDev E, uint32_t par) {
     const PathBuf& B = E.buf[par ^ 1];
     const wr::EngineCounters& c = *E.cnt;
@@ -620,7 +646,7 @@ __global__ __launch_bounds__(256) void k_wf_generate(EngineDev E, uint32_t par) 
         const wr::JobDesc& J = E.jobs[jb];
         const CamDev cam = job_cam(J);
         const uint64_t w = c.seg_work[s] + (j - c.seg_base[s]);
-        const uint32_t ps = (uint32_t)(w / J.npix), p = (uint32_t)(w % J.npix);
+        const uint32_t ps = (uint32_t)(w / J.npix), p = tile_pixel((uint32_t)(w % J.npix), J.W, J.H);
         float rn[4];
         rng4(J.seed, p, J.pass0 + ps, 0, rn);
         const f3 d = cam_dir(cam, p, rn[0], rn[1]);
